@@ -1,0 +1,192 @@
+/*
+ * pic2d.h -- C ABI of the MI355X-native 2-D electrostatic PIC timestep.
+ *
+ * Drop-in seam for the reference's pic / particle / fields / Mglin /
+ * collisions / boundary classes.  Every entry point returns an int status
+ * (PIC_OK or a negative PIC_ERR_*); none exits the process or throws.
+ * Plain C types only: caller-owned host buffers are copied in or out and
+ * never retained.  A handle is single-thread affine; calls are ordered on the
+ * handle's HIP stream and the getters synchronise it.
+ *
+ * Species index s: 0 = electrons, 1 = ions (NO_ION_SPECIES is 1 in every
+ * reference configuration, parameters.h:283).
+ */
+#ifndef PIC2D_H
+#define PIC2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define PIC_OK 0
+#define PIC_ERR_ARG (-1)         /* bad argument / handle                    */
+#define PIC_ERR_GRID (-2)        /* nodes-1 not a power of two (mglin.cpp:25-27) */
+#define PIC_ERR_FILE (-3)        /* file cannot be opened                    */
+#define PIC_ERR_FORMAT (-4)      /* garbled parameters.h or dataDump         */
+#define PIC_ERR_DEVICE (-5)      /* HIP runtime error or no GPU              */
+#define PIC_ERR_CAPACITY (-6)    /* particle store full                      */
+#define PIC_ERR_UNSUPPORTED (-7) /* option outside this build's scope        */
+#define PIC_ERR_ALLOC (-8)       /* device or host allocation failed         */
+
+/* --------------------------------------------------------------- options */
+#define PIC_MODE_REF 0      /* the reference's wall tests, frozen particles,
+                               corner x8, collision decision rules           */
+#define PIC_MODE_PHYSICAL 1 /* x<0 or x>=L leaves, null-collision MCC        */
+
+#define PIC_MG_FIXED 0 /* FP64 FMG, ncycle V(1,1) per level (fields.cpp:174) */
+#define PIC_MG_TOL 1   /* FP32 smoothing + FP64 residual refinement to tol  */
+
+#define PIC_LOAD_RANDOM 0  /* Philox uniform positions, Maxwellian velocities */
+#define PIC_LOAD_LATTICE 1 /* particle.cpp:410-443 lattice, both species co-located */
+
+#define PIC_WALL_ABSORB 0 /* boundary.cpp:581-677, 765-824 with zero yields  */
+#define PIC_WALL_STEADY 1 /* boundary.cpp:854-1103 electron-ion "plop"       */
+
+#define PIC_GAS_ARGON 0
+#define PIC_GAS_HYDROGEN 1
+
+#define PIC_SIGMA_CONSTANT 0
+#define PIC_SIGMA_ENERGY 1
+
+/*
+ * Runtime mirror of parameters.h (parameters.h:41-366, constants.h:15-36).
+ * Derived quantities are stored already evaluated, as the reference's macros
+ * evaluate them.  pic_params_default() fills the reference defaults;
+ * pic_params_load() evaluates a parameters.h-style file.
+ */
+typedef struct pic_params {
+    int32_t nodes;            /* X_NO_OF_CELLS == Y_NO_OF_CELLS (node count, 2^k+1) */
+    int32_t init_particle_no; /* INIT_PARTICLE_NO, per species                      */
+    int32_t mode;             /* PIC_MODE_*                                         */
+    int32_t loader;           /* PIC_LOAD_*                                         */
+    int32_t wall;             /* PIC_WALL_* (STEADYSTATE)                           */
+    int32_t have_collisions;  /* HAVE_COLLISIONS                                    */
+    int32_t gas;              /* SPECIES                                            */
+    int32_t sigma_cx;         /* SIGMA            (PIC_SIGMA_*)                     */
+    int32_t sigma_elastic;    /* SIGMA_ELASTIC    (PIC_SIGMA_*)                     */
+    int32_t sigma_ionization; /* SIGMA_IONIZATION (PIC_SIGMA_*)                     */
+    int32_t mg_mode;          /* PIC_MG_*                                           */
+    int32_t mg_ncycle;        /* V-cycles per FMG level (fields.cpp:174 uses 2)     */
+    int32_t mg_max_iter;      /* PIC_MG_TOL: refinement sweeps cap                  */
+    int32_t dt_dti;           /* DT_DTI                                             */
+    int32_t run_time;         /* TOTAL_RUN_TIME                                     */
+    int32_t number_to_inject; /* NUMBER_TO_INJECT (only 0 is supported)             */
+    int32_t no_ion_species;   /* NO_ION_SPECIES (only 1 is supported)               */
+    int32_t corner8;          /* charge post-scale at corners: 1 -> x8 (reference)  */
+    uint64_t seed;            /* RAND_SEED                                          */
+    double length;            /* XLENGTH == YLENGTH [m]                             */
+    double dt, dti;           /* DT, DTI [s]                                        */
+    double electron_mass, ion_mass;     /* ELECTRON_MASS, ION_MASS [kg]             */
+    double electron_charge, ion_charge; /* ELECTRON_CHARGE (<0), ION_CHARGE [C]     */
+    double qce, qci;          /* QCE, QCI: macro-particle charges [C]               */
+    double vte, vti, vtn;     /* VTE, VTI, VTN thermal speeds [m/s]                 */
+    double sigma_load_e, sigma_load_i; /* Maxwellian sigma used by the loader       */
+    double epsilon;           /* EPSILON                                            */
+    double neutral_density;   /* NEUTRAL_DENSITY [m^-3]                             */
+    double threshold;         /* THRESHOLD [eV]                                     */
+    double sigma_value, elastic_sigma_value, ionization_sigma_value; /* constants   */
+    double magnetic_field, angle;       /* MAGNETIC_FIELD [T], ANGLE [rad]          */
+    double phi_left, phi_right, phi_bottom, phi_top, phi_period;
+    double mg_tol;            /* PIC_MG_TOL relative residual target                */
+} pic_params;
+
+/* particleVec.h:26-35: the 48-byte AoS record of the reference */
+typedef struct pic_particle {
+    double x, y, vx, vy, vz, weight;
+} pic_particle;
+
+typedef struct pic_handle pic_handle;
+
+/* ------------------------------------------------------------ setup, run */
+
+/* Reference defaults of parameters.h/constants.h (parameters.h:41-366). */
+int pic_params_default(pic_params *p);
+
+/* Evaluates the #define lines of a parameters.h-style file (parameters.h:41-366)
+ * on top of the defaults; replaces the compile-time configuration step. */
+int pic_params_load(const char *path, pic_params *p);
+
+/* pic::pic (pic.cpp:55-221).  restart != 0 loads dump_dir/dataDump_e.txt and
+ * dump_dir/dataDump_i1.txt (particle.cpp:359-401); otherwise particles are
+ * created by p->loader (particle.cpp:321-482). */
+int pic_create(const pic_params *p, int restart, const char *dump_dir, pic_handle **out);
+int pic_destroy(pic_handle *h);
+
+/* pic::setRuntime (pic.h) */
+int pic_set_runtime(pic_handle *h, int32_t run_time);
+
+/* pic::pic_run (pic.cpp:1040-1279): pre-loop wall pass, then run_time steps of
+ * the fused timestep (collisions, gather+push, walls, deposit, rho, phi, E). */
+int pic_run(pic_handle *h);
+
+/* pic::dump_data (pic.cpp:1285-1295) -> particle::dump_data
+ * (particle.cpp:1361-1382): dir/dataDump_e.txt, dir/dataDump_i1.txt.
+ * precision = significant digits (6 = the reference's ostream default). */
+int pic_dump_data(pic_handle *h, const char *dir, int32_t precision);
+
+/* ------------------------------------------------ per-phase entry points */
+/* particle::get_particle_electric_field (particle.cpp:870-1085, pic.cl:56-92) */
+int pic_species_gather(pic_handle *h, int32_t s);
+/* particle::move_particles (particle.cpp:1101-1341, pic.cl:95-149) */
+int pic_species_move(pic_handle *h, int32_t s);
+/* particle::collect_charge (particle.cpp:663-858, pic.cl:22-52) */
+int pic_species_collect_charge(pic_handle *h, int32_t s);
+/* fields::grid_rho (fields.cpp:100-143) */
+int pic_fields_grid_rho(pic_handle *h);
+/* fields::grid_potential (fields.cpp:169-220) */
+int pic_fields_grid_potential(pic_handle *h, double phi0X, double phiLX, double phi0Y,
+                              double phiLY, double timestep, double phi_period);
+/* fields::grid_electric_field (fields.cpp:230-268) */
+int pic_fields_grid_electric_field(pic_handle *h);
+/* e_collisions::collision (e_collisions.cpp:480-597) */
+int pic_e_collisions(pic_handle *h);
+/* ion_collisions::collision (ion_collisions.cpp:255-308) */
+int pic_ion_collisions(pic_handle *h);
+/* (sec_ions.*implement_i_boundary) / (sec_e.*implement_e_boundary)
+ * (boundary.h:372-383; pic.cpp:1114-1128): which = 0 ion wall, 1 electron wall */
+int pic_boundary(pic_handle *h, int32_t which);
+
+/* Standalone Mglin (mglin.cpp:21-55) on the GPU, FP64: u is n*n row-major,
+ * holds the rhs on entry and the solution on exit. */
+int pic_mglin(double *u, int32_t n, int32_t ncycle, double length);
+
+/* --------------------------------------------------------------- getters */
+int pic_particle_size(pic_handle *h, int32_t s, int64_t *n);        /* particle::particle_size */
+int pic_get_particles(pic_handle *h, int32_t s, pic_particle *out, int64_t cap); /* 48-B AoS copy */
+int pic_get_phi(pic_handle *h, double *out);            /* nodes*nodes (fields::get_phi)      */
+int pic_get_rho(pic_handle *h, double *out);            /* nodes*nodes (fields::get_rho)      */
+int pic_get_E(pic_handle *h, double *ex, double *ey);   /* (nodes+1)^2 each, stride nodes
+                                                           (fields::get_elec_fieldX/Y)        */
+int pic_get_q(pic_handle *h, int32_t s, double *out);   /* post-scaled charge (particle::get_q) */
+int pic_get_step(pic_handle *h, int64_t *step);
+int pic_get_loss_counts(pic_handle *h, int32_t s, int64_t *out4); /* left,right,bottom,top */
+int pic_get_nodes(pic_handle *h, int32_t *nodes);
+const char *pic_last_error(pic_handle *h);              /* NULL handle: last global error */
+
+/* Native-layout state access (SoA: packed int16 cell (jx | jy<<16), fp32
+ * offsets in [0,1), fp32 velocities, uint32 id) for tests and tools. */
+int pic_get_state(pic_handle *h, int32_t s, uint32_t *cell, float *ox, float *oy, float *vx,
+                  float *vy, float *vz, uint32_t *id, int64_t cap);
+int pic_set_state(pic_handle *h, int32_t s, const uint32_t *cell, const float *ox,
+                  const float *oy, const float *vx, const float *vy, const float *vz,
+                  const uint32_t *id, int64_t n);
+int pic_get_qfixed(pic_handle *h, int32_t s, int64_t *out); /* nodes*nodes, 2^-40 units   */
+int pic_get_E32(pic_handle *h, float *out);  /* fp32 (Ex,Ey) pairs, (nodes+1)*nodes+1      */
+int pic_set_E32(pic_handle *h, const float *in);
+
+/* ------------------------------------------------------------ profiling */
+/* Event timing of the push pipeline (collide/gather/push/sort/deposit) and
+ * of the field solve, on the handle's stream. */
+int pic_profile_enable(pic_handle *h, int32_t on);
+/* out[0..7]: steps timed, push-phase ms, scatter-kernel ms, field ms,
+ * particles pushed (sum over timed steps), 0, 0, 0 */
+int pic_profile_read(pic_handle *h, double *out8);
+int pic_sync(pic_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

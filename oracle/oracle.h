@@ -1,0 +1,210 @@
+/*
+ * oracle.h -- CPU oracle for the 2-D electrostatic PIC timestep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code.  It is used by tests/ (as the checker), by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.
+ *
+ * Two families of functions live here:
+ *
+ *  orc_ref_*  FP64, array-of-structs restatement of the reference's algorithm
+ *             (the 48-byte particleVec of particleVec.h:26-35, absolute
+ *             positions, the reference's truncation-based cell tests).  These
+ *             follow pic.cl, particle.cpp, fields.cpp, mglin.cpp and
+ *             boundary.cpp statement by statement; each function cites the
+ *             lines it follows.  They are also the CPU baseline.
+ *
+ *  orc32_*    the same algorithm restated on the GPU library's data layout:
+ *             SoA fp32 state, position = packed int16 cell (jx, jy) + fp32
+ *             in-cell offset, 64-bit fixed-point charge.  Every arithmetic
+ *             operation is spelled out in the order the HIP kernels use, so
+ *             that with -ffp-contract=off on both sides the integer outputs
+ *             (cells, fixed-point charge, loss counts) and the fp32/fp64
+ *             state are bit-identical to the GPU's.
+ *
+ * How each function is pinned (independently of its own formula) is listed in
+ * the test files under tests/ (test_oracle_*.py); functions with no
+ * independent pin say "parity unpinned" below.
+ */
+#ifndef PIC2D_ORACLE_H
+#define PIC2D_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 counter-based RNG (Salmon et al., SC'11, "Parallel random   */
+/* numbers: as easy as 1, 2, 3").  Pinned by the published Random123 KAT     */
+/* vectors in tests/test_oracle_philox.py.                                   */
+/* ------------------------------------------------------------------------ */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* uniform in the open interval (0,1): ((x >> 9) + 0.5) * 2^-23, exact in fp32 */
+float orc_u01(uint32_t x);
+
+/* ------------------------------------------------------------------------ */
+/* FP64 reference semantics (particleVec AoS)                               */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double x, y, vx, vy, vz, weight;   /* particleVec.h:26-35 */
+} orc_particle;
+
+/* pic.cl:56-92 (get_particle_electric_field).  Ex/Ey use the reference's
+ * (N+1)^2 layout with row stride N (fields.cpp:65-76). */
+void orc_ref_gather(const orc_particle *p, long n, const double *Ex, const double *Ey,
+                    double dx, double dy, int nx, int ny, double *Exp, double *Eyp);
+
+/* pic.cl:95-149 (move_particles); constants as set in particle.cpp:1103-1112 */
+void orc_ref_push(orc_particle *p, long n, const double *Exp, const double *Eyp,
+                  double dx, double dy, int nx, int ny, double coef, double cos_a,
+                  double sin_a, double dt, double cos_w, double sin_w, double bfield);
+
+/* pic.cl:22-52 (collectCharge) with serial (race-free) accumulation into q,
+ * which the caller zeroes (particle.cpp:681-685). */
+void orc_ref_deposit(const orc_particle *p, long n, double qc_dxdy, double dx, double dy,
+                     int nx, int ny, double *q);
+
+/* particle.cpp:830-845: edges x2, then corners x2 again (corners end x8). */
+void orc_ref_charge_postscale(double *q, int nx, int ny);
+
+/* fields.cpp:100-143 (one ion species): rho = q_ion; rho += q_e */
+void orc_ref_grid_rho(const double *q_ion, const double *q_e, int n, double *rho);
+
+/* mglin.cpp:21-196: full multigrid, ncycle V(1,1) cycles per level, red-black
+ * Gauss-Seidel, half-weighting restriction, bilinear prolongation.  u (n*n,
+ * row-major, u[i*n+j] = NR's u[i][j]) holds the rhs on entry and the solution
+ * on exit.  Returns 0, or -1 if n-1 is not a power of two (mglin.cpp:25-27). */
+int orc_mglin(double *u, int n, int ncycle, double length);
+
+/* fields.cpp:169-220: rhs = -rho/EPSILON, Dirichlet folding, Mglin, boundary
+ * overwrite.  eps is the permittivity (constants.h:28). */
+int orc_ref_grid_potential(const double *rho, int n, double dx, double dy, double eps,
+                           double phi0X, double phiLX, double phi0Y, double phiLY,
+                           double timestep, double phi_period, int ncycle, double length,
+                           double *phi);
+
+/* fields.cpp:230-268.  Ex/Ey are (n+1)^2 arrays with row stride n that the
+ * caller zero-initialises once and keeps (entries the reference never writes
+ * stay 0, see the never-written edge rows). */
+void orc_ref_grid_efield(const double *phi, int n, double dx, double *Ex, double *Ey);
+
+/* boundary.cpp:581-677 (e_boundary) and 765-824 (ion_boundary) with all
+ * surface yields zero: every particle with (int)(x/dx) or (int)(y/dy)
+ * outside [0, N) is removed by swap-with-last + pop.  id (may be NULL) is
+ * permuted alongside.  Returns the new count. */
+long orc_ref_absorb(orc_particle *p, uint32_t *id, long n, double dx, double dy, int nx, int ny);
+
+/* ------------------------------------------------------------------------ */
+/* Build-layout (fp32 SoA, cell + offset) restatement                        */
+/* ------------------------------------------------------------------------ */
+#define ORC_MODE_REF 0        /* reference wall / frozen-particle semantics  */
+#define ORC_MODE_PHYS 1       /* particles leave at x<0 or x>=L              */
+
+#define ORC_ST_KEPT 0
+#define ORC_ST_LOST_LEFT 1
+#define ORC_ST_LOST_RIGHT 2
+#define ORC_ST_LOST_BOTTOM 3
+#define ORC_ST_LOST_TOP 4
+
+static inline uint32_t orc_cell_pack(int jx, int jy) {
+    return (uint32_t)(uint16_t)(int16_t)jx | ((uint32_t)(uint16_t)(int16_t)jy << 16);
+}
+static inline int orc_cell_jx(uint32_t c) { return (int)(int16_t)(c & 0xFFFFu); }
+static inline int orc_cell_jy(uint32_t c) { return (int)(int16_t)(c >> 16); }
+
+typedef struct {
+    int N;                     /* nodes per side (2^k + 1)                    */
+    int mode;                  /* ORC_MODE_REF / ORC_MODE_PHYS                */
+    float cE;                  /* fp32(0.5 * dt * q/m)                        */
+    float dtdx, dtdy;          /* fp32(dt/dx), fp32(dt/dy)                    */
+    int bfield;                /* magnetic rotation on/off                    */
+    float cos_a, sin_a, cos_w, sin_w;
+} orc32_push_params;
+
+/* Gather + push + reposition + wall classification of one species
+ * (pic.cl:56-149 on the cell/offset layout; wall test boundary.cpp:595-599).
+ * E32 holds (Ex, Ey) fp32 pairs in the reference's padded stride-N layout,
+ * (N+1)*N + 1 pairs.  status[i] receives ORC_ST_*. */
+void orc32_push(long n, uint32_t *cell, float *ox, float *oy, float *vx, float *vy, float *vz,
+                const float *E32, const orc32_push_params *pp, uint8_t *status);
+
+/* Fixed-point CIC deposit (pic.cl:22-52 on the cell/offset layout): weights
+ * w = rint(offset * 2^20), corner products summed in int64, q[N*N]. */
+void orc32_deposit(long n, const uint32_t *cell, const float *ox, const float *oy, int N,
+                   int mode, int64_t *qfix);
+
+/* Fixed-point charge -> rho -> Poisson rhs (particle.cpp:830-845 post-scale,
+ * fields.cpp:100-143 sum order, fields.cpp:184-204 rhs and Dirichlet folding).
+ * scale_s = qc_s/(dx*dy) * weight_s * 2^-40.  corner8: corners x8 (reference)
+ * else x4.  rho and rhs are n*n. */
+void orc_fixed_to_rhs(const int64_t *q_ion, const int64_t *q_e, int n, double scale_i,
+                      double scale_e, int corner8, double eps, double phi0X, double phiR,
+                      double phi0Y, double phiLY, double dxdy_inv, double *rho, double *rhs);
+
+/* fields.cpp:209-219 boundary overwrite + fields.cpp:230-268 E stencil,
+ * computed per node from phi (boundary already overwritten by the caller),
+ * and packed into the fp32 (Ex, Ey) gather layout.  Ex64/Ey64 may be NULL. */
+void orc_efield_pack(const double *phi, int n, double dx, double *Ex64, double *Ey64, float *E32);
+
+/* Residual-controlled FP64 multigrid (the TOL mode): FMG with ncycle cycles,
+ * then V(1,1) cycles until ||f - A u||_2 <= tol * ||f||_2 over interior nodes
+ * or max_cycles.  Returns the number of extra V-cycles, -1 on a bad n. */
+int orc_mg_tol(double *u_rhs, int n, double length, double tol, int max_cycles, double *rel_res_out);
+
+/* relative residual ||f - A u|| / ||f|| over interior nodes (A as mglin.cpp:133-144) */
+double orc_mg_relres(const double *u, const double *f, int n, double length);
+
+
+/* ------------------------------------------------------------------------ */
+/* Whole timesteps (pic::pic_run order, pic.cpp:1069-1162, collisionless)     */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int N, ncycle, dt_dti, pad;
+    double L, dx, dy, eps;
+    double coef_e, coef_i, dt_e, dt_i, qc_dxdy_e, qc_dxdy_i;
+    double cos_a, sin_a, cos_we, sin_we, cos_wi, sin_wi, bfield;
+    double phi_left, phi_right, phi_bottom, phi_top, phi_period;
+    long start_step;
+} orc_ref_config;
+
+/* FP64 reference step loop (also the CPU baseline; OpenMP-parallel gather,
+ * push and deposit when more than one thread is set).  Ex/Ey: (N+1)^2. */
+int orc_ref_run(const orc_ref_config *c, int nsteps, long *ne, orc_particle *e, uint32_t *eid,
+                long *ni, orc_particle *ip, uint32_t *iid, double *Ex, double *Ey, double *qe,
+                double *qi, double *rho, double *phi);
+/* OpenMP thread count for orc_ref_run; returns the count in effect */
+int orc_set_threads(int n);
+
+typedef struct {
+    int N, mode, corner8, mg_mode; /* mg_mode 0: FMG ncycle fixed; 1: residual tolerance */
+    int mg_ncycle, mg_max_cycles, dt_dti, preloop;
+    double L, dx, eps, mg_tol;
+    double phi_left, phi_right, phi_bottom, phi_top, phi_period;
+    double scale_e, scale_i;
+    long start_step;
+    orc32_push_params pe, pi;
+} orc32_config;
+
+/* Build-layout step loop: push + wall pass (stable compaction, per-wall loss
+ * counts into lost8[0..3] electrons, [4..7] ions), fixed-point deposit,
+ * rho/rhs, multigrid, phi boundary overwrite, packed fp32 E. */
+int orc32_run(const orc32_config *c, int nsteps, long *ne, uint32_t *ecell, float *eox, float *eoy,
+              float *evx, float *evy, float *evz, uint32_t *eid, long *ni, uint32_t *icell,
+              float *iox, float *ioy, float *ivx, float *ivy, float *ivz, uint32_t *iid,
+              float *E32, int64_t *qe, int64_t *qi, double *rho, double *phi, long *lost8,
+              int *mg_cycles_out);
+
+/* ------------------------------------------------------------------------ */
+/* dataDump text I/O (particle.cpp:359-401 reader, 1361-1382 writer)          */
+/* ------------------------------------------------------------------------ */
+/* precision = significant digits (6 reproduces the reference's ostream default) */
+int orc_dump_write(const char *path, const orc_particle *p, long n, int precision);
+/* returns count read, or -1 if the file cannot be opened; reads at most cap */
+long orc_dump_read(const char *path, orc_particle *p, long cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,289 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) -- test infrastructure.
+
+The oracle is the checker: tests compare the HIP library against it.  Nothing
+in the product imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_MGLIN_SO = os.path.join(ROOT, "oracle", "_ref", "libref_mglin.so")
+
+vp = C.c_void_p
+MODE_REF, MODE_PHYS = 0, 1
+ST_KEPT, ST_LEFT, ST_RIGHT, ST_BOTTOM, ST_TOP = range(5)
+
+PARTICLE_DTYPE = np.dtype(
+    [("x", "f8"), ("y", "f8"), ("vx", "f8"), ("vy", "f8"), ("vz", "f8"), ("weight", "f8")]
+)
+
+
+class PushParams(C.Structure):
+    _fields_ = [("N", C.c_int), ("mode", C.c_int), ("cE", C.c_float), ("dtdx", C.c_float),
+                ("dtdy", C.c_float), ("bfield", C.c_int), ("cos_a", C.c_float),
+                ("sin_a", C.c_float), ("cos_w", C.c_float), ("sin_w", C.c_float)]
+
+
+class Cfg32(C.Structure):
+    _fields_ = [("N", C.c_int), ("mode", C.c_int), ("corner8", C.c_int), ("mg_mode", C.c_int),
+                ("mg_ncycle", C.c_int), ("mg_max_cycles", C.c_int), ("dt_dti", C.c_int),
+                ("preloop", C.c_int), ("L", C.c_double), ("dx", C.c_double),
+                ("eps", C.c_double), ("mg_tol", C.c_double), ("phi_left", C.c_double),
+                ("phi_right", C.c_double), ("phi_bottom", C.c_double), ("phi_top", C.c_double),
+                ("phi_period", C.c_double), ("scale_e", C.c_double), ("scale_i", C.c_double),
+                ("start_step", C.c_long), ("pe", PushParams), ("pi", PushParams)]
+
+
+class CfgRef(C.Structure):
+    _fields_ = [("N", C.c_int), ("ncycle", C.c_int), ("dt_dti", C.c_int), ("pad", C.c_int)] + [
+        (k, C.c_double) for k in ("L", "dx", "dy", "eps", "coef_e", "coef_i", "dt_e", "dt_i",
+                                  "qc_dxdy_e", "qc_dxdy_i", "cos_a", "sin_a", "cos_we", "sin_we",
+                                  "cos_wi", "sin_wi", "bfield", "phi_left", "phi_right",
+                                  "phi_bottom", "phi_top", "phi_period")
+    ] + [("start_step", C.c_long)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(ORACLE_SO)
+        _lib.orc_u01.restype = C.c_float
+        _lib.orc_u01.argtypes = [C.c_uint32]
+        _lib.orc_mglin.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+        _lib.orc_mg_tol.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, vp]
+        _lib.orc_mg_relres.argtypes = [vp, vp, C.c_int, C.c_double]
+        _lib.orc_mg_relres.restype = C.c_double
+        _lib.orc_ref_gather.argtypes = [vp, C.c_long, vp, vp, C.c_double, C.c_double, C.c_int,
+                                        C.c_int, vp, vp]
+        _lib.orc_ref_push.argtypes = [vp, C.c_long, vp, vp, C.c_double, C.c_double, C.c_int,
+                                      C.c_int] + [C.c_double] * 7
+        _lib.orc_ref_deposit.argtypes = [vp, C.c_long, C.c_double, C.c_double, C.c_double,
+                                         C.c_int, C.c_int, vp]
+        _lib.orc_ref_charge_postscale.argtypes = [vp, C.c_int, C.c_int]
+        _lib.orc_ref_grid_rho.argtypes = [vp, vp, C.c_int, vp]
+        _lib.orc_ref_grid_potential.argtypes = [vp, C.c_int] + [C.c_double] * 9 + [C.c_int,
+                                                                                  C.c_double, vp]
+        _lib.orc_ref_grid_efield.argtypes = [vp, C.c_int, C.c_double, vp, vp]
+        _lib.orc_ref_absorb.argtypes = [vp, vp, C.c_long, C.c_double, C.c_double, C.c_int,
+                                        C.c_int]
+        _lib.orc_ref_absorb.restype = C.c_long
+        _lib.orc32_push.argtypes = [C.c_long] + [vp] * 7 + [C.POINTER(PushParams), vp]
+        _lib.orc32_deposit.argtypes = [C.c_long, vp, vp, vp, C.c_int, C.c_int, vp]
+        _lib.orc_fixed_to_rhs.argtypes = [vp, vp, C.c_int, C.c_double, C.c_double, C.c_int] + \
+            [C.c_double] * 6 + [vp, vp]
+        _lib.orc_efield_pack.argtypes = [vp, C.c_int, C.c_double, vp, vp, vp]
+        _lib.orc_ref_run.argtypes = [C.POINTER(CfgRef), C.c_int] + [vp] * 12
+        _lib.orc32_run.argtypes = [C.POINTER(Cfg32), C.c_int] + [vp] * 23
+        _lib.orc_set_threads.argtypes = [C.c_int]
+        _lib.orc_dump_write.argtypes = [C.c_char_p, vp, C.c_long, C.c_int]
+        _lib.orc_dump_read.argtypes = [C.c_char_p, vp, C.c_long]
+        _lib.orc_dump_read.restype = C.c_long
+    return _lib
+
+
+def P(a):
+    return a.ctypes.data_as(vp) if a is not None else None
+
+
+def philox(ctr, key):
+    out = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10((C.c_uint32 * 4)(*ctr), (C.c_uint32 * 2)(*key), out)
+    return list(out)
+
+
+def mglin(u, ncycle=2, length=1.0):
+    a = np.ascontiguousarray(u, dtype=np.float64).copy()
+    rc = lib().orc_mglin(P(a), a.shape[0], ncycle, length)
+    if rc != 0:
+        raise ValueError("n-1 must be a power of 2")
+    return a
+
+
+def mg_tol(f, length, tol, max_cycles=200):
+    a = np.ascontiguousarray(f, dtype=np.float64).copy()
+    rr = C.c_double()
+    c = lib().orc_mg_tol(P(a), a.shape[0], length, tol, max_cycles, C.byref(rr))
+    return a, c, rr.value
+
+
+def relres(u, f, length):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().orc_mg_relres(P(u), P(f), u.shape[0], length)
+
+
+def ref_mglin_available():
+    return os.path.exists(REF_MGLIN_SO)
+
+
+def ref_mglin(u, ncycle=2, length=1.0):
+    """The reference's own Mglin (mglin.cpp), built into oracle/_ref."""
+    r = C.CDLL(REF_MGLIN_SO)
+    r.ref_mglin.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+    a = np.ascontiguousarray(u, dtype=np.float64).copy()
+    if r.ref_mglin(P(a), a.shape[0], ncycle, length) != 0:
+        raise ValueError("reference Mglin rejected n")
+    return a
+
+
+def cell_pack(jx, jy):
+    jx = np.asarray(jx).astype(np.int16).view(np.uint16).astype(np.uint32)
+    jy = np.asarray(jy).astype(np.int16).view(np.uint16).astype(np.uint32)
+    return jx | (jy << 16)
+
+
+def cell_unpack(cell):
+    c = np.asarray(cell, dtype=np.uint32)
+    return ((c & 0xFFFF).astype(np.uint16).view(np.int16).astype(np.int32),
+            (c >> 16).astype(np.uint16).view(np.int16).astype(np.int32))
+
+
+def push32(state, E32, pp):
+    """orc32_push on copies of the SoA arrays; returns (new_state, status)."""
+    s = {k: np.array(v, copy=True) for k, v in state.items()}
+    n = len(s["cell"])
+    st = np.zeros(n, dtype=np.uint8)
+    lib().orc32_push(n, P(s["cell"]), P(s["ox"]), P(s["oy"]), P(s["vx"]), P(s["vy"]), P(s["vz"]),
+                     P(np.ascontiguousarray(E32, dtype=np.float32)), C.byref(pp), P(st))
+    return s, st
+
+
+def deposit32(state, N, mode):
+    q = np.zeros(N * N, dtype=np.int64)
+    n = len(state["cell"])
+    lib().orc32_deposit(n, P(state["cell"]), P(state["ox"]), P(state["oy"]), N, mode, P(q))
+    return q.reshape(N, N)
+
+
+def efield_pack(phi, dx):
+    N = phi.shape[0]
+    ex = np.zeros(N * N)
+    ey = np.zeros(N * N)
+    e32 = np.zeros(2 * ((N + 1) * N + 1), dtype=np.float32)
+    lib().orc_efield_pack(P(np.ascontiguousarray(phi)), N, dx, P(ex), P(ey), P(e32))
+    return ex.reshape(N, N), ey.reshape(N, N), e32
+
+
+def ref_efield(phi, dx, ex=None, ey=None):
+    N = phi.shape[0]
+    ex = np.zeros((N + 1) * (N + 1)) if ex is None else ex
+    ey = np.zeros((N + 1) * (N + 1)) if ey is None else ey
+    lib().orc_ref_grid_efield(P(np.ascontiguousarray(phi)), N, dx, P(ex), P(ey))
+    return ex, ey
+
+
+def run32(cfg, nsteps, e, i, E32, mg_cycles=False):
+    """orc32_run on copies; returns dict with the new states and fields."""
+    N = cfg.N
+    e = {k: np.array(v, copy=True) for k, v in e.items()}
+    i = {k: np.array(v, copy=True) for k, v in i.items()}
+    ne = C.c_long(len(e["cell"]))
+    ni = C.c_long(len(i["cell"]))
+    E = np.array(E32, dtype=np.float32, copy=True)
+    qe = np.zeros(N * N, dtype=np.int64)
+    qi = np.zeros(N * N, dtype=np.int64)
+    rho = np.zeros(N * N)
+    phi = np.zeros(N * N)
+    lost = np.zeros(8, dtype=np.int64)
+    cyc = np.zeros(max(nsteps, 1), dtype=np.int32)
+    keys = ("cell", "ox", "oy", "vx", "vy", "vz", "id")
+    rc = lib().orc32_run(C.byref(cfg), nsteps, C.byref(ne), *[P(e[k]) for k in keys], C.byref(ni),
+                         *[P(i[k]) for k in keys], P(E), P(qe), P(qi), P(rho), P(phi), P(lost),
+                         P(cyc))
+    if rc != 0:
+        raise RuntimeError("orc32_run failed")
+    e = {k: v[: ne.value] for k, v in e.items()}
+    i = {k: v[: ni.value] for k, v in i.items()}
+    return dict(e=e, i=i, E32=E, qe=qe.reshape(N, N), qi=qi.reshape(N, N), rho=rho.reshape(N, N),
+                phi=phi.reshape(N, N), lost=lost, mg_cycles=cyc)
+
+
+def run_ref(cfg, nsteps, e, i, eid, iid, Ex, Ey):
+    """orc_ref_run (FP64 AoS reference semantics) on copies."""
+    N = cfg.N
+    e = np.array(e, dtype=PARTICLE_DTYPE, copy=True)
+    i = np.array(i, dtype=PARTICLE_DTYPE, copy=True)
+    eid = np.array(eid, dtype=np.uint32, copy=True)
+    iid = np.array(iid, dtype=np.uint32, copy=True)
+    ne = C.c_long(len(e))
+    ni = C.c_long(len(i))
+    Ex = np.array(Ex, dtype=np.float64, copy=True)
+    Ey = np.array(Ey, dtype=np.float64, copy=True)
+    qe = np.zeros(N * N)
+    qi = np.zeros(N * N)
+    rho = np.zeros(N * N)
+    phi = np.zeros(N * N)
+    rc = lib().orc_ref_run(C.byref(cfg), nsteps, C.byref(ne), P(e), P(eid), C.byref(ni), P(i),
+                           P(iid), P(Ex), P(Ey), P(qe), P(qi), P(rho), P(phi))
+    if rc != 0:
+        raise RuntimeError("orc_ref_run failed")
+    return dict(e=e[: ne.value], i=i[: ni.value], eid=eid[: ne.value], iid=iid[: ni.value], Ex=Ex,
+                Ey=Ey, qe=qe.reshape(N, N), qi=qi.reshape(N, N), rho=rho.reshape(N, N),
+                phi=phi.reshape(N, N))
+
+
+def configs_from_params(p, mode=None):
+    """(Cfg32, CfgRef) matching a pic2d.Params the way the library derives them."""
+    import math
+    N = p.nodes
+    dx = p.length / (N - 1)
+    mode = p.mode if mode is None else mode
+
+    def pp(q_over_m, dt):
+        coef = 0.5 * dt * q_over_m
+        w = dt * q_over_m * p.magnetic_field
+        return PushParams(N=N, mode=mode, cE=coef, dtdx=dt / dx, dtdy=dt / dx,
+                          bfield=int(p.magnetic_field > 0), cos_a=math.cos(p.angle),
+                          sin_a=math.sin(p.angle), cos_w=math.cos(w), sin_w=math.sin(w))
+
+    qme = p.electron_charge / p.electron_mass
+    qmi = p.ion_charge / p.ion_mass
+    c32 = Cfg32(N=N, mode=mode, corner8=p.corner8, mg_mode=0, mg_ncycle=p.mg_ncycle,
+                mg_max_cycles=p.mg_max_iter, dt_dti=p.dt_dti, preloop=0, L=p.length, dx=dx,
+                eps=p.epsilon, mg_tol=p.mg_tol, phi_left=p.phi_left, phi_right=p.phi_right,
+                phi_bottom=p.phi_bottom, phi_top=p.phi_top, phi_period=p.phi_period,
+                scale_e=math.ldexp(p.qce / (dx * dx) * 1.0, -40),
+                scale_i=math.ldexp(p.qci / (dx * dx) * 1.0, -40), start_step=0,
+                pe=pp(qme, p.dt), pi=pp(qmi, p.dti))
+    we = p.dt * qme * p.magnetic_field
+    wi = p.dti * qmi * p.magnetic_field
+    cref = CfgRef(N=N, ncycle=p.mg_ncycle, dt_dti=p.dt_dti, L=p.length, dx=dx, dy=dx,
+                  eps=p.epsilon, coef_e=0.5 * p.dt * qme, coef_i=0.5 * p.dti * qmi, dt_e=p.dt,
+                  dt_i=p.dti, qc_dxdy_e=p.qce / (dx * dx), qc_dxdy_i=p.qci / (dx * dx),
+                  cos_a=math.cos(p.angle), sin_a=math.sin(p.angle), cos_we=math.cos(we),
+                  sin_we=math.sin(we), cos_wi=math.cos(wi), sin_wi=math.sin(wi),
+                  bfield=p.magnetic_field, phi_left=p.phi_left, phi_right=p.phi_right,
+                  phi_bottom=p.phi_bottom, phi_top=p.phi_top, phi_period=p.phi_period,
+                  start_step=0)
+    return c32, cref
+
+
+def state_to_aos(state, dx, weight=1.0):
+    jx, jy = cell_unpack(state["cell"])
+    a = np.zeros(len(jx), dtype=PARTICLE_DTYPE)
+    a["x"] = (jx + state["ox"].astype(np.float64)) * dx
+    a["y"] = (jy + state["oy"].astype(np.float64)) * dx
+    a["vx"] = state["vx"]
+    a["vy"] = state["vy"]
+    a["vz"] = state["vz"]
+    a["weight"] = weight
+    return a
+
+
+def random_state(n, N, sigma, seed, rng=None):
+    """uniform positions in the domain and a Maxwellian, fp32 SoA (for CPU tests)"""
+    rng = np.random.default_rng(seed) if rng is None else rng
+    t = rng.random((2, n)) * (N - 1)
+    j = np.floor(t)
+    o = (t - j).astype(np.float32)
+    o[o >= 1.0] = 0.0
+    v = (rng.standard_normal((3, n)) * sigma).astype(np.float32)
+    return dict(cell=cell_pack(j[0].astype(np.int32), j[1].astype(np.int32)), ox=o[0].copy(),
+                oy=o[1].copy(), vx=v[0].copy(), vy=v[1].copy(), vz=v[2].copy(),
+                id=np.arange(n, dtype=np.uint32))

@@ -1,0 +1,176 @@
+"""Fused GPU timestep (pic_run) against the oracle.
+
+Bit-exact against the build-layout restatement (orc32_run): particle cells,
+offsets, velocities (matched by id), fixed-point charge, loss counts, phi and
+the packed E, after 1 and 10 steps.  Within 1e-6 relative L2 against the FP64
+reference-layout restatement (orc_ref_run) for positions, velocities, phi, E.
+The per-phase entry points reproduce the fused step bit for bit.
+"""
+import numpy as np
+import pytest
+
+import orc
+import pic2d
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("cell", "ox", "oy", "vx", "vy", "vz")
+
+
+def by_id(s):
+    o = np.argsort(s["id"], kind="stable")
+    return {k: v[o] for k, v in s.items()}
+
+
+def assert_states_equal(a, b, what):
+    a, b = by_id(a), by_id(b)
+    assert np.array_equal(a["id"], b["id"]), f"{what}: particle sets differ"
+    for k in KEYS:
+        bad = np.flatnonzero(a[k] != b[k])
+        assert bad.size == 0, f"{what}.{k}: {bad.size} mismatches, first id {a['id'][bad[0]]}"
+
+
+def make(cells=64, ppc=20, mode=pic2d.MODE_REF, **kw):
+    p = workloads.discharge(cells, ppc, mode=mode, mg_mode=pic2d.MG_FIXED, **kw)
+    return p, pic2d.Pic(p)
+
+
+@pytest.mark.parametrize("mode", [pic2d.MODE_REF, pic2d.MODE_PHYSICAL])
+def test_fused_step_bit_exact_vs_build_oracle(mode):
+    p, sim = make(mode=mode)
+    print(f"fused step parity: mode={mode} N={p.nodes} n={p.init_particle_no}", flush=True)
+    with sim:
+        e0, i0, E0 = sim.get_state(0), sim.get_state(1), sim.get_E32()
+        c32, _ = orc.configs_from_params(p)
+        c32.preloop = 1
+        r = orc.run32(c32, 1, e0, i0, E0)
+        sim.run(1)
+        assert_states_equal(sim.get_state(0), r["e"], "e@1")
+        assert_states_equal(sim.get_state(1), r["i"], "i@1")
+        assert np.array_equal(sim.get_qfixed(0), r["qe"])
+        assert np.array_equal(sim.get_qfixed(1), r["qi"])
+        assert np.array_equal(sim.get_phi(), r["phi"])
+        assert np.array_equal(sim.get_E32(), r["E32"])
+        # nine more steps from the GPU's own state
+        c32.preloop = 0
+        c32.start_step = 1
+        r9 = orc.run32(c32, 9, r["e"], r["i"], r["E32"])
+        sim.run(9)
+        assert_states_equal(sim.get_state(0), r9["e"], "e@10")
+        assert_states_equal(sim.get_state(1), r9["i"], "i@10")
+        assert np.array_equal(sim.get_qfixed(0), r9["qe"])
+        assert np.array_equal(sim.get_phi(), r9["phi"])
+        assert np.array_equal(sim.get_E32(), r9["E32"])
+        lost = sim.loss_counts(0).sum() + sim.loss_counts(1).sum()
+        assert lost == r["lost"].sum() + r9["lost"].sum()
+        assert np.array_equal(np.concatenate([sim.loss_counts(0), sim.loss_counts(1)]),
+                              r["lost"] + r9["lost"])
+        assert sim.get_step() == 10
+
+
+def test_fused_step_tracks_fp64_reference():
+    p, sim = make(cells=64, ppc=20)
+    N, dx = p.nodes, p.dx
+    with sim:
+        e0, i0 = sim.get_state(0), sim.get_state(1)
+        _, cref = orc.configs_from_params(p)
+        Ex = np.zeros((N + 1) * (N + 1))
+        for steps in (1, 10):
+            rr = orc.run_ref(cref, steps, orc.state_to_aos(e0, dx), orc.state_to_aos(i0, dx),
+                             e0["id"], i0["id"], Ex, Ex.copy())
+            with pic2d.Pic(p) as s2:
+                s2.run(steps)
+                g = by_id(s2.get_state(0))
+                phi = s2.get_phi()
+                gex, _ = s2.get_E()
+            a = orc.state_to_aos(g, dx)
+            idx = {int(k): j for j, k in enumerate(g["id"])}
+            pairs = [(idx[int(k)], j) for j, k in enumerate(rr["eid"]) if int(k) in idx]
+            assert len(pairs) >= 0.99 * len(rr["eid"])
+            ga, gb = a[[x for x, _ in pairs]], rr["e"][[y for _, y in pairs]]
+            for f in ("x", "y", "vx", "vy"):
+                rel = np.linalg.norm(ga[f] - gb[f]) / np.linalg.norm(gb[f])
+                assert rel < 1e-6, (steps, f, rel)
+            rel_phi = np.linalg.norm(phi - rr["phi"]) / np.linalg.norm(rr["phi"])
+            assert rel_phi < 2e-5, (steps, rel_phi)
+
+
+def test_phased_entry_points_reproduce_fused_step():
+    p, fused = make(cells=32, ppc=30)
+    _, phased = make(cells=32, ppc=30)
+    with fused, phased:
+        fused.run(3)
+        for step in range(3):
+            if step == 0:                      # pic.cpp:1049-1057 pre-loop wall pass
+                phased.boundary(0)
+                phased.boundary(1)
+            phased.e_collisions()
+            phased.get_particle_electric_field(0)
+            phased.move_particles(0)
+            phased.ion_collisions()
+            phased.get_particle_electric_field(1)
+            phased.move_particles(1)
+            phased.boundary(0)
+            phased.boundary(1)
+            phased.collect_charge(1)
+            phased.collect_charge(0)
+            phased.grid_rho()
+            phased.grid_potential(p.phi_left, p.phi_right, p.phi_bottom, p.phi_top, step,
+                                  p.phi_period)
+            phased.grid_electric_field()
+        assert_states_equal(phased.get_state(0), fused.get_state(0), "e")
+        assert_states_equal(phased.get_state(1), fused.get_state(1), "i")
+        assert np.array_equal(phased.get_qfixed(0), fused.get_qfixed(0))
+        assert np.array_equal(phased.get_rho(), fused.get_rho())
+        assert np.array_equal(phased.get_phi(), fused.get_phi())
+        a, b = phased.get_E(), fused.get_E()
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_loader_statistics_and_ids():
+    p, sim = make(cells=64, ppc=50)
+    with sim:
+        for s, sig in ((0, p.sigma_load_e), (1, p.sigma_load_i)):
+            st = sim.get_state(s)
+            n = len(st["id"])
+            assert n == p.init_particle_no
+            assert np.array_equal(np.sort(st["id"]), np.arange(n, dtype=np.uint32))
+            jx, jy = pic2d.cell_unpack(st["cell"])
+            assert jx.min() >= 0 and jx.max() <= p.nodes - 2
+            assert np.all((st["ox"] >= 0) & (st["ox"] < 1))
+            for k in ("vx", "vy", "vz"):
+                v = st[k].astype(np.float64)
+                assert abs(v.mean()) < 5 * sig / np.sqrt(n)
+                assert abs(v.std() / sig - 1) < 5 / np.sqrt(2 * n) + 1e-3
+            # positions uniform: counts per cell within 6 sigma of the mean
+            h = np.bincount(jy * (p.nodes - 1) + jx, minlength=(p.nodes - 1) ** 2)
+            assert np.all(np.abs(h - h.mean()) < 6 * np.sqrt(h.mean()))
+            # sorted by the counting-sort key after creation
+            key = (((jy + 1) >> 3) * ((p.nodes + 8) // 8) + ((jx + 1) >> 3)) * 64 + \
+                ((jy + 1) & 7) * 8 + ((jx + 1) & 7)
+            assert np.all(np.diff(key) >= 0)
+
+
+def test_dump_and_restart_roundtrip(tmp_path):
+    p, sim = make(cells=32, ppc=10)
+    with sim:
+        sim.run(2)
+        sim.dump_data(tmp_path, 17)
+        e, i = by_id(sim.get_state(0)), by_id(sim.get_state(1))
+        ne, ni = len(e["id"]), len(i["id"])
+    lines = open(tmp_path / "dataDump_e.txt").read().split("\n")
+    assert len(lines) == ne + 1 and len(lines[0].split()) == 6
+    with pic2d.Pic(p, restart=True, dump_dir=tmp_path) as r:
+        assert r.particle_size(0) == ne and r.particle_size(1) == ni
+        aos = r.get_particles(0)
+    # restart positions are the dumped doubles to the last bit (17 digits)
+    dumped = np.loadtxt(tmp_path / "dataDump_e.txt")
+    assert np.array_equal(np.sort(aos["x"]), np.sort(dumped[:, 0]))
+    assert np.array_equal(np.sort(aos["vx"]), np.sort(dumped[:, 2]))
+    # truncated file is rejected (the reference would silently drop the record)
+    txt = open(tmp_path / "dataDump_i1.txt").read()
+    open(tmp_path / "dataDump_i1.txt", "w").write(txt[: len(txt) // 2] + " 1.0 garbage\n")
+    with pytest.raises(pic2d.PicError) as err:
+        pic2d.Pic(p, restart=True, dump_dir=tmp_path)
+    assert err.value.code == -4

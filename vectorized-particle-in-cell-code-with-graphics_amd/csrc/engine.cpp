@@ -1,0 +1,821 @@
+// engine.cpp -- the C ABI: handle lifecycle, the fused timestep, the
+// reference-shaped per-phase entry points, getters and dataDump I/O.
+//
+// One handle = one simulation (the reference's pic object, pic.h:89-214).
+// All state lives in device memory; every call is enqueued on the handle's
+// stream.  Getters synchronise the stream and copy into caller buffers.
+#include <hip/hip_runtime.h>
+
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pic_host.h"
+
+using namespace pic;
+using namespace pic_host;
+
+namespace pic_host {
+static std::mutex g_err_mu;
+static std::string g_err;
+int set_global_error(int code, const std::string &msg) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = msg;
+    return code;
+}
+const char *global_error() { return g_err.c_str(); }
+}  // namespace pic_host
+
+namespace {
+
+int seterr(pic_handle *h, int code, const std::string &m) {
+    if (h) h->err = m;
+    return set_global_error(code, m);
+}
+
+#define HIPCHK(h, expr)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return seterr((h), PIC_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK(expr)              \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_ != PIC_OK) return rc_; \
+    } while (0)
+
+template <class T>
+int dalloc(pic_handle *h, T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess)
+        return seterr(h, PIC_ERR_ALLOC, std::string("hipMalloc ") + std::to_string(n * sizeof(T)) +
+                                            " bytes: " + hipGetErrorString(e));
+    return PIC_OK;
+}
+
+bool grid_ok(int n) {
+    if (n < 5) return false;
+    int m = n - 1;
+    return (m & (m - 1)) == 0;
+}
+
+int levels_of(int n) {
+    int ng = 0, nn = n;
+    while (nn >>= 1) ng++;
+    return ng;
+}
+
+long lds_limit() {
+    int dev = 0, v = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        v <= 0)
+        v = 65536;
+    return v;
+}
+
+int alloc_mg(pic_handle *h, MgLevels &M, int N) {
+    M.ng = levels_of(N);
+    long lds = lds_limit();
+    M.lc = mg_coarse_level_for(lds, M.ng);
+    mg_set_coarse_attr(lds);
+    for (int l = M.lc; l < M.ng; l++) {
+        size_t s = (size_t)((1 << (l + 1)) + 1) * ((1 << (l + 1)) + 1);
+        CHECK(dalloc(h, &M.rhsF[l], s));
+        CHECK(dalloc(h, &M.u[l], s));
+        if (l < M.ng - 1) {
+            CHECK(dalloc(h, &M.cres[l], s));
+            CHECK(dalloc(h, &M.cv[l], s));
+        }
+    }
+    return PIC_OK;
+}
+
+void free_mg(MgLevels &M) {
+    for (int l = 0; l < 16; l++) {
+        if (M.rhsF[l]) hipFree(M.rhsF[l]);
+        if (M.u[l]) hipFree(M.u[l]);
+        if (M.cres[l]) hipFree(M.cres[l]);
+        if (M.cv[l]) hipFree(M.cv[l]);
+        M.rhsF[l] = M.u[l] = M.cres[l] = M.cv[l] = nullptr;
+    }
+}
+
+// ----------------------------------------------------------- profiling
+int prof_begin(pic_handle *h, int tag) {
+    if (!h->prof) return -1;
+    ProfPair p;
+    if (!h->prof_pool.empty()) {
+        p = h->prof_pool.back();
+        h->prof_pool.pop_back();
+    } else {
+        hipEventCreate(&p.a);
+        hipEventCreate(&p.b);
+    }
+    p.tag = tag;
+    hipEventRecord(p.a, h->st);
+    h->prof_live.push_back(p);
+    return (int)h->prof_live.size() - 1;
+}
+void prof_end(pic_handle *h, int idx) {
+    if (idx < 0) return;
+    hipEventRecord(h->prof_live[idx].b, h->st);
+}
+void prof_flush(pic_handle *h) {
+    hipStreamSynchronize(h->st);
+    for (ProfPair &p : h->prof_live) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        h->prof_acc[p.tag] += ms;
+        if (p.tag == 2) h->prof_acc[7] += 1.0;
+        h->prof_pool.push_back(p);
+    }
+    h->prof_live.clear();
+}
+
+// ------------------------------------------------------------ species
+int species_alloc(pic_handle *h, Species &s, int64_t cap) {
+    s.cap = cap;
+    for (int b = 0; b < 2; b++) {
+        CHECK(dalloc(h, &s.buf[b].cell, cap));
+        CHECK(dalloc(h, &s.buf[b].ox, cap));
+        CHECK(dalloc(h, &s.buf[b].oy, cap));
+        CHECK(dalloc(h, &s.buf[b].vx, cap));
+        CHECK(dalloc(h, &s.buf[b].vy, cap));
+        CHECK(dalloc(h, &s.buf[b].vz, cap));
+        CHECK(dalloc(h, &s.buf[b].id, cap));
+    }
+    CHECK(dalloc(h, &s.q, (size_t)h->N * h->N));
+    HIPCHK(h, hipMemsetAsync(s.q, 0, sizeof(unsigned long long) * h->N * h->N, h->st));
+    return PIC_OK;
+}
+
+void species_free(Species &s) {
+    for (int b = 0; b < 2; b++) {
+        void *ptrs[] = {s.buf[b].cell, s.buf[b].ox, s.buf[b].oy, s.buf[b].vx,
+                        s.buf[b].vy,   s.buf[b].vz, s.buf[b].id};
+        for (void *p : ptrs)
+            if (p) hipFree(p);
+    }
+    if (s.q) hipFree(s.q);
+    if (s.exp_) hipFree(s.exp_);
+    if (s.eyp) hipFree(s.eyp);
+    s = Species();
+}
+
+void setup_species(pic_handle *h, Species &s, bool electron) {
+    const pic_params &P = h->P;
+    s.q_over_m = electron ? P.electron_charge / P.electron_mass : P.ion_charge / P.ion_mass;
+    s.qc = electron ? P.qce : P.qci;
+    s.mass = electron ? P.electron_mass : P.ion_mass;
+    s.dt = electron ? P.dt : P.dti;
+    s.sigma_load = electron ? P.sigma_load_e : P.sigma_load_i;
+    s.type = electron ? "e" : "i1";
+    s.weight = 1.0;
+    PushConst &pc = s.pc;
+    pc.N = h->N;
+    pc.mode = P.mode == PIC_MODE_PHYSICAL ? PIC_DEV_MODE_PHYS : PIC_DEV_MODE_REF;
+    pc.bfield = P.magnetic_field > 0.0;
+    pc.tilesX = h->tilesX;
+    double coef = 0.5 * s.dt * s.q_over_m;  // particle.cpp:1103
+    double omega_dt = s.dt * s.q_over_m * P.magnetic_field;
+    pc.cE = (float)coef;
+    pc.dtdx = (float)(s.dt / h->dx);
+    pc.dtdy = (float)(s.dt / h->dx);
+    pc.cos_a = (float)cos(P.angle);
+    pc.sin_a = (float)sin(P.angle);
+    pc.cos_w = (float)cos(omega_dt);
+    pc.sin_w = (float)sin(omega_dt);
+}
+
+void update_scale(pic_handle *h, Species &s) {
+    s.scale = ldexp(s.qc / (h->dx * h->dx) * s.weight, -40);
+}
+
+// absolute position (m) -> cell + offset
+void to_cell(double x, double dx, int &j, float &o) {
+    double t = x / dx;
+    if (!(t > -8192.0 && t < 8192.0)) {
+        j = (t > 0.0) ? 16000 : -16000;
+        o = 0.0f;
+        return;
+    }
+    double k = floor(t);
+    float f = (float)(t - k);
+    if (f >= 1.0f) {
+        f = 0.0f;
+        k += 1.0;
+    }
+    j = (int)k;
+    o = f;
+}
+
+int upload_particles(pic_handle *h, int si, const std::vector<double> &v) {
+    Species &s = h->sp[si];
+    size_t n = v.size() / 6;
+    if ((int64_t)n > s.cap) return seterr(h, PIC_ERR_CAPACITY, "particle store too small");
+    std::vector<uint32_t> cell(n), id(n);
+    std::vector<float> ox(n), oy(n), vx(n), vy(n), vz(n);
+    for (size_t i = 0; i < n; i++) {
+        int jx, jy;
+        to_cell(v[6 * i], h->dx, jx, ox[i]);
+        to_cell(v[6 * i + 1], h->dx, jy, oy[i]);
+        cell[i] = cell_pack(jx, jy);
+        vx[i] = (float)v[6 * i + 2];
+        vy[i] = (float)v[6 * i + 3];
+        vz[i] = (float)v[6 * i + 4];
+        id[i] = (uint32_t)i;
+    }
+    SpView &b = s.buf[s.cur];
+    HIPCHK(h, hipMemcpyAsync(b.cell, cell.data(), 4 * n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(b.ox, ox.data(), 4 * n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(b.oy, oy.data(), 4 * n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(b.vx, vx.data(), 4 * n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(b.vy, vy.data(), 4 * n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(b.vz, vz.data(), 4 * n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(b.id, id.data(), 4 * n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return PIC_OK;
+}
+
+int set_count(pic_handle *h, int si, int buf, uint32_t n) {
+    launch_fill<uint32_t>(h->st, &h->cnt[si].n[buf], 1, n);
+    return PIC_OK;
+}
+
+int read_counters(pic_handle *h) {
+    HIPCHK(h, hipMemcpyAsync(h->cnt_host, h->cnt, 2 * sizeof(SpCounters), hipMemcpyDeviceToHost,
+                             h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return PIC_OK;
+}
+
+// counting sort + (optional) push + deposit of one species
+void push_species(pic_handle *h, int si, int do_push, int deposit) {
+    Species &s = h->sp[si];
+    int c = s.cur;
+    int p1 = prof_begin(h, 4);
+    launch_push_count(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, h->binCount,
+                      h->cnt[si].lost, do_push);
+    prof_end(h, p1);
+    int p2 = prof_begin(h, 5);
+    launch_scan(h->st, h->binCount, h->binStart, h->nbins_pad, h->scanPartial, &h->cnt[si].n[1 - c]);
+    prof_end(h, p2);
+    int p3 = prof_begin(h, 2);
+    launch_push_scatter(h->st, s.buf[c], s.buf[1 - c], &h->cnt[si].n[c], s.cap, h->E32, s.pc,
+                        h->binCount, h->binStart, s.q, do_push, deposit);
+    prof_end(h, p3);
+    s.cur = 1 - c;
+}
+
+void absorb_species(pic_handle *h, int si) {
+    Species &s = h->sp[si];
+    int c = s.cur;
+    set_count(h, si, 1 - c, 0);
+    launch_absorb(h->st, s.buf[c], s.buf[1 - c], &h->cnt[si].n[c], &h->cnt[si].n[1 - c], s.cap,
+                  h->N, s.pc.mode, h->cnt[si].lost);
+    s.cur = 1 - c;
+}
+
+void solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, double phiLY) {
+    const pic_params &P = h->P;
+    int N = h->N;
+    launch_fold(h->st, h->rho, N, 1.0 / P.epsilon, phi0X, phiR, phi0Y, phiLY,
+                1.0 / (h->dx * h->dx), h->M.rhsF[h->M.ng - 1]);
+    mg_fixed_solve(h->st, h->M, P.mg_ncycle, P.length);
+    launch_phi_bc(h->st, h->M.u[h->M.ng - 1], N, phi0X, phiR, phi0Y, phiLY, h->phi);
+    h->have_phi = true;
+}
+
+void field_step(pic_handle *h, int64_t step) {
+    const pic_params &P = h->P;
+    double phiR = P.phi_right * cos(P.phi_period * (double)step);
+    int p = prof_begin(h, 3);
+    launch_rho(h->st, h->sp[1].q, h->sp[0].q, h->N, h->sp[1].scale, h->sp[0].scale,
+               P.corner8 ? 8.0 : 4.0, h->rho);
+    solve_potential(h, P.phi_left, phiR, P.phi_bottom, P.phi_top);
+    launch_efield(h->st, h->phi, h->N, h->dx, h->ex64, h->ey64, h->E32);
+    prof_end(h, p);
+}
+
+void fused_step(pic_handle *h) {
+    int N = h->N;
+    int pt = prof_begin(h, 0);
+    int pp = prof_begin(h, 1);
+    launch_fill<unsigned long long>(h->st, h->sp[0].q, (long)N * N, 0ull);
+    launch_fill<unsigned long long>(h->st, h->sp[1].q, (long)N * N, 0ull);
+    push_species(h, 0, 1, 1);
+    if (h->step % h->P.dt_dti == 0) {
+        push_species(h, 1, 1, 1);
+    } else {
+        Species &s = h->sp[1];
+        launch_deposit(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], s.cap, N, s.pc.mode, s.q);
+    }
+    prof_end(h, pp);
+    field_step(h, h->step);
+    prof_end(h, pt);
+    h->have_rho = true;
+    h->step++;
+}
+
+int check_species(pic_handle *h, int s) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    if (s < 0 || s > 1) return seterr(h, PIC_ERR_ARG, "species index must be 0 (e) or 1 (ion)");
+    return PIC_OK;
+}
+
+// lattice positions as particle.cpp:410-443 (start at dxp, step dxp, row by row,
+// wrapping to a second layer if the lattice is exhausted)
+void lattice_positions(int64_t n, double dxp, double L, std::vector<double> &xy) {
+    xy.resize(2 * n);
+    int64_t i = 0;
+    while (i < n) {
+        double y = dxp;
+        while (y < L && i < n) {
+            double x = dxp;
+            while (x < L && i < n) {
+                xy[2 * i] = x;
+                xy[2 * i + 1] = y;
+                x += dxp;
+                ++i;
+            }
+            y += dxp;
+        }
+    }
+}
+
+}  // namespace
+
+// dump_io.cpp
+int read_dump_file(const std::string &path, std::vector<double> &vals, std::string &err);
+int write_dump_file(const std::string &path, const std::vector<pic_particle> &p, int precision);
+
+extern "C" {
+
+const char *pic_last_error(pic_handle *h) {
+    if (h) return h->err.c_str();
+    return global_error();
+}
+
+int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_handle **out) {
+    if (!Pin || !out) return set_global_error(PIC_ERR_ARG, "null argument");
+    *out = nullptr;
+    const pic_params &P = *Pin;
+    if (!grid_ok(P.nodes))
+        return set_global_error(PIC_ERR_GRID, "nodes-1 must be a power of 2 (and nodes >= 5)");
+    if (P.number_to_inject != 0)
+        return set_global_error(PIC_ERR_UNSUPPORTED, "NUMBER_TO_INJECT != 0 is not supported");
+    if (P.no_ion_species != 1)
+        return set_global_error(PIC_ERR_UNSUPPORTED, "NO_ION_SPECIES must be 1");
+    if (P.wall != PIC_WALL_ABSORB)
+        return set_global_error(PIC_ERR_UNSUPPORTED, "steady-state plop walls: not available yet");
+    if (P.have_collisions)
+        return set_global_error(PIC_ERR_UNSUPPORTED, "collisions: not available yet");
+    if (P.mg_mode != PIC_MG_FIXED)
+        return set_global_error(PIC_ERR_UNSUPPORTED, "PIC_MG_TOL: not available yet");
+    if (P.dt_dti < 1) return set_global_error(PIC_ERR_ARG, "DT_DTI must be >= 1");
+    if (P.init_particle_no < 0) return set_global_error(PIC_ERR_ARG, "negative particle count");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_global_error(PIC_ERR_DEVICE, "no HIP device available");
+
+    pic_handle *h = new pic_handle();
+    h->P = P;
+    h->N = P.nodes;
+    h->dx = P.length / (double)(P.nodes - 1);  // parameters.h:193
+    h->run_time = P.run_time;
+    h->tilesX = (h->N + 1 + 7) / 8;
+    h->nbins = (uint32_t)h->tilesX * h->tilesX * 64u;
+    h->nbins_pad = scan_pad(h->nbins);
+    int rc = PIC_OK;
+    auto fail = [&](int code) {
+        pic_destroy(h);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_global_error(PIC_ERR_DEVICE, "hipStreamCreate failed"));
+    setup_species(h, h->sp[0], true);
+    setup_species(h, h->sp[1], false);
+
+    // initial particles
+    std::vector<double> dump[2];
+    int64_t n0[2] = {P.init_particle_no, P.init_particle_no};
+    if (restart) {
+        std::string dir = dump_dir ? dump_dir : ".";
+        for (int s = 0; s < 2; s++) {
+            std::string path = dir + "/dataDump_" + h->sp[s].type + ".txt";
+            std::string err;
+            rc = read_dump_file(path, dump[s], err);
+            if (rc != PIC_OK) return fail(set_global_error(rc, err));
+            n0[s] = (int64_t)(dump[s].size() / 6);
+            double w0 = n0[s] ? dump[s][5] : 1.0;
+            for (int64_t i = 0; i < n0[s]; i++)
+                if (dump[s][6 * i + 5] != w0)
+                    return fail(set_global_error(
+                        PIC_ERR_UNSUPPORTED, path + ": per-particle weights differ (only one "
+                                                    "weight per species is supported)"));
+            h->sp[s].weight = w0;
+        }
+    }
+    for (int s = 0; s < 2; s++) {
+        if (n0[s] > 0x7fffffffLL) return fail(set_global_error(PIC_ERR_CAPACITY, "too many particles"));
+        rc = species_alloc(h, h->sp[s], n0[s] > 0 ? n0[s] : 1);
+        if (rc != PIC_OK) return fail(rc);
+        update_scale(h, h->sp[s]);
+    }
+    if ((rc = dalloc(h, &h->cnt, 2)) != PIC_OK) return fail(rc);
+    if (hipHostMalloc((void **)&h->cnt_host, 2 * sizeof(SpCounters), 0) != hipSuccess)
+        return fail(set_global_error(PIC_ERR_ALLOC, "hipHostMalloc failed"));
+    memset(h->cnt_host, 0, 2 * sizeof(SpCounters));
+    for (int s = 0; s < 2; s++) {
+        h->cnt_host[s].n[0] = (uint32_t)n0[s];
+        h->cnt_host[s].next_id = (uint32_t)n0[s];
+    }
+    if (hipMemcpy(h->cnt, h->cnt_host, 2 * sizeof(SpCounters), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_global_error(PIC_ERR_DEVICE, "counter upload failed"));
+    int N = h->N;
+    if ((rc = dalloc(h, &h->binCount, h->nbins_pad)) != PIC_OK ||
+        (rc = dalloc(h, &h->binStart, h->nbins_pad + 1)) != PIC_OK ||
+        (rc = dalloc(h, &h->scanPartial, h->nbins_pad / 4096 + 1)) != PIC_OK ||
+        (rc = dalloc(h, &h->rho, (size_t)N * N)) != PIC_OK ||
+        (rc = dalloc(h, &h->phi, (size_t)N * N)) != PIC_OK ||
+        (rc = dalloc(h, &h->ex64, (size_t)(N + 1) * (N + 1))) != PIC_OK ||
+        (rc = dalloc(h, &h->ey64, (size_t)(N + 1) * (N + 1))) != PIC_OK ||
+        (rc = dalloc(h, &h->E32, (size_t)(N + 1) * N + 1)) != PIC_OK)
+        return fail(rc);
+    hipMemsetAsync(h->binCount, 0, 4ull * h->nbins_pad, h->st);
+    hipMemsetAsync(h->rho, 0, 8ull * N * N, h->st);
+    hipMemsetAsync(h->phi, 0, 8ull * N * N, h->st);
+    hipMemsetAsync(h->ex64, 0, 8ull * (N + 1) * (N + 1), h->st);
+    hipMemsetAsync(h->ey64, 0, 8ull * (N + 1) * (N + 1), h->st);
+    hipMemsetAsync(h->E32, 0, 8ull * ((N + 1) * N + 1), h->st);  // pic.cpp:1086: first gather sees E = 0
+    if ((rc = alloc_mg(h, h->M, N)) != PIC_OK) return fail(rc);
+
+    for (int s = 0; s < 2; s++) {
+        Species &sp = h->sp[s];
+        if (restart) {
+            if ((rc = upload_particles(h, s, dump[s])) != PIC_OK) return fail(rc);
+            continue;
+        }
+        if (n0[s] == 0) continue;
+        launch_load_random(h->st, sp.buf[sp.cur], (uint32_t)n0[s], P.seed, (uint32_t)s, N,
+                           sp.sigma_load);
+        if (P.loader == PIC_LOAD_LATTICE) {
+            // particle.cpp:410-443: DXPE = XLENGTH/(sqrt(INIT_PARTICLE_NO)+1)
+            double dxp = P.length / (sqrt((double)P.init_particle_no) + 1.0);
+            std::vector<double> xy;
+            lattice_positions(n0[s], dxp, P.length, xy);
+            std::vector<uint32_t> cell(n0[s]);
+            std::vector<float> ox(n0[s]), oy(n0[s]);
+            for (int64_t i = 0; i < n0[s]; i++) {
+                int jx, jy;
+                to_cell(xy[2 * i], h->dx, jx, ox[i]);
+                to_cell(xy[2 * i + 1], h->dx, jy, oy[i]);
+                cell[i] = cell_pack(jx, jy);
+            }
+            hipMemcpyAsync(sp.buf[sp.cur].cell, cell.data(), 4 * n0[s], hipMemcpyHostToDevice, h->st);
+            hipMemcpyAsync(sp.buf[sp.cur].ox, ox.data(), 4 * n0[s], hipMemcpyHostToDevice, h->st);
+            hipMemcpyAsync(sp.buf[sp.cur].oy, oy.data(), 4 * n0[s], hipMemcpyHostToDevice, h->st);
+            hipStreamSynchronize(h->st);
+        }
+        // sort into cell order (no push, no deposit); generated particles are all inside
+        push_species(h, s, 0, 0);
+    }
+    // the loader's sort pass must not count as losses
+    if (read_counters(h) != PIC_OK) return fail(PIC_ERR_DEVICE);
+    for (int s = 0; s < 2; s++)
+        for (int w = 0; w < 4; w++) h->cnt_host[s].lost[w] = 0;
+    if (hipMemcpy(h->cnt, h->cnt_host, 2 * sizeof(SpCounters), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_global_error(PIC_ERR_DEVICE, "counter upload failed"));
+    hipError_t e = hipStreamSynchronize(h->st);
+    if (e != hipSuccess)
+        return fail(set_global_error(PIC_ERR_DEVICE, std::string("setup: ") + hipGetErrorString(e)));
+    *out = h;
+    return PIC_OK;
+}
+
+int pic_destroy(pic_handle *h) {
+    if (!h) return PIC_ERR_ARG;
+    if (h->st) hipStreamSynchronize(h->st);
+    for (int s = 0; s < 2; s++) species_free(h->sp[s]);
+    void *ptrs[] = {h->cnt, h->binCount, h->binStart, h->scanPartial, h->rho,
+                    h->phi, h->ex64,     h->ey64,     h->E32};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (h->cnt_host) hipHostFree(h->cnt_host);
+    free_mg(h->M);
+    for (ProfPair &p : h->prof_live) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    for (ProfPair &p : h->prof_pool) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    if (h->st) hipStreamDestroy(h->st);
+    delete h;
+    return PIC_OK;
+}
+
+int pic_set_runtime(pic_handle *h, int32_t run_time) {
+    if (!h || run_time < 0) return set_global_error(PIC_ERR_ARG, "bad argument");
+    h->run_time = run_time;
+    return PIC_OK;
+}
+
+int pic_run(pic_handle *h) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    // pic.cpp:1049-1057: wall pass before the loop (ion wall, then electron wall)
+    absorb_species(h, 1);
+    absorb_species(h, 0);
+    for (int32_t t = 0; t < h->run_time; t++) fused_step(h);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return seterr(h, PIC_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
+    return PIC_OK;
+}
+
+int pic_sync(pic_handle *h) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return PIC_OK;
+}
+
+// ------------------------------------------------------ per-phase entry
+int pic_species_gather(pic_handle *h, int32_t s) {
+    CHECK(check_species(h, s));
+    Species &sp = h->sp[s];
+    if (!sp.exp_) {
+        CHECK(dalloc(h, &sp.exp_, sp.cap));
+        CHECK(dalloc(h, &sp.eyp, sp.cap));
+    }
+    launch_gather(h->st, sp.buf[sp.cur], &h->cnt[s].n[sp.cur], sp.cap, h->E32, sp.pc, sp.exp_,
+                  sp.eyp);
+    sp.gathered = true;
+    return PIC_OK;
+}
+
+int pic_species_move(pic_handle *h, int32_t s) {
+    CHECK(check_species(h, s));
+    Species &sp = h->sp[s];
+    if (!sp.gathered)
+        return seterr(h, PIC_ERR_ARG, "pic_species_move needs pic_species_gather first");
+    launch_move(h->st, sp.buf[sp.cur], &h->cnt[s].n[sp.cur], sp.cap, sp.exp_, sp.eyp, sp.pc);
+    sp.gathered = false;  // particle.cpp:1336-1339 frees the per-particle field
+    return PIC_OK;
+}
+
+int pic_species_collect_charge(pic_handle *h, int32_t s) {
+    CHECK(check_species(h, s));
+    Species &sp = h->sp[s];
+    launch_fill<unsigned long long>(h->st, sp.q, (long)h->N * h->N, 0ull);
+    launch_deposit(h->st, sp.buf[sp.cur], &h->cnt[s].n[sp.cur], sp.cap, h->N, sp.pc.mode, sp.q);
+    return PIC_OK;
+}
+
+int pic_fields_grid_rho(pic_handle *h) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    launch_rho(h->st, h->sp[1].q, h->sp[0].q, h->N, h->sp[1].scale, h->sp[0].scale,
+               h->P.corner8 ? 8.0 : 4.0, h->rho);
+    h->have_rho = true;
+    return PIC_OK;
+}
+
+int pic_fields_grid_potential(pic_handle *h, double phi0X, double phiLX, double phi0Y,
+                              double phiLY, double timestep, double phi_period) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    double phiR = phiLX * cos(phi_period * timestep);  // fields.cpp:177
+    solve_potential(h, phi0X, phiR, phi0Y, phiLY);
+    return PIC_OK;
+}
+
+int pic_fields_grid_electric_field(pic_handle *h) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    launch_efield(h->st, h->phi, h->N, h->dx, h->ex64, h->ey64, h->E32);
+    return PIC_OK;
+}
+
+int pic_e_collisions(pic_handle *h) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    if (h->P.have_collisions) return seterr(h, PIC_ERR_UNSUPPORTED, "collisions not available yet");
+    return PIC_OK;
+}
+
+int pic_ion_collisions(pic_handle *h) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    if (h->P.have_collisions) return seterr(h, PIC_ERR_UNSUPPORTED, "collisions not available yet");
+    return PIC_OK;
+}
+
+int pic_boundary(pic_handle *h, int32_t which) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    if (which != 0 && which != 1) return seterr(h, PIC_ERR_ARG, "which must be 0 (ion) or 1 (electron)");
+    absorb_species(h, which == 0 ? 1 : 0);
+    return PIC_OK;
+}
+
+int pic_mglin(double *u, int32_t n, int32_t ncycle, double length) {
+    if (!u || ncycle < 1) return set_global_error(PIC_ERR_ARG, "bad argument");
+    int m = n - 1;
+    if (n < 3 || (m & (m - 1)) != 0)
+        return set_global_error(PIC_ERR_GRID, "n-1 must be a power of 2 in mglin.");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_global_error(PIC_ERR_DEVICE, "no HIP device available");
+    pic_handle tmp;
+    int rc = alloc_mg(&tmp, tmp.M, n);
+    hipStream_t st = nullptr;
+    if (rc == PIC_OK && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+        rc = set_global_error(PIC_ERR_DEVICE, "stream");
+    if (rc == PIC_OK) {
+        size_t bytes = sizeof(double) * n * n;
+        MgLevels &M = tmp.M;
+        hipMemcpyAsync(M.rhsF[M.ng - 1], u, bytes, hipMemcpyHostToDevice, st);
+        mg_fixed_solve(st, M, ncycle, length);
+        hipMemcpyAsync(u, M.u[M.ng - 1], bytes, hipMemcpyDeviceToHost, st);
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = set_global_error(PIC_ERR_DEVICE, hipGetErrorString(e));
+    }
+    if (st) hipStreamDestroy(st);
+    free_mg(tmp.M);
+    return rc;
+}
+
+// ------------------------------------------------------------ getters
+int pic_particle_size(pic_handle *h, int32_t s, int64_t *n) {
+    CHECK(check_species(h, s));
+    if (!n) return seterr(h, PIC_ERR_ARG, "null output");
+    CHECK(read_counters(h));
+    *n = h->cnt_host[s].n[h->sp[s].cur];
+    return PIC_OK;
+}
+
+int pic_get_state(pic_handle *h, int32_t s, uint32_t *cell, float *ox, float *oy, float *vx,
+                  float *vy, float *vz, uint32_t *id, int64_t cap) {
+    int64_t n;
+    CHECK(pic_particle_size(h, s, &n));
+    if (n > cap) return seterr(h, PIC_ERR_CAPACITY, "output buffer too small");
+    const SpView &b = h->sp[s].buf[h->sp[s].cur];
+    size_t bytes = 4 * (size_t)n;
+    if (cell) HIPCHK(h, hipMemcpyAsync(cell, b.cell, bytes, hipMemcpyDeviceToHost, h->st));
+    if (ox) HIPCHK(h, hipMemcpyAsync(ox, b.ox, bytes, hipMemcpyDeviceToHost, h->st));
+    if (oy) HIPCHK(h, hipMemcpyAsync(oy, b.oy, bytes, hipMemcpyDeviceToHost, h->st));
+    if (vx) HIPCHK(h, hipMemcpyAsync(vx, b.vx, bytes, hipMemcpyDeviceToHost, h->st));
+    if (vy) HIPCHK(h, hipMemcpyAsync(vy, b.vy, bytes, hipMemcpyDeviceToHost, h->st));
+    if (vz) HIPCHK(h, hipMemcpyAsync(vz, b.vz, bytes, hipMemcpyDeviceToHost, h->st));
+    if (id) HIPCHK(h, hipMemcpyAsync(id, b.id, bytes, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return PIC_OK;
+}
+
+int pic_set_state(pic_handle *h, int32_t s, const uint32_t *cell, const float *ox, const float *oy,
+                  const float *vx, const float *vy, const float *vz, const uint32_t *id,
+                  int64_t n) {
+    CHECK(check_species(h, s));
+    if (n < 0 || n > h->sp[s].cap) return seterr(h, PIC_ERR_CAPACITY, "more particles than capacity");
+    if (!cell || !ox || !oy || !vx || !vy || !vz || !id) return seterr(h, PIC_ERR_ARG, "null array");
+    Species &sp = h->sp[s];
+    const SpView &b = sp.buf[sp.cur];
+    size_t bytes = 4 * (size_t)n;
+    HIPCHK(h, hipMemcpyAsync(b.cell, cell, bytes, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(b.ox, ox, bytes, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(b.oy, oy, bytes, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(b.vx, vx, bytes, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(b.vy, vy, bytes, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(b.vz, vz, bytes, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(b.id, id, bytes, hipMemcpyHostToDevice, h->st));
+    set_count(h, s, sp.cur, (uint32_t)n);
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return PIC_OK;
+}
+
+int pic_get_particles(pic_handle *h, int32_t s, pic_particle *out, int64_t cap) {
+    int64_t n;
+    CHECK(pic_particle_size(h, s, &n));
+    if (n > cap || !out) return seterr(h, PIC_ERR_CAPACITY, "output buffer too small");
+    std::vector<uint32_t> cell(n);
+    std::vector<float> ox(n), oy(n), vx(n), vy(n), vz(n);
+    CHECK(pic_get_state(h, s, cell.data(), ox.data(), oy.data(), vx.data(), vy.data(), vz.data(),
+                        nullptr, n));
+    double dx = h->dx, w = h->sp[s].weight;
+    for (int64_t i = 0; i < n; i++) {
+        out[i].x = ((double)cell_jx(cell[i]) + (double)ox[i]) * dx;
+        out[i].y = ((double)cell_jy(cell[i]) + (double)oy[i]) * dx;
+        out[i].vx = vx[i];
+        out[i].vy = vy[i];
+        out[i].vz = vz[i];
+        out[i].weight = w;
+    }
+    return PIC_OK;
+}
+
+static int copy_out(pic_handle *h, void *dst, const void *src, size_t bytes) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    if (!dst) return seterr(h, PIC_ERR_ARG, "null output");
+    HIPCHK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return PIC_OK;
+}
+
+int pic_get_phi(pic_handle *h, double *out) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    return copy_out(h, out, h->phi, 8ull * h->N * h->N);
+}
+int pic_get_rho(pic_handle *h, double *out) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    return copy_out(h, out, h->rho, 8ull * h->N * h->N);
+}
+int pic_get_E(pic_handle *h, double *ex, double *ey) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    size_t b = 8ull * (h->N + 1) * (h->N + 1);
+    CHECK(copy_out(h, ex, h->ex64, b));
+    return copy_out(h, ey, h->ey64, b);
+}
+int pic_get_E32(pic_handle *h, float *out) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    return copy_out(h, out, h->E32, 8ull * ((h->N + 1) * h->N + 1));
+}
+int pic_set_E32(pic_handle *h, const float *in) {
+    if (!h || !in) return set_global_error(PIC_ERR_ARG, "null argument");
+    HIPCHK(h, hipMemcpyAsync(h->E32, in, 8ull * ((h->N + 1) * h->N + 1), hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return PIC_OK;
+}
+int pic_get_qfixed(pic_handle *h, int32_t s, int64_t *out) {
+    CHECK(check_species(h, s));
+    return copy_out(h, out, h->sp[s].q, 8ull * h->N * h->N);
+}
+int pic_get_q(pic_handle *h, int32_t s, double *out) {
+    CHECK(check_species(h, s));
+    int N = h->N;
+    std::vector<int64_t> q((size_t)N * N);
+    CHECK(pic_get_qfixed(h, s, q.data()));
+    double corner = h->P.corner8 ? 8.0 : 4.0, sc = h->sp[s].scale;
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+            bool ex = (i == 0 || i == N - 1), ey = (j == 0 || j == N - 1);
+            double fac = (ex && ey) ? corner : ((ex || ey) ? 2.0 : 1.0);
+            out[(size_t)j * N + i] = ((double)q[(size_t)j * N + i] * sc) * fac;
+        }
+    return PIC_OK;
+}
+int pic_get_step(pic_handle *h, int64_t *step) {
+    if (!h || !step) return set_global_error(PIC_ERR_ARG, "null argument");
+    *step = h->step;
+    return PIC_OK;
+}
+int pic_get_nodes(pic_handle *h, int32_t *nodes) {
+    if (!h || !nodes) return set_global_error(PIC_ERR_ARG, "null argument");
+    *nodes = h->N;
+    return PIC_OK;
+}
+int pic_get_loss_counts(pic_handle *h, int32_t s, int64_t *out4) {
+    CHECK(check_species(h, s));
+    if (!out4) return seterr(h, PIC_ERR_ARG, "null output");
+    CHECK(read_counters(h));
+    for (int w = 0; w < 4; w++) out4[w] = h->cnt_host[s].lost[w];
+    return PIC_OK;
+}
+
+int pic_dump_data(pic_handle *h, const char *dir, int32_t precision) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    if (precision < 1 || precision > 17) return seterr(h, PIC_ERR_ARG, "precision must be 1..17");
+    std::string d = dir ? dir : ".";
+    for (int s = 0; s < 2; s++) {
+        int64_t n;
+        CHECK(pic_particle_size(h, s, &n));
+        std::vector<pic_particle> p(n);
+        CHECK(pic_get_particles(h, s, p.data(), n));
+        int rc = write_dump_file(d + "/dataDump_" + h->sp[s].type + ".txt", p, precision);
+        if (rc != PIC_OK) return seterr(h, rc, "cannot write dataDump in " + d);
+    }
+    return PIC_OK;
+}
+
+int pic_profile_enable(pic_handle *h, int32_t on) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    if (!on && h->prof) prof_flush(h);
+    h->prof = on != 0;
+    for (double &v : h->prof_acc) v = 0.0;
+    return PIC_OK;
+}
+
+int pic_profile_read(pic_handle *h, double *out8) {
+    if (!h || !out8) return set_global_error(PIC_ERR_ARG, "null argument");
+    prof_flush(h);
+    // out: [0] step ms, [1] push-phase ms, [2] scatter-kernel ms, [3] field ms,
+    //      [4] count-kernel ms, [5] scan ms, [6] unused, [7] scatter launches
+    for (int k = 0; k < 8; k++) out8[k] = h->prof_acc[k];
+    for (double &v : h->prof_acc) v = 0.0;
+    return PIC_OK;
+}
+
+}  // extern "C"

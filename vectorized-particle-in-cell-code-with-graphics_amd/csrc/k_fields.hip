@@ -1,0 +1,465 @@
+// k_fields.hip -- charge -> rhs, multigrid Poisson solve, E = -grad(phi).
+//
+// Multigrid (mglin.cpp:21-196): full multigrid with ncycle V(1,1) cycles per
+// level, red-black Gauss-Seidel, half-weighting restriction, bilinear
+// prolongation.  Every per-node expression keeps the reference's FP64
+// association, so FIXED mode reproduces Mglin bit for bit.  Levels up to
+// 65x65 (or what the LDS holds) run inside one workgroup with all arrays in
+// LDS; finer levels use one launch per operation.
+#include "pic_internal.h"
+
+namespace pic {
+
+// ---------------------------------------------------------------- rho
+// particle.cpp:830-845 (edges x2, corners x8) applied to each species'
+// fixed-point charge, then fields.cpp:100-143 (rho = ion + electron).
+__global__ void k_rho(const unsigned long long *__restrict__ qi,
+                      const unsigned long long *__restrict__ qe, int N, double scale_i,
+                      double scale_e, double corner, double *__restrict__ rho) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (i >= N) return;
+    bool ex = (i == 0 || i == N - 1), ey = (j == 0 || j == N - 1);
+    double fac = (ex && ey) ? corner : ((ex || ey) ? 2.0 : 1.0);
+    long k = (long)j * N + i;
+    double ri = ((double)(long long)qi[k] * scale_i) * fac;
+    double re = ((double)(long long)qe[k] * scale_e) * fac;
+    rho[k] = ri + re;
+}
+
+// fields.cpp:184-204: rhs = -rho/EPSILON, then the Dirichlet values folded
+// into the rows/columns next to the walls (x folds first, then y folds).
+__global__ void k_fold(const double *__restrict__ rho, int N, double coef, double phi0X,
+                       double phiR, double phi0Y, double phiLY, double dxdy_inv,
+                       double *__restrict__ rhs) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (i >= N) return;
+    long k = (long)j * N + i;
+    double f = -coef * rho[k];
+    if (i == 1) f = f - phi0X * dxdy_inv;
+    if (i == N - 2) f = f - phiR * dxdy_inv;
+    if (j == 1) f = f - phi0Y * dxdy_inv;
+    if (j == N - 2) f = f - phiLY * dxdy_inv;
+    rhs[k] = f;
+}
+
+void launch_rho(hipStream_t st, const unsigned long long *qi, const unsigned long long *qe, int N,
+                double scale_i, double scale_e, double corner, double *rho) {
+    dim3 g((N + 127) / 128, N);
+    hipLaunchKernelGGL(k_rho, g, dim3(128), 0, st, qi, qe, N, scale_i, scale_e, corner, rho);
+}
+
+void launch_fold(hipStream_t st, const double *rho, int N, double coef, double phi0X, double phiR,
+                 double phi0Y, double phiLY, double dxdy_inv, double *rhs) {
+    dim3 g((N + 127) / 128, N);
+    hipLaunchKernelGGL(k_fold, g, dim3(128), 0, st, rho, N, coef, phi0X, phiR, phi0Y, phiLY,
+                       dxdy_inv, rhs);
+}
+
+// ------------------------------------------------------------ phi, E
+// fields.cpp:209-219: boundary columns, then boundary rows (rows win corners).
+__global__ void k_phi_bc(const double *__restrict__ u, int N, double phi0X, double phiR,
+                         double phi0Y, double phiLY, double *__restrict__ phi) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (i >= N) return;
+    double v = u[(long)j * N + i];
+    if (i == 0) v = phi0X;
+    if (i == N - 1) v = phiR;
+    if (j == 0) v = phi0Y;
+    if (j == N - 1) v = phiLY;
+    phi[(long)j * N + i] = v;
+}
+
+void launch_phi_bc(hipStream_t st, const double *u, int N, double phi0X, double phiR,
+                   double phi0Y, double phiLY, double *phi) {
+    dim3 g((N + 127) / 128, N);
+    hipLaunchKernelGGL(k_phi_bc, g, dim3(128), 0, st, u, N, phi0X, phiR, phi0Y, phiLY, phi);
+}
+
+// fields.cpp:230-268 evaluated per node: central differences inside, linear
+// extrapolation at the walls, never-written entries (Ex in rows 0 and N-1, Ey
+// in columns 0 and N-1, the padding row) are 0.
+__device__ __forceinline__ double exi(const double *phi, int N, int j, int i, double twoDX) {
+    if (j < 1 || j > N - 2 || i < 1 || i > N - 2) return 0.0;
+    return (phi[(long)j * N + i - 1] - phi[(long)j * N + i + 1]) / twoDX;
+}
+__device__ __forceinline__ double eyi(const double *phi, int N, int j, int i, double twoDX) {
+    if (j < 1 || j > N - 2 || i < 1 || i > N - 2) return 0.0;
+    return (phi[(long)(j - 1) * N + i] - phi[(long)(j + 1) * N + i]) / twoDX;
+}
+
+__global__ void k_efield(const double *__restrict__ phi, int N, double twoDX,
+                         double *__restrict__ ex64, double *__restrict__ ey64,
+                         float2 *__restrict__ E32) {
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long tot32 = (long)(N + 1) * N + 1, tot64 = (long)(N + 1) * (N + 1);
+    if (k >= tot64) return;
+    if (k >= (long)N * N) {
+        if (k < tot32) E32[k] = make_float2(0.0f, 0.0f);
+        ex64[k] = 0.0;
+        ey64[k] = 0.0;
+        return;
+    }
+    int j = (int)(k / N), i = (int)(k % N);
+    double ex, ey;
+    if (i == 0)
+        ex = 2.0 * exi(phi, N, j, 1, twoDX) - exi(phi, N, j, 2, twoDX);
+    else if (i == N - 1)
+        ex = 2.0 * exi(phi, N, j, N - 2, twoDX) - exi(phi, N, j, N - 3, twoDX);
+    else
+        ex = exi(phi, N, j, i, twoDX);
+    if (j == 0)
+        ey = 2.0 * eyi(phi, N, 1, i, twoDX) - eyi(phi, N, 2, i, twoDX);
+    else if (j == N - 1)
+        ey = 2.0 * eyi(phi, N, N - 2, i, twoDX) - eyi(phi, N, N - 3, i, twoDX);
+    else
+        ey = eyi(phi, N, j, i, twoDX);
+    ex64[k] = ex;
+    ey64[k] = ey;
+    E32[k] = make_float2((float)ex, (float)ey);
+}
+
+void launch_efield(hipStream_t st, const double *phi, int N, double dx, double *ex64, double *ey64,
+                   float2 *E32) {
+    long tot = (long)(N + 1) * (N + 1);
+    hipLaunchKernelGGL(k_efield, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, phi, N,
+                       2.0 * dx, ex64, ey64, E32);
+}
+
+// ------------------------------------------------------ multigrid: nodes
+// Arrays are row-major, u[i*n + j] == NR's u[i][j].
+
+template <typename T>
+__device__ __forceinline__ T relax_node(const T *u, const T *rhs, int n, int i, int j, T h2) {
+    return (T)0.25 * (u[(i + 1) * n + j] + u[(i - 1) * n + j] + u[i * n + j + 1] +
+                      u[i * n + j - 1] - h2 * rhs[i * n + j]);
+}
+
+template <typename T>
+__device__ __forceinline__ T resid_node(const T *u, const T *rhs, int n, int i, int j, T h2i) {
+    return -h2i * (u[(i + 1) * n + j] + u[(i - 1) * n + j] + u[i * n + j + 1] +
+                   u[i * n + j - 1] - (T)4.0 * u[i * n + j]) +
+           rhs[i * n + j];
+}
+
+// half weighting of the residual at coarse (ic,jc), residual evaluated on
+// the fly (mglin.cpp:151-169 applied to mglin.cpp:133-144's output)
+template <typename T>
+__device__ __forceinline__ T resid_rstrct_node(const T *u, const T *rhs, int nf, int ic, int jc,
+                                               T h2i) {
+    int nc = (nf + 1) / 2;
+    if (ic == 0 || jc == 0 || ic == nc - 1 || jc == nc - 1) return (T)0.0;
+    int i = 2 * ic, j = 2 * jc;
+    return (T)0.5 * resid_node(u, rhs, nf, i, j, h2i) +
+           (T)0.125 * (resid_node(u, rhs, nf, i + 1, j, h2i) + resid_node(u, rhs, nf, i - 1, j, h2i) +
+                       resid_node(u, rhs, nf, i, j + 1, h2i) + resid_node(u, rhs, nf, i, j - 1, h2i));
+}
+
+template <typename T>
+__device__ __forceinline__ T rstrct_node(const T *uf, int nc, int ic, int jc) {
+    int nf = 2 * nc - 1, ncc = 2 * nc - 2;
+    if (jc == 0) return uf[(2 * ic) * nf + 0];
+    if (jc == nc - 1 && ic != 0 && ic != nc - 1) return uf[(2 * ic) * nf + ncc];
+    if (ic == 0) return uf[0 * nf + 2 * jc];
+    if (ic == nc - 1) return uf[ncc * nf + 2 * jc];
+    int i = 2 * ic, j = 2 * jc;
+    return (T)0.5 * uf[i * nf + j] + (T)0.125 * (uf[(i + 1) * nf + j] + uf[(i - 1) * nf + j] +
+                                                 uf[i * nf + j + 1] + uf[i * nf + j - 1]);
+}
+
+// bilinear prolongation value at fine (i,j) (mglin.cpp:70-82)
+template <typename T>
+__device__ __forceinline__ T interp_even_col(const T *uc, int nc, int i, int j) {
+    if ((i & 1) == 0) return uc[(i >> 1) * nc + (j >> 1)];
+    return (T)0.5 * (uc[((i + 1) >> 1) * nc + (j >> 1)] + uc[((i - 1) >> 1) * nc + (j >> 1)]);
+}
+template <typename T>
+__device__ __forceinline__ T interp_node(const T *uc, int nc, int i, int j) {
+    if ((j & 1) == 0) return interp_even_col(uc, nc, i, j);
+    return (T)0.5 * (interp_even_col(uc, nc, i, j + 1) + interp_even_col(uc, nc, i, j - 1));
+}
+
+// ------------------------------------------- multigrid: grid-wide kernels
+
+template <typename T>
+__global__ void k_mg_relax(T *__restrict__ u, const T *__restrict__ rhs, int n, T h2, int color) {
+    int i = blockIdx.y + 1;            // interior row
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = 1 + ((i + 1 + color) & 1) + 2 * t;  // (i+j)%2 == color
+    if (i > n - 2 || j > n - 2) return;
+    u[i * n + j] = relax_node(u, rhs, n, i, j, h2);
+}
+
+template <typename T>
+__global__ void k_mg_resid_rstrct(const T *__restrict__ u, const T *__restrict__ rhs, int nf, T h2i,
+                                  T *__restrict__ uc) {
+    int nc = (nf + 1) / 2;
+    int jc = blockIdx.x * blockDim.x + threadIdx.x, ic = blockIdx.y;
+    if (jc >= nc) return;
+    uc[ic * nc + jc] = resid_rstrct_node(u, rhs, nf, ic, jc, h2i);
+}
+
+template <typename T>
+__global__ void k_mg_rstrct(T *__restrict__ uc, int nc, const T *__restrict__ uf) {
+    int jc = blockIdx.x * blockDim.x + threadIdx.x, ic = blockIdx.y;
+    if (jc >= nc) return;
+    uc[ic * nc + jc] = rstrct_node(uf, nc, ic, jc);
+}
+
+template <typename T>
+__global__ void k_mg_interp(T *__restrict__ uf, int nf, const T *__restrict__ uc) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= nf) return;
+    uf[i * nf + j] = interp_node(uc, (nf + 1) / 2, i, j);
+}
+
+template <typename T>
+__global__ void k_mg_addint(T *__restrict__ uf, int nf, const T *__restrict__ uc) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= nf) return;
+    uf[i * nf + j] = uf[i * nf + j] + interp_node(uc, (nf + 1) / 2, i, j);
+}
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ p, long n, T v) {
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+template <typename T>
+void launch_fill(hipStream_t st, T *p, long n, T v) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+template void launch_fill<double>(hipStream_t, double *, long, double);
+template void launch_fill<float>(hipStream_t, float *, long, float);
+template void launch_fill<unsigned long long>(hipStream_t, unsigned long long *, long,
+                                              unsigned long long);
+template void launch_fill<uint32_t>(hipStream_t, uint32_t *, long, uint32_t);
+
+// --------------------------------------- multigrid: one-workgroup coarse part
+
+static __host__ __device__ inline int lvl_size(int l) { return (1 << (l + 1)) + 1; }
+
+template <typename T>
+struct CoarseLayout {
+    T *rhs[8];
+    T *cres[8];
+    T *cv[8];
+    T *u;
+    T *tmp;
+};
+
+// doubles (or floats) of LDS needed for a coarse part topped at level lc
+static __host__ __device__ inline long coarse_lds_elems(int lc) {
+    long e = 0;
+    for (int l = 0; l <= lc; l++) e += (long)lvl_size(l) * lvl_size(l);  // rhs
+    for (int l = 0; l < lc; l++) e += 2l * lvl_size(l) * lvl_size(l);    // cres, cv
+    e += 2l * lvl_size(lc) * lvl_size(lc);                               // u, tmp
+    return e;
+}
+
+template <typename T>
+__device__ void co_relax(T *u, const T *rhs, int n, T h2) {
+    int m = n - 2;  // interior per row
+    for (int color = 0; color < 2; color++) {
+        int half = (m + 1) / 2;
+        for (int k = threadIdx.x; k < m * half; k += blockDim.x) {
+            int i = 1 + k / half, t = k % half;
+            int j = 1 + ((i + 1 + color) & 1) + 2 * t;
+            if (j <= n - 2) u[i * n + j] = relax_node(u, rhs, n, i, j, h2);
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__device__ void co_resid(T *res, const T *u, const T *rhs, int n, T h2i) {
+    for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+        int i = k / n, j = k % n;
+        res[k] = (i == 0 || j == 0 || i == n - 1 || j == n - 1) ? (T)0.0
+                                                                 : resid_node(u, rhs, n, i, j, h2i);
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void co_rstrct(T *uc, int nc, const T *uf) {
+    for (int k = threadIdx.x; k < nc * nc; k += blockDim.x) uc[k] = rstrct_node(uf, nc, k / nc, k % nc);
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void co_interp(T *uf, int nf, const T *uc, bool add) {
+    int nc = (nf + 1) / 2;
+    for (int k = threadIdx.x; k < nf * nf; k += blockDim.x) {
+        T v = interp_node(uc, nc, k / nf, k % nf);
+        uf[k] = add ? uf[k] + v : v;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void co_zero(T *p, int n) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) p[k] = (T)0.0;
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void co_slvsml(T *u, const T *rhs, double length) {
+    // mglin.cpp:103-110
+    if (threadIdx.x < 9) {
+        T h = (T)(0.5 * length);
+        u[threadIdx.x] = (threadIdx.x == 4) ? -h * h * rhs[4] / (T)4.0 : (T)0.0;
+    }
+    __syncthreads();
+}
+
+// V(1,1) cycle at level j on (U, R), recursion unrolled (mglin.cpp:176-196)
+template <typename T>
+__device__ void co_vcycle(CoarseLayout<T> &L, int j, T *U, const T *R, const T *h2, const T *h2i,
+                          double length) {
+    T *Ul = U;
+    const T *Rl = R;
+    for (int l = j; l >= 1; l--) {
+        int n = lvl_size(l);
+        co_relax(Ul, Rl, n, h2[l]);
+        co_resid(L.tmp, Ul, Rl, n, h2i[l]);
+        co_rstrct(L.cres[l - 1], lvl_size(l - 1), L.tmp);
+        co_zero(L.cv[l - 1], lvl_size(l - 1) * lvl_size(l - 1));
+        Ul = L.cv[l - 1];
+        Rl = L.cres[l - 1];
+    }
+    co_slvsml(L.cv[0], L.cres[0], length);
+    for (int l = 1; l <= j; l++) {
+        T *Uu = (l == j) ? U : L.cv[l];
+        const T *Ru = (l == j) ? R : L.cres[l];
+        co_interp(Uu, lvl_size(l), L.cv[l - 1], true);
+        co_relax(Uu, Ru, lvl_size(l), h2[l]);
+    }
+}
+
+// mode 0: full multigrid from rhs_in (level lc) -> u_out;  mode 1: ncycle
+// V-cycles on a zero initial guess (the coarse-grid correction of a finer
+// V-cycle) -> u_out.
+template <typename T>
+__global__ void __launch_bounds__(1024)
+    k_mg_coarse(const T *__restrict__ rhs_in, T *__restrict__ u_out, int lc, int mode, int ncycle,
+                double length) {
+    extern __shared__ double lds_raw[];
+    T *base = reinterpret_cast<T *>(lds_raw);
+    CoarseLayout<T> L;
+    long off = 0;
+    for (int l = 0; l <= lc; l++) {
+        L.rhs[l] = base + off;
+        off += (long)lvl_size(l) * lvl_size(l);
+    }
+    for (int l = 0; l < lc; l++) {
+        L.cres[l] = base + off;
+        off += (long)lvl_size(l) * lvl_size(l);
+        L.cv[l] = base + off;
+        off += (long)lvl_size(l) * lvl_size(l);
+    }
+    int nc = lvl_size(lc);
+    L.u = base + off;
+    off += (long)nc * nc;
+    L.tmp = base + off;
+    T h2[8], h2i[8];
+    for (int l = 0; l <= lc; l++) {
+        double h = length / (lvl_size(l) - 1);
+        h2[l] = (T)(h * h);
+        h2i[l] = (T)(1.0 / (h * h));
+    }
+    for (int k = threadIdx.x; k < nc * nc; k += blockDim.x) L.rhs[lc][k] = rhs_in[k];
+    __syncthreads();
+    if (mode == 0) {
+        for (int l = lc - 1; l >= 0; l--) co_rstrct(L.rhs[l], lvl_size(l), L.rhs[l + 1]);
+        T *prev = (lc == 0) ? L.u : L.cv[0];
+        co_slvsml(prev, L.rhs[0], length);
+        for (int j = 1; j <= lc; j++) {
+            T *Uj = (j == lc) ? L.u : L.cv[j];
+            co_interp(Uj, lvl_size(j), prev, false);
+            for (int c = 0; c < ncycle; c++) co_vcycle(L, j, Uj, L.rhs[j], h2, h2i, length);
+            prev = Uj;
+        }
+    } else {
+        co_zero(L.u, nc * nc);
+        for (int c = 0; c < ncycle; c++) co_vcycle(L, lc, L.u, L.rhs[lc], h2, h2i, length);
+    }
+    for (int k = threadIdx.x; k < nc * nc; k += blockDim.x) u_out[k] = L.u[k];
+}
+
+// ------------------------------------------------------------ MG driver
+
+template <typename T>
+static void relax_launch(hipStream_t st, T *u, const T *rhs, int n, double length) {
+    double h = length / (n - 1);
+    T h2 = (T)(h * h);
+    int m = n - 2, half = (m + 1) / 2;
+    dim3 g((half + 127) / 128, m);
+    for (int c = 0; c < 2; c++) hipLaunchKernelGGL(k_mg_relax<T>, g, dim3(128), 0, st, u, rhs, n, h2, c);
+}
+
+template <typename T>
+static void vcycle_launch(hipStream_t st, MgLevels &M, int j, T *Uj, const T *Rj, double length);
+
+template <>
+void vcycle_launch<double>(hipStream_t st, MgLevels &M, int j, double *Uj, const double *Rj,
+                           double length) {
+    int nf = lvl_size(j), nc = lvl_size(j - 1);
+    relax_launch<double>(st, Uj, Rj, nf, length);
+    double h = length / (nf - 1);
+    hipLaunchKernelGGL(k_mg_resid_rstrct<double>, dim3((nc + 127) / 128, nc), dim3(128), 0, st, Uj,
+                       Rj, nf, 1.0 / (h * h), M.cres[j - 1]);
+    if (j - 1 == M.lc) {
+        hipLaunchKernelGGL(k_mg_coarse<double>, dim3(1), dim3(1024),
+                           coarse_lds_elems(M.lc) * sizeof(double), st, M.cres[j - 1], M.cv[j - 1],
+                           M.lc, 1, 1, length);
+    } else {
+        long s = (long)nc * nc;
+        hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)((s + 255) / 256)), dim3(256), 0, st,
+                           M.cv[j - 1], s, 0.0);
+        vcycle_launch<double>(st, M, j - 1, M.cv[j - 1], M.cres[j - 1], length);
+    }
+    hipLaunchKernelGGL(k_mg_addint<double>, dim3((nf + 127) / 128, nf), dim3(128), 0, st, Uj, nf,
+                       M.cv[j - 1]);
+    relax_launch<double>(st, Uj, Rj, nf, length);
+}
+
+int mg_coarse_level_for(long lds_bytes, int ng) {
+    int lc = 0;
+    for (int l = 1; l <= 5 && l < ng; l++)
+        if (coarse_lds_elems(l) * (long)sizeof(double) <= lds_bytes) lc = l;
+    return lc;
+}
+
+void mg_set_coarse_attr(long lds_bytes) {
+    hipFuncSetAttribute((const void *)k_mg_coarse<double>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    hipFuncSetAttribute((const void *)k_mg_coarse<float>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+}
+
+// FP64 full multigrid (mglin.cpp:21-55) on the device.  rhs = M.rhsF[ng-1]
+// (finest) on entry; the solution is written to M.u[ng-1].
+void mg_fixed_solve(hipStream_t st, MgLevels &M, int ncycle, double length) {
+    int lf = M.ng - 1, lc = M.lc;
+    for (int l = lf - 1; l >= lc; l--) {
+        int nc = lvl_size(l);
+        hipLaunchKernelGGL(k_mg_rstrct<double>, dim3((nc + 127) / 128, nc), dim3(128), 0, st,
+                           M.rhsF[l], nc, M.rhsF[l + 1]);
+    }
+    hipLaunchKernelGGL(k_mg_coarse<double>, dim3(1), dim3(1024),
+                       coarse_lds_elems(lc) * sizeof(double), st, M.rhsF[lc], M.u[lc], lc, 0,
+                       ncycle, length);
+    for (int j = lc + 1; j <= lf; j++) {
+        int nf = lvl_size(j);
+        hipLaunchKernelGGL(k_mg_interp<double>, dim3((nf + 127) / 128, nf), dim3(128), 0, st,
+                           M.u[j], nf, M.u[j - 1]);
+        for (int c = 0; c < ncycle; c++) vcycle_launch<double>(st, M, j, M.u[j], M.rhsF[j], length);
+    }
+}
+
+}  // namespace pic

@@ -1,0 +1,449 @@
+// k_particles.hip -- particle kernels.
+//
+// Fused timestep of one species (two passes over the particle store):
+//   pass 1  k_push_count:   gather + push + wall test, histogram of the new
+//                           cells (counting-sort key), per-wall loss counts;
+//   scan    k_scan_*:       exclusive scan of the histogram -> bin starts;
+//   pass 2  k_push_scatter: the identical gather + push (bitwise the same
+//                           result), scatter of the survivor into its cell's
+//                           slot of the other buffer (stream compaction of the
+//                           lost particles comes for free), and the fixed-point
+//                           CIC deposit accumulated in an LDS window around the
+//                           block's cells, flushed with one 64-bit atomic per
+//                           touched node.
+// Per-phase kernels for the reference-shaped entry points (gather, move,
+// collect_charge, wall pass) reuse the same device functions.
+#include "pic_internal.h"
+
+namespace pic {
+
+__device__ __forceinline__ PState load_state(const SpView &v, uint32_t i, bool need_vz) {
+    PState s;
+    uint32_t c = v.cell[i];
+    s.jx = cell_jx(c);
+    s.jy = cell_jy(c);
+    s.ox = v.ox[i];
+    s.oy = v.oy[i];
+    s.vx = v.vx[i];
+    s.vy = v.vy[i];
+    s.vz = need_vz ? v.vz[i] : 0.0f;
+    return s;
+}
+
+__device__ __forceinline__ void store_state(const SpView &v, uint32_t i, const PState &s,
+                                            uint32_t id) {
+    v.cell[i] = cell_pack(s.jx, s.jy);
+    v.ox[i] = s.ox;
+    v.oy[i] = s.oy;
+    v.vx[i] = s.vx;
+    v.vy[i] = s.vy;
+    v.vz[i] = s.vz;
+    v.id[i] = id;
+}
+
+// ------------------------------------------------------------ pass 1
+__global__ void __launch_bounds__(256)
+    k_push_count(SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E,
+                 PushConst pc, uint32_t *__restrict__ binCount, uint32_t *__restrict__ lost4,
+                 int do_push) {
+    __shared__ uint32_t sl[4];
+    if (threadIdx.x < 4) sl[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t n = *nptr;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        PState s = load_state(sp, i, pc.bfield != 0);
+        if (do_push) push_one(s, E, pc);
+        int st = classify(s, pc.N, pc.mode);
+        if (st == ST_KEPT)
+            atomicAdd(&binCount[bin_key(s.jx, s.jy, pc.tilesX)], 1u);
+        else
+            atomicAdd(&sl[st - 1], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && sl[threadIdx.x]) atomicAdd(&lost4[threadIdx.x], sl[threadIdx.x]);
+}
+
+// ------------------------------------------------------------ scan
+// exclusive scan of SCAN_TILE-element tiles: partial sums, top-level scan in
+// one block, then per-tile scan plus offset.
+#define SCAN_THREADS 256
+#define SCAN_PER_THREAD 16
+#define SCAN_TILE (SCAN_THREADS * SCAN_PER_THREAD)
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
+    __shared__ uint32_t wsum[SCAN_THREADS / 64];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int k = 0; k < SCAN_THREADS / 64; k++) {
+        if (k < w) pre += wsum[k];
+        tot += wsum[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS)
+    k_scan_partial(const uint32_t *__restrict__ in, uint32_t *__restrict__ partial) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(in + (size_t)blockIdx.x * SCAN_TILE) +
+                     threadIdx.x * (SCAN_PER_THREAD / 4);
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD / 4; k++) {
+        uint4 q = p[k];
+        s += q.x + q.y + q.z + q.w;
+    }
+    uint32_t tot;
+    block_excl_scan(s, &tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS)
+    k_scan_top(uint32_t *__restrict__ partial, int nparts, uint32_t *__restrict__ start_total,
+               uint32_t *__restrict__ n_out) {
+    uint32_t carry = 0;
+    for (int base = 0; base < nparts; base += SCAN_THREADS) {
+        int k = base + threadIdx.x;
+        uint32_t v = (k < nparts) ? partial[k] : 0u;
+        uint32_t tot;
+        uint32_t ex = block_excl_scan(v, &tot);
+        if (k < nparts) partial[k] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        *start_total = carry;
+        *n_out = carry;
+    }
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS)
+    k_scan_final(const uint32_t *__restrict__ in, const uint32_t *__restrict__ partial,
+                 uint32_t *__restrict__ out) {
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
+    uint32_t v[SCAN_PER_THREAD];
+    const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD / 4; k++) {
+        uint4 q = p[k];
+        v[4 * k] = q.x;
+        v[4 * k + 1] = q.y;
+        v[4 * k + 2] = q.z;
+        v[4 * k + 3] = q.w;
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD; k++) s += v[k];
+    uint32_t tot;
+    uint32_t run = partial[blockIdx.x] + block_excl_scan(s, &tot);
+    uint4 *o = reinterpret_cast<uint4 *>(out + base);
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD / 4; k++) {
+        uint4 q;
+        q.x = run;
+        run += v[4 * k];
+        q.y = run;
+        run += v[4 * k + 1];
+        q.z = run;
+        run += v[4 * k + 2];
+        q.w = run;
+        run += v[4 * k + 3];
+        o[k] = q;
+    }
+}
+
+void launch_scan(hipStream_t st, const uint32_t *count, uint32_t *start, uint32_t nbins_pad,
+                 uint32_t *partial, uint32_t *n_out) {
+    int nparts = (int)(nbins_pad / SCAN_TILE);
+    hipLaunchKernelGGL(k_scan_partial, dim3(nparts), dim3(SCAN_THREADS), 0, st, count, partial);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, st, partial, nparts,
+                       start + nbins_pad, n_out);
+    hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(SCAN_THREADS), 0, st, count, partial, start);
+}
+
+uint32_t scan_pad(uint32_t nbins) { return (nbins + SCAN_TILE - 1) / SCAN_TILE * SCAN_TILE; }
+
+// ------------------------------------------------------------ pass 2
+#define DEP_WCAP 4096  // LDS window capacity (nodes), 32 KB of uint64
+#define DEP_MARGIN 3   // cells of motion allowed around the block's old cells
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ void dep_global(unsigned long long *q, int N, int jx, int jy,
+                                           const FxW &w) {
+    long b = (long)jy * N + jx;
+    atomicAdd(&q[b], w.a4);
+    atomicAdd(&q[b + 1], w.a3);
+    atomicAdd(&q[b + N], w.a2);
+    atomicAdd(&q[b + N + 1], w.a1);
+}
+
+// Deposit the block's particles (dep[] true) through an LDS window spanning
+// the block's old cells +- DEP_MARGIN; particles landing outside the window,
+// or blocks whose window is too large, use global atomics directly.
+__device__ __forceinline__ void block_deposit(bool dep, const PState &s, bool valid, int ojx,
+                                              int ojy, unsigned long long *__restrict__ q, int N) {
+    __shared__ unsigned long long sq[DEP_WCAP];
+    __shared__ int red[4][4];
+    int big = 1 << 30;
+    int mnx = valid ? max(0, min(ojx, N - 2)) : big, mny = valid ? max(0, min(ojy, N - 2)) : big;
+    int mxx = valid ? max(0, min(ojx, N - 2)) : -big, mxy = valid ? max(0, min(ojy, N - 2)) : -big;
+    mnx = wave_min(mnx);
+    mny = wave_min(mny);
+    mxx = wave_max(mxx);
+    mxy = wave_max(mxy);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[w][0] = mnx;
+        red[w][1] = mny;
+        red[w][2] = mxx;
+        red[w][3] = mxy;
+    }
+    __syncthreads();
+    int nw = blockDim.x >> 6;
+    mnx = red[0][0];
+    mny = red[0][1];
+    mxx = red[0][2];
+    mxy = red[0][3];
+    for (int k = 1; k < nw; k++) {
+        mnx = min(mnx, red[k][0]);
+        mny = min(mny, red[k][1]);
+        mxx = max(mxx, red[k][2]);
+        mxy = max(mxy, red[k][3]);
+    }
+    if (mxx < mnx) return;  // block without particles (uniform: all threads)
+    int x0 = max(0, mnx - DEP_MARGIN), y0 = max(0, mny - DEP_MARGIN);
+    int x1 = min(N - 1, mxx + 1 + DEP_MARGIN), y1 = min(N - 1, mxy + 1 + DEP_MARGIN);
+    int ww = x1 - x0 + 1, wh = y1 - y0 + 1;
+    if (ww * wh > DEP_WCAP) {
+        if (dep) dep_global(q, N, s.jx, s.jy, fx_weights(s.ox, s.oy));
+        return;
+    }
+    for (int k = threadIdx.x; k < ww * wh; k += blockDim.x) sq[k] = 0ull;
+    __syncthreads();
+    if (dep) {
+        FxW fw = fx_weights(s.ox, s.oy);
+        int lx = s.jx - x0, ly = s.jy - y0;
+        if (lx >= 0 && ly >= 0 && lx + 1 < ww && ly + 1 < wh) {
+            int b = ly * ww + lx;
+            atomicAdd(&sq[b], fw.a4);
+            atomicAdd(&sq[b + 1], fw.a3);
+            atomicAdd(&sq[b + ww], fw.a2);
+            atomicAdd(&sq[b + ww + 1], fw.a1);
+        } else {
+            dep_global(q, N, s.jx, s.jy, fw);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < ww * wh; k += blockDim.x) {
+        unsigned long long v = sq[k];
+        if (v) atomicAdd(&q[(long)(y0 + k / ww) * N + x0 + k % ww], v);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    k_push_scatter(SpView src, SpView dst, const uint32_t *__restrict__ nptr, uint32_t cap,
+                   const float2 *__restrict__ E, PushConst pc, uint32_t *__restrict__ binCount,
+                   const uint32_t *__restrict__ binStart, unsigned long long *__restrict__ q,
+                   int do_push, int deposit) {
+    uint32_t n = *nptr;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = i < n;
+    PState s{};
+    int ojx = 0, ojy = 0;
+    int st = -1;
+    if (valid) {
+        s = load_state(src, i, true);
+        ojx = s.jx;
+        ojy = s.jy;
+        if (do_push) push_one(s, E, pc);
+        st = classify(s, pc.N, pc.mode);
+        if (st == ST_KEPT) {
+            uint32_t key = bin_key(s.jx, s.jy, pc.tilesX);
+            uint32_t c = atomicSub(&binCount[key], 1u);
+            // pass 1 counted this particle into the same bin, so c >= 1; the
+            // test only keeps a corrupted count from writing out of bounds
+            uint32_t slot = binStart[key] + c - 1u;
+            if (c != 0u && slot < cap) store_state(dst, slot, s, src.id[i]);
+        }
+    }
+    if (!deposit) return;
+    bool dep = (st == ST_KEPT) && deposit_ok(s, pc.N, pc.mode);
+    block_deposit(dep, s, valid, ojx, ojy, q, pc.N);
+}
+
+// ------------------------------------------------------- per-phase path
+__global__ void __launch_bounds__(256)
+    k_gather(SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E,
+             PushConst pc, float *__restrict__ exp_, float *__restrict__ eyp) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *nptr) return;
+    PState s = load_state(sp, i, false);
+    float ex = 0.0f, ey = 0.0f;  // pic.cl:70-71
+    if (can_move(s, pc)) gather_one(s, E, pc, ex, ey);
+    exp_[i] = ex;
+    eyp[i] = ey;
+}
+
+__global__ void __launch_bounds__(256)
+    k_move(SpView sp, const uint32_t *__restrict__ nptr, const float *__restrict__ exp_,
+           const float *__restrict__ eyp, PushConst pc) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *nptr) return;
+    PState s = load_state(sp, i, true);
+    if (!can_move(s, pc)) return;
+    move_one(s, exp_[i], eyp[i], pc);
+    sp.cell[i] = cell_pack(s.jx, s.jy);
+    sp.ox[i] = s.ox;
+    sp.oy[i] = s.oy;
+    sp.vx[i] = s.vx;
+    sp.vy[i] = s.vy;
+    sp.vz[i] = s.vz;
+}
+
+__global__ void __launch_bounds__(256)
+    k_deposit(SpView sp, const uint32_t *__restrict__ nptr, int N, int mode,
+              unsigned long long *__restrict__ q) {
+    uint32_t n = *nptr;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = i < n;
+    PState s{};
+    if (valid) s = load_state(sp, i, false);
+    bool dep = valid && deposit_ok(s, N, mode);
+    block_deposit(dep, s, valid, s.jx, s.jy, q, N);
+}
+
+// wall pass with stream compaction into the other buffer (boundary.cpp:581-677,
+// 765-824 with zero yields).  Survivor order is not preserved.
+__global__ void __launch_bounds__(256)
+    k_absorb(SpView src, SpView dst, const uint32_t *__restrict__ nptr, uint32_t *__restrict__ nout,
+             int N, int mode, uint32_t *__restrict__ lost4) {
+    __shared__ uint32_t sl[4];
+    __shared__ uint32_t wbase[4], wcnt[4], bbase;
+    if (threadIdx.x < 4) sl[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t n = *nptr;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool keep = false;
+    PState s{};
+    if (i < n) {
+        s = load_state(src, i, true);
+        int st = classify(s, N, mode);
+        keep = (st == ST_KEPT);
+        if (!keep) atomicAdd(&sl[st - 1], 1u);
+    }
+    // block-level compaction: one global atomic per block
+    uint64_t bal = __ballot(keep);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t rank = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wcnt[w] = __popcll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+            wbase[k] = t;
+            t += wcnt[k];
+        }
+        bbase = t ? atomicAdd(nout, t) : 0u;
+    }
+    __syncthreads();
+    if (keep) store_state(dst, bbase + wbase[w] + rank, s, src.id[i]);
+    if (threadIdx.x < 4 && sl[threadIdx.x]) atomicAdd(&lost4[threadIdx.x], sl[threadIdx.x]);
+}
+
+// ------------------------------------------------------------ loader
+// Uniform positions in (0,L)^2 and a 3-V Maxwellian, from Philox4x32-10 keyed
+// by the seed with counter (id, species, call, 0).
+__global__ void __launch_bounds__(256)
+    k_load_random(SpView sp, uint32_t n, uint32_t key0, uint32_t key1, uint32_t species, int N,
+                  double sigma) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t r[4], g[4], z[4];
+    philox4x32_10(i, species, 0u, 0u, key0, key1, r);
+    philox4x32_10(i, species, 1u, 0u, key0, key1, g);
+    philox4x32_10(i, species, 2u, 0u, key0, key1, z);
+    double cells = (double)(N - 1);
+    double tx = u01d(r[0], r[1]) * cells, ty = u01d(r[2], r[3]) * cells;
+    int jx = (int)floor(tx), jy = (int)floor(ty);
+    float ox = (float)(tx - (double)jx), oy = (float)(ty - (double)jy);
+    if (ox >= 1.0f) { ox = 0.0f; jx += 1; }
+    if (oy >= 1.0f) { oy = 0.0f; jy += 1; }
+    double u1 = u01d(g[0], g[1]), u2 = u01d(g[2], g[3]);
+    double rad = sqrt(-2.0 * log(u1));
+    double u3 = u01d(z[0], z[1]), u4 = u01d(z[2], z[3]);
+    double two_pi = 6.283185307179586;
+    sp.cell[i] = cell_pack(jx, jy);
+    sp.ox[i] = ox;
+    sp.oy[i] = oy;
+    sp.vx[i] = (float)(sigma * rad * cos(two_pi * u2));
+    sp.vy[i] = (float)(sigma * rad * sin(two_pi * u2));
+    sp.vz[i] = (float)(sigma * sqrt(-2.0 * log(u3)) * cos(two_pi * u4));
+    sp.id[i] = i;
+}
+
+// ------------------------------------------------------------ launchers
+static inline unsigned blocks_for(int64_t n, int t) {
+    int64_t b = (n + t - 1) / t;
+    return (unsigned)(b > 0 ? b : 1);
+}
+
+void launch_push_count(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                       const float2 *E, const PushConst &pc, uint32_t *binCount, uint32_t *lost4,
+                       int do_push) {
+    hipLaunchKernelGGL(k_push_count, dim3(blocks_for(cap, 256)), dim3(256), 0, st, sp, n, E, pc,
+                       binCount, lost4, do_push);
+}
+
+void launch_push_scatter(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
+                         int64_t cap, const float2 *E, const PushConst &pc, uint32_t *binCount,
+                         const uint32_t *binStart, unsigned long long *q, int do_push,
+                         int deposit) {
+    hipLaunchKernelGGL(k_push_scatter, dim3(blocks_for(cap, 256)), dim3(256), 0, st, src, dst, n,
+                       (uint32_t)cap, E, pc, binCount, binStart, q, do_push, deposit);
+}
+
+void launch_gather(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                   const float2 *E, const PushConst &pc, float *exp_, float *eyp) {
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(cap, 256)), dim3(256), 0, st, sp, n, E, pc, exp_,
+                       eyp);
+}
+
+void launch_move(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                 const float *exp_, const float *eyp, const PushConst &pc) {
+    hipLaunchKernelGGL(k_move, dim3(blocks_for(cap, 256)), dim3(256), 0, st, sp, n, exp_, eyp, pc);
+}
+
+void launch_deposit(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap, int N,
+                    int mode, unsigned long long *q) {
+    hipLaunchKernelGGL(k_deposit, dim3(blocks_for(cap, 256)), dim3(256), 0, st, sp, n, N, mode, q);
+}
+
+void launch_absorb(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
+                   uint32_t *nout, int64_t cap, int N, int mode, uint32_t *lost4) {
+    hipLaunchKernelGGL(k_absorb, dim3(blocks_for(cap, 256)), dim3(256), 0, st, src, dst, n, nout,
+                       N, mode, lost4);
+}
+
+void launch_load_random(hipStream_t st, const SpView &sp, uint32_t n, uint64_t seed,
+                        uint32_t species, int N, double sigma) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_load_random, dim3(blocks_for(n, 256)), dim3(256), 0, st, sp, n,
+                       (uint32_t)seed, (uint32_t)(seed >> 32), species, N, sigma);
+}
+
+}  // namespace pic

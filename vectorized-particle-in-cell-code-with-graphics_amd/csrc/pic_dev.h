@@ -1,0 +1,217 @@
+// pic_dev.h -- device helpers shared by the particle kernels.
+//
+// Particle position = packed int16 cell (jx | jy << 16) + fp32 offset in
+// [0,1) per axis, i.e. x = (jx + ox) * dx.  The reference keeps absolute FP64
+// positions and recomputes jp = (int)(x/dx) in every kernel (pic.cl:31-37,
+// 66-76, 108-112); storing the cell makes the per-cell counting sort free of
+// divisions and keeps the offsets at full fp32 resolution.
+//
+// All arithmetic below is written out operation by operation; the library is
+// compiled with -ffp-contract=off so every product and sum rounds once, as
+// IEEE single/double precision.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PIC_DEV_MODE_REF 0
+#define PIC_DEV_MODE_PHYS 1
+
+#define ST_KEPT 0
+#define ST_LEFT 1
+#define ST_RIGHT 2
+#define ST_BOTTOM 3
+#define ST_TOP 4
+
+#define POS_GUARD 8192.0f
+#define CELL_FAR 16000
+
+// fixed-point deposit: weights quantised to 2^-20 of a cell, 2^40 per particle
+#define FX_BITS 20
+#define FX_ONE (1ll << FX_BITS)
+
+// per-species constants of one push (particle.cpp:1103-1112 in fp32)
+struct PushConst {
+    int N;      // nodes per side
+    int mode;   // PIC_DEV_MODE_*
+    int bfield; // magnetic rotation on/off (pic.cl:121)
+    int tilesX; // sort-key tiles per side
+    float cE;   // 0.5 * dt * q/m
+    float dtdx, dtdy;
+    float cos_a, sin_a, cos_w, sin_w;
+};
+
+__host__ __device__ inline uint32_t cell_pack(int jx, int jy) {
+    return (uint32_t)(uint16_t)(int16_t)jx | ((uint32_t)(uint16_t)(int16_t)jy << 16);
+}
+__host__ __device__ inline int cell_jx(uint32_t c) { return (int)(int16_t)(c & 0xFFFFu); }
+__host__ __device__ inline int cell_jy(uint32_t c) { return (int)(int16_t)(c >> 16); }
+
+// new cell and offset from cell j and unnormalised offset t (cells)
+__device__ __forceinline__ void reposition(int j, float t, int &jn, float &on) {
+    if (!(t > -POS_GUARD && t < POS_GUARD)) {  // NaN or a jump far off the grid
+        jn = (t > 0.0f) ? CELL_FAR : -CELL_FAR;
+        on = 0.0f;
+        return;
+    }
+    float k = floorf(t);
+    float f = t - k;
+    if (f >= 1.0f) {  // t just below an integer: t - k rounded up to 1
+        f = 0.0f;
+        k = k + 1.0f;
+    }
+    int jj = j + (int)k;
+    jj = jj > CELL_FAR ? CELL_FAR : jj;
+    jj = jj < -CELL_FAR ? -CELL_FAR : jj;
+    jn = jj;
+    on = f;
+}
+
+// pic.cl:74 / 112: jp < N && x > 0  (truncation of x/dx, so x in (0, N*dx))
+__device__ __forceinline__ bool ref_inside_push(int j, float o, int N) {
+    return j >= 0 && j <= N - 1 && !(j == 0 && o == 0.0f);
+}
+
+struct PState {
+    int jx, jy;
+    float ox, oy, vx, vy, vz;
+};
+
+// pic.cl:74 / pic.cl:112: only particles strictly inside (0, N*dx) move (REF);
+// in PHYS mode every particle inside [0, L) moves (others leave at the next
+// wall pass and are never gathered, so no read leaves the field array).
+__device__ __forceinline__ bool can_move(const PState &s, const PushConst &pc) {
+    if (pc.mode == PIC_DEV_MODE_PHYS)
+        return s.jx >= 0 && s.jx <= pc.N - 2 && s.jy >= 0 && s.jy <= pc.N - 2;
+    return ref_inside_push(s.jx, s.ox, pc.N) && ref_inside_push(s.jy, s.oy, pc.N);
+}
+
+// CIC gather of (Ex, Ey) at the particle (pic.cl:75-88).  E is the reference's
+// padded stride-N node layout, so a particle in the last column reads the
+// next row's first entry exactly as the reference does.
+__device__ __forceinline__ void gather_one(const PState &s, const float2 *__restrict__ E,
+                                           const PushConst &pc, float &epx, float &epy) {
+    const int N = pc.N;
+    float wj = s.ox, wk = s.oy;
+    float a4 = (1.0f - wj) * (1.0f - wk);
+    float a3 = wj * (1.0f - wk);
+    float a2 = wk * (1.0f - wj);
+    float a1 = wj * wk;
+    long b = (long)s.jy * N + s.jx;
+    float2 e00 = E[b], e01 = E[b + 1], e10 = E[b + N], e11 = E[b + N + 1];
+    epx = a4 * e00.x + a3 * e01.x + a2 * e10.x + a1 * e11.x;
+    epy = a4 * e00.y + a3 * e01.y + a2 * e10.y + a1 * e11.y;
+}
+
+// half kick, optional rotation about B, half kick, drift (pic.cl:114-146)
+__device__ __forceinline__ void move_one(PState &s, float epx, float epy, const PushConst &pc) {
+    float ux = s.vx + pc.cE * epx;
+    float uy = s.vy + pc.cE * epy;
+    float uz = s.vz;
+    if (pc.bfield) {
+        float vpar = pc.cos_a * ux + pc.sin_a * uy;
+        float vperp = -pc.sin_a * ux + pc.cos_a * uy;
+        float vperp_after = vperp * pc.cos_w + pc.sin_w * uz;
+        uz = -vperp * pc.sin_w + pc.cos_w * uz;
+        ux = pc.cos_a * vpar - pc.sin_a * vperp_after;
+        uy = pc.sin_a * vpar + pc.cos_a * vperp_after;
+    }
+    ux = ux + pc.cE * epx;
+    uy = uy + pc.cE * epy;
+    float tx = s.ox + ux * pc.dtdx;
+    float ty = s.oy + uy * pc.dtdy;
+    reposition(s.jx, tx, s.jx, s.ox);
+    reposition(s.jy, ty, s.jy, s.oy);
+    s.vx = ux;
+    s.vy = uy;
+    s.vz = uz;
+}
+
+__device__ __forceinline__ bool push_one(PState &s, const float2 *__restrict__ E,
+                                         const PushConst &pc) {
+    if (!can_move(s, pc)) return false;
+    float epx, epy;
+    gather_one(s, E, pc, epx, epy);
+    move_one(s, epx, epy, pc);
+    return true;
+}
+
+// Wall test after the push.  REF: boundary.cpp:595-599 / 774-778, lost iff
+// (int)(x/dx) < 0 or >= N, i.e. x <= -dx or x >= N*dx.  PHYS: x < 0 or x >= L.
+__device__ __forceinline__ int classify(const PState &s, int N, int mode) {
+    bool lx, ly;
+    if (mode == PIC_DEV_MODE_PHYS) {
+        lx = (s.jx < 0) || (s.jx > N - 2);
+        ly = (s.jy < 0) || (s.jy > N - 2);
+    } else {
+        lx = (s.jx < -1) || (s.jx == -1 && s.ox == 0.0f) || (s.jx >= N);
+        ly = (s.jy < -1) || (s.jy == -1 && s.oy == 0.0f) || (s.jy >= N);
+    }
+    if (lx) return s.jx < 0 ? ST_LEFT : ST_RIGHT;
+    if (ly) return s.jy < 0 ? ST_BOTTOM : ST_TOP;
+    return ST_KEPT;
+}
+
+// pic.cl:35: deposit only for jp < N-1 && x > 0 (REF); every particle inside
+// [0, L) (PHYS)
+__device__ __forceinline__ bool deposit_ok(const PState &s, int N, int mode) {
+    if (mode == PIC_DEV_MODE_PHYS) return s.jx >= 0 && s.jx <= N - 2 && s.jy >= 0 && s.jy <= N - 2;
+    return s.jx >= 0 && s.jx <= N - 2 && !(s.jx == 0 && s.ox == 0.0f) && s.jy >= 0 &&
+           s.jy <= N - 2 && !(s.jy == 0 && s.oy == 0.0f);
+}
+
+// fixed-point CIC corner weights (Birdsall & Langdon bilinear weighting, as in
+// pic.cl:40-43), summing to exactly 2^40
+struct FxW {
+    unsigned long long a4, a3, a2, a1;
+};
+__device__ __forceinline__ FxW fx_weights(float ox, float oy) {
+    long long wx = (long long)rintf(ox * 1048576.0f);
+    long long wy = (long long)rintf(oy * 1048576.0f);
+    FxW w;
+    w.a4 = (unsigned long long)((FX_ONE - wx) * (FX_ONE - wy));
+    w.a3 = (unsigned long long)(wx * (FX_ONE - wy));
+    w.a2 = (unsigned long long)((FX_ONE - wx) * wy);
+    w.a1 = (unsigned long long)(wx * wy);
+    return w;
+}
+
+// counting-sort key on the (N+1)^2 halo grid (jx, jy in [-1, N-1]), ordered
+// tile-major with 8x8 tiles so a contiguous key range is a compact patch
+__device__ __forceinline__ uint32_t bin_key(int jx, int jy, int tilesX) {
+    uint32_t hx = (uint32_t)(jx + 1), hy = (uint32_t)(jy + 1);
+    return ((((hy >> 3) * (uint32_t)tilesX + (hx >> 3)) << 6) | ((hy & 7u) << 3) | (hx & 7u));
+}
+
+// ----------------------------------------------------------- Philox4x32-10
+__host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+// uniform in (0,1), exact in fp32
+__host__ __device__ inline float u01(uint32_t x) {
+    return ((float)(x >> 9) + 0.5f) * (1.0f / 8388608.0f);
+}
+__host__ __device__ inline double u01d(uint32_t hi, uint32_t lo) {
+    // 53-bit uniform in (0,1): 27 bits of hi above 26 bits of lo
+    uint64_t m = ((uint64_t)(hi >> 5) << 26) | (uint64_t)(lo >> 6);
+    return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+}

@@ -1,0 +1,73 @@
+// pic_internal.h -- host/device structures shared by the library's sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pic_dev.h"
+
+namespace pic {
+
+// SoA view of one particle buffer (kernel argument, by value)
+struct SpView {
+    uint32_t *cell;
+    float *ox, *oy, *vx, *vy, *vz;
+    uint32_t *id;
+};
+
+// Device counters of one species.  n[b] is the particle count of buffer b.
+struct SpCounters {
+    uint32_t n[2];
+    uint32_t lost[4];  // left, right, bottom, top (accumulated)
+    uint32_t next_id;
+    uint32_t overflow;
+    uint32_t pad[8];
+};
+
+// Multigrid level storage on the device.  Level l has (2^(l+1)+1)^2 nodes;
+// level ng-1 is the finest.  Levels <= lc are solved inside one workgroup.
+struct MgLevels {
+    int ng = 0, lc = 0;
+    double *rhsF[16] = {};  // full-multigrid right-hand side per level
+    double *u[16] = {};     // full-multigrid solution per level (u[ng-1]: result)
+    double *cres[16] = {};  // V-cycle coarse right-hand side
+    double *cv[16] = {};    // V-cycle coarse correction
+};
+
+// ---- k_particles.hip
+void launch_push_count(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                       const float2 *E, const PushConst &pc, uint32_t *binCount, uint32_t *lost4,
+                       int do_push);
+void launch_push_scatter(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
+                         int64_t cap, const float2 *E, const PushConst &pc, uint32_t *binCount,
+                         const uint32_t *binStart, unsigned long long *q, int do_push, int deposit);
+void launch_scan(hipStream_t st, const uint32_t *count, uint32_t *start, uint32_t nbins_pad,
+                 uint32_t *partial, uint32_t *n_out);
+uint32_t scan_pad(uint32_t nbins);
+void launch_gather(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                   const float2 *E, const PushConst &pc, float *exp_, float *eyp);
+void launch_move(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                 const float *exp_, const float *eyp, const PushConst &pc);
+void launch_deposit(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap, int N,
+                    int mode, unsigned long long *q);
+void launch_absorb(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
+                   uint32_t *nout, int64_t cap, int N, int mode, uint32_t *lost4);
+void launch_load_random(hipStream_t st, const SpView &sp, uint32_t n, uint64_t seed,
+                        uint32_t species, int N, double sigma);
+
+// ---- k_fields.hip
+void launch_rho(hipStream_t st, const unsigned long long *qi, const unsigned long long *qe, int N,
+                double scale_i, double scale_e, double corner, double *rho);
+void launch_fold(hipStream_t st, const double *rho, int N, double coef, double phi0X, double phiR,
+                 double phi0Y, double phiLY, double dxdy_inv, double *rhs);
+void launch_phi_bc(hipStream_t st, const double *u, int N, double phi0X, double phiR,
+                   double phi0Y, double phiLY, double *phi);
+void launch_efield(hipStream_t st, const double *phi, int N, double dx, double *ex64, double *ey64,
+                   float2 *E32);
+template <typename T>
+void launch_fill(hipStream_t st, T *p, long n, T v);
+int mg_coarse_level_for(long lds_bytes, int ng);
+void mg_set_coarse_attr(long lds_bytes);
+void mg_fixed_solve(hipStream_t st, MgLevels &M, int ncycle, double length);
+
+}  // namespace pic

@@ -1,0 +1,388 @@
+"""pic2d -- Python binding of the MI355X-native 2-D electrostatic PIC timestep.
+
+A thin ctypes layer over ``libpic2d.so`` (the C ABI declared in
+``include/pic2d.h``).  The classes mirror the reference's C++ objects so that
+driving code reads like the reference's ``main.cpp`` / ``pic.cpp``:
+
+* :class:`Params`  -- ``parameters.h`` (parameters.h:41-366) as a runtime struct;
+* :class:`Pic`     -- class ``pic`` (pic.h:89-214): ``pic_run``, ``dump_data``,
+  ``setRuntime`` and the per-phase methods of ``particle``/``fields``;
+* :func:`mglin`    -- ``Mglin(u, ncycle, length)`` (mglin.cpp:21-55).
+
+Every call raises :class:`PicError` (carrying the library's status code and
+message) on failure; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpic2d.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "pic2d.h")
+
+PIC_OK = 0
+ERRORS = {
+    -1: "PIC_ERR_ARG",
+    -2: "PIC_ERR_GRID",
+    -3: "PIC_ERR_FILE",
+    -4: "PIC_ERR_FORMAT",
+    -5: "PIC_ERR_DEVICE",
+    -6: "PIC_ERR_CAPACITY",
+    -7: "PIC_ERR_UNSUPPORTED",
+    -8: "PIC_ERR_ALLOC",
+}
+MODE_REF, MODE_PHYSICAL = 0, 1
+MG_FIXED, MG_TOL = 0, 1
+LOAD_RANDOM, LOAD_LATTICE = 0, 1
+WALL_ABSORB, WALL_STEADY = 0, 1
+GAS_ARGON, GAS_HYDROGEN = 0, 1
+SIGMA_CONSTANT, SIGMA_ENERGY = 0, 1
+ELECTRONS, IONS = 0, 1
+
+
+class PicError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    """Mirror of ``pic_params`` (include/pic2d.h)."""
+
+    _fields_ = [
+        ("nodes", C.c_int32),
+        ("init_particle_no", C.c_int32),
+        ("mode", C.c_int32),
+        ("loader", C.c_int32),
+        ("wall", C.c_int32),
+        ("have_collisions", C.c_int32),
+        ("gas", C.c_int32),
+        ("sigma_cx", C.c_int32),
+        ("sigma_elastic", C.c_int32),
+        ("sigma_ionization", C.c_int32),
+        ("mg_mode", C.c_int32),
+        ("mg_ncycle", C.c_int32),
+        ("mg_max_iter", C.c_int32),
+        ("dt_dti", C.c_int32),
+        ("run_time", C.c_int32),
+        ("number_to_inject", C.c_int32),
+        ("no_ion_species", C.c_int32),
+        ("corner8", C.c_int32),
+        ("seed", C.c_uint64),
+        ("length", C.c_double),
+        ("dt", C.c_double),
+        ("dti", C.c_double),
+        ("electron_mass", C.c_double),
+        ("ion_mass", C.c_double),
+        ("electron_charge", C.c_double),
+        ("ion_charge", C.c_double),
+        ("qce", C.c_double),
+        ("qci", C.c_double),
+        ("vte", C.c_double),
+        ("vti", C.c_double),
+        ("vtn", C.c_double),
+        ("sigma_load_e", C.c_double),
+        ("sigma_load_i", C.c_double),
+        ("epsilon", C.c_double),
+        ("neutral_density", C.c_double),
+        ("threshold", C.c_double),
+        ("sigma_value", C.c_double),
+        ("elastic_sigma_value", C.c_double),
+        ("ionization_sigma_value", C.c_double),
+        ("magnetic_field", C.c_double),
+        ("angle", C.c_double),
+        ("phi_left", C.c_double),
+        ("phi_right", C.c_double),
+        ("phi_bottom", C.c_double),
+        ("phi_top", C.c_double),
+        ("phi_period", C.c_double),
+        ("mg_tol", C.c_double),
+    ]
+
+    @property
+    def dx(self):
+        return self.length / (self.nodes - 1)
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+PARTICLE_DTYPE = np.dtype(
+    [("x", "f8"), ("y", "f8"), ("vx", "f8"), ("vy", "f8"), ("vz", "f8"), ("weight", "f8")]
+)
+
+_lib = None
+
+
+def declared_functions(header=HEADER_PATH):
+    """Names of every function declared in include/pic2d.h."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(pic_\w+)\s*\(", text, flags=re.M)))
+
+
+def load_library(path=LIB_PATH):
+    """Load libpic2d.so; raises if it is missing (no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PicError(-5, f"{path} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    vp = C.c_void_p
+    i32, i64, dbl = C.c_int32, C.c_int64, C.c_double
+    sig = {
+        "pic_params_default": [P(Params)],
+        "pic_params_load": [C.c_char_p, P(Params)],
+        "pic_create": [P(Params), C.c_int, C.c_char_p, P(vp)],
+        "pic_destroy": [vp],
+        "pic_set_runtime": [vp, i32],
+        "pic_run": [vp],
+        "pic_dump_data": [vp, C.c_char_p, i32],
+        "pic_species_gather": [vp, i32],
+        "pic_species_move": [vp, i32],
+        "pic_species_collect_charge": [vp, i32],
+        "pic_fields_grid_rho": [vp],
+        "pic_fields_grid_potential": [vp, dbl, dbl, dbl, dbl, dbl, dbl],
+        "pic_fields_grid_electric_field": [vp],
+        "pic_e_collisions": [vp],
+        "pic_ion_collisions": [vp],
+        "pic_boundary": [vp, i32],
+        "pic_mglin": [vp, i32, i32, dbl],
+        "pic_particle_size": [vp, i32, P(i64)],
+        "pic_get_particles": [vp, i32, vp, i64],
+        "pic_get_phi": [vp, vp],
+        "pic_get_rho": [vp, vp],
+        "pic_get_E": [vp, vp, vp],
+        "pic_get_q": [vp, i32, vp],
+        "pic_get_step": [vp, P(i64)],
+        "pic_get_loss_counts": [vp, i32, vp],
+        "pic_get_nodes": [vp, P(i32)],
+        "pic_get_state": [vp, i32, vp, vp, vp, vp, vp, vp, vp, i64],
+        "pic_set_state": [vp, i32, vp, vp, vp, vp, vp, vp, vp, i64],
+        "pic_get_qfixed": [vp, i32, vp],
+        "pic_get_E32": [vp, vp],
+        "pic_set_E32": [vp, vp],
+        "pic_profile_enable": [vp, i32],
+        "pic_profile_read": [vp, vp],
+        "pic_sync": [vp],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    lib.pic_last_error.argtypes = [vp]
+    lib.pic_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _check(rc, h=None):
+    if rc != PIC_OK:
+        msg = load_library().pic_last_error(h)
+        raise PicError(rc, msg.decode() if msg else "")
+
+
+def default_params() -> Params:
+    p = Params()
+    _check(load_library().pic_params_default(C.byref(p)))
+    return p
+
+
+def load_params(path) -> Params:
+    """Evaluate a parameters.h-style file (parameters.h:41-366)."""
+    p = Params()
+    _check(load_library().pic_params_load(str(path).encode(), C.byref(p)))
+    return p
+
+
+def mglin(u, ncycle=2, length=1.0):
+    """GPU FP64 full multigrid (mglin.cpp:21-55); returns the solution."""
+    a = np.ascontiguousarray(u, dtype=np.float64).copy()
+    n = a.shape[0]
+    _check(load_library().pic_mglin(_ptr(a), n, ncycle, length))
+    return a
+
+
+class Pic:
+    """One simulation (the reference's ``pic`` object, pic.cpp:55-221)."""
+
+    def __init__(self, params: Params, restart=False, dump_dir="."):
+        self.lib = load_library()
+        self.params = params
+        self.h = C.c_void_p()
+        _check(self.lib.pic_create(C.byref(params), int(restart), str(dump_dir).encode(),
+                                   C.byref(self.h)))
+        n = C.c_int32()
+        _check(self.lib.pic_get_nodes(self.h, C.byref(n)), self.h)
+        self.N = n.value
+
+    # -- lifecycle
+    def close(self):
+        if self.h:
+            self.lib.pic_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        _check(rc, self.h)
+
+    # -- pic.h
+    def setRuntime(self, run_time):
+        self._c(self.lib.pic_set_runtime(self.h, int(run_time)))
+
+    def pic_run(self):
+        self._c(self.lib.pic_run(self.h))
+
+    def run(self, steps):
+        self.setRuntime(steps)
+        self.pic_run()
+
+    def dump_data(self, directory=".", precision=17):
+        self._c(self.lib.pic_dump_data(self.h, str(directory).encode(), int(precision)))
+
+    def sync(self):
+        self._c(self.lib.pic_sync(self.h))
+
+    # -- per-phase methods (particle.h / fields.h / collisions / boundary)
+    def get_particle_electric_field(self, s):
+        self._c(self.lib.pic_species_gather(self.h, s))
+
+    def move_particles(self, s):
+        self._c(self.lib.pic_species_move(self.h, s))
+
+    def collect_charge(self, s):
+        self._c(self.lib.pic_species_collect_charge(self.h, s))
+
+    def grid_rho(self):
+        self._c(self.lib.pic_fields_grid_rho(self.h))
+
+    def grid_potential(self, phi0X, phiLX, phi0Y, phiLY, timestep, phi_period):
+        self._c(self.lib.pic_fields_grid_potential(self.h, phi0X, phiLX, phi0Y, phiLY,
+                                                   float(timestep), phi_period))
+
+    def grid_electric_field(self):
+        self._c(self.lib.pic_fields_grid_electric_field(self.h))
+
+    def e_collisions(self):
+        self._c(self.lib.pic_e_collisions(self.h))
+
+    def ion_collisions(self):
+        self._c(self.lib.pic_ion_collisions(self.h))
+
+    def boundary(self, which):
+        """which = 0: ion wall (implement_i_boundary), 1: electron wall."""
+        self._c(self.lib.pic_boundary(self.h, which))
+
+    # -- getters
+    def particle_size(self, s):
+        n = C.c_int64()
+        self._c(self.lib.pic_particle_size(self.h, s, C.byref(n)))
+        return n.value
+
+    def get_particles(self, s):
+        n = self.particle_size(s)
+        out = np.zeros(n, dtype=PARTICLE_DTYPE)
+        self._c(self.lib.pic_get_particles(self.h, s, _ptr(out), n))
+        return out
+
+    def get_state(self, s):
+        n = self.particle_size(s)
+        d = {k: np.zeros(n, dtype=np.uint32 if k in ("cell", "id") else np.float32)
+             for k in ("cell", "ox", "oy", "vx", "vy", "vz", "id")}
+        self._c(self.lib.pic_get_state(self.h, s, *[_ptr(d[k]) for k in
+                                                    ("cell", "ox", "oy", "vx", "vy", "vz", "id")], n))
+        return d
+
+    def set_state(self, s, d):
+        arrs = [np.ascontiguousarray(d[k], dtype=np.uint32 if k in ("cell", "id") else np.float32)
+                for k in ("cell", "ox", "oy", "vx", "vy", "vz", "id")]
+        self._c(self.lib.pic_set_state(self.h, s, *[_ptr(a) for a in arrs], len(arrs[0])))
+
+    def get_phi(self):
+        a = np.zeros((self.N, self.N))
+        self._c(self.lib.pic_get_phi(self.h, _ptr(a)))
+        return a
+
+    def get_rho(self):
+        a = np.zeros((self.N, self.N))
+        self._c(self.lib.pic_get_rho(self.h, _ptr(a)))
+        return a
+
+    def get_E(self):
+        ex = np.zeros((self.N + 1) * (self.N + 1))
+        ey = np.zeros_like(ex)
+        self._c(self.lib.pic_get_E(self.h, _ptr(ex), _ptr(ey)))
+        return ex, ey
+
+    def get_E32(self):
+        a = np.zeros(2 * ((self.N + 1) * self.N + 1), dtype=np.float32)
+        self._c(self.lib.pic_get_E32(self.h, _ptr(a)))
+        return a
+
+    def set_E32(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        self._c(self.lib.pic_set_E32(self.h, _ptr(a)))
+
+    def get_q(self, s):
+        a = np.zeros((self.N, self.N))
+        self._c(self.lib.pic_get_q(self.h, s, _ptr(a)))
+        return a
+
+    def get_qfixed(self, s):
+        a = np.zeros((self.N, self.N), dtype=np.int64)
+        self._c(self.lib.pic_get_qfixed(self.h, s, _ptr(a)))
+        return a
+
+    def get_step(self):
+        v = C.c_int64()
+        self._c(self.lib.pic_get_step(self.h, C.byref(v)))
+        return v.value
+
+    def loss_counts(self, s):
+        a = np.zeros(4, dtype=np.int64)
+        self._c(self.lib.pic_get_loss_counts(self.h, s, _ptr(a)))
+        return a
+
+    # -- profiling
+    def profile(self, on=True):
+        self._c(self.lib.pic_profile_enable(self.h, int(on)))
+
+    def profile_read(self):
+        a = np.zeros(8)
+        self._c(self.lib.pic_profile_read(self.h, _ptr(a)))
+        return a
+
+
+def cell_unpack(cell):
+    """packed int16 cell -> (jx, jy) int arrays"""
+    c = np.asarray(cell, dtype=np.uint32)
+    jx = (c & 0xFFFF).astype(np.uint16).view(np.int16).astype(np.int32)
+    jy = (c >> 16).astype(np.uint16).view(np.int16).astype(np.int32)
+    return jx, jy
+
+
+def cell_pack(jx, jy):
+    jx = np.asarray(jx).astype(np.int16).view(np.uint16).astype(np.uint32)
+    jy = np.asarray(jy).astype(np.int16).view(np.uint16).astype(np.uint32)
+    return jx | (jy << 16)

@@ -1,0 +1,73 @@
+"""Named workloads: the discharge configurations this library is benchmarked on.
+
+Each function returns a :class:`pic2d.Params`.  Physical constants are the
+reference's (constants.h:15-36); the grid is given in nodes (2^k + 1).
+
+* ``plain``      -- 64x64 cells (65^2 nodes), 1 cm, 100 particles/cell/species,
+                    Te = 2 eV, Ti = 0.026 eV, n = 1e15 m^-3, collisionless,
+                    absorbing walls, w_pe dt = 0.1.
+* ``argon_256``  -- 256x256 cells (257^2 nodes), 200 particles/cell/species
+                    (13.1 M each), the same initial state, with collisions.
+* ``large_1024`` -- 1024x1024 cells (1025^2 nodes), 100 particles/cell/species
+                    (104.9 M each).
+"""
+import math
+
+import pic2d
+
+CHARGE = 1.602177e-19
+ELECTRON_MASS = 9.10939e-31
+AMU = 1.66054e-27
+EPSILON = 8.854e-12
+ELECTRON_CHARGE2 = 2.566971e-38
+AR_MASS = 39.948 * AMU
+H_MASS = 1.00728 * AMU
+SEED = 0x5EED0001
+
+
+def discharge(cells, ppc, length=0.01, te=2.0, ti=0.026, density=1e15, ion_mass=AR_MASS,
+              mode=pic2d.MODE_PHYSICAL, mg_mode=pic2d.MG_TOL, seed=SEED):
+    p = pic2d.default_params()
+    N = cells + 1
+    n = cells * cells * ppc
+    p.nodes = N
+    p.init_particle_no = n
+    p.mode = mode
+    p.loader = pic2d.LOAD_RANDOM
+    p.wall = pic2d.WALL_ABSORB
+    p.have_collisions = 0
+    p.length = length
+    wpe = math.sqrt(density * ELECTRON_CHARGE2 / (EPSILON * ELECTRON_MASS))
+    p.dt = 0.1 / wpe
+    p.dti = p.dt
+    p.dt_dti = 1
+    p.ion_mass = ion_mass
+    w = density * length * length / n          # real particles per macro-particle (1 m depth)
+    p.qce = -CHARGE * w
+    p.qci = CHARGE * w
+    p.vte = math.sqrt(te * CHARGE / ELECTRON_MASS)
+    p.vti = math.sqrt(ti * CHARGE / ion_mass)
+    p.vtn = math.sqrt(0.025 * CHARGE / ion_mass)
+    p.sigma_load_e = p.vte                     # 3-V Maxwellian, sigma = sqrt(kT/m)
+    p.sigma_load_i = p.vti
+    p.mg_mode = mg_mode
+    p.mg_ncycle = 2
+    p.mg_tol = 1e-8
+    p.seed = seed
+    p.run_time = 1000
+    return p
+
+
+def plain(**kw):
+    """64x64 cells, 100 ppc, collisionless (409 600 particles per species)."""
+    return discharge(64, 100, **kw)
+
+
+def argon_256(**kw):
+    """256x256 cells, 200 ppc (13.1 M particles per species)."""
+    return discharge(256, 200, **kw)
+
+
+def large_1024(**kw):
+    """1024x1024 cells, 100 ppc (104.9 M particles per species)."""
+    return discharge(1024, 100, **kw)
