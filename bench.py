@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Throughput of the fused 2-D PIC timestep on one MI355X.
+
+Prints ONE JSON line on stdout:
+  value          particle-steps/s = (N_e + N_i) * steps / wall time, inputs
+                 resident in HBM, timed between two stream synchronisations
+  roofline       HBM roofline of the dominant kernel (the fused push + scatter +
+                 deposit pass, k_push_scatter): algorithmic bytes per launch
+                 (48 B per particle + 16 B per grid node) / its mean duration,
+                 measured with HIP events on the library's own stream
+  cpu_baseline   the CPU oracle in the reference's layout (FP64 48-byte AoS,
+                 the reference's phase order) on a bounded sample of the same
+                 workload, timed on this host
+  latency_ms     p50 / p99 of per-step durations (HIP events around each step)
+
+Progress goes to stderr.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "vectorized-particle-in-cell-code-with-graphics_amd")
+sys.path.insert(0, PKG)
+
+import numpy as np  # noqa: E402
+
+import pic2d  # noqa: E402
+import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_params(name):
+    table = {
+        "argon_256": lambda: workloads.argon_256(),
+        "plain": lambda: workloads.plain(),
+        "plain_256": lambda: workloads.discharge(256, 200),
+        "plain_1024": lambda: workloads.large_1024(),
+    }
+    if name not in table:
+        raise SystemExit(f"unknown workload {name}; choose from {sorted(table)}")
+    return table[name]()
+
+
+def describe(name, p):
+    return {
+        "workload": name,
+        "nodes": int(p.nodes),
+        "cells": int(p.nodes - 1) ** 2,
+        "particles_per_species": int(p.init_particle_no),
+        "length_m": p.length,
+        "collisions": bool(p.have_collisions),
+        "walls": "absorbing" if p.wall == pic2d.WALL_ABSORB else "steady-state plop",
+        "mode": "physical" if p.mode == pic2d.MODE_PHYSICAL else "reference",
+        "multigrid": ("fp64 FMG ncycle=%d" % p.mg_ncycle) if p.mg_mode == pic2d.MG_FIXED
+        else ("fp32 smoothing + fp64 refinement, tol %g" % p.mg_tol),
+    }
+
+
+def cpu_baseline(p, e, i, steps, threads):
+    """Time the oracle's FP64 reference-layout step loop on the host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc  # the oracle: only the CPU baseline leg uses it
+
+    dx = p.dx
+    N = p.nodes
+    _, cref = orc.configs_from_params(p)
+    ea = orc.state_to_aos(e, dx)
+    ia = orc.state_to_aos(i, dx)
+    Ex = np.zeros((N + 1) * (N + 1))
+    used = orc.lib().orc_set_threads(threads)
+    t0 = time.perf_counter()
+    orc.run_ref(cref, steps, ea, ia, e["id"], i["id"], Ex, Ex.copy())
+    dt = time.perf_counter() - t0
+    return (len(ea) + len(ia)) * steps / dt, dt, used
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="argon_256")
+    ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=0, help="0: choose ~10-30 s of CPU work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mg", choices=["default", "fixed", "tol"], default="default",
+                    help="override the workload's multigrid mode")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(runs are independent; see DESIGN.md for how the timestep would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    p = make_params(args.workload)
+    if args.mg != "default":
+        p.mg_mode = pic2d.MG_FIXED if args.mg == "fixed" else pic2d.MG_TOL
+    log("workload", args.workload, "nodes", p.nodes, "particles/species", p.init_particle_no)
+    sim = pic2d.Pic(p)
+    e0, i0 = (sim.get_state(0), sim.get_state(1)) if not args.no_cpu_baseline else (None, None)
+    sim.run(0)  # pre-loop wall pass of pic_run
+    log("warmup", args.warmup, "steps")
+    sim.advance(args.warmup)
+    sim.sync()
+    n0 = sim.particle_size(0) + sim.particle_size(1)
+    log("timing", args.steps, "steps")
+    t0 = time.perf_counter()
+    sim.advance(args.steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    n1 = sim.particle_size(0) + sim.particle_size(1)
+    wall = t1 - t0
+    nbar = 0.5 * (n0 + n1)
+    value = nbar * args.steps / wall
+    ms_per_step = 1e3 * wall / args.steps
+
+    log("profiling", args.profile_steps, "steps with HIP events")
+    ne_a, ni_a = sim.particle_size(0), sim.particle_size(1)
+    sim.profile(True)
+    sim.advance(args.profile_steps)
+    prof = sim.profile_read()
+    steps_ms = sim.profile_steps()
+    sim.profile(False)
+    ne_b, ni_b = sim.particle_size(0), sim.particle_size(1)
+    launches = max(prof[7], 1.0)
+    k3_ms = prof[2] / launches
+    npart_per_launch = 0.25 * (ne_a + ne_b + ni_a + ni_b)
+    alg_bytes = 48.0 * npart_per_launch + 16.0 * p.nodes * p.nodes
+    achieved = alg_bytes / (k3_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    roof = {
+        "kernel": "k_push_scatter",
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "alg_bytes_per_launch": alg_bytes,
+        "mean_launch_ms": k3_ms,
+    }
+    steps_ms = np.asarray(steps_ms)
+    phase = {
+        "push_phase_ms_per_step": prof[1] / args.profile_steps,
+        "count_kernel_ms": prof[4] / launches,
+        "scan_ms": prof[5] / launches,
+        "scatter_kernel_ms": k3_ms,
+        "field_solve_ms_per_step": prof[3] / args.profile_steps,
+        "push_phase_frac_of_hbm_peak": round(
+            (48.0 * 2 * npart_per_launch + 16.0 * 2 * p.nodes ** 2) /
+            (prof[1] / args.profile_steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    out = {
+        "metric": "particle-steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Philox-seeded uniform positions, Maxwellian velocities)",
+        "config": describe(args.workload, p),
+        "latency_ms": {"p50": float(np.percentile(steps_ms, 50)) if steps_ms.size else None,
+                       "p99": float(np.percentile(steps_ms, 99)) if steps_ms.size else None},
+        "roofline": roof,
+        "phases": phase,
+        "cpu_baseline": None,
+    }
+    sim.close()
+
+    if not args.no_cpu_baseline:
+        n_all = len(e0["id"]) + len(i0["id"])
+        steps = args.cpu_steps or max(1, min(50, int(round(20.0 / (n_all * 60e-9 + 1e-3)))))
+        log("cpu baseline:", steps, "steps of the FP64 reference-layout oracle, 1 thread")
+        v1, t_1, _ = cpu_baseline(p, e0, i0, steps, 1)
+        out["cpu_baseline"] = {
+            "value": v1,
+            "unit": "particle-steps/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{steps} steps of the full workload ({n_all} particles), "
+                      f"FP64 AoS reference layout, {t_1:.1f} s",
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -122,6 +122,11 @@ int pic_set_runtime(pic_handle *h, int32_t run_time);
  * the fused timestep (collisions, gather+push, walls, deposit, rho, phi, E). */
 int pic_run(pic_handle *h);
 
+/* The timestep loop of pic_run without its pre-loop wall pass: advances
+ * nsteps steps (no reference counterpart; used by drivers that step a run
+ * repeatedly, e.g. a benchmark or a serving loop). */
+int pic_advance(pic_handle *h, int32_t nsteps);
+
 /* pic::dump_data (pic.cpp:1285-1295) -> particle::dump_data
  * (particle.cpp:1361-1382): dir/dataDump_e.txt, dir/dataDump_i1.txt.
  * precision = significant digits (6 = the reference's ostream default). */
@@ -184,6 +189,9 @@ int pic_profile_enable(pic_handle *h, int32_t on);
 /* out[0..7]: steps timed, push-phase ms, scatter-kernel ms, field ms,
  * particles pushed (sum over timed steps), 0, 0, 0 */
 int pic_profile_read(pic_handle *h, double *out8);
+/* per-step durations (ms) of the profiled steps since the last read, oldest
+ * first; *n receives the count written (at most cap) */
+int pic_profile_steps(pic_handle *h, double *out, int32_t cap, int32_t *n);
 int pic_sync(pic_handle *h);
 
 #ifdef __cplusplus

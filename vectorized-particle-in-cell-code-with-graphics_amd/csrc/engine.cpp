@@ -137,6 +137,7 @@ void prof_flush(pic_handle *h) {
         hipEventElapsedTime(&ms, p.a, p.b);
         h->prof_acc[p.tag] += ms;
         if (p.tag == 2) h->prof_acc[7] += 1.0;
+        if (p.tag == 0) h->prof_steps.push_back(ms);
         h->prof_pool.push_back(p);
     }
     h->prof_live.clear();
@@ -543,6 +544,14 @@ int pic_run(pic_handle *h) {
     return PIC_OK;
 }
 
+int pic_advance(pic_handle *h, int32_t nsteps) {
+    if (!h || nsteps < 0) return set_global_error(PIC_ERR_ARG, "bad argument");
+    for (int32_t t = 0; t < nsteps; t++) fused_step(h);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return seterr(h, PIC_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
+    return PIC_OK;
+}
+
 int pic_sync(pic_handle *h) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
     HIPCHK(h, hipStreamSynchronize(h->st));
@@ -805,6 +814,18 @@ int pic_profile_enable(pic_handle *h, int32_t on) {
     if (!on && h->prof) prof_flush(h);
     h->prof = on != 0;
     for (double &v : h->prof_acc) v = 0.0;
+    h->prof_steps.clear();
+    return PIC_OK;
+}
+
+int pic_profile_steps(pic_handle *h, double *out, int32_t cap, int32_t *n) {
+    if (!h || !n || (cap > 0 && !out)) return set_global_error(PIC_ERR_ARG, "null argument");
+    prof_flush(h);
+    int32_t k = 0;
+    for (double v : h->prof_steps)
+        if (k < cap) out[k++] = v;
+    *n = k;
+    h->prof_steps.clear();
     return PIC_OK;
 }
 

@@ -243,29 +243,43 @@ template void launch_fill<uint32_t>(hipStream_t, uint32_t *, long, uint32_t);
 
 static __host__ __device__ inline int lvl_size(int l) { return (1 << (l + 1)) + 1; }
 
-template <typename T>
-struct CoarseLayout {
-    T *rhs[8];
-    T *cres[8];
-    T *cv[8];
-    T *u;
-    T *tmp;
-};
+// LDS layout of the coarse part topped at level lc (levels 0..lc):
+//   rhs[0..lc] | cres[0], cv[0], cres[1], cv[1], ... cres[lc-1], cv[lc-1] | u
+// Offsets are computed arithmetically (wave-uniform) so no pointer table is
+// kept per lane.
+static __host__ __device__ inline long sq_below(int l) {
+    long s = 0;
+    for (int k = 0; k < l; k++) s += (long)lvl_size(k) * lvl_size(k);
+    return s;
+}
 
 // doubles (or floats) of LDS needed for a coarse part topped at level lc
 static __host__ __device__ inline long coarse_lds_elems(int lc) {
-    long e = 0;
-    for (int l = 0; l <= lc; l++) e += (long)lvl_size(l) * lvl_size(l);  // rhs
-    for (int l = 0; l < lc; l++) e += 2l * lvl_size(l) * lvl_size(l);    // cres, cv
-    e += 2l * lvl_size(lc) * lvl_size(lc);                               // u, tmp
-    return e;
+    return sq_below(lc + 1) + 2 * sq_below(lc) + (long)lvl_size(lc) * lvl_size(lc);
+}
+
+template <typename T>
+struct CoarseLds {
+    T *base;
+    int lc;
+    __device__ __forceinline__ T *rhs(int l) const { return base + sq_below(l); }
+    __device__ __forceinline__ T *cres(int l) const {
+        return base + sq_below(lc + 1) + 2 * sq_below(l);
+    }
+    __device__ __forceinline__ T *cv(int l) const { return cres(l) + (long)lvl_size(l) * lvl_size(l); }
+    __device__ __forceinline__ T *u() const { return base + sq_below(lc + 1) + 2 * sq_below(lc); }
+};
+
+__device__ __forceinline__ double lvl_h2(int l, double length) {
+    double h = length / (lvl_size(l) - 1);
+    return h * h;
 }
 
 template <typename T>
 __device__ void co_relax(T *u, const T *rhs, int n, T h2) {
     int m = n - 2;  // interior per row
+    int half = (m + 1) / 2;
     for (int color = 0; color < 2; color++) {
-        int half = (m + 1) / 2;
         for (int k = threadIdx.x; k < m * half; k += blockDim.x) {
             int i = 1 + k / half, t = k % half;
             int j = 1 + ((i + 1 + color) & 1) + 2 * t;
@@ -275,12 +289,14 @@ __device__ void co_relax(T *u, const T *rhs, int n, T h2) {
     }
 }
 
+// residual restricted to the coarse grid (mglin.cpp:189-190) and the coarse
+// correction zeroed (mglin.cpp:185), one pass
 template <typename T>
-__device__ void co_resid(T *res, const T *u, const T *rhs, int n, T h2i) {
-    for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
-        int i = k / n, j = k % n;
-        res[k] = (i == 0 || j == 0 || i == n - 1 || j == n - 1) ? (T)0.0
-                                                                 : resid_node(u, rhs, n, i, j, h2i);
+__device__ void co_resid_rstrct_zero(const T *u, const T *rhs, int nf, T h2i, T *cres, T *cv) {
+    int nc = (nf + 1) / 2;
+    for (int k = threadIdx.x; k < nc * nc; k += blockDim.x) {
+        cres[k] = resid_rstrct_node(u, rhs, nf, k / nc, k % nc, h2i);
+        cv[k] = (T)0.0;
     }
     __syncthreads();
 }
@@ -319,25 +335,23 @@ __device__ void co_slvsml(T *u, const T *rhs, double length) {
 
 // V(1,1) cycle at level j on (U, R), recursion unrolled (mglin.cpp:176-196)
 template <typename T>
-__device__ void co_vcycle(CoarseLayout<T> &L, int j, T *U, const T *R, const T *h2, const T *h2i,
-                          double length) {
+__device__ void co_vcycle(const CoarseLds<T> &L, int j, T *U, const T *R, double length) {
     T *Ul = U;
     const T *Rl = R;
     for (int l = j; l >= 1; l--) {
         int n = lvl_size(l);
-        co_relax(Ul, Rl, n, h2[l]);
-        co_resid(L.tmp, Ul, Rl, n, h2i[l]);
-        co_rstrct(L.cres[l - 1], lvl_size(l - 1), L.tmp);
-        co_zero(L.cv[l - 1], lvl_size(l - 1) * lvl_size(l - 1));
-        Ul = L.cv[l - 1];
-        Rl = L.cres[l - 1];
+        double h2 = lvl_h2(l, length);
+        co_relax(Ul, Rl, n, (T)h2);
+        co_resid_rstrct_zero(Ul, Rl, n, (T)(1.0 / h2), L.cres(l - 1), L.cv(l - 1));
+        Ul = L.cv(l - 1);
+        Rl = L.cres(l - 1);
     }
-    co_slvsml(L.cv[0], L.cres[0], length);
+    co_slvsml(L.cv(0), L.cres(0), length);
     for (int l = 1; l <= j; l++) {
-        T *Uu = (l == j) ? U : L.cv[l];
-        const T *Ru = (l == j) ? R : L.cres[l];
-        co_interp(Uu, lvl_size(l), L.cv[l - 1], true);
-        co_relax(Uu, Ru, lvl_size(l), h2[l]);
+        T *Uu = (l == j) ? U : L.cv(l);
+        const T *Ru = (l == j) ? R : L.cres(l);
+        co_interp(Uu, lvl_size(l), L.cv(l - 1), true);
+        co_relax(Uu, Ru, lvl_size(l), (T)lvl_h2(l, length));
     }
 }
 
@@ -349,46 +363,29 @@ __global__ void __launch_bounds__(1024)
     k_mg_coarse(const T *__restrict__ rhs_in, T *__restrict__ u_out, int lc, int mode, int ncycle,
                 double length) {
     extern __shared__ double lds_raw[];
-    T *base = reinterpret_cast<T *>(lds_raw);
-    CoarseLayout<T> L;
-    long off = 0;
-    for (int l = 0; l <= lc; l++) {
-        L.rhs[l] = base + off;
-        off += (long)lvl_size(l) * lvl_size(l);
-    }
-    for (int l = 0; l < lc; l++) {
-        L.cres[l] = base + off;
-        off += (long)lvl_size(l) * lvl_size(l);
-        L.cv[l] = base + off;
-        off += (long)lvl_size(l) * lvl_size(l);
-    }
+    CoarseLds<T> L{reinterpret_cast<T *>(lds_raw), lc};
     int nc = lvl_size(lc);
-    L.u = base + off;
-    off += (long)nc * nc;
-    L.tmp = base + off;
-    T h2[8], h2i[8];
-    for (int l = 0; l <= lc; l++) {
-        double h = length / (lvl_size(l) - 1);
-        h2[l] = (T)(h * h);
-        h2i[l] = (T)(1.0 / (h * h));
-    }
-    for (int k = threadIdx.x; k < nc * nc; k += blockDim.x) L.rhs[lc][k] = rhs_in[k];
+    T *top = L.rhs(lc);
+    for (int k = threadIdx.x; k < nc * nc; k += blockDim.x) top[k] = rhs_in[k];
     __syncthreads();
     if (mode == 0) {
-        for (int l = lc - 1; l >= 0; l--) co_rstrct(L.rhs[l], lvl_size(l), L.rhs[l + 1]);
-        T *prev = (lc == 0) ? L.u : L.cv[0];
-        co_slvsml(prev, L.rhs[0], length);
+        for (int l = lc - 1; l >= 0; l--) co_rstrct(L.rhs(l), lvl_size(l), L.rhs(l + 1));
+        // the full-multigrid iterate of level j < lc lives in cv(j) (free at
+        // that point: a V-cycle at level j only touches cv(0..j-1))
+        T *prev = (lc == 0) ? L.u() : L.cv(0);
+        co_slvsml(prev, L.rhs(0), length);
         for (int j = 1; j <= lc; j++) {
-            T *Uj = (j == lc) ? L.u : L.cv[j];
+            T *Uj = (j == lc) ? L.u() : L.cv(j);
             co_interp(Uj, lvl_size(j), prev, false);
-            for (int c = 0; c < ncycle; c++) co_vcycle(L, j, Uj, L.rhs[j], h2, h2i, length);
+            for (int c = 0; c < ncycle; c++) co_vcycle(L, j, Uj, L.rhs(j), length);
             prev = Uj;
         }
     } else {
-        co_zero(L.u, nc * nc);
-        for (int c = 0; c < ncycle; c++) co_vcycle(L, lc, L.u, L.rhs[lc], h2, h2i, length);
+        co_zero(L.u(), nc * nc);
+        for (int c = 0; c < ncycle; c++) co_vcycle(L, lc, L.u(), L.rhs(lc), length);
     }
-    for (int k = threadIdx.x; k < nc * nc; k += blockDim.x) u_out[k] = L.u[k];
+    T *u = L.u();
+    for (int k = threadIdx.x; k < nc * nc; k += blockDim.x) u_out[k] = u[k];
 }
 
 // ------------------------------------------------------------ MG driver

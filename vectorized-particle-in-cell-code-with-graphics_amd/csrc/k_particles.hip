@@ -41,6 +41,75 @@ __device__ __forceinline__ void store_state(const SpView &v, uint32_t i, const P
     v.id[i] = id;
 }
 
+// Wavefront key grouping: lanes holding the same key form one group (one per
+// distinct key in the wavefront; leader = lowest lane).  Pure ALU/ballot work,
+// so the atomics that follow are issued by all leaders in a single
+// instruction instead of one round trip per group.  Sorted input keeps the
+// number of groups per wavefront small.
+struct WaveGroup {
+    int leader;      // lane of this lane's group leader (-1 if inactive)
+    uint32_t rank;   // position of this lane within its group
+    uint32_t count;  // group size (meaningful on the leader)
+};
+
+__device__ __forceinline__ WaveGroup wave_group(uint32_t key, bool active) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    WaveGroup g{-1, 0u, 0u};
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        int leader = __ffsll((unsigned long long)pending) - 1;  // wave-uniform
+        uint32_t lkey = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+        uint64_t grp = __ballot(active && key == lkey) & pending;
+        if ((grp >> lane) & 1ull) {
+            g.leader = leader;
+            g.rank = (uint32_t)__popcll(grp & below);
+            g.count = (uint32_t)__popcll(grp);
+        }
+        pending &= ~grp;
+    }
+    return g;
+}
+
+// histogram update: one atomic per distinct key of the wavefront
+__device__ __forceinline__ void wave_count(uint32_t *__restrict__ binCount, uint32_t key,
+                                           bool active) {
+    WaveGroup g = wave_group(key, active);
+    if (active && g.rank == 0) atomicAdd(&binCount[key], g.count);
+}
+
+// scatter slots: each leader takes count slots of its bin with one atomicSub
+// (counts run down to zero, leaving the histogram cleared for the next step);
+// members take distinct slots by rank.  Returns 0xFFFFFFFF for inactive lanes.
+// The claim is split in two so the atomic's round trip overlaps other work:
+// wave_claim_issue() starts the leader's atomicSub and the bin-start load,
+// wave_claim_slot() consumes them.
+struct Claim {
+    WaveGroup g;
+    uint32_t old;    // leader: bin count before the subtraction
+    uint32_t start;  // bin start of this lane's key
+};
+
+__device__ __forceinline__ Claim wave_claim_issue(uint32_t *__restrict__ binCount,
+                                                  const uint32_t *__restrict__ binStart,
+                                                  uint32_t key, bool active) {
+    Claim c;
+    c.g = wave_group(key, active);
+    c.old = 0;
+    c.start = 0;
+    if (active) c.start = binStart[key];
+    if (active && c.g.rank == 0) c.old = atomicSub(&binCount[key], c.g.count);
+    return c;
+}
+
+// Returns the slot, or 0xFFFFFFFF for inactive lanes.
+__device__ __forceinline__ uint32_t wave_claim_slot(const Claim &c, bool active) {
+    uint32_t old = __shfl(c.old, c.g.leader < 0 ? 0 : c.g.leader, 64);
+    if (!active) return 0xFFFFFFFFu;
+    // pass 1 counted every member into this bin, so old >= count > rank
+    return (old > c.g.rank) ? c.start + old - 1u - c.g.rank : 0xFFFFFFFFu;
+}
+
 // ------------------------------------------------------------ pass 1
 __global__ void __launch_bounds__(256)
     k_push_count(SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E,
@@ -51,15 +120,19 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     uint32_t n = *nptr;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool kept = false;
+    uint32_t key = 0;
     if (i < n) {
         PState s = load_state(sp, i, pc.bfield != 0);
         if (do_push) push_one(s, E, pc);
         int st = classify(s, pc.N, pc.mode);
-        if (st == ST_KEPT)
-            atomicAdd(&binCount[bin_key(s.jx, s.jy, pc.tilesX)], 1u);
+        kept = (st == ST_KEPT);
+        if (kept)
+            key = bin_key(s.jx, s.jy, pc.tilesX);
         else
             atomicAdd(&sl[st - 1], 1u);
     }
+    wave_count(binCount, key, kept);
     __syncthreads();
     if (threadIdx.x < 4 && sl[threadIdx.x]) atomicAdd(&lost4[threadIdx.x], sl[threadIdx.x]);
 }
@@ -171,8 +244,8 @@ void launch_scan(hipStream_t st, const uint32_t *count, uint32_t *start, uint32_
 uint32_t scan_pad(uint32_t nbins) { return (nbins + SCAN_TILE - 1) / SCAN_TILE * SCAN_TILE; }
 
 // ------------------------------------------------------------ pass 2
-#define DEP_WCAP 4096  // LDS window capacity (nodes), 32 KB of uint64
-#define DEP_MARGIN 3   // cells of motion allowed around the block's old cells
+#define DEP_WCAP 2048  // LDS window capacity (nodes), 16 KB of uint64
+#define DEP_MARGIN 2   // cells of motion allowed around the block's old cells
 
 __device__ __forceinline__ int wave_min(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
@@ -266,24 +339,23 @@ __global__ void __launch_bounds__(256)
     PState s{};
     int ojx = 0, ojy = 0;
     int st = -1;
+    uint32_t key = 0;
     if (valid) {
         s = load_state(src, i, true);
         ojx = s.jx;
         ojy = s.jy;
         if (do_push) push_one(s, E, pc);
         st = classify(s, pc.N, pc.mode);
-        if (st == ST_KEPT) {
-            uint32_t key = bin_key(s.jx, s.jy, pc.tilesX);
-            uint32_t c = atomicSub(&binCount[key], 1u);
-            // pass 1 counted this particle into the same bin, so c >= 1; the
-            // test only keeps a corrupted count from writing out of bounds
-            uint32_t slot = binStart[key] + c - 1u;
-            if (c != 0u && slot < cap) store_state(dst, slot, s, src.id[i]);
-        }
+        if (st == ST_KEPT) key = bin_key(s.jx, s.jy, pc.tilesX);
     }
-    if (!deposit) return;
-    bool dep = (st == ST_KEPT) && deposit_ok(s, pc.N, pc.mode);
-    block_deposit(dep, s, valid, ojx, ojy, q, pc.N);
+    const bool kept = (st == ST_KEPT);
+    uint32_t pid = valid ? src.id[i] : 0u;
+    Claim claim = wave_claim_issue(binCount, binStart, key, kept);
+    // the deposit runs while the claim's atomic is in flight
+    if (deposit) block_deposit(kept && deposit_ok(s, pc.N, pc.mode), s, valid, ojx, ojy, q, pc.N);
+    uint32_t slot = wave_claim_slot(claim, kept);
+    // the bound test only keeps a corrupted histogram from writing out of range
+    if (kept && slot < cap) store_state(dst, slot, s, pid);
 }
 
 // ------------------------------------------------------- per-phase path

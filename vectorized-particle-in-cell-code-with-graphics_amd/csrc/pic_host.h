@@ -56,4 +56,5 @@ struct pic_handle {
     bool prof = false;
     std::vector<pic_host::ProfPair> prof_live, prof_pool;
     double prof_acc[8] = {};
+    std::vector<double> prof_steps;  // per-step durations (ms)
 };

@@ -170,6 +170,8 @@ def load_library(path=LIB_PATH):
         "pic_set_E32": [vp, vp],
         "pic_profile_enable": [vp, i32],
         "pic_profile_read": [vp, vp],
+        "pic_profile_steps": [vp, vp, i32, P(i32)],
+        "pic_advance": [vp, i32],
         "pic_sync": [vp],
     }
     for name, args in sig.items():
@@ -257,6 +259,10 @@ class Pic:
     def run(self, steps):
         self.setRuntime(steps)
         self.pic_run()
+
+    def advance(self, steps):
+        """the step loop without pic_run's pre-loop wall pass"""
+        self._c(self.lib.pic_advance(self.h, int(steps)))
 
     def dump_data(self, directory=".", precision=17):
         self._c(self.lib.pic_dump_data(self.h, str(directory).encode(), int(precision)))
@@ -369,9 +375,17 @@ class Pic:
         self._c(self.lib.pic_profile_enable(self.h, int(on)))
 
     def profile_read(self):
+        """[step ms, push-phase ms, scatter-kernel ms, field ms, count-kernel ms,
+        scan ms, -, scatter launches] summed over the profiled steps"""
         a = np.zeros(8)
         self._c(self.lib.pic_profile_read(self.h, _ptr(a)))
         return a
+
+    def profile_steps(self, cap=1 << 16):
+        a = np.zeros(cap)
+        n = C.c_int32()
+        self._c(self.lib.pic_profile_steps(self.h, _ptr(a), cap, C.byref(n)))
+        return a[: n.value]
 
 
 def cell_unpack(cell):
