@@ -158,7 +158,18 @@ int pic_boundary(pic_handle *h, int32_t which);
  * holds the rhs on entry and the solution on exit. */
 int pic_mglin(double *u, int32_t n, int32_t ncycle, double length);
 
+/* Residual-controlled Poisson solve (no reference counterpart; the
+ * PIC_MG_TOL solver of the timestep): A u = f with the 5-point Laplacian of
+ * mglin.cpp:133-144, zero walls; u (n*n) is the initial guess on entry and
+ * the solution on exit.  FP32 V(1,1) cycles on the correction equation, FP64
+ * residual and accumulation, until ||f-Au||_2 <= tol ||f||_2 (interior). */
+int pic_poisson_tol(const double *f, double *u, int32_t n, double length, double tol,
+                    int32_t max_iter, int32_t *cycles, double *relres);
+
 /* --------------------------------------------------------------- getters */
+/* multigrid work of the last solve: V-cycles (tolerance mode) or ncycle,
+ * cumulative V-cycles, and the relative residual reached (tolerance mode) */
+int pic_get_mg_stats(pic_handle *h, int32_t *last_cycles, int64_t *total_cycles, double *relres);
 int pic_particle_size(pic_handle *h, int32_t s, int64_t *n);        /* particle::particle_size */
 int pic_get_particles(pic_handle *h, int32_t s, pic_particle *out, int64_t cap); /* 48-B AoS copy */
 int pic_get_phi(pic_handle *h, double *out);            /* nodes*nodes (fields::get_phi)      */

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -90,24 +91,46 @@ int alloc_mg(pic_handle *h, MgLevels &M, int N) {
     mg_set_coarse_attr(lds);
     for (int l = M.lc; l < M.ng; l++) {
         size_t s = (size_t)((1 << (l + 1)) + 1) * ((1 << (l + 1)) + 1);
-        CHECK(dalloc(h, &M.rhsF[l], s));
-        CHECK(dalloc(h, &M.u[l], s));
+        CHECK(dalloc(h, &M.d.rhsF[l], s));
+        CHECK(dalloc(h, &M.d.u[l], s));
+        CHECK(dalloc(h, &M.f.rhsF[l], s));
+        CHECK(dalloc(h, &M.f.u[l], s));
         if (l < M.ng - 1) {
-            CHECK(dalloc(h, &M.cres[l], s));
-            CHECK(dalloc(h, &M.cv[l], s));
+            CHECK(dalloc(h, &M.d.cres[l], s));
+            CHECK(dalloc(h, &M.d.cv[l], s));
+            CHECK(dalloc(h, &M.f.cres[l], s));
+            CHECK(dalloc(h, &M.f.cv[l], s));
         }
     }
+    size_t nf = (size_t)N * N;
+    HIPCHK(h, hipMemset(M.d.u[M.ng - 1], 0, nf * sizeof(double)));  // cold start of the iterate
+    M.nparts = (int)std::min<size_t>(1024, (nf + 255) / 256);
+    CHECK(dalloc(h, &M.partials, 2 * (size_t)M.nparts));
+    CHECK(dalloc(h, &M.flags, 2));
+    HIPCHK(h, hipHostMalloc((void **)&M.relres, sizeof(double), 0));
+    HIPCHK(h, hipHostMalloc((void **)&M.host_flags, 2 * sizeof(int), 0));
+    *M.relres = 0.0;
+    M.host_flags[0] = M.host_flags[1] = 0;
     return PIC_OK;
 }
 
 void free_mg(MgLevels &M) {
     for (int l = 0; l < 16; l++) {
-        if (M.rhsF[l]) hipFree(M.rhsF[l]);
-        if (M.u[l]) hipFree(M.u[l]);
-        if (M.cres[l]) hipFree(M.cres[l]);
-        if (M.cv[l]) hipFree(M.cv[l]);
-        M.rhsF[l] = M.u[l] = M.cres[l] = M.cv[l] = nullptr;
+        void *ptrs[] = {M.d.rhsF[l], M.d.u[l], M.d.cres[l], M.d.cv[l],
+                        M.f.rhsF[l], M.f.u[l], M.f.cres[l], M.f.cv[l]};
+        for (void *p : ptrs)
+            if (p) hipFree(p);
     }
+    M.d = MgArrays<double>();
+    M.f = MgArrays<float>();
+    if (M.partials) hipFree(M.partials);
+    if (M.flags) hipFree(M.flags);
+    if (M.relres) hipHostFree(M.relres);
+    if (M.host_flags) hipHostFree(M.host_flags);
+    M.partials = nullptr;
+    M.flags = nullptr;
+    M.relres = nullptr;
+    M.host_flags = nullptr;
 }
 
 // ----------------------------------------------------------- profiling
@@ -287,28 +310,39 @@ void absorb_species(pic_handle *h, int si) {
     s.cur = 1 - c;
 }
 
-void solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, double phiLY) {
+int solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, double phiLY) {
     const pic_params &P = h->P;
     int N = h->N;
+    MgLevels &M = h->M;
     launch_fold(h->st, h->rho, N, 1.0 / P.epsilon, phi0X, phiR, phi0Y, phiLY,
-                1.0 / (h->dx * h->dx), h->M.rhsF[h->M.ng - 1]);
-    mg_fixed_solve(h->st, h->M, P.mg_ncycle, P.length);
-    launch_phi_bc(h->st, h->M.u[h->M.ng - 1], N, phi0X, phiR, phi0Y, phiLY, h->phi);
+                1.0 / (h->dx * h->dx), M.d.rhsF[M.ng - 1]);
+    if (P.mg_mode == PIC_MG_TOL) {
+        int used = mg_tol_solve(h->st, M, P.length, P.mg_tol, P.mg_max_iter, &h->mg_relres);
+        if (used < 0) return seterr(h, PIC_ERR_DEVICE, "multigrid: stream error");
+        h->mg_cycles = used;
+        h->mg_cycles_total += used;
+    } else {
+        mg_fixed_solve(h->st, M, P.mg_ncycle, P.length);
+        h->mg_cycles = P.mg_ncycle;
+    }
+    launch_phi_bc(h->st, M.d.u[M.ng - 1], N, phi0X, phiR, phi0Y, phiLY, h->phi);
     h->have_phi = true;
+    return PIC_OK;
 }
 
-void field_step(pic_handle *h, int64_t step) {
+int field_step(pic_handle *h, int64_t step) {
     const pic_params &P = h->P;
     double phiR = P.phi_right * cos(P.phi_period * (double)step);
     int p = prof_begin(h, 3);
     launch_rho(h->st, h->sp[1].q, h->sp[0].q, h->N, h->sp[1].scale, h->sp[0].scale,
                P.corner8 ? 8.0 : 4.0, h->rho);
-    solve_potential(h, P.phi_left, phiR, P.phi_bottom, P.phi_top);
+    CHECK(solve_potential(h, P.phi_left, phiR, P.phi_bottom, P.phi_top));
     launch_efield(h->st, h->phi, h->N, h->dx, h->ex64, h->ey64, h->E32);
     prof_end(h, p);
+    return PIC_OK;
 }
 
-void fused_step(pic_handle *h) {
+int fused_step(pic_handle *h) {
     int N = h->N;
     int pt = prof_begin(h, 0);
     int pp = prof_begin(h, 1);
@@ -322,10 +356,11 @@ void fused_step(pic_handle *h) {
         launch_deposit(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], s.cap, N, s.pc.mode, s.q);
     }
     prof_end(h, pp);
-    field_step(h, h->step);
+    CHECK(field_step(h, h->step));
     prof_end(h, pt);
     h->have_rho = true;
     h->step++;
+    return PIC_OK;
 }
 
 int check_species(pic_handle *h, int s) {
@@ -381,8 +416,10 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         return set_global_error(PIC_ERR_UNSUPPORTED, "steady-state plop walls: not available yet");
     if (P.have_collisions)
         return set_global_error(PIC_ERR_UNSUPPORTED, "collisions: not available yet");
-    if (P.mg_mode != PIC_MG_FIXED)
-        return set_global_error(PIC_ERR_UNSUPPORTED, "PIC_MG_TOL: not available yet");
+    if (P.mg_mode != PIC_MG_FIXED && P.mg_mode != PIC_MG_TOL)
+        return set_global_error(PIC_ERR_ARG, "unknown multigrid mode");
+    if (P.mg_mode == PIC_MG_TOL && (!(P.mg_tol > 0.0) || P.mg_max_iter < 1))
+        return set_global_error(PIC_ERR_ARG, "PIC_MG_TOL needs mg_tol > 0 and mg_max_iter >= 1");
     if (P.dt_dti < 1) return set_global_error(PIC_ERR_ARG, "DT_DTI must be >= 1");
     if (P.init_particle_no < 0) return set_global_error(PIC_ERR_ARG, "negative particle count");
     int ndev = 0;
@@ -538,7 +575,7 @@ int pic_run(pic_handle *h) {
     // pic.cpp:1049-1057: wall pass before the loop (ion wall, then electron wall)
     absorb_species(h, 1);
     absorb_species(h, 0);
-    for (int32_t t = 0; t < h->run_time; t++) fused_step(h);
+    for (int32_t t = 0; t < h->run_time; t++) CHECK(fused_step(h));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return seterr(h, PIC_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
     return PIC_OK;
@@ -546,7 +583,7 @@ int pic_run(pic_handle *h) {
 
 int pic_advance(pic_handle *h, int32_t nsteps) {
     if (!h || nsteps < 0) return set_global_error(PIC_ERR_ARG, "bad argument");
-    for (int32_t t = 0; t < nsteps; t++) fused_step(h);
+    for (int32_t t = 0; t < nsteps; t++) CHECK(fused_step(h));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return seterr(h, PIC_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
     return PIC_OK;
@@ -602,8 +639,7 @@ int pic_fields_grid_potential(pic_handle *h, double phi0X, double phiLX, double 
                               double phiLY, double timestep, double phi_period) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
     double phiR = phiLX * cos(phi_period * timestep);  // fields.cpp:177
-    solve_potential(h, phi0X, phiR, phi0Y, phiLY);
-    return PIC_OK;
+    return solve_potential(h, phi0X, phiR, phi0Y, phiLY);
 }
 
 int pic_fields_grid_electric_field(pic_handle *h) {
@@ -647,15 +683,56 @@ int pic_mglin(double *u, int32_t n, int32_t ncycle, double length) {
     if (rc == PIC_OK) {
         size_t bytes = sizeof(double) * n * n;
         MgLevels &M = tmp.M;
-        hipMemcpyAsync(M.rhsF[M.ng - 1], u, bytes, hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(M.d.rhsF[M.ng - 1], u, bytes, hipMemcpyHostToDevice, st);
         mg_fixed_solve(st, M, ncycle, length);
-        hipMemcpyAsync(u, M.u[M.ng - 1], bytes, hipMemcpyDeviceToHost, st);
+        hipMemcpyAsync(u, M.d.u[M.ng - 1], bytes, hipMemcpyDeviceToHost, st);
         hipError_t e = hipStreamSynchronize(st);
         if (e != hipSuccess) rc = set_global_error(PIC_ERR_DEVICE, hipGetErrorString(e));
     }
     if (st) hipStreamDestroy(st);
     free_mg(tmp.M);
     return rc;
+}
+
+int pic_poisson_tol(const double *f, double *u, int32_t n, double length, double tol,
+                    int32_t max_iter, int32_t *cycles, double *relres) {
+    if (!f || !u || !(tol > 0.0) || max_iter < 1) return set_global_error(PIC_ERR_ARG, "bad argument");
+    int m = n - 1;
+    if (n < 5 || (m & (m - 1)) != 0)
+        return set_global_error(PIC_ERR_GRID, "n-1 must be a power of 2 in mglin.");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_global_error(PIC_ERR_DEVICE, "no HIP device available");
+    pic_handle tmp;
+    int rc = alloc_mg(&tmp, tmp.M, n);
+    hipStream_t st = nullptr;
+    if (rc == PIC_OK && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+        rc = set_global_error(PIC_ERR_DEVICE, "stream");
+    if (rc == PIC_OK) {
+        size_t bytes = sizeof(double) * n * n;
+        MgLevels &M = tmp.M;
+        hipMemcpyAsync(M.d.rhsF[M.ng - 1], f, bytes, hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(M.d.u[M.ng - 1], u, bytes, hipMemcpyHostToDevice, st);
+        double rr = 0.0;
+        int used = mg_tol_solve(st, M, length, tol, max_iter, &rr);
+        hipMemcpyAsync(u, M.d.u[M.ng - 1], bytes, hipMemcpyDeviceToHost, st);
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess || used < 0)
+            rc = set_global_error(PIC_ERR_DEVICE, e != hipSuccess ? hipGetErrorString(e) : "mg");
+        if (cycles) *cycles = used;
+        if (relres) *relres = rr;
+    }
+    if (st) hipStreamDestroy(st);
+    free_mg(tmp.M);
+    return rc;
+}
+
+int pic_get_mg_stats(pic_handle *h, int32_t *last_cycles, int64_t *total_cycles, double *relres) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    if (last_cycles) *last_cycles = h->mg_cycles;
+    if (total_cycles) *total_cycles = h->mg_cycles_total;
+    if (relres) *relres = h->mg_relres;
+    return PIC_OK;
 }
 
 // ------------------------------------------------------------ getters

@@ -182,10 +182,16 @@ __device__ __forceinline__ T interp_node(const T *uc, int nc, int i, int j) {
 }
 
 // ------------------------------------------- multigrid: grid-wide kernels
+// `skip` (may be null) is the tolerance solver's converged flag: once set,
+// the remaining queued iterations return immediately.
+#define SKIP_IF_DONE(skip) \
+    if ((skip) && *(skip)) return
 
 template <typename T>
-__global__ void k_mg_relax(T *__restrict__ u, const T *__restrict__ rhs, int n, T h2, int color) {
-    int i = blockIdx.y + 1;            // interior row
+__global__ void k_mg_relax(T *__restrict__ u, const T *__restrict__ rhs, int n, T h2, int color,
+                           const int *__restrict__ skip) {
+    SKIP_IF_DONE(skip);
+    int i = blockIdx.y + 1;  // interior row
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int j = 1 + ((i + 1 + color) & 1) + 2 * t;  // (i+j)%2 == color
     if (i > n - 2 || j > n - 2) return;
@@ -194,7 +200,8 @@ __global__ void k_mg_relax(T *__restrict__ u, const T *__restrict__ rhs, int n, 
 
 template <typename T>
 __global__ void k_mg_resid_rstrct(const T *__restrict__ u, const T *__restrict__ rhs, int nf, T h2i,
-                                  T *__restrict__ uc) {
+                                  T *__restrict__ uc, const int *__restrict__ skip) {
+    SKIP_IF_DONE(skip);
     int nc = (nf + 1) / 2;
     int jc = blockIdx.x * blockDim.x + threadIdx.x, ic = blockIdx.y;
     if (jc >= nc) return;
@@ -216,14 +223,17 @@ __global__ void k_mg_interp(T *__restrict__ uf, int nf, const T *__restrict__ uc
 }
 
 template <typename T>
-__global__ void k_mg_addint(T *__restrict__ uf, int nf, const T *__restrict__ uc) {
+__global__ void k_mg_addint(T *__restrict__ uf, int nf, const T *__restrict__ uc,
+                            const int *__restrict__ skip) {
+    SKIP_IF_DONE(skip);
     int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
     if (j >= nf) return;
     uf[i * nf + j] = uf[i * nf + j] + interp_node(uc, (nf + 1) / 2, i, j);
 }
 
 template <typename T>
-__global__ void k_fill(T *__restrict__ p, long n, T v) {
+__global__ void k_fill(T *__restrict__ p, long n, T v, const int *__restrict__ skip) {
+    SKIP_IF_DONE(skip);
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = v;
 }
@@ -231,7 +241,8 @@ __global__ void k_fill(T *__restrict__ p, long n, T v) {
 template <typename T>
 void launch_fill(hipStream_t st, T *p, long n, T v) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v,
+                       (const int *)nullptr);
 }
 template void launch_fill<double>(hipStream_t, double *, long, double);
 template void launch_fill<float>(hipStream_t, float *, long, float);
@@ -361,7 +372,8 @@ __device__ void co_vcycle(const CoarseLds<T> &L, int j, T *U, const T *R, double
 template <typename T>
 __global__ void __launch_bounds__(1024)
     k_mg_coarse(const T *__restrict__ rhs_in, T *__restrict__ u_out, int lc, int mode, int ncycle,
-                double length) {
+                double length, const int *__restrict__ skip) {
+    SKIP_IF_DONE(skip);
     extern __shared__ double lds_raw[];
     CoarseLds<T> L{reinterpret_cast<T *>(lds_raw), lc};
     int nc = lvl_size(lc);
@@ -391,38 +403,43 @@ __global__ void __launch_bounds__(1024)
 // ------------------------------------------------------------ MG driver
 
 template <typename T>
-static void relax_launch(hipStream_t st, T *u, const T *rhs, int n, double length) {
+static void relax_launch(hipStream_t st, T *u, const T *rhs, int n, double length,
+                         const int *skip) {
     double h = length / (n - 1);
     T h2 = (T)(h * h);
     int m = n - 2, half = (m + 1) / 2;
     dim3 g((half + 127) / 128, m);
-    for (int c = 0; c < 2; c++) hipLaunchKernelGGL(k_mg_relax<T>, g, dim3(128), 0, st, u, rhs, n, h2, c);
+    for (int c = 0; c < 2; c++)
+        hipLaunchKernelGGL(k_mg_relax<T>, g, dim3(128), 0, st, u, rhs, n, h2, c, skip);
 }
 
 template <typename T>
-static void vcycle_launch(hipStream_t st, MgLevels &M, int j, T *Uj, const T *Rj, double length);
+static void coarse_launch(hipStream_t st, int lc, const T *rhs, T *u, int mode, int ncycle,
+                          double length, const int *skip) {
+    hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(1024), coarse_lds_elems(lc) * sizeof(T), st,
+                       rhs, u, lc, mode, ncycle, length, skip);
+}
 
-template <>
-void vcycle_launch<double>(hipStream_t st, MgLevels &M, int j, double *Uj, const double *Rj,
-                           double length) {
+// V(1,1) cycle at level j > lc on (Uj, Rj) (mglin.cpp:176-196)
+template <typename T>
+static void vcycle_launch(hipStream_t st, MgArrays<T> &A, int lc, int j, T *Uj, const T *Rj,
+                          double length, const int *skip) {
     int nf = lvl_size(j), nc = lvl_size(j - 1);
-    relax_launch<double>(st, Uj, Rj, nf, length);
+    relax_launch<T>(st, Uj, Rj, nf, length, skip);
     double h = length / (nf - 1);
-    hipLaunchKernelGGL(k_mg_resid_rstrct<double>, dim3((nc + 127) / 128, nc), dim3(128), 0, st, Uj,
-                       Rj, nf, 1.0 / (h * h), M.cres[j - 1]);
-    if (j - 1 == M.lc) {
-        hipLaunchKernelGGL(k_mg_coarse<double>, dim3(1), dim3(1024),
-                           coarse_lds_elems(M.lc) * sizeof(double), st, M.cres[j - 1], M.cv[j - 1],
-                           M.lc, 1, 1, length);
+    hipLaunchKernelGGL(k_mg_resid_rstrct<T>, dim3((nc + 127) / 128, nc), dim3(128), 0, st, Uj, Rj,
+                       nf, (T)(1.0 / (h * h)), A.cres[j - 1], skip);
+    if (j - 1 == lc) {
+        coarse_launch<T>(st, lc, A.cres[j - 1], A.cv[j - 1], 1, 1, length, skip);
     } else {
         long s = (long)nc * nc;
-        hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)((s + 255) / 256)), dim3(256), 0, st,
-                           M.cv[j - 1], s, 0.0);
-        vcycle_launch<double>(st, M, j - 1, M.cv[j - 1], M.cres[j - 1], length);
+        hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((s + 255) / 256)), dim3(256), 0, st,
+                           A.cv[j - 1], s, (T)0, skip);
+        vcycle_launch<T>(st, A, lc, j - 1, A.cv[j - 1], A.cres[j - 1], length, skip);
     }
-    hipLaunchKernelGGL(k_mg_addint<double>, dim3((nf + 127) / 128, nf), dim3(128), 0, st, Uj, nf,
-                       M.cv[j - 1]);
-    relax_launch<double>(st, Uj, Rj, nf, length);
+    hipLaunchKernelGGL(k_mg_addint<T>, dim3((nf + 127) / 128, nf), dim3(128), 0, st, Uj, nf,
+                       A.cv[j - 1], skip);
+    relax_launch<T>(st, Uj, Rj, nf, length, skip);
 }
 
 int mg_coarse_level_for(long lds_bytes, int ng) {
@@ -439,24 +456,153 @@ void mg_set_coarse_attr(long lds_bytes) {
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
 
-// FP64 full multigrid (mglin.cpp:21-55) on the device.  rhs = M.rhsF[ng-1]
-// (finest) on entry; the solution is written to M.u[ng-1].
+// FP64 full multigrid (mglin.cpp:21-55) on the device.  rhs = M.d.rhsF[ng-1]
+// (finest) on entry; the solution is written to M.d.u[ng-1].
 void mg_fixed_solve(hipStream_t st, MgLevels &M, int ncycle, double length) {
+    MgArrays<double> &A = M.d;
     int lf = M.ng - 1, lc = M.lc;
     for (int l = lf - 1; l >= lc; l--) {
         int nc = lvl_size(l);
         hipLaunchKernelGGL(k_mg_rstrct<double>, dim3((nc + 127) / 128, nc), dim3(128), 0, st,
-                           M.rhsF[l], nc, M.rhsF[l + 1]);
+                           A.rhsF[l], nc, A.rhsF[l + 1]);
     }
-    hipLaunchKernelGGL(k_mg_coarse<double>, dim3(1), dim3(1024),
-                       coarse_lds_elems(lc) * sizeof(double), st, M.rhsF[lc], M.u[lc], lc, 0,
-                       ncycle, length);
+    coarse_launch<double>(st, lc, A.rhsF[lc], A.u[lc], 0, ncycle, length, nullptr);
     for (int j = lc + 1; j <= lf; j++) {
         int nf = lvl_size(j);
         hipLaunchKernelGGL(k_mg_interp<double>, dim3((nf + 127) / 128, nf), dim3(128), 0, st,
-                           M.u[j], nf, M.u[j - 1]);
-        for (int c = 0; c < ncycle; c++) vcycle_launch<double>(st, M, j, M.u[j], M.rhsF[j], length);
+                           A.u[j], nf, A.u[j - 1]);
+        for (int c = 0; c < ncycle; c++)
+            vcycle_launch<double>(st, A, lc, j, A.u[j], A.rhsF[j], length, nullptr);
     }
+}
+
+// ---------------------------------------------- tolerance (mixed precision)
+// Iterative refinement: r = f - A u in FP64 (the reference's stencil and
+// association), the correction equation A e = r solved approximately by one
+// FP32 V(1,1) cycle from e = 0, u += e in FP64; repeated until
+// ||r||_2 <= tol ||f||_2 over interior nodes.
+
+#define TOL_THREADS 256
+
+__device__ __forceinline__ double block_sum_d(double v) {
+    __shared__ double ws[TOL_THREADS / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) ws[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < TOL_THREADS / 64; k++) t += ws[k];  // fixed order
+    __syncthreads();
+    return t;
+}
+
+// r32 = (float)(f - A u) at interior nodes (0 on the walls); per-block sums
+// of r^2 and f^2 in a fixed order
+__global__ void __launch_bounds__(TOL_THREADS)
+    k_tol_resid(const double *__restrict__ u, const double *__restrict__ f, int n, double h2i,
+                float *__restrict__ r32, double *__restrict__ partials,
+                const int *__restrict__ skip) {
+    SKIP_IF_DONE(skip);
+    double rr = 0.0, ff = 0.0;
+    long tot = (long)n * n;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < tot;
+         k += (long)gridDim.x * blockDim.x) {
+        int i = (int)(k / n), j = (int)(k % n);
+        float out = 0.0f;
+        if (i > 0 && j > 0 && i < n - 1 && j < n - 1) {
+            double r = resid_node(u, f, n, i, j, h2i);
+            rr += r * r;
+            ff += f[k] * f[k];
+            out = (float)r;
+        }
+        r32[k] = out;
+    }
+    rr = block_sum_d(rr);
+    ff = block_sum_d(ff);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = rr;
+        partials[2 * blockIdx.x + 1] = ff;
+    }
+}
+
+// u += e (FP64 accumulation of the FP32 correction); counts applied cycles
+__global__ void k_tol_correct(double *__restrict__ u, const float *__restrict__ e, long tot,
+                              int *__restrict__ flags) {
+    SKIP_IF_DONE(flags);
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < tot) u[k] = u[k] + (double)e[k];
+    if (k == 0) flags[1] += 1;
+}
+
+// converged?  sums the block partials in a fixed order
+__global__ void __launch_bounds__(TOL_THREADS)
+    k_tol_check(const double *__restrict__ partials, int nparts, double tol,
+                int *__restrict__ flags, double *__restrict__ relres_host,
+                int *__restrict__ host_flags) {
+    if (flags[0]) return;
+    double rr = 0.0, ff = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+        rr += partials[2 * k];
+        ff += partials[2 * k + 1];
+    }
+    rr = block_sum_d(rr);
+    ff = block_sum_d(ff);
+    if (threadIdx.x == 0) {
+        double rel = (ff > 0.0) ? sqrt(rr / ff) : sqrt(rr);
+        if (rel <= tol) flags[0] = 1;
+        *relres_host = rel;
+        host_flags[0] = flags[0];
+        host_flags[1] = flags[1];
+    }
+}
+
+__global__ void k_tol_reset(int *flags) {
+    flags[0] = 0;
+    flags[1] = 0;
+}
+
+int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max_iter,
+                 double *relres_out) {
+    int lf = M.ng - 1, n = lvl_size(lf);
+    long tot = (long)n * n;
+    double h = length / (n - 1), h2i = 1.0 / (h * h);
+    double *u = M.d.u[lf];
+    const double *f = M.d.rhsF[lf];
+    float *r32 = M.f.rhsF[lf], *e32 = M.f.u[lf];
+    const int *skip = M.flags;
+    auto resid = [&]() {
+        hipLaunchKernelGGL(k_tol_resid, dim3(M.nparts), dim3(TOL_THREADS), 0, st, u, f, n, h2i, r32,
+                           M.partials, skip);
+        hipLaunchKernelGGL(k_tol_check, dim3(1), dim3(TOL_THREADS), 0, st, M.partials, M.nparts,
+                           tol, M.flags, M.relres, M.host_flags);
+    };
+    auto cycle = [&]() {
+        hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                           e32, tot, 0.0f, skip);
+        if (lf == M.lc)
+            coarse_launch<float>(st, M.lc, r32, e32, 1, 1, length, skip);
+        else
+            vcycle_launch<float>(st, M.f, M.lc, lf, e32, r32, length, skip);
+        hipLaunchKernelGGL(k_tol_correct, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                           u, e32, tot, M.flags);
+        resid();
+    };
+    hipLaunchKernelGGL(k_tol_reset, dim3(1), dim3(1), 0, st, M.flags);
+    resid();
+    // queue the predicted number of cycles without waiting (converged cycles
+    // skip their work on the device), then confirm on the host
+    int queued = 0;
+    int batch = M.pred < 1 ? 1 : M.pred;
+    for (;;) {
+        for (int k = 0; k < batch && queued < max_iter; k++, queued++) cycle();
+        if (hipStreamSynchronize(st) != hipSuccess) return -1;
+        if (M.host_flags[0] || queued >= max_iter) break;
+        batch = 1;
+    }
+    int used = M.host_flags[1];
+    M.pred = used + (M.host_flags[0] ? 0 : 1);
+    if (relres_out) *relres_out = *M.relres;
+    return used;
 }
 
 }  // namespace pic

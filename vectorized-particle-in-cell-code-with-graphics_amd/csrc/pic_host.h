@@ -48,6 +48,9 @@ struct pic_handle {
     float2 *E32 = nullptr;
     pic::MgLevels M;
     double last_phiR = 0.0;
+    int mg_cycles = 0;          // V-cycles (tolerance mode) of the last solve
+    int64_t mg_cycles_total = 0;
+    double mg_relres = 0.0;     // relative residual after the last solve
     bool have_rho = false, have_phi = false;
     int64_t step = 0;
     int32_t run_time = 0;

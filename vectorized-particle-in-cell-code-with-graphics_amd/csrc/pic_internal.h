@@ -26,12 +26,25 @@ struct SpCounters {
 
 // Multigrid level storage on the device.  Level l has (2^(l+1)+1)^2 nodes;
 // level ng-1 is the finest.  Levels <= lc are solved inside one workgroup.
+template <typename T>
+struct MgArrays {
+    T *rhsF[16] = {};  // full-multigrid right-hand side per level
+    T *u[16] = {};     // full-multigrid solution per level (u[ng-1]: result)
+    T *cres[16] = {};  // V-cycle coarse right-hand side
+    T *cv[16] = {};    // V-cycle coarse correction
+};
+
 struct MgLevels {
     int ng = 0, lc = 0;
-    double *rhsF[16] = {};  // full-multigrid right-hand side per level
-    double *u[16] = {};     // full-multigrid solution per level (u[ng-1]: result)
-    double *cres[16] = {};  // V-cycle coarse right-hand side
-    double *cv[16] = {};    // V-cycle coarse correction
+    MgArrays<double> d;  // FP64 full multigrid (fixed mode) and the FP64 iterate
+    MgArrays<float> f;   // FP32 correction solves (tolerance mode)
+    // tolerance mode state (device): flags[0] done, flags[1] V-cycles applied
+    int *flags = nullptr;
+    double *partials = nullptr;  // per-block sums of r^2 and f^2
+    int nparts = 0;
+    double *relres = nullptr;    // pinned host: last relative residual
+    int *host_flags = nullptr;   // pinned host copy of flags
+    int pred = 4;                // predicted V-cycles per solve
 };
 
 // ---- k_particles.hip
@@ -69,5 +82,9 @@ void launch_fill(hipStream_t st, T *p, long n, T v);
 int mg_coarse_level_for(long lds_bytes, int ng);
 void mg_set_coarse_attr(long lds_bytes);
 void mg_fixed_solve(hipStream_t st, MgLevels &M, int ncycle, double length);
+// residual-controlled solve of A u = M.d.rhsF[ng-1] into M.d.u[ng-1] (warm
+// started from its current content); returns V-cycles used, -1 on HIP error
+int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max_iter,
+                 double *relres_out);
 
 }  // namespace pic

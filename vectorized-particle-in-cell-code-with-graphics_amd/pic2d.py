@@ -172,6 +172,8 @@ def load_library(path=LIB_PATH):
         "pic_profile_read": [vp, vp],
         "pic_profile_steps": [vp, vp, i32, P(i32)],
         "pic_advance": [vp, i32],
+        "pic_poisson_tol": [vp, vp, i32, dbl, dbl, i32, P(i32), P(dbl)],
+        "pic_get_mg_stats": [vp, P(i32), P(i64), P(dbl)],
         "pic_sync": [vp],
     }
     for name, args in sig.items():
@@ -213,6 +215,17 @@ def mglin(u, ncycle=2, length=1.0):
     n = a.shape[0]
     _check(load_library().pic_mglin(_ptr(a), n, ncycle, length))
     return a
+
+
+def poisson_tol(f, u0=None, length=1.0, tol=1e-10, max_iter=60):
+    """GPU mixed-precision multigrid to a relative residual; returns (u, cycles, relres)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    u = np.zeros_like(f) if u0 is None else np.ascontiguousarray(u0, dtype=np.float64).copy()
+    cyc = C.c_int32()
+    rr = C.c_double()
+    _check(load_library().pic_poisson_tol(_ptr(f), _ptr(u), f.shape[0], length, tol, max_iter,
+                                          C.byref(cyc), C.byref(rr)))
+    return u, cyc.value, rr.value
 
 
 class Pic:
@@ -364,6 +377,12 @@ class Pic:
         v = C.c_int64()
         self._c(self.lib.pic_get_step(self.h, C.byref(v)))
         return v.value
+
+    def mg_stats(self):
+        """(V-cycles of the last solve, cumulative V-cycles, last relative residual)"""
+        a, b, c = C.c_int32(), C.c_int64(), C.c_double()
+        self._c(self.lib.pic_get_mg_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def loss_counts(self, s):
         a = np.zeros(4, dtype=np.int64)
