@@ -154,6 +154,11 @@ int pic_ion_collisions(pic_handle *h);
  * (boundary.h:372-383; pic.cpp:1114-1128): which = 0 ion wall, 1 electron wall */
 int pic_boundary(pic_handle *h, int32_t which);
 
+/* Step counter used by the per-phase entry points (collision random streams
+ * and the ion sub-cycling test); pic_fields_grid_electric_field, the last
+ * phase of a step, advances it by one.  No reference counterpart. */
+int pic_set_step(pic_handle *h, int64_t step);
+
 /* Standalone Mglin (mglin.cpp:21-55) on the GPU, FP64: u is n*n row-major,
  * holds the rhs on entry and the solution on exit. */
 int pic_mglin(double *u, int32_t n, int32_t ncycle, double length);
@@ -178,7 +183,16 @@ int pic_get_E(pic_handle *h, double *ex, double *ey);   /* (nodes+1)^2 each, str
                                                            (fields::get_elec_fieldX/Y)        */
 int pic_get_q(pic_handle *h, int32_t s, double *out);   /* post-scaled charge (particle::get_q) */
 int pic_get_step(pic_handle *h, int64_t *step);
-int pic_get_loss_counts(pic_handle *h, int32_t s, int64_t *out4); /* left,right,bottom,top */
+/* particles removed: out5 = left, right, bottom, top wall, removed by a
+ * collision (the reference's ionisation rule deletes the parent) */
+int pic_get_loss_counts(pic_handle *h, int32_t s, int64_t *out5);
+/* collision events of species s: out8[k] counts events of kind k =
+ * 1 elastic, 2 excitation, 3 ionisation, 4 charge exchange, 5 isotropic ion
+ * elastic, 6 candidates whose total collision frequency exceeded the
+ * null-collision bound (probability clipped; 0 in a well-posed run) */
+int pic_get_collision_counts(pic_handle *h, int32_t s, int64_t *out8);
+/* particles dropped because a store or staging buffer was full (0 normally) */
+int pic_get_overflow(pic_handle *h, int64_t *dropped);
 int pic_get_nodes(pic_handle *h, int32_t *nodes);
 const char *pic_last_error(pic_handle *h);              /* NULL handle: last global error */
 

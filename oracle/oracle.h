@@ -197,6 +197,21 @@ int orc32_run(const orc32_config *c, int nsteps, long *ne, uint32_t *ecell, floa
               int *mg_cycles_out);
 
 /* ------------------------------------------------------------------------ */
+/* Collision cross sections [1e-20 m^2], energies in eV (orc_collisions.c)    */
+/* ------------------------------------------------------------------------ */
+#define ORC_SIG_AR_EL_REF 0   /* e_collisions.cpp:205-224, branches as written     */
+#define ORC_SIG_AR_EL_PHYS 1  /* same fit, closed breakpoints, clamped >= 0        */
+#define ORC_SIG_AR_IZ_REF 2   /* e_collisions.cpp:235-256                          */
+#define ORC_SIG_AR_IZ_PHYS 3  /* same fit above threshold thr, clamped >= 0        */
+#define ORC_SIG_AR_EXC 4      /* physical-mode excitation, threshold thr           */
+#define ORC_SIG_H_EL 5        /* e_collisions.cpp:267-283                          */
+#define ORC_SIG_H_IZ 6        /* e_collisions.cpp:293-308 (bracket / (I E))        */
+#define ORC_SIG_AR_CX 7       /* ion_collisions.cpp:170-176                        */
+#define ORC_SIG_AR_ISO 8      /* Phelps (1994) Ar+ - Ar isotropic, physical mode   */
+#define ORC_SIG_H_CX 9        /* ion_collisions.cpp:185-199, E in keV/amu          */
+double orc_sigma(int kind, double E, double thr);
+
+/* ------------------------------------------------------------------------ */
 /* dataDump text I/O (particle.cpp:359-401 reader, 1361-1382 writer)          */
 /* ------------------------------------------------------------------------ */
 /* precision = significant digits (6 reproduces the reference's ostream default) */

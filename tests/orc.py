@@ -81,6 +81,8 @@ def lib():
         _lib.orc_ref_run.argtypes = [C.POINTER(CfgRef), C.c_int] + [vp] * 12
         _lib.orc32_run.argtypes = [C.POINTER(Cfg32), C.c_int] + [vp] * 23
         _lib.orc_set_threads.argtypes = [C.c_int]
+        _lib.orc_sigma.argtypes = [C.c_int, C.c_double, C.c_double]
+        _lib.orc_sigma.restype = C.c_double
         _lib.orc_dump_write.argtypes = [C.c_char_p, vp, C.c_long, C.c_int]
         _lib.orc_dump_read.argtypes = [C.c_char_p, vp, C.c_long]
         _lib.orc_dump_read.restype = C.c_long
@@ -287,3 +289,35 @@ def random_state(n, N, sigma, seed, rng=None):
     return dict(cell=cell_pack(j[0].astype(np.int32), j[1].astype(np.int32)), ox=o[0].copy(),
                 oy=o[1].copy(), vx=v[0].copy(), vy=v[1].copy(), vz=v[2].copy(),
                 id=np.arange(n, dtype=np.uint32))
+
+
+SIG = dict(AR_EL_REF=0, AR_EL_PHYS=1, AR_IZ_REF=2, AR_IZ_PHYS=3, AR_EXC=4, H_EL=5, H_IZ=6,
+           AR_CX=7, AR_ISO=8, H_CX=9)
+
+
+def sigma(kind, E, thr=0.0):
+    """cross section [1e-20 m^2] of the oracle (restated reference fits)"""
+    f = lib().orc_sigma
+    return np.array([f(SIG[kind], float(e), thr) for e in np.atleast_1d(E)])
+
+
+def e_nu_parts(E, g, ng, gas="AR", e_exc=11.55, e_iz=15.76):
+    """electron collision frequencies (elastic, excitation, ionisation) [1/s]"""
+    f = ng * 1e-20 * g
+    if gas == "AR":
+        return (f * sigma("AR_EL_PHYS", E), f * sigma("AR_EXC", E, e_exc),
+                f * sigma("AR_IZ_PHYS", E, e_iz))
+    return f * np.maximum(sigma("H_EL", E), 0), 0 * E, f * 1000 * sigma("H_IZ", E)
+
+
+def i_nu_parts(E, g, ng):
+    f = ng * 1e-20 * g
+    return f * sigma("AR_ISO", E), f * sigma("AR_CX", E)
+
+
+def nu_max(mass, ng, electron, q=1.602177e-19):
+    """null-collision bound; the energy scan ends at 3e4 eV (electrons) / 1e3 eV (ions)"""
+    E = 10.0 ** np.linspace(-6, 4.5 if electron else 3.0, 20001)
+    g = np.sqrt(2 * q * E / mass)
+    parts = e_nu_parts(E, g, ng) if electron else i_nu_parts(E, g, ng)
+    return float(np.max(sum(parts))) * 1.02

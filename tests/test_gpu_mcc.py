@@ -1,0 +1,211 @@
+"""Monte-Carlo collisions on the GPU.
+
+Mono-energetic beams against a cold neutral background (vtn = 0) make the
+kinematics exact: excitation leaves exactly E - 11.55 eV, ionisation splits
+exactly E - 15.76 eV between the two electrons, charge exchange leaves the ion
+at rest, isotropic ion scattering keeps |v' - v/2| = |v|/2; event counts are
+binomial with probabilities from the oracle's cross sections and null-collision
+bound (checked at 4 sigma).  The reference rule set: elastic with probability
+P_el/2, every electron above 15.755 eV removed.  Collisional runs are
+deterministic and the per-phase entry points reproduce the fused step."""
+import math
+
+import numpy as np
+import pytest
+
+import orc
+import pic2d
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+Q, ME = 1.602177e-19, 9.10939e-31
+MI = 39.948 * 1.66054e-27
+
+
+def collisional(cells=32, ppc=10, mode=pic2d.MODE_PHYSICAL, vtn=0.0, plop=False, te=2.0):
+    p = workloads.discharge(cells, ppc, mode=mode, mg_mode=pic2d.MG_FIXED, te=te)
+    p.have_collisions = 1
+    p.gas = pic2d.GAS_ARGON
+    p.neutral_density = 13.332 / (1.38066e-23 * 300.0)
+    p.vtn = vtn
+    if plop:
+        p.wall = pic2d.WALL_STEADY
+    if mode == pic2d.MODE_REF:
+        p.sigma_elastic = pic2d.SIGMA_ENERGY
+        p.sigma_ionization = pic2d.SIGMA_ENERGY
+        p.sigma_cx = pic2d.SIGMA_CONSTANT
+        p.sigma_value = 0.0
+    return p
+
+
+def beam(n, N, speed, axis="vx"):
+    st = {k: np.zeros(n, np.float32) for k in ("ox", "oy", "vx", "vy", "vz")}
+    st["ox"][:] = 0.5
+    st["oy"][:] = 0.5
+    st[axis][:] = speed
+    c = N // 2
+    st["cell"] = pic2d.cell_pack(np.full(n, c), np.full(n, c))
+    st["id"] = np.arange(n, dtype=np.uint32)
+    return st
+
+
+def binom_ok(k, n, p):
+    return abs(k - n * p) <= 4 * math.sqrt(n * p * (1 - p)) + 1
+
+
+def energy(st, m):
+    v2 = st["vx"].astype(np.float64) ** 2 + st["vy"] ** 2 + st["vz"] ** 2
+    return 0.5 * m * v2 / Q
+
+
+def test_electron_beam_rates_and_exact_kinematics():
+    p = collisional(ppc=200)
+    n, E0 = 200000, 30.0
+    v0 = math.sqrt(2 * Q * E0 / ME)
+    with pic2d.Pic(p) as sim:
+        ni0 = sim.particle_size(1)
+        sim.set_state(0, beam(n, p.nodes, v0))
+        sim.set_step(3)
+        sim.e_collisions()
+        ev = sim.collision_counts(0)
+        st = sim.get_state(0)
+        ions = sim.get_state(1)
+    nu = orc.nu_max(ME, p.neutral_density, True)
+    pn = 1 - math.exp(-nu * p.dt)
+    nel, nex, niz = (x[0] for x in orc.e_nu_parts(np.array([E0]), np.array([v0]),
+                                                    p.neutral_density))
+    assert binom_ok(ev[1], n, pn * nel / nu), (ev[1], n * pn * nel / nu)
+    assert binom_ok(ev[2], n, pn * nex / nu), (ev[2], n * pn * nex / nu)
+    assert binom_ok(ev[3], n, pn * niz / nu), (ev[3], n * pn * niz / nu)
+    E = energy(st, ME)
+    assert len(st["id"]) == n + ev[3]                 # one ejected electron per ionisation
+    assert len(ions["id"]) == ni0 + ev[3]             # and one new ion
+    new_ions = ions["id"] >= ni0
+    assert np.all(ions["vx"][new_ions] == 0) and np.all(ions["vz"][new_ions] == 0)
+    parent = st["id"] < n
+    Ep = E[parent]
+    dm = 4 * ME / MI * 1.01
+    elastic_or_none = np.abs(Ep / E0 - 1) <= dm
+    excited = np.abs(Ep - (E0 - 11.55)) <= 1e-4 * E0
+    ionized = ~(elastic_or_none | excited)
+    assert excited.sum() == ev[2] and ionized.sum() == ev[3]
+    # energy of every ionised parent plus every ejected electron = n_iz (E0 - 15.76)
+    total = Ep[ionized].sum() + E[~parent].sum()
+    assert total == pytest.approx(ev[3] * (E0 - 15.76), rel=2e-5)
+    # ejected electrons are created at the parent's position with fresh,
+    # consecutive ids (the id counter continues after the loader's 0..n0-1)
+    kid = np.sort(st["id"][~parent])
+    assert kid[0] >= n and np.array_equal(kid, np.arange(kid[0], kid[0] + ev[3], dtype=np.uint32))
+    assert np.all(st["cell"][~parent] == st["cell"][0]) and np.all(st["ox"][~parent] == 0.5)
+
+
+def test_ion_beam_charge_exchange_and_isotropic_scattering():
+    p = collisional(ppc=200)
+    n, E0 = 200000, 5.0
+    v0 = math.sqrt(2 * Q * E0 / MI)
+    p.dt = p.dti = 2e-8                       # enough ion collisions per call
+    with pic2d.Pic(p) as sim:
+        sim.set_state(1, beam(n, p.nodes, v0))
+        sim.ion_collisions()
+        ev = sim.collision_counts(1)
+        st = sim.get_state(1)
+    nu = orc.nu_max(MI, p.neutral_density, False)
+    pn = 1 - math.exp(-nu * p.dt)
+    niso, ncx = (x[0] for x in orc.i_nu_parts(np.array([E0]), np.array([v0]), p.neutral_density))
+    assert binom_ok(ev[5], n, pn * niso / nu), (ev[5], n * pn * niso / nu)
+    assert binom_ok(ev[4], n, pn * ncx / nu), (ev[4], n * pn * ncx / nu)
+    vx, vy, vz = (st[k].astype(np.float64) for k in ("vx", "vy", "vz"))
+    at_rest = (vx == 0) & (vy == 0) & (vz == 0)
+    assert at_rest.sum() == ev[4]
+    moved = ~at_rest & ~((vx == np.float32(v0)) & (vy == 0) & (vz == 0))
+    assert moved.sum() == ev[5]
+    r = np.sqrt((vx[moved] - v0 / 2) ** 2 + vy[moved] ** 2 + vz[moved] ** 2)
+    assert np.allclose(r, v0 / 2, rtol=1e-5)
+
+
+def test_reference_rules_elastic_half_probability_and_ionisation_removes():
+    p = collisional(ppc=200, mode=pic2d.MODE_REF)
+    n, E0 = 200000, 10.0
+    v0 = math.sqrt(2 * Q * E0 / ME)
+    with pic2d.Pic(p) as sim:
+        sim.set_state(0, beam(n, p.nodes, v0))
+        sim.e_collisions()
+        ev = sim.collision_counts(0)
+        pel = 1 - math.exp(-(v0 * p.dt * 1e-20 * p.neutral_density * orc.sigma("AR_EL_REF", E0)[0]))
+        assert binom_ok(ev[1], n, pel / 2), (ev[1], n * pel / 2)   # elastic iff P > 2R
+        assert ev[3] == 0
+        # above 15.755 eV P_iz > 0 for every electron: all are removed
+        sim.set_state(0, beam(n, p.nodes, math.sqrt(2 * Q * 20.0 / ME)))
+        sim.e_collisions()
+        sim.boundary(1)
+        assert sim.particle_size(0) == 0
+        assert sim.loss_counts(0)[4] == n
+
+
+def by_id(s):
+    o = np.argsort(s["id"])
+    return {k: v[o] for k, v in s.items()}
+
+
+def test_collisional_run_is_deterministic_and_ids_unique():
+    p = collisional(cells=32, ppc=40, vtn=math.sqrt(0.025 * Q / MI), plop=True, te=10.0)
+    out = []
+    for _ in range(2):
+        with pic2d.Pic(p) as sim:
+            sim.run(6)
+            out.append((by_id(sim.get_state(0)), by_id(sim.get_state(1)), sim.get_qfixed(0),
+                        sim.get_phi(), sim.collision_counts(0)))
+    (e1, i1, q1, f1, c1), (e2, i2, q2, f2, c2) = out
+    assert c1[1] > 0 and c1[3] > 0                # elastic and ionising events happened
+    for a, b in ((e1, e2), (i1, i2)):
+        assert len(np.unique(a["id"])) == len(a["id"])
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(q1, q2) and np.array_equal(f1, f2)
+
+
+def test_phased_entry_points_reproduce_collisional_fused_step():
+    p = collisional(cells=32, ppc=40, vtn=math.sqrt(0.025 * Q / MI), plop=True, te=10.0)
+    with pic2d.Pic(p) as fused, pic2d.Pic(p) as phased:
+        fused.run(3)
+        for step in range(3):
+            if step == 0:
+                phased.boundary(0)
+                phased.boundary(1)
+            phased.e_collisions()
+            phased.get_particle_electric_field(0)
+            phased.move_particles(0)
+            phased.ion_collisions()
+            phased.get_particle_electric_field(1)
+            phased.move_particles(1)
+            phased.boundary(0)
+            phased.boundary(1)
+            phased.collect_charge(1)
+            phased.collect_charge(0)
+            phased.grid_rho()
+            phased.grid_potential(0, 0, 0, 0, step, 0)
+            phased.grid_electric_field()
+        assert fused.collision_counts(0)[3] > 0
+        for s in (0, 1):
+            a, b = by_id(phased.get_state(s)), by_id(fused.get_state(s))
+            assert np.array_equal(a["id"], b["id"]), s
+            for k in ("cell", "ox", "oy", "vx", "vy", "vz"):
+                assert np.array_equal(a[k], b[k]), (s, k)
+            assert np.array_equal(phased.get_qfixed(s), fused.get_qfixed(s))
+            assert np.array_equal(phased.collision_counts(s), fused.collision_counts(s))
+        assert np.array_equal(phased.get_phi(), fused.get_phi())
+        assert fused.overflow() == 0
+
+
+def test_pair_reinjection_keeps_ion_count():
+    p = collisional(cells=32, ppc=20, plop=True)
+    p.have_collisions = 0
+    p.sigma_load_i = math.sqrt(Q * 200.0 / MI)    # hot ions reach the walls within the run
+    with pic2d.Pic(p) as sim:
+        n0 = sim.particle_size(1)
+        sim.run(60)
+        lost_i = sim.loss_counts(1)[:4].sum()
+        assert lost_i > 0
+        assert sim.particle_size(1) == n0
+        assert sim.particle_size(0) == n0 - sim.loss_counts(0)[:4].sum() + lost_i

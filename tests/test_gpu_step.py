@@ -64,7 +64,7 @@ def test_fused_step_bit_exact_vs_build_oracle(mode):
         assert np.array_equal(sim.get_E32(), r9["E32"])
         lost = sim.loss_counts(0).sum() + sim.loss_counts(1).sum()
         assert lost == r["lost"].sum() + r9["lost"].sum()
-        assert np.array_equal(np.concatenate([sim.loss_counts(0), sim.loss_counts(1)]),
+        assert np.array_equal(np.concatenate([sim.loss_counts(0)[:4], sim.loss_counts(1)[:4]]),
                               r["lost"] + r9["lost"])
         assert sim.get_step() == 10
 

@@ -193,6 +193,10 @@ void species_free(Species &s) {
     if (s.q) hipFree(s.q);
     if (s.exp_) hipFree(s.exp_);
     if (s.eyp) hipFree(s.eyp);
+    void *st[] = {s.stage.cell, s.stage.ox, s.stage.oy, s.stage.vx,
+                  s.stage.vy,   s.stage.vz, s.stage.key, s.stage.idx};
+    for (void *p : st)
+        if (p) hipFree(p);
     s = Species();
 }
 
@@ -219,6 +223,124 @@ void setup_species(pic_handle *h, Species &s, bool electron) {
     pc.sin_a = (float)sin(P.angle);
     pc.cos_w = (float)cos(omega_dt);
     pc.sin_w = (float)sin(omega_dt);
+}
+
+const double kCharge = 1.602177e-19, kAmu = 1.66054e-27;
+
+// null-collision bound: max over energy of n_g * sigma_total(E) * g(E)
+double nu_max_scan(const mcc::MccConst &c) {
+    // electrons: 1e-6 .. 3e4 eV; ions: 1e-6 .. 1e3 eV (an ion above the bound
+    // would have its collision probability clipped; such events are counted)
+    double best = 0.0, top = c.electron ? 4.5 : 3.0;
+    for (int k = 0; k <= 200000; k++) {
+        double E = pow(10.0, -6.0 + (top + 6.0) * k / 200000.0);
+        double g = sqrt(2.0 * c.q * E / c.mass);
+        double a, b, d = 0.0;
+        if (c.electron)
+            mcc::e_nu_parts<double>(c, E, g, a, b, d);
+        else
+            mcc::i_nu_parts<double>(c, E, g, a, b);
+        double nu = a + b + d;
+        if (nu > best) best = nu;
+    }
+    return best * 1.02;
+}
+
+void setup_mcc(pic_handle *h, Species &s, bool electron) {
+    const pic_params &P = h->P;
+    mcc::MccConst &c = s.mc;
+    c = mcc::MccConst();
+    c.mode = P.mode == PIC_MODE_PHYSICAL ? PIC_DEV_MODE_PHYS : PIC_DEV_MODE_REF;
+    c.gas = P.gas == PIC_GAS_HYDROGEN ? mcc::GAS_H : mcc::GAS_AR;
+    c.electron = electron;
+    c.sig_el_energy = P.sigma_elastic == PIC_SIGMA_ENERGY;
+    c.sig_iz_energy = P.sigma_ionization == PIC_SIGMA_ENERGY;
+    c.sig_cx_energy = P.sigma_cx == PIC_SIGMA_ENERGY;
+    c.sig_el_const = (float)P.elastic_sigma_value;
+    c.sig_iz_const = (float)P.ionization_sigma_value;
+    c.sig_cx_const = (float)P.sigma_value;
+    c.N = h->N;
+    c.ng = (float)P.neutral_density;
+    c.dt = (float)s.dt;
+    c.mass = (float)s.mass;
+    c.M = (float)P.ion_mass;
+    c.q = (float)kCharge;
+    c.vtn = (float)P.vtn;
+    c.e_exc = 11.55f;
+    c.e_iz = c.gas == mcc::GAS_AR ? 15.76f : 13.6f;
+    c.opal_B = c.gas == mcc::GAS_AR ? 10.0f : 8.3f;
+    c.kev_amu_div = (float)(P.ion_mass / kAmu);
+    c.ntab = h->ntab;
+    c.ntab_n = 10000;
+    c.key0 = (uint32_t)P.seed;
+    c.key1 = (uint32_t)(P.seed >> 32);
+    c.on = P.have_collisions != 0;
+    if (!electron && c.mode == PIC_DEV_MODE_REF) c.on = 0;  // see pic_create: REF ion CX rejected
+    if (c.on && c.mode == PIC_DEV_MODE_PHYS) {
+        double nu = nu_max_scan(c);
+        c.nu_max = (float)nu;
+        c.p_null = (float)(1.0 - exp(-nu * s.dt));
+    }
+}
+
+// collisions.cpp:66-89: the reference's neutral velocity table, drawn from
+// the seed's Philox stream instead of rand()
+void make_neutral_table(const pic_params &P, std::vector<float> &t) {
+    const int n = 10000;
+    t.assign(3 * n, 0.0f);
+    uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
+    for (int i = 0; i < n; i++) {
+        uint32_t r[4];
+        philox4x32_10((uint32_t)i, 0x7AB1Eu, 0u, 0x4E455554u, k0, k1, r);
+        double R1 = u01d(r[0], r[1]), R2 = u01d(r[2], r[3]);
+        t[i] = (float)(P.vtn * cos(3.141592654 * R2) * sqrt(fabs(log(R1))));
+        philox4x32_10((uint32_t)i, 0x7AB1Eu, 1u, 0x4E455554u, k0, k1, r);
+        R1 = u01d(r[0], r[1]);
+        R2 = u01d(r[2], r[3]);
+        t[n + i] = (float)(P.vtn * cos(3.141592654 * R2) * sqrt(fabs(log(R1))));
+        t[2 * n + i] = (float)(P.vtn * sin(3.141592654 * R2) * sqrt(fabs(log(R1))));
+    }
+}
+
+int alloc_stage(pic_handle *h, Species &s, int64_t n0) {
+    int64_t want = std::max<int64_t>(16384, std::min<int64_t>(n0 / 16, (int64_t)1 << 22));
+    uint32_t cap = 1;
+    while ((int64_t)cap < want) cap <<= 1;
+    StageView &v = s.stage;
+    v.cap = cap;
+    CHECK(dalloc(h, &v.cell, cap));
+    CHECK(dalloc(h, &v.ox, cap));
+    CHECK(dalloc(h, &v.oy, cap));
+    CHECK(dalloc(h, &v.vx, cap));
+    CHECK(dalloc(h, &v.vy, cap));
+    CHECK(dalloc(h, &v.vz, cap));
+    CHECK(dalloc(h, &v.key, cap));
+    CHECK(dalloc(h, &v.idx, cap));
+    return PIC_OK;
+}
+
+Spawn make_spawn(pic_handle *h, bool plop) {
+    Spawn sp;
+    sp.se = h->sp[0].stage;
+    sp.si = h->sp[1].stage;
+    sp.ce = &h->cnt[0];
+    sp.ci = &h->cnt[1];
+    sp.plop = plop ? 1 : 0;
+    sp.se_sigma = (float)h->P.vte;
+    sp.si_sigma = (float)h->P.vti;
+    sp.plop_ref = 0;
+    return sp;
+}
+
+bool plop_on(pic_handle *h) {
+    return h->P.wall == PIC_WALL_STEADY && h->P.mode == PIC_MODE_PHYSICAL;
+}
+
+// append species `src`'s staged particles to species `dst_s`'s current buffer
+void commit(pic_handle *h, int src, int dst_s, int deposit) {
+    Species &d = h->sp[dst_s];
+    launch_commit(h->st, h->sp[src].stage, &h->cnt[src], d.buf[d.cur], &h->cnt[dst_s].n[d.cur],
+                  d.cap, &h->cnt[dst_s], deposit, h->N, d.pc.mode, d.q);
 }
 
 void update_scale(pic_handle *h, Species &s) {
@@ -283,30 +405,36 @@ int read_counters(pic_handle *h) {
     return PIC_OK;
 }
 
-// counting sort + (optional) push + deposit of one species
+// counting sort + (optional) collisions and push + deposit of one species
 void push_species(pic_handle *h, int si, int do_push, int deposit) {
     Species &s = h->sp[si];
     int c = s.cur;
+    uint32_t step = (uint32_t)h->step;
+    mcc::MccConst mc = s.mc;
+    if (!do_push) mc.on = 0;
+    Spawn spn = make_spawn(h, si == 1 && do_push && plop_on(h));
     int p1 = prof_begin(h, 4);
-    launch_push_count(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, h->binCount,
-                      h->cnt[si].lost, do_push);
+    launch_push_count(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, mc, step,
+                      h->binCount, h->cnt[si].lost, do_push);
     prof_end(h, p1);
     int p2 = prof_begin(h, 5);
     launch_scan(h->st, h->binCount, h->binStart, h->nbins_pad, h->scanPartial, &h->cnt[si].n[1 - c]);
     prof_end(h, p2);
     int p3 = prof_begin(h, 2);
-    launch_push_scatter(h->st, s.buf[c], s.buf[1 - c], &h->cnt[si].n[c], s.cap, h->E32, s.pc,
-                        h->binCount, h->binStart, s.q, do_push, deposit);
+    launch_push_scatter(h->st, s.buf[c], s.buf[1 - c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, mc,
+                        step, h->binCount, h->binStart, s.q, do_push, deposit, spn,
+                        h->cnt[si].events, h->cnt[si].lost);
     prof_end(h, p3);
     s.cur = 1 - c;
 }
 
-void absorb_species(pic_handle *h, int si) {
+void absorb_species(pic_handle *h, int si, bool plop) {
     Species &s = h->sp[si];
     int c = s.cur;
     set_count(h, si, 1 - c, 0);
     launch_absorb(h->st, s.buf[c], s.buf[1 - c], &h->cnt[si].n[c], &h->cnt[si].n[1 - c], s.cap,
-                  h->N, s.pc.mode, h->cnt[si].lost);
+                  h->N, s.pc.mode, h->cnt[si].lost, h->sp[1].mc, (uint32_t)h->step,
+                  make_spawn(h, plop));
     s.cur = 1 - c;
 }
 
@@ -349,11 +477,22 @@ int fused_step(pic_handle *h) {
     launch_fill<unsigned long long>(h->st, h->sp[0].q, (long)N * N, 0ull);
     launch_fill<unsigned long long>(h->st, h->sp[1].q, (long)N * N, 0ull);
     push_species(h, 0, 1, 1);
+    if (h->spawning) {
+        // ionisation products: new ions join this step's ion pass (they were
+        // appended before the ion push in the reference, pic.cpp:1071-1111);
+        // new electrons were pushed with their parents and are deposited here
+        commit(h, 1, 1, 0);
+        commit(h, 0, 0, 1);
+    }
     if (h->step % h->P.dt_dti == 0) {
         push_species(h, 1, 1, 1);
     } else {
         Species &s = h->sp[1];
         launch_deposit(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], s.cap, N, s.pc.mode, s.q);
+    }
+    if (h->spawning && plop_on(h)) {  // pairs re-injected for ions lost this step
+        commit(h, 1, 1, 1);
+        commit(h, 0, 0, 1);
     }
     prof_end(h, pp);
     CHECK(field_step(h, h->step));
@@ -412,10 +551,14 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         return set_global_error(PIC_ERR_UNSUPPORTED, "NUMBER_TO_INJECT != 0 is not supported");
     if (P.no_ion_species != 1)
         return set_global_error(PIC_ERR_UNSUPPORTED, "NO_ION_SPECIES must be 1");
-    if (P.wall != PIC_WALL_ABSORB)
-        return set_global_error(PIC_ERR_UNSUPPORTED, "steady-state plop walls: not available yet");
-    if (P.have_collisions)
-        return set_global_error(PIC_ERR_UNSUPPORTED, "collisions: not available yet");
+    if (P.wall == PIC_WALL_STEADY && P.mode == PIC_MODE_REF)
+        return set_global_error(PIC_ERR_UNSUPPORTED,
+                                "reference steady-state pairing wall: not available yet");
+    if (P.have_collisions && P.mode == PIC_MODE_REF &&
+        !(P.sigma_cx == PIC_SIGMA_CONSTANT && P.sigma_value == 0.0))
+        return set_global_error(PIC_ERR_UNSUPPORTED,
+                                "reference ion charge exchange (per-column accumulator, "
+                                "ion_collisions.cpp:297-303) is not available");
     if (P.mg_mode != PIC_MG_FIXED && P.mg_mode != PIC_MG_TOL)
         return set_global_error(PIC_ERR_ARG, "unknown multigrid mode");
     if (P.mg_mode == PIC_MG_TOL && (!(P.mg_tol > 0.0) || P.mg_max_iter < 1))
@@ -464,12 +607,25 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
             h->sp[s].weight = w0;
         }
     }
+    h->spawning = (P.have_collisions && P.mode == PIC_MODE_PHYSICAL) ||
+                  (P.wall == PIC_WALL_STEADY && P.mode == PIC_MODE_PHYSICAL);
     for (int s = 0; s < 2; s++) {
-        if (n0[s] > 0x7fffffffLL) return fail(set_global_error(PIC_ERR_CAPACITY, "too many particles"));
-        rc = species_alloc(h, h->sp[s], n0[s] > 0 ? n0[s] : 1);
+        // created particles need headroom; otherwise counts only shrink
+        int64_t cap = n0[s] + (h->spawning ? std::max<int64_t>(n0[s] / 2, 65536) : 0);
+        if (cap > 0x7fffffffLL) return fail(set_global_error(PIC_ERR_CAPACITY, "too many particles"));
+        rc = species_alloc(h, h->sp[s], cap > 0 ? cap : 1);
         if (rc != PIC_OK) return fail(rc);
+        if ((rc = alloc_stage(h, h->sp[s], n0[s])) != PIC_OK) return fail(rc);
         update_scale(h, h->sp[s]);
     }
+    if (P.have_collisions) {
+        std::vector<float> tab;
+        make_neutral_table(P, tab);
+        if ((rc = dalloc(h, &h->ntab, tab.size())) != PIC_OK) return fail(rc);
+        hipMemcpy(h->ntab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice);
+    }
+    setup_mcc(h, h->sp[0], true);
+    setup_mcc(h, h->sp[1], false);
     if ((rc = dalloc(h, &h->cnt, 2)) != PIC_OK) return fail(rc);
     if (hipHostMalloc((void **)&h->cnt_host, 2 * sizeof(SpCounters), 0) != hipSuccess)
         return fail(set_global_error(PIC_ERR_ALLOC, "hipHostMalloc failed"));
@@ -531,7 +687,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     // the loader's sort pass must not count as losses
     if (read_counters(h) != PIC_OK) return fail(PIC_ERR_DEVICE);
     for (int s = 0; s < 2; s++)
-        for (int w = 0; w < 4; w++) h->cnt_host[s].lost[w] = 0;
+        for (int w = 0; w < 6; w++) h->cnt_host[s].lost[w] = 0;
     if (hipMemcpy(h->cnt, h->cnt_host, 2 * sizeof(SpCounters), hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_global_error(PIC_ERR_DEVICE, "counter upload failed"));
     hipError_t e = hipStreamSynchronize(h->st);
@@ -546,7 +702,7 @@ int pic_destroy(pic_handle *h) {
     if (h->st) hipStreamSynchronize(h->st);
     for (int s = 0; s < 2; s++) species_free(h->sp[s]);
     void *ptrs[] = {h->cnt, h->binCount, h->binStart, h->scanPartial, h->rho,
-                    h->phi, h->ex64,     h->ey64,     h->E32};
+                    h->phi, h->ex64,     h->ey64,     h->E32,         h->ntab};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (h->cnt_host) hipHostFree(h->cnt_host);
@@ -573,8 +729,12 @@ int pic_set_runtime(pic_handle *h, int32_t run_time) {
 int pic_run(pic_handle *h) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
     // pic.cpp:1049-1057: wall pass before the loop (ion wall, then electron wall)
-    absorb_species(h, 1);
-    absorb_species(h, 0);
+    absorb_species(h, 1, plop_on(h));
+    absorb_species(h, 0, false);
+    if (h->spawning && plop_on(h)) {
+        commit(h, 1, 1, 0);
+        commit(h, 0, 0, 0);
+    }
     for (int32_t t = 0; t < h->run_time; t++) CHECK(fused_step(h));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return seterr(h, PIC_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
@@ -645,25 +805,62 @@ int pic_fields_grid_potential(pic_handle *h, double phi0X, double phiLX, double 
 int pic_fields_grid_electric_field(pic_handle *h) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
     launch_efield(h->st, h->phi, h->N, h->dx, h->ex64, h->ey64, h->E32);
+    h->step++;  // the last phase of pic_run's loop body: the step is complete
     return PIC_OK;
 }
 
 int pic_e_collisions(pic_handle *h) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
-    if (h->P.have_collisions) return seterr(h, PIC_ERR_UNSUPPORTED, "collisions not available yet");
+    Species &s = h->sp[0];
+    if (!s.mc.on) return PIC_OK;
+    launch_mcc(h->st, s.buf[s.cur], &h->cnt[0].n[s.cur], s.cap, s.pc, s.mc, (uint32_t)h->step,
+               make_spawn(h, false), h->cnt[0].events);
+    if (s.mc.mode == PIC_DEV_MODE_PHYS) {  // e_collisions.cpp:405-441 appends the products
+        commit(h, 1, 1, 0);
+        commit(h, 0, 0, 0);
+    }
     return PIC_OK;
 }
 
 int pic_ion_collisions(pic_handle *h) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
-    if (h->P.have_collisions) return seterr(h, PIC_ERR_UNSUPPORTED, "collisions not available yet");
+    Species &s = h->sp[1];
+    if (!s.mc.on) return PIC_OK;
+    launch_mcc(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], s.cap, s.pc, s.mc, (uint32_t)h->step,
+               make_spawn(h, false), h->cnt[1].events);
+    return PIC_OK;
+}
+
+int pic_set_step(pic_handle *h, int64_t step) {
+    if (!h || step < 0) return set_global_error(PIC_ERR_ARG, "bad argument");
+    h->step = step;
+    return PIC_OK;
+}
+
+int pic_get_collision_counts(pic_handle *h, int32_t s, int64_t *out8) {
+    CHECK(check_species(h, s));
+    if (!out8) return seterr(h, PIC_ERR_ARG, "null output");
+    CHECK(read_counters(h));
+    for (int k = 0; k < 8; k++) out8[k] = (int64_t)h->cnt_host[s].events[k];
+    return PIC_OK;
+}
+
+int pic_get_overflow(pic_handle *h, int64_t *dropped) {
+    if (!h || !dropped) return set_global_error(PIC_ERR_ARG, "null argument");
+    CHECK(read_counters(h));
+    *dropped = (int64_t)h->cnt_host[0].overflow + h->cnt_host[1].overflow;
     return PIC_OK;
 }
 
 int pic_boundary(pic_handle *h, int32_t which) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
     if (which != 0 && which != 1) return seterr(h, PIC_ERR_ARG, "which must be 0 (ion) or 1 (electron)");
-    absorb_species(h, which == 0 ? 1 : 0);
+    bool plop = which == 0 && plop_on(h);
+    absorb_species(h, which == 0 ? 1 : 0, plop);
+    if (plop) {
+        commit(h, 1, 1, 0);
+        commit(h, 0, 0, 0);
+    }
     return PIC_OK;
 }
 
@@ -779,6 +976,14 @@ int pic_set_state(pic_handle *h, int32_t s, const uint32_t *cell, const float *o
     HIPCHK(h, hipMemcpyAsync(b.vz, vz, bytes, hipMemcpyHostToDevice, h->st));
     HIPCHK(h, hipMemcpyAsync(b.id, id, bytes, hipMemcpyHostToDevice, h->st));
     set_count(h, s, sp.cur, (uint32_t)n);
+    // ids handed out later must not collide with the uploaded ones
+    uint32_t top = 0;
+    for (int64_t k = 0; k < n; k++) top = id[k] >= top ? id[k] + 1u : top;
+    CHECK(read_counters(h));
+    if (h->cnt_host[s].next_id < top) {
+        h->cnt_host[s].next_id = top;
+        HIPCHK(h, hipMemcpy(&h->cnt[s].next_id, &top, sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
     HIPCHK(h, hipStreamSynchronize(h->st));
     return PIC_OK;
 }
@@ -867,7 +1072,7 @@ int pic_get_loss_counts(pic_handle *h, int32_t s, int64_t *out4) {
     CHECK(check_species(h, s));
     if (!out4) return seterr(h, PIC_ERR_ARG, "null output");
     CHECK(read_counters(h));
-    for (int w = 0; w < 4; w++) out4[w] = h->cnt_host[s].lost[w];
+    for (int w = 0; w < 5; w++) out4[w] = h->cnt_host[s].lost[w];
     return PIC_OK;
 }
 

@@ -13,6 +13,7 @@
 //                           touched node.
 // Per-phase kernels for the reference-shaped entry points (gather, move,
 // collect_charge, wall pass) reuse the same device functions.
+#include "mcc.h"
 #include "pic_internal.h"
 
 namespace pic {
@@ -111,21 +112,51 @@ __device__ __forceinline__ uint32_t wave_claim_slot(const Claim &c, bool active)
 }
 
 // ------------------------------------------------------------ pass 1
+
+// e_collisions.cpp:507-511 / ion_collisions.cpp:273-277: collide only while
+// (int)(x/dx) and (int)(y/dy) are in [0, N), i.e. x in (-dx, N*dx)
+__device__ __forceinline__ bool ref_mcc_inside(const PState &s, int N) {
+    return s.jx >= -1 && s.jx <= N - 1 && !(s.jx == -1 && s.ox == 0.0f) && s.jy >= -1 &&
+           s.jy <= N - 1 && !(s.jy == -1 && s.oy == 0.0f);
+}
+
+// collision (if enabled) then push; returns the particle's status.  `ev` and
+// `ch` report the collision.  Both passes call this with identical inputs.
+__device__ __forceinline__ int advance_one(PState &s, uint32_t id, const float2 *__restrict__ E,
+                                           const PushConst &pc, const mcc::MccConst &mc,
+                                           uint32_t step, int do_push, int &ev, mcc::Child &ch) {
+    ev = mcc::EV_NONE;
+    if (mc.on) {
+        if (mc.electron) {
+            ev = mcc::collide_electron(mc, s.vx, s.vy, s.vz, id, step, ref_mcc_inside(s, pc.N), ch);
+            if (EV_KIND(ev) == mcc::EV_IONIZE && mc.mode == PIC_DEV_MODE_REF) {
+                s.jx = s.jy = CELL_DEAD;  // e_collisions.cpp:377-381: the parent does not survive
+                return ST_DEAD;
+            }
+        } else {
+            ev = mcc::collide_ion(mc, s.vx, s.vy, s.vz, id, step);
+        }
+    }
+    if (do_push) push_one(s, E, pc);
+    return classify(s, pc.N, pc.mode);
+}
+
 __global__ void __launch_bounds__(256)
     k_push_count(SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E,
-                 PushConst pc, uint32_t *__restrict__ binCount, uint32_t *__restrict__ lost4,
-                 int do_push) {
-    __shared__ uint32_t sl[4];
-    if (threadIdx.x < 4) sl[threadIdx.x] = 0;
+                 PushConst pc, mcc::MccConst mc, uint32_t step, uint32_t *__restrict__ binCount,
+                 uint32_t *__restrict__ lost6, int do_push) {
+    __shared__ uint32_t sl[6];
+    if (threadIdx.x < 6) sl[threadIdx.x] = 0;
     __syncthreads();
     uint32_t n = *nptr;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool kept = false;
     uint32_t key = 0;
     if (i < n) {
-        PState s = load_state(sp, i, pc.bfield != 0);
-        if (do_push) push_one(s, E, pc);
-        int st = classify(s, pc.N, pc.mode);
+        PState s = load_state(sp, i, pc.bfield != 0 || mc.on);
+        int ev;
+        mcc::Child ch;
+        int st = advance_one(s, mc.on ? sp.id[i] : 0u, E, pc, mc, step, do_push, ev, ch);
         kept = (st == ST_KEPT);
         if (kept)
             key = bin_key(s.jx, s.jy, pc.tilesX);
@@ -134,7 +165,7 @@ __global__ void __launch_bounds__(256)
     }
     wave_count(binCount, key, kept);
     __syncthreads();
-    if (threadIdx.x < 4 && sl[threadIdx.x]) atomicAdd(&lost4[threadIdx.x], sl[threadIdx.x]);
+    if (threadIdx.x < 6 && sl[threadIdx.x]) atomicAdd(&lost6[threadIdx.x], sl[threadIdx.x]);
 }
 
 // ------------------------------------------------------------ scan
@@ -156,6 +187,27 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total)
     __syncthreads();
     uint32_t pre = 0, tot = 0;
     for (int k = 0; k < SCAN_THREADS / 64; k++) {
+        if (k < w) pre += wsum[k];
+        tot += wsum[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+// exclusive scan over a 1024-thread block
+__device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t *total) {
+    __shared__ uint32_t wsum[16];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
         if (k < w) pre += wsum[k];
         tot += wsum[k];
     }
@@ -328,34 +380,96 @@ __device__ __forceinline__ void block_deposit(bool dep, const PState &s, bool va
     }
 }
 
+// staging of a created particle (slot order is arbitrary; the commit pass
+// sorts by key)
+__device__ __forceinline__ void stage_put(const StageView &sv, uint32_t *__restrict__ count,
+                                          uint32_t *__restrict__ overflow, const PState &s,
+                                          unsigned long long key) {
+    uint32_t slot = atomicAdd(count, 1u);
+    if (slot >= sv.cap) {
+        atomicAdd(overflow, 1u);
+        return;
+    }
+    sv.cell[slot] = cell_pack(s.jx, s.jy);
+    sv.ox[slot] = s.ox;
+    sv.oy[slot] = s.oy;
+    sv.vx[slot] = s.vx;
+    sv.vy[slot] = s.vy;
+    sv.vz[slot] = s.vz;
+    sv.key[slot] = key;
+}
+
+__device__ __forceinline__ void count_events(int ev, unsigned long long *__restrict__ events) {
+    __shared__ uint32_t se[8];
+    if (threadIdx.x < 8) se[threadIdx.x] = 0;
+    __syncthreads();
+    if (EV_KIND(ev) > 0) atomicAdd(&se[EV_KIND(ev)], 1u);
+    if (ev & EV_OVER_BOUND) atomicAdd(&se[6], 1u);
+    __syncthreads();
+    if (threadIdx.x < 8 && se[threadIdx.x]) atomicAdd(&events[threadIdx.x], (unsigned long long)se[threadIdx.x]);
+}
+
 __global__ void __launch_bounds__(256)
     k_push_scatter(SpView src, SpView dst, const uint32_t *__restrict__ nptr, uint32_t cap,
-                   const float2 *__restrict__ E, PushConst pc, uint32_t *__restrict__ binCount,
-                   const uint32_t *__restrict__ binStart, unsigned long long *__restrict__ q,
-                   int do_push, int deposit) {
+                   const float2 *__restrict__ E, PushConst pc, mcc::MccConst mc, uint32_t step,
+                   uint32_t *__restrict__ binCount, const uint32_t *__restrict__ binStart,
+                   unsigned long long *__restrict__ q, int do_push, int deposit, Spawn sp,
+                   unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6) {
     uint32_t n = *nptr;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = i < n;
-    PState s{};
+    PState s{}, pre{};
     int ojx = 0, ojy = 0;
-    int st = -1;
-    uint32_t key = 0;
+    int st = -1, ev = mcc::EV_NONE;
+    mcc::Child ch{};
+    uint32_t key = 0, pid = 0;
     if (valid) {
         s = load_state(src, i, true);
+        pid = src.id[i];
         ojx = s.jx;
         ojy = s.jy;
-        if (do_push) push_one(s, E, pc);
-        st = classify(s, pc.N, pc.mode);
+        pre = s;
+        st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch);
         if (st == ST_KEPT) key = bin_key(s.jx, s.jy, pc.tilesX);
     }
     const bool kept = (st == ST_KEPT);
-    uint32_t pid = valid ? src.id[i] : 0u;
     Claim claim = wave_claim_issue(binCount, binStart, key, kept);
     // the deposit runs while the claim's atomic is in flight
     if (deposit) block_deposit(kept && deposit_ok(s, pc.N, pc.mode), s, valid, ojx, ojy, q, pc.N);
     uint32_t slot = wave_claim_slot(claim, kept);
     // the bound test only keeps a corrupted histogram from writing out of range
     if (kept && slot < cap) store_state(dst, slot, s, pid);
+    if (mc.on) count_events(ev, events);
+    if (EV_KIND(ev) == mcc::EV_IONIZE && mc.mode == PIC_DEV_MODE_PHYS) {
+        // ejected electron: born at the parent's position, pushed this step
+        // as the reference pushes particles appended before move_particles
+        PState ce = pre;
+        ce.vx = ch.vx;
+        ce.vy = ch.vy;
+        ce.vz = ch.vz;
+        if (do_push) push_one(ce, E, pc);
+        int cst = classify(ce, pc.N, pc.mode);
+        if (cst != ST_KEPT) {
+            // left the domain during its first push: it still takes its id
+            // (as it does when created before the push), but is not stored
+            atomicAdd(&lost6[cst - 1], 1u);
+            ce.jx = ce.jy = CELL_DEAD;
+        }
+        stage_put(sp.se, &sp.ce->stage_count, &sp.ce->overflow, ce, (unsigned long long)pid);
+        // new ion at the same place; the ion pass of this step moves it
+        PState ci = pre;
+        ci.vx = ch.ivx;
+        ci.vy = ch.ivy;
+        ci.vz = ch.ivz;
+        stage_put(sp.si, &sp.ci->stage_count, &sp.ci->overflow, ci, (unsigned long long)pid);
+    }
+    if (sp.plop && valid && st >= ST_LEFT && st <= ST_TOP) {
+        PState pe, pi;
+        mcc::plop_pair(mc, pid, step, pc.N, sp.se_sigma, sp.si_sigma, sp.plop_ref, pe, pi);
+        unsigned long long k = (1ull << 32) | pid;
+        stage_put(sp.se, &sp.ce->stage_count, &sp.ce->overflow, pe, k);
+        stage_put(sp.si, &sp.ci->stage_count, &sp.ci->overflow, pi, k);
+    }
 }
 
 // ------------------------------------------------------- per-phase path
@@ -399,22 +513,66 @@ __global__ void __launch_bounds__(256)
     block_deposit(dep, s, valid, s.jx, s.jy, q, N);
 }
 
+// e_collisions::collision / ion_collisions::collision as a phase of its own
+// (e_collisions.cpp:480-597, ion_collisions.cpp:255-308): velocities are
+// updated in place; created particles are staged at the parent's position.
+__global__ void __launch_bounds__(256)
+    k_mcc(SpView sp, const uint32_t *__restrict__ nptr, PushConst pc, mcc::MccConst mc,
+          uint32_t step, Spawn spn, unsigned long long *__restrict__ events) {
+    uint32_t n = *nptr;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int ev = mcc::EV_NONE;
+    mcc::Child ch{};
+    PState s{};
+    uint32_t pid = 0;
+    if (i < n) {
+        s = load_state(sp, i, true);
+        pid = sp.id[i];
+        if (mc.electron)
+            ev = mcc::collide_electron(mc, s.vx, s.vy, s.vz, pid, step, ref_mcc_inside(s, pc.N), ch);
+        else
+            ev = mcc::collide_ion(mc, s.vx, s.vy, s.vz, pid, step);
+        sp.vx[i] = s.vx;
+        sp.vy[i] = s.vy;
+        sp.vz[i] = s.vz;
+        if (EV_KIND(ev) == mcc::EV_IONIZE && mc.mode == PIC_DEV_MODE_REF)
+            sp.cell[i] = cell_pack(CELL_DEAD, CELL_DEAD);
+    }
+    count_events(ev, events);
+    if (EV_KIND(ev) == mcc::EV_IONIZE && mc.mode == PIC_DEV_MODE_PHYS) {
+        PState ce = s, ci = s;
+        ce.vx = ch.vx;
+        ce.vy = ch.vy;
+        ce.vz = ch.vz;
+        ci.vx = ch.ivx;
+        ci.vy = ch.ivy;
+        ci.vz = ch.ivz;
+        stage_put(spn.se, &spn.ce->stage_count, &spn.ce->overflow, ce, (unsigned long long)pid);
+        stage_put(spn.si, &spn.ci->stage_count, &spn.ci->overflow, ci, (unsigned long long)pid);
+    }
+}
+
 // wall pass with stream compaction into the other buffer (boundary.cpp:581-677,
-// 765-824 with zero yields).  Survivor order is not preserved.
+// 765-824 with zero yields).  Survivor order is not preserved.  With
+// spn.plop, every ion leaving through a wall stages a re-injected pair.
 __global__ void __launch_bounds__(256)
     k_absorb(SpView src, SpView dst, const uint32_t *__restrict__ nptr, uint32_t *__restrict__ nout,
-             int N, int mode, uint32_t *__restrict__ lost4) {
-    __shared__ uint32_t sl[4];
+             int N, int mode, uint32_t *__restrict__ lost6, mcc::MccConst mc, uint32_t step,
+             Spawn spn) {
+    __shared__ uint32_t sl[6];
     __shared__ uint32_t wbase[4], wcnt[4], bbase;
-    if (threadIdx.x < 4) sl[threadIdx.x] = 0;
+    if (threadIdx.x < 6) sl[threadIdx.x] = 0;
     __syncthreads();
     uint32_t n = *nptr;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool keep = false;
+    int st = -1;
     PState s{};
+    uint32_t pid = 0;
     if (i < n) {
         s = load_state(src, i, true);
-        int st = classify(s, N, mode);
+        pid = src.id[i];
+        st = classify(s, N, mode);
         keep = (st == ST_KEPT);
         if (!keep) atomicAdd(&sl[st - 1], 1u);
     }
@@ -433,8 +591,104 @@ __global__ void __launch_bounds__(256)
         bbase = t ? atomicAdd(nout, t) : 0u;
     }
     __syncthreads();
-    if (keep) store_state(dst, bbase + wbase[w] + rank, s, src.id[i]);
-    if (threadIdx.x < 4 && sl[threadIdx.x]) atomicAdd(&lost4[threadIdx.x], sl[threadIdx.x]);
+    if (keep) store_state(dst, bbase + wbase[w] + rank, s, pid);
+    if (threadIdx.x < 6 && sl[threadIdx.x]) atomicAdd(&lost6[threadIdx.x], sl[threadIdx.x]);
+    if (spn.plop && st >= ST_LEFT && st <= ST_TOP) {
+        PState pe, pi;
+        mcc::plop_pair(mc, pid, step, N, spn.se_sigma, spn.si_sigma, spn.plop_ref, pe, pi);
+        unsigned long long k = (1ull << 32) | pid;
+        stage_put(spn.se, &spn.ce->stage_count, &spn.ce->overflow, pe, k);
+        stage_put(spn.si, &spn.ci->stage_count, &spn.ci->overflow, pi, k);
+    }
+}
+
+// Commit the staged particles of one species: bitonic sort of (key, index)
+// in one workgroup, then append in key order to dst with consecutive new
+// ids, optionally depositing their charge.  Sorting makes the result
+// independent of the order in which the staging atomics fired.
+__global__ void __launch_bounds__(1024)
+    k_commit(StageView sv, SpCounters *__restrict__ scnt, SpView dst, uint32_t *__restrict__ n_dst,
+             uint32_t cap_dst, SpCounters *__restrict__ dcnt, int deposit, int N, int mode,
+             unsigned long long *__restrict__ q) {
+    __shared__ uint32_t s_c, s_base, s_nid;
+    if (threadIdx.x == 0) {
+        uint32_t c = scnt->stage_count;
+        s_c = c < sv.cap ? c : sv.cap;
+        s_base = *n_dst;
+        s_nid = dcnt->next_id;
+    }
+    __syncthreads();
+    const uint32_t c = s_c;
+    if (c == 0) return;
+    uint32_t P = 1;
+    while (P < c) P <<= 1;
+    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+        sv.idx[t] = t;
+        if (t >= c) sv.key[t] = ~0ull;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+                uint32_t o = t ^ j;
+                if (o > t) {
+                    unsigned long long a = sv.key[t], b = sv.key[o];
+                    bool up = (t & k) == 0;
+                    if ((a > b) == up) {
+                        sv.key[t] = b;
+                        sv.key[o] = a;
+                        uint32_t x = sv.idx[t];
+                        sv.idx[t] = sv.idx[o];
+                        sv.idx[o] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // entries staged as CELL_DEAD take an id but are not stored: exclusive
+    // scan of the stored flags gives each stored entry its slot
+    __shared__ uint32_t chunk_total;
+    const uint32_t base = s_base, nid = s_nid;
+    const uint32_t dead = cell_pack(CELL_DEAD, CELL_DEAD);
+    uint32_t stored_before = 0;
+    for (uint32_t r0 = 0; r0 < c; r0 += blockDim.x) {
+        uint32_t r = r0 + threadIdx.x;
+        bool live = r < c && sv.cell[sv.idx[r]] != dead;
+        uint32_t tot;
+        uint32_t ex = block_excl_scan_1024(live ? 1u : 0u, &tot);
+        if (threadIdx.x == 0) chunk_total = tot;
+        __syncthreads();
+        if (live) {
+            uint32_t k = sv.idx[r];
+            uint32_t slot = base + stored_before + ex;
+            if (slot < cap_dst) {
+                PState p;
+                uint32_t cc = sv.cell[k];
+                p.jx = cell_jx(cc);
+                p.jy = cell_jy(cc);
+                p.ox = sv.ox[k];
+                p.oy = sv.oy[k];
+                p.vx = sv.vx[k];
+                p.vy = sv.vy[k];
+                p.vz = sv.vz[k];
+                store_state(dst, slot, p, nid + r);
+                if (deposit && deposit_ok(p, N, mode))
+                    dep_global(q, N, p.jx, p.jy, fx_weights(p.ox, p.oy));
+            }
+        }
+        stored_before += chunk_total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        uint32_t room = base < cap_dst ? cap_dst - base : 0u;
+        uint32_t written = stored_before < room ? stored_before : room;
+        *n_dst = base + written;
+        dcnt->next_id = nid + c;
+        uint32_t dropped = (stored_before - written) + (scnt->stage_count - c);
+        if (dropped) dcnt->overflow += dropped;
+        scnt->stage_count = 0;
+    }
 }
 
 // ------------------------------------------------------------ loader
@@ -475,18 +729,34 @@ static inline unsigned blocks_for(int64_t n, int t) {
 }
 
 void launch_push_count(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
-                       const float2 *E, const PushConst &pc, uint32_t *binCount, uint32_t *lost4,
-                       int do_push) {
-    hipLaunchKernelGGL(k_push_count, dim3(blocks_for(cap, 256)), dim3(256), 0, st, sp, n, E, pc,
-                       binCount, lost4, do_push);
+                       const float2 *E, const PushConst &pc, const mcc::MccConst &mc, uint32_t step,
+                       uint32_t *binCount, uint32_t *lost6, int do_push) {
+    hipLaunchKernelGGL(k_push_count, dim3(blocks_for(cap, 256)), dim3(256), 0, st, sp, n, E, pc, mc,
+                       step, binCount, lost6, do_push);
 }
 
 void launch_push_scatter(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
-                         int64_t cap, const float2 *E, const PushConst &pc, uint32_t *binCount,
-                         const uint32_t *binStart, unsigned long long *q, int do_push,
-                         int deposit) {
+                         int64_t cap, const float2 *E, const PushConst &pc, const mcc::MccConst &mc,
+                         uint32_t step, uint32_t *binCount, const uint32_t *binStart,
+                         unsigned long long *q, int do_push, int deposit, const Spawn &spn,
+                         unsigned long long *events, uint32_t *lost6) {
     hipLaunchKernelGGL(k_push_scatter, dim3(blocks_for(cap, 256)), dim3(256), 0, st, src, dst, n,
-                       (uint32_t)cap, E, pc, binCount, binStart, q, do_push, deposit);
+                       (uint32_t)cap, E, pc, mc, step, binCount, binStart, q, do_push, deposit, spn,
+                       events, lost6);
+}
+
+void launch_mcc(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                const PushConst &pc, const mcc::MccConst &mc, uint32_t step, const Spawn &spn,
+                unsigned long long *events) {
+    hipLaunchKernelGGL(k_mcc, dim3(blocks_for(cap, 256)), dim3(256), 0, st, sp, n, pc, mc, step,
+                       spn, events);
+}
+
+void launch_commit(hipStream_t st, const StageView &sv, SpCounters *scnt, const SpView &dst,
+                   uint32_t *n_dst, int64_t cap_dst, SpCounters *dcnt, int deposit, int N, int mode,
+                   unsigned long long *q) {
+    hipLaunchKernelGGL(k_commit, dim3(1), dim3(1024), 0, st, sv, scnt, dst, n_dst,
+                       (uint32_t)cap_dst, dcnt, deposit, N, mode, q);
 }
 
 void launch_gather(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
@@ -506,9 +776,10 @@ void launch_deposit(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t
 }
 
 void launch_absorb(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
-                   uint32_t *nout, int64_t cap, int N, int mode, uint32_t *lost4) {
+                   uint32_t *nout, int64_t cap, int N, int mode, uint32_t *lost6,
+                   const mcc::MccConst &mc, uint32_t step, const Spawn &spn) {
     hipLaunchKernelGGL(k_absorb, dim3(blocks_for(cap, 256)), dim3(256), 0, st, src, dst, n, nout,
-                       N, mode, lost4);
+                       N, mode, lost6, mc, step, spn);
 }
 
 void launch_load_random(hipStream_t st, const SpView &sp, uint32_t n, uint64_t seed,

@@ -1,0 +1,454 @@
+// mcc.h -- Monte-Carlo collisions: cross sections and per-particle collision
+// kernels (device), shared with the host for the null-collision bound.
+//
+// Cross sections are in units of 1e-20 m^2 as the reference uses them
+// (e_collisions.cpp:533,551 and ion_collisions.cpp:297 multiply by 1e-20).
+// Energies in eV.
+//
+// Two rule sets:
+//  REF   -- the reference's decision rules (e_collisions.cpp:480-597): one
+//           uniform R, elastic iff P_el > 2R, ionisation iff P_iz > 0 (which
+//           in the reference deletes the parent: its excess-energy formula
+//           mixes joules and eV and yields NaN, e_collisions.cpp:377-381);
+//           scattering angle and azimuth from one shared uniform
+//           (collisions.cpp:118-133); neutral velocity from a 10000-entry
+//           table (collisions.cpp:66-89, 135-139).
+//  PHYS  -- null-collision Monte Carlo (Vahedi & Surendra, Comput. Phys.
+//           Commun. 87 (1995) 179): a candidate fraction 1-exp(-nu' dt) of
+//           particles, channel chosen by nu_k(E)/nu'; the neutral partner is
+//           sampled from a Maxwellian; electrons: elastic, excitation,
+//           ionisation; ions: isotropic elastic and charge exchange.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "pic_dev.h"
+
+#define HD __host__ __device__ inline
+
+namespace mcc {
+
+enum Gas { GAS_AR = 0, GAS_H = 1 };
+
+// -------------------------------------------------------- cross sections
+// e-Ar elastic (e_collisions.cpp:205-224).  REF keeps the reference's exact
+// branch tests (E exactly 0.35 or 15.0 falls in no branch: sigma 0).
+template <typename T>
+HD T ar_el_ref(T E) {
+    if (E < (T)0.35)
+        return (T)28019.0 * E * E * E * E * E * E - (T)32798.0 * E * E * E * E * E +
+               (T)15048.0 * E * E * E * E - (T)34312.0 * E * E * E + (T)4084.0 * E * E -
+               (T)249.0 * E + (T)7.576;
+    if (E < (T)15.0 && E > (T)0.35)
+        return -(T)0.013 * E * E * E + (T)0.232 * E * E + (T)1.066 * E + (T)0.085;
+    if (E > (T)15.0) return (T)126.2 * pow(E, (T)-0.69);
+    return (T)0.0;
+}
+// PHYS: same fit, breakpoints closed, negative values (0.061-0.349 eV) clamped
+template <typename T>
+HD T ar_el_phys(T E) {
+    T s;
+    if (E < (T)0.35)
+        s = (T)28019.0 * E * E * E * E * E * E - (T)32798.0 * E * E * E * E * E +
+            (T)15048.0 * E * E * E * E - (T)34312.0 * E * E * E + (T)4084.0 * E * E -
+            (T)249.0 * E + (T)7.576;
+    else if (E < (T)15.0)
+        s = -(T)0.013 * E * E * E + (T)0.232 * E * E + (T)1.066 * E + (T)0.085;
+    else
+        s = (T)126.2 * pow(E, (T)-0.69);
+    return s > (T)0 ? s : (T)0;
+}
+
+// e-Ar ionisation (e_collisions.cpp:235-256); REF threshold 15.755, PHYS threshold thr
+template <typename T>
+HD T ar_iz_ref(T E) {
+    if (E < (T)15.755) return (T)0.0;
+    if (E < (T)100.0 && E > (T)15.755)
+        return (T)-3.0e-7 * E * E * E * E + (T)8.0e-5 * E * E * E - (T)0.008 * E * E +
+               (T)0.416 * E - (T)4.758;
+    if (E > (T)100.0) return (T)130.0 * pow(E, (T)-0.74);
+    return (T)0.0;
+}
+template <typename T>
+HD T ar_iz_phys(T E, T thr) {
+    if (E < thr) return (T)0.0;
+    T s;
+    if (E < (T)100.0)
+        s = (T)-3.0e-7 * E * E * E * E + (T)8.0e-5 * E * E * E - (T)0.008 * E * E + (T)0.416 * E -
+            (T)4.758;
+    else
+        s = (T)130.0 * pow(E, (T)-0.74);
+    return s > (T)0 ? s : (T)0;
+}
+
+// e-Ar excitation, lumped 11.55 eV level.  The reference has no excitation
+// channel; this Bethe-type shape (peak 1.2e-20 m^2 at e * 11.55 eV) is this
+// library's choice, used only in PHYS mode.
+template <typename T>
+HD T ar_exc_phys(T E, T thr) {
+    if (E <= thr) return (T)0.0;
+    T x = E / thr;
+    return (T)3.262 * log(x) / x;
+}
+
+// e-H elastic (e_collisions.cpp:267-283)
+template <typename T>
+HD T h_el(T E) {
+    if (E < (T)1.0e-2) return (T)3.834 * E * E - (T)20.82 * E + (T)40.98;
+    if (E < (T)20.0) return (T)-6.15 * log(E) + (T)20.91;
+    return (T)102.2 * pow(E, (T)-1.19);
+}
+// e-H ionisation, Janev (e_collisions.cpp:293-308).  The bracket/(I E) is in
+// units of 1e-17 m^2; REF uses the number as if it were 1e-20 m^2 (as the
+// reference does), PHYS converts (x 1000).
+template <typename T>
+HD T h_iz_janev(T E) {
+    const T I = (T)13.6;
+    if (E < I) return (T)0.0;
+    T c = (T)1.0 - I / E;
+    return ((T)1.0 / (I * E)) * ((T)0.18450 * log(E / I) + (T)-0.032226 * c +
+                                 (T)-0.034539 * c * c + (T)1.4003 * c * c * c +
+                                 (T)-2.8115 * c * c * c * c + (T)2.2986 * c * c * c * c * c);
+}
+
+// Ar+ - Ar charge exchange (ion_collisions.cpp:170-176)
+template <typename T>
+HD T ar_cx(T E) {
+    if (E > (T)4.0) return ((T)2.0 + (T)5.5 / sqrt(E)) * (T)10.0;
+    return ((T)-2.95 * sqrt(E) + (T)10.65) * (T)10.0;
+}
+// Ar+ - Ar isotropic elastic (absent from the reference; PHYS only):
+// Phelps, J. Appl. Phys. 76 (1994) 747: 2e-19/(sqrt(e)(1+e)) + 3e-19 e/(1+e/3)^2 m^2
+template <typename T>
+HD T ar_iso_phys(T E) {
+    T e = E > (T)1e-8 ? E : (T)1e-8;
+    T d = (T)1.0 + e / (T)3.0;
+    return (T)20.0 / (sqrt(e) * ((T)1.0 + e)) + (T)30.0 * e / (d * d);
+}
+// H+ - H charge exchange, Janev (ion_collisions.cpp:185-199) in 1e-20 m^2
+// (the fit's 1e-16 cm^2).  E_kev_amu = lab energy / 1000 / mass in amu; the
+// reference divides by AMU in kilograms instead, which makes sigma vanish
+// (REF reproduces that through kev_amu_div).
+template <typename T>
+HD T h_cx_janev(T E_kev_amu) {
+    if (E_kev_amu < (T)1.0e-4) return (T)0.0;
+    T e = E_kev_amu;
+    return ((T)3.2345 * log((T)235.88 / e + (T)2.3713)) /
+           ((T)1.0 + (T)0.038371 * e + (T)3.8068e-6 * pow(e, (T)3.5) + (T)1.1832e-10 * pow(e, (T)5.4));
+}
+
+// --------------------------------------------------------- parameters
+struct MccConst {
+    int on;         // collisions enabled for this species
+    int mode;       // PIC_DEV_MODE_REF / PIC_DEV_MODE_PHYS
+    int gas;        // GAS_AR / GAS_H
+    int electron;   // 1 electrons, 0 ions
+    int sig_el_energy, sig_iz_energy, sig_cx_energy;  // REF: energy-dependent vs constant
+    int N;          // nodes (REF in-domain test)
+    float sig_el_const, sig_iz_const, sig_cx_const;    // REF constants (1e-20 m^2)
+    float ng;       // neutral density [m^-3]
+    float dt;       // species time step
+    float mass;     // species mass [kg]
+    float M;        // neutral (ion) mass [kg]
+    float q;        // elementary charge [C]
+    float vtn;      // neutral Maxwellian sigma per component [m/s] (PHYS)
+    float nu_max;   // null-collision bound nu' [1/s] (PHYS)
+    float p_null;   // 1 - exp(-nu' dt) (PHYS)
+    float e_exc, e_iz;  // thresholds [eV] (PHYS)
+    float opal_B;   // secondary-energy parameter [eV] (PHYS)
+    float kev_amu_div;  // H+ CX energy divisor (eV -> keV/amu)
+    const float *ntab;  // REF neutral velocity table (3 x 10000, SoA)
+    int ntab_n;
+    uint32_t key0, key1;
+};
+
+// event codes (counters)
+enum Event { EV_NONE = 0, EV_ELASTIC = 1, EV_EXCITE = 2, EV_IONIZE = 3, EV_CX = 4, EV_ISO = 5 };
+// flag or-ed into the returned event when the particle's total collision
+// frequency exceeded the null-collision bound (its probability was clipped)
+#define EV_OVER_BOUND 0x100
+#define EV_KIND(e) ((e) & 0xFF)
+
+// total electron collision frequency at relative energy E and speed g (PHYS)
+template <typename T>
+HD void e_nu_parts(const MccConst &c, T E, T g, T &nu_el, T &nu_ex, T &nu_iz) {
+    T f = (T)c.ng * (T)1e-20 * g;
+    if (c.gas == GAS_AR) {
+        nu_el = f * ar_el_phys(E);
+        nu_ex = f * ar_exc_phys(E, (T)c.e_exc);
+        nu_iz = f * ar_iz_phys(E, (T)c.e_iz);
+    } else {
+        T s = h_el(E);
+        nu_el = f * (s > (T)0 ? s : (T)0);
+        nu_ex = (T)0;
+        nu_iz = f * (T)1000.0 * h_iz_janev(E);
+    }
+}
+
+template <typename T>
+HD void i_nu_parts(const MccConst &c, T E, T g, T &nu_iso, T &nu_cx) {
+    T f = (T)c.ng * (T)1e-20 * g;
+    if (c.gas == GAS_AR) {
+        nu_iso = f * ar_iso_phys(E);
+        nu_cx = f * ar_cx(E);
+    } else {
+        nu_iso = (T)0;
+        nu_cx = f * h_cx_janev(E / (T)1000.0 / (T)c.kev_amu_div);
+    }
+}
+
+// ------------------------------------------------------- device helpers
+
+// polar/azimuthal rotation of vector g by (chi, phi) (Nanbu, IEEE TPS 28
+// (2000) 971, eq. 56; the same construction as collisions.cpp:141-169)
+__device__ __forceinline__ void rotate(float gx, float gy, float gz, float cosc, float sinc,
+                                       float cosp, float sinp, float &ox, float &oy, float &oz) {
+    float g = sqrtf(gx * gx + gy * gy + gz * gz);
+    float gp = sqrtf(gy * gy + gz * gz);
+    if (gp < 1e-6f * g || gp == 0.0f) {  // g along x: any perpendicular pair
+        ox = gx * cosc;
+        oy = g * sinc * cosp;
+        oz = g * sinc * sinp;
+        return;
+    }
+    ox = gx * cosc + gp * sinc * cosp;
+    oy = gy * cosc - (gx * gy * cosp - g * gz * sinp) / gp * sinc;
+    oz = gz * cosc - (gx * gz * cosp + g * gy * sinp) / gp * sinc;
+}
+
+// anisotropic scattering angle (Vahedi & Surendra eq. 13; collisions.cpp:121-124)
+__device__ __forceinline__ float cos_chi(float E, float R) {
+    if (E < 1e-4f) return 1.0f - 2.0f * R;
+    float c = (2.0f + E - 2.0f * powf(1.0f + E, R)) / E;
+    return fminf(1.0f, fmaxf(-1.0f, c));
+}
+
+// fourth Philox counter word of collision draws (the loader uses 0)
+#define RNG_DOMAIN_MCC 0x4D434331u
+
+__device__ __forceinline__ void normals3(uint32_t id, uint32_t step, uint32_t stream,
+                                         const MccConst &c, float &a, float &b, float &d) {
+    uint32_t r[4];
+    philox4x32_10(id, step, stream, RNG_DOMAIN_MCC, c.key0, c.key1, r);
+    float r1 = sqrtf(-2.0f * logf(u01(r[0]))), t1 = 6.2831853f * u01(r[1]);
+    float r2 = sqrtf(-2.0f * logf(u01(r[2]))), t2 = 6.2831853f * u01(r[3]);
+    a = r1 * cosf(t1);
+    b = r1 * sinf(t1);
+    d = r2 * cosf(t2);
+}
+
+// Children produced by one collision (PHYS ionisation).
+struct Child {
+    float vx, vy, vz;     // new electron velocity
+    float ivx, ivy, ivz;  // new ion velocity
+};
+
+// RNG streams (third counter word = species * 16 + stream)
+#define RS_MAIN 0
+#define RS_NEUTRAL 1
+#define RS_IONIZE 2
+#define RS_PLOP 3
+#define RS_PLOP2 4
+
+// Electron collision.  Returns the event; on EV_IONIZE in PHYS mode `ch`
+// holds the ejected electron and the new ion; in REF mode EV_IONIZE means the
+// parent is removed.  inside: the particle passes the reference's
+// "inside the domain" test (e_collisions.cpp:507-511), used in REF mode.
+__device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, float &vy, float &vz,
+                                                uint32_t id, uint32_t step, bool inside, Child &ch) {
+    const uint32_t s0 = (c.electron ? 0u : 16u);
+    uint32_t r[4];
+    philox4x32_10(id, step, s0 + RS_MAIN, RNG_DOMAIN_MCC, c.key0, c.key1, r);
+    if (c.mode == PIC_DEV_MODE_REF) {
+        if (!inside) return EV_NONE;
+        float v2 = vx * vx + vy * vy + vz * vz;
+        float E = (0.5f / c.q) * c.mass * v2;
+        float u = sqrtf(2.0f * c.q * E / c.mass);
+        float sel = c.sig_el_energy ? (c.gas == GAS_AR ? ar_el_ref(E) : h_el(E)) : c.sig_el_const;
+        float siz = c.sig_iz_energy ? (c.gas == GAS_AR ? ar_iz_ref(E) : h_iz_janev(E)) : c.sig_iz_const;
+        float R = u01(r[0]);
+        float pel = 1.0f - expf(-(u * c.dt * 1.0e-20f * c.ng * sel));
+        int ev = EV_NONE;
+        if (pel > 2.0f * R) {
+            // collisions.cpp:107-171 with coll_factor = M/(m+M)
+            float Rs = u01(r[1]);
+            float cc = (E < 1.0e-2f) ? sqrtf(1.0f - Rs)
+                                     : 2.0f * (1.0f - powf(1.0f + E, Rs)) / E + 1.0f;
+            cc = fminf(1.0f, cc);
+            float sc = sqrtf(fmaxf(0.0f, 1.0f - cc * cc));
+            float cp = cosf(6.2831853f * Rs), sp = sinf(6.2831853f * Rs);
+            int idx = (int)(u01(r[2]) * (float)(c.ntab_n - 1));
+            idx = idx < 0 ? 0 : (idx > c.ntab_n - 2 ? c.ntab_n - 2 : idx);
+            float nvx = c.ntab[idx], nvy = c.ntab[c.ntab_n + idx], nvz = c.ntab[2 * c.ntab_n + idx];
+            float ux = vx - nvx, uy = vy - nvy, uz = vz - nvz;
+            float uperp = sqrtf(uy * uy + uz * uz);
+            float um = sqrtf(ux * ux + uperp * uperp);
+            float uyp = 1.0f, uzp = 1.0f;
+            if (!(uperp < 1.0f)) {
+                uyp = uy / uperp;
+                uzp = uz / uperp;
+            }
+            float cf = c.M / (c.mass + c.M);
+            vz -= cf * (uz * (1.0f - cc) - (ux * uzp * cp - um * uyp * sp) * sc);
+            vy -= cf * (uy * (1.0f - cc) - (ux * uyp * cp + um * uzp * sp) * sc);
+            vx -= cf * (ux * (1.0f - cc) + uperp * cp * sc);
+            ev = EV_ELASTIC;
+        }
+        float piz = 1.0f - expf(-(u * c.dt * 1.0e-20f * c.ng * siz));
+        if (piz > 0.0f) ev = EV_IONIZE;
+        return ev;
+    }
+    // PHYS null collision
+    if (!(u01(r[0]) < c.p_null)) return EV_NONE;
+    float nx, ny, nz;
+    normals3(id, step, s0 + RS_NEUTRAL, c, nx, ny, nz);
+    nx *= c.vtn;
+    ny *= c.vtn;
+    nz *= c.vtn;
+    float gx = vx - nx, gy = vy - ny, gz = vz - nz;
+    float g2 = gx * gx + gy * gy + gz * gz;
+    float g = sqrtf(g2);
+    float E = (0.5f / c.q) * c.mass * g2;
+    float nel, nex, niz;
+    e_nu_parts<float>(c, E, g, nel, nex, niz);
+    const int over = (nel + nex + niz > c.nu_max) ? EV_OVER_BOUND : 0;
+    float x = u01(r[1]) * c.nu_max;
+    float cp = cosf(6.2831853f * u01(r[3])), sp = sinf(6.2831853f * u01(r[3]));
+    if (x < nel) {
+        float cc = cos_chi(E, u01(r[2]));
+        float sc = sqrtf(fmaxf(0.0f, 1.0f - cc * cc));
+        float hx, hy, hz;
+        rotate(gx, gy, gz, cc, sc, cp, sp, hx, hy, hz);
+        float cf = c.M / (c.mass + c.M);
+        vx -= cf * (gx - hx);
+        vy -= cf * (gy - hy);
+        vz -= cf * (gz - hz);
+        return EV_ELASTIC | over;
+    }
+    if (x < nel + nex) {
+        float Ep = E - c.e_exc;
+        if (!(Ep > 0.0f)) return EV_NONE;
+        float cc = cos_chi(Ep, u01(r[2]));
+        float sc = sqrtf(fmaxf(0.0f, 1.0f - cc * cc));
+        float hx, hy, hz;
+        rotate(gx, gy, gz, cc, sc, cp, sp, hx, hy, hz);
+        float k = sqrtf(2.0f * c.q * Ep / c.mass) / g;
+        vx = nx + hx * k;
+        vy = ny + hy * k;
+        vz = nz + hz * k;
+        return EV_EXCITE | over;
+    }
+    if (x < nel + nex + niz) {
+        float Ex = E - c.e_iz;
+        if (!(Ex > 0.0f)) return EV_NONE;
+        uint32_t q4[4];
+        philox4x32_10(id, step, s0 + RS_IONIZE, RNG_DOMAIN_MCC, c.key0, c.key1, q4);
+        // secondary energy (Opal, Peterson & Beaty 1971, as in Vahedi & Surendra)
+        float Eej = c.opal_B * tanf(u01(q4[0]) * atanf(Ex / (2.0f * c.opal_B)));
+        float Esc = Ex - Eej;
+        float cc = cos_chi(Esc, u01(r[2]));
+        float sc = sqrtf(fmaxf(0.0f, 1.0f - cc * cc));
+        float hx, hy, hz;
+        rotate(gx, gy, gz, cc, sc, cp, sp, hx, hy, hz);
+        float k = sqrtf(2.0f * c.q * Esc / c.mass) / g;
+        vx = nx + hx * k;
+        vy = ny + hy * k;
+        vz = nz + hz * k;
+        float cc2 = cos_chi(Eej, u01(q4[1]));
+        float sc2 = sqrtf(fmaxf(0.0f, 1.0f - cc2 * cc2));
+        float ph2 = 6.2831853f * u01(q4[2]);
+        rotate(gx, gy, gz, cc2, sc2, cosf(ph2), sinf(ph2), hx, hy, hz);
+        float k2 = sqrtf(2.0f * c.q * Eej / c.mass) / g;
+        ch.vx = nx + hx * k2;
+        ch.vy = ny + hy * k2;
+        ch.vz = nz + hz * k2;
+        ch.ivx = nx;
+        ch.ivy = ny;
+        ch.ivz = nz;
+        return EV_IONIZE | over;
+    }
+    return EV_NONE | over;
+}
+
+// Ion collision (PHYS: isotropic elastic + charge exchange against a
+// Maxwellian neutral).  REF ion charge exchange is a per-column sequential
+// accumulator and is not handled here.
+__device__ __forceinline__ int collide_ion(const MccConst &c, float &vx, float &vy, float &vz,
+                                           uint32_t id, uint32_t step) {
+    const uint32_t s0 = 16u;
+    uint32_t r[4];
+    philox4x32_10(id, step, s0 + RS_MAIN, RNG_DOMAIN_MCC, c.key0, c.key1, r);
+    if (!(u01(r[0]) < c.p_null)) return EV_NONE;
+    float nx, ny, nz;
+    normals3(id, step, s0 + RS_NEUTRAL, c, nx, ny, nz);
+    nx *= c.vtn;
+    ny *= c.vtn;
+    nz *= c.vtn;
+    float gx = vx - nx, gy = vy - ny, gz = vz - nz;
+    float g2 = gx * gx + gy * gy + gz * gz;
+    float g = sqrtf(g2);
+    float E = (0.5f / c.q) * c.mass * g2;
+    float niso, ncx;
+    i_nu_parts<float>(c, E, g, niso, ncx);
+    const int over = (niso + ncx > c.nu_max) ? EV_OVER_BOUND : 0;
+    float x = u01(r[1]) * c.nu_max;
+    if (x < niso) {
+        // equal masses: centre-of-mass velocity plus half the relative
+        // velocity turned to an isotropic direction
+        float cc = 1.0f - 2.0f * u01(r[2]);
+        float sc = sqrtf(fmaxf(0.0f, 1.0f - cc * cc));
+        float ph = 6.2831853f * u01(r[3]);
+        float cmx = 0.5f * (vx + nx), cmy = 0.5f * (vy + ny), cmz = 0.5f * (vz + nz);
+        vx = cmx + 0.5f * g * sc * cosf(ph);
+        vy = cmy + 0.5f * g * sc * sinf(ph);
+        vz = cmz + 0.5f * g * cc;
+        return EV_ISO | over;
+    }
+    if (x < niso + ncx) {
+        vx = nx;
+        vy = ny;
+        vz = nz;
+        return EV_CX | over;
+    }
+    return EV_NONE | over;
+}
+
+// Position uniform in [dx, L-dx]^2 (boundary.cpp:860-866) and Maxwellian
+// velocities with per-component sigma se / si for a re-injected electron-ion
+// pair, drawn from the lost ion's stream.  REF uses the reference's sampler
+// v = vt cos(2 pi R2) sqrt|ln R1| (boundary.cpp:833-845), sigma = vt/sqrt 2.
+__device__ __forceinline__ void plop_pair(const MccConst &c, uint32_t id, uint32_t step, int N,
+                                          float se, float si, int ref, PState &e, PState &i) {
+    uint32_t r[4], a[4], b[4];
+    philox4x32_10(id, step, 16u + RS_PLOP, RNG_DOMAIN_MCC, c.key0, c.key1, r);
+    philox4x32_10(id, step, 16u + RS_PLOP2, RNG_DOMAIN_MCC, c.key0, c.key1, a);
+    philox4x32_10(id, step, 16u + RS_PLOP2 + 1u, RNG_DOMAIN_MCC, c.key0, c.key1, b);
+    float span = (float)(N - 3);
+    float tx = 1.0f + u01(r[0]) * span, ty = 1.0f + u01(r[1]) * span;
+    int jx = (int)floorf(tx), jy = (int)floorf(ty);
+    float ox = tx - (float)jx, oy = ty - (float)jy;
+    if (ox >= 1.0f) { ox = 0.0f; jx += 1; }
+    if (oy >= 1.0f) { oy = 0.0f; jy += 1; }
+    e.jx = i.jx = jx;
+    e.jy = i.jy = jy;
+    e.ox = i.ox = ox;
+    e.oy = i.oy = oy;
+    float n[6];
+    uint32_t w[6] = {a[0], a[1], a[2], a[3], b[0], b[1]};
+    uint32_t w2[6] = {b[2], b[3], r[2], r[3], a[1] ^ b[2], a[3] ^ b[0]};
+    for (int k = 0; k < 6; k++) {
+        float R1 = u01(w[k]), R2 = u01(w2[k]);
+        n[k] = ref ? cosf(6.2831853f * R2) * sqrtf(fabsf(logf(R1)))
+                   : sqrtf(-2.0f * logf(R1)) * cosf(6.2831853f * R2);
+    }
+    e.vx = se * n[0];
+    e.vy = se * n[1];
+    e.vz = se * n[2];
+    i.vx = si * n[3];
+    i.vy = si * n[4];
+    i.vz = si * n[5];
+}
+
+}  // namespace mcc

@@ -22,6 +22,10 @@
 #define ST_RIGHT 2
 #define ST_BOTTOM 3
 #define ST_TOP 4
+#define ST_DEAD 5  // removed by a collision (reference ionisation rule)
+
+// cell index of a particle a collision removed (outside every wall)
+#define CELL_DEAD (-16001)
 
 #define POS_GUARD 8192.0f
 #define CELL_FAR 16000
@@ -139,6 +143,7 @@ __device__ __forceinline__ bool push_one(PState &s, const float2 *__restrict__ E
 // Wall test after the push.  REF: boundary.cpp:595-599 / 774-778, lost iff
 // (int)(x/dx) < 0 or >= N, i.e. x <= -dx or x >= N*dx.  PHYS: x < 0 or x >= L.
 __device__ __forceinline__ int classify(const PState &s, int N, int mode) {
+    if (s.jx == CELL_DEAD) return ST_DEAD;
     bool lx, ly;
     if (mode == PIC_DEV_MODE_PHYS) {
         lx = (s.jx < 0) || (s.jx > N - 2);

@@ -23,6 +23,8 @@ struct Species {
     double q_over_m = 0, qc = 0, mass = 0, dt = 0, weight = 1.0, sigma_load = 0;
     double scale = 0;  // qc/(dx*dy) * weight * 2^-40
     PushConst pc = {};
+    mcc::MccConst mc = {};
+    pic::StageView stage = {};  // particles created during a step
     const char *type = "";  // dataDump suffix (parameters.h:316-318)
 };
 
@@ -46,6 +48,8 @@ struct pic_handle {
     uint32_t *binCount = nullptr, *binStart = nullptr, *scanPartial = nullptr;
     double *rho = nullptr, *phi = nullptr, *ex64 = nullptr, *ey64 = nullptr;
     float2 *E32 = nullptr;
+    float *ntab = nullptr;       // reference neutral velocity table (collisions.cpp:66-89)
+    bool spawning = false;       // collisions or re-injection create particles
     pic::MgLevels M;
     double last_phiR = 0.0;
     int mg_cycles = 0;          // V-cycles (tolerance mode) of the last solve

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mcc.h"
 #include "pic_dev.h"
 
 namespace pic {
@@ -18,10 +19,24 @@ struct SpView {
 // Device counters of one species.  n[b] is the particle count of buffer b.
 struct SpCounters {
     uint32_t n[2];
-    uint32_t lost[4];  // left, right, bottom, top (accumulated)
-    uint32_t next_id;
-    uint32_t overflow;
-    uint32_t pad[8];
+    uint32_t lost[6];        // left, right, bottom, top, removed by collision, -
+    uint32_t next_id;        // id of the next particle created
+    uint32_t overflow;       // particles dropped for lack of capacity
+    uint32_t stage_count;    // entries in the staging buffer
+    uint32_t pad;
+    unsigned long long events[8];  // collision events by mcc::Event
+};
+
+// Particles created during a step (ionisation products, re-injected pairs)
+// wait in a staging buffer; a commit pass sorts them by their creator's key
+// (so the layout and the new ids are deterministic), appends them to a
+// particle buffer and optionally deposits their charge.
+struct StageView {
+    uint32_t *cell;
+    float *ox, *oy, *vx, *vy, *vz;
+    unsigned long long *key;  // creator key (source << 32 | creator id)
+    uint32_t *idx;            // sort permutation
+    uint32_t cap;             // entries (power of two)
 };
 
 // Multigrid level storage on the device.  Level l has (2^(l+1)+1)^2 nodes;
@@ -47,13 +62,30 @@ struct MgLevels {
     int pred = 4;                // predicted V-cycles per solve
 };
 
+// where created particles go during a pass (see k_particles.hip)
+struct Spawn {
+    StageView se, si;
+    SpCounters *ce, *ci;
+    int plop;
+    float se_sigma, si_sigma;
+    int plop_ref;
+};
+
 // ---- k_particles.hip
 void launch_push_count(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
-                       const float2 *E, const PushConst &pc, uint32_t *binCount, uint32_t *lost4,
-                       int do_push);
+                       const float2 *E, const PushConst &pc, const mcc::MccConst &mc, uint32_t step,
+                       uint32_t *binCount, uint32_t *lost6, int do_push);
 void launch_push_scatter(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
-                         int64_t cap, const float2 *E, const PushConst &pc, uint32_t *binCount,
-                         const uint32_t *binStart, unsigned long long *q, int do_push, int deposit);
+                         int64_t cap, const float2 *E, const PushConst &pc, const mcc::MccConst &mc,
+                         uint32_t step, uint32_t *binCount, const uint32_t *binStart,
+                         unsigned long long *q, int do_push, int deposit, const Spawn &spn,
+                         unsigned long long *events, uint32_t *lost6);
+void launch_mcc(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                const PushConst &pc, const mcc::MccConst &mc, uint32_t step, const Spawn &spn,
+                unsigned long long *events);
+void launch_commit(hipStream_t st, const StageView &sv, SpCounters *scnt, const SpView &dst,
+                   uint32_t *n_dst, int64_t cap_dst, SpCounters *dcnt, int deposit, int N, int mode,
+                   unsigned long long *q);
 void launch_scan(hipStream_t st, const uint32_t *count, uint32_t *start, uint32_t nbins_pad,
                  uint32_t *partial, uint32_t *n_out);
 uint32_t scan_pad(uint32_t nbins);
@@ -64,7 +96,8 @@ void launch_move(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t ca
 void launch_deposit(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap, int N,
                     int mode, unsigned long long *q);
 void launch_absorb(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
-                   uint32_t *nout, int64_t cap, int N, int mode, uint32_t *lost4);
+                   uint32_t *nout, int64_t cap, int N, int mode, uint32_t *lost6,
+                   const mcc::MccConst &mc, uint32_t step, const Spawn &spn);
 void launch_load_random(hipStream_t st, const SpView &sp, uint32_t n, uint64_t seed,
                         uint32_t species, int N, double sigma);
 

@@ -173,6 +173,9 @@ def load_library(path=LIB_PATH):
         "pic_profile_steps": [vp, vp, i32, P(i32)],
         "pic_advance": [vp, i32],
         "pic_poisson_tol": [vp, vp, i32, dbl, dbl, i32, P(i32), P(dbl)],
+        "pic_set_step": [vp, i64],
+        "pic_get_collision_counts": [vp, i32, vp],
+        "pic_get_overflow": [vp, P(i64)],
         "pic_get_mg_stats": [vp, P(i32), P(i64), P(dbl)],
         "pic_sync": [vp],
     }
@@ -385,9 +388,25 @@ class Pic:
         return a.value, b.value, c.value
 
     def loss_counts(self, s):
-        a = np.zeros(4, dtype=np.int64)
+        """[left, right, bottom, top, removed by collision]"""
+        a = np.zeros(5, dtype=np.int64)
         self._c(self.lib.pic_get_loss_counts(self.h, s, _ptr(a)))
         return a
+
+    def collision_counts(self, s):
+        """events by kind: [-, elastic, excitation, ionisation, charge exchange,
+        isotropic ion elastic, -, -]"""
+        a = np.zeros(8, dtype=np.int64)
+        self._c(self.lib.pic_get_collision_counts(self.h, s, _ptr(a)))
+        return a
+
+    def overflow(self):
+        v = C.c_int64()
+        self._c(self.lib.pic_get_overflow(self.h, C.byref(v)))
+        return v.value
+
+    def set_step(self, step):
+        self._c(self.lib.pic_set_step(self.h, int(step)))
 
     # -- profiling
     def profile(self, on=True):
