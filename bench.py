@@ -44,6 +44,7 @@ def make_params(name):
         "plain": lambda: workloads.plain(),
         "plain_256": lambda: workloads.discharge(256, 200),
         "plain_1024": lambda: workloads.large_1024(),
+        "argon_1024": lambda: workloads.argon_1024(),
     }
     if name not in table:
         raise SystemExit(f"unknown workload {name}; choose from {sorted(table)}")
@@ -66,7 +67,9 @@ def describe(name, p):
 
 
 def cpu_baseline(p, e, i, steps, threads):
-    """Time the oracle's FP64 reference-layout step loop on the host."""
+    """Time the oracle's FP64 reference-layout step loop on the host (with the
+    FP64 physical-mode collisions and pair re-injection when the workload has
+    them)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc  # the oracle: only the CPU baseline leg uses it
 
@@ -77,6 +80,16 @@ def cpu_baseline(p, e, i, steps, threads):
     ia = orc.state_to_aos(i, dx)
     Ex = np.zeros((N + 1) * (N + 1))
     used = orc.lib().orc_set_threads(threads)
+    if p.have_collisions:
+        me, mi = orc.mcc_params(p, True), orc.mcc_params(p, False)
+        plop = p.wall == pic2d.WALL_STEADY
+        nid = [len(ea), len(ia)]
+        t0 = time.perf_counter()
+        r = orc.run_ref_mcc(cref, me, mi, plop, p.vte, p.vti, steps, ea, ia, e["id"], i["id"],
+                            nid, Ex, Ex.copy(), grow=1.5)
+        dt = time.perf_counter() - t0
+        nbar = 0.5 * (len(ea) + len(ia) + len(r["e"]) + len(r["i"]))
+        return nbar * steps / dt, dt, 1
     t0 = time.perf_counter()
     orc.run_ref(cref, steps, ea, ia, e["id"], i["id"], Ex, Ex.copy())
     dt = time.perf_counter() - t0
@@ -189,7 +202,8 @@ def main():
 
     if not args.no_cpu_baseline:
         n_all = len(e0["id"]) + len(i0["id"])
-        steps = args.cpu_steps or max(1, min(50, int(round(20.0 / (n_all * 60e-9 + 1e-3)))))
+        per = 120e-9 if p.have_collisions else 60e-9
+        steps = args.cpu_steps or max(1, min(50, int(round(20.0 / (n_all * per + 1e-3)))))
         log("cpu baseline:", steps, "steps of the FP64 reference-layout oracle, 1 thread")
         v1, t_1, _ = cpu_baseline(p, e0, i0, steps, 1)
         out["cpu_baseline"] = {
@@ -198,7 +212,9 @@ def main():
             "cores": 1,
             "kind": "port",
             "sample": f"{steps} steps of the full workload ({n_all} particles), "
-                      f"FP64 AoS reference layout, {t_1:.1f} s",
+                      f"FP64 AoS reference layout"
+                      f"{' with FP64 MCC and pair re-injection' if p.have_collisions else ''}"
+                      f", {t_1:.1f} s",
         }
     print(json.dumps(out), flush=True)
 

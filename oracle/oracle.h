@@ -212,6 +212,54 @@ int orc32_run(const orc32_config *c, int nsteps, long *ne, uint32_t *ecell, floa
 double orc_sigma(int kind, double E, double thr);
 
 /* ------------------------------------------------------------------------ */
+/* Physical-mode Monte-Carlo collisions in FP64 (orc_mcc.c).  Null-collision */
+/* method of Vahedi & Surendra (Comput. Phys. Commun. 87 (1995) 179) with the */
+/* cross sections above; draws = Philox4x32-10 with counter (id, step,        */
+/* species*16 + stream, 0x4D434331) and key (key0, key1), the GPU library's   */
+/* streams.  Pinned by tests/test_oracle_collisions.py (rates against         */
+/* sigma*g*n, energy and momentum bookkeeping of every channel).             */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int gas;          /* 0 argon, 1 hydrogen                                  */
+    int electron;     /* 1 electrons, 0 ions                                  */
+    double ng;        /* neutral density [m^-3]                               */
+    double dt;        /* species time step [s]                                */
+    double mass, M;   /* species and neutral mass [kg]                        */
+    double q;         /* elementary charge [C]                                */
+    double vtn;       /* neutral sigma per component [m/s]                    */
+    double e_exc, e_iz, opal_B, kev_amu_div;
+    double nu_max, p_null; /* set by orc_mcc_setup                            */
+    uint32_t key0, key1;
+} orc_mcc_params;
+
+/* nu' = 1.02 * max of the total collision frequency over 1e-6 .. 3e4 eV
+ * (electrons) or 1e-6 .. 1e3 eV (ions); p_null = 1 - exp(-nu' dt) */
+void orc_mcc_setup(orc_mcc_params *c);
+/* returns 0 none, 1 elastic, 2 excitation, 3 ionisation (child = ejected
+ * electron v[0..2], new ion v[3..5]) */
+int orc_collide_electron(const orc_mcc_params *c, double *vx, double *vy, double *vz, uint32_t id,
+                         uint32_t step, double child[6]);
+/* returns 0 none, 4 charge exchange, 5 isotropic elastic */
+int orc_collide_ion(const orc_mcc_params *c, double *vx, double *vy, double *vz, uint32_t id,
+                    uint32_t step);
+/* the two above over arrays: event[k] per particle, child6 (may be NULL) 6
+ * doubles per particle (zeros unless ionised) */
+void orc_mcc_apply(const orc_mcc_params *c, long n, double *vx, double *vy, double *vz,
+                   const uint32_t *id, uint32_t step, int32_t *event, double *child6);
+
+/* Collisional FP64 step loop in the reference's phase order (pic.cpp:1069-
+ * 1162): electron MCC (children appended in creator order), electron push,
+ * ion MCC + push every dt_dti steps, walls x<0 or x>=L with, when plop is
+ * set, an electron-ion pair re-injected per lost ion (boundary.cpp:854-878
+ * placement, Maxwellian sigma vte / vti), deposit, fields.  events8 counts
+ * collision events by kind.  next_id[2] is advanced by the ids handed out. */
+int orc_ref_run_mcc(const orc_ref_config *c, const orc_mcc_params *me, const orc_mcc_params *mi,
+                    int plop, double vte, double vti, int nsteps, long *ne, long cap_e,
+                    orc_particle *e, uint32_t *eid, long *ni, long cap_i, orc_particle *ip,
+                    uint32_t *iid, uint32_t *next_id, double *Ex, double *Ey, double *qe,
+                    double *qi, double *rho, double *phi, long *events8);
+
+/* ------------------------------------------------------------------------ */
 /* dataDump text I/O (particle.cpp:359-401 reader, 1361-1382 writer)          */
 /* ------------------------------------------------------------------------ */
 /* precision = significant digits (6 reproduces the reference's ostream default) */

@@ -1,0 +1,358 @@
+/*
+ * orc_mcc.c -- physical-mode Monte-Carlo collisions of the oracle and a
+ * collisional step loop in the reference's layout
+ * (TEST INFRASTRUCTURE ONLY; also the collisional CPU baseline).
+ *
+ * Null-collision MCC after Vahedi & Surendra (Comput. Phys. Commun. 87
+ * (1995) 179) with the reference's cross-section fits (orc_collisions.c) in
+ * FP64, drawing the same Philox4x32-10 uniforms as the GPU library
+ * (counter = (particle id, step, species*16 + stream, domain word)), so the
+ * two agree event by event except where FP32 rounding moves a draw across a
+ * decision boundary.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+#define DOMAIN_MCC 0x4D434331u
+#define RS_MAIN 0u
+#define RS_NEUTRAL 1u
+#define RS_IONIZE 2u
+#define RS_PLOP 3u
+#define RS_PLOP2 4u
+
+static void draw(const orc_mcc_params *c, uint32_t id, uint32_t step, uint32_t stream,
+                 double u[4], uint32_t raw[4]) {
+    uint32_t ctr[4] = {id, step, stream, DOMAIN_MCC}, key[2] = {c->key0, c->key1};
+    orc_philox4x32_10(ctr, key, raw);
+    for (int k = 0; k < 4; k++) u[k] = (double)orc_u01(raw[k]);
+}
+
+static void neutral(const orc_mcc_params *c, uint32_t id, uint32_t step, uint32_t s0, double *nx,
+                    double *ny, double *nz) {
+    double u[4];
+    uint32_t r[4];
+    draw(c, id, step, s0 + RS_NEUTRAL, u, r);
+    double r1 = sqrt(-2.0 * log(u[0])), t1 = 6.2831853 * u[1];
+    double r2 = sqrt(-2.0 * log(u[2])), t2 = 6.2831853 * u[3];
+    *nx = c->vtn * r1 * cos(t1);
+    *ny = c->vtn * r1 * sin(t1);
+    *nz = c->vtn * r2 * cos(t2);
+}
+
+static void e_nu(const orc_mcc_params *c, double E, double g, double *nel, double *nex, double *niz) {
+    double f = c->ng * 1e-20 * g;
+    if (c->gas == 0) {
+        *nel = f * orc_sigma(ORC_SIG_AR_EL_PHYS, E, 0.0);
+        *nex = f * orc_sigma(ORC_SIG_AR_EXC, E, c->e_exc);
+        *niz = f * orc_sigma(ORC_SIG_AR_IZ_PHYS, E, c->e_iz);
+    } else {
+        double s = orc_sigma(ORC_SIG_H_EL, E, 0.0);
+        *nel = f * (s > 0.0 ? s : 0.0);
+        *nex = 0.0;
+        *niz = f * 1000.0 * orc_sigma(ORC_SIG_H_IZ, E, 0.0);
+    }
+}
+
+static void i_nu(const orc_mcc_params *c, double E, double g, double *niso, double *ncx) {
+    double f = c->ng * 1e-20 * g;
+    if (c->gas == 0) {
+        *niso = f * orc_sigma(ORC_SIG_AR_ISO, E, 0.0);
+        *ncx = f * orc_sigma(ORC_SIG_AR_CX, E, 0.0);
+    } else {
+        *niso = 0.0;
+        *ncx = f * orc_sigma(ORC_SIG_H_CX, E / 1000.0 / c->kev_amu_div, 0.0);
+    }
+}
+
+void orc_mcc_setup(orc_mcc_params *c) {
+    double best = 0.0, top = c->electron ? 4.5 : 3.0;
+    for (int k = 0; k <= 200000; k++) {
+        double E = pow(10.0, -6.0 + (top + 6.0) * k / 200000.0);
+        double g = sqrt(2.0 * c->q * E / c->mass), a, b, d = 0.0;
+        if (c->electron)
+            e_nu(c, E, g, &a, &b, &d);
+        else
+            i_nu(c, E, g, &a, &b);
+        if (a + b + d > best) best = a + b + d;
+    }
+    c->nu_max = best * 1.02;
+    c->p_null = 1.0 - exp(-c->nu_max * c->dt);
+}
+
+static double cos_chi(double E, double R) {
+    if (E < 1e-4) return 1.0 - 2.0 * R;
+    double cc = (2.0 + E - 2.0 * pow(1.0 + E, R)) / E;
+    return cc > 1.0 ? 1.0 : (cc < -1.0 ? -1.0 : cc);
+}
+
+/* Nanbu (IEEE TPS 28 (2000) 971) eq. 56 rotation of g by (chi, phi) */
+static void rotate(double gx, double gy, double gz, double cc, double sc, double cp, double sp,
+                   double *ox, double *oy, double *oz) {
+    double g = sqrt(gx * gx + gy * gy + gz * gz), gp = sqrt(gy * gy + gz * gz);
+    if (gp < 1e-6 * g || gp == 0.0) {
+        *ox = gx * cc;
+        *oy = g * sc * cp;
+        *oz = g * sc * sp;
+        return;
+    }
+    *ox = gx * cc + gp * sc * cp;
+    *oy = gy * cc - (gx * gy * cp - g * gz * sp) / gp * sc;
+    *oz = gz * cc - (gx * gz * cp + g * gy * sp) / gp * sc;
+}
+
+int orc_collide_electron(const orc_mcc_params *c, double *vx, double *vy, double *vz, uint32_t id,
+                         uint32_t step, double child[6]) {
+    double u[4];
+    uint32_t raw[4];
+    draw(c, id, step, RS_MAIN, u, raw);
+    if (!(u[0] < c->p_null)) return 0;
+    double nx, ny, nz;
+    neutral(c, id, step, 0u, &nx, &ny, &nz);
+    double gx = *vx - nx, gy = *vy - ny, gz = *vz - nz;
+    double g2 = gx * gx + gy * gy + gz * gz, g = sqrt(g2);
+    double E = (0.5 / c->q) * c->mass * g2;
+    double nel, nex, niz;
+    e_nu(c, E, g, &nel, &nex, &niz);
+    double x = u[1] * c->nu_max;
+    double cp = cos(6.2831853 * u[3]), sp = sin(6.2831853 * u[3]);
+    double hx, hy, hz;
+    if (x < nel) {
+        double cc = cos_chi(E, u[2]), sc = sqrt(fmax(0.0, 1.0 - cc * cc));
+        rotate(gx, gy, gz, cc, sc, cp, sp, &hx, &hy, &hz);
+        double cf = c->M / (c->mass + c->M);
+        *vx -= cf * (gx - hx);
+        *vy -= cf * (gy - hy);
+        *vz -= cf * (gz - hz);
+        return 1;
+    }
+    if (x < nel + nex) {
+        double Ep = E - c->e_exc;
+        if (!(Ep > 0.0)) return 0;
+        double cc = cos_chi(Ep, u[2]), sc = sqrt(fmax(0.0, 1.0 - cc * cc));
+        rotate(gx, gy, gz, cc, sc, cp, sp, &hx, &hy, &hz);
+        double k = sqrt(2.0 * c->q * Ep / c->mass) / g;
+        *vx = nx + hx * k;
+        *vy = ny + hy * k;
+        *vz = nz + hz * k;
+        return 2;
+    }
+    if (x < nel + nex + niz) {
+        double Ex = E - c->e_iz;
+        if (!(Ex > 0.0)) return 0;
+        double q4[4];
+        uint32_t r4[4];
+        draw(c, id, step, RS_IONIZE, q4, r4);
+        double Eej = c->opal_B * tan(q4[0] * atan(Ex / (2.0 * c->opal_B)));
+        double Esc = Ex - Eej;
+        double cc = cos_chi(Esc, u[2]), sc = sqrt(fmax(0.0, 1.0 - cc * cc));
+        rotate(gx, gy, gz, cc, sc, cp, sp, &hx, &hy, &hz);
+        double k = sqrt(2.0 * c->q * Esc / c->mass) / g;
+        *vx = nx + hx * k;
+        *vy = ny + hy * k;
+        *vz = nz + hz * k;
+        double cc2 = cos_chi(Eej, q4[1]), sc2 = sqrt(fmax(0.0, 1.0 - cc2 * cc2));
+        double ph2 = 6.2831853 * q4[2];
+        rotate(gx, gy, gz, cc2, sc2, cos(ph2), sin(ph2), &hx, &hy, &hz);
+        double k2 = sqrt(2.0 * c->q * Eej / c->mass) / g;
+        child[0] = nx + hx * k2;
+        child[1] = ny + hy * k2;
+        child[2] = nz + hz * k2;
+        child[3] = nx;
+        child[4] = ny;
+        child[5] = nz;
+        return 3;
+    }
+    return 0;
+}
+
+int orc_collide_ion(const orc_mcc_params *c, double *vx, double *vy, double *vz, uint32_t id,
+                    uint32_t step) {
+    double u[4];
+    uint32_t raw[4];
+    draw(c, id, step, 16u + RS_MAIN, u, raw);
+    if (!(u[0] < c->p_null)) return 0;
+    double nx, ny, nz;
+    neutral(c, id, step, 16u, &nx, &ny, &nz);
+    double gx = *vx - nx, gy = *vy - ny, gz = *vz - nz;
+    double g2 = gx * gx + gy * gy + gz * gz, g = sqrt(g2);
+    double E = (0.5 / c->q) * c->mass * g2;
+    double niso, ncx;
+    i_nu(c, E, g, &niso, &ncx);
+    double x = u[1] * c->nu_max;
+    if (x < niso) {
+        double cc = 1.0 - 2.0 * u[2], sc = sqrt(fmax(0.0, 1.0 - cc * cc)), ph = 6.2831853 * u[3];
+        double cmx = 0.5 * (*vx + nx), cmy = 0.5 * (*vy + ny), cmz = 0.5 * (*vz + nz);
+        *vx = cmx + 0.5 * g * sc * cos(ph);
+        *vy = cmy + 0.5 * g * sc * sin(ph);
+        *vz = cmz + 0.5 * g * cc;
+        return 5;
+    }
+    if (x < niso + ncx) {
+        *vx = nx;
+        *vy = ny;
+        *vz = nz;
+        return 4;
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------- collisional step loop */
+
+typedef struct {
+    uint64_t key;
+    long src;
+} orc_kv;
+
+static int kv_cmp(const void *a, const void *b) {
+    uint64_t x = ((const orc_kv *)a)->key, y = ((const orc_kv *)b)->key;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* append staged particles in creator-key order with consecutive ids */
+static long append_sorted(orc_particle *dst, uint32_t *did, long n, long cap, const orc_particle *st,
+                          orc_kv *kv, long ns, uint32_t *next_id) {
+    qsort(kv, (size_t)ns, sizeof(orc_kv), kv_cmp);
+    for (long r = 0; r < ns; r++) {
+        if (n < cap) {
+            dst[n] = st[kv[r].src];
+            did[n] = *next_id + (uint32_t)r;
+            n++;
+        }
+    }
+    *next_id += (uint32_t)ns;
+    return n;
+}
+
+int orc_ref_run_mcc(const orc_ref_config *c, const orc_mcc_params *me, const orc_mcc_params *mi,
+                    int plop, double vte, double vti, int nsteps, long *ne, long cap_e,
+                    orc_particle *e, uint32_t *eid, long *ni, long cap_i, orc_particle *ip,
+                    uint32_t *iid, uint32_t *next_id, double *Ex, double *Ey, double *qe,
+                    double *qi, double *rho, double *phi, long *events8) {
+    int N = c->N;
+    long scap = 1 << 20;
+    orc_particle *se = (orc_particle *)malloc(sizeof(orc_particle) * scap);
+    orc_particle *si = (orc_particle *)malloc(sizeof(orc_particle) * scap);
+    orc_kv *ke = (orc_kv *)malloc(sizeof(orc_kv) * scap);
+    orc_kv *ki = (orc_kv *)malloc(sizeof(orc_kv) * scap);
+    long capmax = cap_e > cap_i ? cap_e : cap_i;
+    double *Exp = (double *)malloc(sizeof(double) * (capmax + 1));
+    double *Eyp = (double *)malloc(sizeof(double) * (capmax + 1));
+    double L = c->L, dx = c->dx;
+    for (int s = 0; s < nsteps; s++) {
+        uint32_t step = (uint32_t)(c->start_step + s);
+        /* e_collisions (pic.cpp:1071-1075): products appended before the push */
+        long nse = 0, nsi = 0;
+        for (long k = 0; k < *ne; k++) {
+            double ch[6];
+            int ev = orc_collide_electron(me, &e[k].vx, &e[k].vy, &e[k].vz, eid[k], step, ch);
+            events8[ev]++;
+            if (ev == 3 && nse < scap) {
+                orc_particle a = e[k], b = e[k];
+                a.vx = ch[0], a.vy = ch[1], a.vz = ch[2];
+                b.vx = ch[3], b.vy = ch[4], b.vz = ch[5];
+                se[nse] = a;
+                ke[nse].key = eid[k];
+                ke[nse].src = nse;
+                si[nsi] = b;
+                ki[nsi].key = eid[k];
+                ki[nsi].src = nsi;
+                nse++, nsi++;
+            }
+        }
+        *ni = append_sorted(ip, iid, *ni, cap_i, si, ki, nsi, &next_id[1]);
+        *ne = append_sorted(e, eid, *ne, cap_e, se, ke, nse, &next_id[0]);
+        orc_ref_gather(e, *ne, Ex, Ey, dx, dx, N, N, Exp, Eyp);
+        orc_ref_push(e, *ne, Exp, Eyp, dx, dx, N, N, c->coef_e, c->cos_a, c->sin_a, c->dt_e,
+                     c->cos_we, c->sin_we, c->bfield);
+        if (step % c->dt_dti == 0) {
+            for (long k = 0; k < *ni; k++)
+                events8[orc_collide_ion(mi, &ip[k].vx, &ip[k].vy, &ip[k].vz, iid[k], step)]++;
+            orc_ref_gather(ip, *ni, Ex, Ey, dx, dx, N, N, Exp, Eyp);
+            orc_ref_push(ip, *ni, Exp, Eyp, dx, dx, N, N, c->coef_i, c->cos_a, c->sin_a, c->dt_i,
+                         c->cos_wi, c->sin_wi, c->bfield);
+        }
+        /* walls: x < 0 or x >= L leaves (physical mode); with plop every lost
+         * ion re-injects an electron-ion pair (boundary.cpp:854-878 placement) */
+        nse = nsi = 0;
+        long w = 0;
+        for (long k = 0; k < *ni; k++) {
+            int out = !(ip[k].x >= 0.0 && ip[k].x < L && ip[k].y >= 0.0 && ip[k].y < L);
+            if (!out) {
+                ip[w] = ip[k];
+                iid[w] = iid[k];
+                w++;
+                continue;
+            }
+            if (plop && nse < scap) {
+                double u0[4], u1[4], u2[4];
+                uint32_t r0[4], r1[4], r2[4];
+                draw(mi, iid[k], step, 16u + RS_PLOP, u0, r0);
+                draw(mi, iid[k], step, 16u + RS_PLOP2, u1, r1);
+                draw(mi, iid[k], step, 16u + RS_PLOP2 + 1u, u2, r2);
+                double cells = (double)(N - 3);
+                double tx = 1.0 + u0[0] * cells, ty = 1.0 + u0[1] * cells;
+                uint32_t wa[6] = {r1[0], r1[1], r1[2], r1[3], r2[0], r2[1]};
+                uint32_t wb[6] = {r2[2], r2[3], r0[2], r0[3], r1[1] ^ r2[2], r1[3] ^ r2[0]};
+                double nrm[6];
+                for (int m = 0; m < 6; m++)
+                    nrm[m] = sqrt(-2.0 * log((double)orc_u01(wa[m]))) *
+                             cos(6.2831853 * (double)orc_u01(wb[m]));
+                orc_particle pe = {tx * dx, ty * dx, vte * nrm[0], vte * nrm[1], vte * nrm[2], 1.0};
+                orc_particle pi = {tx * dx, ty * dx, vti * nrm[3], vti * nrm[4], vti * nrm[5], 1.0};
+                se[nse] = pe;
+                ke[nse].key = (1ull << 32) | iid[k];
+                ke[nse].src = nse;
+                si[nsi] = pi;
+                ki[nsi].key = (1ull << 32) | iid[k];
+                ki[nsi].src = nsi;
+                nse++, nsi++;
+            }
+        }
+        *ni = w;
+        w = 0;
+        for (long k = 0; k < *ne; k++) {
+            int out = !(e[k].x >= 0.0 && e[k].x < L && e[k].y >= 0.0 && e[k].y < L);
+            if (!out) {
+                e[w] = e[k];
+                eid[w] = eid[k];
+                w++;
+            }
+        }
+        *ne = w;
+        *ni = append_sorted(ip, iid, *ni, cap_i, si, ki, nsi, &next_id[1]);
+        *ne = append_sorted(e, eid, *ne, cap_e, se, ke, nse, &next_id[0]);
+        memset(qi, 0, sizeof(double) * N * N);
+        memset(qe, 0, sizeof(double) * N * N);
+        orc_ref_deposit(ip, *ni, c->qc_dxdy_i, dx, dx, N, N, qi);
+        orc_ref_charge_postscale(qi, N, N);
+        orc_ref_deposit(e, *ne, c->qc_dxdy_e, dx, dx, N, N, qe);
+        orc_ref_charge_postscale(qe, N, N);
+        orc_ref_grid_rho(qi, qe, N, rho);
+        if (orc_ref_grid_potential(rho, N, dx, dx, c->eps, c->phi_left, c->phi_right, c->phi_bottom,
+                                   c->phi_top, (double)step, c->phi_period, c->ncycle, L, phi) != 0)
+            break;
+        orc_ref_grid_efield(phi, N, dx, Ex, Ey);
+    }
+    free(se);
+    free(si);
+    free(ke);
+    free(ki);
+    free(Exp);
+    free(Eyp);
+    return 0;
+}
+
+void orc_mcc_apply(const orc_mcc_params *c, long n, double *vx, double *vy, double *vz,
+                   const uint32_t *id, uint32_t step, int32_t *event, double *child6) {
+    for (long k = 0; k < n; k++) {
+        double ch[6] = {0, 0, 0, 0, 0, 0};
+        int ev = c->electron ? orc_collide_electron(c, &vx[k], &vy[k], &vz[k], id[k], step, ch)
+                             : orc_collide_ion(c, &vx[k], &vy[k], &vz[k], id[k], step);
+        event[k] = ev;
+        if (child6)
+            for (int m = 0; m < 6; m++) child6[6 * k + m] = ch[m];
+    }
+}

@@ -46,6 +46,13 @@ class CfgRef(C.Structure):
     ] + [("start_step", C.c_long)]
 
 
+class MccParams(C.Structure):
+    _fields_ = [("gas", C.c_int), ("electron", C.c_int)] + [
+        (k, C.c_double) for k in ("ng", "dt", "mass", "M", "q", "vtn", "e_exc", "e_iz", "opal_B",
+                                  "kev_amu_div", "nu_max", "p_null")
+    ] + [("key0", C.c_uint32), ("key1", C.c_uint32)]
+
+
 _lib = None
 
 
@@ -83,6 +90,13 @@ def lib():
         _lib.orc_set_threads.argtypes = [C.c_int]
         _lib.orc_sigma.argtypes = [C.c_int, C.c_double, C.c_double]
         _lib.orc_sigma.restype = C.c_double
+        _lib.orc_mcc_setup.argtypes = [C.POINTER(MccParams)]
+        _lib.orc_mcc_apply.argtypes = [C.POINTER(MccParams), C.c_long, vp, vp, vp, vp, C.c_uint32,
+                                       vp, vp]
+        _lib.orc_ref_run_mcc.argtypes = [C.POINTER(CfgRef), C.POINTER(MccParams),
+                                         C.POINTER(MccParams), C.c_int, C.c_double, C.c_double,
+                                         C.c_int, vp, C.c_long, vp, vp, vp, C.c_long, vp, vp, vp,
+                                         vp, vp, vp, vp, vp, vp, vp]
         _lib.orc_dump_write.argtypes = [C.c_char_p, vp, C.c_long, C.c_int]
         _lib.orc_dump_read.argtypes = [C.c_char_p, vp, C.c_long]
         _lib.orc_dump_read.restype = C.c_long
@@ -321,3 +335,68 @@ def nu_max(mass, ng, electron, q=1.602177e-19):
     g = np.sqrt(2 * q * E / mass)
     parts = e_nu_parts(E, g, ng) if electron else i_nu_parts(E, g, ng)
     return float(np.max(sum(parts))) * 1.02
+
+
+Q_E = 1.602177e-19
+AMU = 1.66054e-27
+
+
+def mcc_params(p, electron):
+    """oracle MCC parameters for one species of a pic2d.Params (physical mode)"""
+    argon = p.gas == 0
+    c = MccParams(gas=0 if argon else 1, electron=int(electron), ng=p.neutral_density,
+                  dt=p.dt if electron else p.dti,
+                  mass=p.electron_mass if electron else p.ion_mass, M=p.ion_mass, q=Q_E,
+                  vtn=p.vtn, e_exc=11.55, e_iz=15.76 if argon else 13.6,
+                  opal_B=10.0 if argon else 8.3, kev_amu_div=p.ion_mass / AMU,
+                  key0=p.seed & 0xFFFFFFFF, key1=(p.seed >> 32) & 0xFFFFFFFF)
+    lib().orc_mcc_setup(C.byref(c))
+    return c
+
+
+def mcc_apply(c, vx, vy, vz, ids, step):
+    """FP64 collisions of arrays of particles; returns (vx, vy, vz, event, child6)"""
+    vx = np.array(vx, dtype=np.float64, copy=True)
+    vy = np.array(vy, dtype=np.float64, copy=True)
+    vz = np.array(vz, dtype=np.float64, copy=True)
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
+    n = len(vx)
+    ev = np.zeros(n, dtype=np.int32)
+    ch = np.zeros((n, 6))
+    lib().orc_mcc_apply(C.byref(c), n, P(vx), P(vy), P(vz), P(ids), step, P(ev), P(ch))
+    return vx, vy, vz, ev, ch
+
+
+def run_ref_mcc(cfg, me, mi, plop, vte, vti, nsteps, e, i, eid, iid, next_id, Ex, Ey,
+                grow=2.0):
+    """orc_ref_run_mcc on copies; capacity grow x the initial counts"""
+    N = cfg.N
+    cap_e = int(len(e) * grow) + 1024
+    cap_i = int(len(i) * grow) + 1024
+    ea = np.zeros(cap_e, dtype=PARTICLE_DTYPE)
+    ea[: len(e)] = e
+    ia = np.zeros(cap_i, dtype=PARTICLE_DTYPE)
+    ia[: len(i)] = i
+    ea_id = np.zeros(cap_e, dtype=np.uint32)
+    ea_id[: len(e)] = eid
+    ia_id = np.zeros(cap_i, dtype=np.uint32)
+    ia_id[: len(i)] = iid
+    ne = C.c_long(len(e))
+    ni = C.c_long(len(i))
+    nid = np.array(next_id, dtype=np.uint32)
+    Ex = np.array(Ex, dtype=np.float64, copy=True)
+    Ey = np.array(Ey, dtype=np.float64, copy=True)
+    qe = np.zeros(N * N)
+    qi = np.zeros(N * N)
+    rho = np.zeros(N * N)
+    phi = np.zeros(N * N)
+    ev = np.zeros(8, dtype=np.int64)
+    rc = lib().orc_ref_run_mcc(C.byref(cfg), C.byref(me), C.byref(mi), int(plop), vte, vti,
+                               nsteps, C.byref(ne), cap_e, P(ea), P(ea_id), C.byref(ni), cap_i,
+                               P(ia), P(ia_id), P(nid), P(Ex), P(Ey), P(qe), P(qi), P(rho),
+                               P(phi), P(ev))
+    if rc != 0:
+        raise RuntimeError("orc_ref_run_mcc failed")
+    return dict(e=ea[: ne.value], i=ia[: ni.value], eid=ea_id[: ne.value], iid=ia_id[: ni.value],
+                next_id=nid, Ex=Ex, Ey=Ey, qe=qe.reshape(N, N), qi=qi.reshape(N, N),
+                rho=rho.reshape(N, N), phi=phi.reshape(N, N), events=ev)

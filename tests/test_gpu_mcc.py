@@ -209,3 +209,91 @@ def test_pair_reinjection_keeps_ion_count():
         assert lost_i > 0
         assert sim.particle_size(1) == n0
         assert sim.particle_size(0) == n0 - sim.loss_counts(0)[:4].sum() + lost_i
+
+
+def _spread_state(n, N, mass, e_lo, e_hi, seed):
+    """log-uniform energies in [e_lo, e_hi] eV with isotropic directions"""
+    rng = np.random.default_rng(seed)
+    E = np.exp(rng.uniform(math.log(e_lo), math.log(e_hi), n))
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    v = (np.sqrt(2 * Q * E / mass)[:, None] * d).astype(np.float32)
+    st = beam(n, N, 0.0)
+    st["vx"], st["vy"], st["vz"] = v[:, 0].copy(), v[:, 1].copy(), v[:, 2].copy()
+    return st
+
+
+def _compare(st_in, st_out, ref, scale):
+    """per-particle comparison by id: events must agree except for draws that
+    FP32 rounding moves across a decision boundary (at most 1 in 2e4), and
+    velocities of agreeing particles to 2e-4 of the speed scale"""
+    n = len(st_in["id"])
+    keep = st_out["id"] < n
+    order = np.argsort(st_out["id"][keep])
+    gv = np.stack([st_out[k][keep][order].astype(np.float64) for k in ("vx", "vy", "vz")], 1)
+    ov = np.stack(ref[:3], 1)
+    v0 = np.stack([st_in[k].astype(np.float64) for k in ("vx", "vy", "vz")], 1)
+    g_changed = np.any(gv != v0, axis=1)
+    o_changed = ref[3] != 0
+    mismatch = g_changed != o_changed
+    assert mismatch.sum() <= max(2, n // 20000), mismatch.sum()
+    ok = ~mismatch
+    err = np.abs(gv[ok] - ov[ok]).max(axis=1) / (np.linalg.norm(v0[ok], axis=1) + scale)
+    assert err.max() < 2e-4, err.max()
+    return mismatch.sum()
+
+
+def test_electron_collisions_match_the_fp64_oracle_particle_by_particle():
+    vtn = math.sqrt(1.38066e-23 * 300.0 / MI)
+    p = collisional(ppc=100, vtn=vtn)
+    p.dt = 1e-10                                  # many collisions per call
+    n = 100000
+    st = _spread_state(n, p.nodes, ME, 0.05, 300.0, 11)
+    with pic2d.Pic(p) as sim:
+        ni0 = sim.particle_size(1)
+        sim.set_state(0, st)
+        sim.set_step(17)
+        sim.e_collisions()
+        out = sim.get_state(0)
+        ions = sim.get_state(1)
+        ev = sim.collision_counts(0)
+    c = orc.mcc_params(p, True)
+    ref = orc.mcc_apply(c, st["vx"], st["vy"], st["vz"], st["id"], 17)
+    for k in (1, 2, 3):
+        assert abs(int((ref[3] == k).sum()) - int(ev[k])) <= max(2, n // 20000), (k, ev)
+    assert ev[1] > 1000 and ev[2] > 100 and ev[3] > 100
+    bad = _compare(st, out, ref, vtn)
+    if bad == 0:
+        # children: appended in parent-id order with consecutive fresh ids
+        iz = np.nonzero(ref[3] == 3)[0]
+        kids = out["id"] >= n
+        order = np.argsort(out["id"][kids])
+        gk = np.stack([out[k][kids][order].astype(np.float64) for k in ("vx", "vy", "vz")], 1)
+        ok = ref[4][iz]
+        assert len(gk) == len(iz)
+        scale = np.linalg.norm(np.stack(ref[:3], 1)[iz], axis=1) + vtn
+        assert (np.abs(gk - ok[:, :3]).max(axis=1) / scale).max() < 2e-4
+        ion_new = ions["id"] >= ni0
+        io = np.argsort(ions["id"][ion_new])
+        gi = np.stack([ions[k][ion_new][io].astype(np.float64) for k in ("vx", "vy", "vz")], 1)
+        assert np.abs(gi - ok[:, 3:]).max() < 2e-4 * 5 * vtn
+
+
+def test_ion_collisions_match_the_fp64_oracle_particle_by_particle():
+    vtn = math.sqrt(1.38066e-23 * 300.0 / MI)
+    p = collisional(ppc=100, vtn=vtn)
+    p.dt = p.dti = 2e-8
+    n = 100000
+    st = _spread_state(n, p.nodes, MI, 0.01, 100.0, 12)
+    with pic2d.Pic(p) as sim:
+        sim.set_state(1, st)
+        sim.set_step(5)
+        sim.ion_collisions()
+        out = sim.get_state(1)
+        ev = sim.collision_counts(1)
+    c = orc.mcc_params(p, False)
+    ref = orc.mcc_apply(c, st["vx"], st["vy"], st["vz"], st["id"], 5)
+    for k in (4, 5):
+        assert abs(int((ref[3] == k).sum()) - int(ev[k])) <= max(2, n // 20000), (k, ev)
+    assert ev[4] > 1000 and ev[5] > 1000
+    _compare(st, out, ref, vtn)

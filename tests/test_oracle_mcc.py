@@ -1,0 +1,180 @@
+"""The oracle's FP64 physical-mode collisions, pinned by what the physics fixes.
+
+With a cold background (vtn = 0) the partner is at rest, so every channel has
+an exact outcome independent of the code's formulas: elastic scattering keeps
+|v' - (1 - c) v| = c |v| with c = M/(m+M) (the electron turns on a sphere around
+the centre of mass); excitation removes exactly 11.55 eV; ionisation shares
+exactly E - 15.76 eV between the two electrons and leaves the ion at rest;
+charge exchange leaves the ion at rest; isotropic ion scattering keeps
+|v' - v/2| = |v|/2 with <cos> of the scattering angle 0.  Event counts are
+binomial with p = (1 - exp(-nu' dt)) nu_k / nu' (null-collision method), and
+the mean scattering cosine matches the closed form of the Vahedi-Surendra
+angular law, <cos chi> = (2+E)/E - 2/ln(1+E).
+"""
+import math
+
+import numpy as np
+
+import orc
+
+Q, ME = 1.602177e-19, 9.10939e-31
+MI = 39.948 * 1.66054e-27
+NG = 13.332 / (1.38066e-23 * 300.0)
+
+
+def params(electron, dt, vtn=0.0, seed=0x5EED0001):
+    c = orc.MccParams(gas=0, electron=int(electron), ng=NG, dt=dt, mass=ME if electron else MI,
+                      M=MI, q=Q, vtn=vtn, e_exc=11.55, e_iz=15.76, opal_B=10.0,
+                      kev_amu_div=39.948, key0=seed & 0xFFFFFFFF, key1=seed >> 32)
+    orc.lib().orc_mcc_setup(orc.C.byref(c))
+    return c
+
+
+def beam(E, mass, n, rng):
+    v = math.sqrt(2 * Q * E / mass)
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return v * d[:, 0], v * d[:, 1], v * d[:, 2]
+
+
+def binomial_ok(count, n, p, z=4.5):
+    mu = n * p
+    return abs(count - mu) <= z * math.sqrt(max(mu * (1 - p), 1.0)) + 1
+
+
+def test_null_collision_bound_matches_the_energy_scan():
+    c = params(True, 1e-11)
+    assert abs(c.nu_max - orc.nu_max(ME, NG, True)) <= 2e-3 * c.nu_max
+    assert abs(c.p_null - (1 - math.exp(-c.nu_max * c.dt))) < 1e-15
+    ci = params(False, 1e-9)
+    assert abs(ci.nu_max - orc.nu_max(MI, NG, False)) <= 2e-3 * ci.nu_max
+
+
+def test_electron_channels_rates_and_exact_kinematics():
+    rng = np.random.default_rng(1)
+    dt = 2e-11
+    c = params(True, dt)
+    n = 200000
+    for E in (5.0, 30.0, 80.0):
+        vx, vy, vz = beam(E, ME, n, rng)
+        ids = np.arange(n, dtype=np.uint32)
+        ox, oy, oz, ev, ch = orc.mcc_apply(c, vx, vy, vz, ids, 7)
+        g = math.sqrt(2 * Q * E / ME)
+        nel, nex, niz = (x[0] for x in orc.e_nu_parts(np.array([E]), g, NG))
+        for kind, nu in ((1, nel), (2, nex), (3, niz)):
+            p = c.p_null * nu / c.nu_max
+            assert binomial_ok(int((ev == kind).sum()), n, p), (E, kind)
+        assert (ev[(ev != 0)] <= 3).all()
+        cf = MI / (ME + MI)
+        v0 = np.stack([vx, vy, vz], 1)
+        v1 = np.stack([ox, oy, oz], 1)
+        el = ev == 1
+        r = np.linalg.norm(v1[el] - (1 - cf) * v0[el], axis=1)
+        np.testing.assert_allclose(r, cf * g, rtol=1e-12)
+        ex = ev == 2
+        if ex.any():
+            E1 = 0.5 * ME * (v1[ex] ** 2).sum(1) / Q
+            np.testing.assert_allclose(E1, E - 11.55, rtol=1e-12, atol=1e-12)
+        iz = ev == 3
+        if iz.any():
+            E1 = 0.5 * ME * (v1[iz] ** 2).sum(1) / Q
+            E2 = 0.5 * ME * (ch[iz, :3] ** 2).sum(1) / Q
+            np.testing.assert_allclose(E1 + E2, E - 15.76, rtol=1e-11, atol=1e-11)
+            assert np.all(ch[iz, 3:] == 0.0)
+            # the ejected electron takes the smaller share on average (Opal law,
+            # B = 10 eV): mean ejected energy < (E - 15.76) / 2
+            assert E2.mean() < 0.5 * (E - 15.76)
+        nothing = ev == 0
+        assert np.all(v1[nothing] == v0[nothing])
+
+
+def test_scattering_cosine_follows_the_angular_law():
+    rng = np.random.default_rng(2)
+    c = params(True, 1e-9)  # large dt: most particles collide
+    n = 400000
+    for E in (0.5, 4.0, 40.0):
+        vx, vy, vz = beam(E, ME, n, rng)
+        ox, oy, oz, ev, _ = orc.mcc_apply(c, vx, vy, vz, np.arange(n, dtype=np.uint32), 3)
+        el = ev == 1
+        cf = MI / (ME + MI)
+        v0 = np.stack([vx, vy, vz], 1)[el]
+        v1 = np.stack([ox, oy, oz], 1)[el]
+        gp = (v1 - (1 - cf) * v0) / cf  # scattered relative velocity
+        cos = (gp * v0).sum(1) / (np.linalg.norm(gp, axis=1) * np.linalg.norm(v0, axis=1))
+        want = (2 + E) / E - 2 / math.log1p(E)
+        sd = cos.std() / math.sqrt(el.sum())
+        assert abs(cos.mean() - want) < 5 * sd + 1e-4, (E, cos.mean(), want)
+
+
+def test_ion_charge_exchange_and_isotropic_scattering():
+    rng = np.random.default_rng(3)
+    c = params(False, 5e-9)
+    n = 200000
+    for E in (0.05, 1.0, 20.0):
+        vx, vy, vz = beam(E, MI, n, rng)
+        ox, oy, oz, ev, _ = orc.mcc_apply(c, vx, vy, vz, np.arange(n, dtype=np.uint32), 11)
+        g = math.sqrt(2 * Q * E / MI)
+        niso, ncx = (x[0] for x in orc.i_nu_parts(np.array([E]), g, NG))
+        assert binomial_ok(int((ev == 4).sum()), n, c.p_null * ncx / c.nu_max), E
+        assert binomial_ok(int((ev == 5).sum()), n, c.p_null * niso / c.nu_max), E
+        v0 = np.stack([vx, vy, vz], 1)
+        v1 = np.stack([ox, oy, oz], 1)
+        assert np.all(v1[ev == 4] == 0.0)
+        iso = ev == 5
+        d = v1[iso] - 0.5 * v0[iso]
+        np.testing.assert_allclose(np.linalg.norm(d, axis=1), 0.5 * g, rtol=1e-12)
+        cos = (d * v0[iso]).sum(1) / (np.linalg.norm(d, axis=1) * g)
+        assert abs(cos.mean()) < 5 / math.sqrt(iso.sum())
+
+
+def test_warm_background_neutral_is_maxwellian():
+    # charge exchange hands the ion the neutral's velocity: a sample of the
+    # background Maxwellian with sigma vtn per component (a fast beam, so the
+    # rate's weighting by the relative speed barely biases the sample)
+    rng = np.random.default_rng(4)
+    vtn = math.sqrt(1.38066e-23 * 300.0 / MI)
+    c = params(False, 5e-8, vtn=vtn)
+    n = 200000
+    vx, vy, vz = beam(20.0, MI, n, rng)
+    ox, oy, oz, ev, _ = orc.mcc_apply(c, vx, vy, vz, np.arange(n, dtype=np.uint32), 5)
+    cx = ev == 4
+    assert cx.sum() > 2000
+    for comp in (ox[cx], oy[cx], oz[cx]):
+        assert abs(comp.mean()) < 5 * vtn / math.sqrt(cx.sum())
+        assert abs(comp.std() / vtn - 1) < 5 * math.sqrt(0.5 / cx.sum()) + 1e-3
+
+
+def test_collisional_step_loop_conserves_pairs_with_reinjection():
+    """orc_ref_run_mcc with plop: ion count stays fixed by re-injection, ids
+    stay unique and every created particle takes a fresh id."""
+    rng = np.random.default_rng(5)
+    N, L = 17, 0.004
+    dx = L / (N - 1)
+    n = 4000
+    cfg = orc.CfgRef(N=N, ncycle=2, dt_dti=1, L=L, dx=dx, dy=dx, eps=8.854188e-12,
+                     coef_e=0.5 * 1e-11 * (-Q / ME), coef_i=0.5 * 1e-11 * (Q / MI), dt_e=1e-11,
+                     dt_i=1e-11, qc_dxdy_e=-Q * 1e3 / dx ** 2, qc_dxdy_i=Q * 1e3 / dx ** 2,
+                     cos_a=1.0, sin_a=0.0, cos_we=1.0, sin_we=0.0, cos_wi=1.0, sin_wi=0.0,
+                     bfield=0.0, phi_period=1.0, start_step=0)
+    vte = math.sqrt(Q * 10.0 / ME)
+    vti = math.sqrt(Q * 50.0 / MI)  # hot ions: some leave every step
+    e = np.zeros(n, dtype=orc.PARTICLE_DTYPE)
+    i = np.zeros(n, dtype=orc.PARTICLE_DTYPE)
+    for a, vt in ((e, vte), (i, vti)):
+        a["x"] = rng.uniform(dx, L - dx, n)
+        a["y"] = rng.uniform(dx, L - dx, n)
+        a["vx"], a["vy"], a["vz"] = rng.standard_normal((3, n)) * vt
+        a["weight"] = 1.0
+    me = params(True, 1e-11)
+    mi = params(False, 1e-11)
+    ids = np.arange(n, dtype=np.uint32)
+    Ex = np.zeros((N + 1) ** 2)
+    r = orc.run_ref_mcc(cfg, me, mi, True, vte, vti, 20, e, i, ids, ids, [n, n], Ex, Ex)
+    created = r["next_id"][1] - n
+    assert created > 0
+    assert len(r["i"]) == n + r["events"][3]  # every lost ion was replaced
+    for key in ("eid", "iid"):
+        assert len(np.unique(r[key])) == len(r[key])
+    assert r["iid"].max() < r["next_id"][1] and r["eid"].max() < r["next_id"][0]
+    for a in (r["e"], r["i"]):
+        assert np.all((a["x"] >= 0) & (a["x"] < L) & (a["y"] >= 0) & (a["y"] < L))

@@ -7,9 +7,14 @@ reference's (constants.h:15-36); the grid is given in nodes (2^k + 1).
                     Te = 2 eV, Ti = 0.026 eV, n = 1e15 m^-3, collisionless,
                     absorbing walls, w_pe dt = 0.1.
 * ``argon_256``  -- 256x256 cells (257^2 nodes), 200 particles/cell/species
-                    (13.1 M each), the same initial state, with collisions.
+                    (13.1 M each), the same initial state, in argon at
+                    100 mTorr / 300 K: null-collision MCC (e-Ar elastic,
+                    11.55 eV excitation, 15.76 eV ionisation; Ar+-Ar isotropic
+                    elastic and charge exchange) and an electron-ion pair
+                    re-injected for every ion lost to a wall.
 * ``large_1024`` -- 1024x1024 cells (1025^2 nodes), 100 particles/cell/species
-                    (104.9 M each).
+                    (104.9 M each), collisionless.
+* ``argon_1024`` -- the 1024x1024 / 100 ppc grid with the argon background.
 """
 import math
 
@@ -23,6 +28,7 @@ ELECTRON_CHARGE2 = 2.566971e-38
 AR_MASS = 39.948 * AMU
 H_MASS = 1.00728 * AMU
 SEED = 0x5EED0001
+BOLTZMANN = 1.38066e-23
 
 
 def discharge(cells, ppc, length=0.01, te=2.0, ti=0.026, density=1e15, ion_mass=AR_MASS,
@@ -63,9 +69,25 @@ def plain(**kw):
     return discharge(64, 100, **kw)
 
 
+def with_argon(p, pressure_pa=13.332, temperature=300.0):
+    """Argon background (default 100 mTorr, 300 K), physical-mode collisions
+    and steady-state pair re-injection on ion loss."""
+    p.have_collisions = 1
+    p.gas = pic2d.GAS_ARGON
+    p.neutral_density = pressure_pa / (BOLTZMANN * temperature)
+    p.vtn = math.sqrt(BOLTZMANN * temperature / p.ion_mass)
+    p.wall = pic2d.WALL_STEADY
+    return p
+
+
 def argon_256(**kw):
-    """256x256 cells, 200 ppc (13.1 M particles per species)."""
-    return discharge(256, 200, **kw)
+    """256x256 cells, 200 ppc (13.1 M particles per species), argon 100 mTorr."""
+    return with_argon(discharge(256, 200, **kw))
+
+
+def argon_1024(**kw):
+    """1024x1024 cells, 100 ppc (104.9 M particles per species), argon 100 mTorr."""
+    return with_argon(discharge(1024, 100, **kw))
 
 
 def large_1024(**kw):
