@@ -66,6 +66,16 @@ def describe(name, p):
     }
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(p, e, i, steps, threads):
     """Time the oracle's FP64 reference-layout step loop on the host (with the
     FP64 physical-mode collisions and pair re-injection when the workload has
@@ -89,11 +99,30 @@ def cpu_baseline(p, e, i, steps, threads):
                             nid, Ex, Ex.copy(), grow=1.5)
         dt = time.perf_counter() - t0
         nbar = 0.5 * (len(ea) + len(ia) + len(r["e"]) + len(r["i"]))
-        return nbar * steps / dt, dt, 1
+        return nbar * steps / dt, dt, used
     t0 = time.perf_counter()
     orc.run_ref(cref, steps, ea, ia, e["id"], i["id"], Ex, Ex.copy())
     dt = time.perf_counter() - t0
     return (len(ea) + len(ia)) * steps / dt, dt, used
+
+
+def calibration_copy(nbytes):
+    """Three device-to-device copies of nbytes through the HIP runtime's blit
+    kernel: a known number of bytes read and written for scaling HBM byte
+    counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    for _ in range(3):
+        hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"calibration copy: 3 x {nbytes >> 20} MiB read + {nbytes >> 20} MiB written")
 
 
 def main():
@@ -105,6 +134,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=0, help="0: choose ~10-30 s of CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="end with three 256 MiB device copies (a reference point for HBM "
+                         "byte counters)")
     ap.add_argument("--mg", choices=["default", "fixed", "tol"], default="default",
                     help="override the workload's multigrid mode")
     args = ap.parse_args()
@@ -202,21 +234,29 @@ def main():
 
     if not args.no_cpu_baseline:
         n_all = len(e0["id"]) + len(i0["id"])
-        per = 120e-9 if p.have_collisions else 60e-9
-        steps = args.cpu_steps or max(1, min(50, int(round(20.0 / (n_all * per + 1e-3)))))
-        log("cpu baseline:", steps, "steps of the FP64 reference-layout oracle, 1 thread")
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        # size the sample from one timed step: about 12 s per thread setting
+        _, t_probe, _ = cpu_baseline(p, e0, i0, 1, 1)
+        steps = args.cpu_steps or max(1, min(50, int(12.0 / max(t_probe, 1e-3))))
+        log(f"cpu baseline: {steps} steps of the FP64 reference-layout oracle, 1 and "
+            f"{threads} threads")
         v1, t_1, _ = cpu_baseline(p, e0, i0, steps, 1)
+        vn, t_n, used = cpu_baseline(p, e0, i0, steps, threads)
+        what = "FP64 AoS reference layout" + (
+            " with FP64 MCC and pair re-injection" if p.have_collisions else "")
         out["cpu_baseline"] = {
-            "value": v1,
+            "value": vn,
             "unit": "particle-steps/s",
-            "cores": 1,
+            "cores": used,
             "kind": "port",
-            "sample": f"{steps} steps of the full workload ({n_all} particles), "
-                      f"FP64 AoS reference layout"
-                      f"{' with FP64 MCC and pair re-injection' if p.have_collisions else ''}"
-                      f", {t_1:.1f} s",
+            "sample": f"{steps} steps of the full workload ({n_all} particles), {what}, "
+                      f"OpenMP {used} threads {t_n:.1f} s; 1 thread {t_1:.1f} s",
+            "single_thread": {"value": v1, "cores": 1},
+            "host_cpu": cpu_model(),
         }
     print(json.dumps(out), flush=True)
+    if args.calibration_copy:
+        calibration_copy(256 << 20)
 
 
 if __name__ == "__main__":

@@ -226,57 +226,108 @@ static long append_sorted(orc_particle *dst, uint32_t *did, long n, long cap, co
     return n;
 }
 
+/* parallel helpers of orc_step.c (any split gives the serial result) */
+void orc_i_gather_push(orc_particle *p, long n, const double *Ex, const double *Ey,
+                       const orc_ref_config *c, double coef, double dt, double cw, double sw,
+                       double *scratch);
+void orc_i_deposit_par(const orc_particle *p, long n, double qc_dxdy, const orc_ref_config *c,
+                       double *q);
+
+typedef struct {
+    orc_particle *a, *b; /* ejected electron, new ion */
+    orc_kv *ka, *kb;
+    long n, cap;
+} orc_stage;
+
+static void stage_push(orc_stage *s, const orc_particle *pa, const orc_particle *pb,
+                       uint64_t key) {
+    if (s->n == s->cap) {
+        s->cap = s->cap ? 2 * s->cap : 1024;
+        s->a = (orc_particle *)realloc(s->a, sizeof(orc_particle) * s->cap);
+        s->b = (orc_particle *)realloc(s->b, sizeof(orc_particle) * s->cap);
+        s->ka = (orc_kv *)realloc(s->ka, sizeof(orc_kv) * s->cap);
+        s->kb = (orc_kv *)realloc(s->kb, sizeof(orc_kv) * s->cap);
+    }
+    s->a[s->n] = *pa;
+    s->b[s->n] = *pb;
+    s->ka[s->n].key = s->kb[s->n].key = key;
+    s->ka[s->n].src = s->kb[s->n].src = s->n;
+    s->n++;
+}
+
+/* merge a thread's staged pairs into the step's stage */
+static void stage_merge(orc_stage *dst, const orc_stage *src) {
+    for (long k = 0; k < src->n; k++) stage_push(dst, &src->a[k], &src->b[k], src->ka[k].key);
+}
+
+static void stage_free(orc_stage *s) {
+    free(s->a);
+    free(s->b);
+    free(s->ka);
+    free(s->kb);
+}
+
 int orc_ref_run_mcc(const orc_ref_config *c, const orc_mcc_params *me, const orc_mcc_params *mi,
                     int plop, double vte, double vti, int nsteps, long *ne, long cap_e,
                     orc_particle *e, uint32_t *eid, long *ni, long cap_i, orc_particle *ip,
                     uint32_t *iid, uint32_t *next_id, double *Ex, double *Ey, double *qe,
                     double *qi, double *rho, double *phi, long *events8) {
     int N = c->N;
-    long scap = 1 << 20;
-    orc_particle *se = (orc_particle *)malloc(sizeof(orc_particle) * scap);
-    orc_particle *si = (orc_particle *)malloc(sizeof(orc_particle) * scap);
-    orc_kv *ke = (orc_kv *)malloc(sizeof(orc_kv) * scap);
-    orc_kv *ki = (orc_kv *)malloc(sizeof(orc_kv) * scap);
     long capmax = cap_e > cap_i ? cap_e : cap_i;
-    double *Exp = (double *)malloc(sizeof(double) * (capmax + 1));
-    double *Eyp = (double *)malloc(sizeof(double) * (capmax + 1));
+    double *scratch = (double *)malloc(sizeof(double) * 2 * (capmax + 1));
     double L = c->L, dx = c->dx;
     for (int s = 0; s < nsteps; s++) {
         uint32_t step = (uint32_t)(c->start_step + s);
-        /* e_collisions (pic.cpp:1071-1075): products appended before the push */
-        long nse = 0, nsi = 0;
-        for (long k = 0; k < *ne; k++) {
-            double ch[6];
-            int ev = orc_collide_electron(me, &e[k].vx, &e[k].vy, &e[k].vz, eid[k], step, ch);
-            events8[ev]++;
-            if (ev == 3 && nse < scap) {
-                orc_particle a = e[k], b = e[k];
-                a.vx = ch[0], a.vy = ch[1], a.vz = ch[2];
-                b.vx = ch[3], b.vy = ch[4], b.vz = ch[5];
-                se[nse] = a;
-                ke[nse].key = eid[k];
-                ke[nse].src = nse;
-                si[nsi] = b;
-                ki[nsi].key = eid[k];
-                ki[nsi].src = nsi;
-                nse++, nsi++;
+        /* e_collisions (pic.cpp:1071-1075): products appended before the push,
+         * in parent-id order whatever the thread split */
+        orc_stage iz = {0};
+        long ev_e[8] = {0};
+#pragma omp parallel
+        {
+            orc_stage loc = {0};
+            long lev[8] = {0};
+#pragma omp for schedule(static)
+            for (long k = 0; k < *ne; k++) {
+                double ch[6];
+                int ev = orc_collide_electron(me, &e[k].vx, &e[k].vy, &e[k].vz, eid[k], step, ch);
+                lev[ev]++;
+                if (ev == 3) {
+                    orc_particle a = e[k], b = e[k];
+                    a.vx = ch[0], a.vy = ch[1], a.vz = ch[2];
+                    b.vx = ch[3], b.vy = ch[4], b.vz = ch[5];
+                    stage_push(&loc, &a, &b, eid[k]);
+                }
             }
+#pragma omp critical
+            {
+                stage_merge(&iz, &loc);
+                for (int m = 0; m < 8; m++) ev_e[m] += lev[m];
+            }
+            stage_free(&loc);
         }
-        *ni = append_sorted(ip, iid, *ni, cap_i, si, ki, nsi, &next_id[1]);
-        *ne = append_sorted(e, eid, *ne, cap_e, se, ke, nse, &next_id[0]);
-        orc_ref_gather(e, *ne, Ex, Ey, dx, dx, N, N, Exp, Eyp);
-        orc_ref_push(e, *ne, Exp, Eyp, dx, dx, N, N, c->coef_e, c->cos_a, c->sin_a, c->dt_e,
-                     c->cos_we, c->sin_we, c->bfield);
+        for (int m = 0; m < 8; m++) events8[m] += ev_e[m];
+        *ni = append_sorted(ip, iid, *ni, cap_i, iz.b, iz.kb, iz.n, &next_id[1]);
+        *ne = append_sorted(e, eid, *ne, cap_e, iz.a, iz.ka, iz.n, &next_id[0]);
+        stage_free(&iz);
+        orc_i_gather_push(e, *ne, Ex, Ey, c, c->coef_e, c->dt_e, c->cos_we, c->sin_we, scratch);
         if (step % c->dt_dti == 0) {
-            for (long k = 0; k < *ni; k++)
-                events8[orc_collide_ion(mi, &ip[k].vx, &ip[k].vy, &ip[k].vz, iid[k], step)]++;
-            orc_ref_gather(ip, *ni, Ex, Ey, dx, dx, N, N, Exp, Eyp);
-            orc_ref_push(ip, *ni, Exp, Eyp, dx, dx, N, N, c->coef_i, c->cos_a, c->sin_a, c->dt_i,
-                         c->cos_wi, c->sin_wi, c->bfield);
+            long ev_i[8] = {0};
+#pragma omp parallel
+            {
+                long lev[8] = {0};
+#pragma omp for schedule(static)
+                for (long k = 0; k < *ni; k++)
+                    lev[orc_collide_ion(mi, &ip[k].vx, &ip[k].vy, &ip[k].vz, iid[k], step)]++;
+#pragma omp critical
+                for (int m = 0; m < 8; m++) ev_i[m] += lev[m];
+            }
+            for (int m = 0; m < 8; m++) events8[m] += ev_i[m];
+            orc_i_gather_push(ip, *ni, Ex, Ey, c, c->coef_i, c->dt_i, c->cos_wi, c->sin_wi,
+                              scratch);
         }
         /* walls: x < 0 or x >= L leaves (physical mode); with plop every lost
          * ion re-injects an electron-ion pair (boundary.cpp:854-878 placement) */
-        nse = nsi = 0;
+        orc_stage pl = {0};
         long w = 0;
         for (long k = 0; k < *ni; k++) {
             int out = !(ip[k].x >= 0.0 && ip[k].x < L && ip[k].y >= 0.0 && ip[k].y < L);
@@ -286,7 +337,7 @@ int orc_ref_run_mcc(const orc_ref_config *c, const orc_mcc_params *me, const orc
                 w++;
                 continue;
             }
-            if (plop && nse < scap) {
+            if (plop) {
                 double u0[4], u1[4], u2[4];
                 uint32_t r0[4], r1[4], r2[4];
                 draw(mi, iid[k], step, 16u + RS_PLOP, u0, r0);
@@ -302,13 +353,7 @@ int orc_ref_run_mcc(const orc_ref_config *c, const orc_mcc_params *me, const orc
                              cos(6.2831853 * (double)orc_u01(wb[m]));
                 orc_particle pe = {tx * dx, ty * dx, vte * nrm[0], vte * nrm[1], vte * nrm[2], 1.0};
                 orc_particle pi = {tx * dx, ty * dx, vti * nrm[3], vti * nrm[4], vti * nrm[5], 1.0};
-                se[nse] = pe;
-                ke[nse].key = (1ull << 32) | iid[k];
-                ke[nse].src = nse;
-                si[nsi] = pi;
-                ki[nsi].key = (1ull << 32) | iid[k];
-                ki[nsi].src = nsi;
-                nse++, nsi++;
+                stage_push(&pl, &pe, &pi, (1ull << 32) | iid[k]);
             }
         }
         *ni = w;
@@ -322,26 +367,22 @@ int orc_ref_run_mcc(const orc_ref_config *c, const orc_mcc_params *me, const orc
             }
         }
         *ne = w;
-        *ni = append_sorted(ip, iid, *ni, cap_i, si, ki, nsi, &next_id[1]);
-        *ne = append_sorted(e, eid, *ne, cap_e, se, ke, nse, &next_id[0]);
-        memset(qi, 0, sizeof(double) * N * N);
-        memset(qe, 0, sizeof(double) * N * N);
-        orc_ref_deposit(ip, *ni, c->qc_dxdy_i, dx, dx, N, N, qi);
+        *ni = append_sorted(ip, iid, *ni, cap_i, pl.b, pl.kb, pl.n, &next_id[1]);
+        *ne = append_sorted(e, eid, *ne, cap_e, pl.a, pl.ka, pl.n, &next_id[0]);
+        stage_free(&pl);
+        orc_i_deposit_par(ip, *ni, c->qc_dxdy_i, c, qi);
         orc_ref_charge_postscale(qi, N, N);
-        orc_ref_deposit(e, *ne, c->qc_dxdy_e, dx, dx, N, N, qe);
+        orc_i_deposit_par(e, *ne, c->qc_dxdy_e, c, qe);
         orc_ref_charge_postscale(qe, N, N);
         orc_ref_grid_rho(qi, qe, N, rho);
         if (orc_ref_grid_potential(rho, N, dx, dx, c->eps, c->phi_left, c->phi_right, c->phi_bottom,
-                                   c->phi_top, (double)step, c->phi_period, c->ncycle, L, phi) != 0)
-            break;
+                                   c->phi_top, (double)step, c->phi_period, c->ncycle, L, phi) != 0) {
+            free(scratch);
+            return -1;
+        }
         orc_ref_grid_efield(phi, N, dx, Ex, Ey);
     }
-    free(se);
-    free(si);
-    free(ke);
-    free(ki);
-    free(Exp);
-    free(Eyp);
+    free(scratch);
     return 0;
 }
 

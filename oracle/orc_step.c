@@ -33,7 +33,7 @@ int orc_set_threads(int n) {
 
 /* parallel gather + push over chunks (the reference's kernels are per-particle
  * independent, so any split gives the serial result bit for bit) */
-static void ref_gather_push(orc_particle *p, long n, const double *Ex, const double *Ey,
+void orc_i_gather_push(orc_particle *p, long n, const double *Ex, const double *Ey,
                             const orc_ref_config *c, double coef, double dt, double cw, double sw,
                             double *scratch) {
     double *Exp = scratch, *Eyp = scratch + n;
@@ -52,7 +52,7 @@ static void ref_gather_push(orc_particle *p, long n, const double *Ex, const dou
 }
 
 /* deposit with per-thread private grids summed in thread order */
-static void ref_deposit_par(const orc_particle *p, long n, double qc_dxdy, const orc_ref_config *c,
+void orc_i_deposit_par(const orc_particle *p, long n, double qc_dxdy, const orc_ref_config *c,
                             double *q) {
     int N = c->N;
     memset(q, 0, sizeof(double) * N * N);
@@ -88,15 +88,15 @@ int orc_ref_run(const orc_ref_config *c, int nsteps, long *ne, orc_particle *e, 
     double *scratch = (double *)malloc(sizeof(double) * 2 * (cap > 0 ? cap : 1));
     for (int s = 0; s < nsteps; s++) {
         long step = c->start_step + s;
-        ref_gather_push(e, *ne, Ex, Ey, c, c->coef_e, c->dt_e, c->cos_we, c->sin_we, scratch);
+        orc_i_gather_push(e, *ne, Ex, Ey, c, c->coef_e, c->dt_e, c->cos_we, c->sin_we, scratch);
         if (step % c->dt_dti == 0) {
-            ref_gather_push(ip, *ni, Ex, Ey, c, c->coef_i, c->dt_i, c->cos_wi, c->sin_wi, scratch);
+            orc_i_gather_push(ip, *ni, Ex, Ey, c, c->coef_i, c->dt_i, c->cos_wi, c->sin_wi, scratch);
             *ni = orc_ref_absorb(ip, iid, *ni, c->dx, c->dy, N, N);
         }
         *ne = orc_ref_absorb(e, eid, *ne, c->dx, c->dy, N, N);
-        ref_deposit_par(ip, *ni, c->qc_dxdy_i, c, qi);
+        orc_i_deposit_par(ip, *ni, c->qc_dxdy_i, c, qi);
         orc_ref_charge_postscale(qi, N, N);
-        ref_deposit_par(e, *ne, c->qc_dxdy_e, c, qe);
+        orc_i_deposit_par(e, *ne, c->qc_dxdy_e, c, qe);
         orc_ref_charge_postscale(qe, N, N);
         orc_ref_grid_rho(qi, qe, N, rho);
         if (orc_ref_grid_potential(rho, N, c->dx, c->dy, c->eps, c->phi_left, c->phi_right,
