@@ -211,8 +211,10 @@ int pic_set_E32(pic_handle *h, const float *in);
 /* Event timing of the push pipeline (collide/gather/push/sort/deposit) and
  * of the field solve, on the handle's stream. */
 int pic_profile_enable(pic_handle *h, int32_t on);
-/* out[0..7]: steps timed, push-phase ms, scatter-kernel ms, field ms,
- * particles pushed (sum over timed steps), 0, 0, 0 */
+/* sums since the last read: out[0] step ms, [1] push-phase ms (collisions,
+ * gather, push, sort, deposit, commits), [2] k_push_scatter ms, [3] field
+ * solve ms, [4] k_push_count ms, [5] bin-scan ms, [6] 0, [7] k_push_scatter
+ * launches */
 int pic_profile_read(pic_handle *h, double *out8);
 /* per-step durations (ms) of the profiled steps since the last read, oldest
  * first; *n receives the count written (at most cap) */
