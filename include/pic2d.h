@@ -91,6 +91,10 @@ typedef struct pic_params {
     double magnetic_field, angle;       /* MAGNETIC_FIELD [T], ANGLE [rad]          */
     double phi_left, phi_right, phi_bottom, phi_top, phi_period;
     double mg_tol;            /* PIC_MG_TOL relative residual target                */
+    int32_t sort_interval;    /* particle passes per counting sort (1 = every step;
+                                 in between, particles are updated in place);
+                                 0 = chosen per species from its thermal
+                                 displacement per step                              */
 } pic_params;
 
 /* particleVec.h:26-35: the 48-byte AoS record of the reference */
@@ -211,11 +215,12 @@ int pic_set_E32(pic_handle *h, const float *in);
 /* Event timing of the push pipeline (collide/gather/push/sort/deposit) and
  * of the field solve, on the handle's stream. */
 int pic_profile_enable(pic_handle *h, int32_t on);
-/* sums since the last read: out[0] step ms, [1] push-phase ms (collisions,
- * gather, push, sort, deposit, commits), [2] k_push_scatter ms, [3] field
- * solve ms, [4] k_push_count ms, [5] bin-scan ms, [6] 0, [7] k_push_scatter
- * launches */
-int pic_profile_read(pic_handle *h, double *out8);
+/* sums since the last read, n entries written (at most 12): out[0] step ms,
+ * [1] push-phase ms (collisions, gather, push, sort, deposit, commits),
+ * [2] k_push_scatter ms, [3] field solve ms, [4] k_push_count ms, [5] bin-scan
+ * ms, [6] k_push_inplace ms, [7] k_push_scatter launches, [8] k_push_inplace
+ * launches, [9..11] 0 */
+int pic_profile_read(pic_handle *h, double *out, int32_t n);
 /* per-step durations (ms) of the profiled steps since the last read, oldest
  * first; *n receives the count written (at most cap) */
 int pic_profile_steps(pic_handle *h, double *out, int32_t cap, int32_t *n);

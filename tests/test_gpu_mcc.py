@@ -297,3 +297,25 @@ def test_ion_collisions_match_the_fp64_oracle_particle_by_particle():
         assert abs(int((ref[3] == k).sum()) - int(ev[k])) <= max(2, n // 20000), (k, ev)
     assert ev[4] > 1000 and ev[5] > 1000
     _compare(st, out, ref, vtn)
+
+
+def test_sort_interval_does_not_change_collisional_results():
+    out = {}
+    for k in (1, 3, 5):
+        p = collisional(cells=32, ppc=60, plop=True, te=10.0)
+        p.sigma_load_i = math.sqrt(Q * 2000.0 / MI)  # ions reach the walls: re-injection
+        p.sort_interval = k
+        with pic2d.Pic(p) as sim:
+            sim.advance(16)
+            out[k] = dict(e=sim.get_state(0), i=sim.get_state(1), phi=sim.get_phi(),
+                          ev0=sim.collision_counts(0), ev1=sim.collision_counts(1),
+                          le=sim.loss_counts(0), li=sim.loss_counts(1))
+    for k in (3, 5):
+        for s in ("e", "i"):
+            a = {kk: v[np.argsort(out[k][s]["id"])] for kk, v in out[k][s].items()}
+            b = {kk: v[np.argsort(out[1][s]["id"])] for kk, v in out[1][s].items()}
+            for kk in ("id", "cell", "ox", "oy", "vx", "vy", "vz"):
+                assert np.array_equal(a[kk], b[kk]), (k, s, kk)
+        for key in ("phi", "ev0", "ev1", "le", "li"):
+            assert np.array_equal(out[k][key], out[1][key]), (k, key)
+    assert out[1]["ev0"][3] > 0 and out[1]["li"][:4].sum() > 0

@@ -160,6 +160,7 @@ void prof_flush(pic_handle *h) {
         hipEventElapsedTime(&ms, p.a, p.b);
         h->prof_acc[p.tag] += ms;
         if (p.tag == 2) h->prof_acc[7] += 1.0;
+        if (p.tag == 6) h->prof_acc[8] += 1.0;
         if (p.tag == 0) h->prof_steps.push_back(ms);
         h->prof_pool.push_back(p);
     }
@@ -216,6 +217,16 @@ void setup_species(pic_handle *h, Species &s, bool electron) {
     pc.tilesX = h->tilesX;
     double coef = 0.5 * s.dt * s.q_over_m;  // particle.cpp:1103
     double omega_dt = s.dt * s.q_over_m * P.magnetic_field;
+    // sort interval: explicit, or chosen from the thermal displacement per
+    // step d = sigma dt / dx (cells): about 6 / (d + 0.5) passes, 1..8
+    // (measured best: 4 at d = 0.85, 2 at d = 3.4; slow ions stay in place)
+    if (P.sort_interval > 0) {
+        s.sort_interval = P.sort_interval;
+    } else {
+        double d = s.sigma_load * s.dt / h->dx;
+        int k = (int)std::lround(6.0 / (d + 0.5));
+        s.sort_interval = std::max(1, std::min(8, k));
+    }
     pc.cE = (float)coef;
     pc.dtdx = (float)(s.dt / h->dx);
     pc.dtdy = (float)(s.dt / h->dx);
@@ -398,6 +409,11 @@ int set_count(pic_handle *h, int si, int buf, uint32_t n) {
     return PIC_OK;
 }
 
+void clear_holes(pic_handle *h, int si) {
+    if (h->sp[si].holes) launch_fill<uint32_t>(h->st, &h->cnt[si].holes[0], CNT_SLOTS, 0u);
+    h->sp[si].holes = false;
+}
+
 int read_counters(pic_handle *h) {
     HIPCHK(h, hipMemcpyAsync(h->cnt_host, h->cnt, 2 * sizeof(SpCounters), hipMemcpyDeviceToHost,
                              h->st));
@@ -415,7 +431,7 @@ void push_species(pic_handle *h, int si, int do_push, int deposit) {
     Spawn spn = make_spawn(h, si == 1 && do_push && plop_on(h));
     int p1 = prof_begin(h, 4);
     launch_push_count(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, mc, step,
-                      h->binCount, h->cnt[si].lost, do_push);
+                      h->binCount, &h->cnt[si].lost[0][0], do_push);
     prof_end(h, p1);
     int p2 = prof_begin(h, 5);
     launch_scan(h->st, h->binCount, h->binStart, h->nbins_pad, h->scanPartial, &h->cnt[si].n[1 - c]);
@@ -423,9 +439,43 @@ void push_species(pic_handle *h, int si, int do_push, int deposit) {
     int p3 = prof_begin(h, 2);
     launch_push_scatter(h->st, s.buf[c], s.buf[1 - c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, mc,
                         step, h->binCount, h->binStart, s.q, do_push, deposit, spn,
-                        h->cnt[si].events, h->cnt[si].lost);
+                        &h->cnt[si].events[0][0], &h->cnt[si].lost[0][0]);
     prof_end(h, p3);
     s.cur = 1 - c;
+    clear_holes(h, si);
+    s.since_sort = 0;
+}
+
+// collisions + push + deposit of one species in place (no sort): removed
+// particles leave holes that the next sort pass compacts
+void push_species_inplace(pic_handle *h, int si) {
+    Species &s = h->sp[si];
+    int c = s.cur;
+    Spawn spn = make_spawn(h, si == 1 && plop_on(h));
+    int p = prof_begin(h, 6);
+    launch_push_inplace(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, s.mc,
+                        (uint32_t)h->step, s.q, 1, spn, &h->cnt[si].events[0][0],
+                        &h->cnt[si].lost[0][0], &h->cnt[si].holes[0]);
+    prof_end(h, p);
+    s.holes = true;
+    s.since_sort++;
+}
+
+// one timestep's particle pass of a species: a sort pass every
+// sort_interval-th pass, in-place passes in between
+void advance_species(pic_handle *h, int si) {
+    Species &s = h->sp[si];
+    const int k = s.sort_interval;
+    if (s.since_sort + 1 >= k)
+        push_species(h, si, 1, 1);
+    else
+        push_species_inplace(h, si);
+}
+
+// drop the holes of the current buffer (before host reads and the per-phase
+// entry points, which expect a compact store)
+void compact_species(pic_handle *h, int si) {
+    if (h->sp[si].holes) push_species(h, si, 0, 0);
 }
 
 void absorb_species(pic_handle *h, int si, bool plop) {
@@ -433,9 +483,10 @@ void absorb_species(pic_handle *h, int si, bool plop) {
     int c = s.cur;
     set_count(h, si, 1 - c, 0);
     launch_absorb(h->st, s.buf[c], s.buf[1 - c], &h->cnt[si].n[c], &h->cnt[si].n[1 - c], s.cap,
-                  h->N, s.pc.mode, h->cnt[si].lost, h->sp[1].mc, (uint32_t)h->step,
+                  h->N, s.pc.mode, &h->cnt[si].lost[0][0], h->sp[1].mc, (uint32_t)h->step,
                   make_spawn(h, plop));
     s.cur = 1 - c;
+    clear_holes(h, si);  // the compaction dropped them
 }
 
 int solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, double phiLY) {
@@ -476,7 +527,7 @@ int fused_step(pic_handle *h) {
     int pp = prof_begin(h, 1);
     launch_fill<unsigned long long>(h->st, h->sp[0].q, (long)N * N, 0ull);
     launch_fill<unsigned long long>(h->st, h->sp[1].q, (long)N * N, 0ull);
-    push_species(h, 0, 1, 1);
+    advance_species(h, 0);
     if (h->spawning) {
         // ionisation products: new ions join this step's ion pass (they were
         // appended before the ion push in the reference, pic.cpp:1071-1111);
@@ -485,7 +536,7 @@ int fused_step(pic_handle *h) {
         commit(h, 0, 0, 1);
     }
     if (h->step % h->P.dt_dti == 0) {
-        push_species(h, 1, 1, 1);
+        advance_species(h, 1);
     } else {
         Species &s = h->sp[1];
         launch_deposit(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], s.cap, N, s.pc.mode, s.q);
@@ -687,7 +738,8 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     // the loader's sort pass must not count as losses
     if (read_counters(h) != PIC_OK) return fail(PIC_ERR_DEVICE);
     for (int s = 0; s < 2; s++)
-        for (int w = 0; w < 6; w++) h->cnt_host[s].lost[w] = 0;
+        for (int k = 0; k < CNT_SLOTS; k++)
+            for (int w = 0; w < 6; w++) h->cnt_host[s].lost[k][w] = 0;
     if (hipMemcpy(h->cnt, h->cnt_host, 2 * sizeof(SpCounters), hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_global_error(PIC_ERR_DEVICE, "counter upload failed"));
     hipError_t e = hipStreamSynchronize(h->st);
@@ -758,6 +810,7 @@ int pic_sync(pic_handle *h) {
 // ------------------------------------------------------ per-phase entry
 int pic_species_gather(pic_handle *h, int32_t s) {
     CHECK(check_species(h, s));
+    compact_species(h, s);
     Species &sp = h->sp[s];
     if (!sp.exp_) {
         CHECK(dalloc(h, &sp.exp_, sp.cap));
@@ -771,6 +824,7 @@ int pic_species_gather(pic_handle *h, int32_t s) {
 
 int pic_species_move(pic_handle *h, int32_t s) {
     CHECK(check_species(h, s));
+    compact_species(h, s);
     Species &sp = h->sp[s];
     if (!sp.gathered)
         return seterr(h, PIC_ERR_ARG, "pic_species_move needs pic_species_gather first");
@@ -781,6 +835,7 @@ int pic_species_move(pic_handle *h, int32_t s) {
 
 int pic_species_collect_charge(pic_handle *h, int32_t s) {
     CHECK(check_species(h, s));
+    compact_species(h, s);
     Species &sp = h->sp[s];
     launch_fill<unsigned long long>(h->st, sp.q, (long)h->N * h->N, 0ull);
     launch_deposit(h->st, sp.buf[sp.cur], &h->cnt[s].n[sp.cur], sp.cap, h->N, sp.pc.mode, sp.q);
@@ -811,10 +866,12 @@ int pic_fields_grid_electric_field(pic_handle *h) {
 
 int pic_e_collisions(pic_handle *h) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    compact_species(h, 0);
+    compact_species(h, 1);
     Species &s = h->sp[0];
     if (!s.mc.on) return PIC_OK;
     launch_mcc(h->st, s.buf[s.cur], &h->cnt[0].n[s.cur], s.cap, s.pc, s.mc, (uint32_t)h->step,
-               make_spawn(h, false), h->cnt[0].events);
+               make_spawn(h, false), &h->cnt[0].events[0][0]);
     if (s.mc.mode == PIC_DEV_MODE_PHYS) {  // e_collisions.cpp:405-441 appends the products
         commit(h, 1, 1, 0);
         commit(h, 0, 0, 0);
@@ -824,10 +881,11 @@ int pic_e_collisions(pic_handle *h) {
 
 int pic_ion_collisions(pic_handle *h) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    compact_species(h, 1);
     Species &s = h->sp[1];
     if (!s.mc.on) return PIC_OK;
     launch_mcc(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], s.cap, s.pc, s.mc, (uint32_t)h->step,
-               make_spawn(h, false), h->cnt[1].events);
+               make_spawn(h, false), &h->cnt[1].events[0][0]);
     return PIC_OK;
 }
 
@@ -841,7 +899,10 @@ int pic_get_collision_counts(pic_handle *h, int32_t s, int64_t *out8) {
     CHECK(check_species(h, s));
     if (!out8) return seterr(h, PIC_ERR_ARG, "null output");
     CHECK(read_counters(h));
-    for (int k = 0; k < 8; k++) out8[k] = (int64_t)h->cnt_host[s].events[k];
+    for (int k = 0; k < 8; k++) {
+        out8[k] = 0;
+        for (int b = 0; b < CNT_SLOTS; b++) out8[k] += (int64_t)h->cnt_host[s].events[b][k];
+    }
     return PIC_OK;
 }
 
@@ -936,6 +997,7 @@ int pic_get_mg_stats(pic_handle *h, int32_t *last_cycles, int64_t *total_cycles,
 int pic_particle_size(pic_handle *h, int32_t s, int64_t *n) {
     CHECK(check_species(h, s));
     if (!n) return seterr(h, PIC_ERR_ARG, "null output");
+    compact_species(h, s);
     CHECK(read_counters(h));
     *n = h->cnt_host[s].n[h->sp[s].cur];
     return PIC_OK;
@@ -976,6 +1038,8 @@ int pic_set_state(pic_handle *h, int32_t s, const uint32_t *cell, const float *o
     HIPCHK(h, hipMemcpyAsync(b.vz, vz, bytes, hipMemcpyHostToDevice, h->st));
     HIPCHK(h, hipMemcpyAsync(b.id, id, bytes, hipMemcpyHostToDevice, h->st));
     set_count(h, s, sp.cur, (uint32_t)n);
+    clear_holes(h, s);
+    sp.since_sort = 1 << 20;  // unsorted upload: the next pass sorts
     // ids handed out later must not collide with the uploaded ones
     uint32_t top = 0;
     for (int64_t k = 0; k < n; k++) top = id[k] >= top ? id[k] + 1u : top;
@@ -1072,7 +1136,10 @@ int pic_get_loss_counts(pic_handle *h, int32_t s, int64_t *out4) {
     CHECK(check_species(h, s));
     if (!out4) return seterr(h, PIC_ERR_ARG, "null output");
     CHECK(read_counters(h));
-    for (int w = 0; w < 5; w++) out4[w] = h->cnt_host[s].lost[w];
+    for (int w = 0; w < 5; w++) {
+        out4[w] = 0;
+        for (int b = 0; b < CNT_SLOTS; b++) out4[w] += h->cnt_host[s].lost[b][w];
+    }
     return PIC_OK;
 }
 
@@ -1111,12 +1178,11 @@ int pic_profile_steps(pic_handle *h, double *out, int32_t cap, int32_t *n) {
     return PIC_OK;
 }
 
-int pic_profile_read(pic_handle *h, double *out8) {
-    if (!h || !out8) return set_global_error(PIC_ERR_ARG, "null argument");
+int pic_profile_read(pic_handle *h, double *out, int32_t n) {
+    if (!h || !out || n < 0) return set_global_error(PIC_ERR_ARG, "bad argument");
     prof_flush(h);
-    // out: [0] step ms, [1] push-phase ms, [2] scatter-kernel ms, [3] field ms,
-    //      [4] count-kernel ms, [5] scan ms, [6] unused, [7] scatter launches
-    for (int k = 0; k < 8; k++) out8[k] = h->prof_acc[k];
+    const int m = n < 12 ? n : 12;
+    for (int k = 0; k < m; k++) out[k] = h->prof_acc[k];
     for (double &v : h->prof_acc) v = 0.0;
     return PIC_OK;
 }

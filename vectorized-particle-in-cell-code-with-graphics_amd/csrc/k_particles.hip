@@ -120,13 +120,16 @@ __device__ __forceinline__ bool ref_mcc_inside(const PState &s, int N) {
            s.jy <= N - 1 && !(s.jy == -1 && s.oy == 0.0f);
 }
 
-// collision (if enabled) then push; returns the particle's status.  `ev` and
-// `ch` report the collision.  Both passes call this with identical inputs.
+// collision (if enabled and not already applied by block_collide) then push;
+// returns the particle's status.  `ev` and `ch` report the collision.  Both
+// passes call this with identical inputs.
 __device__ __forceinline__ int advance_one(PState &s, uint32_t id, const float2 *__restrict__ E,
                                            const PushConst &pc, const mcc::MccConst &mc,
-                                           uint32_t step, int do_push, int &ev, mcc::Child &ch) {
-    ev = mcc::EV_NONE;
-    if (mc.on) {
+                                           uint32_t step, int do_push, int &ev, mcc::Child &ch,
+                                           bool mcc_done = false) {
+    if (!mcc_done) ev = mcc::EV_NONE;
+    if (s.jx == CELL_HOLE) return ST_HOLE;
+    if (mc.on && !mcc_done) {
         if (mc.electron) {
             ev = mcc::collide_electron(mc, s.vx, s.vy, s.vz, id, step, ref_mcc_inside(s, pc.N), ch);
             if (EV_KIND(ev) == mcc::EV_IONIZE && mc.mode == PIC_DEV_MODE_REF) {
@@ -141,31 +144,120 @@ __device__ __forceinline__ int advance_one(PState &s, uint32_t id, const float2 
     return classify(s, pc.N, pc.mode);
 }
 
+
+// LDS shared by the block-level collision queue and the deposit window
+#define BLOCK_LDS_U64 2048
+
+// Block-level null-collision step (physical mode).  Every thread makes its
+// particle's candidate test (u < 1 - exp(-nu' dt)); the candidates -- about
+// 13% of the electrons in the argon discharge -- are packed into LDS so that
+// the collision kinematics run on dense lanes instead of on a few lanes of
+// every wavefront; each owner then reads its result back.  The draws are the
+// per-particle Philox streams of collide_electron / collide_ion, so the
+// result is the per-thread one bit for bit.  Every thread of the block must
+// call this (uniform condition: mc.on in physical mode).
+__device__ __forceinline__ void block_collide(bool active, PState &s, uint32_t id,
+                                              const mcc::MccConst &mc, uint32_t step, int &ev,
+                                              mcc::Child &ch, unsigned long long *lds) {
+    __shared__ uint32_t wcnt[4];
+    float *qvx = reinterpret_cast<float *>(lds);
+    float *qvy = qvx + 256, *qvz = qvx + 512;
+    uint32_t *qid = reinterpret_cast<uint32_t *>(qvx + 768);
+    int *qev = reinterpret_cast<int *>(qvx + 1024);
+    float *qc = qvx + 1280;  // 6 x 256 child components
+    bool cand = false;
+    if (active) {
+        uint32_t r[4];
+        philox4x32_10(id, step, (mc.electron ? 0u : 16u) + RS_MAIN, RNG_DOMAIN_MCC, mc.key0,
+                      mc.key1, r);
+        cand = u01(r[0]) < mc.p_null;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t bal = __ballot(cand);
+    const uint32_t rank = (uint32_t)__popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+    for (int k = 0; k < 4; k++) {
+        if (k < w) base += wcnt[k];
+        total += wcnt[k];
+    }
+    const uint32_t slot = base + rank;
+    if (cand) {
+        qvx[slot] = s.vx;
+        qvy[slot] = s.vy;
+        qvz[slot] = s.vz;
+        qid[slot] = id;
+    }
+    __syncthreads();
+    if (threadIdx.x < total) {
+        const uint32_t t = threadIdx.x;
+        float vx = qvx[t], vy = qvy[t], vz = qvz[t];
+        mcc::Child c{};
+        int e = mc.electron ? mcc::collide_electron(mc, vx, vy, vz, qid[t], step, true, c)
+                            : mcc::collide_ion(mc, vx, vy, vz, qid[t], step);
+        qvx[t] = vx;
+        qvy[t] = vy;
+        qvz[t] = vz;
+        qev[t] = e;
+        qc[t] = c.vx;
+        qc[256 + t] = c.vy;
+        qc[512 + t] = c.vz;
+        qc[768 + t] = c.ivx;
+        qc[1024 + t] = c.ivy;
+        qc[1280 + t] = c.ivz;
+    }
+    __syncthreads();
+    ev = mcc::EV_NONE;
+    if (cand) {
+        s.vx = qvx[slot];
+        s.vy = qvy[slot];
+        s.vz = qvz[slot];
+        ev = qev[slot];
+        ch.vx = qc[slot];
+        ch.vy = qc[256 + slot];
+        ch.vz = qc[512 + slot];
+        ch.ivx = qc[768 + slot];
+        ch.ivy = qc[1024 + slot];
+        ch.ivz = qc[1280 + slot];
+    }
+    __syncthreads();  // the buffer is reused (deposit window) after this
+}
+
 __global__ void __launch_bounds__(256)
     k_push_count(SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E,
                  PushConst pc, mcc::MccConst mc, uint32_t step, uint32_t *__restrict__ binCount,
                  uint32_t *__restrict__ lost6, int do_push) {
     __shared__ uint32_t sl[6];
+    __shared__ unsigned long long lds[BLOCK_LDS_U64];
     if (threadIdx.x < 6) sl[threadIdx.x] = 0;
     __syncthreads();
     uint32_t n = *nptr;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool kept = false;
     uint32_t key = 0;
+    const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
+    PState s{};
+    uint32_t pid = 0;
     if (i < n) {
-        PState s = load_state(sp, i, pc.bfield != 0 || mc.on);
-        int ev;
-        mcc::Child ch;
-        int st = advance_one(s, mc.on ? sp.id[i] : 0u, E, pc, mc, step, do_push, ev, ch);
+        s = load_state(sp, i, pc.bfield != 0 || mc.on);
+        pid = mc.on ? sp.id[i] : 0u;
+    }
+    int ev = mcc::EV_NONE;
+    mcc::Child ch{};
+    if (blk) block_collide(i < n && s.jx != CELL_HOLE, s, pid, mc, step, ev, ch, lds);
+    if (i < n) {
+        int st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, blk);
         kept = (st == ST_KEPT);
         if (kept)
             key = bin_key(s.jx, s.jy, pc.tilesX);
-        else
+        else if (st != ST_HOLE)
             atomicAdd(&sl[st - 1], 1u);
     }
     wave_count(binCount, key, kept);
     __syncthreads();
-    if (threadIdx.x < 6 && sl[threadIdx.x]) atomicAdd(&lost6[threadIdx.x], sl[threadIdx.x]);
+    if (threadIdx.x < 6 && sl[threadIdx.x])
+        atomicAdd(&lost6[cnt_slot() * 6 + threadIdx.x], sl[threadIdx.x]);
 }
 
 // ------------------------------------------------------------ scan
@@ -296,7 +388,7 @@ void launch_scan(hipStream_t st, const uint32_t *count, uint32_t *start, uint32_
 uint32_t scan_pad(uint32_t nbins) { return (nbins + SCAN_TILE - 1) / SCAN_TILE * SCAN_TILE; }
 
 // ------------------------------------------------------------ pass 2
-#define DEP_WCAP 2048  // LDS window capacity (nodes), 16 KB of uint64
+#define DEP_WCAP BLOCK_LDS_U64  // LDS window capacity (nodes), 16 KB of uint64
 #define DEP_MARGIN 2   // cells of motion allowed around the block's old cells
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -317,16 +409,23 @@ __device__ __forceinline__ void dep_global(unsigned long long *q, int N, int jx,
     atomicAdd(&q[b + N + 1], w.a1);
 }
 
-// Deposit the block's particles (dep[] true) through an LDS window spanning
-// the block's old cells +- DEP_MARGIN; particles landing outside the window,
-// or blocks whose window is too large, use global atomics directly.
-__device__ __forceinline__ void block_deposit(bool dep, const PState &s, bool valid, int ojx,
-                                              int ojy, unsigned long long *__restrict__ q, int N) {
-    __shared__ unsigned long long sq[DEP_WCAP];
+// Deposit the block's particles (dep true) through an LDS window spanning the
+// bounding box of their cells (block reduction), flushed with one 64-bit
+// atomic per touched node; a block whose box exceeds DEP_WCAP nodes deposits
+// with global atomics directly.  Sorted particles put most lanes of a
+// wavefront on the same four nodes, and LDS atomics to one address serialise,
+// so the window is replicated (up to DEP_COPIES copies, as many as fit) and
+// lane l adds into copy l mod copies; the flush sums the copies.  Integer
+// sums: the result does not depend on the split.  Every thread of the block
+// must call this.
+#define DEP_COPIES 8
+__device__ __forceinline__ void block_deposit(bool dep, const PState &s,
+                                              unsigned long long *__restrict__ q, int N,
+                                              unsigned long long *sq) {
     __shared__ int red[4][4];
-    int big = 1 << 30;
-    int mnx = valid ? max(0, min(ojx, N - 2)) : big, mny = valid ? max(0, min(ojy, N - 2)) : big;
-    int mxx = valid ? max(0, min(ojx, N - 2)) : -big, mxy = valid ? max(0, min(ojy, N - 2)) : -big;
+    const int big = 1 << 30;
+    int mnx = dep ? s.jx : big, mny = dep ? s.jy : big;
+    int mxx = dep ? s.jx : -big, mxy = dep ? s.jy : -big;
     mnx = wave_min(mnx);
     mny = wave_min(mny);
     mxx = wave_max(mxx);
@@ -350,32 +449,31 @@ __device__ __forceinline__ void block_deposit(bool dep, const PState &s, bool va
         mxx = max(mxx, red[k][2]);
         mxy = max(mxy, red[k][3]);
     }
-    if (mxx < mnx) return;  // block without particles (uniform: all threads)
-    int x0 = max(0, mnx - DEP_MARGIN), y0 = max(0, mny - DEP_MARGIN);
-    int x1 = min(N - 1, mxx + 1 + DEP_MARGIN), y1 = min(N - 1, mxy + 1 + DEP_MARGIN);
-    int ww = x1 - x0 + 1, wh = y1 - y0 + 1;
-    if (ww * wh > DEP_WCAP) {
+    __syncthreads();  // red[] is reused by the next call in the same kernel
+    if (mxx < mnx) return;  // nothing to deposit (uniform: all threads)
+    // a depositing particle has jx in [0, N-2]: its nodes jx, jx+1 <= N-1
+    const int x0 = mnx, y0 = mny, ww = mxx - mnx + 2, wh = mxy - mny + 2;
+    const int area = ww * wh;
+    if (area > DEP_WCAP) {
         if (dep) dep_global(q, N, s.jx, s.jy, fx_weights(s.ox, s.oy));
         return;
     }
-    for (int k = threadIdx.x; k < ww * wh; k += blockDim.x) sq[k] = 0ull;
+    int copies = DEP_COPIES;
+    while (copies > 1 && copies * area > DEP_WCAP) copies >>= 1;
+    for (int k = threadIdx.x; k < copies * area; k += blockDim.x) sq[k] = 0ull;
     __syncthreads();
     if (dep) {
         FxW fw = fx_weights(s.ox, s.oy);
-        int lx = s.jx - x0, ly = s.jy - y0;
-        if (lx >= 0 && ly >= 0 && lx + 1 < ww && ly + 1 < wh) {
-            int b = ly * ww + lx;
-            atomicAdd(&sq[b], fw.a4);
-            atomicAdd(&sq[b + 1], fw.a3);
-            atomicAdd(&sq[b + ww], fw.a2);
-            atomicAdd(&sq[b + ww + 1], fw.a1);
-        } else {
-            dep_global(q, N, s.jx, s.jy, fw);
-        }
+        int b = (int)(threadIdx.x & (copies - 1)) * area + (s.jy - y0) * ww + (s.jx - x0);
+        atomicAdd(&sq[b], fw.a4);
+        atomicAdd(&sq[b + 1], fw.a3);
+        atomicAdd(&sq[b + ww], fw.a2);
+        atomicAdd(&sq[b + ww + 1], fw.a1);
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < ww * wh; k += blockDim.x) {
-        unsigned long long v = sq[k];
+    for (int k = threadIdx.x; k < area; k += blockDim.x) {
+        unsigned long long v = 0ull;
+        for (int c = 0; c < copies; c++) v += sq[c * area + k];
         if (v) atomicAdd(&q[(long)(y0 + k / ww) * N + x0 + k % ww], v);
     }
 }
@@ -406,40 +504,18 @@ __device__ __forceinline__ void count_events(int ev, unsigned long long *__restr
     if (EV_KIND(ev) > 0) atomicAdd(&se[EV_KIND(ev)], 1u);
     if (ev & EV_OVER_BOUND) atomicAdd(&se[6], 1u);
     __syncthreads();
-    if (threadIdx.x < 8 && se[threadIdx.x]) atomicAdd(&events[threadIdx.x], (unsigned long long)se[threadIdx.x]);
+    if (threadIdx.x < 8 && se[threadIdx.x])
+        atomicAdd(&events[cnt_slot() * 8 + threadIdx.x], (unsigned long long)se[threadIdx.x]);
 }
 
-__global__ void __launch_bounds__(256)
-    k_push_scatter(SpView src, SpView dst, const uint32_t *__restrict__ nptr, uint32_t cap,
-                   const float2 *__restrict__ E, PushConst pc, mcc::MccConst mc, uint32_t step,
-                   uint32_t *__restrict__ binCount, const uint32_t *__restrict__ binStart,
-                   unsigned long long *__restrict__ q, int do_push, int deposit, Spawn sp,
-                   unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6) {
-    uint32_t n = *nptr;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = i < n;
-    PState s{}, pre{};
-    int ojx = 0, ojy = 0;
-    int st = -1, ev = mcc::EV_NONE;
-    mcc::Child ch{};
-    uint32_t key = 0, pid = 0;
-    if (valid) {
-        s = load_state(src, i, true);
-        pid = src.id[i];
-        ojx = s.jx;
-        ojy = s.jy;
-        pre = s;
-        st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch);
-        if (st == ST_KEPT) key = bin_key(s.jx, s.jy, pc.tilesX);
-    }
-    const bool kept = (st == ST_KEPT);
-    Claim claim = wave_claim_issue(binCount, binStart, key, kept);
-    // the deposit runs while the claim's atomic is in flight
-    if (deposit) block_deposit(kept && deposit_ok(s, pc.N, pc.mode), s, valid, ojx, ojy, q, pc.N);
-    uint32_t slot = wave_claim_slot(claim, kept);
-    // the bound test only keeps a corrupted histogram from writing out of range
-    if (kept && slot < cap) store_state(dst, slot, s, pid);
-    if (mc.on) count_events(ev, events);
+// Products of one particle's step: the ejected electron and the new ion of a
+// physical-mode ionisation, and the electron-ion pair re-injected for an ion
+// lost through a wall (sp.plop).  `pre` is the state before the push.
+__device__ __forceinline__ void stage_products(int ev, const mcc::Child &ch, const PState &pre,
+                                               int st, bool valid, uint32_t pid,
+                                               const float2 *__restrict__ E, const PushConst &pc,
+                                               const mcc::MccConst &mc, uint32_t step,
+                                               const Spawn &sp, uint32_t *__restrict__ lost6) {
     if (EV_KIND(ev) == mcc::EV_IONIZE && mc.mode == PIC_DEV_MODE_PHYS) {
         // ejected electron: born at the parent's position, pushed this step
         // as the reference pushes particles appended before move_particles
@@ -447,12 +523,12 @@ __global__ void __launch_bounds__(256)
         ce.vx = ch.vx;
         ce.vy = ch.vy;
         ce.vz = ch.vz;
-        if (do_push) push_one(ce, E, pc);
+        push_one(ce, E, pc);
         int cst = classify(ce, pc.N, pc.mode);
         if (cst != ST_KEPT) {
             // left the domain during its first push: it still takes its id
             // (as it does when created before the push), but is not stored
-            atomicAdd(&lost6[cst - 1], 1u);
+            atomicAdd(&lost6[cnt_slot() * 6 + cst - 1], 1u);
             ce.jx = ce.jy = CELL_DEAD;
         }
         stage_put(sp.se, &sp.ce->stage_count, &sp.ce->overflow, ce, (unsigned long long)pid);
@@ -470,6 +546,103 @@ __global__ void __launch_bounds__(256)
         stage_put(sp.se, &sp.ce->stage_count, &sp.ce->overflow, pe, k);
         stage_put(sp.si, &sp.ci->stage_count, &sp.ci->overflow, pi, k);
     }
+}
+
+__global__ void __launch_bounds__(256)
+    k_push_scatter(SpView src, SpView dst, const uint32_t *__restrict__ nptr, uint32_t cap,
+                   const float2 *__restrict__ E, PushConst pc, mcc::MccConst mc, uint32_t step,
+                   uint32_t *__restrict__ binCount, const uint32_t *__restrict__ binStart,
+                   unsigned long long *__restrict__ q, int do_push, int deposit, Spawn sp,
+                   unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6) {
+    uint32_t n = *nptr;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = i < n;
+    PState s{}, pre{};
+    int st = -1, ev = mcc::EV_NONE;
+    mcc::Child ch{};
+    uint32_t key = 0, pid = 0;
+    __shared__ unsigned long long lds[BLOCK_LDS_U64];
+    const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
+    if (valid) {
+        s = load_state(src, i, true);
+        pid = src.id[i];
+        pre = s;
+    }
+    if (blk) block_collide(valid && s.jx != CELL_HOLE, s, pid, mc, step, ev, ch, lds);
+    if (valid) {
+        st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, blk);
+        if (st == ST_KEPT) key = bin_key(s.jx, s.jy, pc.tilesX);
+    }
+    const bool kept = (st == ST_KEPT);
+    Claim claim = wave_claim_issue(binCount, binStart, key, kept);
+    // the deposit runs while the claim's atomic is in flight
+    if (deposit) block_deposit(kept && deposit_ok(s, pc.N, pc.mode), s, q, pc.N, lds);
+    uint32_t slot = wave_claim_slot(claim, kept);
+    // the bound test only keeps a corrupted histogram from writing out of range
+    if (kept && slot < cap) store_state(dst, slot, s, pid);
+    if (mc.on) count_events(ev, events);
+    stage_products(ev, ch, pre, st, valid, pid, E, pc, mc, step, sp, lost6);
+}
+
+
+// In-place timestep of one species between sort passes: the same collide +
+// push as the sort pass, the state written back to its own slot, the
+// deposit through the LDS window of the block's new cells.  A particle that
+// leaves (or that a collision removes) becomes a hole; the next sort pass
+// compacts the store.  Reads 20-28 B and writes 20-24 B per particle, all
+// coalesced, with no histogram or slot claims.
+__global__ void __launch_bounds__(256)
+    k_push_inplace(SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E,
+                   PushConst pc, mcc::MccConst mc, uint32_t step, unsigned long long *__restrict__ q,
+                   int deposit, Spawn spn, unsigned long long *__restrict__ events,
+                   uint32_t *__restrict__ lost6, uint32_t *__restrict__ holes) {
+    __shared__ uint32_t sl[6];
+    __shared__ uint32_t shole;
+    if (threadIdx.x < 6) sl[threadIdx.x] = 0;
+    if (threadIdx.x == 0) shole = 0;
+    __syncthreads();
+    const uint32_t n = *nptr;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const bool need_vz = pc.bfield != 0 || mc.on;
+    const bool need_id = mc.on || spn.plop;
+    PState s{}, pre{};
+    int st = ST_HOLE, ev = mcc::EV_NONE;
+    mcc::Child ch{};
+    uint32_t pid = 0;
+    __shared__ unsigned long long lds[BLOCK_LDS_U64];
+    const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
+    bool live = false;
+    if (valid) {
+        s = load_state(sp, i, need_vz);
+        live = s.jx != CELL_HOLE;
+        if (live) {
+            pid = need_id ? sp.id[i] : 0u;
+            pre = s;
+        }
+    }
+    if (blk) block_collide(live, s, pid, mc, step, ev, ch, lds);
+    if (live) st = advance_one(s, pid, E, pc, mc, step, 1, ev, ch, blk);
+    const bool kept = (st == ST_KEPT);
+    if (deposit) block_deposit(kept && deposit_ok(s, pc.N, pc.mode), s, q, pc.N, lds);
+    if (kept) {
+        sp.cell[i] = cell_pack(s.jx, s.jy);
+        sp.ox[i] = s.ox;
+        sp.oy[i] = s.oy;
+        sp.vx[i] = s.vx;
+        sp.vy[i] = s.vy;
+        if (need_vz) sp.vz[i] = s.vz;
+    } else if (valid && st != ST_HOLE) {
+        sp.cell[i] = cell_pack(CELL_HOLE, CELL_HOLE);
+        atomicAdd(&sl[st - 1], 1u);
+        atomicAdd(&shole, 1u);
+    }
+    if (mc.on) count_events(ev, events);
+    stage_products(ev, ch, pre, st, valid, pid, E, pc, mc, step, spn, lost6);
+    __syncthreads();
+    if (threadIdx.x < 6 && sl[threadIdx.x])
+        atomicAdd(&lost6[cnt_slot() * 6 + threadIdx.x], sl[threadIdx.x]);
+    if (threadIdx.x == 0 && shole) atomicAdd(&holes[cnt_slot()], shole);
 }
 
 // ------------------------------------------------------- per-phase path
@@ -510,7 +683,8 @@ __global__ void __launch_bounds__(256)
     PState s{};
     if (valid) s = load_state(sp, i, false);
     bool dep = valid && deposit_ok(s, N, mode);
-    block_deposit(dep, s, valid, s.jx, s.jy, q, N);
+    __shared__ unsigned long long lds[BLOCK_LDS_U64];
+    block_deposit(dep, s, q, N, lds);
 }
 
 // e_collisions::collision / ion_collisions::collision as a phase of its own
@@ -525,8 +699,8 @@ __global__ void __launch_bounds__(256)
     mcc::Child ch{};
     PState s{};
     uint32_t pid = 0;
-    if (i < n) {
-        s = load_state(sp, i, true);
+    if (i < n) s = load_state(sp, i, true);
+    if (i < n && s.jx != CELL_HOLE) {
         pid = sp.id[i];
         if (mc.electron)
             ev = mcc::collide_electron(mc, s.vx, s.vy, s.vz, pid, step, ref_mcc_inside(s, pc.N), ch);
@@ -574,7 +748,7 @@ __global__ void __launch_bounds__(256)
         pid = src.id[i];
         st = classify(s, N, mode);
         keep = (st == ST_KEPT);
-        if (!keep) atomicAdd(&sl[st - 1], 1u);
+        if (!keep && st != ST_HOLE) atomicAdd(&sl[st - 1], 1u);
     }
     // block-level compaction: one global atomic per block
     uint64_t bal = __ballot(keep);
@@ -592,7 +766,8 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
     if (keep) store_state(dst, bbase + wbase[w] + rank, s, pid);
-    if (threadIdx.x < 6 && sl[threadIdx.x]) atomicAdd(&lost6[threadIdx.x], sl[threadIdx.x]);
+    if (threadIdx.x < 6 && sl[threadIdx.x])
+        atomicAdd(&lost6[cnt_slot() * 6 + threadIdx.x], sl[threadIdx.x]);
     if (spn.plop && st >= ST_LEFT && st <= ST_TOP) {
         PState pe, pi;
         mcc::plop_pair(mc, pid, step, N, spn.se_sigma, spn.si_sigma, spn.plop_ref, pe, pi);
@@ -743,6 +918,14 @@ void launch_push_scatter(hipStream_t st, const SpView &src, const SpView &dst, c
     hipLaunchKernelGGL(k_push_scatter, dim3(blocks_for(cap, 256)), dim3(256), 0, st, src, dst, n,
                        (uint32_t)cap, E, pc, mc, step, binCount, binStart, q, do_push, deposit, spn,
                        events, lost6);
+}
+
+void launch_push_inplace(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                         const float2 *E, const PushConst &pc, const mcc::MccConst &mc,
+                         uint32_t step, unsigned long long *q, int deposit, const Spawn &spn,
+                         unsigned long long *events, uint32_t *lost6, uint32_t *holes) {
+    hipLaunchKernelGGL(k_push_inplace, dim3(blocks_for(cap, 256)), dim3(256), 0, st, sp, n, E, pc,
+                       mc, step, q, deposit, spn, events, lost6, holes);
 }
 
 void launch_mcc(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,

@@ -369,6 +369,7 @@ extern "C" int pic_params_default(pic_params *P) {
     P->mg_ncycle = 2;
     P->mg_max_iter = 30;
     P->mg_tol = 1e-8;
+    P->sort_interval = 0;
     P->number_to_inject = 0;
     P->no_ion_species = 1;
     P->corner8 = 1;
@@ -557,6 +558,7 @@ extern "C" int pic_params_load(const char *path, pic_params *P) {
     ok = ok && num("PIC_MG_TOL", P->mg_tol);
     ok = ok && inum("PIC_MG_NCYCLE", P->mg_ncycle);
     ok = ok && inum("PIC_MG_MAX_ITER", P->mg_max_iter);
+    ok = ok && inum("PIC_SORT_INTERVAL", P->sort_interval);
     ok = ok && num("PIC_SIGMA_LOAD_E", P->sigma_load_e);
     ok = ok && num("PIC_SIGMA_LOAD_I", P->sigma_load_i);
     if (!ok)

@@ -25,6 +25,9 @@ struct Species {
     PushConst pc = {};
     mcc::MccConst mc = {};
     pic::StageView stage = {};  // particles created during a step
+    int sort_interval = 1;  // particle passes per counting sort
+    int since_sort = 0;   // in-place steps since the last sort pass
+    bool holes = false;   // the current buffer may hold empty slots
     const char *type = "";  // dataDump suffix (parameters.h:316-318)
 };
 
@@ -62,6 +65,6 @@ struct pic_handle {
     // profiling
     bool prof = false;
     std::vector<pic_host::ProfPair> prof_live, prof_pool;
-    double prof_acc[8] = {};
+    double prof_acc[12] = {};
     std::vector<double> prof_steps;  // per-step durations (ms)
 };

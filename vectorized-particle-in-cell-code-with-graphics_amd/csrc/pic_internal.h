@@ -17,15 +17,24 @@ struct SpView {
 };
 
 // Device counters of one species.  n[b] is the particle count of buffer b.
+// Counters that every block flushes (losses, holes, collision events) are
+// spread over CNT_SLOTS copies, block b adding into copy b mod CNT_SLOTS:
+// atomics of all blocks on one address serialise at the memory side.  The
+// host sums the copies.
+#define CNT_SLOTS 64
 struct SpCounters {
     uint32_t n[2];
-    uint32_t lost[6];        // left, right, bottom, top, removed by collision, -
     uint32_t next_id;        // id of the next particle created
     uint32_t overflow;       // particles dropped for lack of capacity
     uint32_t stage_count;    // entries in the staging buffer
     uint32_t pad;
-    unsigned long long events[8];  // collision events by mcc::Event
+    // left, right, bottom, top, removed by collision, -
+    uint32_t lost[CNT_SLOTS][6];
+    uint32_t holes[CNT_SLOTS];  // empty slots left by in-place steps (current buffer)
+    unsigned long long events[CNT_SLOTS][8];  // collision events by mcc::Event
 };
+
+__device__ __forceinline__ uint32_t cnt_slot() { return blockIdx.x & (CNT_SLOTS - 1); }
 
 // Particles created during a step (ionisation products, re-injected pairs)
 // wait in a staging buffer; a commit pass sorts them by their creator's key
@@ -80,6 +89,10 @@ void launch_push_scatter(hipStream_t st, const SpView &src, const SpView &dst, c
                          uint32_t step, uint32_t *binCount, const uint32_t *binStart,
                          unsigned long long *q, int do_push, int deposit, const Spawn &spn,
                          unsigned long long *events, uint32_t *lost6);
+void launch_push_inplace(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                         const float2 *E, const PushConst &pc, const mcc::MccConst &mc,
+                         uint32_t step, unsigned long long *q, int deposit, const Spawn &spn,
+                         unsigned long long *events, uint32_t *lost6, uint32_t *holes);
 void launch_mcc(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
                 const PushConst &pc, const mcc::MccConst &mc, uint32_t step, const Spawn &spn,
                 unsigned long long *events);

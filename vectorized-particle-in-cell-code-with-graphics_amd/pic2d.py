@@ -101,6 +101,7 @@ class Params(C.Structure):
         ("phi_top", C.c_double),
         ("phi_period", C.c_double),
         ("mg_tol", C.c_double),
+        ("sort_interval", C.c_int32),
     ]
 
     @property
@@ -169,7 +170,7 @@ def load_library(path=LIB_PATH):
         "pic_get_E32": [vp, vp],
         "pic_set_E32": [vp, vp],
         "pic_profile_enable": [vp, i32],
-        "pic_profile_read": [vp, vp],
+        "pic_profile_read": [vp, vp, C.c_int32],
         "pic_profile_steps": [vp, vp, i32, P(i32)],
         "pic_advance": [vp, i32],
         "pic_poisson_tol": [vp, vp, i32, dbl, dbl, i32, P(i32), P(dbl)],
@@ -414,9 +415,10 @@ class Pic:
 
     def profile_read(self):
         """[step ms, push-phase ms, scatter-kernel ms, field ms, count-kernel ms,
-        scan ms, -, scatter launches] summed over the profiled steps"""
-        a = np.zeros(8)
-        self._c(self.lib.pic_profile_read(self.h, _ptr(a)))
+        scan ms, in-place-kernel ms, scatter launches, in-place launches, 0, 0, 0]
+        summed over the profiled steps"""
+        a = np.zeros(12)
+        self._c(self.lib.pic_profile_read(self.h, _ptr(a), len(a)))
         return a
 
     def profile_steps(self, cap=1 << 16):
