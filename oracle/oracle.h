@@ -101,6 +101,9 @@ long orc_ref_absorb(orc_particle *p, uint32_t *id, long n, double dx, double dy,
 /* ------------------------------------------------------------------------ */
 #define ORC_MODE_REF 0        /* reference wall / frozen-particle semantics  */
 #define ORC_MODE_PHYS 1       /* particles leave at x<0 or x>=L              */
+#define ORC_MODE_REF_STEADY 2 /* REF with steady-state walls: nobody leaves;
+                                 outside electrons and ions are paired and
+                                 re-injected (boundary.cpp:890-1097)          */
 
 #define ORC_ST_KEPT 0
 #define ORC_ST_LOST_LEFT 1
@@ -185,16 +188,31 @@ typedef struct {
     double scale_e, scale_i;
     long start_step;
     orc32_push_params pe, pi;
+    uint32_t key0, key1;       /* Philox key (seed) of the re-injection draws */
+    float steady_se, steady_si; /* re-injection velocity scales (VTE, VTI)   */
 } orc32_config;
 
 /* Build-layout step loop: push + wall pass (stable compaction, per-wall loss
- * counts into lost8[0..3] electrons, [4..7] ions), fixed-point deposit,
- * rho/rhs, multigrid, phi boundary overwrite, packed fp32 E. */
+ * counts into lost8[0..3] electrons, [4..7] ions; in ORC_MODE_REF_STEADY the
+ * pairing pass of orc32_steady instead), fixed-point deposit, rho/rhs,
+ * multigrid, phi boundary overwrite, packed fp32 E. */
 int orc32_run(const orc32_config *c, int nsteps, long *ne, uint32_t *ecell, float *eox, float *eoy,
               float *evx, float *evy, float *evz, uint32_t *eid, long *ni, uint32_t *icell,
               float *iox, float *ioy, float *ivx, float *ivy, float *ivz, uint32_t *iid,
               float *E32, int64_t *qe, int64_t *qi, double *rho, double *phi, long *lost8,
               int *mg_cycles_out);
+
+/* Reference steady-state electron wall (boundary.cpp:890-1097) on the build
+ * layout: per side (left x<0, right (int)(x/dx)>=N, then bottom, top among
+ * the particles the x pass left outside) the m-th electron outside, in id
+ * order, pairs with the m-th ion outside; both move to one point uniform in
+ * [dx, L-dx]^2 with velocities v = vt cos(2 pi R2) sqrt|ln R1| (boundary.cpp:
+ * 833-878), draws keyed by the ion's id (counter (id, step, 16+3..5,
+ * 0x4D434331)).  Returns the number of pairs. */
+long orc32_steady(long ne, uint32_t *ecell, float *eox, float *eoy, float *evx, float *evy,
+                  float *evz, const uint32_t *eid, long ni, uint32_t *icell, float *iox,
+                  float *ioy, float *ivx, float *ivy, float *ivz, const uint32_t *iid, int N,
+                  uint32_t step, uint32_t key0, uint32_t key1, float se, float si);
 
 /* ------------------------------------------------------------------------ */
 /* Collision cross sections [1e-20 m^2], energies in eV (orc_collisions.c)    */

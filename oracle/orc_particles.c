@@ -246,7 +246,9 @@ void orc32_push(long n, uint32_t *cell, float *ox, float *oy, float *vx, float *
             cell[g] = orc_cell_pack(jx, jy);
         }
         int lx, ly;
-        if (pp->mode == ORC_MODE_PHYS) {
+        if (pp->mode == ORC_MODE_REF_STEADY) {
+            lx = ly = 0;
+        } else if (pp->mode == ORC_MODE_PHYS) {
             lx = (jx < 0) || (jx > N - 2);
             ly = (jy < 0) || (jy > N - 2);
         } else {

@@ -152,7 +152,9 @@ static void classify32(const sp32 *s, int N, int mode, uint8_t *st) {
         int jx = orc_cell_jx(s->cell[g]), jy = orc_cell_jy(s->cell[g]);
         float fx = s->ox[g], fy = s->oy[g];
         int lx, ly;
-        if (mode == ORC_MODE_PHYS) {
+        if (mode == ORC_MODE_REF_STEADY) {
+            lx = ly = 0;
+        } else if (mode == ORC_MODE_PHYS) {
             lx = (jx < 0) || (jx > N - 2);
             ly = (jy < 0) || (jy > N - 2);
         } else {
@@ -176,11 +178,18 @@ int orc32_run(const orc32_config *c, int nsteps, long *ne, uint32_t *ecell, floa
     uint8_t *st = (uint8_t *)malloc((size_t)cap);
     double *rhs = (double *)malloc(sizeof(double) * N * N);
     int rc = 0;
+    const int steady = c->mode == ORC_MODE_REF_STEADY;
     if (c->preloop) {
-        classify32(&si, N, c->mode, st);
-        compact32(&si, st, lost8 + 4);
-        classify32(&se, N, c->mode, st);
-        compact32(&se, st, lost8);
+        if (steady) {
+            orc32_steady(*ne, ecell, eox, eoy, evx, evy, evz, eid, *ni, icell, iox, ioy, ivx, ivy,
+                         ivz, iid, N, (uint32_t)c->start_step, c->key0, c->key1, c->steady_se,
+                         c->steady_si);
+        } else {
+            classify32(&si, N, c->mode, st);
+            compact32(&si, st, lost8 + 4);
+            classify32(&se, N, c->mode, st);
+            compact32(&se, st, lost8);
+        }
     }
     for (int s = 0; s < nsteps; s++) {
         long step = c->start_step + s;
@@ -190,6 +199,10 @@ int orc32_run(const orc32_config *c, int nsteps, long *ne, uint32_t *ecell, floa
             orc32_push(*ni, icell, iox, ioy, ivx, ivy, ivz, E32, &c->pi, st);
             compact32(&si, st, lost8 + 4);
         }
+        if (steady)
+            orc32_steady(*ne, ecell, eox, eoy, evx, evy, evz, eid, *ni, icell, iox, ioy, ivx, ivy,
+                         ivz, iid, N, (uint32_t)step, c->key0, c->key1, c->steady_se,
+                         c->steady_si);
         memset(qe, 0, sizeof(int64_t) * N * N);
         memset(qi, 0, sizeof(int64_t) * N * N);
         orc32_deposit(*ni, icell, iox, ioy, N, c->mode, qi);
@@ -226,4 +239,105 @@ int orc32_run(const orc32_config *c, int nsteps, long *ne, uint32_t *ecell, floa
     free(st);
     free(rhs);
     return rc;
+}
+
+/* ------------------------------------------- reference steady-state walls */
+
+typedef struct {
+    int side;
+    uint32_t id;
+    long idx;
+} orc_cand;
+
+static int cand_cmp(const void *a, const void *b) {
+    const orc_cand *x = (const orc_cand *)a, *y = (const orc_cand *)b;
+    if (x->side != y->side) return x->side < y->side ? -1 : 1;
+    return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+/* candidates of one species sorted by (side, id); sides as boundary.cpp:906-912 */
+static long collect_cands(long n, const uint32_t *cell, const uint32_t *id, int N, orc_cand *out) {
+    long m = 0;
+    for (long k = 0; k < n; k++) {
+        int jx = orc_cell_jx(cell[k]), jy = orc_cell_jy(cell[k]);
+        if (jx < 0) out[m++] = (orc_cand){0, id[k], k};
+        else if (jx >= N) out[m++] = (orc_cand){1, id[k], k};
+        if (jy < 0) out[m++] = (orc_cand){2, id[k], k};
+        else if (jy >= N) out[m++] = (orc_cand){3, id[k], k};
+    }
+    qsort(out, (size_t)m, sizeof(orc_cand), cand_cmp);
+    return m;
+}
+
+static void plop32(uint32_t ion_id, uint32_t step, int N, uint32_t key0, uint32_t key1, int *jx,
+                   int *jy, float *ox, float *oy, float nrm[6]) {
+    const uint32_t key[2] = {key0, key1};
+    uint32_t r[4], a[4], b[4];
+    uint32_t c0[4] = {ion_id, step, 16u + 3u, 0x4D434331u};
+    uint32_t c1[4] = {ion_id, step, 16u + 4u, 0x4D434331u};
+    uint32_t c2[4] = {ion_id, step, 16u + 5u, 0x4D434331u};
+    orc_philox4x32_10(c0, key, r);
+    orc_philox4x32_10(c1, key, a);
+    orc_philox4x32_10(c2, key, b);
+    float span = (float)(N - 3);
+    float tx = 1.0f + orc_u01(r[0]) * span, ty = 1.0f + orc_u01(r[1]) * span;
+    int ix = (int)floorf(tx), iy = (int)floorf(ty);
+    float fx = tx - (float)ix, fy = ty - (float)iy;
+    if (fx >= 1.0f) { fx = 0.0f; ix += 1; }
+    if (fy >= 1.0f) { fy = 0.0f; iy += 1; }
+    *jx = ix;
+    *jy = iy;
+    *ox = fx;
+    *oy = fy;
+    uint32_t w[6] = {a[0], a[1], a[2], a[3], b[0], b[1]};
+    uint32_t w2[6] = {b[2], b[3], r[2], r[3], a[1] ^ b[2], a[3] ^ b[0]};
+    for (int k = 0; k < 6; k++) {
+        float R1 = orc_u01(w[k]), R2 = orc_u01(w2[k]);
+        nrm[k] = cosf(6.2831853f * R2) * sqrtf(fabsf(logf(R1)));
+    }
+}
+
+long orc32_steady(long ne, uint32_t *ecell, float *eox, float *eoy, float *evx, float *evy,
+                  float *evz, const uint32_t *eid, long ni, uint32_t *icell, float *iox,
+                  float *ioy, float *ivx, float *ivy, float *ivz, const uint32_t *iid, int N,
+                  uint32_t step, uint32_t key0, uint32_t key1, float se, float si) {
+    orc_cand *ce = (orc_cand *)malloc(sizeof(orc_cand) * (2 * ne + 1));
+    orc_cand *ci = (orc_cand *)malloc(sizeof(orc_cand) * (2 * ni + 1));
+    uint8_t *pe = (uint8_t *)calloc((size_t)ne + 1, 1), *pi = (uint8_t *)calloc((size_t)ni + 1, 1);
+    long me = collect_cands(ne, ecell, eid, N, ce), mi = collect_cands(ni, icell, iid, N, ci);
+    long pairs = 0;
+    for (int side = 0; side < 4; side++) {
+        /* entries of this side still outside (y sides: not re-injected by x) */
+        long ke = 0, ki = 0;
+        long *le = (long *)malloc(sizeof(long) * (me + 1)), *li = (long *)malloc(sizeof(long) * (mi + 1));
+        for (long k = 0; k < me; k++)
+            if (ce[k].side == side && !pe[ce[k].idx]) le[ke++] = ce[k].idx;
+        for (long k = 0; k < mi; k++)
+            if (ci[k].side == side && !pi[ci[k].idx]) li[ki++] = ci[k].idx;
+        long m = ke < ki ? ke : ki;
+        for (long t = 0; t < m; t++) {
+            long a = le[t], b = li[t];
+            int jx, jy;
+            float ox, oy, nrm[6];
+            plop32(iid[b], step, N, key0, key1, &jx, &jy, &ox, &oy, nrm);
+            ecell[a] = icell[b] = orc_cell_pack(jx, jy);
+            eox[a] = iox[b] = ox;
+            eoy[a] = ioy[b] = oy;
+            evx[a] = se * nrm[0];
+            evy[a] = se * nrm[1];
+            evz[a] = se * nrm[2];
+            ivx[b] = si * nrm[3];
+            ivy[b] = si * nrm[4];
+            ivz[b] = si * nrm[5];
+            pe[a] = pi[b] = 1;
+        }
+        pairs += m;
+        free(le);
+        free(li);
+    }
+    free(ce);
+    free(ci);
+    free(pe);
+    free(pi);
+    return pairs;
 }

@@ -13,7 +13,7 @@ ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_MGLIN_SO = os.path.join(ROOT, "oracle", "_ref", "libref_mglin.so")
 
 vp = C.c_void_p
-MODE_REF, MODE_PHYS = 0, 1
+MODE_REF, MODE_PHYS, MODE_REF_STEADY = 0, 1, 2
 ST_KEPT, ST_LEFT, ST_RIGHT, ST_BOTTOM, ST_TOP = range(5)
 
 PARTICLE_DTYPE = np.dtype(
@@ -34,7 +34,9 @@ class Cfg32(C.Structure):
                 ("eps", C.c_double), ("mg_tol", C.c_double), ("phi_left", C.c_double),
                 ("phi_right", C.c_double), ("phi_bottom", C.c_double), ("phi_top", C.c_double),
                 ("phi_period", C.c_double), ("scale_e", C.c_double), ("scale_i", C.c_double),
-                ("start_step", C.c_long), ("pe", PushParams), ("pi", PushParams)]
+                ("start_step", C.c_long), ("pe", PushParams), ("pi", PushParams),
+                ("key0", C.c_uint32), ("key1", C.c_uint32), ("steady_se", C.c_float),
+                ("steady_si", C.c_float)]
 
 
 class CfgRef(C.Structure):
@@ -97,6 +99,9 @@ def lib():
                                          C.POINTER(MccParams), C.c_int, C.c_double, C.c_double,
                                          C.c_int, vp, C.c_long, vp, vp, vp, C.c_long, vp, vp, vp,
                                          vp, vp, vp, vp, vp, vp, vp]
+        _lib.orc32_steady.argtypes = [C.c_long] + [vp] * 7 + [C.c_long] + [vp] * 7 + \
+            [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
+        _lib.orc32_steady.restype = C.c_long
         _lib.orc_dump_write.argtypes = [C.c_char_p, vp, C.c_long, C.c_int]
         _lib.orc_dump_read.argtypes = [C.c_char_p, vp, C.c_long]
         _lib.orc_dump_read.restype = C.c_long
@@ -249,7 +254,10 @@ def configs_from_params(p, mode=None):
     import math
     N = p.nodes
     dx = p.length / (N - 1)
-    mode = p.mode if mode is None else mode
+    if mode is None:
+        mode = p.mode
+        if p.mode == MODE_REF and getattr(p, "wall", 0) == 1:   # reference steady-state walls
+            mode = MODE_REF_STEADY
 
     def pp(q_over_m, dt):
         coef = 0.5 * dt * q_over_m
@@ -266,7 +274,9 @@ def configs_from_params(p, mode=None):
                 phi_bottom=p.phi_bottom, phi_top=p.phi_top, phi_period=p.phi_period,
                 scale_e=math.ldexp(p.qce / (dx * dx) * 1.0, -40),
                 scale_i=math.ldexp(p.qci / (dx * dx) * 1.0, -40), start_step=0,
-                pe=pp(qme, p.dt), pi=pp(qmi, p.dti))
+                pe=pp(qme, p.dt), pi=pp(qmi, p.dti), key0=getattr(p, "seed", 0) & 0xFFFFFFFF,
+                key1=(getattr(p, "seed", 0) >> 32) & 0xFFFFFFFF, steady_se=getattr(p, "vte", 0.0),
+                steady_si=getattr(p, "vti", 0.0))
     we = p.dt * qme * p.magnetic_field
     wi = p.dti * qmi * p.magnetic_field
     cref = CfgRef(N=N, ncycle=p.mg_ncycle, dt_dti=p.dt_dti, L=p.length, dx=dx, dy=dx,

@@ -196,3 +196,44 @@ def test_sort_interval_does_not_change_results(mode):
         for key in ("qe", "qi", "phi", "le", "li"):
             assert np.array_equal(out[k][key], out[1][key]), (k, key)
     assert out[1]["le"][:4].sum() > 0
+
+
+def test_reference_steady_walls_match_the_oracle():
+    """REF mode with the steady-state walls: nobody leaves; each step the
+    outside electrons and ions are paired per side in id order and
+    re-injected.  Cells, offsets, ids and charge are bit-exact against the
+    oracle after each step run from the same state; the re-injected
+    velocities (fp32 cos/log of the device and of the host libm) agree to
+    a few ulp."""
+    p = workloads.discharge(32, 40, mode=pic2d.MODE_REF, mg_mode=pic2d.MG_FIXED, te=8.0)
+    p.wall = pic2d.WALL_STEADY
+    p.sigma_load_i = 3.0e5   # hot ions: ions leave too, so pairs form on every side
+    with pic2d.Pic(p) as sim:
+        c32, _ = orc.configs_from_params(p)
+        assert c32.mode == orc.MODE_REF_STEADY
+        pairs = 0
+        for step in range(3):
+            e0, i0, E0 = sim.get_state(0), sim.get_state(1), sim.get_E32()
+            c32.preloop = 1 if step == 0 else 0
+            c32.start_step = step
+            r = orc.run32(c32, 1, e0, i0, E0)
+            if step == 0:
+                sim.run(1)
+            else:
+                sim.advance(1)
+            e1, i1 = by_id(sim.get_state(0)), by_id(sim.get_state(1))
+            re, ri = by_id(r["e"]), by_id(r["i"])
+            for a, b, what in ((e1, re, "e"), (i1, ri, "i")):
+                assert np.array_equal(a["id"], b["id"]), what
+                assert len(a["id"]) == p.init_particle_no          # counts are constant
+                for k in ("cell", "ox", "oy"):
+                    assert np.array_equal(a[k], b[k]), (what, k, step)
+                for k in ("vx", "vy", "vz"):
+                    np.testing.assert_allclose(a[k], b[k], rtol=4e-7, atol=0,
+                                               err_msg=f"{what}.{k} step {step}")
+            assert np.array_equal(sim.get_qfixed(0), r["qe"])
+            assert np.array_equal(sim.get_qfixed(1), r["qi"])
+            moved = (i1["cell"] != by_id(i0)["cell"]) & (np.abs(i1["vx"]) > 0)
+            pairs += int(moved.sum())
+        assert pairs > 0
+        assert sim.loss_counts(0).sum() == 0 and sim.loss_counts(1).sum() == 0

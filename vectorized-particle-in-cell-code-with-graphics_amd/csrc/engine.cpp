@@ -212,7 +212,9 @@ void setup_species(pic_handle *h, Species &s, bool electron) {
     s.weight = 1.0;
     PushConst &pc = s.pc;
     pc.N = h->N;
-    pc.mode = P.mode == PIC_MODE_PHYSICAL ? PIC_DEV_MODE_PHYS : PIC_DEV_MODE_REF;
+    pc.mode = P.mode == PIC_MODE_PHYSICAL
+                  ? PIC_DEV_MODE_PHYS
+                  : (P.wall == PIC_WALL_STEADY ? PIC_DEV_MODE_REF_STEADY : PIC_DEV_MODE_REF);
     pc.bfield = P.magnetic_field > 0.0;
     pc.tilesX = h->tilesX;
     double coef = 0.5 * s.dt * s.q_over_m;  // particle.cpp:1103
@@ -345,6 +347,21 @@ Spawn make_spawn(pic_handle *h, bool plop) {
 
 bool plop_on(pic_handle *h) {
     return h->P.wall == PIC_WALL_STEADY && h->P.mode == PIC_MODE_PHYSICAL;
+}
+
+// the reference's steady-state walls (pairing of outside electrons and ions)
+bool ref_steady(pic_handle *h) {
+    return h->P.wall == PIC_WALL_STEADY && h->P.mode == PIC_MODE_REF;
+}
+
+// steady_e_boundary (boundary.cpp:1094-1097) on the current buffers; the
+// re-injected pairs are deposited into this step's charge
+void steady_boundary(pic_handle *h) {
+    Species &e = h->sp[0], &i = h->sp[1];
+    launch_steady(h->st, e.buf[e.cur], &h->cnt[0].n[e.cur], e.cap, i.buf[i.cur],
+                  &h->cnt[1].n[i.cur], i.cap, e.stage, i.stage, &h->cnt[0], &h->cnt[1], i.mc,
+                  (uint32_t)h->step, h->N, (float)h->P.vte, (float)h->P.vti, e.q, i.q,
+                  e.pc.mode);
 }
 
 // append species `src`'s staged particles to species `dst_s`'s current buffer
@@ -545,6 +562,7 @@ int fused_step(pic_handle *h) {
         commit(h, 1, 1, 1);
         commit(h, 0, 0, 1);
     }
+    if (ref_steady(h)) steady_boundary(h);
     prof_end(h, pp);
     CHECK(field_step(h, h->step));
     prof_end(h, pt);
@@ -602,9 +620,6 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         return set_global_error(PIC_ERR_UNSUPPORTED, "NUMBER_TO_INJECT != 0 is not supported");
     if (P.no_ion_species != 1)
         return set_global_error(PIC_ERR_UNSUPPORTED, "NO_ION_SPECIES must be 1");
-    if (P.wall == PIC_WALL_STEADY && P.mode == PIC_MODE_REF)
-        return set_global_error(PIC_ERR_UNSUPPORTED,
-                                "reference steady-state pairing wall: not available yet");
     if (P.have_collisions && P.mode == PIC_MODE_REF &&
         !(P.sigma_cx == PIC_SIGMA_CONSTANT && P.sigma_value == 0.0))
         return set_global_error(PIC_ERR_UNSUPPORTED,
@@ -781,8 +796,12 @@ int pic_set_runtime(pic_handle *h, int32_t run_time) {
 int pic_run(pic_handle *h) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
     // pic.cpp:1049-1057: wall pass before the loop (ion wall, then electron wall)
-    absorb_species(h, 1, plop_on(h));
-    absorb_species(h, 0, false);
+    if (ref_steady(h)) {
+        steady_boundary(h);
+    } else {
+        absorb_species(h, 1, plop_on(h));
+        absorb_species(h, 0, false);
+    }
     if (h->spawning && plop_on(h)) {
         commit(h, 1, 1, 0);
         commit(h, 0, 0, 0);
@@ -916,6 +935,10 @@ int pic_get_overflow(pic_handle *h, int64_t *dropped) {
 int pic_boundary(pic_handle *h, int32_t which) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
     if (which != 0 && which != 1) return seterr(h, PIC_ERR_ARG, "which must be 0 (ion) or 1 (electron)");
+    if (ref_steady(h)) {  // steady_i_boundary is empty; steady_e_boundary pairs
+        if (which == 1) steady_boundary(h);
+        return PIC_OK;
+    }
     bool plop = which == 0 && plop_on(h);
     absorb_species(h, which == 0 ? 1 : 0, plop);
     if (plop) {

@@ -777,6 +777,35 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Sort (key, val) pairs [0, c) ascending by key in one workgroup: bitonic
+// network over the next power of two, padded with ~0 keys (the arrays hold
+// at least that many entries: the staging capacity is a power of two).
+__device__ void block_sort_pairs(unsigned long long *key, uint32_t *val, uint32_t c) {
+    uint32_t P = 1;
+    while (P < c) P <<= 1;
+    for (uint32_t t = c + threadIdx.x; t < P; t += blockDim.x) key[t] = ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+                uint32_t o = t ^ j;
+                if (o > t) {
+                    unsigned long long a = key[t], b = key[o];
+                    bool up = (t & k) == 0;
+                    if ((a > b) == up) {
+                        key[t] = b;
+                        key[o] = a;
+                        uint32_t x = val[t];
+                        val[t] = val[o];
+                        val[o] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Commit the staged particles of one species: bitonic sort of (key, index)
 // in one workgroup, then append in key order to dst with consecutive new
 // ids, optionally depositing their charge.  Sorting makes the result
@@ -795,32 +824,9 @@ __global__ void __launch_bounds__(1024)
     __syncthreads();
     const uint32_t c = s_c;
     if (c == 0) return;
-    uint32_t P = 1;
-    while (P < c) P <<= 1;
-    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
-        sv.idx[t] = t;
-        if (t >= c) sv.key[t] = ~0ull;
-    }
+    for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) sv.idx[t] = t;
     __syncthreads();
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
-                uint32_t o = t ^ j;
-                if (o > t) {
-                    unsigned long long a = sv.key[t], b = sv.key[o];
-                    bool up = (t & k) == 0;
-                    if ((a > b) == up) {
-                        sv.key[t] = b;
-                        sv.key[o] = a;
-                        uint32_t x = sv.idx[t];
-                        sv.idx[t] = sv.idx[o];
-                        sv.idx[o] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
+    block_sort_pairs(sv.key, sv.idx, c);
     // entries staged as CELL_DEAD take an id but are not stored: exclusive
     // scan of the stored flags gives each stored entry its slot
     __shared__ uint32_t chunk_total;
@@ -869,6 +875,185 @@ __global__ void __launch_bounds__(1024)
 // ------------------------------------------------------------ loader
 // Uniform positions in (0,L)^2 and a 3-V Maxwellian, from Philox4x32-10 keyed
 // by the seed with counter (id, species, call, 0).
+// ---------------------------------------------- reference steady-state walls
+// Side flags of a particle for boundary.cpp:906-912 (x) and the y analogue:
+// 1 left (x < 0), 2 right ((int)(x/dx) >= N), 4 bottom (y < 0), 8 top.
+__device__ __forceinline__ int steady_sides(int jx, int jy, int N) {
+    int f = 0;
+    if (jx < 0) f |= 1;
+    else if (jx >= N) f |= 2;
+    if (jy < 0) f |= 4;
+    else if (jy >= N) f |= 8;
+    return f;
+}
+
+// Candidates of the steady walls: one entry (side << 32 | id, index) per
+// side a particle is outside of (a corner particle has an x and a y entry).
+__global__ void __launch_bounds__(256)
+    k_steady_collect(SpView sp, const uint32_t *__restrict__ nptr, int N, StageView sv,
+                     uint32_t *__restrict__ count, uint32_t *__restrict__ overflow) {
+    const uint32_t n = *nptr;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int f = 0;
+    uint32_t c = 0;
+    if (i < n) {
+        c = sp.cell[i];
+        const int jx = cell_jx(c);
+        if (jx != CELL_HOLE && jx != CELL_DEAD) f = steady_sides(jx, cell_jy(c), N);
+    }
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int axis = 0; axis < 2; axis++) {
+        const int side = axis == 0 ? ((f & 1) ? 0 : ((f & 2) ? 1 : -1))
+                                   : ((f & 4) ? 2 : ((f & 8) ? 3 : -1));
+        const bool has = side >= 0;
+        const uint64_t bal = __ballot(has);
+        if (!bal) continue;
+        const int leader = __ffsll((unsigned long long)bal) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(bal));
+        base = __shfl(base, leader, 64);
+        if (has) {
+            uint32_t slot = base + (uint32_t)__popcll(bal & below);
+            if (slot < sv.cap) {
+                sv.key[slot] = ((unsigned long long)side << 32) | sp.id[i];
+                sv.idx[slot] = i;
+            } else {
+                atomicAdd(overflow, 1u);
+            }
+        }
+    }
+}
+
+// first position in sorted key[0, c) whose key >= v
+__device__ __forceinline__ uint32_t lower_bound_u64(const unsigned long long *key, uint32_t c,
+                                                    unsigned long long v) {
+    uint32_t lo = 0, hi = c;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (key[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// is id among the first m entries of the sorted range [b, b + m) of side s
+__device__ __forceinline__ bool in_prefix(const unsigned long long *key, uint32_t b, uint32_t m,
+                                          int s, uint32_t id) {
+    unsigned long long v = ((unsigned long long)s << 32) | id;
+    uint32_t p = b + lower_bound_u64(key + b, m, v);
+    return p < b + m && key[p] == v;
+}
+
+// move the pair (e[ie], ion[ii]) to one point with fresh velocities and
+// deposit both (boundary.cpp:854-878); the ion's id keys the draws
+__device__ __forceinline__ void plop_store(SpView e, SpView ion, uint32_t ie, uint32_t ii,
+                                           const mcc::MccConst &mc, uint32_t step, int N,
+                                           float se, float si, unsigned long long *qe,
+                                           unsigned long long *qi, int mode) {
+    PState pe, pi;
+    mcc::plop_pair(mc, ion.id[ii], step, N, se, si, 1, pe, pi);
+    e.cell[ie] = cell_pack(pe.jx, pe.jy);
+    e.ox[ie] = pe.ox;
+    e.oy[ie] = pe.oy;
+    e.vx[ie] = pe.vx;
+    e.vy[ie] = pe.vy;
+    e.vz[ie] = pe.vz;
+    ion.cell[ii] = cell_pack(pi.jx, pi.jy);
+    ion.ox[ii] = pi.ox;
+    ion.oy[ii] = pi.oy;
+    ion.vx[ii] = pi.vx;
+    ion.vy[ii] = pi.vy;
+    ion.vz[ii] = pi.vz;
+    if (deposit_ok(pe, N, mode)) dep_global(qe, N, pe.jx, pe.jy, fx_weights(pe.ox, pe.oy));
+    if (deposit_ok(pi, N, mode)) dep_global(qi, N, pi.jx, pi.jy, fx_weights(pi.ox, pi.oy));
+}
+
+// Reference steady-state electron wall (steady_e_boundary, boundary.cpp:
+// 890-1097): per side, the m-th electron outside pairs with the m-th ion
+// outside (the reference's scan with a moving ion cursor is an order-
+// preserving match; ion_proportion = 1); each pair is re-injected.  X sides
+// first, then the y sides among the particles the x pass left outside.
+// Order = particle id (the reference's vector order depends on its removal
+// history, which this store does not keep).  One workgroup.
+__global__ void __launch_bounds__(1024)
+    k_steady_pair(SpView e, SpView ion, StageView ce, StageView ci, SpCounters *__restrict__ cnte,
+                  SpCounters *__restrict__ cnti, mcc::MccConst mc, uint32_t step, int N, float se,
+                  float si, unsigned long long *__restrict__ qe, unsigned long long *__restrict__ qi,
+                  int mode) {
+    __shared__ uint32_t rb_e[5], rb_i[5], s_ne, s_ni, chunk_e, chunk_i;
+    if (threadIdx.x == 0) {
+        uint32_t a = cnte->cand_count, b = cnti->cand_count;
+        s_ne = a < ce.cap ? a : ce.cap;
+        s_ni = b < ci.cap ? b : ci.cap;
+    }
+    __syncthreads();
+    const uint32_t ne = s_ne, ni = s_ni;
+    block_sort_pairs(ce.key, ce.idx, ne);
+    block_sort_pairs(ci.key, ci.idx, ni);
+    if (threadIdx.x < 5) {
+        unsigned long long v = (unsigned long long)threadIdx.x << 32;
+        rb_e[threadIdx.x] = lower_bound_u64(ce.key, ne, v);
+        rb_i[threadIdx.x] = lower_bound_u64(ci.key, ni, v);
+    }
+    __syncthreads();
+    // x sides
+    uint32_t mx[2];
+    for (int sd = 0; sd < 2; sd++) {
+        uint32_t a = rb_e[sd + 1] - rb_e[sd], b = rb_i[sd + 1] - rb_i[sd];
+        mx[sd] = a < b ? a : b;
+        for (uint32_t m = threadIdx.x; m < mx[sd]; m += blockDim.x)
+            plop_store(e, ion, ce.idx[rb_e[sd] + m], ci.idx[rb_i[sd] + m], mc, step, N, se, si, qe,
+                       qi, mode);
+    }
+    __syncthreads();
+    // y sides: drop the entries of particles re-injected by the x pass (stable
+    // compaction in place), then pair
+    for (int sd = 2; sd < 4; sd++) {
+        uint32_t ce_n = 0, ci_n = 0;
+        for (int sp = 0; sp < 2; sp++) {
+            unsigned long long *key = sp == 0 ? ce.key : ci.key;
+            uint32_t *val = sp == 0 ? ce.idx : ci.idx;
+            const uint32_t *rb = sp == 0 ? rb_e : rb_i;
+            uint32_t kept_before = 0;
+            for (uint32_t r0 = rb[sd]; r0 < rb[sd + 1]; r0 += blockDim.x) {
+                uint32_t r = r0 + threadIdx.x;
+                bool live = false;
+                unsigned long long kv = 0;
+                uint32_t vv = 0;
+                if (r < rb[sd + 1]) {
+                    kv = key[r];
+                    vv = val[r];
+                    uint32_t id = (uint32_t)kv;
+                    live = !in_prefix(key, rb[0], mx[0], 0, id) &&
+                           !in_prefix(key, rb[1], mx[1], 1, id);
+                }
+                uint32_t tot;
+                uint32_t ex = block_excl_scan_1024(live ? 1u : 0u, &tot);
+                if (threadIdx.x == 0) (sp == 0 ? chunk_e : chunk_i) = tot;
+                __syncthreads();  // every read of this chunk precedes the writes
+                if (live) {
+                    key[rb[sd] + kept_before + ex] = kv;
+                    val[rb[sd] + kept_before + ex] = vv;
+                }
+                kept_before += (sp == 0 ? chunk_e : chunk_i);
+                __syncthreads();
+            }
+            if (sp == 0) ce_n = kept_before;
+            else ci_n = kept_before;
+        }
+        uint32_t m_n = ce_n < ci_n ? ce_n : ci_n;
+        for (uint32_t m = threadIdx.x; m < m_n; m += blockDim.x)
+            plop_store(e, ion, ce.idx[rb_e[sd] + m], ci.idx[rb_i[sd] + m], mc, step, N, se, si, qe,
+                       qi, mode);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cnte->cand_count = 0;
+        cnti->cand_count = 0;
+    }
+}
+
 __global__ void __launch_bounds__(256)
     k_load_random(SpView sp, uint32_t n, uint32_t key0, uint32_t key1, uint32_t species, int N,
                   double sigma) {
@@ -963,6 +1148,19 @@ void launch_absorb(hipStream_t st, const SpView &src, const SpView &dst, const u
                    const mcc::MccConst &mc, uint32_t step, const Spawn &spn) {
     hipLaunchKernelGGL(k_absorb, dim3(blocks_for(cap, 256)), dim3(256), 0, st, src, dst, n, nout,
                        N, mode, lost6, mc, step, spn);
+}
+
+void launch_steady(hipStream_t st, const SpView &e, const uint32_t *ne, int64_t cap_e,
+                   const SpView &ion, const uint32_t *ni, int64_t cap_i, const StageView &ce,
+                   const StageView &ci, SpCounters *cnte, SpCounters *cnti, const mcc::MccConst &mc,
+                   uint32_t step, int N, float se, float si, unsigned long long *qe,
+                   unsigned long long *qi, int mode) {
+    hipLaunchKernelGGL(k_steady_collect, dim3(blocks_for(cap_e, 256)), dim3(256), 0, st, e, ne, N,
+                       ce, &cnte->cand_count, &cnte->overflow);
+    hipLaunchKernelGGL(k_steady_collect, dim3(blocks_for(cap_i, 256)), dim3(256), 0, st, ion, ni,
+                       N, ci, &cnti->cand_count, &cnti->overflow);
+    hipLaunchKernelGGL(k_steady_pair, dim3(1), dim3(1024), 0, st, e, ion, ce, ci, cnte, cnti, mc,
+                       step, N, se, si, qe, qi, mode);
 }
 
 void launch_load_random(hipStream_t st, const SpView &sp, uint32_t n, uint64_t seed,

@@ -16,6 +16,9 @@
 
 #define PIC_DEV_MODE_REF 0
 #define PIC_DEV_MODE_PHYS 1
+// reference semantics with the steady-state walls: particles outside are
+// never removed (they wait, frozen, for the pairing pass); otherwise REF
+#define PIC_DEV_MODE_REF_STEADY 2
 
 #define ST_KEPT 0
 #define ST_LEFT 1
@@ -150,6 +153,7 @@ __device__ __forceinline__ bool push_one(PState &s, const float2 *__restrict__ E
 __device__ __forceinline__ int classify(const PState &s, int N, int mode) {
     if (s.jx == CELL_HOLE) return ST_HOLE;
     if (s.jx == CELL_DEAD) return ST_DEAD;
+    if (mode == PIC_DEV_MODE_REF_STEADY) return ST_KEPT;
     bool lx, ly;
     if (mode == PIC_DEV_MODE_PHYS) {
         lx = (s.jx < 0) || (s.jx > N - 2);

@@ -27,7 +27,7 @@ struct SpCounters {
     uint32_t next_id;        // id of the next particle created
     uint32_t overflow;       // particles dropped for lack of capacity
     uint32_t stage_count;    // entries in the staging buffer
-    uint32_t pad;
+    uint32_t cand_count;     // steady-wall candidates in the staging buffer
     // left, right, bottom, top, removed by collision, -
     uint32_t lost[CNT_SLOTS][6];
     uint32_t holes[CNT_SLOTS];  // empty slots left by in-place steps (current buffer)
@@ -111,6 +111,11 @@ void launch_deposit(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t
 void launch_absorb(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
                    uint32_t *nout, int64_t cap, int N, int mode, uint32_t *lost6,
                    const mcc::MccConst &mc, uint32_t step, const Spawn &spn);
+void launch_steady(hipStream_t st, const SpView &e, const uint32_t *ne, int64_t cap_e,
+                   const SpView &ion, const uint32_t *ni, int64_t cap_i, const StageView &ce,
+                   const StageView &ci, SpCounters *cnte, SpCounters *cnti, const mcc::MccConst &mc,
+                   uint32_t step, int N, float se, float si, unsigned long long *qe,
+                   unsigned long long *qi, int mode);
 void launch_load_random(hipStream_t st, const SpView &sp, uint32_t n, uint64_t seed,
                         uint32_t species, int N, double sigma);
 
