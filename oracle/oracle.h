@@ -43,6 +43,8 @@ extern "C" {
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* uniform in the open interval (0,1): ((x >> 9) + 0.5) * 2^-23, exact in fp32 */
 float orc_u01(uint32_t x);
+/* 53-bit uniform in (0,1) from two words (the loader's and the neutral table's) */
+double orc_u01d(uint32_t hi, uint32_t lo);
 
 /* ------------------------------------------------------------------------ */
 /* FP64 reference semantics (particleVec AoS)                               */
@@ -264,6 +266,27 @@ int orc_collide_ion(const orc_mcc_params *c, double *vx, double *vy, double *vz,
  * doubles per particle (zeros unless ionised) */
 void orc_mcc_apply(const orc_mcc_params *c, long n, double *vx, double *vy, double *vz,
                    const uint32_t *id, uint32_t step, int32_t *event, double *child6);
+
+/* Reference-mode ion charge exchange (ion_collisions.cpp:255-308) on the
+ * build layout, fp32 in the GPU's operation order: eligible ions ((int)(x/dx)
+ * and (int)(y/dy) in [0, N)) visited in (column, id) order add
+ * P = 1 - exp(-u dt 1e-20 n_g sigma) to pcol[column]; above 1 the ion takes
+ * the velocity of neutral-table entry (int)(u01 * 9999) (Philox counter (id,
+ * step, 16+9, 0x4D434331)) and pcol restarts at 0.  Returns the events. */
+typedef struct {
+    int gas;         /* 0 argon, 1 hydrogen                                   */
+    int sig_energy;  /* 1: energy-dependent sigma, 0: sig_const               */
+    float sig_const; /* [1e-20 m^2]                                           */
+    float q, mass, dt, ng;
+    uint32_t key0, key1;
+} orc_refcx_params;
+float orc32_ref_cx_prob(const orc_refcx_params *c, float vx, float vy, float vz);
+long orc32_ref_cx(long n, const uint32_t *cell, const float *ox, const float *oy, float *vx,
+                  float *vy, float *vz, const uint32_t *id, int N, const orc_refcx_params *c,
+                  float *pcol, const float *ntab, int ntab_n, uint32_t step);
+/* the reference's neutral velocity table (collisions.cpp:66-89) drawn from
+ * Philox: 3 x 10000 floats (vx | vy | vz) */
+void orc_neutral_table(uint64_t seed, double vtn, float *t);
 
 /* Collisional FP64 step loop in the reference's phase order (pic.cpp:1069-
  * 1162): electron MCC (children appended in creator order), electron push,

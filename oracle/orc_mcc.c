@@ -397,3 +397,92 @@ void orc_mcc_apply(const orc_mcc_params *c, long n, double *vx, double *vy, doub
             for (int m = 0; m < 6; m++) child6[6 * k + m] = ch[m];
     }
 }
+
+/* ------------------------------------ reference-mode ion charge exchange */
+
+typedef struct {
+    uint64_t key; /* jp << 32 | id */
+    long idx;
+    float P;
+} orc_cx_ent;
+
+static int cx_cmp(const void *a, const void *b) {
+    uint64_t x = ((const orc_cx_ent *)a)->key, y = ((const orc_cx_ent *)b)->key;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+float orc32_ref_cx_prob(const orc_refcx_params *c, float vx, float vy, float vz) {
+    float E = (0.5f / c->q) * c->mass * (vx * vx + vy * vy + vz * vz);
+    float u = sqrtf(2.0f * c->q * E / c->mass);
+    float sig;
+    if (!c->sig_energy) {
+        sig = c->sig_const;
+    } else if (c->gas == 0) {
+        sig = (E > 4.0f) ? (2.0f + 5.5f / sqrtf(E)) * 10.0f : (-2.95f * sqrtf(E) + 10.65f) * 10.0f;
+    } else {
+        float e = E / 1000.0f / 1.66054e-27f;
+        if (e < 1.0e-4f) {
+            sig = 0.0f;
+        } else {
+            sig = (3.2345f * logf(235.88f / e + 2.3713f)) /
+                  (1.0f + 0.038371f * e + 3.8068e-6f * powf(e, 3.5f) + 1.1832e-10f * powf(e, 5.4f));
+        }
+    }
+    return 1.0f - expf(-(u * c->dt * 1.0e-20f * c->ng * sig));
+}
+
+long orc32_ref_cx(long n, const uint32_t *cell, const float *ox, const float *oy, float *vx,
+                  float *vy, float *vz, const uint32_t *id, int N, const orc_refcx_params *c,
+                  float *pcol, const float *ntab, int ntab_n, uint32_t step) {
+    orc_cx_ent *v = (orc_cx_ent *)malloc(sizeof(orc_cx_ent) * (size_t)(n + 1));
+    long m = 0;
+    for (long k = 0; k < n; k++) {
+        int jx = orc_cell_jx(cell[k]), jy = orc_cell_jy(cell[k]);
+        int inx = jx >= -1 && jx <= N - 1 && !(jx == -1 && ox[k] == 0.0f);
+        int iny = jy >= -1 && jy <= N - 1 && !(jy == -1 && oy[k] == 0.0f);
+        if (!inx || !iny) continue;
+        uint32_t jp = jx < 0 ? 0u : (uint32_t)jx;
+        v[m].key = ((uint64_t)jp << 32) | id[k];
+        v[m].idx = k;
+        v[m].P = orc32_ref_cx_prob(c, vx[k], vy[k], vz[k]);
+        m++;
+    }
+    qsort(v, (size_t)m, sizeof(orc_cx_ent), cx_cmp);
+    long fired = 0;
+    const uint32_t key[2] = {c->key0, c->key1};
+    for (long t = 0; t < m; t++) {
+        uint32_t jp = (uint32_t)(v[t].key >> 32);
+        pcol[jp] = pcol[jp] + v[t].P;
+        if (pcol[jp] > 1.0f) {
+            uint32_t ctr[4] = {(uint32_t)v[t].key, step, 16u + 9u, 0x4D434331u}, r[4];
+            orc_philox4x32_10(ctr, key, r);
+            int ix = (int)(orc_u01(r[0]) * (float)(ntab_n - 1));
+            ix = ix < 0 ? 0 : (ix > ntab_n - 2 ? ntab_n - 2 : ix);
+            long k = v[t].idx;
+            vx[k] = ntab[ix];
+            vy[k] = ntab[ntab_n + ix];
+            vz[k] = ntab[2 * ntab_n + ix];
+            pcol[jp] = 0.0f;
+            fired++;
+        }
+    }
+    free(v);
+    return fired;
+}
+
+void orc_neutral_table(uint64_t seed, double vtn, float *t) {
+    const int n = 10000;
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int i = 0; i < n; i++) {
+        uint32_t c0[4] = {(uint32_t)i, 0x7AB1Eu, 0u, 0x4E455554u}, r[4];
+        orc_philox4x32_10(c0, key, r);
+        double R1 = orc_u01d(r[0], r[1]), R2 = orc_u01d(r[2], r[3]);
+        t[i] = (float)(vtn * cos(3.141592654 * R2) * sqrt(fabs(log(R1))));
+        uint32_t c1[4] = {(uint32_t)i, 0x7AB1Eu, 1u, 0x4E455554u};
+        orc_philox4x32_10(c1, key, r);
+        R1 = orc_u01d(r[0], r[1]);
+        R2 = orc_u01d(r[2], r[3]);
+        t[n + i] = (float)(vtn * cos(3.141592654 * R2) * sqrt(fabs(log(R1))));
+        t[2 * n + i] = (float)(vtn * sin(3.141592654 * R2) * sqrt(fabs(log(R1))));
+    }
+}

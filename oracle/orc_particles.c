@@ -28,6 +28,12 @@ static inline void mulhilo32(uint32_t a, uint32_t b, uint32_t *hi, uint32_t *lo)
     *lo = (uint32_t)p;
 }
 
+/* 53-bit uniform in (0,1): 27 bits of hi above 26 bits of lo */
+double orc_u01d(uint32_t hi, uint32_t lo) {
+    uint64_t m = ((uint64_t)(hi >> 5) << 26) | (uint64_t)(lo >> 6);
+    return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+}
+
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
     uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
     uint32_t k0 = key[0], k1 = key[1];

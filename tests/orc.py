@@ -55,6 +55,12 @@ class MccParams(C.Structure):
     ] + [("key0", C.c_uint32), ("key1", C.c_uint32)]
 
 
+class RefCxParams(C.Structure):
+    _fields_ = [("gas", C.c_int), ("sig_energy", C.c_int), ("sig_const", C.c_float),
+                ("q", C.c_float), ("mass", C.c_float), ("dt", C.c_float), ("ng", C.c_float),
+                ("key0", C.c_uint32), ("key1", C.c_uint32)]
+
+
 _lib = None
 
 
@@ -102,6 +108,13 @@ def lib():
         _lib.orc32_steady.argtypes = [C.c_long] + [vp] * 7 + [C.c_long] + [vp] * 7 + \
             [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
         _lib.orc32_steady.restype = C.c_long
+        _lib.orc32_ref_cx_prob.argtypes = [C.POINTER(RefCxParams), C.c_float, C.c_float,
+                                           C.c_float]
+        _lib.orc32_ref_cx_prob.restype = C.c_float
+        _lib.orc32_ref_cx.argtypes = [C.c_long] + [vp] * 7 + [C.c_int, C.POINTER(RefCxParams), vp,
+                                                              vp, C.c_int, C.c_uint32]
+        _lib.orc32_ref_cx.restype = C.c_long
+        _lib.orc_neutral_table.argtypes = [C.c_uint64, C.c_double, vp]
         _lib.orc_dump_write.argtypes = [C.c_char_p, vp, C.c_long, C.c_int]
         _lib.orc_dump_read.argtypes = [C.c_char_p, vp, C.c_long]
         _lib.orc_dump_read.restype = C.c_long
@@ -410,3 +423,25 @@ def run_ref_mcc(cfg, me, mi, plop, vte, vti, nsteps, e, i, eid, iid, next_id, Ex
     return dict(e=ea[: ne.value], i=ia[: ni.value], eid=ea_id[: ne.value], iid=ia_id[: ni.value],
                 next_id=nid, Ex=Ex, Ey=Ey, qe=qe.reshape(N, N), qi=qi.reshape(N, N),
                 rho=rho.reshape(N, N), phi=phi.reshape(N, N), events=ev)
+
+
+def refcx_params(p):
+    """oracle parameters of the reference ion charge exchange for a pic2d.Params"""
+    return RefCxParams(gas=int(p.gas), sig_energy=int(p.sigma_cx == 1), sig_const=p.sigma_value,
+                       q=Q_E, mass=p.ion_mass, dt=p.dti, ng=p.neutral_density,
+                       key0=p.seed & 0xFFFFFFFF, key1=(p.seed >> 32) & 0xFFFFFFFF)
+
+
+def neutral_table(seed, vtn):
+    t = np.zeros(30000, dtype=np.float32)
+    lib().orc_neutral_table(seed, vtn, P(t))
+    return t
+
+
+def ref_cx(state, N, c, pcol, table, step):
+    """orc32_ref_cx on a copy of a species state; returns (state, events)"""
+    st = {k: np.array(v, copy=True) for k, v in state.items()}
+    ev = lib().orc32_ref_cx(len(st["id"]), P(st["cell"]), P(st["ox"]), P(st["oy"]), P(st["vx"]),
+                            P(st["vy"]), P(st["vz"]), P(st["id"]), N, C.byref(c), P(pcol),
+                            P(table), 10000, step)
+    return st, ev

@@ -319,3 +319,47 @@ def test_sort_interval_does_not_change_collisional_results():
         for key in ("phi", "ev0", "ev1", "le", "li"):
             assert np.array_equal(out[k][key], out[1][key]), (k, key)
     assert out[1]["ev0"][3] > 0 and out[1]["li"][:4].sum() > 0
+
+
+def test_reference_ion_charge_exchange_matches_the_oracle():
+    """Reference mode: the per-column accumulator (ion_collisions.cpp:297-303)
+    in (column, id) order, persisting across calls.  The device and host
+    expf may differ in the last bit, which can move a column's crossing of 1
+    by one ion; the fired sets agree up to such shifts (a handful)."""
+    p = collisional(ppc=40, mode=pic2d.MODE_REF)
+    p.sigma_cx = pic2d.SIGMA_ENERGY
+    p.vtn = math.sqrt(1.38066e-23 * 300.0 / MI)
+    p.dt = p.dti = 1e-8
+    N = p.nodes
+    rng = np.random.default_rng(21)
+    n = 30000
+    jx = rng.integers(-1, N + 1, n)
+    jy = rng.integers(-1, N + 1, n)
+    st = dict(cell=pic2d.cell_pack(jx, jy), ox=rng.random(n).astype(np.float32),
+              oy=rng.random(n).astype(np.float32),
+              vx=(rng.standard_normal(n) * 3000).astype(np.float32),
+              vy=(rng.standard_normal(n) * 3000).astype(np.float32),
+              vz=(rng.standard_normal(n) * 3000).astype(np.float32),
+              id=rng.permutation(n).astype(np.uint32))
+    c = orc.refcx_params(p)
+    table = orc.neutral_table(p.seed, p.vtn)
+    pcol = np.zeros(N, np.float32)
+    ref = st
+    total = 0
+    with pic2d.Pic(p) as sim:
+        sim.set_state(1, st)
+        for step in range(4):
+            sim.set_step(step)
+            sim.ion_collisions()
+            ref, ev = orc.ref_cx(ref, N, c, pcol, table, step)
+            got = sim.get_state(1)
+            o = np.argsort(got["id"])
+            r = np.argsort(ref["id"])
+            diff = np.flatnonzero((got["vx"][o] != ref["vx"][r]) | (got["vz"][o] != ref["vz"][r]))
+            assert diff.size <= 6, (step, diff.size)
+            total += ev
+            # keep both sides on the same state for the next call
+            ref = {k: v[r] for k, v in ref.items()}
+            ref.update({k: got[k][o] for k in ("vx", "vy", "vz")})
+        assert abs(int(sim.collision_counts(1)[4]) - total) <= 6 * 4
+    assert total > 100

@@ -354,6 +354,15 @@ bool ref_steady(pic_handle *h) {
     return h->P.wall == PIC_WALL_STEADY && h->P.mode == PIC_MODE_REF;
 }
 
+// ion_collisions::collision in reference mode (ion_collisions.cpp:255-308)
+int ref_ion_cx(pic_handle *h) {
+    Species &s = h->sp[1];
+    if (launch_refcx(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], (uint32_t)s.cap, h->N, s.mc,
+                     (uint32_t)h->step, h->cx, &h->cnt[1].events[0][0]) != 0)
+        return seterr(h, PIC_ERR_DEVICE, "reference charge exchange: radix sort failed");
+    return PIC_OK;
+}
+
 // steady_e_boundary (boundary.cpp:1094-1097) on the current buffers; the
 // re-injected pairs are deposited into this step's charge
 void steady_boundary(pic_handle *h) {
@@ -553,6 +562,7 @@ int fused_step(pic_handle *h) {
         commit(h, 0, 0, 1);
     }
     if (h->step % h->P.dt_dti == 0) {
+        if (h->refcx) CHECK(ref_ion_cx(h));
         advance_species(h, 1);
     } else {
         Species &s = h->sp[1];
@@ -620,11 +630,6 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         return set_global_error(PIC_ERR_UNSUPPORTED, "NUMBER_TO_INJECT != 0 is not supported");
     if (P.no_ion_species != 1)
         return set_global_error(PIC_ERR_UNSUPPORTED, "NO_ION_SPECIES must be 1");
-    if (P.have_collisions && P.mode == PIC_MODE_REF &&
-        !(P.sigma_cx == PIC_SIGMA_CONSTANT && P.sigma_value == 0.0))
-        return set_global_error(PIC_ERR_UNSUPPORTED,
-                                "reference ion charge exchange (per-column accumulator, "
-                                "ion_collisions.cpp:297-303) is not available");
     if (P.mg_mode != PIC_MG_FIXED && P.mg_mode != PIC_MG_TOL)
         return set_global_error(PIC_ERR_ARG, "unknown multigrid mode");
     if (P.mg_mode == PIC_MG_TOL && (!(P.mg_tol > 0.0) || P.mg_max_iter < 1))
@@ -692,6 +697,22 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     }
     setup_mcc(h, h->sp[0], true);
     setup_mcc(h, h->sp[1], false);
+    // reference mode: ions collide through the per-column accumulator
+    h->refcx = P.have_collisions && P.mode == PIC_MODE_REF &&
+               !(P.sigma_cx == PIC_SIGMA_CONSTANT && P.sigma_value == 0.0);
+    if (h->refcx) {
+        RefCxBuffers &B = h->cx;
+        const size_t cap = (size_t)h->sp[1].cap;
+        for (int k = 0; k < 2; k++) {
+            if ((rc = dalloc(h, &B.key[k], cap)) != PIC_OK) return fail(rc);
+            if ((rc = dalloc(h, &B.val[k], cap)) != PIC_OK) return fail(rc);
+        }
+        B.temp_bytes = refcx_temp_bytes((uint32_t)cap);
+        if ((rc = dalloc(h, (char **)&B.temp, B.temp_bytes)) != PIC_OK) return fail(rc);
+        if ((rc = dalloc(h, &B.pcol, (size_t)h->N)) != PIC_OK) return fail(rc);
+        if (hipMemset(B.pcol, 0, sizeof(float) * h->N) != hipSuccess)
+            return fail(set_global_error(PIC_ERR_DEVICE, "memset failed"));
+    }
     if ((rc = dalloc(h, &h->cnt, 2)) != PIC_OK) return fail(rc);
     if (hipHostMalloc((void **)&h->cnt_host, 2 * sizeof(SpCounters), 0) != hipSuccess)
         return fail(set_global_error(PIC_ERR_ALLOC, "hipHostMalloc failed"));
@@ -773,6 +794,9 @@ int pic_destroy(pic_handle *h) {
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (h->cnt_host) hipHostFree(h->cnt_host);
+    void *cx[] = {h->cx.key[0], h->cx.key[1], h->cx.val[0], h->cx.val[1], h->cx.temp, h->cx.pcol};
+    for (void *p : cx)
+        if (p) hipFree(p);
     free_mg(h->M);
     for (ProfPair &p : h->prof_live) {
         hipEventDestroy(p.a);
@@ -901,6 +925,7 @@ int pic_e_collisions(pic_handle *h) {
 int pic_ion_collisions(pic_handle *h) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
     compact_species(h, 1);
+    if (h->refcx) return ref_ion_cx(h);
     Species &s = h->sp[1];
     if (!s.mc.on) return PIC_OK;
     launch_mcc(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], s.cap, s.pc, s.mc, (uint32_t)h->step,

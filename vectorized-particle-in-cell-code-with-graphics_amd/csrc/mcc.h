@@ -251,6 +251,8 @@ struct Child {
 #define RS_IONIZE 2
 #define RS_PLOP 3
 #define RS_PLOP2 4
+// 5: second draw of a re-injected pair
+#define RS_REFCX 9  // reference-mode charge exchange: neutral table index
 
 // Electron collision.  Returns the event; on EV_IONIZE in PHYS mode `ch`
 // holds the ejected electron and the new ion; in REF mode EV_IONIZE means the

@@ -54,6 +54,8 @@ struct pic_handle {
     float *ntab = nullptr;       // reference neutral velocity table (collisions.cpp:66-89)
     bool spawning = false;       // collisions or re-injection create particles
     pic::MgLevels M;
+    bool refcx = false;         // the reference's per-column ion charge exchange
+    pic::RefCxBuffers cx;
     double last_phiR = 0.0;
     int mg_cycles = 0;          // V-cycles (tolerance mode) of the last solve
     int64_t mg_cycles_total = 0;

@@ -119,6 +119,19 @@ void launch_steady(hipStream_t st, const SpView &e, const uint32_t *ne, int64_t 
 void launch_load_random(hipStream_t st, const SpView &sp, uint32_t n, uint64_t seed,
                         uint32_t species, int N, double sigma);
 
+// ---- k_refcx.hip: the reference's per-column ion charge exchange
+struct RefCxBuffers {
+    unsigned long long *key[2] = {};  // sort keys (jp << 32 | id), in / out
+    unsigned long long *val[2] = {};  // probability bits << 32 | store index
+    void *temp = nullptr;             // radix-sort scratch
+    size_t temp_bytes = 0;
+    float *pcol = nullptr;            // probabilityCX[N], kept across steps
+};
+size_t refcx_temp_bytes(uint32_t cap);
+int launch_refcx(hipStream_t st, const SpView &sp, const uint32_t *n, uint32_t cap, int N,
+                 const mcc::MccConst &mc, uint32_t step, RefCxBuffers &B,
+                 unsigned long long *events);
+
 // ---- k_fields.hip
 void launch_rho(hipStream_t st, const unsigned long long *qi, const unsigned long long *qe, int N,
                 double scale_i, double scale_e, double corner, double *rho);
