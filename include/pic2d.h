@@ -95,6 +95,17 @@ typedef struct pic_params {
                                  in between, particles are updated in place);
                                  0 = chosen per species from its thermal
                                  displacement per step                              */
+    /* batch-mode diagnostics (parameters.h:225-313; graphs.cpp) */
+    int32_t graph_start;      /* START_GRAPH_TIME: first step of the averaging window */
+    int32_t graph_steps;      /* NO_GRAPH_STEPS (0: no graph averages)                */
+    int32_t time_data_freq;   /* TIME_DATA_FREQUENCY (0: no timeData rows)            */
+    int32_t no_bins;          /* NO_BINS velocity bins (odd)                          */
+    int32_t divide_bins_by;   /* DIVIDE_BINS_BY: bins per thermal speed               */
+    int32_t spread;           /* SPREAD: cells about the distribution locations       */
+    int32_t dist_loc[4];      /* DIST_LOCATIONX, DIST_LOCATIONY, DIST_LOCATION2X,
+                                 DIST_LOCATION2Y                                      */
+    int32_t probe[6];         /* POINT1X, POINT1Y, POINT2X, POINT2Y, POINT3X, POINT3Y */
+    int32_t write_graphs;     /* pic::writeGraphDataToFiles (pic.cpp:58)              */
 } pic_params;
 
 /* particleVec.h:26-35: the 48-byte AoS record of the reference */
@@ -210,6 +221,40 @@ int pic_set_state(pic_handle *h, int32_t s, const uint32_t *cell, const float *o
 int pic_get_qfixed(pic_handle *h, int32_t s, int64_t *out); /* nodes*nodes, 2^-40 units   */
 int pic_get_E32(pic_handle *h, float *out);  /* fp32 (Ex,Ey) pairs, (nodes+1)*nodes+1      */
 int pic_set_E32(pic_handle *h, const float *in);
+
+/* ------------------------------------------------------ diagnostics */
+/* Batch-mode graph averages and time data (graphs.cpp:412-1369, pic.cpp:
+ * 1167-1274), collected on the device inside pic_run / pic_advance over the
+ * window [graph_start, graph_start + graph_steps) and every time_data_freq
+ * steps (pic_params).  Velocity sums use 64-bit fixed point and histograms
+ * integer counts, so the averages are deterministic. */
+
+/* directory where pic_run writes the graph files at the end of the window
+ * (if write_graphs) and timeData.txt at the end of the run; NULL or "" (the
+ * default) writes nothing.  The reference writes into its working directory. */
+int pic_set_output_dir(pic_handle *h, const char *dir);
+/* graphs::average_fields: rho, phi, Ex, Ey window sums / NO_GRAPH_STEPS, each
+ * nodes*nodes (any may be NULL) */
+int pic_get_field_averages(pic_handle *h, double *rho, double *phi, double *ex, double *ey);
+/* graphs::average_density: sum of get_q / particle charge over the window /
+ * NO_GRAPH_STEPS, nodes*nodes */
+int pic_get_density_average(pic_handle *h, int32_t s, double *out);
+/* graphs::average_velocities: per-cell mean vx along the centre row and vy
+ * along the centre column, / (VTE * NO_GRAPH_STEPS) for both species
+ * (graphs.cpp:1114-1121), nodes entries each (either may be NULL) */
+int pic_get_velocity_slices(pic_handle *h, int32_t s, double *vx, double *vy);
+/* graphs::average_dist1/2: 4 * no_bins values -- first X, first Y, second
+ * X, second Y histograms / NO_GRAPH_STEPS */
+int pic_get_distributions(pic_handle *h, int32_t s, double *out);
+/* timeData rows recorded so far: 8 doubles per row (step, n_e, n_i,
+ * phi(POINT1), phi(POINT2), phi(POINT3), q_e(POINT3), q_i(POINT3)) */
+int pic_get_time_data(pic_handle *h, double *out, int32_t cap_rows, int32_t *rows);
+/* graphs::write_*_to_file (graphs.cpp:1167-1369) into dir: rho_phi_EgridX.txt,
+ * rho_phi_EgridY.txt, rho2d.txt, phi2d.txt, egrid2dX.txt, egrid2dY.txt and per
+ * species density2d_<t>.txt, av_velocity_<t>.txt, first/second_v_distX/Y_<t>.txt */
+int pic_write_graphs(pic_handle *h, const char *dir);
+/* timeData.txt (pic.cpp:1262-1274) into dir */
+int pic_write_time_data(pic_handle *h, const char *dir);
 
 /* ------------------------------------------------------------ profiling */
 /* Event timing of the push pipeline (collide/gather/push/sort/deposit) and

@@ -162,6 +162,21 @@ int orc_mg_tol(double *u_rhs, int n, double length, double tol, int max_cycles, 
 double orc_mg_relres(const double *u, const double *f, int n, double length);
 
 
+/* Batch diagnostics of one species for one step (graphs.cpp:459-691, kernels
+ * pic.cl:151-258) on the build layout: velocity sums along the centre row
+ * (vx, cells with (int)(y/dy) == N/2 and x > 0) and column (vy) in 64-bit
+ * fixed point (2^-20 m/s) with counts; histograms of v/vt * divide_bins_by
+ * (truncated) at the two locations +- spread: dist = 4 x no_bins (first X,
+ * first Y, second X, second Y).  Accumulates into the outputs. */
+void orc32_diag_particles(long n, const uint32_t *cell, const float *ox, const float *oy,
+                          const float *vx, const float *vy, int N, double vt, int no_bins,
+                          int divide_bins_by, int spread, const int loc[4], int64_t *vsx,
+                          int64_t *vsy, uint32_t *cntx, uint32_t *cnty, uint64_t *dist);
+/* end of a step: vp += mean velocity of every cell holding a particle
+ * (graphs.cpp:683-691); the step sums are cleared */
+void orc_diag_vfinal(int N, int64_t *vsx, int64_t *vsy, uint32_t *cntx, uint32_t *cnty,
+                     double *vpx, double *vpy);
+
 /* ------------------------------------------------------------------------ */
 /* Whole timesteps (pic::pic_run order, pic.cpp:1069-1162, collisionless)     */
 /* ------------------------------------------------------------------------ */

@@ -317,3 +317,47 @@ long orc_dump_read(const char *path, orc_particle *p, long cap) {
     fclose(f);
     return n;
 }
+
+/* ------------------------------------------------- batch diagnostics */
+
+void orc32_diag_particles(long n, const uint32_t *cell, const float *ox, const float *oy,
+                          const float *vx, const float *vy, int N, double vt, int no_bins,
+                          int divide_bins_by, int spread, const int loc[4], int64_t *vsx,
+                          int64_t *vsy, uint32_t *cntx, uint32_t *cnty, uint64_t *dist) {
+    const int mid = N / 2, half = (no_bins - 1) / 2;
+    for (long k = 0; k < n; k++) {
+        int jx = orc_cell_jx(cell[k]), jy = orc_cell_jy(cell[k]);
+        int xpos = jx > 0 || (jx == 0 && ox[k] > 0.0f);
+        int ypos = jy > 0 || (jy == 0 && oy[k] > 0.0f);
+        if (xpos && jx < N && jy == mid) {
+            vsx[jx] += llrint((double)vx[k] * 1048576.0);
+            cntx[jx]++;
+            double q = ((double)vx[k] / vt) * (double)divide_bins_by;
+            int vb = fabs(q) < (double)half + 1.0 ? (int)q : half + 1;
+            if (abs(vb) <= half) {
+                if (jx >= loc[0] - spread && jx <= loc[0] + spread) dist[vb + half]++;
+                if (jx >= loc[2] - spread && jx <= loc[2] + spread) dist[2 * no_bins + vb + half]++;
+            }
+        }
+        if (ypos && jy < N && jx == mid) {
+            vsy[jy] += llrint((double)vy[k] * 1048576.0);
+            cnty[jy]++;
+            double q = ((double)vy[k] / vt) * (double)divide_bins_by;
+            int vb = fabs(q) < (double)half + 1.0 ? (int)q : half + 1;
+            if (abs(vb) <= half) {
+                if (jy >= loc[1] - spread && jy <= loc[1] + spread) dist[no_bins + vb + half]++;
+                if (jy >= loc[3] - spread && jy <= loc[3] + spread) dist[3 * no_bins + vb + half]++;
+            }
+        }
+    }
+}
+
+void orc_diag_vfinal(int N, int64_t *vsx, int64_t *vsy, uint32_t *cntx, uint32_t *cnty,
+                     double *vpx, double *vpy) {
+    for (int j = 0; j < N; j++) {
+        if (cntx[j] > 0) vpx[j] += ((double)vsx[j] / 1048576.0) / (double)cntx[j];
+        if (cnty[j] > 0) vpy[j] += ((double)vsy[j] / 1048576.0) / (double)cnty[j];
+        vsx[j] = vsy[j] = 0;
+        cntx[j] = cnty[j] = 0;
+    }
+}

@@ -115,6 +115,9 @@ def lib():
                                                               vp, C.c_int, C.c_uint32]
         _lib.orc32_ref_cx.restype = C.c_long
         _lib.orc_neutral_table.argtypes = [C.c_uint64, C.c_double, vp]
+        _lib.orc32_diag_particles.argtypes = [C.c_long] + [vp] * 5 + [C.c_int, C.c_double, C.c_int,
+                                                                       C.c_int, C.c_int] + [vp] * 6
+        _lib.orc_diag_vfinal.argtypes = [C.c_int] + [vp] * 6
         _lib.orc_dump_write.argtypes = [C.c_char_p, vp, C.c_long, C.c_int]
         _lib.orc_dump_read.argtypes = [C.c_char_p, vp, C.c_long]
         _lib.orc_dump_read.restype = C.c_long
@@ -445,3 +448,26 @@ def ref_cx(state, N, c, pcol, table, step):
                             P(st["vy"]), P(st["vz"]), P(st["id"]), N, C.byref(c), P(pcol),
                             P(table), 10000, step)
     return st, ev
+
+
+class Diag:
+    """oracle accumulators of the batch diagnostics of one species"""
+
+    def __init__(self, N, no_bins):
+        self.N, self.nb = N, no_bins
+        self.vsx = np.zeros(N, np.int64)
+        self.vsy = np.zeros(N, np.int64)
+        self.cx = np.zeros(N, np.uint32)
+        self.cy = np.zeros(N, np.uint32)
+        self.vpx = np.zeros(N)
+        self.vpy = np.zeros(N)
+        self.dist = np.zeros(4 * no_bins, np.uint64)
+
+    def step(self, st, vt, divide_bins_by, spread, loc):
+        loc = np.asarray(loc, dtype=np.int32)
+        lib().orc32_diag_particles(len(st["cell"]), P(st["cell"]), P(st["ox"]), P(st["oy"]),
+                                   P(st["vx"]), P(st["vy"]), self.N, vt, self.nb, divide_bins_by,
+                                   spread, P(loc), P(self.vsx), P(self.vsy), P(self.cx),
+                                   P(self.cy), P(self.dist))
+        lib().orc_diag_vfinal(self.N, P(self.vsx), P(self.vsy), P(self.cx), P(self.cy),
+                              P(self.vpx), P(self.vpy))

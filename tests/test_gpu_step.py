@@ -237,3 +237,90 @@ def test_reference_steady_walls_match_the_oracle():
             pairs += int(moved.sum())
         assert pairs > 0
         assert sim.loss_counts(0).sum() == 0 and sim.loss_counts(1).sum() == 0
+
+
+def test_batch_diagnostics_match_the_oracle():
+    """Graph-window sums and timeData rows collected on the device against the
+    oracle fed with the same per-step states: field and density averages,
+    velocity slices and histograms bit-exact."""
+    p = workloads.with_diagnostics(
+        workloads.discharge(32, 40, mode=pic2d.MODE_PHYSICAL, mg_mode=pic2d.MG_FIXED, te=4.0),
+        start=2, steps=4, time_data_freq=3, no_bins=301, bins_per_vt=30)
+    N = p.nodes
+    loc = list(p.dist_loc)
+    sums = {k: np.zeros((N, N)) for k in ("rho", "phi", "ex", "ey", "q0", "q1")}
+    d = [orc.Diag(N, p.no_bins), orc.Diag(N, p.no_bins)]
+    rows = []
+    qc = (p.electron_charge, p.ion_charge)
+    with pic2d.Pic(p) as sim:
+        for step in range(8):
+            sim.advance(1)
+            if step % 3 == 0:
+                phi = sim.get_phi()
+                qe3 = sim.get_q(0)[p.probe[5], p.probe[4]]
+                qi3 = sim.get_q(1)[p.probe[5], p.probe[4]]
+                rows.append([step, sim.particle_size(0), sim.particle_size(1),
+                             phi[p.probe[1], p.probe[0]], phi[p.probe[3], p.probe[2]],
+                             phi[p.probe[5], p.probe[4]], qe3, qi3])
+            if 2 <= step < 6:
+                ex, ey = sim.get_E()
+                sums["rho"] += sim.get_rho()
+                sums["phi"] += sim.get_phi()
+                sums["ex"] += ex.reshape(-1)[: N * N].reshape(N, N)
+                sums["ey"] += ey.reshape(-1)[: N * N].reshape(N, N)
+                for s in (0, 1):
+                    sums[f"q{s}"] += sim.get_q(s) / qc[s]
+                    d[s].step(sim.get_state(s), p.vte if s == 0 else p.vti, p.divide_bins_by,
+                              p.spread, loc)
+        fa = sim.field_averages()
+        for k in ("rho", "phi", "ex", "ey"):
+            assert np.array_equal(fa[k], sums[k] / 4), k
+        for s in (0, 1):
+            assert np.array_equal(sim.density_average(s), sums[f"q{s}"] / 4)
+            vx, vy = sim.velocity_slices(s)
+            assert np.array_equal(vx, d[s].vpx / (p.vte * 4))
+            assert np.array_equal(vy, d[s].vpy / (p.vte * 4))
+            dist = sim.distributions(s, p.no_bins)
+            assert np.array_equal(dist, d[s].dist.reshape(4, -1).astype(np.float64) / 4)
+        assert d[0].dist.sum() > 0
+        td = sim.time_data()
+        assert np.array_equal(td, np.array(rows, dtype=np.float64))
+
+
+def test_batch_output_files_have_the_reference_format(tmp_path):
+    """pic_run writes the graph files at the end of the window and timeData.txt
+    at the end of the run, with the line counts, field counts and number
+    formats of the reference's example batch output (257x257 nodes)."""
+    import json
+    import os
+    import re
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                       "batch_output_format.json")))
+    p = workloads.with_diagnostics(
+        workloads.discharge(256, 1, mode=pic2d.MODE_REF, mg_mode=pic2d.MG_FIXED),
+        start=0, steps=2, time_data_freq=1)
+    with pic2d.Pic(p) as sim:
+        sim.set_output_dir(tmp_path)
+        sim.run(3)
+    efmt = re.compile(r"^-?\d\.\d{6}e[+-]\d{2}$")
+    for name, g in gold.items():
+        path = tmp_path / name
+        assert path.exists(), name
+        lines = path.read_text().split("\n")
+        assert lines[-1] == ""
+        lines = lines[:-1]
+        if name == "timeData.txt":
+            assert len(lines) == 3 // 1 + 1          # SIZE_TIME_DATA = run / freq + 1
+        else:
+            assert len(lines) == g["lines"], name
+        for ln in lines[:3]:
+            f = ln.split()
+            assert len(f) == g["fields"], (name, ln)
+            if name != "timeData.txt":
+                assert all(efmt.match(x) for x in f), (name, ln)
+                # trailing-space convention of each writer
+                assert ln.endswith(" ") == g["trailing_space"], name
+        if name == "timeData.txt":
+            f0 = lines[0].split()   # recorded after step 0: step, then integer counts
+            assert f0[0] == "0" and f0[1].isdigit() and f0[2].isdigit()
+            assert int(f0[2]) <= p.init_particle_no

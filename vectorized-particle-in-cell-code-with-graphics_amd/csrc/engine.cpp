@@ -575,6 +575,7 @@ int fused_step(pic_handle *h) {
     if (ref_steady(h)) steady_boundary(h);
     prof_end(h, pp);
     CHECK(field_step(h, h->step));
+    diag_step(h, h->step);
     prof_end(h, pt);
     h->have_rho = true;
     h->step++;
@@ -714,6 +715,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
             return fail(set_global_error(PIC_ERR_DEVICE, "memset failed"));
     }
     if ((rc = dalloc(h, &h->cnt, 2)) != PIC_OK) return fail(rc);
+    if ((rc = diag_alloc(h)) != PIC_OK) return fail(rc);
     if (hipHostMalloc((void **)&h->cnt_host, 2 * sizeof(SpCounters), 0) != hipSuccess)
         return fail(set_global_error(PIC_ERR_ALLOC, "hipHostMalloc failed"));
     memset(h->cnt_host, 0, 2 * sizeof(SpCounters));
@@ -794,6 +796,7 @@ int pic_destroy(pic_handle *h) {
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (h->cnt_host) hipHostFree(h->cnt_host);
+    diag_free(h);
     void *cx[] = {h->cx.key[0], h->cx.key[1], h->cx.val[0], h->cx.val[1], h->cx.temp, h->cx.pcol};
     for (void *p : cx)
         if (p) hipFree(p);
@@ -830,14 +833,18 @@ int pic_run(pic_handle *h) {
         commit(h, 1, 1, 0);
         commit(h, 0, 0, 0);
     }
+    CHECK(diag_prepare_time(h, (int32_t)std::min<int64_t>(h->step + h->run_time, INT32_MAX)));
     for (int32_t t = 0; t < h->run_time; t++) CHECK(fused_step(h));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return seterr(h, PIC_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
+    // pic.cpp:1262-1274: timeData.txt at the end of the run
+    if (diag_time_on(h) && !h->out_dir.empty()) CHECK(diag_write_time(h, h->out_dir));
     return PIC_OK;
 }
 
 int pic_advance(pic_handle *h, int32_t nsteps) {
     if (!h || nsteps < 0) return set_global_error(PIC_ERR_ARG, "bad argument");
+    CHECK(diag_prepare_time(h, (int32_t)std::min<int64_t>(h->step + nsteps, INT32_MAX)));
     for (int32_t t = 0; t < nsteps; t++) CHECK(fused_step(h));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return seterr(h, PIC_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
@@ -1189,6 +1196,58 @@ int pic_get_loss_counts(pic_handle *h, int32_t s, int64_t *out4) {
         for (int b = 0; b < CNT_SLOTS; b++) out4[w] += h->cnt_host[s].lost[b][w];
     }
     return PIC_OK;
+}
+
+// ------------------------------------------------------- diagnostics
+int pic_set_output_dir(pic_handle *h, const char *dir) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    h->out_dir = dir ? dir : "";
+    return PIC_OK;
+}
+
+int pic_get_field_averages(pic_handle *h, double *rho, double *phi, double *ex, double *ey) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    return diag_fields(h, rho, phi, ex, ey);
+}
+
+int pic_get_density_average(pic_handle *h, int32_t s, double *out) {
+    CHECK(check_species(h, s));
+    if (!out) return seterr(h, PIC_ERR_ARG, "null output");
+    return diag_density(h, s, out);
+}
+
+int pic_get_velocity_slices(pic_handle *h, int32_t s, double *vx, double *vy) {
+    CHECK(check_species(h, s));
+    return diag_velocities(h, s, vx, vy);
+}
+
+int pic_get_distributions(pic_handle *h, int32_t s, double *out) {
+    CHECK(check_species(h, s));
+    if (!out) return seterr(h, PIC_ERR_ARG, "null output");
+    return diag_distributions(h, s, out);
+}
+
+int pic_get_time_data(pic_handle *h, double *out, int32_t cap_rows, int32_t *rows) {
+    if (!h || !rows || cap_rows < 0) return set_global_error(PIC_ERR_ARG, "bad argument");
+    std::vector<double> r;
+    int n = 0;
+    CHECK(diag_time_rows(h, r, n));
+    *rows = n;
+    if (out) {
+        int m = n < cap_rows ? n : cap_rows;
+        for (size_t k = 0; k < (size_t)m * DIAG_TCOLS; k++) out[k] = r[k];
+    }
+    return PIC_OK;
+}
+
+int pic_write_graphs(pic_handle *h, const char *dir) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    return diag_write_graphs(h, dir ? dir : ".");
+}
+
+int pic_write_time_data(pic_handle *h, const char *dir) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    return diag_write_time(h, dir ? dir : ".");
 }
 
 int pic_dump_data(pic_handle *h, const char *dir, int32_t precision) {

@@ -348,6 +348,28 @@ std::string strip_comments(const std::string &text) {
 
 }  // namespace
 
+// parameters.h:225-313 for the current node count; period = (int)PERIOD
+static void pic_params_default_diagnostics(pic_params *P, int period) {
+    const int N = P->nodes;
+    P->graph_steps = 50 * period;
+    P->graph_start = 250 * period;
+    P->time_data_freq = 100;
+    P->no_bins = 30001;
+    P->divide_bins_by = (P->no_bins - 1) / 100;
+    P->dist_loc[0] = (int)(0.9f * (float)N);
+    P->dist_loc[1] = (int)(0.5f * (float)N);
+    P->dist_loc[2] = (int)(0.1f * (float)N);
+    P->dist_loc[3] = (int)(0.5f * (float)N);
+    P->spread = (int)(0.05f * (float)N);
+    P->probe[0] = 3;
+    P->probe[1] = N / 2;
+    P->probe[2] = N / 2;
+    P->probe[3] = N / 2;
+    P->probe[4] = N - 3;
+    P->probe[5] = N / 2;
+    P->write_graphs = 1;
+}
+
 extern "C" int pic_params_default(pic_params *P) {
     if (!P) return PIC_ERR_ARG;
     memset(P, 0, sizeof(*P));
@@ -399,6 +421,7 @@ extern "C" int pic_params_default(pic_params *P) {
     P->threshold = 15.75;
     P->magnetic_field = 0.0;
     P->angle = 30.0 * PI / 180.0;
+    pic_params_default_diagnostics(P, (int)period);
     return PIC_OK;
 }
 
@@ -559,6 +582,27 @@ extern "C" int pic_params_load(const char *path, pic_params *P) {
     ok = ok && inum("PIC_MG_NCYCLE", P->mg_ncycle);
     ok = ok && inum("PIC_MG_MAX_ITER", P->mg_max_iter);
     ok = ok && inum("PIC_SORT_INTERVAL", P->sort_interval);
+    // diagnostic defaults follow the loaded grid and period; explicit macros win
+    double period = 0.0;
+    if (has("PERIOD")) ok = ok && num("PERIOD", period);
+    else period = 1.0 / (2.0 * P->dt * 3.141592654 * 13.6e6);
+    pic_params_default_diagnostics(P, (int)period);
+    ok = ok && inum("START_GRAPH_TIME", P->graph_start);
+    ok = ok && inum("NO_GRAPH_STEPS", P->graph_steps);
+    ok = ok && inum("TIME_DATA_FREQUENCY", P->time_data_freq);
+    ok = ok && inum("NO_BINS", P->no_bins);
+    ok = ok && inum("DIVIDE_BINS_BY", P->divide_bins_by);
+    ok = ok && inum("SPREAD", P->spread);
+    ok = ok && inum("DIST_LOCATIONX", P->dist_loc[0]);
+    ok = ok && inum("DIST_LOCATIONY", P->dist_loc[1]);
+    ok = ok && inum("DIST_LOCATION2X", P->dist_loc[2]);
+    ok = ok && inum("DIST_LOCATION2Y", P->dist_loc[3]);
+    ok = ok && inum("POINT1X", P->probe[0]);
+    ok = ok && inum("POINT1Y", P->probe[1]);
+    ok = ok && inum("POINT2X", P->probe[2]);
+    ok = ok && inum("POINT2Y", P->probe[3]);
+    ok = ok && inum("POINT3X", P->probe[4]);
+    ok = ok && inum("POINT3Y", P->probe[5]);
     ok = ok && num("PIC_SIGMA_LOAD_E", P->sigma_load_e);
     ok = ok && num("PIC_SIGMA_LOAD_I", P->sigma_load_i);
     if (!ok)

@@ -38,6 +38,24 @@ struct ProfPair {
 
 }  // namespace pic_host
 
+struct pic_handle;
+namespace pic_host {
+// diag.cpp
+bool diag_graphs_on(const pic_handle *h);
+bool diag_time_on(const pic_handle *h);
+int diag_alloc(pic_handle *h);
+void diag_free(pic_handle *h);
+int diag_prepare_time(pic_handle *h, int32_t last_step);
+void diag_step(pic_handle *h, int64_t t);
+int diag_fields(pic_handle *h, double *rho, double *phi, double *ex, double *ey);
+int diag_density(pic_handle *h, int s, double *out);
+int diag_velocities(pic_handle *h, int s, double *vx, double *vy);
+int diag_distributions(pic_handle *h, int s, double *out);
+int diag_time_rows(pic_handle *h, std::vector<double> &rows, int &n);
+int diag_write_graphs(pic_handle *h, const std::string &dir);
+int diag_write_time(pic_handle *h, const std::string &dir);
+}  // namespace pic_host
+
 struct pic_handle {
     pic_params P{};
     int N = 0;
@@ -56,6 +74,11 @@ struct pic_handle {
     pic::MgLevels M;
     bool refcx = false;         // the reference's per-column ion charge exchange
     pic::RefCxBuffers cx;
+    // batch-mode diagnostics
+    pic::DiagDev diag;
+    int diag_rows = 0, diag_rows_cap = 0;  // timeData rows written / allocated
+    int64_t diag_graph_steps = 0;          // steps summed into the graph window
+    std::string out_dir;                   // where pic_run writes its files ("" = none)
     double last_phiR = 0.0;
     int mg_cycles = 0;          // V-cycles (tolerance mode) of the last solve
     int64_t mg_cycles_total = 0;

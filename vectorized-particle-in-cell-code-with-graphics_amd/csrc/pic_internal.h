@@ -132,6 +132,29 @@ int launch_refcx(hipStream_t st, const SpView &sp, const uint32_t *n, uint32_t c
                  const mcc::MccConst &mc, uint32_t step, RefCxBuffers &B,
                  unsigned long long *events);
 
+// ---- k_diag.hip: batch-mode diagnostics
+#define DIAG_TCOLS 8  // step, n_e, n_i, phi(P1), phi(P2), phi(P3), q_e(P3), q_i(P3)
+struct Probe6 {
+    int x0, y0, x1, y1, x2, y2;  // POINT1..3 (x, y)
+};
+struct DiagDev {
+    double *rho = nullptr, *phi = nullptr, *ex = nullptr, *ey = nullptr;  // N^2 window sums
+    double *q[2] = {};                      // N^2: density (charge / particle charge) sums
+    long long *vsx[2] = {}, *vsy[2] = {};   // N: this step's fixed-point velocity sums
+    uint32_t *cntx[2] = {}, *cnty[2] = {};  // N: this step's counts
+    double *vpx[2] = {}, *vpy[2] = {};      // N: window sums of per-cell mean velocity
+    unsigned long long *dist[2] = {};       // 4 x no_bins: X1, Y1, X2, Y2 counts
+    double *tdata = nullptr;                // rows x DIAG_TCOLS
+};
+void launch_diag_step(hipStream_t st, DiagDev &D, bool graphs, bool time_row, int row,
+                      long long step, int N, const double *rho, const double *phi,
+                      const double *ex64, const double *ey64, const unsigned long long *qe,
+                      const unsigned long long *qi, double scale_e, double scale_i, double corner,
+                      double qc_e, double qc_i, const SpView &e, const uint32_t *ne, int64_t cap_e,
+                      const SpView &ion, const uint32_t *ni, int64_t cap_i, double vt_e,
+                      double vt_i, int no_bins, int divide_bins_by, int spread, int4 loc,
+                      const SpCounters *cnt, int cur_e, int cur_i, Probe6 probe);
+
 // ---- k_fields.hip
 void launch_rho(hipStream_t st, const unsigned long long *qi, const unsigned long long *qe, int N,
                 double scale_i, double scale_e, double corner, double *rho);

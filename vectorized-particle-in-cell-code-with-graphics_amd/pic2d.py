@@ -102,6 +102,15 @@ class Params(C.Structure):
         ("phi_period", C.c_double),
         ("mg_tol", C.c_double),
         ("sort_interval", C.c_int32),
+        ("graph_start", C.c_int32),
+        ("graph_steps", C.c_int32),
+        ("time_data_freq", C.c_int32),
+        ("no_bins", C.c_int32),
+        ("divide_bins_by", C.c_int32),
+        ("spread", C.c_int32),
+        ("dist_loc", C.c_int32 * 4),
+        ("probe", C.c_int32 * 6),
+        ("write_graphs", C.c_int32),
     ]
 
     @property
@@ -179,6 +188,14 @@ def load_library(path=LIB_PATH):
         "pic_get_overflow": [vp, P(i64)],
         "pic_get_mg_stats": [vp, P(i32), P(i64), P(dbl)],
         "pic_sync": [vp],
+        "pic_set_output_dir": [vp, C.c_char_p],
+        "pic_get_field_averages": [vp, vp, vp, vp, vp],
+        "pic_get_density_average": [vp, i32, vp],
+        "pic_get_velocity_slices": [vp, i32, vp, vp],
+        "pic_get_distributions": [vp, i32, vp],
+        "pic_get_time_data": [vp, vp, i32, P(i32)],
+        "pic_write_graphs": [vp, C.c_char_p],
+        "pic_write_time_data": [vp, C.c_char_p],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -408,6 +425,45 @@ class Pic:
 
     def set_step(self, step):
         self._c(self.lib.pic_set_step(self.h, int(step)))
+
+    # -- diagnostics (graphs.cpp / timeData)
+    def set_output_dir(self, directory):
+        self._c(self.lib.pic_set_output_dir(self.h, None if directory is None
+                                            else str(directory).encode()))
+
+    def field_averages(self):
+        n = self.N
+        a = [np.zeros((n, n)) for _ in range(4)]
+        self._c(self.lib.pic_get_field_averages(self.h, *[_ptr(x) for x in a]))
+        return dict(rho=a[0], phi=a[1], ex=a[2], ey=a[3])
+
+    def density_average(self, s):
+        a = np.zeros((self.N, self.N))
+        self._c(self.lib.pic_get_density_average(self.h, s, _ptr(a)))
+        return a
+
+    def velocity_slices(self, s):
+        vx, vy = np.zeros(self.N), np.zeros(self.N)
+        self._c(self.lib.pic_get_velocity_slices(self.h, s, _ptr(vx), _ptr(vy)))
+        return vx, vy
+
+    def distributions(self, s, no_bins):
+        a = np.zeros(4 * no_bins)
+        self._c(self.lib.pic_get_distributions(self.h, s, _ptr(a)))
+        return a.reshape(4, no_bins)
+
+    def time_data(self, cap=1 << 16):
+        rows = C.c_int32()
+        self._c(self.lib.pic_get_time_data(self.h, None, 0, C.byref(rows)))
+        a = np.zeros((max(rows.value, 1), 8))
+        self._c(self.lib.pic_get_time_data(self.h, _ptr(a), rows.value, C.byref(rows)))
+        return a[: rows.value]
+
+    def write_graphs(self, directory="."):
+        self._c(self.lib.pic_write_graphs(self.h, str(directory).encode()))
+
+    def write_time_data(self, directory="."):
+        self._c(self.lib.pic_write_time_data(self.h, str(directory).encode()))
 
     # -- profiling
     def profile(self, on=True):

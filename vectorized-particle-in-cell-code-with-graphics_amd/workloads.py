@@ -61,6 +61,29 @@ def discharge(cells, ppc, length=0.01, te=2.0, ti=0.026, density=1e15, ion_mass=
     p.mg_tol = 1e-8
     p.seed = seed
     p.run_time = 1000
+    p.graph_steps = 0        # batch diagnostics off unless asked for (with_diagnostics)
+    p.time_data_freq = 0
+    return p
+
+
+def with_diagnostics(p, start, steps, time_data_freq=100, no_bins=30001, bins_per_vt=300):
+    """Batch-mode graph averages over [start, start + steps) and timeData rows,
+    at the reference's locations for this grid (parameters.h:288-313)."""
+    N = p.nodes
+    p.graph_start = start
+    p.graph_steps = steps
+    p.time_data_freq = time_data_freq
+    p.no_bins = no_bins
+    p.divide_bins_by = bins_per_vt
+    p.dist_loc[0] = int(0.9 * N)
+    p.dist_loc[1] = int(0.5 * N)
+    p.dist_loc[2] = int(0.1 * N)
+    p.dist_loc[3] = int(0.5 * N)
+    p.spread = int(0.05 * N)
+    p.probe[0], p.probe[1] = 3, N // 2
+    p.probe[2], p.probe[3] = N // 2, N // 2
+    p.probe[4], p.probe[5] = N - 3, N // 2
+    p.write_graphs = 1
     return p
 
 
