@@ -255,6 +255,21 @@ int pic_get_time_data(pic_handle *h, double *out, int32_t cap_rows, int32_t *row
 int pic_write_graphs(pic_handle *h, const char *dir);
 /* timeData.txt (pic.cpp:1262-1274) into dir */
 int pic_write_time_data(pic_handle *h, const char *dir);
+/* graphs::getPhiMax, getExMax, getEyMax, getQMax of both species
+ * (graphs.cpp:1000-1112; pic.cpp:924-975): out5 = max averaged phi, Ex, Ey,
+ * electron density, ion density */
+int pic_get_graph_max(pic_handle *h, double *out5);
+
+/* -------------------------------------------------- snapshot export */
+/* pic::getElectrons / getIons (pic.cpp:901-910) thinned on the device: the
+ * particles whose id is a multiple of stride (stride 1: all), as 48-byte
+ * records; *n receives the subset size (PIC_ERR_CAPACITY if above cap) */
+int pic_get_particle_subset(pic_handle *h, int32_t s, int32_t stride, pic_particle *out,
+                            int64_t cap, int64_t *n);
+/* dir/snapshot_<step>_phi.txt and dir/snapshot_<step>_<e|i1>.txt (the subset
+ * of pic_get_particle_subset, "x y vx vy vz weight" per line): frames for
+ * offline plotting in place of the OpenGL views */
+int pic_write_snapshot(pic_handle *h, const char *dir, int32_t stride);
 
 /* ------------------------------------------------------------ profiling */
 /* Event timing of the push pipeline (collide/gather/push/sort/deposit) and

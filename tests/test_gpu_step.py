@@ -324,3 +324,31 @@ def test_batch_output_files_have_the_reference_format(tmp_path):
             f0 = lines[0].split()   # recorded after step 0: step, then integer counts
             assert f0[0] == "0" and f0[1].isdigit() and f0[2].isdigit()
             assert int(f0[2]) <= p.init_particle_no
+
+
+def test_snapshot_subset_and_graph_maxima(tmp_path):
+    p = workloads.with_diagnostics(
+        workloads.discharge(32, 20, mode=pic2d.MODE_PHYSICAL, mg_mode=pic2d.MG_FIXED),
+        start=0, steps=2, time_data_freq=0, no_bins=101, bins_per_vt=10)
+    with pic2d.Pic(p) as sim:
+        sim.run(3)
+        for s in (0, 1):
+            st = sim.get_state(s)
+            want = orc.state_to_aos({k: v[st["id"] % 7 == 0] for k, v in st.items()}, p.dx)
+            got = sim.particle_subset(s, 7)
+            key = lambda a: np.lexsort((a["vx"], a["y"], a["x"]))
+            assert len(got) == len(want) > 0
+            for f in ("x", "y", "vx", "vy", "vz", "weight"):
+                assert np.array_equal(got[f][key(got)], want[f][key(want)]), f
+        fa = sim.field_averages()
+        mx = sim.graph_max()
+        assert mx[0] == max(fa["phi"].max(), fa["phi"][0, 0])
+        assert mx[1] == fa["ex"].max() and mx[2] == fa["ey"].max()
+        assert mx[3] == sim.density_average(0).max()
+        sim.write_snapshot(tmp_path, stride=5)
+        step = sim.get_step()
+        phi = np.loadtxt(tmp_path / f"snapshot_{step}_phi.txt")
+        assert phi.shape == (p.nodes, p.nodes)
+        np.testing.assert_allclose(phi, sim.get_phi(), rtol=1e-6, atol=1e-12)
+        e = np.loadtxt(tmp_path / f"snapshot_{step}_e.txt")
+        assert e.shape == (len(sim.particle_subset(0, 5)), 6)

@@ -1127,6 +1127,38 @@ int pic_get_particles(pic_handle *h, int32_t s, pic_particle *out, int64_t cap) 
     return PIC_OK;
 }
 
+// pic::getElectrons / getIons (pic.cpp:901-910) thinned on the device: the
+// particles whose id is a multiple of stride, as 48-byte records
+int pic_get_particle_subset(pic_handle *h, int32_t s, int32_t stride, pic_particle *out,
+                            int64_t cap, int64_t *n) {
+    CHECK(check_species(h, s));
+    if (stride < 1 || !n || cap < 0) return seterr(h, PIC_ERR_ARG, "bad argument");
+    Species &sp = h->sp[s];
+    const uint32_t room = (uint32_t)std::min<int64_t>(cap, sp.cap);
+    double *buf = nullptr;
+    uint32_t *cnt = nullptr;
+    CHECK(dalloc(h, &buf, 6 * (size_t)std::max<uint32_t>(room, 1)));
+    if (dalloc(h, &cnt, 1) != PIC_OK) {
+        hipFree(buf);
+        return PIC_ERR_ALLOC;
+    }
+    launch_fill<uint32_t>(h->st, cnt, 1, 0u);
+    launch_subset(h->st, sp.buf[sp.cur], &h->cnt[s].n[sp.cur], sp.cap, (uint32_t)stride, h->dx,
+                  sp.weight, buf, room, cnt);
+    uint32_t c = 0;
+    hipError_t e = hipMemcpyAsync(&c, cnt, sizeof(c), hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    const uint32_t m = std::min(c, room);
+    if (e == hipSuccess && m > 0 && out)
+        e = hipMemcpy(out, buf, (size_t)m * sizeof(pic_particle), hipMemcpyDeviceToHost);
+    hipFree(buf);
+    hipFree(cnt);
+    if (e != hipSuccess) return seterr(h, PIC_ERR_DEVICE, hipGetErrorString(e));
+    *n = c;
+    if (c > room) return seterr(h, PIC_ERR_CAPACITY, "output buffer too small");
+    return PIC_OK;
+}
+
 static int copy_out(pic_handle *h, void *dst, const void *src, size_t bytes) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
     if (!dst) return seterr(h, PIC_ERR_ARG, "null output");
@@ -1243,6 +1275,63 @@ int pic_get_time_data(pic_handle *h, double *out, int32_t cap_rows, int32_t *row
 int pic_write_graphs(pic_handle *h, const char *dir) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
     return diag_write_graphs(h, dir ? dir : ".");
+}
+
+// graphs::getPhiMax / getExMax / getEyMax / getQMax (graphs.cpp:1000-1112):
+// out5 = max of the averaged phi, Ex, Ey, electron and ion density; each
+// maximum starts from the reference's first element ([0][0]; [1][1] for q)
+int pic_get_graph_max(pic_handle *h, double *out5) {
+    if (!h || !out5) return set_global_error(PIC_ERR_ARG, "null argument");
+    const int N = h->N;
+    const size_t nn = (size_t)N * N;
+    std::vector<double> a[4];
+    for (auto &v : a) v.resize(nn);
+    CHECK(diag_fields(h, nullptr, a[0].data(), a[1].data(), a[2].data()));
+    for (int k = 0; k < 3; k++) {
+        double m = a[k][0];
+        for (double v : a[k]) m = v > m ? v : m;
+        out5[k] = m;
+    }
+    for (int s = 0; s < 2; s++) {
+        CHECK(diag_density(h, s, a[3].data()));
+        double m = a[3][(size_t)1 * N + 1];
+        for (double v : a[3]) m = v > m ? v : m;
+        out5[3 + s] = m;
+    }
+    return PIC_OK;
+}
+
+// snapshot for offline plotting (replaces the OpenGL views): dir/snapshot_<step>_phi.txt
+// (nodes x nodes, "%e ") and dir/snapshot_<step>_<e|i1>.txt ("x y vx vy vz weight"
+// per particle of the id-strided subset)
+int pic_write_snapshot(pic_handle *h, const char *dir, int32_t stride) {
+    if (!h || stride < 1) return set_global_error(PIC_ERR_ARG, "bad argument");
+    std::string d = dir ? dir : ".";
+    if (!d.empty() && d.back() != '/') d += "/";
+    const int N = h->N;
+    std::vector<double> phi((size_t)N * N);
+    CHECK(pic_get_phi(h, phi.data()));
+    std::string tag = "snapshot_" + std::to_string((long long)h->step);
+    FILE *f = fopen((d + tag + "_phi.txt").c_str(), "w");
+    if (!f) return seterr(h, PIC_ERR_FILE, "cannot write " + d + tag + "_phi.txt");
+    for (int j = 0; j < N; j++) {
+        for (int i = 0; i < N; i++) fprintf(f, "%e ", phi[(size_t)j * N + i]);
+        fprintf(f, "\n");
+    }
+    fclose(f);
+    const char *type[2] = {"e", "i1"};
+    for (int s = 0; s < 2; s++) {
+        int64_t n = 0, cap = h->sp[s].cap / stride + 64;
+        std::vector<pic_particle> p((size_t)cap);
+        CHECK(pic_get_particle_subset(h, s, stride, p.data(), cap, &n));
+        std::string path = d + tag + "_" + type[s] + ".txt";
+        FILE *g = fopen(path.c_str(), "w");
+        if (!g) return seterr(h, PIC_ERR_FILE, "cannot write " + path);
+        for (int64_t k = 0; k < n; k++)
+            fprintf(g, "%e %e %e %e %e %e\n", p[k].x, p[k].y, p[k].vx, p[k].vy, p[k].vz, p[k].weight);
+        fclose(g);
+    }
+    return PIC_OK;
 }
 
 int pic_write_time_data(pic_handle *h, const char *dir) {

@@ -154,3 +154,47 @@ void launch_diag_step(hipStream_t st, DiagDev &D, bool graphs, bool time_row, in
 }
 
 }  // namespace pic
+
+namespace pic {
+
+// particles whose id is a multiple of `stride` (a subset independent of the
+// store's order), compacted into 48-byte records (x, y, vx, vy, vz, weight)
+__global__ void __launch_bounds__(256)
+    k_subset(SpView sp, const uint32_t *__restrict__ nptr, uint32_t stride, double dx,
+             double weight, double *__restrict__ out, uint32_t cap, uint32_t *__restrict__ count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool take = false;
+    int jx = 0, jy = 0;
+    if (i < *nptr) {
+        const uint32_t c = sp.cell[i];
+        jx = cell_jx(c);
+        jy = cell_jy(c);
+        take = jx != CELL_HOLE && jx != CELL_DEAD && (sp.id[i] % stride) == 0u;
+    }
+    const int lane = threadIdx.x & 63;
+    const uint64_t bal = __ballot(take);
+    if (!bal) return;
+    const int leader = __ffsll((unsigned long long)bal) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(bal));
+    base = __shfl(base, leader, 64);
+    if (!take) return;
+    const uint32_t slot = base + (uint32_t)__popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    if (slot >= cap) return;
+    double *r = out + 6 * (size_t)slot;
+    r[0] = ((double)jx + (double)sp.ox[i]) * dx;
+    r[1] = ((double)jy + (double)sp.oy[i]) * dx;
+    r[2] = sp.vx[i];
+    r[3] = sp.vy[i];
+    r[4] = sp.vz[i];
+    r[5] = weight;
+}
+
+void launch_subset(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap_sp,
+                   uint32_t stride, double dx, double weight, double *out, uint32_t cap,
+                   uint32_t *count) {
+    hipLaunchKernelGGL(k_subset, dim3((unsigned)((cap_sp + 255) / 256)), dim3(256), 0, st, sp, n,
+                       stride, dx, weight, out, cap, count);
+}
+
+}  // namespace pic

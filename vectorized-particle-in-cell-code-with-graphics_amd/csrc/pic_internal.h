@@ -155,6 +155,10 @@ void launch_diag_step(hipStream_t st, DiagDev &D, bool graphs, bool time_row, in
                       double vt_i, int no_bins, int divide_bins_by, int spread, int4 loc,
                       const SpCounters *cnt, int cur_e, int cur_i, Probe6 probe);
 
+void launch_subset(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap_sp,
+                   uint32_t stride, double dx, double weight, double *out, uint32_t cap,
+                   uint32_t *count);
+
 // ---- k_fields.hip
 void launch_rho(hipStream_t st, const unsigned long long *qi, const unsigned long long *qe, int N,
                 double scale_i, double scale_e, double corner, double *rho);

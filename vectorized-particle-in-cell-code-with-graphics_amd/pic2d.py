@@ -196,6 +196,9 @@ def load_library(path=LIB_PATH):
         "pic_get_time_data": [vp, vp, i32, P(i32)],
         "pic_write_graphs": [vp, C.c_char_p],
         "pic_write_time_data": [vp, C.c_char_p],
+        "pic_get_graph_max": [vp, vp],
+        "pic_get_particle_subset": [vp, i32, i32, vp, i64, P(i64)],
+        "pic_write_snapshot": [vp, C.c_char_p, i32],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -464,6 +467,23 @@ class Pic:
 
     def write_time_data(self, directory="."):
         self._c(self.lib.pic_write_time_data(self.h, str(directory).encode()))
+
+    def graph_max(self):
+        """[phi max, Ex max, Ey max, electron density max, ion density max]"""
+        a = np.zeros(5)
+        self._c(self.lib.pic_get_graph_max(self.h, _ptr(a)))
+        return a
+
+    # -- snapshot export
+    def particle_subset(self, s, stride):
+        cap = self.particle_size(s) // stride + 64
+        a = np.zeros(cap, dtype=PARTICLE_DTYPE)
+        n = C.c_int64()
+        self._c(self.lib.pic_get_particle_subset(self.h, s, stride, _ptr(a), cap, C.byref(n)))
+        return a[: n.value]
+
+    def write_snapshot(self, directory=".", stride=1):
+        self._c(self.lib.pic_write_snapshot(self.h, str(directory).encode(), int(stride)))
 
     # -- profiling
     def profile(self, on=True):
