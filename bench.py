@@ -174,12 +174,14 @@ def main():
 
     log("profiling", args.profile_steps, "steps with HIP events")
     ne_a, ni_a = sim.particle_size(0), sim.particle_size(1)
+    mg0 = sim.mg_stats()
     sim.profile(True)
     sim.advance(args.profile_steps)
     prof = sim.profile_read()
     steps_ms = sim.profile_steps()
     sim.profile(False)
     ne_b, ni_b = sim.particle_size(0), sim.particle_size(1)
+    mg1 = sim.mg_stats()
     npart_per_launch = 0.25 * (ne_a + ne_b + ni_a + ni_b)
     alg_bytes = 48.0 * npart_per_launch + 16.0 * p.nodes * p.nodes
     # per-species particle passes: sort passes (k_push_scatter) and in-place
@@ -221,6 +223,9 @@ def main():
         "scan_ms": prof[5] / k3n,
         "sort_interval": int(p.sort_interval),
         "field_solve_ms_per_step": prof[3] / args.profile_steps,
+        "mg_cycles_per_step": (mg1[1] - mg0[1]) / args.profile_steps
+        if p.mg_mode == pic2d.MG_TOL else p.mg_ncycle,
+        "mg_relres_last": mg1[2],
         # SURVEY-style push+gather+deposit roofline: 48 B/particle + 16 B/node
         # per species pass over the whole push phase (sort, collisions and
         # commits included)

@@ -89,6 +89,7 @@ int alloc_mg(pic_handle *h, MgLevels &M, int N) {
     long lds = lds_limit();
     M.lc = mg_coarse_level_for(lds, M.ng);
     mg_set_coarse_attr(lds);
+    if (mg_dst_tables(M) != 0) return seterr(h, PIC_ERR_ALLOC, "multigrid: coarse tables");
     for (int l = M.lc; l < M.ng; l++) {
         size_t s = (size_t)((1 << (l + 1)) + 1) * ((1 << (l + 1)) + 1);
         CHECK(dalloc(h, &M.d.rhsF[l], s));
@@ -123,6 +124,9 @@ void free_mg(MgLevels &M) {
     }
     M.d = MgArrays<double>();
     M.f = MgArrays<float>();
+    if (M.dst_S) hipFree(M.dst_S);
+    if (M.dst_D) hipFree(M.dst_D);
+    M.dst_S = M.dst_D = nullptr;
     if (M.partials) hipFree(M.partials);
     if (M.flags) hipFree(M.flags);
     if (M.relres) hipHostFree(M.relres);

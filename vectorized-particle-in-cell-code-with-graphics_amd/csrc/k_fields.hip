@@ -6,6 +6,10 @@
 // association, so FIXED mode reproduces Mglin bit for bit.  Levels up to
 // 65x65 (or what the LDS holds) run inside one workgroup with all arrays in
 // LDS; finer levels use one launch per operation.
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
 #include "pic_internal.h"
 
 namespace pic {
@@ -400,6 +404,97 @@ __global__ void __launch_bounds__(1024)
     for (int k = threadIdx.x; k < nc * nc; k += blockDim.x) u_out[k] = u[k];
 }
 
+// ------------------------------------- exact coarsest-level solve (TOL)
+// The 5-point Laplacian with zero walls is diagonalised by the discrete sine
+// transform: with S[p][q] = sin(pi (p+1)(q+1)/M) (S S = M/2 I) and
+// lambda_k = -4 sin^2(pi (k+1) / 2M), the solution of A u = f on the m = M-1
+// interior nodes is u = (2/M)^2 S ((S f S) ./ ((lambda_p + lambda_q)/h^2)) S:
+// four M x M products, done here in LDS by one workgroup (2x2 register tiles,
+// fp32).  It replaces the coarse V-cycle of the tolerance mode's correction
+// solve; the FP64 refinement around it is unchanged.
+
+// C = A B for M x M row-major matrices in LDS; thread t < (M/2)^2 computes a
+// 2x2 tile.  Ends with a barrier.
+__device__ void lds_gemm(const float *A, const float *B, float *C, int M, int logT) {
+    const int T = 1 << logT;
+    const int t = threadIdx.x;
+    if (t < T * T) {
+        const int ti = t >> logT, tj = t & (T - 1);
+        const float *a0 = A + (2 * ti) * M, *a1 = a0 + M;
+        float c00 = 0.0f, c01 = 0.0f, c10 = 0.0f, c11 = 0.0f;
+        for (int k = 0; k < M; k++) {
+            const float x0 = a0[k], x1 = a1[k];
+            const float2 b = *reinterpret_cast<const float2 *>(B + k * M + 2 * tj);
+            c00 = fmaf(x0, b.x, c00);
+            c01 = fmaf(x0, b.y, c01);
+            c10 = fmaf(x1, b.x, c10);
+            c11 = fmaf(x1, b.y, c11);
+        }
+        *reinterpret_cast<float2 *>(C + (2 * ti) * M + 2 * tj) = make_float2(c00, c01);
+        *reinterpret_cast<float2 *>(C + (2 * ti + 1) * M + 2 * tj) = make_float2(c10, c11);
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(1024)
+    k_mg_coarse_dst(const float *__restrict__ rhs_in, float *__restrict__ u_out, int lc, float h2,
+                    const float *__restrict__ Sg, const float *__restrict__ Dg,
+                    const int *__restrict__ skip) {
+    SKIP_IF_DONE(skip);
+    extern __shared__ double lds_raw[];
+    const int M = 1 << (lc + 1), n = M + 1, logT = lc;  // T = M/2 = 2^lc
+    float *S = reinterpret_cast<float *>(lds_raw);
+    float *X = S + M * M, *Y = X + M * M;
+    for (int k = threadIdx.x; k < M * M; k += blockDim.x) {
+        const int p = k >> (lc + 1), q = k & (M - 1);
+        S[k] = Sg[k];
+        X[k] = (p < M - 1 && q < M - 1) ? rhs_in[(p + 1) * n + q + 1] : 0.0f;
+    }
+    __syncthreads();
+    lds_gemm(S, X, Y, M, logT);  // Y = S f
+    lds_gemm(Y, S, X, M, logT);  // X = S f S
+    for (int k = threadIdx.x; k < M * M; k += blockDim.x) X[k] = X[k] * (Dg[k] * h2);
+    __syncthreads();
+    lds_gemm(S, X, Y, M, logT);
+    lds_gemm(Y, S, X, M, logT);  // X = u on the interior
+    for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+        const int i = k / n, j = k - (k / n) * n;
+        const bool in = i > 0 && j > 0 && i < n - 1 && j < n - 1;
+        u_out[k] = in ? X[(i - 1) * M + (j - 1)] : 0.0f;
+    }
+}
+
+static void coarse_dst_launch(hipStream_t st, const MgLevels &M, const float *rhs, float *u,
+                              double length, const int *skip) {
+    const int lc = M.lc, m = 1 << (lc + 1);
+    double h = length / (double)m;
+    hipLaunchKernelGGL(k_mg_coarse_dst, dim3(1), dim3(1024), 3ull * m * m * sizeof(float), st, rhs,
+                       u, lc, (float)(h * h), M.dst_S, M.dst_D, skip);
+}
+
+int mg_dst_tables(MgLevels &M) {
+    const int m = 1 << (M.lc + 1);
+    const double pi = 3.14159265358979323846;
+    std::vector<float> S((size_t)m * m, 0.0f), D((size_t)m * m, 0.0f);
+    for (int p = 0; p < m - 1; p++) {
+        double lp = -4.0 * std::pow(std::sin(pi * (p + 1) / (2.0 * m)), 2);
+        for (int q = 0; q < m - 1; q++) {
+            // (p+1)(q+1) mod 2m keeps the sine argument small
+            long r = ((long)(p + 1) * (q + 1)) % (2L * m);
+            S[(size_t)p * m + q] = (float)std::sin(pi * (double)r / (double)m);
+            double lq = -4.0 * std::pow(std::sin(pi * (q + 1) / (2.0 * m)), 2);
+            D[(size_t)p * m + q] = (float)((2.0 / m) * (2.0 / m) / (lp + lq));
+        }
+    }
+    if (hipMalloc((void **)&M.dst_S, S.size() * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&M.dst_D, D.size() * sizeof(float)) != hipSuccess)
+        return -1;
+    if (hipMemcpy(M.dst_S, S.data(), S.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(M.dst_D, D.data(), D.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return -1;
+    return 0;
+}
+
 // ------------------------------------------------------------ MG driver
 
 template <typename T>
@@ -421,21 +516,27 @@ static void coarse_launch(hipStream_t st, int lc, const T *rhs, T *u, int mode, 
 }
 
 // V(1,1) cycle at level j > lc on (Uj, Rj) (mglin.cpp:176-196)
+// dst (fp32 only): solve the coarsest level exactly instead of by a V-cycle
 template <typename T>
 static void vcycle_launch(hipStream_t st, MgArrays<T> &A, int lc, int j, T *Uj, const T *Rj,
-                          double length, const int *skip) {
+                          double length, const int *skip, const MgLevels *dst = nullptr) {
     int nf = lvl_size(j), nc = lvl_size(j - 1);
     relax_launch<T>(st, Uj, Rj, nf, length, skip);
     double h = length / (nf - 1);
     hipLaunchKernelGGL(k_mg_resid_rstrct<T>, dim3((nc + 127) / 128, nc), dim3(128), 0, st, Uj, Rj,
                        nf, (T)(1.0 / (h * h)), A.cres[j - 1], skip);
     if (j - 1 == lc) {
-        coarse_launch<T>(st, lc, A.cres[j - 1], A.cv[j - 1], 1, 1, length, skip);
+        if constexpr (std::is_same<T, float>::value) {
+            if (dst) coarse_dst_launch(st, *dst, A.cres[j - 1], A.cv[j - 1], length, skip);
+            else coarse_launch<T>(st, lc, A.cres[j - 1], A.cv[j - 1], 1, 1, length, skip);
+        } else {
+            coarse_launch<T>(st, lc, A.cres[j - 1], A.cv[j - 1], 1, 1, length, skip);
+        }
     } else {
         long s = (long)nc * nc;
         hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((s + 255) / 256)), dim3(256), 0, st,
                            A.cv[j - 1], s, (T)0, skip);
-        vcycle_launch<T>(st, A, lc, j - 1, A.cv[j - 1], A.cres[j - 1], length, skip);
+        vcycle_launch<T>(st, A, lc, j - 1, A.cv[j - 1], A.cres[j - 1], length, skip, dst);
     }
     hipLaunchKernelGGL(k_mg_addint<T>, dim3((nf + 127) / 128, nf), dim3(128), 0, st, Uj, nf,
                        A.cv[j - 1], skip);
@@ -452,6 +553,8 @@ int mg_coarse_level_for(long lds_bytes, int ng) {
 void mg_set_coarse_attr(long lds_bytes) {
     hipFuncSetAttribute((const void *)k_mg_coarse<double>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    hipFuncSetAttribute((const void *)k_mg_coarse_dst, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds_bytes);
     hipFuncSetAttribute((const void *)k_mg_coarse<float>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
@@ -579,10 +682,13 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     auto cycle = [&]() {
         hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                            e32, tot, 0.0f, skip);
-        if (lf == M.lc)
-            coarse_launch<float>(st, M.lc, r32, e32, 1, 1, length, skip);
-        else
-            vcycle_launch<float>(st, M.f, M.lc, lf, e32, r32, length, skip);
+        const MgLevels *dst = M.dst_S ? &M : nullptr;
+        if (lf == M.lc) {
+            if (dst) coarse_dst_launch(st, M, r32, e32, length, skip);
+            else coarse_launch<float>(st, M.lc, r32, e32, 1, 1, length, skip);
+        } else {
+            vcycle_launch<float>(st, M.f, M.lc, lf, e32, r32, length, skip, dst);
+        }
         hipLaunchKernelGGL(k_tol_correct, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                            u, e32, tot, M.flags);
         resid();

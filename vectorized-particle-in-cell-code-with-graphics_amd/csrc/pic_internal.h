@@ -69,6 +69,10 @@ struct MgLevels {
     double *relres = nullptr;    // pinned host: last relative residual
     int *host_flags = nullptr;   // pinned host copy of flags
     int pred = 4;                // predicted V-cycles per solve
+    // exact coarsest-level solve of the tolerance mode: discrete sine
+    // transform tables of level lc, M x M with M = 2^(lc+1) (padded by one)
+    float *dst_S = nullptr;      // S[p][q] = sin(pi (p+1)(q+1) / M)
+    float *dst_D = nullptr;      // (2/M)^2 / (lambda_p + lambda_q), h^2 applied in the kernel
 };
 
 // where created particles go during a pass (see k_particles.hip)
@@ -171,6 +175,7 @@ void launch_efield(hipStream_t st, const double *phi, int N, double dx, double *
 template <typename T>
 void launch_fill(hipStream_t st, T *p, long n, T v);
 int mg_coarse_level_for(long lds_bytes, int ng);
+int mg_dst_tables(MgLevels &M);
 void mg_set_coarse_attr(long lds_bytes);
 void mg_fixed_solve(hipStream_t st, MgLevels &M, int ncycle, double length);
 // residual-controlled solve of A u = M.d.rhsF[ng-1] into M.d.u[ng-1] (warm
