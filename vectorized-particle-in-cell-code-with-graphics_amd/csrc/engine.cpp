@@ -96,6 +96,7 @@ int alloc_mg(pic_handle *h, MgLevels &M, int N) {
         CHECK(dalloc(h, &M.d.u[l], s));
         CHECK(dalloc(h, &M.f.rhsF[l], s));
         CHECK(dalloc(h, &M.f.u[l], s));
+        if (l > M.lc) CHECK(dalloc(h, &M.f.us[l], s));
         if (l < M.ng - 1) {
             CHECK(dalloc(h, &M.d.cres[l], s));
             CHECK(dalloc(h, &M.d.cv[l], s));
@@ -118,7 +119,7 @@ int alloc_mg(pic_handle *h, MgLevels &M, int N) {
 void free_mg(MgLevels &M) {
     for (int l = 0; l < 16; l++) {
         void *ptrs[] = {M.d.rhsF[l], M.d.u[l], M.d.cres[l], M.d.cv[l],
-                        M.f.rhsF[l], M.f.u[l], M.f.cres[l], M.f.cv[l]};
+                        M.f.rhsF[l], M.f.u[l], M.f.cres[l], M.f.cv[l], M.f.us[l]};
         for (void *p : ptrs)
             if (p) hipFree(p);
     }

@@ -464,6 +464,125 @@ __global__ void __launch_bounds__(1024)
     }
 }
 
+// ------------------------------------------- fused fp32 V-cycle levels
+// One V(1,1) visit of a level above lc takes two launches instead of seven:
+//   pre:  red then black Gauss-Seidel (mglin.cpp:116-127) and the half-weighted
+//         residual on the coarse nodes (mglin.cpp:133-169);
+//   post: prolongation-add (mglin.cpp:70-97) then red and black again.
+// Each workgroup owns a 32 x 32 tile of fine nodes and recomputes the halo it
+// needs (4 rings for pre, 2 for post) in LDS, so no other tile's output is
+// read; the per-node arithmetic and colour order are those of the unfused
+// kernels, so the iterates are identical.
+#define MGT 32
+#define MGH_PRE 4
+#define MGW_PRE (MGT + 2 * MGH_PRE)
+#define MGH_POST 2
+#define MGW_POST (MGT + 2 * MGH_POST)
+
+// Gauss-Seidel sweep of one colour over the tile plus `ring` halo rings
+// (interior nodes only).  s, f: W x W LDS tiles whose (0,0) is global
+// (y0 - H, x0 - H).
+__device__ __forceinline__ void tile_sweep(float *s, const float *f, int W, int H, int x0, int y0,
+                                           int ring, int n, float h2, int color) {
+    const int lo = H - ring, w = MGT + 2 * ring;
+    for (int k = threadIdx.x; k < w * w; k += blockDim.x) {
+        const int ly = lo + k / w, lx = lo + k % w;
+        const int i = y0 - H + ly, j = x0 - H + lx;
+        if (i < 1 || j < 1 || i > n - 2 || j > n - 2 || ((i + j) & 1) != color) continue;
+        const int c = ly * W + lx;
+        s[c] = 0.25f * (s[c + W] + s[c - W] + s[c + 1] + s[c - 1] - h2 * f[c]);
+    }
+}
+
+__device__ __forceinline__ float tile_resid(const float *s, const float *f, int c, int W, float h2i) {
+    return -h2i * (s[c + W] + s[c - W] + s[c + 1] + s[c - 1] - 4.0f * s[c]) + f[c];
+}
+
+__global__ void __launch_bounds__(256)
+    k_mg_pre(const float *__restrict__ u_in, const float *__restrict__ rhs, int n, float h2,
+             float h2i, float *__restrict__ u_out, float *__restrict__ cres,
+             const int *__restrict__ skip) {
+    SKIP_IF_DONE(skip);
+    __shared__ float su[MGW_PRE * MGW_PRE], sf[MGW_PRE * MGW_PRE];
+    const int x0 = blockIdx.x * MGT, y0 = blockIdx.y * MGT;
+    for (int k = threadIdx.x; k < MGW_PRE * MGW_PRE; k += blockDim.x) {
+        const int ly = k / MGW_PRE, lx = k % MGW_PRE;
+        const int i = y0 - MGH_PRE + ly, j = x0 - MGH_PRE + lx;
+        const bool in = i >= 0 && j >= 0 && i < n && j < n;
+        su[k] = (in && u_in) ? u_in[(long)i * n + j] : 0.0f;  // null: zero initial guess
+        sf[k] = in ? rhs[(long)i * n + j] : 0.0f;
+    }
+    __syncthreads();
+    tile_sweep(su, sf, MGW_PRE, MGH_PRE, x0, y0, 3, n, h2, 0);
+    __syncthreads();
+    tile_sweep(su, sf, MGW_PRE, MGH_PRE, x0, y0, 2, n, h2, 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < MGT * MGT; k += blockDim.x) {
+        const int ly = k / MGT, lx = k % MGT, i = y0 + ly, j = x0 + lx;
+        if (i < n && j < n) u_out[(long)i * n + j] = su[(ly + MGH_PRE) * MGW_PRE + lx + MGH_PRE];
+    }
+    const int nc = (n + 1) / 2;
+    for (int k = threadIdx.x; k < (MGT / 2) * (MGT / 2); k += blockDim.x) {
+        const int ic = y0 / 2 + k / (MGT / 2), jc = x0 / 2 + k % (MGT / 2);
+        if (ic >= nc || jc >= nc) continue;
+        float v = 0.0f;
+        if (ic > 0 && jc > 0 && ic < nc - 1 && jc < nc - 1) {
+            const int c = (2 * ic - y0 + MGH_PRE) * MGW_PRE + (2 * jc - x0 + MGH_PRE);
+            v = 0.5f * tile_resid(su, sf, c, MGW_PRE, h2i) +
+                0.125f * (tile_resid(su, sf, c + MGW_PRE, MGW_PRE, h2i) +
+                          tile_resid(su, sf, c - MGW_PRE, MGW_PRE, h2i) +
+                          tile_resid(su, sf, c + 1, MGW_PRE, h2i) +
+                          tile_resid(su, sf, c - 1, MGW_PRE, h2i));
+        }
+        cres[(long)ic * nc + jc] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    k_mg_post(const float *__restrict__ us, const float *__restrict__ rhs,
+              const float *__restrict__ cv, int n, float h2, float *__restrict__ u_out,
+              const int *__restrict__ skip) {
+    SKIP_IF_DONE(skip);
+    __shared__ float su[MGW_POST * MGW_POST], sf[MGW_POST * MGW_POST];
+    const int x0 = blockIdx.x * MGT, y0 = blockIdx.y * MGT, nc = (n + 1) / 2;
+    for (int k = threadIdx.x; k < MGW_POST * MGW_POST; k += blockDim.x) {
+        const int ly = k / MGW_POST, lx = k % MGW_POST;
+        const int i = y0 - MGH_POST + ly, j = x0 - MGH_POST + lx;
+        const bool in = i >= 0 && j >= 0 && i < n && j < n;
+        su[k] = in ? us[(long)i * n + j] + interp_node(cv, nc, i, j) : 0.0f;
+        sf[k] = in ? rhs[(long)i * n + j] : 0.0f;
+    }
+    __syncthreads();
+    tile_sweep(su, sf, MGW_POST, MGH_POST, x0, y0, 1, n, h2, 0);
+    __syncthreads();
+    tile_sweep(su, sf, MGW_POST, MGH_POST, x0, y0, 0, n, h2, 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < MGT * MGT; k += blockDim.x) {
+        const int ly = k / MGT, lx = k % MGT, i = y0 + ly, j = x0 + lx;
+        if (i < n && j < n) u_out[(long)i * n + j] = su[(ly + MGH_POST) * MGW_POST + lx + MGH_POST];
+    }
+}
+
+static void coarse_dst_launch(hipStream_t st, const MgLevels &M, const float *rhs, float *u,
+                              double length, const int *skip);
+
+// fused V(1,1) at level j > lc on (Uj, Rj); Uj's initial guess is zero
+static void vcycle_fused(hipStream_t st, MgLevels &M, int j, float *Uj, const float *Rj,
+                         double length, const int *skip) {
+    MgArrays<float> &A = M.f;
+    const int nf = lvl_size(j);
+    const double h = length / (nf - 1);
+    const int tiles = (nf + MGT - 1) / MGT;
+    hipLaunchKernelGGL(k_mg_pre, dim3(tiles, tiles), dim3(256), 0, st, (const float *)nullptr, Rj,
+                       nf, (float)(h * h), (float)(1.0 / (h * h)), A.us[j], A.cres[j - 1], skip);
+    if (j - 1 == M.lc)
+        coarse_dst_launch(st, M, A.cres[j - 1], A.cv[j - 1], length, skip);
+    else
+        vcycle_fused(st, M, j - 1, A.cv[j - 1], A.cres[j - 1], length, skip);
+    hipLaunchKernelGGL(k_mg_post, dim3(tiles, tiles), dim3(256), 0, st, A.us[j], Rj, A.cv[j - 1],
+                       nf, (float)(h * h), Uj, skip);
+}
+
 static void coarse_dst_launch(hipStream_t st, const MgLevels &M, const float *rhs, float *u,
                               double length, const int *skip) {
     const int lc = M.lc, m = 1 << (lc + 1);
@@ -680,15 +799,10 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
                            tol, M.flags, M.relres, M.host_flags);
     };
     auto cycle = [&]() {
-        hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                           e32, tot, 0.0f, skip);
-        const MgLevels *dst = M.dst_S ? &M : nullptr;
-        if (lf == M.lc) {
-            if (dst) coarse_dst_launch(st, M, r32, e32, length, skip);
-            else coarse_launch<float>(st, M.lc, r32, e32, 1, 1, length, skip);
-        } else {
-            vcycle_launch<float>(st, M.f, M.lc, lf, e32, r32, length, skip, dst);
-        }
+        if (lf == M.lc)
+            coarse_dst_launch(st, M, r32, e32, length, skip);
+        else
+            vcycle_fused(st, M, lf, e32, r32, length, skip);
         hipLaunchKernelGGL(k_tol_correct, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                            u, e32, tot, M.flags);
         resid();

@@ -478,6 +478,79 @@ __device__ __forceinline__ void block_deposit(bool dep, const PState &s,
     }
 }
 
+// block_deposit for P particles per thread (dep[r] true: deposit s[r]); one
+// window and one flush for the whole block
+template <int P>
+__device__ __forceinline__ void block_deposit_n(const bool (&dep)[P], const PState (&s)[P],
+                                                unsigned long long *__restrict__ q, int N,
+                                                unsigned long long *sq) {
+    __shared__ int red[4][4];
+    const int big = 1 << 30;
+    int mnx = big, mny = big, mxx = -big, mxy = -big;
+#pragma unroll
+    for (int r = 0; r < P; r++)
+        if (dep[r]) {
+            mnx = min(mnx, s[r].jx);
+            mny = min(mny, s[r].jy);
+            mxx = max(mxx, s[r].jx);
+            mxy = max(mxy, s[r].jy);
+        }
+    mnx = wave_min(mnx);
+    mny = wave_min(mny);
+    mxx = wave_max(mxx);
+    mxy = wave_max(mxy);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[w][0] = mnx;
+        red[w][1] = mny;
+        red[w][2] = mxx;
+        red[w][3] = mxy;
+    }
+    __syncthreads();
+    const int nw = blockDim.x >> 6;
+    mnx = red[0][0];
+    mny = red[0][1];
+    mxx = red[0][2];
+    mxy = red[0][3];
+    for (int k = 1; k < nw; k++) {
+        mnx = min(mnx, red[k][0]);
+        mny = min(mny, red[k][1]);
+        mxx = max(mxx, red[k][2]);
+        mxy = max(mxy, red[k][3]);
+    }
+    __syncthreads();
+    if (mxx < mnx) return;
+    const int x0 = mnx, y0 = mny, ww = mxx - mnx + 2, wh = mxy - mny + 2;
+    const int area = ww * wh;
+    if (area > DEP_WCAP) {
+#pragma unroll
+        for (int r = 0; r < P; r++)
+            if (dep[r]) dep_global(q, N, s[r].jx, s[r].jy, fx_weights(s[r].ox, s[r].oy));
+        return;
+    }
+    int copies = DEP_COPIES;
+    while (copies > 1 && copies * area > DEP_WCAP) copies >>= 1;
+    for (int k = threadIdx.x; k < copies * area; k += blockDim.x) sq[k] = 0ull;
+    __syncthreads();
+    const int cb = (int)(threadIdx.x & (copies - 1)) * area;
+#pragma unroll
+    for (int r = 0; r < P; r++)
+        if (dep[r]) {
+            FxW fw = fx_weights(s[r].ox, s[r].oy);
+            int b = cb + (s[r].jy - y0) * ww + (s[r].jx - x0);
+            atomicAdd(&sq[b], fw.a4);
+            atomicAdd(&sq[b + 1], fw.a3);
+            atomicAdd(&sq[b + ww], fw.a2);
+            atomicAdd(&sq[b + ww + 1], fw.a1);
+        }
+    __syncthreads();
+    for (int k = threadIdx.x; k < area; k += blockDim.x) {
+        unsigned long long v = 0ull;
+        for (int c = 0; c < copies; c++) v += sq[c * area + k];
+        if (v) atomicAdd(&q[(long)(y0 + k / ww) * N + x0 + k % ww], v);
+    }
+}
+
 // staging of a created particle (slot order is arbitrary; the commit pass
 // sorts by key)
 __device__ __forceinline__ void stage_put(const StageView &sv, uint32_t *__restrict__ count,
@@ -585,46 +658,23 @@ __global__ void __launch_bounds__(256)
 }
 
 
-// In-place timestep of one species between sort passes: the same collide +
-// push as the sort pass, the state written back to its own slot, the
-// deposit through the LDS window of the block's new cells.  A particle that
-// leaves (or that a collision removes) becomes a hole; the next sort pass
-// compacts the store.  Reads 20-28 B and writes 20-24 B per particle, all
-// coalesced, with no histogram or slot claims.
-__global__ void __launch_bounds__(256)
-    k_push_inplace(SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E,
-                   PushConst pc, mcc::MccConst mc, uint32_t step, unsigned long long *__restrict__ q,
-                   int deposit, Spawn spn, unsigned long long *__restrict__ events,
-                   uint32_t *__restrict__ lost6, uint32_t *__restrict__ holes) {
-    __shared__ uint32_t sl[6];
-    __shared__ uint32_t shole;
-    if (threadIdx.x < 6) sl[threadIdx.x] = 0;
-    if (threadIdx.x == 0) shole = 0;
-    __syncthreads();
-    const uint32_t n = *nptr;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    const bool need_vz = pc.bfield != 0 || mc.on;
-    const bool need_id = mc.on || spn.plop;
-    PState s{}, pre{};
-    int st = ST_HOLE, ev = mcc::EV_NONE;
+// one particle of k_push_inplace: collide (block-wide when blk), push, store
+// in place or leave a hole, count, stage products; dep = deposit it
+__device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint32_t n, PState &s,
+                                              uint32_t pid, bool live, bool &dep,
+                                              const float2 *__restrict__ E, const PushConst &pc,
+                                              const mcc::MccConst &mc, uint32_t step,
+                                              const Spawn &spn, uint32_t *__restrict__ lost6,
+                                              bool need_vz, bool blk, uint32_t *sl, uint32_t *shole,
+                                              uint32_t *se, unsigned long long *lds) {
+    const PState pre = s;
+    int ev = mcc::EV_NONE;
     mcc::Child ch{};
-    uint32_t pid = 0;
-    __shared__ unsigned long long lds[BLOCK_LDS_U64];
-    const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
-    bool live = false;
-    if (valid) {
-        s = load_state(sp, i, need_vz);
-        live = s.jx != CELL_HOLE;
-        if (live) {
-            pid = need_id ? sp.id[i] : 0u;
-            pre = s;
-        }
-    }
     if (blk) block_collide(live, s, pid, mc, step, ev, ch, lds);
+    int st = ST_HOLE;
     if (live) st = advance_one(s, pid, E, pc, mc, step, 1, ev, ch, blk);
     const bool kept = (st == ST_KEPT);
-    if (deposit) block_deposit(kept && deposit_ok(s, pc.N, pc.mode), s, q, pc.N, lds);
+    dep = kept && deposit_ok(s, pc.N, pc.mode);
     if (kept) {
         sp.cell[i] = cell_pack(s.jx, s.jy);
         sp.ox[i] = s.ox;
@@ -632,16 +682,76 @@ __global__ void __launch_bounds__(256)
         sp.vx[i] = s.vx;
         sp.vy[i] = s.vy;
         if (need_vz) sp.vz[i] = s.vz;
-    } else if (valid && st != ST_HOLE) {
+    } else if (live) {
         sp.cell[i] = cell_pack(CELL_HOLE, CELL_HOLE);
         atomicAdd(&sl[st - 1], 1u);
-        atomicAdd(&shole, 1u);
+        atomicAdd(shole, 1u);
     }
-    if (mc.on) count_events(ev, events);
-    stage_products(ev, ch, pre, st, valid, pid, E, pc, mc, step, spn, lost6);
+    if (mc.on) {
+        if (EV_KIND(ev) > 0) atomicAdd(&se[EV_KIND(ev)], 1u);
+        if (ev & EV_OVER_BOUND) atomicAdd(&se[6], 1u);
+    }
+    stage_products(ev, ch, pre, st, i < n, pid, E, pc, mc, step, spn, lost6);
+}
+
+// In-place timestep of one species between sort passes: the same collide +
+// push as the sort pass, the state written back to its own slot, the
+// deposit through one LDS window per block.  A particle that leaves (or that
+// a collision removes) becomes a hole; the next sort pass compacts the store.
+// Each thread handles IPP particles (strided by the block size, so every
+// load and store stays coalesced): one deposit window and one flush of the
+// block's counters then serve 256 * IPP particles.  Reads 20-28 B and writes
+// 20-24 B per particle, with no histogram or slot claims.
+#define IPP 4
+__global__ void __launch_bounds__(256)
+    k_push_inplace(SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E,
+                   PushConst pc, mcc::MccConst mc, uint32_t step, unsigned long long *__restrict__ q,
+                   int deposit, Spawn spn, unsigned long long *__restrict__ events,
+                   uint32_t *__restrict__ lost6, uint32_t *__restrict__ holes) {
+    __shared__ uint32_t sl[6];
+    __shared__ uint32_t shole;
+    __shared__ uint32_t se[8];
+    __shared__ unsigned long long lds[BLOCK_LDS_U64];
+    if (threadIdx.x < 6) sl[threadIdx.x] = 0;
+    if (threadIdx.x < 8) se[threadIdx.x] = 0;
+    if (threadIdx.x == 0) shole = 0;
+    const uint32_t n = *nptr;
+    const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
+    const bool need_vz = pc.bfield != 0 || mc.on;
+    const bool need_id = mc.on || spn.plop;
+    const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
+    PState s[IPP];
+    uint32_t pid[IPP];
+    bool live[IPP];
+#pragma unroll
+    for (int r = 0; r < IPP; r++) {  // every load first
+        const uint32_t i = base + r * blockDim.x;
+        s[r] = PState{};
+        pid[r] = 0u;
+        live[r] = false;
+        if (i < n) {
+            s[r] = load_state(sp, i, need_vz);
+            live[r] = s[r].jx != CELL_HOLE;
+            if (live[r] && need_id) pid[r] = sp.id[i];
+        }
+    }
+    __syncthreads();  // counters initialised
+    bool dep[IPP];
+    // the rounds are spelled out so every array index is a compile-time constant
+    inplace_round(sp, base + 0 * blockDim.x, n, s[0], pid[0], live[0], dep[0], E, pc, mc, step,
+                  spn, lost6, need_vz, blk, sl, &shole, se, lds);
+    inplace_round(sp, base + 1 * blockDim.x, n, s[1], pid[1], live[1], dep[1], E, pc, mc, step,
+                  spn, lost6, need_vz, blk, sl, &shole, se, lds);
+    inplace_round(sp, base + 2 * blockDim.x, n, s[2], pid[2], live[2], dep[2], E, pc, mc, step,
+                  spn, lost6, need_vz, blk, sl, &shole, se, lds);
+    inplace_round(sp, base + 3 * blockDim.x, n, s[3], pid[3], live[3], dep[3], E, pc, mc, step,
+                  spn, lost6, need_vz, blk, sl, &shole, se, lds);
+    if (deposit) block_deposit_n<IPP>(dep, s, q, pc.N, lds);
     __syncthreads();
     if (threadIdx.x < 6 && sl[threadIdx.x])
         atomicAdd(&lost6[cnt_slot() * 6 + threadIdx.x], sl[threadIdx.x]);
+    if (threadIdx.x < 8 && se[threadIdx.x])
+        atomicAdd(&events[cnt_slot() * 8 + threadIdx.x], (unsigned long long)se[threadIdx.x]);
     if (threadIdx.x == 0 && shole) atomicAdd(&holes[cnt_slot()], shole);
 }
 
@@ -1109,7 +1219,7 @@ void launch_push_inplace(hipStream_t st, const SpView &sp, const uint32_t *n, in
                          const float2 *E, const PushConst &pc, const mcc::MccConst &mc,
                          uint32_t step, unsigned long long *q, int deposit, const Spawn &spn,
                          unsigned long long *events, uint32_t *lost6, uint32_t *holes) {
-    hipLaunchKernelGGL(k_push_inplace, dim3(blocks_for(cap, 256)), dim3(256), 0, st, sp, n, E, pc,
+    hipLaunchKernelGGL(k_push_inplace, dim3(blocks_for(cap, 256 * IPP)), dim3(256), 0, st, sp, n, E, pc,
                        mc, step, q, deposit, spn, events, lost6, holes);
 }
 

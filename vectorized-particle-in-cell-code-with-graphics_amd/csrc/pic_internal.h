@@ -56,6 +56,7 @@ struct MgArrays {
     T *u[16] = {};     // full-multigrid solution per level (u[ng-1]: result)
     T *cres[16] = {};  // V-cycle coarse right-hand side
     T *cv[16] = {};    // V-cycle coarse correction
+    T *us[16] = {};    // pre-smoothed iterate of the fused V-cycle (fp32 levels > lc)
 };
 
 struct MgLevels {
