@@ -73,7 +73,8 @@ typedef struct pic_params {
     int32_t mg_max_iter;      /* PIC_MG_TOL: refinement sweeps cap                  */
     int32_t dt_dti;           /* DT_DTI                                             */
     int32_t run_time;         /* TOTAL_RUN_TIME                                     */
-    int32_t number_to_inject; /* NUMBER_TO_INJECT (only 0 is supported)             */
+    int32_t number_to_inject; /* NUMBER_TO_INJECT: particles of each species created
+                                 per step (particle.cpp:491-510)                    */
     int32_t no_ion_species;   /* NO_ION_SPECIES (only 1 is supported)               */
     int32_t corner8;          /* charge post-scale at corners: 1 -> x8 (reference)  */
     uint64_t seed;            /* RAND_SEED                                          */
@@ -106,6 +107,18 @@ typedef struct pic_params {
                                  DIST_LOCATION2Y                                      */
     int32_t probe[6];         /* POINT1X, POINT1Y, POINT2X, POINT2Y, POINT3X, POINT3Y */
     int32_t write_graphs;     /* pic::writeGraphDataToFiles (pic.cpp:58)              */
+    /* particle-count control: contain_particle_growth (particle.cpp:520-653,
+       parameters.h:320-366, set up at pic.cpp:257-327).  Index [0] electrons,
+       [1] ions; group g holds the particles with |(vx,vy)| <= v_limit[s][g]
+       not in a lower group. */
+    int32_t reduce;              /* 1: run it every step (the reference does when
+                                    STEADYSTATE is "NO", pic.cpp:1132-1141)       */
+    int32_t no_groups;           /* NO_GROUPS (1..8)                              */
+    int32_t size_to_check[2];    /* SIZE_TO_CHECK_E, SIZE_TO_CHECK                */
+    int32_t group_limit[2][8];   /* UPPER_GROUP_LIMIT, UPPER_GROUP_LIMIT2         */
+    int32_t new_group_size[2][8];/* NEW_GROUP_SIZE, NEW_GROUP_SIZE2               */
+    double v_limit[2][8];        /* VE1, VE2, UPPER_V_LIMIT / VI1, VI2,
+                                    UPPER_V_LIMIT_I [m/s]                         */
 } pic_params;
 
 /* particleVec.h:26-35: the 48-byte AoS record of the reference */
@@ -168,6 +181,18 @@ int pic_ion_collisions(pic_handle *h);
 /* (sec_ions.*implement_i_boundary) / (sec_e.*implement_e_boundary)
  * (boundary.h:372-383; pic.cpp:1114-1128): which = 0 ion wall, 1 electron wall */
 int pic_boundary(pic_handle *h, int32_t which);
+/* particle::inject_more_particles (particle.cpp:491-510): NUMBER_TO_INJECT
+ * particles of species s appended unmoved, uniform in [dx, L-dx]^2, each
+ * velocity component vt cos(2 pi R2) sqrt|ln R1|; weight 0 in reference mode
+ * (particleVec() value-initialises it), 1 in physical mode.  The fused
+ * timestep does the same after the electron collisions (pic.cpp:1079-1082). */
+int pic_inject_particles(pic_handle *h, int32_t s);
+/* particle::contain_particle_growth (particle.cpp:640-653; reduce_population
+ * 538-632): above size_to_check particles, every velocity group above its
+ * limit drops to its new size by a uniform random choice and its survivors'
+ * weights are multiplied by Ngroup / (Ngroup - removed).  Needs
+ * pic_params.reduce; the fused timestep runs it every step. */
+int pic_contain_particle_growth(pic_handle *h, int32_t s);
 
 /* Step counter used by the per-phase entry points (collision random streams
  * and the ion sub-cycling test); pic_fields_grid_electric_field, the last
@@ -219,6 +244,14 @@ int pic_set_state(pic_handle *h, int32_t s, const uint32_t *cell, const float *o
                   const float *oy, const float *vx, const float *vy, const float *vz,
                   const uint32_t *id, int64_t n);
 int pic_get_qfixed(pic_handle *h, int32_t s, int64_t *out); /* nodes*nodes, 2^-40 units   */
+/* per-particle weights (particleVec.h: weight) in pic_get_state order; the
+ * charge of a particle is weight * qc (pic.cl:46-49).  Setting them switches
+ * the species to per-particle weights. */
+int pic_get_weights(pic_handle *h, int32_t s, float *w, int64_t cap);
+int pic_set_weights(pic_handle *h, int32_t s, const float *w, int64_t n);
+/* out3 = population reductions performed (particle.cpp:648 count_reduce),
+ * particles they removed, particles created by the source */
+int pic_get_population_stats(pic_handle *h, int32_t s, int64_t *out3);
 int pic_get_E32(pic_handle *h, float *out);  /* fp32 (Ex,Ey) pairs, (nodes+1)*nodes+1      */
 int pic_set_E32(pic_handle *h, const float *in);
 

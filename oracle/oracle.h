@@ -323,6 +323,28 @@ int orc_dump_write(const char *path, const orc_particle *p, long n, int precisio
 /* returns count read, or -1 if the file cannot be opened; reads at most cap */
 long orc_dump_read(const char *path, orc_particle *p, long cap);
 
+/* ------------------------------------------------------------------------ */
+/* Particle-count control (orc_control.c): source, reduction, weighted       */
+/* deposit.  Pinned by tests/test_oracle_control.py (distribution of the     */
+/* source against its closed form; quotas, group weights and conserved       */
+/* weight against a literal restatement of reduce_population; weighted      */
+/* deposit against the unweighted one).                                      */
+/* ------------------------------------------------------------------------ */
+void orc_inject(long n, uint32_t species, uint32_t step, uint32_t key0, uint32_t key1, double dx,
+                double L, double vt, uint32_t *cell, float *ox, float *oy, float *vx, float *vy,
+                float *vz);
+typedef struct {
+    int ng;
+    uint32_t size_check, species, key0, key1;
+    double vlim[8];
+    uint32_t glimit[8], gnew[8];
+} orc_reduce_params;
+long orc_reduce(const orc_reduce_params *c, uint32_t step, long n, const float *vx,
+                const float *vy, const uint32_t *id, float *w, uint8_t *keep, uint32_t *ngroup,
+                uint32_t *nrem);
+void orc32_deposit_w(long n, const uint32_t *cell, const float *ox, const float *oy,
+                     const float *w, int N, int mode, int64_t *qfix);
+
 #ifdef __cplusplus
 }
 #endif

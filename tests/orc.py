@@ -61,6 +61,12 @@ class RefCxParams(C.Structure):
                 ("key0", C.c_uint32), ("key1", C.c_uint32)]
 
 
+class ReduceParams(C.Structure):
+    _fields_ = [("ng", C.c_int), ("size_check", C.c_uint32), ("species", C.c_uint32),
+                ("key0", C.c_uint32), ("key1", C.c_uint32), ("vlim", C.c_double * 8),
+                ("glimit", C.c_uint32 * 8), ("gnew", C.c_uint32 * 8)]
+
+
 _lib = None
 
 
@@ -121,6 +127,11 @@ def lib():
         _lib.orc_dump_write.argtypes = [C.c_char_p, vp, C.c_long, C.c_int]
         _lib.orc_dump_read.argtypes = [C.c_char_p, vp, C.c_long]
         _lib.orc_dump_read.restype = C.c_long
+        _lib.orc_inject.argtypes = [C.c_long, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_double, C.c_double, C.c_double] + [vp] * 6
+        _lib.orc_reduce.argtypes = [C.POINTER(ReduceParams), C.c_uint32, C.c_long] + [vp] * 7
+        _lib.orc_reduce.restype = C.c_long
+        _lib.orc32_deposit_w.argtypes = [C.c_long, vp, vp, vp, vp, C.c_int, C.c_int, vp]
     return _lib
 
 
@@ -471,3 +482,48 @@ class Diag:
                                    P(self.cy), P(self.dist))
         lib().orc_diag_vfinal(self.N, P(self.vsx), P(self.vsy), P(self.cx), P(self.cy),
                               P(self.vpx), P(self.vpy))
+
+
+# ---------------------------------------------------------- particle-count control
+def inject(n, species, step, seed, dx, L, vt):
+    """orc_inject: the source's particles (unmoved), fp32 SoA"""
+    d = {k: np.zeros(n, dtype=np.uint32 if k == "cell" else np.float32)
+         for k in ("cell", "ox", "oy", "vx", "vy", "vz")}
+    lib().orc_inject(n, species, step, seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF, dx, L, vt,
+                     *[P(d[k]) for k in ("cell", "ox", "oy", "vx", "vy", "vz")])
+    return d
+
+
+def reduce_params(p, species, seed=None):
+    """ReduceParams of species 0 (electrons) / 1 (ions) from a pic2d.Params"""
+    seed = p.seed if seed is None else seed
+    c = ReduceParams(ng=p.no_groups, size_check=p.size_to_check[species], species=species,
+                     key0=seed & 0xFFFFFFFF, key1=(seed >> 32) & 0xFFFFFFFF)
+    for g in range(8):
+        c.vlim[g] = p.v_limit[species][g]
+        c.glimit[g] = p.group_limit[species][g]
+        c.gnew[g] = p.new_group_size[species][g]
+    return c
+
+
+def reduce(c, step, vx, vy, ids, w):
+    """orc_reduce on copies: (keep mask, new weights, Ngroup[8], Nremove[8], removed)"""
+    n = len(ids)
+    w = np.array(w, dtype=np.float32, copy=True)
+    keep = np.zeros(n, dtype=np.uint8)
+    ng = np.zeros(8, dtype=np.uint32)
+    nr = np.zeros(8, dtype=np.uint32)
+    vx = np.ascontiguousarray(vx, dtype=np.float32)
+    vy = np.ascontiguousarray(vy, dtype=np.float32)
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
+    r = lib().orc_reduce(C.byref(c), step, n, P(vx), P(vy), P(ids), P(w), P(keep), P(ng), P(nr))
+    return keep.astype(bool), w, ng, nr, r
+
+
+def deposit32_w(state, w, N, mode):
+    q = np.zeros(N * N, dtype=np.int64)
+    n = len(state["cell"])
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    lib().orc32_deposit_w(n, P(state["cell"]), P(state["ox"]), P(state["oy"]), P(w), N, mode,
+                          P(q))
+    return q.reshape(N, N)

@@ -74,6 +74,48 @@ def test_loader_evaluates_c_expressions(tmp_path):
     assert p.loader == pic2d.LOAD_RANDOM and p.length == 0.01
 
 
+def test_loader_particle_count_control(tmp_path):
+    """parameters.h:320-366 as pic.cpp:257-327 uses them (three groups)"""
+    f = tmp_path / "parameters.h"
+    f.write_text("""
+#define TEMPERATURE  (double)2.0
+#define VTE sqrt(TEMPERATURE*CHARGE/ELECTRON_MASS)
+#define ION_MASS ((double)39.948*AMU)
+#define VTI sqrt(0.05*CHARGE/ION_MASS)
+#define STEADYSTATE "NO"
+#define NUMBER_TO_INJECT 12
+#define NO_GROUPS (int)3
+#define UPPER_GROUP_LIMIT (int)4000
+#define UPPER_GROUP_LIMIT2 (int)1000
+#define NEW_GROUP_SIZE (int)2000
+#define NEW_GROUP_SIZE2 (int)500
+#define UPPER_V_LIMIT ((double)100.0*VTE)
+#define VE1 (double)1.5*VTE
+#define VI2 (double)4.0*VTI
+#define SIZE_TO_CHECK (int)77
+#define SIZE_TO_CHECK_E (int)99
+""")
+    p = pic2d.load_params(f)
+    vte, vti = p.vte, p.vti
+    assert p.reduce == 1 and p.number_to_inject == 12 and p.no_groups == 3
+    assert list(p.size_to_check) == [99, 77]
+    assert [p.v_limit[0][g] for g in range(3)] == [1.5 * vte, 3.0 * vte, 100.0 * vte]
+    assert [p.v_limit[1][g] for g in range(3)] == [2.0 * vti, 4.0 * vti, 1000.0 * vti]
+    assert [p.group_limit[0][g] for g in range(3)] == [4000] * 3
+    assert [p.new_group_size[0][g] for g in range(3)] == [2000] * 3
+    assert [p.group_limit[1][g] for g in range(3)] == [4000, 1000, 1000]
+    assert [p.new_group_size[1][g] for g in range(3)] == [2000, 500, 500]
+    # steady-state walls: the reference never reduces (pic.cpp:1132)
+    f.write_text('#define STEADYSTATE "YES"\n')
+    assert pic2d.load_params(f).reduce == 0
+    f.write_text('#define STEADYSTATE "YES"\n#define PIC_REDUCE 1\n')
+    assert pic2d.load_params(f).reduce == 1
+    f.write_text('#define NO_GROUPS 4\n')
+    with pytest.raises(pic2d.PicError) as e:
+        pic2d.load_params(f)
+    assert e.value.code == -7
+
+
 def test_loader_errors(tmp_path):
     with pytest.raises(pic2d.PicError) as e:
         pic2d.load_params(tmp_path / "missing.h")

@@ -196,6 +196,10 @@ void species_free(Species &s) {
         for (void *p : ptrs)
             if (p) hipFree(p);
     }
+    for (int b = 0; b < 2; b++)
+        if (s.buf[b].w) hipFree(s.buf[b].w);
+    if (s.stage.w) hipFree(s.stage.w);
+    if (s.rstate) hipFree(s.rstate);
     if (s.q) hipFree(s.q);
     if (s.exp_) hipFree(s.exp_);
     if (s.eyp) hipFree(s.eyp);
@@ -293,7 +297,8 @@ void setup_mcc(pic_handle *h, Species &s, bool electron) {
     c.key0 = (uint32_t)P.seed;
     c.key1 = (uint32_t)(P.seed >> 32);
     c.on = P.have_collisions != 0;
-    if (!electron && c.mode == PIC_DEV_MODE_REF) c.on = 0;  // see pic_create: REF ion CX rejected
+    // reference-mode ions collide through the per-column accumulator (k_refcx.hip)
+    if (!electron && c.mode == PIC_DEV_MODE_REF) c.on = 0;
     if (c.on && c.mode == PIC_DEV_MODE_PHYS) {
         double nu = nu_max_scan(c);
         c.nu_max = (float)nu;
@@ -320,8 +325,8 @@ void make_neutral_table(const pic_params &P, std::vector<float> &t) {
     }
 }
 
-int alloc_stage(pic_handle *h, Species &s, int64_t n0) {
-    int64_t want = std::max<int64_t>(16384, std::min<int64_t>(n0 / 16, (int64_t)1 << 22));
+int alloc_stage(pic_handle *h, Species &s, int64_t n0, int64_t extra) {
+    int64_t want = std::max<int64_t>(16384, std::min<int64_t>(n0 / 16, (int64_t)1 << 22)) + extra;
     uint32_t cap = 1;
     while ((int64_t)cap < want) cap <<= 1;
     StageView &v = s.stage;
@@ -389,6 +394,48 @@ void update_scale(pic_handle *h, Species &s) {
     s.scale = ldexp(s.qc / (h->dx * h->dx) * s.weight, -40);
 }
 
+// switch a species to per-particle weights: every particle takes the
+// species weight as its own, the species factor becomes 1
+int enable_weights(pic_handle *h, Species &s) {
+    if (s.weighted) return PIC_OK;
+    for (int b = 0; b < 2; b++) {
+        CHECK(dalloc(h, &s.buf[b].w, (size_t)s.cap));
+        launch_fill<float>(h->st, s.buf[b].w, (long)s.cap, (float)s.weight);
+    }
+    CHECK(dalloc(h, &s.stage.w, (size_t)s.stage.cap));
+    s.weight = 1.0;
+    s.weighted = true;
+    update_scale(h, s);
+    return PIC_OK;
+}
+
+// inject_more_particles (particle.cpp:491-510) into the species' staging
+// buffer; push: move and wall-test them with this step's field first
+void inject(pic_handle *h, int si, int push) {
+    Species &s = h->sp[si];
+    const pic_params &P = h->P;
+    if (P.number_to_inject <= 0) return;
+    // particleVec() value-initialises the weight to 0 (particleVec.h:26-35):
+    // the reference's injected particles carry no charge; physical mode
+    // gives them weight 1
+    const float w = P.mode == PIC_MODE_REF ? 0.0f : 1.0f;
+    const double vt = si == 0 ? P.vte : P.vti;
+    launch_inject(h->st, s.stage, &h->cnt[si], h->E32, s.pc, s.mc.key0, s.mc.key1,
+                  (uint32_t)h->step, (uint32_t)P.number_to_inject, w, h->dx, P.length, vt,
+                  (uint32_t)si, push, &h->cnt[si].lost[0][0]);
+    s.injected += P.number_to_inject;
+}
+
+// contain_particle_growth (particle.cpp:640-653) on the current buffer;
+// fix_charge: correct the charge deposited this step for the removed and
+// re-weighted particles (the fused step deposits before the reduction)
+void reduce_species(pic_handle *h, int si, bool fix_charge) {
+    Species &s = h->sp[si];
+    launch_reduce(h->st, s.buf[s.cur], &h->cnt[si].n[s.cur], s.cap, s.rc, s.rstate, &h->cnt[si],
+                  (uint32_t)h->step, fix_charge ? s.q : nullptr, h->N, s.pc.mode);
+    s.holes = true;
+}
+
 // absolute position (m) -> cell + offset
 void to_cell(double x, double dx, int &j, float &o) {
     double t = x / dx;
@@ -412,8 +459,9 @@ int upload_particles(pic_handle *h, int si, const std::vector<double> &v) {
     size_t n = v.size() / 6;
     if ((int64_t)n > s.cap) return seterr(h, PIC_ERR_CAPACITY, "particle store too small");
     std::vector<uint32_t> cell(n), id(n);
-    std::vector<float> ox(n), oy(n), vx(n), vy(n), vz(n);
+    std::vector<float> ox(n), oy(n), vx(n), vy(n), vz(n), w(n);
     for (size_t i = 0; i < n; i++) {
+        w[i] = (float)v[6 * i + 5];
         int jx, jy;
         to_cell(v[6 * i], h->dx, jx, ox[i]);
         to_cell(v[6 * i + 1], h->dx, jy, oy[i]);
@@ -431,6 +479,7 @@ int upload_particles(pic_handle *h, int si, const std::vector<double> &v) {
     HIPCHK(h, hipMemcpyAsync(b.vy, vy.data(), 4 * n, hipMemcpyHostToDevice, h->st));
     HIPCHK(h, hipMemcpyAsync(b.vz, vz.data(), 4 * n, hipMemcpyHostToDevice, h->st));
     HIPCHK(h, hipMemcpyAsync(b.id, id.data(), 4 * n, hipMemcpyHostToDevice, h->st));
+    if (b.w) HIPCHK(h, hipMemcpyAsync(b.w, w.data(), 4 * n, hipMemcpyHostToDevice, h->st));
     HIPCHK(h, hipStreamSynchronize(h->st));
     return PIC_OK;
 }
@@ -558,6 +607,10 @@ int fused_step(pic_handle *h) {
     int pp = prof_begin(h, 1);
     launch_fill<unsigned long long>(h->st, h->sp[0].q, (long)N * N, 0ull);
     launch_fill<unsigned long long>(h->st, h->sp[1].q, (long)N * N, 0ull);
+    // particle source after the electron collisions, before the gather
+    // (pic.cpp:1071-1082): electrons are staged already moved, ions unmoved
+    inject(h, 0, 1);
+    inject(h, 1, 0);
     advance_species(h, 0);
     if (h->spawning) {
         // ionisation products: new ions join this step's ion pass (they were
@@ -578,6 +631,9 @@ int fused_step(pic_handle *h) {
         commit(h, 0, 0, 1);
     }
     if (ref_steady(h)) steady_boundary(h);
+    // pic.cpp:1130-1141: electrons, then ions, before the charge is used
+    for (int s = 0; s < 2; s++)
+        if (h->sp[s].reduce) reduce_species(h, s, true);
     prof_end(h, pp);
     CHECK(field_step(h, h->step));
     diag_step(h, h->step);
@@ -632,8 +688,19 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     const pic_params &P = *Pin;
     if (!grid_ok(P.nodes))
         return set_global_error(PIC_ERR_GRID, "nodes-1 must be a power of 2 (and nodes >= 5)");
-    if (P.number_to_inject != 0)
-        return set_global_error(PIC_ERR_UNSUPPORTED, "NUMBER_TO_INJECT != 0 is not supported");
+    if (P.number_to_inject < 0) return set_global_error(PIC_ERR_ARG, "NUMBER_TO_INJECT < 0");
+    if (P.reduce) {
+        if (P.no_groups < 1 || P.no_groups > RD_MAX_GROUPS)
+            return set_global_error(PIC_ERR_ARG, "NO_GROUPS must be 1..8");
+        for (int s = 0; s < 2; s++) {
+            if (P.size_to_check[s] < 0)
+                return set_global_error(PIC_ERR_ARG, "SIZE_TO_CHECK must be >= 0");
+            for (int g = 0; g < P.no_groups; g++)
+                if (P.group_limit[s][g] < 0 || P.new_group_size[s][g] < 0 ||
+                    !(P.v_limit[s][g] >= 0.0))
+                    return set_global_error(PIC_ERR_ARG, "bad velocity-group limits");
+        }
+    }
     if (P.no_ion_species != 1)
         return set_global_error(PIC_ERR_UNSUPPORTED, "NO_ION_SPECIES must be 1");
     if (P.mg_mode != PIC_MG_FIXED && P.mg_mode != PIC_MG_TOL)
@@ -666,6 +733,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
 
     // initial particles
     std::vector<double> dump[2];
+    bool mixed[2] = {false, false};  // a dump whose particles' weights differ
     int64_t n0[2] = {P.init_particle_no, P.init_particle_no};
     if (restart) {
         std::string dir = dump_dir ? dump_dir : ".";
@@ -677,23 +745,43 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
             n0[s] = (int64_t)(dump[s].size() / 6);
             double w0 = n0[s] ? dump[s][5] : 1.0;
             for (int64_t i = 0; i < n0[s]; i++)
-                if (dump[s][6 * i + 5] != w0)
-                    return fail(set_global_error(
-                        PIC_ERR_UNSUPPORTED, path + ": per-particle weights differ (only one "
-                                                    "weight per species is supported)"));
-            h->sp[s].weight = w0;
+                if (dump[s][6 * i + 5] != w0) mixed[s] = true;
+            h->sp[s].weight = mixed[s] ? 1.0 : w0;
         }
     }
     h->spawning = (P.have_collisions && P.mode == PIC_MODE_PHYSICAL) ||
-                  (P.wall == PIC_WALL_STEADY && P.mode == PIC_MODE_PHYSICAL);
+                  (P.wall == PIC_WALL_STEADY && P.mode == PIC_MODE_PHYSICAL) ||
+                  P.number_to_inject > 0;
+    const int64_t n_inj = P.number_to_inject;
     for (int s = 0; s < 2; s++) {
         // created particles need headroom; otherwise counts only shrink
-        int64_t cap = n0[s] + (h->spawning ? std::max<int64_t>(n0[s] / 2, 65536) : 0);
+        int64_t cap = n0[s] + (h->spawning ? std::max<int64_t>(n0[s] / 2, 65536) : 0) + 4096 * n_inj;
         if (cap > 0x7fffffffLL) return fail(set_global_error(PIC_ERR_CAPACITY, "too many particles"));
         rc = species_alloc(h, h->sp[s], cap > 0 ? cap : 1);
         if (rc != PIC_OK) return fail(rc);
-        if ((rc = alloc_stage(h, h->sp[s], n0[s])) != PIC_OK) return fail(rc);
+        if ((rc = alloc_stage(h, h->sp[s], n0[s], 2 * n_inj)) != PIC_OK) return fail(rc);
         update_scale(h, h->sp[s]);
+        Species &sp = h->sp[s];
+        sp.reduce = P.reduce != 0;
+        if (sp.reduce || n_inj > 0 || mixed[s])
+            if ((rc = enable_weights(h, sp)) != PIC_OK) return fail(rc);
+        if (sp.reduce) {
+            ReduceConst &c = sp.rc;
+            c.ng = P.no_groups;
+            c.size_check = (uint32_t)P.size_to_check[s];
+            c.species = (uint32_t)s;
+            c.key0 = (uint32_t)P.seed;  // the collision streams' key
+            c.key1 = (uint32_t)(P.seed >> 32);
+            for (int g = 0; g < RD_MAX_GROUPS; g++) {
+                const bool on = g < P.no_groups;
+                c.vlim[g] = on ? P.v_limit[s][g] : 0.0;
+                c.glimit[g] = on ? (uint32_t)P.group_limit[s][g] : 0u;
+                c.gnew[g] = on ? (uint32_t)P.new_group_size[s][g] : 0u;
+            }
+            if ((rc = dalloc(h, (char **)&sp.rstate, sizeof(ReduceState))) != PIC_OK) return fail(rc);
+            if (hipMemset(sp.rstate, 0, sizeof(ReduceState)) != hipSuccess)
+                return fail(set_global_error(PIC_ERR_DEVICE, "memset failed"));
+        }
     }
     if (P.have_collisions) {
         std::vector<float> tab;
@@ -1097,6 +1185,7 @@ int pic_set_state(pic_handle *h, int32_t s, const uint32_t *cell, const float *o
     HIPCHK(h, hipMemcpyAsync(b.vy, vy, bytes, hipMemcpyHostToDevice, h->st));
     HIPCHK(h, hipMemcpyAsync(b.vz, vz, bytes, hipMemcpyHostToDevice, h->st));
     HIPCHK(h, hipMemcpyAsync(b.id, id, bytes, hipMemcpyHostToDevice, h->st));
+    if (b.w) launch_fill<float>(h->st, b.w, (long)n, 1.0f);  // pic_set_weights sets others
     set_count(h, s, sp.cur, (uint32_t)n);
     clear_holes(h, s);
     sp.since_sort = 1 << 20;  // unsorted upload: the next pass sorts
@@ -1117,18 +1206,86 @@ int pic_get_particles(pic_handle *h, int32_t s, pic_particle *out, int64_t cap) 
     CHECK(pic_particle_size(h, s, &n));
     if (n > cap || !out) return seterr(h, PIC_ERR_CAPACITY, "output buffer too small");
     std::vector<uint32_t> cell(n);
-    std::vector<float> ox(n), oy(n), vx(n), vy(n), vz(n);
+    std::vector<float> ox(n), oy(n), vx(n), vy(n), vz(n), wt;
     CHECK(pic_get_state(h, s, cell.data(), ox.data(), oy.data(), vx.data(), vy.data(), vz.data(),
                         nullptr, n));
-    double dx = h->dx, w = h->sp[s].weight;
+    const Species &sp = h->sp[s];
+    if (sp.weighted) {
+        wt.resize(n);
+        HIPCHK(h, hipMemcpy(wt.data(), sp.buf[sp.cur].w, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    }
+    double dx = h->dx, w = sp.weight;
     for (int64_t i = 0; i < n; i++) {
         out[i].x = ((double)cell_jx(cell[i]) + (double)ox[i]) * dx;
         out[i].y = ((double)cell_jy(cell[i]) + (double)oy[i]) * dx;
         out[i].vx = vx[i];
         out[i].vy = vy[i];
         out[i].vz = vz[i];
-        out[i].weight = w;
+        out[i].weight = sp.weighted ? w * (double)wt[i] : w;
     }
+    return PIC_OK;
+}
+
+// per-particle weights, in the order of pic_get_state (particle.h: weight)
+int pic_get_weights(pic_handle *h, int32_t s, float *w, int64_t cap) {
+    int64_t n;
+    CHECK(pic_particle_size(h, s, &n));
+    if (n > cap || (!w && n > 0)) return seterr(h, PIC_ERR_CAPACITY, "output buffer too small");
+    const Species &sp = h->sp[s];
+    if (!sp.weighted) {
+        for (int64_t i = 0; i < n; i++) w[i] = (float)sp.weight;
+        return PIC_OK;
+    }
+    HIPCHK(h, hipMemcpyAsync(w, sp.buf[sp.cur].w, 4 * (size_t)n, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return PIC_OK;
+}
+
+int pic_set_weights(pic_handle *h, int32_t s, const float *w, int64_t n) {
+    int64_t cur;
+    CHECK(pic_particle_size(h, s, &cur));
+    if (n != cur) return seterr(h, PIC_ERR_ARG, "weight count must equal the particle count");
+    if (!w && n > 0) return seterr(h, PIC_ERR_ARG, "null array");
+    Species &sp = h->sp[s];
+    CHECK(enable_weights(h, sp));
+    HIPCHK(h, hipMemcpyAsync(sp.buf[sp.cur].w, w, 4 * (size_t)n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return PIC_OK;
+}
+
+// particle::inject_more_particles (particle.cpp:491-510) as a phase of its
+// own: NUMBER_TO_INJECT new particles appended unmoved
+int pic_inject_particles(pic_handle *h, int32_t s) {
+    CHECK(check_species(h, s));
+    if (h->P.number_to_inject <= 0) return PIC_OK;
+    compact_species(h, s);
+    inject(h, s, 0);
+    commit(h, s, s, 0);
+    h->sp[s].since_sort = 1 << 20;  // appended unsorted: the next pass sorts
+    return PIC_OK;
+}
+
+// particle::contain_particle_growth (particle.cpp:640-653) as a phase of its own
+int pic_contain_particle_growth(pic_handle *h, int32_t s) {
+    CHECK(check_species(h, s));
+    Species &sp = h->sp[s];
+    if (!sp.reduce) return seterr(h, PIC_ERR_ARG, "population reduction is off (pic_params.reduce)");
+    compact_species(h, s);
+    reduce_species(h, s, false);  // collect_charge follows (pic.cpp:1139-1150)
+    compact_species(h, s);
+    return PIC_OK;
+}
+
+// out3 = reductions performed (particle.cpp:648 count_reduce), particles
+// removed by them, particles created by the source
+int pic_get_population_stats(pic_handle *h, int32_t s, int64_t *out3) {
+    CHECK(check_species(h, s));
+    if (!out3) return seterr(h, PIC_ERR_ARG, "null output");
+    CHECK(read_counters(h));
+    out3[0] = h->cnt_host[s].reductions;
+    out3[1] = 0;
+    for (int b = 0; b < CNT_SLOTS; b++) out3[1] += h->cnt_host[s].lost[b][5];
+    out3[2] = h->sp[s].injected;
     return PIC_OK;
 }
 

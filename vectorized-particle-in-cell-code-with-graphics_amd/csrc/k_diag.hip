@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256)
     r[2] = sp.vx[i];
     r[3] = sp.vy[i];
     r[4] = sp.vz[i];
-    r[5] = weight;
+    r[5] = weight * (double)load_w(sp, i);
 }
 
 void launch_subset(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap_sp,

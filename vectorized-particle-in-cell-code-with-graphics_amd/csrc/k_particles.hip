@@ -42,6 +42,7 @@ __device__ __forceinline__ void store_state(const SpView &v, uint32_t i, const P
     v.id[i] = id;
 }
 
+
 // Wavefront key grouping: lanes holding the same key form one group (one per
 // distinct key in the wavefront; leader = lowest lane).  Pure ALU/ballot work,
 // so the atomics that follow are issued by all leaders in a single
@@ -400,15 +401,6 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-__device__ __forceinline__ void dep_global(unsigned long long *q, int N, int jx, int jy,
-                                           const FxW &w) {
-    long b = (long)jy * N + jx;
-    atomicAdd(&q[b], w.a4);
-    atomicAdd(&q[b + 1], w.a3);
-    atomicAdd(&q[b + N], w.a2);
-    atomicAdd(&q[b + N + 1], w.a1);
-}
-
 // Deposit the block's particles (dep true) through an LDS window spanning the
 // bounding box of their cells (block reduction), flushed with one 64-bit
 // atomic per touched node; a block whose box exceeds DEP_WCAP nodes deposits
@@ -421,7 +413,7 @@ __device__ __forceinline__ void dep_global(unsigned long long *q, int N, int jx,
 #define DEP_COPIES 8
 __device__ __forceinline__ void block_deposit(bool dep, const PState &s,
                                               unsigned long long *__restrict__ q, int N,
-                                              unsigned long long *sq) {
+                                              unsigned long long *sq, float pw = 1.0f) {
     __shared__ int red[4][4];
     const int big = 1 << 30;
     int mnx = dep ? s.jx : big, mny = dep ? s.jy : big;
@@ -455,7 +447,7 @@ __device__ __forceinline__ void block_deposit(bool dep, const PState &s,
     const int x0 = mnx, y0 = mny, ww = mxx - mnx + 2, wh = mxy - mny + 2;
     const int area = ww * wh;
     if (area > DEP_WCAP) {
-        if (dep) dep_global(q, N, s.jx, s.jy, fx_weights(s.ox, s.oy));
+        if (dep) dep_global(q, N, s.jx, s.jy, fx_weights_w(s.ox, s.oy, pw));
         return;
     }
     int copies = DEP_COPIES;
@@ -463,7 +455,7 @@ __device__ __forceinline__ void block_deposit(bool dep, const PState &s,
     for (int k = threadIdx.x; k < copies * area; k += blockDim.x) sq[k] = 0ull;
     __syncthreads();
     if (dep) {
-        FxW fw = fx_weights(s.ox, s.oy);
+        FxW fw = fx_weights_w(s.ox, s.oy, pw);
         int b = (int)(threadIdx.x & (copies - 1)) * area + (s.jy - y0) * ww + (s.jx - x0);
         atomicAdd(&sq[b], fw.a4);
         atomicAdd(&sq[b + 1], fw.a3);
@@ -482,6 +474,7 @@ __device__ __forceinline__ void block_deposit(bool dep, const PState &s,
 // window and one flush for the whole block
 template <int P>
 __device__ __forceinline__ void block_deposit_n(const bool (&dep)[P], const PState (&s)[P],
+                                                const float (&pw)[P],
                                                 unsigned long long *__restrict__ q, int N,
                                                 unsigned long long *sq) {
     __shared__ int red[4][4];
@@ -525,7 +518,7 @@ __device__ __forceinline__ void block_deposit_n(const bool (&dep)[P], const PSta
     if (area > DEP_WCAP) {
 #pragma unroll
         for (int r = 0; r < P; r++)
-            if (dep[r]) dep_global(q, N, s[r].jx, s[r].jy, fx_weights(s[r].ox, s[r].oy));
+            if (dep[r]) dep_global(q, N, s[r].jx, s[r].jy, fx_weights_w(s[r].ox, s[r].oy, pw[r]));
         return;
     }
     int copies = DEP_COPIES;
@@ -536,7 +529,7 @@ __device__ __forceinline__ void block_deposit_n(const bool (&dep)[P], const PSta
 #pragma unroll
     for (int r = 0; r < P; r++)
         if (dep[r]) {
-            FxW fw = fx_weights(s[r].ox, s[r].oy);
+            FxW fw = fx_weights_w(s[r].ox, s[r].oy, pw[r]);
             int b = cb + (s[r].jy - y0) * ww + (s[r].jx - x0);
             atomicAdd(&sq[b], fw.a4);
             atomicAdd(&sq[b + 1], fw.a3);
@@ -555,7 +548,7 @@ __device__ __forceinline__ void block_deposit_n(const bool (&dep)[P], const PSta
 // sorts by key)
 __device__ __forceinline__ void stage_put(const StageView &sv, uint32_t *__restrict__ count,
                                           uint32_t *__restrict__ overflow, const PState &s,
-                                          unsigned long long key) {
+                                          unsigned long long key, float w = 1.0f) {
     uint32_t slot = atomicAdd(count, 1u);
     if (slot >= sv.cap) {
         atomicAdd(overflow, 1u);
@@ -568,6 +561,7 @@ __device__ __forceinline__ void stage_put(const StageView &sv, uint32_t *__restr
     sv.vy[slot] = s.vy;
     sv.vz[slot] = s.vz;
     sv.key[slot] = key;
+    if (sv.w) sv.w[slot] = w;
 }
 
 __device__ __forceinline__ void count_events(int ev, unsigned long long *__restrict__ events) {
@@ -588,7 +582,8 @@ __device__ __forceinline__ void stage_products(int ev, const mcc::Child &ch, con
                                                int st, bool valid, uint32_t pid,
                                                const float2 *__restrict__ E, const PushConst &pc,
                                                const mcc::MccConst &mc, uint32_t step,
-                                               const Spawn &sp, uint32_t *__restrict__ lost6) {
+                                               const Spawn &sp, uint32_t *__restrict__ lost6,
+                                               float w = 1.0f) {
     if (EV_KIND(ev) == mcc::EV_IONIZE && mc.mode == PIC_DEV_MODE_PHYS) {
         // ejected electron: born at the parent's position, pushed this step
         // as the reference pushes particles appended before move_particles
@@ -604,20 +599,22 @@ __device__ __forceinline__ void stage_products(int ev, const mcc::Child &ch, con
             atomicAdd(&lost6[cnt_slot() * 6 + cst - 1], 1u);
             ce.jx = ce.jy = CELL_DEAD;
         }
-        stage_put(sp.se, &sp.ce->stage_count, &sp.ce->overflow, ce, (unsigned long long)pid);
+        stage_put(sp.se, &sp.ce->stage_count, &sp.ce->overflow, ce, (unsigned long long)pid, w);
         // new ion at the same place; the ion pass of this step moves it
         PState ci = pre;
         ci.vx = ch.ivx;
         ci.vy = ch.ivy;
         ci.vz = ch.ivz;
-        stage_put(sp.si, &sp.ci->stage_count, &sp.ci->overflow, ci, (unsigned long long)pid);
+        // products take the parent's weight (e_collisions.cpp:413,441)
+        stage_put(sp.si, &sp.ci->stage_count, &sp.ci->overflow, ci, (unsigned long long)pid, w);
     }
     if (sp.plop && valid && st >= ST_LEFT && st <= ST_TOP) {
         PState pe, pi;
         mcc::plop_pair(mc, pid, step, pc.N, sp.se_sigma, sp.si_sigma, sp.plop_ref, pe, pi);
         unsigned long long k = (1ull << 32) | pid;
-        stage_put(sp.se, &sp.ce->stage_count, &sp.ce->overflow, pe, k);
-        stage_put(sp.si, &sp.ci->stage_count, &sp.ci->overflow, pi, k);
+        // the re-injected pair replaces the lost ion: it carries its weight
+        stage_put(sp.se, &sp.ce->stage_count, &sp.ce->overflow, pe, k, w);
+        stage_put(sp.si, &sp.ci->stage_count, &sp.ci->overflow, pi, k, w);
     }
 }
 
@@ -634,11 +631,13 @@ __global__ void __launch_bounds__(256)
     int st = -1, ev = mcc::EV_NONE;
     mcc::Child ch{};
     uint32_t key = 0, pid = 0;
+    float w = 1.0f;
     __shared__ unsigned long long lds[BLOCK_LDS_U64];
     const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
     if (valid) {
         s = load_state(src, i, true);
         pid = src.id[i];
+        w = load_w(src, i);
         pre = s;
     }
     if (blk) block_collide(valid && s.jx != CELL_HOLE, s, pid, mc, step, ev, ch, lds);
@@ -649,12 +648,15 @@ __global__ void __launch_bounds__(256)
     const bool kept = (st == ST_KEPT);
     Claim claim = wave_claim_issue(binCount, binStart, key, kept);
     // the deposit runs while the claim's atomic is in flight
-    if (deposit) block_deposit(kept && deposit_ok(s, pc.N, pc.mode), s, q, pc.N, lds);
+    if (deposit) block_deposit(kept && deposit_ok(s, pc.N, pc.mode), s, q, pc.N, lds, w);
     uint32_t slot = wave_claim_slot(claim, kept);
     // the bound test only keeps a corrupted histogram from writing out of range
-    if (kept && slot < cap) store_state(dst, slot, s, pid);
+    if (kept && slot < cap) {
+        store_state(dst, slot, s, pid);
+        if (dst.w) dst.w[slot] = w;
+    }
     if (mc.on) count_events(ev, events);
-    stage_products(ev, ch, pre, st, valid, pid, E, pc, mc, step, sp, lost6);
+    stage_products(ev, ch, pre, st, valid, pid, E, pc, mc, step, sp, lost6, w);
 }
 
 
@@ -666,7 +668,7 @@ __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint
                                               const mcc::MccConst &mc, uint32_t step,
                                               const Spawn &spn, uint32_t *__restrict__ lost6,
                                               bool need_vz, bool blk, uint32_t *sl, uint32_t *shole,
-                                              uint32_t *se, unsigned long long *lds) {
+                                              uint32_t *se, unsigned long long *lds, float w) {
     const PState pre = s;
     int ev = mcc::EV_NONE;
     mcc::Child ch{};
@@ -691,7 +693,7 @@ __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint
         if (EV_KIND(ev) > 0) atomicAdd(&se[EV_KIND(ev)], 1u);
         if (ev & EV_OVER_BOUND) atomicAdd(&se[6], 1u);
     }
-    stage_products(ev, ch, pre, st, i < n, pid, E, pc, mc, step, spn, lost6);
+    stage_products(ev, ch, pre, st, i < n, pid, E, pc, mc, step, spn, lost6, w);
 }
 
 // In-place timestep of one species between sort passes: the same collide +
@@ -723,30 +725,33 @@ __global__ void __launch_bounds__(256)
     PState s[IPP];
     uint32_t pid[IPP];
     bool live[IPP];
+    float w[IPP];
 #pragma unroll
     for (int r = 0; r < IPP; r++) {  // every load first
         const uint32_t i = base + r * blockDim.x;
         s[r] = PState{};
         pid[r] = 0u;
         live[r] = false;
+        w[r] = 1.0f;
         if (i < n) {
             s[r] = load_state(sp, i, need_vz);
             live[r] = s[r].jx != CELL_HOLE;
             if (live[r] && need_id) pid[r] = sp.id[i];
+            if (live[r]) w[r] = load_w(sp, i);
         }
     }
     __syncthreads();  // counters initialised
     bool dep[IPP];
     // the rounds are spelled out so every array index is a compile-time constant
     inplace_round(sp, base + 0 * blockDim.x, n, s[0], pid[0], live[0], dep[0], E, pc, mc, step,
-                  spn, lost6, need_vz, blk, sl, &shole, se, lds);
+                  spn, lost6, need_vz, blk, sl, &shole, se, lds, w[0]);
     inplace_round(sp, base + 1 * blockDim.x, n, s[1], pid[1], live[1], dep[1], E, pc, mc, step,
-                  spn, lost6, need_vz, blk, sl, &shole, se, lds);
+                  spn, lost6, need_vz, blk, sl, &shole, se, lds, w[1]);
     inplace_round(sp, base + 2 * blockDim.x, n, s[2], pid[2], live[2], dep[2], E, pc, mc, step,
-                  spn, lost6, need_vz, blk, sl, &shole, se, lds);
+                  spn, lost6, need_vz, blk, sl, &shole, se, lds, w[2]);
     inplace_round(sp, base + 3 * blockDim.x, n, s[3], pid[3], live[3], dep[3], E, pc, mc, step,
-                  spn, lost6, need_vz, blk, sl, &shole, se, lds);
-    if (deposit) block_deposit_n<IPP>(dep, s, q, pc.N, lds);
+                  spn, lost6, need_vz, blk, sl, &shole, se, lds, w[3]);
+    if (deposit) block_deposit_n<IPP>(dep, s, w, q, pc.N, lds);
     __syncthreads();
     if (threadIdx.x < 6 && sl[threadIdx.x])
         atomicAdd(&lost6[cnt_slot() * 6 + threadIdx.x], sl[threadIdx.x]);
@@ -791,10 +796,14 @@ __global__ void __launch_bounds__(256)
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = i < n;
     PState s{};
-    if (valid) s = load_state(sp, i, false);
+    float w = 1.0f;
+    if (valid) {
+        s = load_state(sp, i, false);
+        w = load_w(sp, i);
+    }
     bool dep = valid && deposit_ok(s, N, mode);
     __shared__ unsigned long long lds[BLOCK_LDS_U64];
-    block_deposit(dep, s, q, N, lds);
+    block_deposit(dep, s, q, N, lds, w);
 }
 
 // e_collisions::collision / ion_collisions::collision as a phase of its own
@@ -831,8 +840,9 @@ __global__ void __launch_bounds__(256)
         ci.vx = ch.ivx;
         ci.vy = ch.ivy;
         ci.vz = ch.ivz;
-        stage_put(spn.se, &spn.ce->stage_count, &spn.ce->overflow, ce, (unsigned long long)pid);
-        stage_put(spn.si, &spn.ci->stage_count, &spn.ci->overflow, ci, (unsigned long long)pid);
+        const float w = load_w(sp, i);
+        stage_put(spn.se, &spn.ce->stage_count, &spn.ce->overflow, ce, (unsigned long long)pid, w);
+        stage_put(spn.si, &spn.ci->stage_count, &spn.ci->overflow, ci, (unsigned long long)pid, w);
     }
 }
 
@@ -853,9 +863,11 @@ __global__ void __launch_bounds__(256)
     int st = -1;
     PState s{};
     uint32_t pid = 0;
+    float wt = 1.0f;
     if (i < n) {
         s = load_state(src, i, true);
         pid = src.id[i];
+        wt = load_w(src, i);
         st = classify(s, N, mode);
         keep = (st == ST_KEPT);
         if (!keep && st != ST_HOLE) atomicAdd(&sl[st - 1], 1u);
@@ -875,15 +887,18 @@ __global__ void __launch_bounds__(256)
         bbase = t ? atomicAdd(nout, t) : 0u;
     }
     __syncthreads();
-    if (keep) store_state(dst, bbase + wbase[w] + rank, s, pid);
+    if (keep) {
+        store_state(dst, bbase + wbase[w] + rank, s, pid);
+        if (dst.w) dst.w[bbase + wbase[w] + rank] = wt;
+    }
     if (threadIdx.x < 6 && sl[threadIdx.x])
         atomicAdd(&lost6[cnt_slot() * 6 + threadIdx.x], sl[threadIdx.x]);
     if (spn.plop && st >= ST_LEFT && st <= ST_TOP) {
         PState pe, pi;
         mcc::plop_pair(mc, pid, step, N, spn.se_sigma, spn.si_sigma, spn.plop_ref, pe, pi);
         unsigned long long k = (1ull << 32) | pid;
-        stage_put(spn.se, &spn.ce->stage_count, &spn.ce->overflow, pe, k);
-        stage_put(spn.si, &spn.ci->stage_count, &spn.ci->overflow, pi, k);
+        stage_put(spn.se, &spn.ce->stage_count, &spn.ce->overflow, pe, k, wt);
+        stage_put(spn.si, &spn.ci->stage_count, &spn.ci->overflow, pi, k, wt);
     }
 }
 
@@ -964,8 +979,10 @@ __global__ void __launch_bounds__(1024)
                 p.vy = sv.vy[k];
                 p.vz = sv.vz[k];
                 store_state(dst, slot, p, nid + r);
+                const float w = sv.w ? sv.w[k] : 1.0f;
+                if (dst.w) dst.w[slot] = w;
                 if (deposit && deposit_ok(p, N, mode))
-                    dep_global(q, N, p.jx, p.jy, fx_weights(p.ox, p.oy));
+                    dep_global(q, N, p.jx, p.jy, fx_weights_w(p.ox, p.oy, w));
             }
         }
         stored_before += chunk_total;
@@ -1075,8 +1092,11 @@ __device__ __forceinline__ void plop_store(SpView e, SpView ion, uint32_t ie, ui
     ion.vx[ii] = pi.vx;
     ion.vy[ii] = pi.vy;
     ion.vz[ii] = pi.vz;
-    if (deposit_ok(pe, N, mode)) dep_global(qe, N, pe.jx, pe.jy, fx_weights(pe.ox, pe.oy));
-    if (deposit_ok(pi, N, mode)) dep_global(qi, N, pi.jx, pi.jy, fx_weights(pi.ox, pi.oy));
+    // each particle keeps its own weight factor
+    if (deposit_ok(pe, N, mode))
+        dep_global(qe, N, pe.jx, pe.jy, fx_weights_w(pe.ox, pe.oy, load_w(e, ie)));
+    if (deposit_ok(pi, N, mode))
+        dep_global(qi, N, pi.jx, pi.jy, fx_weights_w(pi.ox, pi.oy, load_w(ion, ii)));
 }
 
 // Reference steady-state electron wall (steady_e_boundary, boundary.cpp:
@@ -1192,10 +1212,76 @@ __global__ void __launch_bounds__(256)
     sp.id[i] = i;
 }
 
+// ------------------------------------------------------------ particle source
+// inject_more_particles (particle.cpp:491-510): n particles a step, uniform in
+// [dx, L-dx]^2, each velocity component vt cos(2 pi R2) sqrt|ln R1|
+// (particle.cpp:452-464).  Draws: Philox counter (k, step, species*16 +
+// RS_INJECT + 256 d, 0x4D434331), d = 0 position, 1..3 vx, vy, vz.  In the
+// reference the source runs after the electron collisions and before the
+// gather (pic.cpp:1071-1082): injected electrons are pushed and wall-tested
+// here (no collision this step) and staged at their new position; injected
+// ions are staged unmoved and join the ion pass.  Staging key 2 << 32 | k.
+__device__ __forceinline__ void inject_pos(double x, double dx, int &j, float &o) {
+    const double t = x / dx;
+    const double k = floor(t);
+    float f = (float)(t - k);
+    int jj = (int)k;
+    if (f >= 1.0f) {
+        f = 0.0f;
+        jj += 1;
+    }
+    j = jj;
+    o = f;
+}
+
+__device__ __forceinline__ float inject_vel(const uint32_t r[4], double vt) {
+    const double R1 = u01d(r[0], r[1]), R2 = u01d(r[2], r[3]);
+    return (float)(vt * cos(6.283185307179586 * R2) * sqrt(fabs(log(R1))));
+}
+
+__global__ void __launch_bounds__(256)
+    k_inject(StageView sv, SpCounters *__restrict__ cnt, const float2 *__restrict__ E, PushConst pc,
+             uint32_t key0, uint32_t key1, uint32_t step, uint32_t n_inj, float w, double dx,
+             double L, double vt, uint32_t species, int push, uint32_t *__restrict__ lost6) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_inj) return;
+    const uint32_t s0 = species * 16u + RS_INJECT;
+    uint32_t a[4], b[4], c[4], d[4];
+    philox4x32_10(k, step, s0, RNG_DOMAIN_MCC, key0, key1, a);
+    philox4x32_10(k, step, s0 + 256u, RNG_DOMAIN_MCC, key0, key1, b);
+    philox4x32_10(k, step, s0 + 512u, RNG_DOMAIN_MCC, key0, key1, c);
+    philox4x32_10(k, step, s0 + 768u, RNG_DOMAIN_MCC, key0, key1, d);
+    const double span = L - 2.0 * dx;
+    PState s;
+    inject_pos(fma(u01d(a[0], a[1]), span, dx), dx, s.jx, s.ox);
+    inject_pos(fma(u01d(a[2], a[3]), span, dx), dx, s.jy, s.oy);
+    s.vx = inject_vel(b, vt);
+    s.vy = inject_vel(c, vt);
+    s.vz = inject_vel(d, vt);
+    int st = ST_KEPT;
+    if (push) {
+        push_one(s, E, pc);
+        st = classify(s, pc.N, pc.mode);
+    }
+    if (st == ST_KEPT)
+        stage_put(sv, &cnt->stage_count, &cnt->overflow, s, (2ull << 32) | k, w);
+    else
+        atomicAdd(&lost6[cnt_slot() * 6 + st - 1], 1u);
+}
+
 // ------------------------------------------------------------ launchers
 static inline unsigned blocks_for(int64_t n, int t) {
     int64_t b = (n + t - 1) / t;
     return (unsigned)(b > 0 ? b : 1);
+}
+
+void launch_inject(hipStream_t st, const StageView &sv, SpCounters *cnt, const float2 *E,
+                   const PushConst &pc, uint32_t key0, uint32_t key1, uint32_t step, uint32_t n_inj,
+                   float w, double dx, double L, double vt, uint32_t species, int push,
+                   uint32_t *lost6) {
+    if (n_inj == 0) return;
+    hipLaunchKernelGGL(k_inject, dim3(blocks_for(n_inj, 256)), dim3(256), 0, st, sv, cnt, E, pc, key0,
+                       key1, step, n_inj, w, dx, L, vt, species, push, lost6);
 }
 
 void launch_push_count(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,

@@ -252,7 +252,9 @@ struct Child {
 #define RS_PLOP 3
 #define RS_PLOP2 4
 // 5: second draw of a re-injected pair
-#define RS_REFCX 9  // reference-mode charge exchange: neutral table index
+#define RS_REFCX 9   // reference-mode charge exchange: neutral table index
+#define RS_INJECT 10  // particle source: + 256 k for the k-th draw of a particle
+#define RS_REDUCE 11  // population reduction: selection key
 
 // Electron collision.  Returns the event; on EV_IONIZE in PHYS mode `ch`
 // holds the ejected electron and the new ion; in REF mode EV_IONIZE means the

@@ -370,6 +370,32 @@ static void pic_params_default_diagnostics(pic_params *P, int period) {
     P->write_graphs = 1;
 }
 
+// particle-count control as pic.cpp:257-327 sets it up from parameters.h:
+// three velocity groups per species; on when the walls are not steady-state
+static void pic_params_default_control(pic_params *P) {
+    P->reduce = P->wall == PIC_WALL_STEADY ? 0 : 1;
+    P->no_groups = 3;
+    P->size_to_check[0] = 1000001;  // SIZE_TO_CHECK_E
+    P->size_to_check[1] = 1000001;  // SIZE_TO_CHECK
+    for (int s = 0; s < 2; s++)
+        for (int g = 0; g < 8; g++) {
+            P->group_limit[s][g] = 0;
+            P->new_group_size[s][g] = 0;
+            P->v_limit[s][g] = 0.0;
+        }
+    const double vte = P->vte, vti = P->vti;
+    const double ve[3] = {2.0 * vte, 3.0 * vte, 1000.0 * vte};  // VE1, VE2, UPPER_V_LIMIT
+    const double vi[3] = {2.0 * vti, 3.0 * vti, 1000.0 * vti};  // VI1, VI2, UPPER_V_LIMIT_I
+    for (int g = 0; g < 3; g++) {
+        P->v_limit[0][g] = ve[g];
+        P->group_limit[0][g] = 400000;  // UPPER_GROUP_LIMIT
+        P->new_group_size[0][g] = 200000;  // NEW_GROUP_SIZE
+        P->v_limit[1][g] = vi[g];
+        P->group_limit[1][g] = g == 0 ? 400000 : 100000;  // UPPER_GROUP_LIMIT(2)
+        P->new_group_size[1][g] = g == 0 ? 200000 : 50000;  // NEW_GROUP_SIZE(2)
+    }
+}
+
 extern "C" int pic_params_default(pic_params *P) {
     if (!P) return PIC_ERR_ARG;
     memset(P, 0, sizeof(*P));
@@ -422,6 +448,7 @@ extern "C" int pic_params_default(pic_params *P) {
     P->magnetic_field = 0.0;
     P->angle = 30.0 * PI / 180.0;
     pic_params_default_diagnostics(P, (int)period);
+    pic_params_default_control(P);
     return PIC_OK;
 }
 
@@ -603,6 +630,36 @@ extern "C" int pic_params_load(const char *path, pic_params *P) {
     ok = ok && inum("POINT2Y", P->probe[3]);
     ok = ok && inum("POINT3X", P->probe[4]);
     ok = ok && inum("POINT3Y", P->probe[5]);
+    // particle-count control (parameters.h:320-366) follows the loaded
+    // thermal speeds and walls; explicit macros win
+    pic_params_default_control(P);
+    if (ok && has("NO_GROUPS")) {
+        ok = inum("NO_GROUPS", P->no_groups);
+        if (ok && P->no_groups != 3)
+            return pic_host::set_global_error(
+                PIC_ERR_UNSUPPORTED, "NO_GROUPS must be 3 (pic.cpp:257-327 sets three groups)");
+    }
+    ok = ok && inum("SIZE_TO_CHECK_E", P->size_to_check[0]);
+    ok = ok && inum("SIZE_TO_CHECK", P->size_to_check[1]);
+    int32_t ugl = P->group_limit[0][0], ugl2 = P->group_limit[1][1];
+    int32_t ngs = P->new_group_size[0][0], ngs2 = P->new_group_size[1][1];
+    ok = ok && inum("UPPER_GROUP_LIMIT", ugl);
+    ok = ok && inum("UPPER_GROUP_LIMIT2", ugl2);
+    ok = ok && inum("NEW_GROUP_SIZE", ngs);
+    ok = ok && inum("NEW_GROUP_SIZE2", ngs2);
+    ok = ok && num("VE1", P->v_limit[0][0]);
+    ok = ok && num("VE2", P->v_limit[0][1]);
+    ok = ok && num("UPPER_V_LIMIT", P->v_limit[0][2]);
+    ok = ok && num("VI1", P->v_limit[1][0]);
+    ok = ok && num("VI2", P->v_limit[1][1]);
+    ok = ok && num("UPPER_V_LIMIT_I", P->v_limit[1][2]);
+    for (int g = 0; g < 3; g++) {
+        P->group_limit[0][g] = ugl;
+        P->new_group_size[0][g] = ngs;
+        P->group_limit[1][g] = g == 0 ? ugl : ugl2;
+        P->new_group_size[1][g] = g == 0 ? ngs : ngs2;
+    }
+    if (ok && has("PIC_REDUCE")) ok = inum("PIC_REDUCE", P->reduce);
     ok = ok && num("PIC_SIGMA_LOAD_E", P->sigma_load_e);
     ok = ok && num("PIC_SIGMA_LOAD_I", P->sigma_load_i);
     if (!ok)

@@ -191,6 +191,30 @@ __device__ __forceinline__ FxW fx_weights(float ox, float oy) {
     return w;
 }
 
+// weighted fixed-point corner weights: the exact integer products scaled by
+// the particle's weight factor and rounded once (w == 1 gives fx_weights)
+__device__ __forceinline__ FxW fx_weights_w(float ox, float oy, float w) {
+    FxW f = fx_weights(ox, oy);
+    if (w != 1.0f) {
+        const double d = (double)w;
+        f.a4 = (unsigned long long)llrint((double)f.a4 * d);
+        f.a3 = (unsigned long long)llrint((double)f.a3 * d);
+        f.a2 = (unsigned long long)llrint((double)f.a2 * d);
+        f.a1 = (unsigned long long)llrint((double)f.a1 * d);
+    }
+    return f;
+}
+
+// four corner atomics into the fixed-point charge
+__device__ __forceinline__ void dep_global(unsigned long long *q, int N, int jx, int jy,
+                                           const FxW &w) {
+    long b = (long)jy * N + jx;
+    atomicAdd(&q[b], w.a4);
+    atomicAdd(&q[b + 1], w.a3);
+    atomicAdd(&q[b + N], w.a2);
+    atomicAdd(&q[b + N + 1], w.a1);
+}
+
 // counting-sort key on the (N+1)^2 halo grid (jx, jy in [-1, N-1]): one bin
 // per cell, ordered tile-major with 8x8 tiles so a contiguous key range is a
 // compact patch.  Per-cell bins spread the histogram and slot-claim atomics

@@ -29,6 +29,14 @@ struct Species {
     int since_sort = 0;   // in-place steps since the last sort pass
     bool holes = false;   // the current buffer may hold empty slots
     const char *type = "";  // dataDump suffix (parameters.h:316-318)
+    // per-particle weights (buf[b].w, stage.w): needed once weights can
+    // differ between particles (source, reduction, a dump with mixed
+    // weights); then `weight` is 1 and w holds each particle's weight
+    bool weighted = false;
+    bool reduce = false;             // contain_particle_growth every step
+    pic::ReduceConst rc = {};
+    pic::ReduceState *rstate = nullptr;
+    int64_t injected = 0;            // particles created by the source
 };
 
 struct ProfPair {

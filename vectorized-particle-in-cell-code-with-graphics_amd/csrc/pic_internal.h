@@ -14,7 +14,10 @@ struct SpView {
     uint32_t *cell;
     float *ox, *oy, *vx, *vy, *vz;
     uint32_t *id;
+    float *w;  // per-particle weight factor; null: every particle has weight 1
 };
+
+__device__ __forceinline__ float load_w(const SpView &v, uint32_t i) { return v.w ? v.w[i] : 1.0f; }
 
 // Device counters of one species.  n[b] is the particle count of buffer b.
 // Counters that every block flushes (losses, holes, collision events) are
@@ -32,6 +35,7 @@ struct SpCounters {
     uint32_t lost[CNT_SLOTS][6];
     uint32_t holes[CNT_SLOTS];  // empty slots left by in-place steps (current buffer)
     unsigned long long events[CNT_SLOTS][8];  // collision events by mcc::Event
+    uint32_t reductions;     // population reductions performed
 };
 
 __device__ __forceinline__ uint32_t cnt_slot() { return blockIdx.x & (CNT_SLOTS - 1); }
@@ -43,6 +47,7 @@ __device__ __forceinline__ uint32_t cnt_slot() { return blockIdx.x & (CNT_SLOTS 
 struct StageView {
     uint32_t *cell;
     float *ox, *oy, *vx, *vy, *vz;
+    float *w;                 // weight factor (null when the species is unweighted)
     unsigned long long *key;  // creator key (source << 32 | creator id)
     uint32_t *idx;            // sort permutation
     uint32_t cap;             // entries (power of two)
@@ -123,6 +128,34 @@ void launch_steady(hipStream_t st, const SpView &e, const uint32_t *ne, int64_t 
                    unsigned long long *qi, int mode);
 void launch_load_random(hipStream_t st, const SpView &sp, uint32_t n, uint64_t seed,
                         uint32_t species, int N, double sigma);
+
+// ---- k_reduce.hip: population reduction (particle.cpp:520-653)
+#define RD_MAX_GROUPS 8
+struct ReduceConst {
+    int ng;                           // velocity groups
+    uint32_t size_check;              // reduce only above this many particles
+    uint32_t species;                 // RNG stream of the selection keys
+    uint32_t key0, key1;
+    double vlim[RD_MAX_GROUPS];       // group upper speed limits [m/s]
+    uint32_t glimit[RD_MAX_GROUPS];   // group size that triggers a reduction
+    uint32_t gnew[RD_MAX_GROUPS];     // group size after it
+};
+struct ReduceState {
+    uint32_t count[RD_MAX_GROUPS];    // Ngroup
+    uint32_t nrem[RD_MAX_GROUPS];     // particles to remove
+    uint32_t rank[RD_MAX_GROUPS];     // radix select: rank within the prefix
+    unsigned long long prefix[RD_MAX_GROUPS];
+    double gw[RD_MAX_GROUPS];         // survivors' weight factor
+    uint32_t active;
+    uint32_t hist[RD_MAX_GROUPS * 65536];
+};
+void launch_reduce(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                   const ReduceConst &rc, ReduceState *state, SpCounters *cnt, uint32_t step,
+                   unsigned long long *q, int N, int mode);
+void launch_inject(hipStream_t st, const StageView &sv, SpCounters *cnt, const float2 *E,
+                   const PushConst &pc, uint32_t key0, uint32_t key1, uint32_t step, uint32_t n_inj,
+                   float w, double dx, double L, double vt, uint32_t species, int push,
+                   uint32_t *lost6);
 
 // ---- k_refcx.hip: the reference's per-column ion charge exchange
 struct RefCxBuffers {

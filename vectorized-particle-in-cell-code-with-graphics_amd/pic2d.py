@@ -111,6 +111,12 @@ class Params(C.Structure):
         ("dist_loc", C.c_int32 * 4),
         ("probe", C.c_int32 * 6),
         ("write_graphs", C.c_int32),
+        ("reduce", C.c_int32),
+        ("no_groups", C.c_int32),
+        ("size_to_check", C.c_int32 * 2),
+        ("group_limit", (C.c_int32 * 8) * 2),
+        ("new_group_size", (C.c_int32 * 8) * 2),
+        ("v_limit", (C.c_double * 8) * 2),
     ]
 
     @property
@@ -199,6 +205,11 @@ def load_library(path=LIB_PATH):
         "pic_get_graph_max": [vp, vp],
         "pic_get_particle_subset": [vp, i32, i32, vp, i64, P(i64)],
         "pic_write_snapshot": [vp, C.c_char_p, i32],
+        "pic_get_weights": [vp, i32, vp, i64],
+        "pic_set_weights": [vp, i32, vp, i64],
+        "pic_inject_particles": [vp, i32],
+        "pic_contain_particle_growth": [vp, i32],
+        "pic_get_population_stats": [vp, i32, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -407,6 +418,30 @@ class Pic:
         a, b, c = C.c_int32(), C.c_int64(), C.c_double()
         self._c(self.lib.pic_get_mg_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def get_weights(self, s):
+        n = self.particle_size(s)
+        w = np.zeros(n, dtype=np.float32)
+        self._c(self.lib.pic_get_weights(self.h, s, _ptr(w), n))
+        return w
+
+    def set_weights(self, s, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._c(self.lib.pic_set_weights(self.h, s, _ptr(w), len(w)))
+
+    def inject_particles(self, s):
+        """particle::inject_more_particles"""
+        self._c(self.lib.pic_inject_particles(self.h, s))
+
+    def contain_particle_growth(self, s):
+        """particle::contain_particle_growth"""
+        self._c(self.lib.pic_contain_particle_growth(self.h, s))
+
+    def population_stats(self, s):
+        """[reductions performed, particles removed by them, particles injected]"""
+        a = np.zeros(3, dtype=np.int64)
+        self._c(self.lib.pic_get_population_stats(self.h, s, _ptr(a)))
+        return a
 
     def loss_counts(self, s):
         """[left, right, bottom, top, removed by collision]"""

@@ -63,6 +63,7 @@ def discharge(cells, ppc, length=0.01, te=2.0, ti=0.026, density=1e15, ion_mass=
     p.run_time = 1000
     p.graph_steps = 0        # batch diagnostics off unless asked for (with_diagnostics)
     p.time_data_freq = 0
+    p.reduce = 0             # fixed particle budget: no population reduction
     return p
 
 
