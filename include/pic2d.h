@@ -119,6 +119,16 @@ typedef struct pic_params {
     int32_t new_group_size[2][8];/* NEW_GROUP_SIZE, NEW_GROUP_SIZE2               */
     double v_limit[2][8];        /* VE1, VE2, UPPER_V_LIMIT / VI1, VI2,
                                     UPPER_V_LIMIT_I [m/s]                         */
+    /* surface physics of the non-steady walls (parameters.h:136-152, set up at
+       pic.cpp:455-505; boundary.cpp:261-529, 586-824) */
+    int32_t e_impact_see;        /* E_IMPACT_SEE: Vaughan electron-impact emission */
+    int32_t ion_impact_see;      /* ION_IMPACT_SEE: Ar+ impact emission (argon)    */
+    int32_t reflection;          /* REFLECTION                                    */
+    int32_t backscatter;         /* BACKSCATTER                                   */
+    double see_max_yield;        /* max_yield at normal incidence (1.4)           */
+    double see_E0;               /* emission threshold E0 [eV] (4.32)             */
+    double see_Emax0;            /* energy of the maximum yield [eV] (650)        */
+    double work_function;        /* WORK_FUNCTION [eV]                            */
 } pic_params;
 
 /* particleVec.h:26-35: the 48-byte AoS record of the reference */
@@ -224,7 +234,9 @@ int pic_get_E(pic_handle *h, double *ex, double *ey);   /* (nodes+1)^2 each, str
 int pic_get_q(pic_handle *h, int32_t s, double *out);   /* post-scaled charge (particle::get_q) */
 int pic_get_step(pic_handle *h, int64_t *step);
 /* particles removed: out5 = left, right, bottom, top wall, removed by a
- * collision (the reference's ionisation rule deletes the parent) */
+ * collision (the reference's ionisation rule deletes the parent).  With
+ * surface physics on, the wall columns count every particle that reached the
+ * wall, the reflected and backscattered ones included (pic_get_wall_counts) */
 int pic_get_loss_counts(pic_handle *h, int32_t s, int64_t *out5);
 /* collision events of species s: out8[k] counts events of kind k =
  * 1 elastic, 2 excitation, 3 ionisation, 4 charge exchange, 5 isotropic ion
@@ -252,6 +264,11 @@ int pic_set_weights(pic_handle *h, int32_t s, const float *w, int64_t n);
 /* out3 = population reductions performed (particle.cpp:648 count_reduce),
  * particles they removed, particles created by the source */
 int pic_get_population_stats(pic_handle *h, int32_t s, int64_t *out3);
+/* surface events of species s since creation (boundary.cpp:586-824, the
+ * reference's count_absorb/count_ref/count_scat/count_sec): out6 = reached a
+ * wall, absorbed, reflected, backscattered, emitting true secondaries,
+ * secondary electrons created */
+int pic_get_wall_counts(pic_handle *h, int32_t s, int64_t *out6);
 int pic_get_E32(pic_handle *h, float *out);  /* fp32 (Ex,Ey) pairs, (nodes+1)*nodes+1      */
 int pic_set_E32(pic_handle *h, const float *in);
 

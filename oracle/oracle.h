@@ -345,6 +345,32 @@ long orc_reduce(const orc_reduce_params *c, uint32_t step, long n, const float *
 void orc32_deposit_w(long n, const uint32_t *cell, const float *ox, const float *oy,
                      const float *w, int N, int mode, int64_t *qfix);
 
+/* ------------------------------------------------------------------------ */
+/* Surface physics of the non-steady walls (orc_wall.c).  Pinned by          */
+/* tests/test_oracle_wall.py (yields at their closed-form points, branch     */
+/* frequencies, emission energy and cosine-law angle distributions, the     */
+/* reference-mode emission positions).                                       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int mode, N;
+    double dx, Lw, dt, me, mi, q, max_yield, E0, Emax0, thr, wf;
+    int see_e, refl, back, see_i;
+    uint32_t key0, key1;
+} orc_wall_params;
+typedef struct {
+    uint32_t *cell;
+    float *ox, *oy, *vx, *vy, *vz, *w;
+    uint64_t *key;
+} orc_wall_out;
+double orc_vaughan(double E, double theta, double max_yield, double E0, double Emax0);
+double orc_ar_ion_yield(double E);
+/* species 0: electrons reaching a wall; 1: ions.  Returns the number of
+ * particles returned or emitted (written up to cap); counts[6] = reached,
+ * absorbed, reflected, backscattered, emitting, secondaries created. */
+long orc_wall(const orc_wall_params *c, int species, uint32_t step, long n, const uint32_t *cell,
+              const float *ox, const float *oy, const float *vx, const float *vy, const float *vz,
+              const float *w, const uint32_t *id, orc_wall_out *out, long cap, int64_t counts[6]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,19 @@ class ReduceParams(C.Structure):
                 ("glimit", C.c_uint32 * 8), ("gnew", C.c_uint32 * 8)]
 
 
+class WallParams(C.Structure):
+    _fields_ = [("mode", C.c_int), ("N", C.c_int)] + [
+        (k, C.c_double) for k in ("dx", "Lw", "dt", "me", "mi", "q", "max_yield", "E0", "Emax0",
+                                  "thr", "wf")] + [
+        (k, C.c_int) for k in ("see_e", "refl", "back", "see_i")] + [
+        ("key0", C.c_uint32), ("key1", C.c_uint32)]
+
+
+class WallOut(C.Structure):
+    _fields_ = [("cell", vp)] + [(k, vp) for k in ("ox", "oy", "vx", "vy", "vz", "w")] + [
+        ("key", vp)]
+
+
 _lib = None
 
 
@@ -132,6 +145,13 @@ def lib():
         _lib.orc_reduce.argtypes = [C.POINTER(ReduceParams), C.c_uint32, C.c_long] + [vp] * 7
         _lib.orc_reduce.restype = C.c_long
         _lib.orc32_deposit_w.argtypes = [C.c_long, vp, vp, vp, vp, C.c_int, C.c_int, vp]
+        _lib.orc_vaughan.argtypes = [C.c_double] * 5
+        _lib.orc_vaughan.restype = C.c_double
+        _lib.orc_ar_ion_yield.argtypes = [C.c_double]
+        _lib.orc_ar_ion_yield.restype = C.c_double
+        _lib.orc_wall.argtypes = [C.POINTER(WallParams), C.c_int, C.c_uint32, C.c_long] + \
+            [vp] * 8 + [C.POINTER(WallOut), C.c_long, vp]
+        _lib.orc_wall.restype = C.c_long
     return _lib
 
 
@@ -527,3 +547,42 @@ def deposit32_w(state, w, N, mode):
     lib().orc32_deposit_w(n, P(state["cell"]), P(state["ox"]), P(state["oy"]), P(w), N, mode,
                           P(q))
     return q.reshape(N, N)
+
+
+# ---------------------------------------------------------- surface physics
+CHARGE = 1.602177e-19
+
+
+def wall_params(p, seed=None):
+    """WallParams as the library derives them from a pic2d.Params"""
+    seed = p.seed if seed is None else seed
+    dx = p.length / (p.nodes - 1)
+    phys = p.mode == MODE_PHYS
+    return WallParams(mode=MODE_PHYS if phys else MODE_REF, N=p.nodes, dx=dx,
+                      Lw=p.length if phys else dx * p.nodes, dt=p.dt, me=p.electron_mass,
+                      mi=p.ion_mass, q=CHARGE, max_yield=p.see_max_yield, E0=p.see_E0,
+                      Emax0=p.see_Emax0, thr=p.threshold, wf=p.work_function,
+                      see_e=int(bool(p.e_impact_see)), refl=int(bool(p.reflection)),
+                      back=int(bool(p.backscatter)),
+                      see_i=int(bool(p.ion_impact_see) and p.gas == 0),
+                      key0=seed & 0xFFFFFFFF, key1=(seed >> 32) & 0xFFFFFFFF)
+
+
+def wall(c, species, step, state, w=None, cap=None):
+    """orc_wall on the particles of `state` (all assumed to have reached a
+    wall); returns (outputs dict with 'key', counts[6])"""
+    n = len(state["cell"])
+    cap = 8 * n + 16 if cap is None else cap
+    out = {k: np.zeros(cap, dtype=np.uint32 if k == "cell" else np.float32)
+           for k in ("cell", "ox", "oy", "vx", "vy", "vz", "w")}
+    out["key"] = np.zeros(cap, dtype=np.uint64)
+    wo = WallOut(**{k: out[k].ctypes.data for k in out})
+    counts = np.zeros(6, dtype=np.int64)
+    w = np.ones(n, dtype=np.float32) if w is None else np.ascontiguousarray(w, dtype=np.float32)
+    arr = [np.ascontiguousarray(state[k], dtype=np.uint32 if k in ("cell", "id") else np.float32)
+           for k in ("cell", "ox", "oy", "vx", "vy", "vz")]
+    ids = np.ascontiguousarray(state["id"], dtype=np.uint32)
+    m = lib().orc_wall(C.byref(c), species, step, n, *[P(a) for a in arr], P(w), P(ids),
+                       C.byref(wo), cap, P(counts))
+    assert m <= cap
+    return {k: v[:m] for k, v in out.items()}, counts

@@ -110,6 +110,9 @@ def test_loader_particle_count_control(tmp_path):
     assert pic2d.load_params(f).reduce == 0
     f.write_text('#define STEADYSTATE "YES"\n#define PIC_REDUCE 1\n')
     assert pic2d.load_params(f).reduce == 1
+    f.write_text('#define E_IMPACT_SEE "YES"\n#define WORK_FUNCTION (double)4.5\n')
+    p = pic2d.load_params(f)
+    assert p.e_impact_see == 1 and p.reflection == 0 and p.work_function == 4.5
     f.write_text('#define NO_GROUPS 4\n')
     with pytest.raises(pic2d.PicError) as e:
         pic2d.load_params(f)
@@ -125,10 +128,10 @@ def test_loader_errors(tmp_path):
     with pytest.raises(pic2d.PicError) as e:
         pic2d.load_params(f)
     assert e.value.code == -4
-    f.write_text('#define E_IMPACT_SEE "YES"\n')
+    f.write_text('#define X_NO_OF_CELLS 65\n#define Y_NO_OF_CELLS 33\n')
     with pytest.raises(pic2d.PicError) as e:
         pic2d.load_params(f)
-    assert e.value.code == -7
+    assert e.value.code == -7             # mglin needs a square grid
 
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference tree not mounted")
@@ -139,13 +142,13 @@ def test_reference_parameter_files_evaluate(rel):
     path = os.path.join(REF, rel)
     if not os.path.exists(path):
         pytest.skip("file absent")
-    try:
-        p = pic2d.load_params(path)
-    except pic2d.PicError as e:           # surface physics switched on: outside this build
-        assert e.code == -7
-        return
+    p = pic2d.load_params(path)
     d = pic2d.default_params()
     assert p.nodes == d.nodes and p.length == d.length and p.dt == d.dt
+    if "coll-boundary" in rel:            # parameters.h:141-152 switched on
+        assert p.e_impact_see and p.reflection and p.backscatter
+        assert p.ion_impact_see == ("Ar " in rel)   # the H file leaves it "NO"
+        assert p.wall == pic2d.WALL_ABSORB and p.work_function == 4.6
 
 
 def test_create_reports_grid_and_device_errors():

@@ -199,6 +199,9 @@ void species_free(Species &s) {
     for (int b = 0; b < 2; b++)
         if (s.buf[b].w) hipFree(s.buf[b].w);
     if (s.stage.w) hipFree(s.stage.w);
+    void *wq[] = {s.wq.cell, s.wq.ox, s.wq.oy, s.wq.vx, s.wq.vy, s.wq.vz, s.wq.w, s.wq.key};
+    for (void *p : wq)
+        if (p) hipFree(p);
     if (s.rstate) hipFree(s.rstate);
     if (s.q) hipFree(s.q);
     if (s.exp_) hipFree(s.exp_);
@@ -342,7 +345,9 @@ int alloc_stage(pic_handle *h, Species &s, int64_t n0, int64_t extra) {
     return PIC_OK;
 }
 
-Spawn make_spawn(pic_handle *h, bool plop) {
+// si >= 0: the pass moves species si; its particles that reach a wall are
+// queued for the surface pass when surface physics is on
+Spawn make_spawn(pic_handle *h, bool plop, int si) {
     Spawn sp;
     sp.se = h->sp[0].stage;
     sp.si = h->sp[1].stage;
@@ -352,7 +357,36 @@ Spawn make_spawn(pic_handle *h, bool plop) {
     sp.se_sigma = (float)h->P.vte;
     sp.si_sigma = (float)h->P.vti;
     sp.plop_ref = 0;
+    sp.wall = (h->wall_on && si >= 0) ? 1 : 0;
+    if (sp.wall) {
+        sp.wq = h->sp[si].wq;
+        sp.cw = &h->cnt[si];
+    } else {
+        sp.wq = StageView{};
+        sp.cw = nullptr;
+    }
     return sp;
+}
+
+int alloc_wall_queue(pic_handle *h, Species &s, int64_t n0) {
+    StageView &v = s.wq;
+    v.cap = (uint32_t)std::min<int64_t>(std::max<int64_t>(65536, n0 / 8), (int64_t)1 << 26);
+    CHECK(dalloc(h, &v.cell, v.cap));
+    CHECK(dalloc(h, &v.ox, v.cap));
+    CHECK(dalloc(h, &v.oy, v.cap));
+    CHECK(dalloc(h, &v.vx, v.cap));
+    CHECK(dalloc(h, &v.vy, v.cap));
+    CHECK(dalloc(h, &v.vz, v.cap));
+    CHECK(dalloc(h, &v.key, v.cap));
+    return PIC_OK;
+}
+
+// boundary::e_boundary / ion_boundary on the queued particles of species si
+// (their created and returning electrons go to the electron stage)
+void wall_pass(pic_handle *h, int si) {
+    if (!h->wall_on) return;
+    launch_wall(h->st, si, h->sp[si].wq, &h->cnt[si], h->sp[0].stage, &h->cnt[0], h->wc,
+                (uint32_t)h->step);
 }
 
 bool plop_on(pic_handle *h) {
@@ -403,6 +437,7 @@ int enable_weights(pic_handle *h, Species &s) {
         launch_fill<float>(h->st, s.buf[b].w, (long)s.cap, (float)s.weight);
     }
     CHECK(dalloc(h, &s.stage.w, (size_t)s.stage.cap));
+    if (s.wq.cap) CHECK(dalloc(h, &s.wq.w, (size_t)s.wq.cap));
     s.weight = 1.0;
     s.weighted = true;
     update_scale(h, s);
@@ -508,7 +543,7 @@ void push_species(pic_handle *h, int si, int do_push, int deposit) {
     uint32_t step = (uint32_t)h->step;
     mcc::MccConst mc = s.mc;
     if (!do_push) mc.on = 0;
-    Spawn spn = make_spawn(h, si == 1 && do_push && plop_on(h));
+    Spawn spn = make_spawn(h, si == 1 && do_push && plop_on(h), do_push ? si : -1);
     int p1 = prof_begin(h, 4);
     launch_push_count(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, mc, step,
                       h->binCount, &h->cnt[si].lost[0][0], do_push);
@@ -531,7 +566,7 @@ void push_species(pic_handle *h, int si, int do_push, int deposit) {
 void push_species_inplace(pic_handle *h, int si) {
     Species &s = h->sp[si];
     int c = s.cur;
-    Spawn spn = make_spawn(h, si == 1 && plop_on(h));
+    Spawn spn = make_spawn(h, si == 1 && plop_on(h), si);
     int p = prof_begin(h, 6);
     launch_push_inplace(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, s.mc,
                         (uint32_t)h->step, s.q, 1, spn, &h->cnt[si].events[0][0],
@@ -564,7 +599,7 @@ void absorb_species(pic_handle *h, int si, bool plop) {
     set_count(h, si, 1 - c, 0);
     launch_absorb(h->st, s.buf[c], s.buf[1 - c], &h->cnt[si].n[c], &h->cnt[si].n[1 - c], s.cap,
                   h->N, s.pc.mode, &h->cnt[si].lost[0][0], h->sp[1].mc, (uint32_t)h->step,
-                  make_spawn(h, plop));
+                  make_spawn(h, plop, si));
     s.cur = 1 - c;
     clear_holes(h, si);  // the compaction dropped them
 }
@@ -612,6 +647,7 @@ int fused_step(pic_handle *h) {
     inject(h, 0, 1);
     inject(h, 1, 0);
     advance_species(h, 0);
+    wall_pass(h, 0);  // boundary::e_boundary on the electrons that reached a wall
     if (h->spawning) {
         // ionisation products: new ions join this step's ion pass (they were
         // appended before the ion push in the reference, pic.cpp:1071-1111);
@@ -622,11 +658,13 @@ int fused_step(pic_handle *h) {
     if (h->step % h->P.dt_dti == 0) {
         if (h->refcx) CHECK(ref_ion_cx(h));
         advance_species(h, 1);
+        wall_pass(h, 1);  // boundary::ion_boundary (secondaries join the electrons)
     } else {
         Species &s = h->sp[1];
         launch_deposit(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], s.cap, N, s.pc.mode, s.q);
     }
-    if (h->spawning && plop_on(h)) {  // pairs re-injected for ions lost this step
+    if (h->spawning && (plop_on(h) || h->wall_on)) {
+        // pairs re-injected for ions lost this step; ion-impact secondaries
         commit(h, 1, 1, 1);
         commit(h, 0, 0, 1);
     }
@@ -749,9 +787,14 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
             h->sp[s].weight = mixed[s] ? 1.0 : w0;
         }
     }
+    // surface physics acts at the non-steady walls only (pic.cpp:516-528);
+    // Ar+ impact emission needs argon (pic.cpp:498)
+    h->wall_on = P.wall == PIC_WALL_ABSORB &&
+                 (P.e_impact_see || P.reflection || P.backscatter ||
+                  (P.ion_impact_see && P.gas == PIC_GAS_ARGON));
     h->spawning = (P.have_collisions && P.mode == PIC_MODE_PHYSICAL) ||
                   (P.wall == PIC_WALL_STEADY && P.mode == PIC_MODE_PHYSICAL) ||
-                  P.number_to_inject > 0;
+                  P.number_to_inject > 0 || h->wall_on;
     const int64_t n_inj = P.number_to_inject;
     for (int s = 0; s < 2; s++) {
         // created particles need headroom; otherwise counts only shrink
@@ -759,9 +802,11 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         if (cap > 0x7fffffffLL) return fail(set_global_error(PIC_ERR_CAPACITY, "too many particles"));
         rc = species_alloc(h, h->sp[s], cap > 0 ? cap : 1);
         if (rc != PIC_OK) return fail(rc);
-        if ((rc = alloc_stage(h, h->sp[s], n0[s], 2 * n_inj)) != PIC_OK) return fail(rc);
+        int64_t extra = 2 * n_inj + (h->wall_on ? n0[0] / 8 + n0[1] / 8 : 0);
+        if ((rc = alloc_stage(h, h->sp[s], n0[s], extra)) != PIC_OK) return fail(rc);
         update_scale(h, h->sp[s]);
         Species &sp = h->sp[s];
+        if (h->wall_on && (rc = alloc_wall_queue(h, sp, n0[s])) != PIC_OK) return fail(rc);
         sp.reduce = P.reduce != 0;
         if (sp.reduce || n_inj > 0 || mixed[s])
             if ((rc = enable_weights(h, sp)) != PIC_OK) return fail(rc);
@@ -791,6 +836,29 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     }
     setup_mcc(h, h->sp[0], true);
     setup_mcc(h, h->sp[1], false);
+    if (h->wall_on) {
+        WallConst &c = h->wc;
+        c.mode = P.mode == PIC_MODE_PHYSICAL ? PIC_DEV_MODE_PHYS : PIC_DEV_MODE_REF;
+        c.N = h->N;
+        c.dx = h->dx;
+        // boundary.cpp:373-374: lengthX = DX * X_NO_OF_CELLS, one cell beyond L
+        c.Lw = P.mode == PIC_MODE_PHYSICAL ? P.length : h->dx * h->N;
+        c.dt = P.dt;  // boundary(double dt = DT), boundary.h:62
+        c.me = P.electron_mass;
+        c.mi = P.ion_mass;
+        c.q = kCharge;
+        c.max_yield = P.see_max_yield;
+        c.E0 = P.see_E0;
+        c.Emax0 = P.see_Emax0;
+        c.thr = P.threshold;
+        c.wf = P.work_function;
+        c.see_e = P.e_impact_see != 0;
+        c.refl = P.reflection != 0;
+        c.back = P.backscatter != 0;
+        c.see_i = P.ion_impact_see != 0 && P.gas == PIC_GAS_ARGON;
+        c.key0 = (uint32_t)P.seed;
+        c.key1 = (uint32_t)(P.seed >> 32);
+    }
     // reference mode: ions collide through the per-column accumulator
     h->refcx = P.have_collisions && P.mode == PIC_MODE_REF &&
                !(P.sigma_cx == PIC_SIGMA_CONSTANT && P.sigma_value == 0.0);
@@ -920,9 +988,11 @@ int pic_run(pic_handle *h) {
         steady_boundary(h);
     } else {
         absorb_species(h, 1, plop_on(h));
+        wall_pass(h, 1);
         absorb_species(h, 0, false);
+        wall_pass(h, 0);
     }
-    if (h->spawning && plop_on(h)) {
+    if (h->spawning && (plop_on(h) || h->wall_on)) {
         commit(h, 1, 1, 0);
         commit(h, 0, 0, 0);
     }
@@ -1014,7 +1084,7 @@ int pic_e_collisions(pic_handle *h) {
     Species &s = h->sp[0];
     if (!s.mc.on) return PIC_OK;
     launch_mcc(h->st, s.buf[s.cur], &h->cnt[0].n[s.cur], s.cap, s.pc, s.mc, (uint32_t)h->step,
-               make_spawn(h, false), &h->cnt[0].events[0][0]);
+               make_spawn(h, false, -1), &h->cnt[0].events[0][0]);
     if (s.mc.mode == PIC_DEV_MODE_PHYS) {  // e_collisions.cpp:405-441 appends the products
         commit(h, 1, 1, 0);
         commit(h, 0, 0, 0);
@@ -1029,7 +1099,7 @@ int pic_ion_collisions(pic_handle *h) {
     Species &s = h->sp[1];
     if (!s.mc.on) return PIC_OK;
     launch_mcc(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], s.cap, s.pc, s.mc, (uint32_t)h->step,
-               make_spawn(h, false), &h->cnt[1].events[0][0]);
+               make_spawn(h, false, -1), &h->cnt[1].events[0][0]);
     return PIC_OK;
 }
 
@@ -1065,8 +1135,10 @@ int pic_boundary(pic_handle *h, int32_t which) {
         return PIC_OK;
     }
     bool plop = which == 0 && plop_on(h);
-    absorb_species(h, which == 0 ? 1 : 0, plop);
-    if (plop) {
+    const int si = which == 0 ? 1 : 0;
+    absorb_species(h, si, plop);
+    wall_pass(h, si);
+    if (plop || h->wall_on) {
         commit(h, 1, 1, 0);
         commit(h, 0, 0, 0);
     }
@@ -1273,6 +1345,21 @@ int pic_contain_particle_growth(pic_handle *h, int32_t s) {
     compact_species(h, s);
     reduce_species(h, s, false);  // collect_charge follows (pic.cpp:1139-1150)
     compact_species(h, s);
+    return PIC_OK;
+}
+
+// surface events of species s: reached a wall, absorbed, reflected,
+// backscattered, emitting true secondaries, secondary electrons created
+// (the reference's count_absorb / count_ref / count_scat / count_sec,
+// boundary.cpp:591, kept here across steps)
+int pic_get_wall_counts(pic_handle *h, int32_t s, int64_t *out6) {
+    CHECK(check_species(h, s));
+    if (!out6) return seterr(h, PIC_ERR_ARG, "null output");
+    CHECK(read_counters(h));
+    for (int k = 0; k < 6; k++) {
+        out6[k] = 0;
+        for (int b = 0; b < CNT_SLOTS; b++) out6[k] += h->cnt_host[s].wall[b][k];
+    }
     return PIC_OK;
 }
 

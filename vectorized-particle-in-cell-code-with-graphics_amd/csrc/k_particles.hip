@@ -583,7 +583,7 @@ __device__ __forceinline__ void stage_products(int ev, const mcc::Child &ch, con
                                                const float2 *__restrict__ E, const PushConst &pc,
                                                const mcc::MccConst &mc, uint32_t step,
                                                const Spawn &sp, uint32_t *__restrict__ lost6,
-                                               float w = 1.0f) {
+                                               float w, const PState &post) {
     if (EV_KIND(ev) == mcc::EV_IONIZE && mc.mode == PIC_DEV_MODE_PHYS) {
         // ejected electron: born at the parent's position, pushed this step
         // as the reference pushes particles appended before move_particles
@@ -616,6 +616,9 @@ __device__ __forceinline__ void stage_products(int ev, const mcc::Child &ch, con
         stage_put(sp.se, &sp.ce->stage_count, &sp.ce->overflow, pe, k, w);
         stage_put(sp.si, &sp.ci->stage_count, &sp.ci->overflow, pi, k, w);
     }
+    // a particle that reached a wall waits for the surface pass
+    if (sp.wall && valid && st >= ST_LEFT && st <= ST_TOP)
+        stage_put(sp.wq, &sp.cw->wall_count, &sp.cw->overflow, post, (unsigned long long)pid, w);
 }
 
 __global__ void __launch_bounds__(256)
@@ -656,7 +659,7 @@ __global__ void __launch_bounds__(256)
         if (dst.w) dst.w[slot] = w;
     }
     if (mc.on) count_events(ev, events);
-    stage_products(ev, ch, pre, st, valid, pid, E, pc, mc, step, sp, lost6, w);
+    stage_products(ev, ch, pre, st, valid, pid, E, pc, mc, step, sp, lost6, w, s);
 }
 
 
@@ -693,7 +696,7 @@ __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint
         if (EV_KIND(ev) > 0) atomicAdd(&se[EV_KIND(ev)], 1u);
         if (ev & EV_OVER_BOUND) atomicAdd(&se[6], 1u);
     }
-    stage_products(ev, ch, pre, st, i < n, pid, E, pc, mc, step, spn, lost6, w);
+    stage_products(ev, ch, pre, st, i < n, pid, E, pc, mc, step, spn, lost6, w, s);
 }
 
 // In-place timestep of one species between sort passes: the same collide +
@@ -719,8 +722,11 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x == 0) shole = 0;
     const uint32_t n = *nptr;
     const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
-    const bool need_vz = pc.bfield != 0 || mc.on;
-    const bool need_id = mc.on || spn.plop;
+    // vz and the id are read only where something uses them: the magnetic
+    // rotation, collisions, re-injection keyed by the id, and the wall
+    // queue (surface physics needs the full velocity and the id's draws)
+    const bool need_vz = pc.bfield != 0 || mc.on || spn.wall;
+    const bool need_id = mc.on || spn.plop || spn.wall;
     const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
     PState s[IPP];
     uint32_t pid[IPP];
@@ -900,6 +906,8 @@ __global__ void __launch_bounds__(256)
         stage_put(spn.se, &spn.ce->stage_count, &spn.ce->overflow, pe, k, wt);
         stage_put(spn.si, &spn.ci->stage_count, &spn.ci->overflow, pi, k, wt);
     }
+    if (spn.wall && st >= ST_LEFT && st <= ST_TOP)
+        stage_put(spn.wq, &spn.cw->wall_count, &spn.cw->overflow, s, (unsigned long long)pid, wt);
 }
 
 // Sort (key, val) pairs [0, c) ascending by key in one workgroup: bitonic
@@ -978,7 +986,10 @@ __global__ void __launch_bounds__(1024)
                 p.vx = sv.vx[k];
                 p.vy = sv.vy[k];
                 p.vz = sv.vz[k];
-                store_state(dst, slot, p, nid + r);
+                // reflected and backscattered particles (sources 4, 5) keep their id
+                const unsigned long long kk = sv.key[r];
+                const uint32_t src = (uint32_t)(kk >> 32);
+                store_state(dst, slot, p, (src == 4u || src == 5u) ? (uint32_t)kk : nid + r);
                 const float w = sv.w ? sv.w[k] : 1.0f;
                 if (dst.w) dst.w[slot] = w;
                 if (deposit && deposit_ok(p, N, mode))

@@ -449,6 +449,12 @@ extern "C" int pic_params_default(pic_params *P) {
     P->angle = 30.0 * PI / 180.0;
     pic_params_default_diagnostics(P, (int)period);
     pic_params_default_control(P);
+    // surface physics (parameters.h:91,141-152; pic.cpp:459-463)
+    P->e_impact_see = P->ion_impact_see = P->reflection = P->backscatter = 0;
+    P->see_max_yield = 1.4;
+    P->see_E0 = 4.32;
+    P->see_Emax0 = 650.0;
+    P->work_function = 4.6;
     return PIC_OK;
 }
 
@@ -584,14 +590,18 @@ extern "C" int pic_params_load(const char *path, pic_params *P) {
         ok = str("STEADYSTATE", s);
         P->wall = (s == "YES") ? PIC_WALL_STEADY : PIC_WALL_ABSORB;
     }
-    // surface physics beyond absorption is outside this build
-    const char *unsupported_flags[] = {"E_IMPACT_SEE", "ION_IMPACT_SEE", "REFLECTION", "BACKSCATTER"};
-    int surface_on = 0;
-    for (const char *f : unsupported_flags)
-        if (ok && has(f)) {
-            ok = str(f, s);
-            if (s == "YES") surface_on = 1;
-        }
+    // surface physics of the non-steady walls (parameters.h:136-152)
+    auto yes = [&](const char *n, int32_t &dst) {
+        if (!ok || !has(n)) return;
+        std::string t;
+        ok = str(n, t);
+        dst = t == "YES";
+    };
+    yes("E_IMPACT_SEE", P->e_impact_see);
+    yes("ION_IMPACT_SEE", P->ion_impact_see);
+    yes("REFLECTION", P->reflection);
+    yes("BACKSCATTER", P->backscatter);
+    ok = ok && num("WORK_FUNCTION", P->work_function);
     // build extensions (absent from the reference): run-time options of this library
     if (ok && has("PIC_MODE")) {
         ok = str("PIC_MODE", s);
@@ -668,9 +678,5 @@ extern "C" int pic_params_load(const char *path, pic_params *P) {
     if (ny != P->nodes || ylen != P->length)
         return pic_host::set_global_error(PIC_ERR_UNSUPPORTED,
                                           "X and Y grids must be identical (mglin needs a square grid)");
-    if (surface_on)
-        return pic_host::set_global_error(
-            PIC_ERR_UNSUPPORTED,
-            "secondary emission / reflection / backscatter walls are not implemented in this build");
     return PIC_OK;
 }

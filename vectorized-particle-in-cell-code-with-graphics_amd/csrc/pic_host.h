@@ -37,6 +37,7 @@ struct Species {
     pic::ReduceConst rc = {};
     pic::ReduceState *rstate = nullptr;
     int64_t injected = 0;            // particles created by the source
+    pic::StageView wq = {};          // particles that reached a wall (surface physics)
 };
 
 struct ProfPair {
@@ -81,6 +82,8 @@ struct pic_handle {
     bool spawning = false;       // collisions or re-injection create particles
     pic::MgLevels M;
     bool refcx = false;         // the reference's per-column ion charge exchange
+    bool wall_on = false;       // surface physics at the non-steady walls
+    pic::WallConst wc = {};
     pic::RefCxBuffers cx;
     // batch-mode diagnostics
     pic::DiagDev diag;

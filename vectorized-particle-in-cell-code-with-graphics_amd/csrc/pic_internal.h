@@ -36,6 +36,10 @@ struct SpCounters {
     uint32_t holes[CNT_SLOTS];  // empty slots left by in-place steps (current buffer)
     unsigned long long events[CNT_SLOTS][8];  // collision events by mcc::Event
     uint32_t reductions;     // population reductions performed
+    uint32_t wall_count;     // entries in the wall queue
+    // surface events: reached a wall, absorbed, reflected, backscattered,
+    // emitting true secondaries, secondary electrons created
+    uint32_t wall[CNT_SLOTS][6];
 };
 
 __device__ __forceinline__ uint32_t cnt_slot() { return blockIdx.x & (CNT_SLOTS - 1); }
@@ -88,6 +92,11 @@ struct Spawn {
     int plop;
     float se_sigma, si_sigma;
     int plop_ref;
+    // surface physics: particles reaching a wall are queued (state after the
+    // push, key = id) for the wall pass (k_wall.hip) instead of dropped
+    int wall;
+    StageView wq;
+    SpCounters *cw;
 };
 
 // ---- k_particles.hip
@@ -156,6 +165,22 @@ void launch_inject(hipStream_t st, const StageView &sv, SpCounters *cnt, const f
                    const PushConst &pc, uint32_t key0, uint32_t key1, uint32_t step, uint32_t n_inj,
                    float w, double dx, double L, double vt, uint32_t species, int push,
                    uint32_t *lost6);
+
+// ---- k_wall.hip: secondary emission, reflection, backscatter (boundary.cpp:281-529, 687-756)
+struct WallConst {
+    int mode;               // PIC_DEV_MODE_REF / PIC_DEV_MODE_PHYS
+    int N;                  // nodes per side
+    double dx;              // node spacing
+    double Lw;              // re-emission wall position: N dx (reference), L (physical)
+    double dt;              // boundary::dt (boundary.h:62: DT)
+    double me, mi, q;       // ELECTRON_MASS, ION_MASS, CHARGE
+    double max_yield, E0, Emax0;  // Vaughan parameters (pic.cpp:459-463)
+    double thr, wf;         // THRESHOLD, WORK_FUNCTION [eV]
+    int see_e, refl, back, see_i;  // E_IMPACT_SEE, REFLECTION, BACKSCATTER, ION_IMPACT_SEE
+    uint32_t key0, key1;
+};
+void launch_wall(hipStream_t st, int species, const StageView &wq, SpCounters *cw,
+                 const StageView &se, SpCounters *ce, const WallConst &wc, uint32_t step);
 
 // ---- k_refcx.hip: the reference's per-column ion charge exchange
 struct RefCxBuffers {

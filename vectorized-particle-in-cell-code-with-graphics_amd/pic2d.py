@@ -117,6 +117,14 @@ class Params(C.Structure):
         ("group_limit", (C.c_int32 * 8) * 2),
         ("new_group_size", (C.c_int32 * 8) * 2),
         ("v_limit", (C.c_double * 8) * 2),
+        ("e_impact_see", C.c_int32),
+        ("ion_impact_see", C.c_int32),
+        ("reflection", C.c_int32),
+        ("backscatter", C.c_int32),
+        ("see_max_yield", C.c_double),
+        ("see_E0", C.c_double),
+        ("see_Emax0", C.c_double),
+        ("work_function", C.c_double),
     ]
 
     @property
@@ -210,6 +218,7 @@ def load_library(path=LIB_PATH):
         "pic_inject_particles": [vp, i32],
         "pic_contain_particle_growth": [vp, i32],
         "pic_get_population_stats": [vp, i32, vp],
+        "pic_get_wall_counts": [vp, i32, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -441,6 +450,13 @@ class Pic:
         """[reductions performed, particles removed by them, particles injected]"""
         a = np.zeros(3, dtype=np.int64)
         self._c(self.lib.pic_get_population_stats(self.h, s, _ptr(a)))
+        return a
+
+    def wall_counts(self, s):
+        """[reached a wall, absorbed, reflected, backscattered, emitting,
+        secondary electrons created]"""
+        a = np.zeros(6, dtype=np.int64)
+        self._c(self.lib.pic_get_wall_counts(self.h, s, _ptr(a)))
         return a
 
     def loss_counts(self, s):
