@@ -13,6 +13,11 @@ Prints ONE JSON line on stdout:
                  workload, timed on this host
   latency_ms     p50 / p99 of per-step durations (HIP events around each step)
 
+Workloads "ensemble" (64 independent runs, the C4 mix) and "serving" (Poisson
+arrivals at five offered loads, continuous batching under a 200 GB budget)
+report the batched path against the plain one-run-at-a-time path and the
+per-step latency percentiles instead of a kernel roofline.
+
 Progress goes to stderr.  `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
 import argparse
@@ -125,6 +130,109 @@ def calibration_copy(nbytes):
     log(f"calibration copy: 3 x {nbytes >> 20} MiB read + {nbytes >> 20} MiB written")
 
 
+def live_count(sims):
+    return sum(s.particle_count(0) + s.particle_count(1) for s in sims)
+
+
+def run_ensemble(args):
+    """C4: the same 64 runs advanced one at a time (plain) and all at once
+    (batched: one host thread and HIP stream per run)."""
+    runs = workloads.ensemble(args.runs)
+    log(f"ensemble: {len(runs)} runs, creating")
+    sims = [pic2d.Pic(p) for p in runs]
+    for s in sims:
+        s.run(0)  # pre-loop wall pass
+    log("warmup", args.warmup, "steps (batched)")
+    pic2d.batch_advance(sims, args.warmup)
+    n0 = live_count(sims)
+    log("plain:", args.steps, "steps per run, one run at a time")
+    t0 = time.perf_counter()
+    for s in sims:
+        s.advance(args.steps)
+        s.sync()
+    t_plain = time.perf_counter() - t0
+    n1 = live_count(sims)
+    log("batched:", args.steps, "steps per run, all runs at once")
+    t0 = time.perf_counter()
+    pic2d.batch_advance(sims, args.steps)
+    t_batch = time.perf_counter() - t0
+    n2 = live_count(sims)
+    plain = 0.5 * (n0 + n1) * args.steps / t_plain
+    batched = 0.5 * (n1 + n2) * args.steps / t_batch
+    cells = sorted({int(p.nodes - 1) for p in runs})
+    for s in sims:
+        s.close()
+    return {
+        "metric": "particle-steps/s",
+        "value": batched,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_batch / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Philox-seeded uniform positions, Maxwellian velocities)",
+        "config": {"workload": "ensemble", "runs": len(runs), "cells_per_side": cells,
+                   "particles": int(n1), "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")},
+        "plain": {"value": plain, "ms_per_step": 1e3 * t_plain / args.steps},
+        "batched_vs_plain": batched / plain,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
+def run_serving(args):
+    """C5: capacity under saturation, then the same requests arriving as a
+    Poisson process at 0.25-1.25x that capacity (unit-rate arrival times
+    scaled by 1/rate, so every load replays identical work)."""
+    budget = int(200e9)
+    base = workloads.serving_requests(1.0, 1e9, seed=11, step_scale=args.step_scale,
+                                      max_requests=args.requests)
+    unit = [t for _, _, t in base]  # arrival times of a unit-rate Poisson process
+    sat = [(p, n, 0.0) for p, n, _ in base]
+    log(f"serving: capacity run, {len(sat)} requests at t = 0")
+    _, st = pic2d.serve(sat, budget, args.max_active)
+    cap = st["throughput"]
+    per_req = st["particle_steps"] / len(sat)
+    loads = []
+    for k, load in enumerate((0.25, 0.5, 0.75, 1.0, 1.25)):
+        rate = load * cap / per_req
+        reqs = [(p, n, t / rate) for (p, n, _), t in zip(base, unit)]
+        log(f"serving: load {load}, {len(reqs)} requests at {rate:.2f}/s")
+        res, st = pic2d.serve(reqs, budget, args.max_active)
+        wait = [r["admit_s"] - q[2] for r, q in zip(res, reqs)]
+        loads.append({"offered_load": load, "requests": len(reqs), "rate_per_s": rate,
+                      "throughput": st["throughput"], "step_p50_ms": st["step_p50_ms"],
+                      "step_p99_ms": st["step_p99_ms"], "queue_wait_mean_s": float(np.mean(wait)),
+                      "max_concurrent": st["max_concurrent"], "peak_bytes": st["peak_bytes"]})
+    full = [l for l in loads if l["offered_load"] == 1.0][0]
+    return {
+        "metric": "particle-steps/s",
+        "value": full["throughput"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": 0,
+        "ms_per_step": full["step_p50_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Philox-seeded uniform positions, Maxwellian velocities)",
+        "config": {"workload": "serving", "hbm_budget_bytes": budget,
+                   "requests_per_load": args.requests, "step_scale": args.step_scale,
+                   "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")},
+        "capacity": cap,
+        "latency_ms": {"p50": full["step_p50_ms"], "p99": full["step_p99_ms"]},
+        "loads": loads,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,12 +249,28 @@ def main():
                     help="override the workload's multigrid mode")
     ap.add_argument("--sort-interval", type=int, default=0,
                     help="particle passes per counting sort (0: the library default)")
+    ap.add_argument("--runs", type=int, default=64, help="ensemble: number of runs")
+    ap.add_argument("--requests", type=int, default=24, help="serving: requests per load")
+    ap.add_argument("--step-scale", type=float, default=0.2,
+                    help="serving: request lengths are U[500, 5000] steps times this")
+    ap.add_argument("--max-active", type=int, default=64, help="serving: most runs at once")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="ensemble/serving: hardware queues for the runs' streams (1-32; "
+                         "0: the runtime's default, 4, measured best for the ensemble)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(runs are independent; see DESIGN.md for how the timestep would shard)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload in ("ensemble", "serving"):
+        # more hardware queues let more runs' streams overlap; set before the
+        # HIP runtime starts (the library's first call)
+        if args.hw_queues > 0:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, args.hw_queues))
+        out = run_ensemble(args) if args.workload == "ensemble" else run_serving(args)
+        print(json.dumps(out), flush=True)
+        return
 
     p = make_params(args.workload)
     if args.mg != "default":

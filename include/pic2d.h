@@ -269,6 +269,53 @@ int pic_get_population_stats(pic_handle *h, int32_t s, int64_t *out3);
  * wall, absorbed, reflected, backscattered, emitting true secondaries,
  * secondary electrons created */
 int pic_get_wall_counts(pic_handle *h, int32_t s, int64_t *out6);
+/* live particles of species s without compacting the store (no reference
+ * counterpart; pic_particle_size compacts first) */
+int pic_particle_count(pic_handle *h, int32_t s, int64_t *n);
+/* device memory the handle allocated (bytes) */
+int pic_get_device_bytes(pic_handle *h, int64_t *bytes);
+/* upper estimate of the device memory pic_create(p, restart=0) allocates */
+int pic_estimate_bytes(const pic_params *p, int64_t *bytes);
+
+/* ------------------------------------------- ensembles and serving */
+/* No reference counterpart (the reference runs one configuration per
+ * process).  Runs are independent handles, each on its own HIP stream; they
+ * share the GPU by being stepped from their own host threads, so one run's
+ * kernels fill the device while another waits on its stream.  A run's results
+ * are bit-identical alone and in a batch. */
+
+/* pic_advance(runs[k], steps) for every k at once, then each synchronised */
+int pic_batch_advance(pic_handle *const *runs, int32_t n, int32_t steps);
+
+typedef struct pic_serve_request {
+    pic_params params;   /* the run (created without a restart file)         */
+    int32_t steps;       /* fused steps after the pre-loop wall pass         */
+    double arrival_s;    /* arrival time, seconds after pic_serve starts     */
+} pic_serve_request;
+
+typedef struct pic_serve_result {
+    double admit_s, finish_s;   /* admission and completion times [s]         */
+    int64_t particle_steps;     /* sum over its steps of the live particles   */
+    double step_p50_ms, step_p99_ms;  /* per-step latency (issue to completion) */
+    int32_t status;             /* PIC_OK or the run's error                   */
+} pic_serve_result;
+
+typedef struct pic_serve_stats {
+    double wall_s;              /* first arrival window to last completion    */
+    int64_t particle_steps;
+    double throughput;          /* particle-steps/s over wall_s               */
+    double step_p50_ms, step_p99_ms;  /* over every step of every request      */
+    int64_t peak_bytes;         /* largest admitted memory estimate in use    */
+    int32_t max_concurrent;     /* most runs active at once                   */
+} pic_serve_stats;
+
+/* Continuous batching: requests are admitted in arrival order as soon as they
+ * have arrived, the estimated memory of the active runs plus theirs fits
+ * hbm_budget (pic_estimate_bytes) and fewer than max_active runs are active;
+ * each runs to completion on its own thread.  Returns the first run error
+ * (every result carries its own status). */
+int pic_serve(const pic_serve_request *reqs, int32_t n, int64_t hbm_budget, int32_t max_active,
+              pic_serve_result *res, pic_serve_stats *stats);
 int pic_get_E32(pic_handle *h, float *out);  /* fp32 (Ex,Ey) pairs, (nodes+1)*nodes+1      */
 int pic_set_E32(pic_handle *h, const float *in);
 

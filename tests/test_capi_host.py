@@ -160,3 +160,20 @@ def test_create_reports_grid_and_device_errors():
     with pytest.raises(pic2d.PicError) as e:
         pic2d.mglin(np.zeros((6, 6)))
     assert e.value.code == -2
+
+
+def test_estimate_and_serve_arguments():
+    import workloads
+    small, big = workloads.plain(), workloads.argon_256()
+    es, eb = pic2d.estimate_bytes(small), pic2d.estimate_bytes(big)
+    assert 0 < es < eb
+    # 13.1 M particles per species, double-buffered 28 B each plus staging: > 1.4 GB
+    assert eb > 2 * 2 * 13107200 * 28
+    res, st = pic2d.serve([], int(1e9))
+    assert res == [] and st["particle_steps"] == 0
+    with pytest.raises(pic2d.PicError) as e:
+        pic2d.serve([(small, 1, 0.0)], 0)
+    assert e.value.code == -1
+    with pytest.raises(pic2d.PicError) as e:       # can never be admitted: refused up front
+        pic2d.serve([(big, 1, 0.0)], eb // 2)
+    assert e.value.code == -6

@@ -60,6 +60,7 @@ int dalloc(pic_handle *h, T **p, size_t n) {
     if (e != hipSuccess)
         return seterr(h, PIC_ERR_ALLOC, std::string("hipMalloc ") + std::to_string(n * sizeof(T)) +
                                             " bytes: " + hipGetErrorString(e));
+    h->dev_bytes += (int64_t)(n * sizeof(T));
     return PIC_OK;
 }
 
@@ -1352,6 +1353,64 @@ int pic_contain_particle_growth(pic_handle *h, int32_t s) {
 // backscattered, emitting true secondaries, secondary electrons created
 // (the reference's count_absorb / count_ref / count_scat / count_sec,
 // boundary.cpp:591, kept here across steps)
+// live particles of species s without compacting the store (the count minus
+// the holes of in-place passes): a cheap read for drivers between steps
+int pic_particle_count(pic_handle *h, int32_t s, int64_t *n) {
+    CHECK(check_species(h, s));
+    if (!n) return seterr(h, PIC_ERR_ARG, "null output");
+    CHECK(read_counters(h));
+    int64_t holes = 0;
+    for (int b = 0; b < CNT_SLOTS; b++) holes += h->cnt_host[s].holes[b];
+    *n = (int64_t)h->cnt_host[s].n[h->sp[s].cur] - (h->sp[s].holes ? holes : 0);
+    return PIC_OK;
+}
+
+int pic_get_device_bytes(pic_handle *h, int64_t *bytes) {
+    if (!h || !bytes) return set_global_error(PIC_ERR_ARG, "null argument");
+    *bytes = h->dev_bytes;
+    return PIC_OK;
+}
+
+// Upper estimate of the device memory pic_create allocates for p (the same
+// capacities as pic_create, without a restart file): used to admit runs
+// against a memory budget before creating them.
+int pic_estimate_bytes(const pic_params *pp, int64_t *bytes) {
+    if (!pp || !bytes) return set_global_error(PIC_ERR_ARG, "null argument");
+    const pic_params &P = *pp;
+    if (!grid_ok(P.nodes)) return set_global_error(PIC_ERR_GRID, "nodes-1 must be a power of 2");
+    const int64_t N = P.nodes, n0 = std::max<int64_t>(P.init_particle_no, 0);
+    const int64_t n_inj = std::max<int64_t>(P.number_to_inject, 0);
+    const bool wall_on = P.wall == PIC_WALL_ABSORB &&
+                         (P.e_impact_see || P.reflection || P.backscatter || P.ion_impact_see);
+    const bool spawning = (P.have_collisions && P.mode == PIC_MODE_PHYSICAL) ||
+                          (P.wall == PIC_WALL_STEADY && P.mode == PIC_MODE_PHYSICAL) ||
+                          n_inj > 0 || wall_on;
+    const bool weighted = P.reduce || n_inj > 0;
+    int64_t b = 0;
+    for (int s = 0; s < 2; s++) {
+        int64_t cap = n0 + (spawning ? std::max<int64_t>(n0 / 2, 65536) : 0) + 4096 * n_inj;
+        cap = std::max<int64_t>(cap, 1);
+        b += 2 * cap * (28 + (weighted ? 4 : 0)) + N * N * 8;
+        int64_t want = std::max<int64_t>(16384, std::min<int64_t>(n0 / 16, (int64_t)1 << 22)) +
+                       2 * n_inj + (wall_on ? n0 / 4 : 0);
+        int64_t sc = 1;
+        while (sc < want) sc <<= 1;
+        b += sc * (36 + (weighted ? 4 : 0));
+        if (wall_on) b += std::min<int64_t>(std::max<int64_t>(65536, n0 / 8), (int64_t)1 << 26) * 36;
+        if (P.reduce) b += (int64_t)sizeof(ReduceState);
+        if (s == 1 && P.have_collisions && P.mode == PIC_MODE_REF) b += cap * 40 + N * 4;
+    }
+    const int64_t tiles = (N + 1 + 7) / 8;
+    b += 2 * (tiles * tiles * 64 + 4096) * 4;           // sort histogram and starts
+    b += N * N * 16 + (N + 1) * (N + 1) * 16 + ((N + 1) * N + 1) * 8;  // rho, phi, E
+    b += (int64_t)(60.0 * 4.0 / 3.0 * (double)(N * N)) + (1 << 20);   // multigrid levels
+    if (P.graph_steps > 0 || P.time_data_freq > 0)
+        b += N * N * 8 * 6 + 4 * (int64_t)std::max(P.no_bins, 1) * 8 * 2 + (1 << 20);
+    b += 3 * 10000 * 4 + 2 * (int64_t)sizeof(SpCounters);
+    *bytes = b + b / 20 + (8 << 20);
+    return PIC_OK;
+}
+
 int pic_get_wall_counts(pic_handle *h, int32_t s, int64_t *out6) {
     CHECK(check_species(h, s));
     if (!out6) return seterr(h, PIC_ERR_ARG, "null output");

@@ -96,6 +96,7 @@ struct pic_handle {
     double mg_relres = 0.0;     // relative residual after the last solve
     bool have_rho = false, have_phi = false;
     int64_t step = 0;
+    int64_t dev_bytes = 0;      // device memory allocated through dalloc
     int32_t run_time = 0;
     std::string err;
     // profiling

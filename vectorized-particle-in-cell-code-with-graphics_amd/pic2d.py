@@ -135,6 +135,26 @@ class Params(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServeRequest(C.Structure):
+    """Mirror of ``pic_serve_request``."""
+    _fields_ = [("params", Params), ("steps", C.c_int32), ("arrival_s", C.c_double)]
+
+
+class ServeResult(C.Structure):
+    """Mirror of ``pic_serve_result``."""
+    _fields_ = [("admit_s", C.c_double), ("finish_s", C.c_double),
+                ("particle_steps", C.c_int64), ("step_p50_ms", C.c_double),
+                ("step_p99_ms", C.c_double), ("status", C.c_int32)]
+
+
+class ServeStats(C.Structure):
+    """Mirror of ``pic_serve_stats``."""
+    _fields_ = [("wall_s", C.c_double), ("particle_steps", C.c_int64),
+                ("throughput", C.c_double), ("step_p50_ms", C.c_double),
+                ("step_p99_ms", C.c_double), ("peak_bytes", C.c_int64),
+                ("max_concurrent", C.c_int32)]
+
+
 PARTICLE_DTYPE = np.dtype(
     [("x", "f8"), ("y", "f8"), ("vx", "f8"), ("vy", "f8"), ("vz", "f8"), ("weight", "f8")]
 )
@@ -219,6 +239,11 @@ def load_library(path=LIB_PATH):
         "pic_contain_particle_growth": [vp, i32],
         "pic_get_population_stats": [vp, i32, vp],
         "pic_get_wall_counts": [vp, i32, vp],
+        "pic_particle_count": [vp, i32, P(i64)],
+        "pic_get_device_bytes": [vp, P(i64)],
+        "pic_estimate_bytes": [P(Params), P(i64)],
+        "pic_batch_advance": [P(vp), i32, i32],
+        "pic_serve": [P(ServeRequest), i32, i64, i32, P(ServeResult), P(ServeStats)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -270,6 +295,37 @@ def poisson_tol(f, u0=None, length=1.0, tol=1e-10, max_iter=60):
     _check(load_library().pic_poisson_tol(_ptr(f), _ptr(u), f.shape[0], length, tol, max_iter,
                                           C.byref(cyc), C.byref(rr)))
     return u, cyc.value, rr.value
+
+
+def estimate_bytes(params: Params) -> int:
+    """upper estimate of the device memory a run of these parameters takes"""
+    b = C.c_int64()
+    _check(load_library().pic_estimate_bytes(C.byref(params), C.byref(b)))
+    return b.value
+
+
+def batch_advance(sims, steps):
+    """advance every run by `steps` steps at once (one host thread per run)"""
+    arr = (C.c_void_p * len(sims))(*[s.h.value for s in sims])
+    _check(load_library().pic_batch_advance(arr, len(sims), int(steps)))
+
+
+def serve(requests, hbm_budget, max_active=64):
+    """continuous-batching serving loop; requests = [(Params, steps, arrival_s)].
+    Returns (list of result dicts, stats dict)."""
+    n = len(requests)
+    reqs = (ServeRequest * n)()
+    for k, (p, steps, t) in enumerate(requests):
+        reqs[k].params = p
+        reqs[k].steps = int(steps)
+        reqs[k].arrival_s = float(t)
+    res = (ServeResult * n)()
+    st = ServeStats()
+    rc = load_library().pic_serve(reqs, n, int(hbm_budget), int(max_active), res, C.byref(st))
+    out = [{k: getattr(r, k) for k, _ in ServeResult._fields_} for r in res]
+    stats = {k: getattr(st, k) for k, _ in ServeStats._fields_}
+    _check(rc)
+    return out, stats
 
 
 class Pic:
@@ -451,6 +507,17 @@ class Pic:
         a = np.zeros(3, dtype=np.int64)
         self._c(self.lib.pic_get_population_stats(self.h, s, _ptr(a)))
         return a
+
+    def particle_count(self, s):
+        """live particles without compacting the store"""
+        n = C.c_int64()
+        self._c(self.lib.pic_particle_count(self.h, s, C.byref(n)))
+        return n.value
+
+    def device_bytes(self):
+        n = C.c_int64()
+        self._c(self.lib.pic_get_device_bytes(self.h, C.byref(n)))
+        return n.value
 
     def wall_counts(self, s):
         """[reached a wall, absorbed, reflected, backscattered, emitting,

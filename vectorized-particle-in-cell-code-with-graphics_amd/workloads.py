@@ -104,6 +104,64 @@ def with_argon(p, pressure_pa=13.332, temperature=300.0):
     return p
 
 
+def with_gas(p, gas, pressure_pa, temperature=300.0):
+    """Argon or hydrogen background, physical-mode collisions and steady-state
+    pair re-injection (hydrogen: H+ ions of 1.00728 amu, 13.6 eV ionisation)."""
+    if gas == pic2d.GAS_HYDROGEN:
+        p.ion_mass = H_MASS
+        p.threshold = 13.6
+        p.vti = math.sqrt(p.vti ** 2 * AR_MASS / H_MASS)   # same ion temperature
+        p.sigma_load_i = p.vti
+    with_argon(p, pressure_pa, temperature)
+    p.gas = gas
+    return p
+
+
+def ensemble(nruns=64, seed0=1000):
+    """The C4 ensemble: run r seeded seed0 + r; cells in {32^2, 64^2, 128^2,
+    256^2}; ppc ~ U[20, 200]; pressure ~ logU[10, 500] mTorr; Te ~ U[1, 5] eV;
+    e + Ar+ or e + H+ with probability 1/2 each."""
+    import numpy as np
+    runs = []
+    for r in range(nruns):
+        g = np.random.default_rng(seed0 + r)
+        cells = int(g.choice([32, 64, 128, 256]))
+        ppc = int(g.integers(20, 201))
+        mtorr = float(np.exp(g.uniform(np.log(10.0), np.log(500.0))))
+        te = float(g.uniform(1.0, 5.0))
+        gas = pic2d.GAS_ARGON if g.random() < 0.5 else pic2d.GAS_HYDROGEN
+        p = discharge(cells, ppc, te=te, seed=seed0 + r)
+        with_gas(p, gas, mtorr * 0.13332)
+        runs.append(p)
+    return runs
+
+
+def serving_requests(rate, window_s, seed=7, step_scale=0.1, max_requests=None):
+    """C5 request stream: Poisson arrivals of `rate` requests/s over window_s;
+    cells in {64^2, 128^2, 256^2, 512^2}; 0.1-20 M particles per species
+    (log-uniform, 10-400 ppc); argon or hydrogen as in `ensemble`;
+    steps ~ U[500, 5000] * step_scale.  Returns [(Params, steps, arrival_s)]."""
+    import numpy as np
+    g = np.random.default_rng(seed)
+    out = []
+    t = float(g.exponential(1.0 / rate))
+    k = 0
+    while t < window_s and (max_requests is None or k < max_requests):
+        cells = int(g.choice([64, 128, 256, 512]))
+        n = float(np.exp(g.uniform(np.log(1e5), np.log(2e7))))
+        ppc = int(min(400, max(10, round(n / (cells * cells)))))
+        mtorr = float(np.exp(g.uniform(np.log(10.0), np.log(500.0))))
+        te = float(g.uniform(1.0, 5.0))
+        gas = pic2d.GAS_ARGON if g.random() < 0.5 else pic2d.GAS_HYDROGEN
+        steps = max(1, int(round(g.uniform(500, 5000) * step_scale)))
+        p = discharge(cells, ppc, te=te, seed=2000 + k)
+        with_gas(p, gas, mtorr * 0.13332)
+        out.append((p, steps, t))
+        t += float(g.exponential(1.0 / rate))
+        k += 1
+    return out
+
+
 def argon_256(**kw):
     """256x256 cells, 200 ppc (13.1 M particles per species), argon 100 mTorr."""
     return with_argon(discharge(256, 200, **kw))
