@@ -249,6 +249,9 @@ def main():
                     help="override the workload's multigrid mode")
     ap.add_argument("--sort-interval", type=int, default=0,
                     help="particle passes per counting sort (0: the library default)")
+    ap.add_argument("--sort-key", type=int, default=-1,
+                    help="counting-sort granularity: cell key >> k (0 cells, 6 8x8 tiles; "
+                         "-1: the library default)")
     ap.add_argument("--runs", type=int, default=64, help="ensemble: number of runs")
     ap.add_argument("--requests", type=int, default=24, help="serving: requests per load")
     ap.add_argument("--step-scale", type=float, default=0.2,
@@ -277,6 +280,8 @@ def main():
         p.mg_mode = pic2d.MG_FIXED if args.mg == "fixed" else pic2d.MG_TOL
     if args.sort_interval > 0:
         p.sort_interval = args.sort_interval
+    if args.sort_key >= 0:
+        p.sort_key = args.sort_key
     log("workload", args.workload, "nodes", p.nodes, "particles/species", p.init_particle_no)
     sim = pic2d.Pic(p)
     e0, i0 = (sim.get_state(0), sim.get_state(1)) if not args.no_cpu_baseline else (None, None)

@@ -146,10 +146,35 @@ def test_loader_statistics_and_ids():
             # positions uniform: counts per cell within 6 sigma of the mean
             h = np.bincount(jy * (p.nodes - 1) + jx, minlength=(p.nodes - 1) ** 2)
             assert np.all(np.abs(h - h.mean()) < 6 * np.sqrt(h.mean()))
-            # sorted by the counting-sort key after creation
+            # sorted by the counting-sort key after creation (8x8-cell tiles,
+            # the library's default granularity)
             key = (((jy + 1) >> 3) * ((p.nodes + 8) // 8) + ((jx + 1) >> 3)) * 64 + \
                 ((jy + 1) & 7) * 8 + ((jx + 1) & 7)
-            assert np.all(np.diff(key) >= 0)
+            assert np.all(np.diff(key >> 6) >= 0)
+
+
+@pytest.mark.parametrize("sort_key", [0, 3, 6])
+def test_sort_key_granularity(sort_key):
+    """the sort orders the store by cell key >> sort_key; results do not depend on it"""
+    p = workloads.discharge(64, 20, mode=pic2d.MODE_PHYSICAL, mg_mode=pic2d.MG_FIXED, te=6.0)
+    p.sort_key = sort_key
+    p.sort_interval = 1
+    with pic2d.Pic(p) as sim:
+        sim.run(3)
+        st = sim.get_state(0)
+        jx, jy = pic2d.cell_unpack(st["cell"])
+        key = (((jy + 1) >> 3) * ((p.nodes + 8) // 8) + ((jx + 1) >> 3)) * 64 + \
+            ((jy + 1) & 7) * 8 + ((jx + 1) & 7)
+        assert np.all(np.diff(key >> sort_key) >= 0)
+        got = by_id(st)
+        q = sim.get_qfixed(0)
+    p.sort_key = 0
+    with pic2d.Pic(p) as ref:
+        ref.run(3)
+        want = by_id(ref.get_state(0))
+        assert np.array_equal(q, ref.get_qfixed(0))
+    for k in ("id",) + KEYS:
+        assert np.array_equal(got[k], want[k]), k
 
 
 def test_dump_and_restart_roundtrip(tmp_path):

@@ -230,6 +230,7 @@ void setup_species(pic_handle *h, Species &s, bool electron) {
                   : (P.wall == PIC_WALL_STEADY ? PIC_DEV_MODE_REF_STEADY : PIC_DEV_MODE_REF);
     pc.bfield = P.magnetic_field > 0.0;
     pc.tilesX = h->tilesX;
+    pc.keyshift = h->keyshift;
     double coef = 0.5 * s.dt * s.q_over_m;  // particle.cpp:1103
     double omega_dt = s.dt * s.q_over_m * P.magnetic_field;
     // sort interval: explicit, or chosen from the thermal displacement per
@@ -728,6 +729,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     if (!grid_ok(P.nodes))
         return set_global_error(PIC_ERR_GRID, "nodes-1 must be a power of 2 (and nodes >= 5)");
     if (P.number_to_inject < 0) return set_global_error(PIC_ERR_ARG, "NUMBER_TO_INJECT < 0");
+    if (P.sort_key < -1 || P.sort_key > 6) return set_global_error(PIC_ERR_ARG, "sort_key must be -1..6");
     if (P.reduce) {
         if (P.no_groups < 1 || P.no_groups > RD_MAX_GROUPS)
             return set_global_error(PIC_ERR_ARG, "NO_GROUPS must be 1..8");
@@ -758,7 +760,11 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     h->dx = P.length / (double)(P.nodes - 1);  // parameters.h:193
     h->run_time = P.run_time;
     h->tilesX = (h->N + 1 + 7) / 8;
-    h->nbins = (uint32_t)h->tilesX * h->tilesX * 64u;
+    // 8x8-cell tile keys by default: a block's particles then share one to a
+    // few keys, so the claims are few and the scattered writes contiguous
+    // (measured 1.6x faster than per-cell keys at 1025^2, 100 ppc)
+    h->keyshift = P.sort_key >= 0 ? P.sort_key : 6;
+    h->nbins = ((uint32_t)h->tilesX * h->tilesX * 64u) >> h->keyshift;
     h->nbins_pad = scan_pad(h->nbins);
     int rc = PIC_OK;
     auto fail = [&](int code) {

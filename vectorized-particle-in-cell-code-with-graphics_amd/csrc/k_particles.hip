@@ -2,15 +2,18 @@
 //
 // Fused timestep of one species (two passes over the particle store):
 //   pass 1  k_push_count:   gather + push + wall test, histogram of the new
-//                           cells (counting-sort key), per-wall loss counts;
+//                           cells (counting-sort key, aggregated per block in
+//                           LDS), per-wall loss counts;
 //   scan    k_scan_*:       exclusive scan of the histogram -> bin starts;
 //   pass 2  k_push_scatter: the identical gather + push (bitwise the same
 //                           result), scatter of the survivor into its cell's
-//                           slot of the other buffer (stream compaction of the
-//                           lost particles comes for free), and the fixed-point
-//                           CIC deposit accumulated in an LDS window around the
-//                           block's cells, flushed with one 64-bit atomic per
-//                           touched node.
+//                           slot of the other buffer (one claim per key per
+//                           block; stream compaction of the lost particles
+//                           comes for free), and the fixed-point CIC deposit
+//                           accumulated in an LDS window around the block's
+//                           cells, flushed with one 64-bit atomic per touched
+//                           node.
+// Between sort passes, k_push_inplace advances a species in its own slots.
 // Per-phase kernels for the reference-shaped entry points (gather, move,
 // collect_charge, wall pass) reuse the same device functions.
 #include "mcc.h"
@@ -42,75 +45,6 @@ __device__ __forceinline__ void store_state(const SpView &v, uint32_t i, const P
     v.id[i] = id;
 }
 
-
-// Wavefront key grouping: lanes holding the same key form one group (one per
-// distinct key in the wavefront; leader = lowest lane).  Pure ALU/ballot work,
-// so the atomics that follow are issued by all leaders in a single
-// instruction instead of one round trip per group.  Sorted input keeps the
-// number of groups per wavefront small.
-struct WaveGroup {
-    int leader;      // lane of this lane's group leader (-1 if inactive)
-    uint32_t rank;   // position of this lane within its group
-    uint32_t count;  // group size (meaningful on the leader)
-};
-
-__device__ __forceinline__ WaveGroup wave_group(uint32_t key, bool active) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    WaveGroup g{-1, 0u, 0u};
-    uint64_t pending = __ballot(active);
-    while (pending) {
-        int leader = __ffsll((unsigned long long)pending) - 1;  // wave-uniform
-        uint32_t lkey = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
-        uint64_t grp = __ballot(active && key == lkey) & pending;
-        if ((grp >> lane) & 1ull) {
-            g.leader = leader;
-            g.rank = (uint32_t)__popcll(grp & below);
-            g.count = (uint32_t)__popcll(grp);
-        }
-        pending &= ~grp;
-    }
-    return g;
-}
-
-// histogram update: one atomic per distinct key of the wavefront
-__device__ __forceinline__ void wave_count(uint32_t *__restrict__ binCount, uint32_t key,
-                                           bool active) {
-    WaveGroup g = wave_group(key, active);
-    if (active && g.rank == 0) atomicAdd(&binCount[key], g.count);
-}
-
-// scatter slots: each leader takes count slots of its bin with one atomicSub
-// (counts run down to zero, leaving the histogram cleared for the next step);
-// members take distinct slots by rank.  Returns 0xFFFFFFFF for inactive lanes.
-// The claim is split in two so the atomic's round trip overlaps other work:
-// wave_claim_issue() starts the leader's atomicSub and the bin-start load,
-// wave_claim_slot() consumes them.
-struct Claim {
-    WaveGroup g;
-    uint32_t old;    // leader: bin count before the subtraction
-    uint32_t start;  // bin start of this lane's key
-};
-
-__device__ __forceinline__ Claim wave_claim_issue(uint32_t *__restrict__ binCount,
-                                                  const uint32_t *__restrict__ binStart,
-                                                  uint32_t key, bool active) {
-    Claim c;
-    c.g = wave_group(key, active);
-    c.old = 0;
-    c.start = 0;
-    if (active) c.start = binStart[key];
-    if (active && c.g.rank == 0) c.old = atomicSub(&binCount[key], c.g.count);
-    return c;
-}
-
-// Returns the slot, or 0xFFFFFFFF for inactive lanes.
-__device__ __forceinline__ uint32_t wave_claim_slot(const Claim &c, bool active) {
-    uint32_t old = __shfl(c.old, c.g.leader < 0 ? 0 : c.g.leader, 64);
-    if (!active) return 0xFFFFFFFFu;
-    // pass 1 counted every member into this bin, so old >= count > rank
-    return (old > c.g.rank) ? c.start + old - 1u - c.g.rank : 0xFFFFFFFFu;
-}
 
 // ------------------------------------------------------------ pass 1
 
@@ -225,38 +159,148 @@ __device__ __forceinline__ void block_collide(bool active, PState &s, uint32_t i
     __syncthreads();  // the buffer is reused (deposit window) after this
 }
 
+// Block-level key aggregation for the sort pass: the keys of the block's
+// 256 x IPP particles go into an LDS hash table (open addressing, at most half
+// full); each distinct key then costs one global atomic per block, and the
+// block's particles of one key take consecutive slots.  Every particle's rank
+// within its key comes from the LDS counter of its entry.
+#define IPP 4
+#define HASH_BITS 11
+#define HASH_SIZE (1 << HASH_BITS)  // >= 2 x 256 x IPP
+#define HASH_EMPTY 0xFFFFFFFFu
+
+struct BlockHash {
+    uint32_t key[HASH_SIZE];   // pass 2, after the claims: slot of the entry's rank-0 particle
+    uint32_t cnt[HASH_SIZE];   // pass 2, after the claims: the bin count before the claim
+};
+
+__device__ __forceinline__ void hash_clear(BlockHash &H) {
+    for (int e = threadIdx.x; e < HASH_SIZE; e += blockDim.x) {
+        H.key[e] = HASH_EMPTY;
+        H.cnt[e] = 0u;
+    }
+}
+
+// Wavefront key grouping: lanes holding the same key form one group (one per
+// distinct key in the wavefront; leader = lowest lane).  Sorted input keeps
+// the number of groups per wavefront small.
+struct WaveGroup {
+    int leader;      // lane of this lane's group leader (-1 if inactive)
+    uint32_t rank;   // position of this lane within its group
+    uint32_t count;  // group size (meaningful on the leader)
+};
+
+__device__ __forceinline__ WaveGroup wave_group(uint32_t key, bool active) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    WaveGroup g{-1, 0u, 0u};
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;  // wave-uniform
+        const uint32_t lkey = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+        const uint64_t grp = __ballot(active && key == lkey) & pending;
+        if ((grp >> lane) & 1ull) {
+            g.leader = leader;
+            g.rank = (uint32_t)__popcll(grp & below);
+            g.count = (uint32_t)__popcll(grp);
+        }
+        pending &= ~grp;
+    }
+    return g;
+}
+
+// entry of `key` in the table (inserted if new)
+__device__ __forceinline__ uint32_t hash_entry(BlockHash &H, uint32_t key) {
+    uint32_t e = (key * 0x9E3779B1u) >> (32 - HASH_BITS);
+    for (;;) {  // the table never fills: at most 256 x IPP keys in 2x as many entries
+        const uint32_t old = atomicCAS(&H.key[e], HASH_EMPTY, key);
+        if (old == HASH_EMPTY || old == key) break;
+        e = (e + 1u) & (HASH_SIZE - 1);
+    }
+    return e;
+}
+
+// Called by every lane of the wavefront.  Active lanes get the entry of their
+// key and their rank among the block's particles of that key: the lanes of
+// one key in a wavefront take consecutive ranks in lane order (one LDS
+// atomic per distinct key per wavefront), so their slots are contiguous.
+__device__ __forceinline__ uint32_t hash_insert(BlockHash &H, uint32_t key, bool active,
+                                                uint32_t &rank) {
+    const WaveGroup g = wave_group(key, active);
+    uint32_t e = 0u, b = 0u;
+    if (active && g.rank == 0) {
+        e = hash_entry(H, key);
+        b = atomicAdd(&H.cnt[e], g.count);
+    }
+    const int src = g.leader < 0 ? 0 : g.leader;
+    e = (uint32_t)__shfl((int)e, src, 64);
+    b = (uint32_t)__shfl((int)b, src, 64);
+    rank = b + g.rank;
+    return e;
+}
+
+// one particle of k_push_count
+__device__ __forceinline__ void count_round(PState &s, uint32_t pid, bool valid,
+                                            const float2 *__restrict__ E, const PushConst &pc,
+                                            const mcc::MccConst &mc, uint32_t step, int do_push,
+                                            bool blk, BlockHash &H, uint32_t *sl,
+                                            unsigned long long *lds) {
+    int ev = mcc::EV_NONE;
+    mcc::Child ch{};
+    if (blk) block_collide(valid && s.jx != CELL_HOLE, s, pid, mc, step, ev, ch, lds);
+    bool kept = false;
+    uint32_t key = 0u;
+    if (valid) {
+        const int st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, blk);
+        kept = st == ST_KEPT;
+        if (kept)
+            key = bin_key(s.jx, s.jy, pc.tilesX) >> pc.keyshift;
+        else if (st != ST_HOLE)
+            atomicAdd(&sl[st - 1], 1u);
+    }
+    uint32_t rank;
+    hash_insert(H, key, kept, rank);
+}
+
+// Pass 1 of the sort pass: collide + push + wall test of IPP particles per
+// thread (strided by the block size), histogram of the new keys through the
+// block's hash table, per-wall loss counts.
 __global__ void __launch_bounds__(256)
     k_push_count(SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E,
                  PushConst pc, mcc::MccConst mc, uint32_t step, uint32_t *__restrict__ binCount,
                  uint32_t *__restrict__ lost6, int do_push) {
     __shared__ uint32_t sl[6];
     __shared__ unsigned long long lds[BLOCK_LDS_U64];
+    __shared__ BlockHash H;
     if (threadIdx.x < 6) sl[threadIdx.x] = 0;
-    __syncthreads();
-    uint32_t n = *nptr;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool kept = false;
-    uint32_t key = 0;
+    hash_clear(H);
+    const uint32_t n = *nptr;
+    const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
+    const bool need_vz = pc.bfield != 0 || mc.on;
     const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
-    PState s{};
-    uint32_t pid = 0;
-    if (i < n) {
-        s = load_state(sp, i, pc.bfield != 0 || mc.on);
-        pid = mc.on ? sp.id[i] : 0u;
+    PState s[IPP];
+    uint32_t pid[IPP];
+#pragma unroll
+    for (int r = 0; r < IPP; r++) {  // every load first
+        const uint32_t i = base + r * blockDim.x;
+        s[r] = PState{};
+        pid[r] = 0u;
+        if (i < n) {
+            s[r] = load_state(sp, i, need_vz);
+            if (mc.on) pid[r] = sp.id[i];
+        }
     }
-    int ev = mcc::EV_NONE;
-    mcc::Child ch{};
-    if (blk) block_collide(i < n && s.jx != CELL_HOLE, s, pid, mc, step, ev, ch, lds);
-    if (i < n) {
-        int st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, blk);
-        kept = (st == ST_KEPT);
-        if (kept)
-            key = bin_key(s.jx, s.jy, pc.tilesX);
-        else if (st != ST_HOLE)
-            atomicAdd(&sl[st - 1], 1u);
-    }
-    wave_count(binCount, key, kept);
+    __syncthreads();  // table and counters cleared
+    // the rounds are spelled out so every array index is a compile-time constant
+    count_round(s[0], pid[0], base + 0 * blockDim.x < n, E, pc, mc, step, do_push, blk, H, sl, lds);
+    count_round(s[1], pid[1], base + 1 * blockDim.x < n, E, pc, mc, step, do_push, blk, H, sl, lds);
+    count_round(s[2], pid[2], base + 2 * blockDim.x < n, E, pc, mc, step, do_push, blk, H, sl, lds);
+    count_round(s[3], pid[3], base + 3 * blockDim.x < n, E, pc, mc, step, do_push, blk, H, sl, lds);
     __syncthreads();
+    for (int e = threadIdx.x; e < HASH_SIZE; e += blockDim.x) {
+        const uint32_t k = H.key[e];
+        if (k != HASH_EMPTY) atomicAdd(&binCount[k], H.cnt[e]);
+    }
     if (threadIdx.x < 6 && sl[threadIdx.x])
         atomicAdd(&lost6[cnt_slot() * 6 + threadIdx.x], sl[threadIdx.x]);
 }
@@ -621,45 +665,104 @@ __device__ __forceinline__ void stage_products(int ev, const mcc::Child &ch, con
         stage_put(sp.wq, &sp.cw->wall_count, &sp.cw->overflow, post, (unsigned long long)pid, w);
 }
 
+// one particle of k_push_scatter (before the claims)
+__device__ __forceinline__ void scatter_round(PState &s, uint32_t pid, float w, bool valid,
+                                              bool &kept, bool &dep, uint32_t &ent,
+                                              uint32_t &rank, const float2 *__restrict__ E,
+                                              const PushConst &pc, const mcc::MccConst &mc,
+                                              uint32_t step, int do_push, bool blk, BlockHash &H,
+                                              uint32_t *se, unsigned long long *lds,
+                                              const Spawn &sp, uint32_t *__restrict__ lost6) {
+    const PState pre = s;
+    int ev = mcc::EV_NONE, st = -1;
+    mcc::Child ch{};
+    if (blk) block_collide(valid && s.jx != CELL_HOLE, s, pid, mc, step, ev, ch, lds);
+    if (valid) st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, blk);
+    kept = st == ST_KEPT;
+    dep = kept && deposit_ok(s, pc.N, pc.mode);
+    ent = hash_insert(H, kept ? bin_key(s.jx, s.jy, pc.tilesX) >> pc.keyshift : 0u, kept, rank);
+    if (mc.on) {
+        if (EV_KIND(ev) > 0) atomicAdd(&se[EV_KIND(ev)], 1u);
+        if (ev & EV_OVER_BOUND) atomicAdd(&se[6], 1u);
+    }
+    stage_products(ev, ch, pre, st, valid, pid, E, pc, mc, step, sp, lost6, w, s);
+}
+
+// Pass 2 of the sort pass: the identical collide + push of IPP particles per
+// thread, one claim (atomicSub on the bin count, which runs back to zero) per
+// distinct key of the block, the survivors written to consecutive slots of
+// their key in the other buffer, the deposit through one LDS window, the
+// products staged.
 __global__ void __launch_bounds__(256)
     k_push_scatter(SpView src, SpView dst, const uint32_t *__restrict__ nptr, uint32_t cap,
                    const float2 *__restrict__ E, PushConst pc, mcc::MccConst mc, uint32_t step,
                    uint32_t *__restrict__ binCount, const uint32_t *__restrict__ binStart,
                    unsigned long long *__restrict__ q, int do_push, int deposit, Spawn sp,
                    unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6) {
-    uint32_t n = *nptr;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = i < n;
-    PState s{}, pre{};
-    int st = -1, ev = mcc::EV_NONE;
-    mcc::Child ch{};
-    uint32_t key = 0, pid = 0;
-    float w = 1.0f;
     __shared__ unsigned long long lds[BLOCK_LDS_U64];
+    __shared__ BlockHash H;
+    __shared__ uint32_t se[8];
+    if (threadIdx.x < 8) se[threadIdx.x] = 0;
+    hash_clear(H);
+    const uint32_t n = *nptr;
+    const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
     const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
-    if (valid) {
-        s = load_state(src, i, true);
-        pid = src.id[i];
-        w = load_w(src, i);
-        pre = s;
+    PState s[IPP];
+    uint32_t pid[IPP], ent[IPP], rank[IPP];
+    float w[IPP];
+    bool kept[IPP], dep[IPP];
+#pragma unroll
+    for (int r = 0; r < IPP; r++) {  // every load first
+        const uint32_t i = base + r * blockDim.x;
+        s[r] = PState{};
+        pid[r] = 0u;
+        w[r] = 1.0f;
+        if (i < n) {
+            s[r] = load_state(src, i, true);
+            pid[r] = src.id[i];
+            w[r] = load_w(src, i);
+        }
     }
-    if (blk) block_collide(valid && s.jx != CELL_HOLE, s, pid, mc, step, ev, ch, lds);
-    if (valid) {
-        st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, blk);
-        if (st == ST_KEPT) key = bin_key(s.jx, s.jy, pc.tilesX);
+    __syncthreads();  // table and counters cleared
+    // the rounds are spelled out so every array index is a compile-time constant
+    scatter_round(s[0], pid[0], w[0], base + 0 * blockDim.x < n, kept[0], dep[0], ent[0], rank[0],
+                  E, pc, mc, step, do_push, blk, H, se, lds, sp, lost6);
+    scatter_round(s[1], pid[1], w[1], base + 1 * blockDim.x < n, kept[1], dep[1], ent[1], rank[1],
+                  E, pc, mc, step, do_push, blk, H, se, lds, sp, lost6);
+    scatter_round(s[2], pid[2], w[2], base + 2 * blockDim.x < n, kept[2], dep[2], ent[2], rank[2],
+                  E, pc, mc, step, do_push, blk, H, se, lds, sp, lost6);
+    scatter_round(s[3], pid[3], w[3], base + 3 * blockDim.x < n, kept[3], dep[3], ent[3], rank[3],
+                  E, pc, mc, step, do_push, blk, H, se, lds, sp, lost6);
+    __syncthreads();
+    // one claim per distinct key: the block's particles of key k take the
+    // slots start(k) + old - 1 - rank, rank < count
+    for (int e = threadIdx.x; e < HASH_SIZE; e += blockDim.x) {
+        const uint32_t k = H.key[e];
+        if (k != HASH_EMPTY) {
+            const uint32_t old = atomicSub(&binCount[k], H.cnt[e]);
+            H.cnt[e] = old;
+            H.key[e] = binStart[k] + old - 1u;
+        }
     }
-    const bool kept = (st == ST_KEPT);
-    Claim claim = wave_claim_issue(binCount, binStart, key, kept);
-    // the deposit runs while the claim's atomic is in flight
-    if (deposit) block_deposit(kept && deposit_ok(s, pc.N, pc.mode), s, q, pc.N, lds, w);
-    uint32_t slot = wave_claim_slot(claim, kept);
-    // the bound test only keeps a corrupted histogram from writing out of range
-    if (kept && slot < cap) {
-        store_state(dst, slot, s, pid);
-        if (dst.w) dst.w[slot] = w;
+    __syncthreads();
+    if (deposit) block_deposit_n<IPP>(dep, s, w, q, pc.N, lds);
+#pragma unroll
+    for (int r = 0; r < IPP; r++) {
+        // pass 1 counted every member into its bin, so old > rank; the bound
+        // tests only keep a corrupted histogram from writing out of range
+        if (kept[r] && rank[r] < H.cnt[ent[r]]) {
+            const uint32_t slot = H.key[ent[r]] - rank[r];
+            if (slot < cap) {
+                store_state(dst, slot, s[r], pid[r]);
+                if (dst.w) dst.w[slot] = w[r];
+            }
+        }
     }
-    if (mc.on) count_events(ev, events);
-    stage_products(ev, ch, pre, st, valid, pid, E, pc, mc, step, sp, lost6, w, s);
+    if (mc.on) {
+        __syncthreads();
+        if (threadIdx.x < 8 && se[threadIdx.x])
+            atomicAdd(&events[cnt_slot() * 8 + threadIdx.x], (unsigned long long)se[threadIdx.x]);
+    }
 }
 
 
@@ -707,7 +810,6 @@ __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint
 // load and store stays coalesced): one deposit window and one flush of the
 // block's counters then serve 256 * IPP particles.  Reads 20-28 B and writes
 // 20-24 B per particle, with no histogram or slot claims.
-#define IPP 4
 __global__ void __launch_bounds__(256)
     k_push_inplace(SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E,
                    PushConst pc, mcc::MccConst mc, uint32_t step, unsigned long long *__restrict__ q,
@@ -1298,7 +1400,7 @@ void launch_inject(hipStream_t st, const StageView &sv, SpCounters *cnt, const f
 void launch_push_count(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
                        const float2 *E, const PushConst &pc, const mcc::MccConst &mc, uint32_t step,
                        uint32_t *binCount, uint32_t *lost6, int do_push) {
-    hipLaunchKernelGGL(k_push_count, dim3(blocks_for(cap, 256)), dim3(256), 0, st, sp, n, E, pc, mc,
+    hipLaunchKernelGGL(k_push_count, dim3(blocks_for(cap, 256 * IPP)), dim3(256), 0, st, sp, n, E, pc, mc,
                        step, binCount, lost6, do_push);
 }
 
@@ -1307,7 +1409,7 @@ void launch_push_scatter(hipStream_t st, const SpView &src, const SpView &dst, c
                          uint32_t step, uint32_t *binCount, const uint32_t *binStart,
                          unsigned long long *q, int do_push, int deposit, const Spawn &spn,
                          unsigned long long *events, uint32_t *lost6) {
-    hipLaunchKernelGGL(k_push_scatter, dim3(blocks_for(cap, 256)), dim3(256), 0, st, src, dst, n,
+    hipLaunchKernelGGL(k_push_scatter, dim3(blocks_for(cap, 256 * IPP)), dim3(256), 0, st, src, dst, n,
                        (uint32_t)cap, E, pc, mc, step, binCount, binStart, q, do_push, deposit, spn,
                        events, lost6);
 }

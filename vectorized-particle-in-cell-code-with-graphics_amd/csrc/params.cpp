@@ -455,6 +455,7 @@ extern "C" int pic_params_default(pic_params *P) {
     P->see_E0 = 4.32;
     P->see_Emax0 = 650.0;
     P->work_function = 4.6;
+    P->sort_key = -1;
     return PIC_OK;
 }
 

@@ -48,6 +48,7 @@ struct PushConst {
     int mode;   // PIC_DEV_MODE_*
     int bfield; // magnetic rotation on/off (pic.cl:121)
     int tilesX; // sort-key tiles per side
+    int keyshift; // sort key = cell key >> keyshift (0: cells, 6: 8x8-cell tiles)
     float cE;   // 0.5 * dt * q/m
     float dtdx, dtdy;
     float cos_a, sin_a, cos_w, sin_w;

@@ -70,6 +70,7 @@ struct pic_handle {
     int N = 0;
     double dx = 0;
     int tilesX = 0;
+    int keyshift = 0;  // sort key = cell key >> keyshift
     uint32_t nbins = 0, nbins_pad = 0;
     hipStream_t st = nullptr;
     pic_host::Species sp[2];

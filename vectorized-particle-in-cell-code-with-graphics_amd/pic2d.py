@@ -125,6 +125,7 @@ class Params(C.Structure):
         ("see_E0", C.c_double),
         ("see_Emax0", C.c_double),
         ("work_function", C.c_double),
+        ("sort_key", C.c_int32),
     ]
 
     @property
