@@ -129,9 +129,10 @@ typedef struct pic_params {
     double see_E0;               /* emission threshold E0 [eV] (4.32)             */
     double see_Emax0;            /* energy of the maximum yield [eV] (650)        */
     double work_function;        /* WORK_FUNCTION [eV]                            */
-    int32_t sort_key;            /* counting-sort granularity: 0 = per cell, 6 = per
-                                    8x8-cell tile (k: cell key >> k, 0-6); -1 =
-                                    chosen by the library                          */
+    int32_t sort_key;            /* counting-sort granularity: key = cell key >> k,
+                                    k = 0..10 (0: per cell; 6: per 8x8-cell tile;
+                                    7..10: 2..16 tiles along x); -1 = chosen by
+                                    the library                                    */
 } pic_params;
 
 /* particleVec.h:26-35: the 48-byte AoS record of the reference */

@@ -729,7 +729,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     if (!grid_ok(P.nodes))
         return set_global_error(PIC_ERR_GRID, "nodes-1 must be a power of 2 (and nodes >= 5)");
     if (P.number_to_inject < 0) return set_global_error(PIC_ERR_ARG, "NUMBER_TO_INJECT < 0");
-    if (P.sort_key < -1 || P.sort_key > 6) return set_global_error(PIC_ERR_ARG, "sort_key must be -1..6");
+    if (P.sort_key < -1 || P.sort_key > 10) return set_global_error(PIC_ERR_ARG, "sort_key must be -1..10");
     if (P.reduce) {
         if (P.no_groups < 1 || P.no_groups > RD_MAX_GROUPS)
             return set_global_error(PIC_ERR_ARG, "NO_GROUPS must be 1..8");

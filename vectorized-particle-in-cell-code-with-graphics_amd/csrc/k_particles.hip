@@ -171,7 +171,7 @@ __device__ __forceinline__ void block_collide(bool active, PState &s, uint32_t i
 
 struct BlockHash {
     uint32_t key[HASH_SIZE];   // pass 2, after the claims: slot of the entry's rank-0 particle
-    uint32_t cnt[HASH_SIZE];   // pass 2, after the claims: the bin count before the claim
+    uint32_t cnt[HASH_SIZE];   // pass 2, after the claims: lowest rank with a valid slot
 };
 
 __device__ __forceinline__ void hash_clear(BlockHash &H) {
@@ -434,7 +434,6 @@ uint32_t scan_pad(uint32_t nbins) { return (nbins + SCAN_TILE - 1) / SCAN_TILE *
 
 // ------------------------------------------------------------ pass 2
 #define DEP_WCAP BLOCK_LDS_U64  // LDS window capacity (nodes), 16 KB of uint64
-#define DEP_MARGIN 2   // cells of motion allowed around the block's old cells
 
 __device__ __forceinline__ int wave_min(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
@@ -444,86 +443,34 @@ __device__ __forceinline__ int wave_max(int v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
     return v;
 }
+__device__ __forceinline__ int wave_sum(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
 
 // Deposit the block's particles (dep true) through an LDS window spanning the
 // bounding box of their cells (block reduction), flushed with one 64-bit
-// atomic per touched node; a block whose box exceeds DEP_WCAP nodes deposits
-// with global atomics directly.  Sorted particles put most lanes of a
+// atomic per touched node.  Sorted particles put most lanes of a
 // wavefront on the same four nodes, and LDS atomics to one address serialise,
 // so the window is replicated (up to DEP_COPIES copies, as many as fit) and
 // lane l adds into copy l mod copies; the flush sums the copies.  Integer
 // sums: the result does not depend on the split.  Every thread of the block
 // must call this.
 #define DEP_COPIES 8
-__device__ __forceinline__ void block_deposit(bool dep, const PState &s,
-                                              unsigned long long *__restrict__ q, int N,
-                                              unsigned long long *sq, float pw = 1.0f) {
-    __shared__ int red[4][4];
-    const int big = 1 << 30;
-    int mnx = dep ? s.jx : big, mny = dep ? s.jy : big;
-    int mxx = dep ? s.jx : -big, mxy = dep ? s.jy : -big;
-    mnx = wave_min(mnx);
-    mny = wave_min(mny);
-    mxx = wave_max(mxx);
-    mxy = wave_max(mxy);
-    int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        red[w][0] = mnx;
-        red[w][1] = mny;
-        red[w][2] = mxx;
-        red[w][3] = mxy;
-    }
-    __syncthreads();
-    int nw = blockDim.x >> 6;
-    mnx = red[0][0];
-    mny = red[0][1];
-    mxx = red[0][2];
-    mxy = red[0][3];
-    for (int k = 1; k < nw; k++) {
-        mnx = min(mnx, red[k][0]);
-        mny = min(mny, red[k][1]);
-        mxx = max(mxx, red[k][2]);
-        mxy = max(mxy, red[k][3]);
-    }
-    __syncthreads();  // red[] is reused by the next call in the same kernel
-    if (mxx < mnx) return;  // nothing to deposit (uniform: all threads)
-    // a depositing particle has jx in [0, N-2]: its nodes jx, jx+1 <= N-1
-    const int x0 = mnx, y0 = mny, ww = mxx - mnx + 2, wh = mxy - mny + 2;
-    const int area = ww * wh;
-    if (area > DEP_WCAP) {
-        if (dep) dep_global(q, N, s.jx, s.jy, fx_weights_w(s.ox, s.oy, pw));
-        return;
-    }
-    int copies = DEP_COPIES;
-    while (copies > 1 && copies * area > DEP_WCAP) copies >>= 1;
-    for (int k = threadIdx.x; k < copies * area; k += blockDim.x) sq[k] = 0ull;
-    __syncthreads();
-    if (dep) {
-        FxW fw = fx_weights_w(s.ox, s.oy, pw);
-        int b = (int)(threadIdx.x & (copies - 1)) * area + (s.jy - y0) * ww + (s.jx - x0);
-        atomicAdd(&sq[b], fw.a4);
-        atomicAdd(&sq[b + 1], fw.a3);
-        atomicAdd(&sq[b + ww], fw.a2);
-        atomicAdd(&sq[b + ww + 1], fw.a1);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < area; k += blockDim.x) {
-        unsigned long long v = 0ull;
-        for (int c = 0; c < copies; c++) v += sq[c * area + k];
-        if (v) atomicAdd(&q[(long)(y0 + k / ww) * N + x0 + k % ww], v);
-    }
-}
-
-// block_deposit for P particles per thread (dep[r] true: deposit s[r]); one
-// window and one flush for the whole block
+// block_deposit_n for P particles per thread (dep[r] true: deposit s[r]); one
+// window and one flush for the whole block.  When the bounding box of the
+// block's cells exceeds the window (a few fast particles far from the rest),
+// the window is a fixed square around the block's mean cell: the particles
+// inside it go through LDS, the others deposit with global atomics.
+#define DEP_HALF 21  // fallback window: (2 DEP_HALF + 2)^2 = 1936 nodes <= DEP_WCAP
 template <int P>
 __device__ __forceinline__ void block_deposit_n(const bool (&dep)[P], const PState (&s)[P],
                                                 const float (&pw)[P],
                                                 unsigned long long *__restrict__ q, int N,
                                                 unsigned long long *sq) {
-    __shared__ int red[4][4];
+    __shared__ int red[4][6];
     const int big = 1 << 30;
-    int mnx = big, mny = big, mxx = -big, mxy = -big;
+    int mnx = big, mny = big, mxx = -big, mxy = -big, sx = 0, sy = 0, cnt = 0;
 #pragma unroll
     for (int r = 0; r < P; r++)
         if (dep[r]) {
@@ -531,40 +478,57 @@ __device__ __forceinline__ void block_deposit_n(const bool (&dep)[P], const PSta
             mny = min(mny, s[r].jy);
             mxx = max(mxx, s[r].jx);
             mxy = max(mxy, s[r].jy);
+            sx += s[r].jx;
+            sy += s[r].jy;
+            cnt++;
         }
     mnx = wave_min(mnx);
     mny = wave_min(mny);
     mxx = wave_max(mxx);
     mxy = wave_max(mxy);
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    cnt = wave_sum(cnt);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         red[w][0] = mnx;
         red[w][1] = mny;
         red[w][2] = mxx;
         red[w][3] = mxy;
+        red[w][4] = sx;
+        red[w][5] = sy;
     }
+    __shared__ int redc[4];
+    if ((threadIdx.x & 63) == 0) redc[w] = cnt;
     __syncthreads();
     const int nw = blockDim.x >> 6;
     mnx = red[0][0];
     mny = red[0][1];
     mxx = red[0][2];
     mxy = red[0][3];
+    sx = red[0][4];
+    sy = red[0][5];
+    cnt = redc[0];
     for (int k = 1; k < nw; k++) {
         mnx = min(mnx, red[k][0]);
         mny = min(mny, red[k][1]);
         mxx = max(mxx, red[k][2]);
         mxy = max(mxy, red[k][3]);
+        sx += red[k][4];
+        sy += red[k][5];
+        cnt += redc[k];
     }
-    __syncthreads();
-    if (mxx < mnx) return;
-    const int x0 = mnx, y0 = mny, ww = mxx - mnx + 2, wh = mxy - mny + 2;
+    __syncthreads();  // red[] is reused by the next call in the same kernel
+    if (cnt == 0) return;  // nothing to deposit (uniform: all threads)
+    // a depositing particle has jx in [0, N-2]: its nodes jx, jx+1 <= N-1
+    int x0 = mnx, y0 = mny, ww = mxx - mnx + 2, wh = mxy - mny + 2;
+    if (ww * wh > DEP_WCAP) {
+        const int cx = sx / cnt, cy = sy / cnt;
+        x0 = cx - DEP_HALF;
+        y0 = cy - DEP_HALF;
+        ww = wh = 2 * DEP_HALF + 2;
+    }
     const int area = ww * wh;
-    if (area > DEP_WCAP) {
-#pragma unroll
-        for (int r = 0; r < P; r++)
-            if (dep[r]) dep_global(q, N, s[r].jx, s[r].jy, fx_weights_w(s[r].ox, s[r].oy, pw[r]));
-        return;
-    }
     int copies = DEP_COPIES;
     while (copies > 1 && copies * area > DEP_WCAP) copies >>= 1;
     for (int k = threadIdx.x; k < copies * area; k += blockDim.x) sq[k] = 0ull;
@@ -573,19 +537,34 @@ __device__ __forceinline__ void block_deposit_n(const bool (&dep)[P], const PSta
 #pragma unroll
     for (int r = 0; r < P; r++)
         if (dep[r]) {
-            FxW fw = fx_weights_w(s[r].ox, s[r].oy, pw[r]);
-            int b = cb + (s[r].jy - y0) * ww + (s[r].jx - x0);
-            atomicAdd(&sq[b], fw.a4);
-            atomicAdd(&sq[b + 1], fw.a3);
-            atomicAdd(&sq[b + ww], fw.a2);
-            atomicAdd(&sq[b + ww + 1], fw.a1);
+            const FxW fw = fx_weights_w(s[r].ox, s[r].oy, pw[r]);
+            const int dx = s[r].jx - x0, dy = s[r].jy - y0;
+            if (dx >= 0 && dx <= ww - 2 && dy >= 0 && dy <= wh - 2) {
+                const int b = cb + dy * ww + dx;
+                atomicAdd(&sq[b], fw.a4);
+                atomicAdd(&sq[b + 1], fw.a3);
+                atomicAdd(&sq[b + ww], fw.a2);
+                atomicAdd(&sq[b + ww + 1], fw.a1);
+            } else {
+                dep_global(q, N, s[r].jx, s[r].jy, fw);
+            }
         }
     __syncthreads();
+    // window nodes outside the grid (fallback window at an edge) stay zero
     for (int k = threadIdx.x; k < area; k += blockDim.x) {
         unsigned long long v = 0ull;
         for (int c = 0; c < copies; c++) v += sq[c * area + k];
         if (v) atomicAdd(&q[(long)(y0 + k / ww) * N + x0 + k % ww], v);
     }
+}
+
+__device__ __forceinline__ void block_deposit(bool dep, const PState &s,
+                                              unsigned long long *__restrict__ q, int N,
+                                              unsigned long long *sq, float pw = 1.0f) {
+    const bool d[1] = {dep};
+    const PState ss[1] = {s};
+    const float ww[1] = {pw};
+    block_deposit_n<1>(d, ss, ww, q, N, sq);
 }
 
 // staging of a created particle (slot order is arbitrary; the commit pass
@@ -690,19 +669,25 @@ __device__ __forceinline__ void scatter_round(PState &s, uint32_t pid, float w, 
 
 // Pass 2 of the sort pass: the identical collide + push of IPP particles per
 // thread, one claim (atomicSub on the bin count, which runs back to zero) per
-// distinct key of the block, the survivors written to consecutive slots of
-// their key in the other buffer, the deposit through one LDS window, the
-// products staged.
+// distinct key of the block, the deposit through one LDS window, the products
+// staged.  The survivors are then staged in LDS in slot order (each key's
+// claimed range contiguous) and written out by consecutive threads, so every
+// store instruction covers consecutive slots.  The 32 KB of LDS serve three
+// phases: collision queue + hash table, deposit window, write staging.
 __global__ void __launch_bounds__(256)
     k_push_scatter(SpView src, SpView dst, const uint32_t *__restrict__ nptr, uint32_t cap,
                    const float2 *__restrict__ E, PushConst pc, mcc::MccConst mc, uint32_t step,
                    uint32_t *__restrict__ binCount, const uint32_t *__restrict__ binStart,
                    unsigned long long *__restrict__ q, int do_push, int deposit, Spawn sp,
                    unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6) {
-    __shared__ unsigned long long lds[BLOCK_LDS_U64];
-    __shared__ BlockHash H;
-    __shared__ uint32_t se[8];
+    __shared__ unsigned long long smem[2 * BLOCK_LDS_U64];
+    __shared__ uint32_t se[8], s_total;
+    static_assert(sizeof(BlockHash) <= BLOCK_LDS_U64 * sizeof(unsigned long long), "hash size");
+    static_assert(8 * 256 * IPP <= 2 * BLOCK_LDS_U64 * 2, "staging size");
+    unsigned long long *lds = smem;  // collision queue, then deposit window
+    BlockHash &H = *reinterpret_cast<BlockHash *>(smem + BLOCK_LDS_U64);
     if (threadIdx.x < 8) se[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_total = 0;
     hash_clear(H);
     const uint32_t n = *nptr;
     const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
@@ -734,32 +719,62 @@ __global__ void __launch_bounds__(256)
     scatter_round(s[3], pid[3], w[3], base + 3 * blockDim.x < n, kept[3], dep[3], ent[3], rank[3],
                   E, pc, mc, step, do_push, blk, H, se, lds, sp, lost6);
     __syncthreads();
-    // one claim per distinct key: the block's particles of key k take the
-    // slots start(k) + old - 1 - rank, rank < count
+    // one claim per distinct key: the bin count runs down from its total, so
+    // the block's c particles of key k take the ascending slots
+    // start(k) + old - c + rank, rank < c, and the staging positions
+    // lbase + rank (a corrupt histogram, old < c, gives slots >= cap, which
+    // are skipped)
     for (int e = threadIdx.x; e < HASH_SIZE; e += blockDim.x) {
         const uint32_t k = H.key[e];
         if (k != HASH_EMPTY) {
-            const uint32_t old = atomicSub(&binCount[k], H.cnt[e]);
-            H.cnt[e] = old;
-            H.key[e] = binStart[k] + old - 1u;
+            const uint32_t c = H.cnt[e];
+            const uint32_t old = atomicSub(&binCount[k], c);
+            H.key[e] = binStart[k] + old - c;
+            H.cnt[e] = atomicAdd(&s_total, c);
         }
     }
     __syncthreads();
-    if (deposit) block_deposit_n<IPP>(dep, s, w, q, pc.N, lds);
+    uint32_t slot[IPP], lidx[IPP];
 #pragma unroll
     for (int r = 0; r < IPP; r++) {
-        // pass 1 counted every member into its bin, so old > rank; the bound
-        // tests only keep a corrupted histogram from writing out of range
-        if (kept[r] && rank[r] < H.cnt[ent[r]]) {
-            const uint32_t slot = H.key[ent[r]] - rank[r];
-            if (slot < cap) {
-                store_state(dst, slot, s[r], pid[r]);
-                if (dst.w) dst.w[slot] = w[r];
-            }
+        slot[r] = kept[r] ? H.key[ent[r]] + rank[r] : 0xFFFFFFFFu;
+        lidx[r] = kept[r] ? H.cnt[ent[r]] + rank[r] : 0u;
+    }
+    __syncthreads();  // the table is read; the deposit window reuses the LDS
+    if (deposit) block_deposit_n<IPP>(dep, s, w, q, pc.N, lds);
+    __syncthreads();
+    uint32_t *stg = reinterpret_cast<uint32_t *>(smem);  // 8 words x 256 IPP, SoA
+    constexpr uint32_t SN = 256 * IPP;
+#pragma unroll
+    for (int r = 0; r < IPP; r++) {
+        if (kept[r]) {
+            const uint32_t l = lidx[r];
+            stg[l] = slot[r];
+            stg[SN + l] = cell_pack(s[r].jx, s[r].jy);
+            stg[2 * SN + l] = __float_as_uint(s[r].ox);
+            stg[3 * SN + l] = __float_as_uint(s[r].oy);
+            stg[4 * SN + l] = __float_as_uint(s[r].vx);
+            stg[5 * SN + l] = __float_as_uint(s[r].vy);
+            stg[6 * SN + l] = __float_as_uint(s[r].vz);
+            stg[7 * SN + l] = pid[r];
+            if (dst.w && slot[r] < cap) dst.w[slot[r]] = w[r];  // weighted stores stay direct
+        }
+    }
+    __syncthreads();
+    const uint32_t tot = s_total;
+    for (uint32_t l = threadIdx.x; l < tot; l += blockDim.x) {
+        const uint32_t sl = stg[l];
+        if (sl < cap) {
+            dst.cell[sl] = stg[SN + l];
+            dst.ox[sl] = __uint_as_float(stg[2 * SN + l]);
+            dst.oy[sl] = __uint_as_float(stg[3 * SN + l]);
+            dst.vx[sl] = __uint_as_float(stg[4 * SN + l]);
+            dst.vy[sl] = __uint_as_float(stg[5 * SN + l]);
+            dst.vz[sl] = __uint_as_float(stg[6 * SN + l]);
+            dst.id[sl] = stg[7 * SN + l];
         }
     }
     if (mc.on) {
-        __syncthreads();
         if (threadIdx.x < 8 && se[threadIdx.x])
             atomicAdd(&events[cnt_slot() * 8 + threadIdx.x], (unsigned long long)se[threadIdx.x]);
     }
