@@ -109,7 +109,9 @@ int alloc_mg(pic_handle *h, MgLevels &M, int N) {
     HIPCHK(h, hipMemset(M.d.u[M.ng - 1], 0, nf * sizeof(double)));  // cold start of the iterate
     M.nparts = (int)std::min<size_t>(1024, (nf + 255) / 256);
     CHECK(dalloc(h, &M.partials, 2 * (size_t)M.nparts));
-    CHECK(dalloc(h, &M.flags, 2));
+    CHECK(dalloc(h, &M.flags, 4));
+    HIPCHK(h, hipMemset(M.flags, 0, 4 * sizeof(int)));
+    CHECK(dalloc(h, &M.ualt, nf));
     HIPCHK(h, hipHostMalloc((void **)&M.relres, sizeof(double), 0));
     HIPCHK(h, hipHostMalloc((void **)&M.host_flags, 2 * sizeof(int), 0));
     *M.relres = 0.0;
@@ -131,6 +133,8 @@ void free_mg(MgLevels &M) {
     M.dst_S = M.dst_D = nullptr;
     if (M.partials) hipFree(M.partials);
     if (M.flags) hipFree(M.flags);
+    if (M.ualt) hipFree(M.ualt);
+    M.ualt = nullptr;
     if (M.relres) hipHostFree(M.relres);
     if (M.host_flags) hipHostFree(M.host_flags);
     M.partials = nullptr;

@@ -413,36 +413,49 @@ __global__ void __launch_bounds__(1024)
 // fp32).  It replaces the coarse V-cycle of the tolerance mode's correction
 // solve; the FP64 refinement around it is unchanged.
 
-// C = A B for M x M row-major matrices in LDS; thread t < (M/2)^2 computes a
-// 2x2 tile.  Ends with a barrier.
-__device__ void lds_gemm(const float *A, const float *B, float *C, int M, int logT) {
-    const int T = 1 << logT;
+// C = A B for M x M row-major matrices in LDS (M a multiple of 4, M <= 64);
+// thread t < (M/4)^2 computes a 4x4 tile: per k four broadcast reads of A
+// and one float4 of B feed 16 FMAs.  Ends with a barrier.
+__device__ void lds_gemm(const float *A, const float *B, float *C, int M) {
+    const int T = M >> 2;
     const int t = threadIdx.x;
     if (t < T * T) {
-        const int ti = t >> logT, tj = t & (T - 1);
-        const float *a0 = A + (2 * ti) * M, *a1 = a0 + M;
-        float c00 = 0.0f, c01 = 0.0f, c10 = 0.0f, c11 = 0.0f;
-        for (int k = 0; k < M; k++) {
-            const float x0 = a0[k], x1 = a1[k];
-            const float2 b = *reinterpret_cast<const float2 *>(B + k * M + 2 * tj);
-            c00 = fmaf(x0, b.x, c00);
-            c01 = fmaf(x0, b.y, c01);
-            c10 = fmaf(x1, b.x, c10);
-            c11 = fmaf(x1, b.y, c11);
+        const int ti = t / T, tj = t - (t / T) * T;
+        const float *a = A + (4 * ti) * M;
+        float c[4][4] = {};
+        for (int k = 0; k < M; k += 4) {  // four k per step: float4 reads of A and B
+            float4 ar[4], br[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                ar[r] = *reinterpret_cast<const float4 *>(a + r * M + k);
+                br[r] = *reinterpret_cast<const float4 *>(B + (k + r) * M + 4 * tj);
+            }
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const float4 b = br[kk];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float x = kk == 0 ? ar[r].x : kk == 1 ? ar[r].y : kk == 2 ? ar[r].z : ar[r].w;
+                    c[r][0] = fmaf(x, b.x, c[r][0]);
+                    c[r][1] = fmaf(x, b.y, c[r][1]);
+                    c[r][2] = fmaf(x, b.z, c[r][2]);
+                    c[r][3] = fmaf(x, b.w, c[r][3]);
+                }
+            }
         }
-        *reinterpret_cast<float2 *>(C + (2 * ti) * M + 2 * tj) = make_float2(c00, c01);
-        *reinterpret_cast<float2 *>(C + (2 * ti + 1) * M + 2 * tj) = make_float2(c10, c11);
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            *reinterpret_cast<float4 *>(C + (4 * ti + r) * M + 4 * tj) =
+                make_float4(c[r][0], c[r][1], c[r][2], c[r][3]);
     }
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(1024)
-    k_mg_coarse_dst(const float *__restrict__ rhs_in, float *__restrict__ u_out, int lc, float h2,
-                    const float *__restrict__ Sg, const float *__restrict__ Dg,
-                    const int *__restrict__ skip) {
-    SKIP_IF_DONE(skip);
-    extern __shared__ double lds_raw[];
-    const int M = 1 << (lc + 1), n = M + 1, logT = lc;  // T = M/2 = 2^lc
+// the DST solve by one workgroup; lds_raw: 3 M^2 floats
+__device__ void dst_solve(const float *__restrict__ rhs_in, float *__restrict__ u_out, int lc,
+                          float h2, const float *__restrict__ Sg, const float *__restrict__ Dg,
+                          double *lds_raw) {
+    const int M = 1 << (lc + 1), n = M + 1;
     float *S = reinterpret_cast<float *>(lds_raw);
     float *X = S + M * M, *Y = X + M * M;
     for (int k = threadIdx.x; k < M * M; k += blockDim.x) {
@@ -451,17 +464,26 @@ __global__ void __launch_bounds__(1024)
         X[k] = (p < M - 1 && q < M - 1) ? rhs_in[(p + 1) * n + q + 1] : 0.0f;
     }
     __syncthreads();
-    lds_gemm(S, X, Y, M, logT);  // Y = S f
-    lds_gemm(Y, S, X, M, logT);  // X = S f S
+    lds_gemm(S, X, Y, M);  // Y = S f
+    lds_gemm(Y, S, X, M);  // X = S f S
     for (int k = threadIdx.x; k < M * M; k += blockDim.x) X[k] = X[k] * (Dg[k] * h2);
     __syncthreads();
-    lds_gemm(S, X, Y, M, logT);
-    lds_gemm(Y, S, X, M, logT);  // X = u on the interior
+    lds_gemm(S, X, Y, M);
+    lds_gemm(Y, S, X, M);  // X = u on the interior
     for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
         const int i = k / n, j = k - (k / n) * n;
         const bool in = i > 0 && j > 0 && i < n - 1 && j < n - 1;
         u_out[k] = in ? X[(i - 1) * M + (j - 1)] : 0.0f;
     }
+}
+
+__global__ void __launch_bounds__(1024)
+    k_mg_coarse_dst(const float *__restrict__ rhs_in, float *__restrict__ u_out, int lc, float h2,
+                    const float *__restrict__ Sg, const float *__restrict__ Dg,
+                    const int *__restrict__ skip) {
+    SKIP_IF_DONE(skip);
+    extern __shared__ double lds_raw[];
+    dst_solve(rhs_in, u_out, lc, h2, Sg, Dg, lds_raw);
 }
 
 // ------------------------------------------- fused fp32 V-cycle levels
@@ -498,13 +520,12 @@ __device__ __forceinline__ float tile_resid(const float *s, const float *f, int 
     return -h2i * (s[c + W] + s[c - W] + s[c + 1] + s[c - 1] - 4.0f * s[c]) + f[c];
 }
 
-__global__ void __launch_bounds__(256)
-    k_mg_pre(const float *__restrict__ u_in, const float *__restrict__ rhs, int n, float h2,
-             float h2i, float *__restrict__ u_out, float *__restrict__ cres,
-             const int *__restrict__ skip) {
-    SKIP_IF_DONE(skip);
-    __shared__ float su[MGW_PRE * MGW_PRE], sf[MGW_PRE * MGW_PRE];
-    const int x0 = blockIdx.x * MGT, y0 = blockIdx.y * MGT;
+// pre leg of one 32 x 32 tile (tx, ty); su, sf: MGW_PRE^2 LDS floats each
+__device__ __forceinline__ void mg_pre_tile(int tx, int ty, const float *__restrict__ u_in,
+                                            const float *__restrict__ rhs, int n, float h2,
+                                            float h2i, float *__restrict__ u_out,
+                                            float *__restrict__ cres, float *su, float *sf) {
+    const int x0 = tx * MGT, y0 = ty * MGT;
     for (int k = threadIdx.x; k < MGW_PRE * MGW_PRE; k += blockDim.x) {
         const int ly = k / MGW_PRE, lx = k % MGW_PRE;
         const int i = y0 - MGH_PRE + ly, j = x0 - MGH_PRE + lx;
@@ -539,12 +560,20 @@ __global__ void __launch_bounds__(256)
 }
 
 __global__ void __launch_bounds__(256)
-    k_mg_post(const float *__restrict__ us, const float *__restrict__ rhs,
-              const float *__restrict__ cv, int n, float h2, float *__restrict__ u_out,
-              const int *__restrict__ skip) {
+    k_mg_pre(const float *__restrict__ u_in, const float *__restrict__ rhs, int n, float h2,
+             float h2i, float *__restrict__ u_out, float *__restrict__ cres,
+             const int *__restrict__ skip) {
     SKIP_IF_DONE(skip);
-    __shared__ float su[MGW_POST * MGW_POST], sf[MGW_POST * MGW_POST];
-    const int x0 = blockIdx.x * MGT, y0 = blockIdx.y * MGT, nc = (n + 1) / 2;
+    __shared__ float su[MGW_PRE * MGW_PRE], sf[MGW_PRE * MGW_PRE];
+    mg_pre_tile(blockIdx.x, blockIdx.y, u_in, rhs, n, h2, h2i, u_out, cres, su, sf);
+}
+
+// post leg of one tile; su, sf: MGW_POST^2 LDS floats each
+__device__ __forceinline__ void mg_post_tile(int tx, int ty, const float *__restrict__ us,
+                                             const float *__restrict__ rhs,
+                                             const float *__restrict__ cv, int n, float h2,
+                                             float *__restrict__ u_out, float *su, float *sf) {
+    const int x0 = tx * MGT, y0 = ty * MGT, nc = (n + 1) / 2;
     for (int k = threadIdx.x; k < MGW_POST * MGW_POST; k += blockDim.x) {
         const int ly = k / MGW_POST, lx = k % MGW_POST;
         const int i = y0 - MGH_POST + ly, j = x0 - MGH_POST + lx;
@@ -561,6 +590,15 @@ __global__ void __launch_bounds__(256)
         const int ly = k / MGT, lx = k % MGT, i = y0 + ly, j = x0 + lx;
         if (i < n && j < n) u_out[(long)i * n + j] = su[(ly + MGH_POST) * MGW_POST + lx + MGH_POST];
     }
+}
+
+__global__ void __launch_bounds__(256)
+    k_mg_post(const float *__restrict__ us, const float *__restrict__ rhs,
+              const float *__restrict__ cv, int n, float h2, float *__restrict__ u_out,
+              const int *__restrict__ skip) {
+    SKIP_IF_DONE(skip);
+    __shared__ float su[MGW_POST * MGW_POST], sf[MGW_POST * MGW_POST];
+    mg_post_tile(blockIdx.x, blockIdx.y, us, rhs, cv, n, h2, u_out, su, sf);
 }
 
 static void coarse_dst_launch(hipStream_t st, const MgLevels &M, const float *rhs, float *u,
@@ -718,69 +756,89 @@ __device__ __forceinline__ double block_sum_d(double v) {
     return t;
 }
 
-// r32 = (float)(f - A u) at interior nodes (0 on the walls); per-block sums
-// of r^2 and f^2 in a fixed order
+// One residual pass of the refinement loop, fused: v = u_in + e (FP64
+// accumulation of the FP32 correction; e null: v = u_in) written to u_out
+// (null: not written), r = f - A v in FP32 for the next correction solve,
+// block partial sums of r^2 and f^2; the last block to finish (ticket
+// counter flags[2]) sums the partials in block order, sets flags[0] when
+// ||r|| <= tol ||f||, and counts an applied cycle in flags[1] when e is set.
+// u_in and u_out are different arrays (neighbours read u_in + e).
 __global__ void __launch_bounds__(TOL_THREADS)
-    k_tol_resid(const double *__restrict__ u, const double *__restrict__ f, int n, double h2i,
-                float *__restrict__ r32, double *__restrict__ partials,
-                const int *__restrict__ skip) {
-    SKIP_IF_DONE(skip);
+    k_tol_update(const double *__restrict__ u_in, const float *__restrict__ e,
+                 double *__restrict__ u_out, const double *__restrict__ f, int n, double h2i,
+                 float *__restrict__ r32, double *__restrict__ partials, double tol,
+                 int *__restrict__ flags, double *__restrict__ relres_host,
+                 int *__restrict__ host_flags) {
+    SKIP_IF_DONE(flags);
     double rr = 0.0, ff = 0.0;
-    long tot = (long)n * n;
+    const long tot = (long)n * n;
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < tot;
          k += (long)gridDim.x * blockDim.x) {
-        int i = (int)(k / n), j = (int)(k % n);
+        const int i = (int)(k / n), j = (int)(k % n);
         float out = 0.0f;
+        double v = u_in[k];
         if (i > 0 && j > 0 && i < n - 1 && j < n - 1) {
-            double r = resid_node(u, f, n, i, j, h2i);
-            rr += r * r;
+            if (e) {
+                v += (double)e[k];
+                const double un = u_in[k + n] + (double)e[k + n], us = u_in[k - n] + (double)e[k - n];
+                const double ue = u_in[k + 1] + (double)e[k + 1], uw = u_in[k - 1] + (double)e[k - 1];
+                const double r = -h2i * (un + us + ue + uw - 4.0 * v) + f[k];
+                rr += r * r;
+                out = (float)r;
+            } else {
+                const double r = resid_node(u_in, f, n, i, j, h2i);
+                rr += r * r;
+                out = (float)r;
+            }
             ff += f[k] * f[k];
-            out = (float)r;
         }
+        if (u_out) u_out[k] = v;
         r32[k] = out;
     }
     rr = block_sum_d(rr);
     ff = block_sum_d(ff);
+    __shared__ bool last;
     if (threadIdx.x == 0) {
         partials[2 * blockIdx.x] = rr;
         partials[2 * blockIdx.x + 1] = ff;
+        __threadfence();
+        last = atomicAdd(&flags[2], 1) == (int)gridDim.x - 1;
     }
-}
-
-// u += e (FP64 accumulation of the FP32 correction); counts applied cycles
-__global__ void k_tol_correct(double *__restrict__ u, const float *__restrict__ e, long tot,
-                              int *__restrict__ flags) {
-    SKIP_IF_DONE(flags);
-    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < tot) u[k] = u[k] + (double)e[k];
-    if (k == 0) flags[1] += 1;
-}
-
-// converged?  sums the block partials in a fixed order
-__global__ void __launch_bounds__(TOL_THREADS)
-    k_tol_check(const double *__restrict__ partials, int nparts, double tol,
-                int *__restrict__ flags, double *__restrict__ relres_host,
-                int *__restrict__ host_flags) {
-    if (flags[0]) return;
-    double rr = 0.0, ff = 0.0;
-    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
-        rr += partials[2 * k];
-        ff += partials[2 * k + 1];
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    rr = 0.0;
+    ff = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+        rr += partials[2 * b];
+        ff += partials[2 * b + 1];
     }
     rr = block_sum_d(rr);
     ff = block_sum_d(ff);
     if (threadIdx.x == 0) {
-        double rel = (ff > 0.0) ? sqrt(rr / ff) : sqrt(rr);
+        const double rel = (ff > 0.0) ? sqrt(rr / ff) : sqrt(rr);
+        if (e) flags[1] += 1;
         if (rel <= tol) flags[0] = 1;
+        flags[2] = 0;
         *relres_host = rel;
         host_flags[0] = flags[0];
         host_flags[1] = flags[1];
     }
 }
 
+// the converged iterate ends in u or ualt (cycles alternate); bring it to u
+__global__ void k_tol_final(double *__restrict__ u, const double *__restrict__ ualt, long tot,
+                            const int *__restrict__ flags) {
+    if ((flags[1] & 1) == 0) return;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < tot;
+         k += (long)gridDim.x * blockDim.x)
+        u[k] = ualt[k];
+}
+
 __global__ void k_tol_reset(int *flags) {
     flags[0] = 0;
     flags[1] = 0;
+    flags[2] = 0;
 }
 
 int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max_iter,
@@ -792,26 +850,25 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     const double *f = M.d.rhsF[lf];
     float *r32 = M.f.rhsF[lf], *e32 = M.f.u[lf];
     const int *skip = M.flags;
-    auto resid = [&]() {
-        hipLaunchKernelGGL(k_tol_resid, dim3(M.nparts), dim3(TOL_THREADS), 0, st, u, f, n, h2i, r32,
-                           M.partials, skip);
-        hipLaunchKernelGGL(k_tol_check, dim3(1), dim3(TOL_THREADS), 0, st, M.partials, M.nparts,
-                           tol, M.flags, M.relres, M.host_flags);
-    };
+    // cycle c reads the iterate from buf[c % 2] and writes buf[(c + 1) % 2]
+    double *buf[2] = {u, M.ualt};
+    int queued = 0;
     auto cycle = [&]() {
         if (lf == M.lc)
             coarse_dst_launch(st, M, r32, e32, length, skip);
         else
             vcycle_fused(st, M, lf, e32, r32, length, skip);
-        hipLaunchKernelGGL(k_tol_correct, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                           u, e32, tot, M.flags);
-        resid();
+        hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st,
+                           (const double *)buf[queued & 1], (const float *)e32,
+                           buf[(queued + 1) & 1], f, n, h2i, r32, M.partials, tol, M.flags,
+                           M.relres, M.host_flags);
     };
     hipLaunchKernelGGL(k_tol_reset, dim3(1), dim3(1), 0, st, M.flags);
-    resid();
+    hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st, (const double *)u,
+                       (const float *)nullptr, (double *)nullptr, f, n, h2i, r32, M.partials, tol,
+                       M.flags, M.relres, M.host_flags);
     // queue the predicted number of cycles without waiting (converged cycles
     // skip their work on the device), then confirm on the host
-    int queued = 0;
     int batch = M.pred < 1 ? 1 : M.pred;
     for (;;) {
         for (int k = 0; k < batch && queued < max_iter; k++, queued++) cycle();
@@ -819,6 +876,8 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
         if (M.host_flags[0] || queued >= max_iter) break;
         batch = 1;
     }
+    hipLaunchKernelGGL(k_tol_final, dim3((unsigned)std::min<long>((tot + 255) / 256, 1024)), dim3(256),
+                       0, st, u, (const double *)M.ualt, tot, (const int *)M.flags);
     int used = M.host_flags[1];
     M.pred = used + (M.host_flags[0] ? 0 : 1);
     if (relres_out) *relres_out = *M.relres;

@@ -73,8 +73,9 @@ struct MgLevels {
     MgArrays<double> d;  // FP64 full multigrid (fixed mode) and the FP64 iterate
     MgArrays<float> f;   // FP32 correction solves (tolerance mode)
     // tolerance mode state (device): flags[0] done, flags[1] V-cycles applied
-    int *flags = nullptr;
+    int *flags = nullptr;        // [2]: blocks done in the current residual pass
     double *partials = nullptr;  // per-block sums of r^2 and f^2
+    double *ualt = nullptr;      // second FP64 iterate: cycles alternate u <-> ualt
     int nparts = 0;
     double *relres = nullptr;    // pinned host: last relative residual
     int *host_flags = nullptr;   // pinned host copy of flags
@@ -241,5 +242,6 @@ void mg_fixed_solve(hipStream_t st, MgLevels &M, int ncycle, double length);
 // started from its current content); returns V-cycles used, -1 on HIP error
 int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max_iter,
                  double *relres_out);
+
 
 }  // namespace pic
