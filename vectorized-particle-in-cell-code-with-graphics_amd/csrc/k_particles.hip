@@ -55,7 +55,7 @@ __device__ __forceinline__ bool ref_mcc_inside(const PState &s, int N) {
            s.jy <= N - 1 && !(s.jy == -1 && s.oy == 0.0f);
 }
 
-// collision (if enabled and not already applied by block_collide) then push;
+// collision (if enabled and not already settled by block_collide_n) then push;
 // returns the particle's status.  `ev` and `ch` report the collision.  Both
 // passes call this with identical inputs.
 __device__ __forceinline__ int advance_one(PState &s, uint32_t id, const float2 *__restrict__ E,
@@ -82,81 +82,115 @@ __device__ __forceinline__ int advance_one(PState &s, uint32_t id, const float2 
 
 // LDS shared by the block-level collision queue and the deposit window
 #define BLOCK_LDS_U64 2048
+#define CQ_CAP 256  // collision queue entries per block (11 words each)
 
-// Block-level null-collision step (physical mode).  Every thread makes its
-// particle's candidate test (u < 1 - exp(-nu' dt)); the candidates -- about
-// 13% of the electrons in the argon discharge -- are packed into LDS so that
-// the collision kinematics run on dense lanes instead of on a few lanes of
-// every wavefront; each owner then reads its result back.  The draws are the
-// per-particle Philox streams of collide_electron / collide_ion, so the
-// result is the per-thread one bit for bit.  Every thread of the block must
-// call this (uniform condition: mc.on in physical mode).
-__device__ __forceinline__ void block_collide(bool active, PState &s, uint32_t id,
-                                              const mcc::MccConst &mc, uint32_t step, int &ev,
-                                              mcc::Child &ch, unsigned long long *lds) {
+// Block-level null-collision step (physical mode) for the P particles of every
+// thread.  Every particle makes its candidate test (u < 1 - exp(-nu' dt)); the
+// candidates -- about 13% of the electrons in the argon discharge, some 130 of
+// a block's 1024 particles -- go into one LDS queue so the collision
+// kinematics run once per block on dense lanes.  qs[r] returns the particle's
+// queue entry (>= 0), -1 (no candidate) or -2 (queue full: the particle
+// collides on its own lane in advance_one).  The results stay in the queue
+// until every round has read them (collide_take): the caller leaves the LDS
+// alone until a barrier after the last round.  The draws are the per-particle
+// Philox streams of collide_electron / collide_ion, so every result is the
+// per-lane one bit for bit.  Every thread of the block must call this (uniform
+// condition: mc.on in physical mode).
+template <int P>
+__device__ __forceinline__ void block_collide_n(const bool (&active)[P], const PState (&s)[P],
+                                                const uint32_t (&id)[P], const mcc::MccConst &mc,
+                                                uint32_t step, int (&qs)[P],
+                                                unsigned long long *lds) {
+    static_assert(11 * CQ_CAP * 4 <= BLOCK_LDS_U64 * 8, "collision queue size");
     __shared__ uint32_t wcnt[4];
-    float *qvx = reinterpret_cast<float *>(lds);
-    float *qvy = qvx + 256, *qvz = qvx + 512;
-    uint32_t *qid = reinterpret_cast<uint32_t *>(qvx + 768);
-    int *qev = reinterpret_cast<int *>(qvx + 1024);
-    float *qc = qvx + 1280;  // 6 x 256 child components
-    bool cand = false;
-    if (active) {
-        uint32_t r[4];
-        philox4x32_10(id, step, (mc.electron ? 0u : 16u) + RS_MAIN, RNG_DOMAIN_MCC, mc.key0,
-                      mc.key1, r);
-        cand = u01(r[0]) < mc.p_null;
-    }
+    float *q = reinterpret_cast<float *>(lds);  // SoA: v (3), id, event, child (6)
+    uint32_t *qid = reinterpret_cast<uint32_t *>(q + 3 * CQ_CAP);
+    int *qev = reinterpret_cast<int *>(q + 4 * CQ_CAP);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint64_t bal = __ballot(cand);
-    const uint32_t rank = (uint32_t)__popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-    if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t pos[P];
+    uint32_t wtot = 0;
+#pragma unroll
+    for (int r = 0; r < P; r++) {
+        bool cand = false;
+        if (active[r]) {
+            uint32_t x[4];
+            philox4x32_10(id[r], step, (mc.electron ? 0u : 16u) + RS_MAIN, RNG_DOMAIN_MCC,
+                          mc.key0, mc.key1, x);
+            cand = u01(x[0]) < mc.p_null;
+        }
+        const uint64_t bal = __ballot(cand);
+        pos[r] = cand ? wtot + (uint32_t)__popcll(bal & below) : 0xFFFFFFFFu;
+        wtot += (uint32_t)__popcll(bal);
+    }
+    if (lane == 0) wcnt[w] = wtot;
     __syncthreads();
     uint32_t base = 0, total = 0;
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
         if (k < w) base += wcnt[k];
         total += wcnt[k];
     }
-    const uint32_t slot = base + rank;
-    if (cand) {
-        qvx[slot] = s.vx;
-        qvy[slot] = s.vy;
-        qvz[slot] = s.vz;
-        qid[slot] = id;
+#pragma unroll
+    for (int r = 0; r < P; r++) {
+        qs[r] = -1;
+        if (pos[r] != 0xFFFFFFFFu) {
+            const uint32_t slot = base + pos[r];
+            if (slot < CQ_CAP) {
+                q[slot] = s[r].vx;
+                q[CQ_CAP + slot] = s[r].vy;
+                q[2 * CQ_CAP + slot] = s[r].vz;
+                qid[slot] = id[r];
+                qs[r] = (int)slot;
+            } else {
+                qs[r] = -2;
+            }
+        }
     }
     __syncthreads();
-    if (threadIdx.x < total) {
-        const uint32_t t = threadIdx.x;
-        float vx = qvx[t], vy = qvy[t], vz = qvz[t];
+    const uint32_t nq = min(total, (uint32_t)CQ_CAP);
+    for (uint32_t t = threadIdx.x; t < nq; t += blockDim.x) {
+        float vx = q[t], vy = q[CQ_CAP + t], vz = q[2 * CQ_CAP + t];
         mcc::Child c{};
-        int e = mc.electron ? mcc::collide_electron(mc, vx, vy, vz, qid[t], step, true, c)
-                            : mcc::collide_ion(mc, vx, vy, vz, qid[t], step);
-        qvx[t] = vx;
-        qvy[t] = vy;
-        qvz[t] = vz;
+        const int e = mc.electron ? mcc::collide_electron(mc, vx, vy, vz, qid[t], step, true, c)
+                                  : mcc::collide_ion(mc, vx, vy, vz, qid[t], step);
+        q[t] = vx;
+        q[CQ_CAP + t] = vy;
+        q[2 * CQ_CAP + t] = vz;
         qev[t] = e;
-        qc[t] = c.vx;
-        qc[256 + t] = c.vy;
-        qc[512 + t] = c.vz;
-        qc[768 + t] = c.ivx;
-        qc[1024 + t] = c.ivy;
-        qc[1280 + t] = c.ivz;
+        if (EV_KIND(e) == mcc::EV_IONIZE) {  // the only event with children
+            q[5 * CQ_CAP + t] = c.vx;
+            q[6 * CQ_CAP + t] = c.vy;
+            q[7 * CQ_CAP + t] = c.vz;
+            q[8 * CQ_CAP + t] = c.ivx;
+            q[9 * CQ_CAP + t] = c.ivy;
+            q[10 * CQ_CAP + t] = c.ivz;
+        }
     }
     __syncthreads();
-    ev = mcc::EV_NONE;
-    if (cand) {
-        s.vx = qvx[slot];
-        s.vy = qvy[slot];
-        s.vz = qvz[slot];
-        ev = qev[slot];
-        ch.vx = qc[slot];
-        ch.vy = qc[256 + slot];
-        ch.vz = qc[512 + slot];
-        ch.ivx = qc[768 + slot];
-        ch.ivy = qc[1024 + slot];
-        ch.ivz = qc[1280 + slot];
+}
+
+// A round's collision after block_collide_n: a queued particle takes its
+// result.  Returns whether the collision is settled (false: not in physical
+// mode, or the queue was full and advance_one collides the particle itself).
+__device__ __forceinline__ bool collide_take(bool blk, int qs, PState &s, int &ev, mcc::Child &ch,
+                                             const unsigned long long *lds) {
+    if (!blk) return false;
+    if (qs >= 0) {
+        const float *q = reinterpret_cast<const float *>(lds);
+        s.vx = q[qs];
+        s.vy = q[CQ_CAP + qs];
+        s.vz = q[2 * CQ_CAP + qs];
+        ev = reinterpret_cast<const int *>(q + 4 * CQ_CAP)[qs];
+        if (EV_KIND(ev) == mcc::EV_IONIZE) {
+            ch.vx = q[5 * CQ_CAP + qs];
+            ch.vy = q[6 * CQ_CAP + qs];
+            ch.vz = q[7 * CQ_CAP + qs];
+            ch.ivx = q[8 * CQ_CAP + qs];
+            ch.ivy = q[9 * CQ_CAP + qs];
+            ch.ivz = q[10 * CQ_CAP + qs];
+        }
     }
-    __syncthreads();  // the buffer is reused (deposit window) after this
+    return qs != -2;
 }
 
 // Block-level key aggregation for the sort pass: the keys of the block's
@@ -243,15 +277,15 @@ __device__ __forceinline__ uint32_t hash_insert(BlockHash &H, uint32_t key, bool
 __device__ __forceinline__ void count_round(PState &s, uint32_t pid, bool valid,
                                             const float2 *__restrict__ E, const PushConst &pc,
                                             const mcc::MccConst &mc, uint32_t step, int do_push,
-                                            bool blk, BlockHash &H, uint32_t *sl,
-                                            unsigned long long *lds) {
+                                            bool blk, int qs, BlockHash &H, uint32_t *sl,
+                                            const unsigned long long *lds) {
     int ev = mcc::EV_NONE;
     mcc::Child ch{};
-    if (blk) block_collide(valid && s.jx != CELL_HOLE, s, pid, mc, step, ev, ch, lds);
+    const bool done = collide_take(blk, qs, s, ev, ch, lds);
     bool kept = false;
     uint32_t key = 0u;
     if (valid) {
-        const int st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, blk);
+        const int st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, done);
         kept = st == ST_KEPT;
         if (kept)
             key = bin_key(s.jx, s.jy, pc.tilesX) >> pc.keyshift;
@@ -291,11 +325,22 @@ __global__ void __launch_bounds__(256)
         }
     }
     __syncthreads();  // table and counters cleared
+    int qs[IPP];
+    if (blk) {
+        bool act[IPP];
+#pragma unroll
+        for (int r = 0; r < IPP; r++) act[r] = base + r * blockDim.x < n && s[r].jx != CELL_HOLE;
+        block_collide_n<IPP>(act, s, pid, mc, step, qs, lds);
+    }
     // the rounds are spelled out so every array index is a compile-time constant
-    count_round(s[0], pid[0], base + 0 * blockDim.x < n, E, pc, mc, step, do_push, blk, H, sl, lds);
-    count_round(s[1], pid[1], base + 1 * blockDim.x < n, E, pc, mc, step, do_push, blk, H, sl, lds);
-    count_round(s[2], pid[2], base + 2 * blockDim.x < n, E, pc, mc, step, do_push, blk, H, sl, lds);
-    count_round(s[3], pid[3], base + 3 * blockDim.x < n, E, pc, mc, step, do_push, blk, H, sl, lds);
+    count_round(s[0], pid[0], base + 0 * blockDim.x < n, E, pc, mc, step, do_push, blk, qs[0], H,
+                sl, lds);
+    count_round(s[1], pid[1], base + 1 * blockDim.x < n, E, pc, mc, step, do_push, blk, qs[1], H,
+                sl, lds);
+    count_round(s[2], pid[2], base + 2 * blockDim.x < n, E, pc, mc, step, do_push, blk, qs[2], H,
+                sl, lds);
+    count_round(s[3], pid[3], base + 3 * blockDim.x < n, E, pc, mc, step, do_push, blk, qs[3], H,
+                sl, lds);
     __syncthreads();
     for (int e = threadIdx.x; e < HASH_SIZE; e += blockDim.x) {
         const uint32_t k = H.key[e];
@@ -649,14 +694,15 @@ __device__ __forceinline__ void scatter_round(PState &s, uint32_t pid, float w, 
                                               bool &kept, bool &dep, uint32_t &ent,
                                               uint32_t &rank, const float2 *__restrict__ E,
                                               const PushConst &pc, const mcc::MccConst &mc,
-                                              uint32_t step, int do_push, bool blk, BlockHash &H,
-                                              uint32_t *se, unsigned long long *lds,
-                                              const Spawn &sp, uint32_t *__restrict__ lost6) {
+                                              uint32_t step, int do_push, bool blk, int qs,
+                                              BlockHash &H, uint32_t *se,
+                                              const unsigned long long *lds, const Spawn &sp,
+                                              uint32_t *__restrict__ lost6) {
     const PState pre = s;
     int ev = mcc::EV_NONE, st = -1;
     mcc::Child ch{};
-    if (blk) block_collide(valid && s.jx != CELL_HOLE, s, pid, mc, step, ev, ch, lds);
-    if (valid) st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, blk);
+    const bool done = collide_take(blk, qs, s, ev, ch, lds);
+    if (valid) st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, done);
     kept = st == ST_KEPT;
     dep = kept && deposit_ok(s, pc.N, pc.mode);
     ent = hash_insert(H, kept ? bin_key(s.jx, s.jy, pc.tilesX) >> pc.keyshift : 0u, kept, rank);
@@ -709,16 +755,23 @@ __global__ void __launch_bounds__(256)
         }
     }
     __syncthreads();  // table and counters cleared
+    int qs[IPP];
+    if (blk) {
+        bool act[IPP];
+#pragma unroll
+        for (int r = 0; r < IPP; r++) act[r] = base + r * blockDim.x < n && s[r].jx != CELL_HOLE;
+        block_collide_n<IPP>(act, s, pid, mc, step, qs, lds);
+    }
     // the rounds are spelled out so every array index is a compile-time constant
     scatter_round(s[0], pid[0], w[0], base + 0 * blockDim.x < n, kept[0], dep[0], ent[0], rank[0],
-                  E, pc, mc, step, do_push, blk, H, se, lds, sp, lost6);
+                  E, pc, mc, step, do_push, blk, qs[0], H, se, lds, sp, lost6);
     scatter_round(s[1], pid[1], w[1], base + 1 * blockDim.x < n, kept[1], dep[1], ent[1], rank[1],
-                  E, pc, mc, step, do_push, blk, H, se, lds, sp, lost6);
+                  E, pc, mc, step, do_push, blk, qs[1], H, se, lds, sp, lost6);
     scatter_round(s[2], pid[2], w[2], base + 2 * blockDim.x < n, kept[2], dep[2], ent[2], rank[2],
-                  E, pc, mc, step, do_push, blk, H, se, lds, sp, lost6);
+                  E, pc, mc, step, do_push, blk, qs[2], H, se, lds, sp, lost6);
     scatter_round(s[3], pid[3], w[3], base + 3 * blockDim.x < n, kept[3], dep[3], ent[3], rank[3],
-                  E, pc, mc, step, do_push, blk, H, se, lds, sp, lost6);
-    __syncthreads();
+                  E, pc, mc, step, do_push, blk, qs[3], H, se, lds, sp, lost6);
+    __syncthreads();  // every round has read its collision result
     // one claim per distinct key: the bin count runs down from its total, so
     // the block's c particles of key k take the ascending slots
     // start(k) + old - c + rank, rank < c, and the staging positions
@@ -788,14 +841,15 @@ __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint
                                               const float2 *__restrict__ E, const PushConst &pc,
                                               const mcc::MccConst &mc, uint32_t step,
                                               const Spawn &spn, uint32_t *__restrict__ lost6,
-                                              bool need_vz, bool blk, uint32_t *sl, uint32_t *shole,
-                                              uint32_t *se, unsigned long long *lds, float w) {
+                                              bool need_vz, bool blk, int qs, uint32_t *sl,
+                                              uint32_t *shole, uint32_t *se,
+                                              const unsigned long long *lds, float w) {
     const PState pre = s;
     int ev = mcc::EV_NONE;
     mcc::Child ch{};
-    if (blk) block_collide(live, s, pid, mc, step, ev, ch, lds);
+    const bool done = collide_take(blk, qs, s, ev, ch, lds);
     int st = ST_HOLE;
-    if (live) st = advance_one(s, pid, E, pc, mc, step, 1, ev, ch, blk);
+    if (live) st = advance_one(s, pid, E, pc, mc, step, 1, ev, ch, done);
     const bool kept = (st == ST_KEPT);
     dep = kept && deposit_ok(s, pc.N, pc.mode);
     if (kept) {
@@ -864,16 +918,20 @@ __global__ void __launch_bounds__(256)
         }
     }
     __syncthreads();  // counters initialised
+    int qs[IPP];
+    if (blk) block_collide_n<IPP>(live, s, pid, mc, step, qs, lds);
     bool dep[IPP];
     // the rounds are spelled out so every array index is a compile-time constant
     inplace_round(sp, base + 0 * blockDim.x, n, s[0], pid[0], live[0], dep[0], E, pc, mc, step,
-                  spn, lost6, need_vz, blk, sl, &shole, se, lds, w[0]);
+                  spn, lost6, need_vz, blk, qs[0], sl, &shole, se, lds, w[0]);
     inplace_round(sp, base + 1 * blockDim.x, n, s[1], pid[1], live[1], dep[1], E, pc, mc, step,
-                  spn, lost6, need_vz, blk, sl, &shole, se, lds, w[1]);
+                  spn, lost6, need_vz, blk, qs[1], sl, &shole, se, lds, w[1]);
     inplace_round(sp, base + 2 * blockDim.x, n, s[2], pid[2], live[2], dep[2], E, pc, mc, step,
-                  spn, lost6, need_vz, blk, sl, &shole, se, lds, w[2]);
+                  spn, lost6, need_vz, blk, qs[2], sl, &shole, se, lds, w[2]);
     inplace_round(sp, base + 3 * blockDim.x, n, s[3], pid[3], live[3], dep[3], E, pc, mc, step,
-                  spn, lost6, need_vz, blk, sl, &shole, se, lds, w[3]);
+                  spn, lost6, need_vz, blk, qs[3], sl, &shole, se, lds, w[3]);
+    // block_deposit_n passes a barrier before it writes the window, so every
+    // round has read its collision result from the shared LDS by then
     if (deposit) block_deposit_n<IPP>(dep, s, w, q, pc.N, lds);
     __syncthreads();
     if (threadIdx.x < 6 && sl[threadIdx.x])

@@ -32,6 +32,33 @@ namespace mcc {
 
 enum Gas { GAS_AR = 0, GAS_H = 1 };
 
+// Math of the cross-section fits.  LibMath: the C library (host, and the
+// reference rule set on the device).  HwMath: the device's hardware
+// transcendentals (v_log / v_exp in base 2, v_sqrt, v_rcp; about 1 ulp each)
+// for the physical rule set, whose results are checked against an FP64
+// restatement to 2e-4 relative, not bit for bit.
+struct LibMath {
+    template <typename T> static HD T powr(T x, T y) { return pow(x, y); }
+    template <typename T> static HD T logn(T x) { return log(x); }
+    template <typename T> static HD T sqroot(T x) { return sqrt(x); }
+    template <typename T> static HD T quot(T a, T b) { return a / b; }
+};
+struct HwMath {
+    static __device__ __forceinline__ float powr(float x, float y) {
+        return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+    }
+    static __device__ __forceinline__ float logn(float x) {
+        return __builtin_amdgcn_logf(x) * 0.693147181f;
+    }
+    static __device__ __forceinline__ float sqroot(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float quot(float a, float b) {
+        return a * __builtin_amdgcn_rcpf(b);
+    }
+};
+// cos(2 pi u), sin(2 pi u): v_cos / v_sin take the angle in revolutions
+__device__ __forceinline__ float cos_rev(float u) { return __builtin_amdgcn_cosf(u); }
+__device__ __forceinline__ float sin_rev(float u) { return __builtin_amdgcn_sinf(u); }
+
 // -------------------------------------------------------- cross sections
 // e-Ar elastic (e_collisions.cpp:205-224).  REF keeps the reference's exact
 // branch tests (E exactly 0.35 or 15.0 falls in no branch: sigma 0).
@@ -47,7 +74,7 @@ HD T ar_el_ref(T E) {
     return (T)0.0;
 }
 // PHYS: same fit, breakpoints closed, negative values (0.061-0.349 eV) clamped
-template <typename T>
+template <typename T, typename M = LibMath>
 HD T ar_el_phys(T E) {
     T s;
     if (E < (T)0.35)
@@ -57,7 +84,7 @@ HD T ar_el_phys(T E) {
     else if (E < (T)15.0)
         s = -(T)0.013 * E * E * E + (T)0.232 * E * E + (T)1.066 * E + (T)0.085;
     else
-        s = (T)126.2 * pow(E, (T)-0.69);
+        s = (T)126.2 * M::powr(E, (T)-0.69);
     return s > (T)0 ? s : (T)0;
 }
 
@@ -71,7 +98,7 @@ HD T ar_iz_ref(T E) {
     if (E > (T)100.0) return (T)130.0 * pow(E, (T)-0.74);
     return (T)0.0;
 }
-template <typename T>
+template <typename T, typename M = LibMath>
 HD T ar_iz_phys(T E, T thr) {
     if (E < thr) return (T)0.0;
     T s;
@@ -79,64 +106,65 @@ HD T ar_iz_phys(T E, T thr) {
         s = (T)-3.0e-7 * E * E * E * E + (T)8.0e-5 * E * E * E - (T)0.008 * E * E + (T)0.416 * E -
             (T)4.758;
     else
-        s = (T)130.0 * pow(E, (T)-0.74);
+        s = (T)130.0 * M::powr(E, (T)-0.74);
     return s > (T)0 ? s : (T)0;
 }
 
 // e-Ar excitation, lumped 11.55 eV level.  The reference has no excitation
 // channel; this Bethe-type shape (peak 1.2e-20 m^2 at e * 11.55 eV) is this
 // library's choice, used only in PHYS mode.
-template <typename T>
+template <typename T, typename M = LibMath>
 HD T ar_exc_phys(T E, T thr) {
     if (E <= thr) return (T)0.0;
-    T x = E / thr;
-    return (T)3.262 * log(x) / x;
+    T x = M::quot(E, thr);
+    return M::quot((T)3.262 * M::logn(x), x);
 }
 
 // e-H elastic (e_collisions.cpp:267-283)
-template <typename T>
+template <typename T, typename M = LibMath>
 HD T h_el(T E) {
     if (E < (T)1.0e-2) return (T)3.834 * E * E - (T)20.82 * E + (T)40.98;
-    if (E < (T)20.0) return (T)-6.15 * log(E) + (T)20.91;
-    return (T)102.2 * pow(E, (T)-1.19);
+    if (E < (T)20.0) return (T)-6.15 * M::logn(E) + (T)20.91;
+    return (T)102.2 * M::powr(E, (T)-1.19);
 }
 // e-H ionisation, Janev (e_collisions.cpp:293-308).  The bracket/(I E) is in
 // units of 1e-17 m^2; REF uses the number as if it were 1e-20 m^2 (as the
 // reference does), PHYS converts (x 1000).
-template <typename T>
+template <typename T, typename M = LibMath>
 HD T h_iz_janev(T E) {
     const T I = (T)13.6;
     if (E < I) return (T)0.0;
-    T c = (T)1.0 - I / E;
-    return ((T)1.0 / (I * E)) * ((T)0.18450 * log(E / I) + (T)-0.032226 * c +
+    T c = (T)1.0 - M::quot(I, E);
+    return M::quot((T)1.0, I * E) * ((T)0.18450 * M::logn(M::quot(E, I)) + (T)-0.032226 * c +
                                  (T)-0.034539 * c * c + (T)1.4003 * c * c * c +
                                  (T)-2.8115 * c * c * c * c + (T)2.2986 * c * c * c * c * c);
 }
 
 // Ar+ - Ar charge exchange (ion_collisions.cpp:170-176)
-template <typename T>
+template <typename T, typename M = LibMath>
 HD T ar_cx(T E) {
-    if (E > (T)4.0) return ((T)2.0 + (T)5.5 / sqrt(E)) * (T)10.0;
-    return ((T)-2.95 * sqrt(E) + (T)10.65) * (T)10.0;
+    if (E > (T)4.0) return ((T)2.0 + M::quot((T)5.5, M::sqroot(E))) * (T)10.0;
+    return ((T)-2.95 * M::sqroot(E) + (T)10.65) * (T)10.0;
 }
 // Ar+ - Ar isotropic elastic (absent from the reference; PHYS only):
 // Phelps, J. Appl. Phys. 76 (1994) 747: 2e-19/(sqrt(e)(1+e)) + 3e-19 e/(1+e/3)^2 m^2
-template <typename T>
+template <typename T, typename M = LibMath>
 HD T ar_iso_phys(T E) {
     T e = E > (T)1e-8 ? E : (T)1e-8;
-    T d = (T)1.0 + e / (T)3.0;
-    return (T)20.0 / (sqrt(e) * ((T)1.0 + e)) + (T)30.0 * e / (d * d);
+    T d = (T)1.0 + M::quot(e, (T)3.0);
+    return M::quot((T)20.0, M::sqroot(e) * ((T)1.0 + e)) + M::quot((T)30.0 * e, d * d);
 }
 // H+ - H charge exchange, Janev (ion_collisions.cpp:185-199) in 1e-20 m^2
 // (the fit's 1e-16 cm^2).  E_kev_amu = lab energy / 1000 / mass in amu; the
 // reference divides by AMU in kilograms instead, which makes sigma vanish
 // (REF reproduces that through kev_amu_div).
-template <typename T>
+template <typename T, typename M = LibMath>
 HD T h_cx_janev(T E_kev_amu) {
     if (E_kev_amu < (T)1.0e-4) return (T)0.0;
     T e = E_kev_amu;
-    return ((T)3.2345 * log((T)235.88 / e + (T)2.3713)) /
-           ((T)1.0 + (T)0.038371 * e + (T)3.8068e-6 * pow(e, (T)3.5) + (T)1.1832e-10 * pow(e, (T)5.4));
+    return M::quot((T)3.2345 * M::logn(M::quot((T)235.88, e) + (T)2.3713),
+                   (T)1.0 + (T)0.038371 * e + (T)3.8068e-6 * M::powr(e, (T)3.5) +
+                       (T)1.1832e-10 * M::powr(e, (T)5.4));
 }
 
 // --------------------------------------------------------- parameters
@@ -172,30 +200,30 @@ enum Event { EV_NONE = 0, EV_ELASTIC = 1, EV_EXCITE = 2, EV_IONIZE = 3, EV_CX = 
 #define EV_KIND(e) ((e) & 0xFF)
 
 // total electron collision frequency at relative energy E and speed g (PHYS)
-template <typename T>
+template <typename T, typename M = LibMath>
 HD void e_nu_parts(const MccConst &c, T E, T g, T &nu_el, T &nu_ex, T &nu_iz) {
     T f = (T)c.ng * (T)1e-20 * g;
     if (c.gas == GAS_AR) {
-        nu_el = f * ar_el_phys(E);
-        nu_ex = f * ar_exc_phys(E, (T)c.e_exc);
-        nu_iz = f * ar_iz_phys(E, (T)c.e_iz);
+        nu_el = f * ar_el_phys<T, M>(E);
+        nu_ex = f * ar_exc_phys<T, M>(E, (T)c.e_exc);
+        nu_iz = f * ar_iz_phys<T, M>(E, (T)c.e_iz);
     } else {
-        T s = h_el(E);
+        T s = h_el<T, M>(E);
         nu_el = f * (s > (T)0 ? s : (T)0);
         nu_ex = (T)0;
-        nu_iz = f * (T)1000.0 * h_iz_janev(E);
+        nu_iz = f * (T)1000.0 * h_iz_janev<T, M>(E);
     }
 }
 
-template <typename T>
+template <typename T, typename M = LibMath>
 HD void i_nu_parts(const MccConst &c, T E, T g, T &nu_iso, T &nu_cx) {
     T f = (T)c.ng * (T)1e-20 * g;
     if (c.gas == GAS_AR) {
-        nu_iso = f * ar_iso_phys(E);
-        nu_cx = f * ar_cx(E);
+        nu_iso = f * ar_iso_phys<T, M>(E);
+        nu_cx = f * ar_cx<T, M>(E);
     } else {
         nu_iso = (T)0;
-        nu_cx = f * h_cx_janev(E / (T)1000.0 / (T)c.kev_amu_div);
+        nu_cx = f * h_cx_janev<T, M>(M::quot(M::quot(E, (T)1000.0), (T)c.kev_amu_div));
     }
 }
 
@@ -205,23 +233,24 @@ HD void i_nu_parts(const MccConst &c, T E, T g, T &nu_iso, T &nu_cx) {
 // (2000) 971, eq. 56; the same construction as collisions.cpp:141-169)
 __device__ __forceinline__ void rotate(float gx, float gy, float gz, float cosc, float sinc,
                                        float cosp, float sinp, float &ox, float &oy, float &oz) {
-    float g = sqrtf(gx * gx + gy * gy + gz * gz);
-    float gp = sqrtf(gy * gy + gz * gz);
+    float g = HwMath::sqroot(gx * gx + gy * gy + gz * gz);
+    float gp = HwMath::sqroot(gy * gy + gz * gz);
     if (gp < 1e-6f * g || gp == 0.0f) {  // g along x: any perpendicular pair
         ox = gx * cosc;
         oy = g * sinc * cosp;
         oz = g * sinc * sinp;
         return;
     }
+    const float s = sinc * __builtin_amdgcn_rcpf(gp);
     ox = gx * cosc + gp * sinc * cosp;
-    oy = gy * cosc - (gx * gy * cosp - g * gz * sinp) / gp * sinc;
-    oz = gz * cosc - (gx * gz * cosp + g * gy * sinp) / gp * sinc;
+    oy = gy * cosc - (gx * gy * cosp - g * gz * sinp) * s;
+    oz = gz * cosc - (gx * gz * cosp + g * gy * sinp) * s;
 }
 
 // anisotropic scattering angle (Vahedi & Surendra eq. 13; collisions.cpp:121-124)
 __device__ __forceinline__ float cos_chi(float E, float R) {
     if (E < 1e-4f) return 1.0f - 2.0f * R;
-    float c = (2.0f + E - 2.0f * powf(1.0f + E, R)) / E;
+    float c = HwMath::quot(2.0f + E - 2.0f * HwMath::powr(1.0f + E, R), E);
     return fminf(1.0f, fmaxf(-1.0f, c));
 }
 
@@ -232,11 +261,12 @@ __device__ __forceinline__ void normals3(uint32_t id, uint32_t step, uint32_t st
                                          const MccConst &c, float &a, float &b, float &d) {
     uint32_t r[4];
     philox4x32_10(id, step, stream, RNG_DOMAIN_MCC, c.key0, c.key1, r);
-    float r1 = sqrtf(-2.0f * logf(u01(r[0]))), t1 = 6.2831853f * u01(r[1]);
-    float r2 = sqrtf(-2.0f * logf(u01(r[2]))), t2 = 6.2831853f * u01(r[3]);
-    a = r1 * cosf(t1);
-    b = r1 * sinf(t1);
-    d = r2 * cosf(t2);
+    // Box-Muller
+    float r1 = HwMath::sqroot(-2.0f * HwMath::logn(u01(r[0])));
+    float r2 = HwMath::sqroot(-2.0f * HwMath::logn(u01(r[2])));
+    a = r1 * cos_rev(u01(r[1]));
+    b = r1 * sin_rev(u01(r[1]));
+    d = r2 * cos_rev(u01(r[3]));
 }
 
 // Children produced by one collision (PHYS ionisation).
@@ -313,16 +343,17 @@ __device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, fl
     nz *= c.vtn;
     float gx = vx - nx, gy = vy - ny, gz = vz - nz;
     float g2 = gx * gx + gy * gy + gz * gz;
-    float g = sqrtf(g2);
+    float g = HwMath::sqroot(g2);
     float E = (0.5f / c.q) * c.mass * g2;
     float nel, nex, niz;
-    e_nu_parts<float>(c, E, g, nel, nex, niz);
+    e_nu_parts<float, HwMath>(c, E, g, nel, nex, niz);
     const int over = (nel + nex + niz > c.nu_max) ? EV_OVER_BOUND : 0;
     float x = u01(r[1]) * c.nu_max;
-    float cp = cosf(6.2831853f * u01(r[3])), sp = sinf(6.2831853f * u01(r[3]));
+    float cp = cos_rev(u01(r[3])), sp = sin_rev(u01(r[3]));
+    const float rg = __builtin_amdgcn_rcpf(g);
     if (x < nel) {
         float cc = cos_chi(E, u01(r[2]));
-        float sc = sqrtf(fmaxf(0.0f, 1.0f - cc * cc));
+        float sc = HwMath::sqroot(fmaxf(0.0f, 1.0f - cc * cc));
         float hx, hy, hz;
         rotate(gx, gy, gz, cc, sc, cp, sp, hx, hy, hz);
         float cf = c.M / (c.mass + c.M);
@@ -335,10 +366,10 @@ __device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, fl
         float Ep = E - c.e_exc;
         if (!(Ep > 0.0f)) return EV_NONE;
         float cc = cos_chi(Ep, u01(r[2]));
-        float sc = sqrtf(fmaxf(0.0f, 1.0f - cc * cc));
+        float sc = HwMath::sqroot(fmaxf(0.0f, 1.0f - cc * cc));
         float hx, hy, hz;
         rotate(gx, gy, gz, cc, sc, cp, sp, hx, hy, hz);
-        float k = sqrtf(2.0f * c.q * Ep / c.mass) / g;
+        float k = HwMath::sqroot(HwMath::quot(2.0f * c.q * Ep, c.mass)) * rg;
         vx = nx + hx * k;
         vy = ny + hy * k;
         vz = nz + hz * k;
@@ -350,21 +381,22 @@ __device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, fl
         uint32_t q4[4];
         philox4x32_10(id, step, s0 + RS_IONIZE, RNG_DOMAIN_MCC, c.key0, c.key1, q4);
         // secondary energy (Opal, Peterson & Beaty 1971, as in Vahedi & Surendra)
-        float Eej = c.opal_B * tanf(u01(q4[0]) * atanf(Ex / (2.0f * c.opal_B)));
+        // tan(a) from the revolutions a / 2 pi
+        const float ta = u01(q4[0]) * atanf(HwMath::quot(Ex, 2.0f * c.opal_B)) * 0.159154943f;
+        float Eej = c.opal_B * HwMath::quot(sin_rev(ta), cos_rev(ta));
         float Esc = Ex - Eej;
         float cc = cos_chi(Esc, u01(r[2]));
-        float sc = sqrtf(fmaxf(0.0f, 1.0f - cc * cc));
+        float sc = HwMath::sqroot(fmaxf(0.0f, 1.0f - cc * cc));
         float hx, hy, hz;
         rotate(gx, gy, gz, cc, sc, cp, sp, hx, hy, hz);
-        float k = sqrtf(2.0f * c.q * Esc / c.mass) / g;
+        float k = HwMath::sqroot(HwMath::quot(2.0f * c.q * Esc, c.mass)) * rg;
         vx = nx + hx * k;
         vy = ny + hy * k;
         vz = nz + hz * k;
         float cc2 = cos_chi(Eej, u01(q4[1]));
-        float sc2 = sqrtf(fmaxf(0.0f, 1.0f - cc2 * cc2));
-        float ph2 = 6.2831853f * u01(q4[2]);
-        rotate(gx, gy, gz, cc2, sc2, cosf(ph2), sinf(ph2), hx, hy, hz);
-        float k2 = sqrtf(2.0f * c.q * Eej / c.mass) / g;
+        float sc2 = HwMath::sqroot(fmaxf(0.0f, 1.0f - cc2 * cc2));
+        rotate(gx, gy, gz, cc2, sc2, cos_rev(u01(q4[2])), sin_rev(u01(q4[2])), hx, hy, hz);
+        float k2 = HwMath::sqroot(HwMath::quot(2.0f * c.q * Eej, c.mass)) * rg;
         ch.vx = nx + hx * k2;
         ch.vy = ny + hy * k2;
         ch.vz = nz + hz * k2;
@@ -392,21 +424,20 @@ __device__ __forceinline__ int collide_ion(const MccConst &c, float &vx, float &
     nz *= c.vtn;
     float gx = vx - nx, gy = vy - ny, gz = vz - nz;
     float g2 = gx * gx + gy * gy + gz * gz;
-    float g = sqrtf(g2);
+    float g = HwMath::sqroot(g2);
     float E = (0.5f / c.q) * c.mass * g2;
     float niso, ncx;
-    i_nu_parts<float>(c, E, g, niso, ncx);
+    i_nu_parts<float, HwMath>(c, E, g, niso, ncx);
     const int over = (niso + ncx > c.nu_max) ? EV_OVER_BOUND : 0;
     float x = u01(r[1]) * c.nu_max;
     if (x < niso) {
         // equal masses: centre-of-mass velocity plus half the relative
         // velocity turned to an isotropic direction
         float cc = 1.0f - 2.0f * u01(r[2]);
-        float sc = sqrtf(fmaxf(0.0f, 1.0f - cc * cc));
-        float ph = 6.2831853f * u01(r[3]);
+        float sc = HwMath::sqroot(fmaxf(0.0f, 1.0f - cc * cc));
         float cmx = 0.5f * (vx + nx), cmy = 0.5f * (vy + ny), cmz = 0.5f * (vz + nz);
-        vx = cmx + 0.5f * g * sc * cosf(ph);
-        vy = cmy + 0.5f * g * sc * sinf(ph);
+        vx = cmx + 0.5f * g * sc * cos_rev(u01(r[3]));
+        vy = cmy + 0.5f * g * sc * sin_rev(u01(r[3]));
         vz = cmz + 0.5f * g * cc;
         return EV_ISO | over;
     }
