@@ -57,6 +57,7 @@ def make_params(name):
 
 
 def describe(name, p):
+    nu = p.mg_sweeps if p.mg_sweeps > 0 else 2  # the library's default
     return {
         "workload": name,
         "nodes": int(p.nodes),
@@ -67,7 +68,8 @@ def describe(name, p):
         "walls": "absorbing" if p.wall == pic2d.WALL_ABSORB else "steady-state plop",
         "mode": "physical" if p.mode == pic2d.MODE_PHYSICAL else "reference",
         "multigrid": ("fp64 FMG ncycle=%d" % p.mg_ncycle) if p.mg_mode == pic2d.MG_FIXED
-        else ("fp32 smoothing + fp64 refinement, tol %g" % p.mg_tol),
+        else ("fp32 red-black V(%d,%d) cycles + fp64 refinement, tol %g"
+              % (nu, nu, p.mg_tol)),
     }
 
 
@@ -252,6 +254,9 @@ def main():
     ap.add_argument("--sort-key", type=int, default=-1,
                     help="counting-sort granularity: cell key >> k (0 cells, 6 8x8 tiles; "
                          "-1: the library default)")
+    ap.add_argument("--mg-sweeps", type=int, default=-1,
+                    help="tolerance mode: red-black sweeps per V-cycle leg, 1-3 (-1: the "
+                         "library default)")
     ap.add_argument("--runs", type=int, default=64, help="ensemble: number of runs")
     ap.add_argument("--requests", type=int, default=24, help="serving: requests per load")
     ap.add_argument("--step-scale", type=float, default=0.2,
@@ -282,6 +287,8 @@ def main():
         p.sort_interval = args.sort_interval
     if args.sort_key >= 0:
         p.sort_key = args.sort_key
+    if args.mg_sweeps > 0:
+        p.mg_sweeps = args.mg_sweeps
     log("workload", args.workload, "nodes", p.nodes, "particles/species", p.init_particle_no)
     sim = pic2d.Pic(p)
     e0, i0 = (sim.get_state(0), sim.get_state(1)) if not args.no_cpu_baseline else (None, None)

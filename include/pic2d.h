@@ -133,6 +133,9 @@ typedef struct pic_params {
                                     k = 0..10 (0: per cell; 6: per 8x8-cell tile;
                                     7..10: 2..16 tiles along x); -1 = chosen by
                                     the library                                    */
+    int32_t mg_sweeps;           /* PIC_MG_TOL: red-black sweeps before and after
+                                    the coarse correction on each level, 1..3
+                                    (V(1,1)..V(3,3)); -1 = chosen by the library   */
 } pic_params;
 
 /* particleVec.h:26-35: the 48-byte AoS record of the reference */

@@ -62,6 +62,7 @@ def test_loader_evaluates_c_expressions(tmp_path):
 #define TOTAL_RUN_TIME ((int)7*(int)3 + 5)/2
 #define PIC_MODE "PHYSICAL"
 #define PIC_LOADER "RANDOM"
+#define PIC_MG_SWEEPS 3
 #endif
 """)
     p = pic2d.load_params(f)
@@ -72,6 +73,7 @@ def test_loader_evaluates_c_expressions(tmp_path):
     assert p.wall == pic2d.WALL_ABSORB and p.have_collisions == 0
     assert p.gas == pic2d.GAS_HYDROGEN and p.mode == pic2d.MODE_PHYSICAL
     assert p.loader == pic2d.LOAD_RANDOM and p.length == 0.01
+    assert p.mg_sweeps == 3
 
 
 def test_loader_particle_count_control(tmp_path):
@@ -157,6 +159,12 @@ def test_create_reports_grid_and_device_errors():
     with pytest.raises(pic2d.PicError) as e:
         pic2d.Pic(p)
     assert e.value.code == -2          # "n-1 must be a power of 2" (mglin.cpp:25-27)
+    p = pic2d.default_params()
+    assert p.mg_sweeps == -1           # chosen by the library
+    p.mg_sweeps = 4
+    with pytest.raises(pic2d.PicError) as e:
+        pic2d.Pic(p)
+    assert e.value.code == -1 and "mg_sweeps" in str(e.value)
     with pytest.raises(pic2d.PicError) as e:
         pic2d.mglin(np.zeros((6, 6)))
     assert e.value.code == -2

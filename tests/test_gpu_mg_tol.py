@@ -43,10 +43,14 @@ def test_poisson_tol_eigenmode_and_warm_start():
     assert c2 == 0 and rr2 <= 1e-10
 
 
-def test_timestep_potential_in_tolerance_mode():
+@pytest.mark.parametrize("sweeps", [1, 2, 3])
+def test_timestep_potential_in_tolerance_mode(sweeps):
+    """V(1,1) .. V(3,3) cycles reach the same converged potential"""
+    print(f"timestep tol sweeps={sweeps}", flush=True)
     p = workloads.discharge(64, 20, mode=pic2d.MODE_PHYSICAL, mg_mode=pic2d.MG_TOL)
     p.mg_tol = 1e-11
     p.mg_max_iter = 60
+    p.mg_sweeps = sweeps
     with pic2d.Pic(p) as sim:
         e0, i0, E0 = sim.get_state(0), sim.get_state(1), sim.get_E32()
         sim.run(1)
@@ -63,3 +67,19 @@ def test_timestep_potential_in_tolerance_mode():
     assert np.array_equal(r["qe"], qe)            # same charge, bit for bit
     err = np.linalg.norm(phi - r["phi"]) / np.linalg.norm(r["phi"])
     assert err < 1e-9, err
+
+
+def test_more_sweeps_need_fewer_cycles():
+    """the smoothing per leg trades against the cycle count: from the same
+    start, V(2,2) reaches 1e-10 in fewer cycles than V(1,1)"""
+    cycles = {}
+    for nu in (1, 2):
+        p = workloads.discharge(128, 10, mode=pic2d.MODE_PHYSICAL, mg_mode=pic2d.MG_TOL)
+        p.mg_tol = 1e-10
+        p.mg_max_iter = 60
+        p.mg_sweeps = nu
+        with pic2d.Pic(p) as sim:
+            sim.run(1)
+            cycles[nu], _, rr = sim.mg_stats()
+            assert rr <= 1e-10
+    assert cycles[2] < cycles[1], cycles

@@ -734,6 +734,8 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         return set_global_error(PIC_ERR_GRID, "nodes-1 must be a power of 2 (and nodes >= 5)");
     if (P.number_to_inject < 0) return set_global_error(PIC_ERR_ARG, "NUMBER_TO_INJECT < 0");
     if (P.sort_key < -1 || P.sort_key > 10) return set_global_error(PIC_ERR_ARG, "sort_key must be -1..10");
+    if (P.mg_sweeps != -1 && (P.mg_sweeps < 1 || P.mg_sweeps > 3))
+        return set_global_error(PIC_ERR_ARG, "mg_sweeps must be -1 or 1..3");
     if (P.reduce) {
         if (P.no_groups < 1 || P.no_groups > RD_MAX_GROUPS)
             return set_global_error(PIC_ERR_ARG, "NO_GROUPS must be 1..8");
@@ -914,6 +916,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     hipMemsetAsync(h->ey64, 0, 8ull * (N + 1) * (N + 1), h->st);
     hipMemsetAsync(h->E32, 0, 8ull * ((N + 1) * N + 1), h->st);  // pic.cpp:1086: first gather sees E = 0
     if ((rc = alloc_mg(h, h->M, N)) != PIC_OK) return fail(rc);
+    h->M.nu = P.mg_sweeps > 0 ? P.mg_sweeps : 2;
 
     for (int s = 0; s < 2; s++) {
         Species &sp = h->sp[s];

@@ -487,19 +487,25 @@ __global__ void __launch_bounds__(1024)
 }
 
 // ------------------------------------------- fused fp32 V-cycle levels
-// One V(1,1) visit of a level above lc takes two launches instead of seven:
-//   pre:  red then black Gauss-Seidel (mglin.cpp:116-127) and the half-weighted
-//         residual on the coarse nodes (mglin.cpp:133-169);
-//   post: prolongation-add (mglin.cpp:70-97) then red and black again.
+// One V(nu,nu) visit of a level above lc takes two launches:
+//   pre:  nu red-black Gauss-Seidel sweep pairs (mglin.cpp:116-127) from a
+//         zero guess and the half-weighted residual on the coarse nodes
+//         (mglin.cpp:133-169);
+//   post: prolongation-add (mglin.cpp:70-97) then nu red-black pairs again.
 // Each workgroup owns a 32 x 32 tile of fine nodes and recomputes the halo it
-// needs (4 rings for pre, 2 for post) in LDS, so no other tile's output is
-// read; the per-node arithmetic and colour order are those of the unfused
-// kernels, so the iterates are identical.
+// needs in LDS (every sweep of one colour shrinks the valid region by a ring:
+// 2 nu + 2 rings for pre, which ends with the residual, 2 nu for post), so no
+// other tile's output is read; the per-node arithmetic and colour order are
+// those of the grid-wide kernels, so the iterates are identical to theirs.
+// nu = 1 is mglin's V(1,1); more sweeps per leg cost little here (a visit is
+// bound by its loads and launch, not by the LDS sweeps) and cut the cycles a
+// solve needs: the measured two-grid factor is 0.13 for V(1,1), 0.03 for
+// V(2,2) (257^2, exact 65^2 coarse level).
 #define MGT 32
-#define MGH_PRE 4
-#define MGW_PRE (MGT + 2 * MGH_PRE)
-#define MGH_POST 2
-#define MGW_POST (MGT + 2 * MGH_POST)
+template <int NU> struct MgHalo {
+    static constexpr int PRE = 2 * NU + 2, WPRE = MGT + 2 * PRE;
+    static constexpr int POST = 2 * NU, WPOST = MGT + 2 * POST;
+};
 
 // Gauss-Seidel sweep of one colour over the tile plus `ring` halo rings
 // (interior nodes only).  s, f: W x W LDS tiles whose (0,0) is global
@@ -516,31 +522,44 @@ __device__ __forceinline__ void tile_sweep(float *s, const float *f, int W, int 
     }
 }
 
+// nu red-black pairs; the valid region ends `last` rings outside the tile
+template <int NU>
+__device__ __forceinline__ void tile_smooth(float *s, const float *f, int W, int H, int x0, int y0,
+                                            int last, int n, float h2) {
+#pragma unroll
+    for (int k = 0; k < NU; k++) {
+        const int ring = last + 2 * (NU - k) - 1;
+        tile_sweep(s, f, W, H, x0, y0, ring, n, h2, 0);
+        __syncthreads();
+        tile_sweep(s, f, W, H, x0, y0, ring - 1, n, h2, 1);
+        __syncthreads();
+    }
+}
+
 __device__ __forceinline__ float tile_resid(const float *s, const float *f, int c, int W, float h2i) {
     return -h2i * (s[c + W] + s[c - W] + s[c + 1] + s[c - 1] - 4.0f * s[c]) + f[c];
 }
 
-// pre leg of one 32 x 32 tile (tx, ty); su, sf: MGW_PRE^2 LDS floats each
+// pre leg of one 32 x 32 tile (tx, ty); su, sf: WPRE^2 LDS floats each
+template <int NU>
 __device__ __forceinline__ void mg_pre_tile(int tx, int ty, const float *__restrict__ u_in,
                                             const float *__restrict__ rhs, int n, float h2,
                                             float h2i, float *__restrict__ u_out,
                                             float *__restrict__ cres, float *su, float *sf) {
+    constexpr int H = MgHalo<NU>::PRE, W = MgHalo<NU>::WPRE;
     const int x0 = tx * MGT, y0 = ty * MGT;
-    for (int k = threadIdx.x; k < MGW_PRE * MGW_PRE; k += blockDim.x) {
-        const int ly = k / MGW_PRE, lx = k % MGW_PRE;
-        const int i = y0 - MGH_PRE + ly, j = x0 - MGH_PRE + lx;
+    for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
+        const int ly = k / W, lx = k % W;
+        const int i = y0 - H + ly, j = x0 - H + lx;
         const bool in = i >= 0 && j >= 0 && i < n && j < n;
         su[k] = (in && u_in) ? u_in[(long)i * n + j] : 0.0f;  // null: zero initial guess
         sf[k] = in ? rhs[(long)i * n + j] : 0.0f;
     }
     __syncthreads();
-    tile_sweep(su, sf, MGW_PRE, MGH_PRE, x0, y0, 3, n, h2, 0);
-    __syncthreads();
-    tile_sweep(su, sf, MGW_PRE, MGH_PRE, x0, y0, 2, n, h2, 1);
-    __syncthreads();
+    tile_smooth<NU>(su, sf, W, H, x0, y0, 2, n, h2);  // valid on the tile + 2 rings
     for (int k = threadIdx.x; k < MGT * MGT; k += blockDim.x) {
         const int ly = k / MGT, lx = k % MGT, i = y0 + ly, j = x0 + lx;
-        if (i < n && j < n) u_out[(long)i * n + j] = su[(ly + MGH_PRE) * MGW_PRE + lx + MGH_PRE];
+        if (i < n && j < n) u_out[(long)i * n + j] = su[(ly + H) * W + lx + H];
     }
     const int nc = (n + 1) / 2;
     for (int k = threadIdx.x; k < (MGT / 2) * (MGT / 2); k += blockDim.x) {
@@ -548,57 +567,58 @@ __device__ __forceinline__ void mg_pre_tile(int tx, int ty, const float *__restr
         if (ic >= nc || jc >= nc) continue;
         float v = 0.0f;
         if (ic > 0 && jc > 0 && ic < nc - 1 && jc < nc - 1) {
-            const int c = (2 * ic - y0 + MGH_PRE) * MGW_PRE + (2 * jc - x0 + MGH_PRE);
-            v = 0.5f * tile_resid(su, sf, c, MGW_PRE, h2i) +
-                0.125f * (tile_resid(su, sf, c + MGW_PRE, MGW_PRE, h2i) +
-                          tile_resid(su, sf, c - MGW_PRE, MGW_PRE, h2i) +
-                          tile_resid(su, sf, c + 1, MGW_PRE, h2i) +
-                          tile_resid(su, sf, c - 1, MGW_PRE, h2i));
+            const int c = (2 * ic - y0 + H) * W + (2 * jc - x0 + H);
+            v = 0.5f * tile_resid(su, sf, c, W, h2i) +
+                0.125f * (tile_resid(su, sf, c + W, W, h2i) + tile_resid(su, sf, c - W, W, h2i) +
+                          tile_resid(su, sf, c + 1, W, h2i) + tile_resid(su, sf, c - 1, W, h2i));
         }
         cres[(long)ic * nc + jc] = v;
     }
 }
 
+template <int NU>
 __global__ void __launch_bounds__(256)
     k_mg_pre(const float *__restrict__ u_in, const float *__restrict__ rhs, int n, float h2,
              float h2i, float *__restrict__ u_out, float *__restrict__ cres,
              const int *__restrict__ skip) {
     SKIP_IF_DONE(skip);
-    __shared__ float su[MGW_PRE * MGW_PRE], sf[MGW_PRE * MGW_PRE];
-    mg_pre_tile(blockIdx.x, blockIdx.y, u_in, rhs, n, h2, h2i, u_out, cres, su, sf);
+    constexpr int W = MgHalo<NU>::WPRE;
+    __shared__ float su[W * W], sf[W * W];
+    mg_pre_tile<NU>(blockIdx.x, blockIdx.y, u_in, rhs, n, h2, h2i, u_out, cres, su, sf);
 }
 
-// post leg of one tile; su, sf: MGW_POST^2 LDS floats each
+// post leg of one tile; su, sf: WPOST^2 LDS floats each
+template <int NU>
 __device__ __forceinline__ void mg_post_tile(int tx, int ty, const float *__restrict__ us,
                                              const float *__restrict__ rhs,
                                              const float *__restrict__ cv, int n, float h2,
                                              float *__restrict__ u_out, float *su, float *sf) {
+    constexpr int H = MgHalo<NU>::POST, W = MgHalo<NU>::WPOST;
     const int x0 = tx * MGT, y0 = ty * MGT, nc = (n + 1) / 2;
-    for (int k = threadIdx.x; k < MGW_POST * MGW_POST; k += blockDim.x) {
-        const int ly = k / MGW_POST, lx = k % MGW_POST;
-        const int i = y0 - MGH_POST + ly, j = x0 - MGH_POST + lx;
+    for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
+        const int ly = k / W, lx = k % W;
+        const int i = y0 - H + ly, j = x0 - H + lx;
         const bool in = i >= 0 && j >= 0 && i < n && j < n;
         su[k] = in ? us[(long)i * n + j] + interp_node(cv, nc, i, j) : 0.0f;
         sf[k] = in ? rhs[(long)i * n + j] : 0.0f;
     }
     __syncthreads();
-    tile_sweep(su, sf, MGW_POST, MGH_POST, x0, y0, 1, n, h2, 0);
-    __syncthreads();
-    tile_sweep(su, sf, MGW_POST, MGH_POST, x0, y0, 0, n, h2, 1);
-    __syncthreads();
+    tile_smooth<NU>(su, sf, W, H, x0, y0, 0, n, h2);  // valid on the tile
     for (int k = threadIdx.x; k < MGT * MGT; k += blockDim.x) {
         const int ly = k / MGT, lx = k % MGT, i = y0 + ly, j = x0 + lx;
-        if (i < n && j < n) u_out[(long)i * n + j] = su[(ly + MGH_POST) * MGW_POST + lx + MGH_POST];
+        if (i < n && j < n) u_out[(long)i * n + j] = su[(ly + H) * W + lx + H];
     }
 }
 
+template <int NU>
 __global__ void __launch_bounds__(256)
     k_mg_post(const float *__restrict__ us, const float *__restrict__ rhs,
               const float *__restrict__ cv, int n, float h2, float *__restrict__ u_out,
               const int *__restrict__ skip) {
     SKIP_IF_DONE(skip);
-    __shared__ float su[MGW_POST * MGW_POST], sf[MGW_POST * MGW_POST];
-    mg_post_tile(blockIdx.x, blockIdx.y, us, rhs, cv, n, h2, u_out, su, sf);
+    constexpr int W = MgHalo<NU>::WPOST;
+    __shared__ float su[W * W], sf[W * W];
+    mg_post_tile<NU>(blockIdx.x, blockIdx.y, us, rhs, cv, n, h2, u_out, su, sf);
 }
 
 static void coarse_dst_launch(hipStream_t st, const MgLevels &M, const float *rhs, float *u,
@@ -611,14 +631,25 @@ static void vcycle_fused(hipStream_t st, MgLevels &M, int j, float *Uj, const fl
     const int nf = lvl_size(j);
     const double h = length / (nf - 1);
     const int tiles = (nf + MGT - 1) / MGT;
-    hipLaunchKernelGGL(k_mg_pre, dim3(tiles, tiles), dim3(256), 0, st, (const float *)nullptr, Rj,
-                       nf, (float)(h * h), (float)(1.0 / (h * h)), A.us[j], A.cres[j - 1], skip);
+    const float h2 = (float)(h * h), h2i = (float)(1.0 / (h * h));
+    const dim3 g(tiles, tiles), b(256);
+    const float *z = nullptr;
+    if (M.nu == 3)
+        hipLaunchKernelGGL(k_mg_pre<3>, g, b, 0, st, z, Rj, nf, h2, h2i, A.us[j], A.cres[j - 1], skip);
+    else if (M.nu == 2)
+        hipLaunchKernelGGL(k_mg_pre<2>, g, b, 0, st, z, Rj, nf, h2, h2i, A.us[j], A.cres[j - 1], skip);
+    else
+        hipLaunchKernelGGL(k_mg_pre<1>, g, b, 0, st, z, Rj, nf, h2, h2i, A.us[j], A.cres[j - 1], skip);
     if (j - 1 == M.lc)
         coarse_dst_launch(st, M, A.cres[j - 1], A.cv[j - 1], length, skip);
     else
         vcycle_fused(st, M, j - 1, A.cv[j - 1], A.cres[j - 1], length, skip);
-    hipLaunchKernelGGL(k_mg_post, dim3(tiles, tiles), dim3(256), 0, st, A.us[j], Rj, A.cv[j - 1],
-                       nf, (float)(h * h), Uj, skip);
+    if (M.nu == 3)
+        hipLaunchKernelGGL(k_mg_post<3>, g, b, 0, st, A.us[j], Rj, A.cv[j - 1], nf, h2, Uj, skip);
+    else if (M.nu == 2)
+        hipLaunchKernelGGL(k_mg_post<2>, g, b, 0, st, A.us[j], Rj, A.cv[j - 1], nf, h2, Uj, skip);
+    else
+        hipLaunchKernelGGL(k_mg_post<1>, g, b, 0, st, A.us[j], Rj, A.cv[j - 1], nf, h2, Uj, skip);
 }
 
 static void coarse_dst_launch(hipStream_t st, const MgLevels &M, const float *rhs, float *u,

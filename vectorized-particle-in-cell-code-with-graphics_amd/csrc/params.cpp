@@ -456,6 +456,7 @@ extern "C" int pic_params_default(pic_params *P) {
     P->see_Emax0 = 650.0;
     P->work_function = 4.6;
     P->sort_key = -1;
+    P->mg_sweeps = -1;
     return PIC_OK;
 }
 
@@ -619,6 +620,7 @@ extern "C" int pic_params_load(const char *path, pic_params *P) {
     ok = ok && num("PIC_MG_TOL", P->mg_tol);
     ok = ok && inum("PIC_MG_NCYCLE", P->mg_ncycle);
     ok = ok && inum("PIC_MG_MAX_ITER", P->mg_max_iter);
+    ok = ok && inum("PIC_MG_SWEEPS", P->mg_sweeps);
     ok = ok && inum("PIC_SORT_INTERVAL", P->sort_interval);
     // diagnostic defaults follow the loaded grid and period; explicit macros win
     double period = 0.0;

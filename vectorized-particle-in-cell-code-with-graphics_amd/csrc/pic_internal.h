@@ -84,6 +84,7 @@ struct MgLevels {
     // transform tables of level lc, M x M with M = 2^(lc+1) (padded by one)
     float *dst_S = nullptr;      // S[p][q] = sin(pi (p+1)(q+1) / M)
     float *dst_D = nullptr;      // (2/M)^2 / (lambda_p + lambda_q), h^2 applied in the kernel
+    int nu = 2;                  // red-black sweeps per leg of the fused V-cycle
 };
 
 // where created particles go during a pass (see k_particles.hip)

@@ -126,6 +126,7 @@ class Params(C.Structure):
         ("see_Emax0", C.c_double),
         ("work_function", C.c_double),
         ("sort_key", C.c_int32),
+        ("mg_sweeps", C.c_int32),
     ]
 
     @property
