@@ -480,18 +480,15 @@ uint32_t scan_pad(uint32_t nbins) { return (nbins + SCAN_TILE - 1) / SCAN_TILE *
 // ------------------------------------------------------------ pass 2
 #define DEP_WCAP BLOCK_LDS_U64  // LDS window capacity (nodes), 16 KB of uint64
 
-__device__ __forceinline__ int wave_min(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_sum(int v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+// wavefront reductions of the device library (DPP row shifts and lane reads:
+// no LDS round trips); every lane gets the result.  Called with all lanes
+// active.
+extern "C" __device__ int __ockl_wfred_min_i32(int);
+extern "C" __device__ int __ockl_wfred_max_i32(int);
+extern "C" __device__ int __ockl_wfred_add_i32(int);
+__device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
+__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
+__device__ __forceinline__ int wave_sum(int v) { return __ockl_wfred_add_i32(v); }
 
 // Deposit the block's particles (dep true) through an LDS window spanning the
 // bounding box of their cells (block reduction), flushed with one 64-bit

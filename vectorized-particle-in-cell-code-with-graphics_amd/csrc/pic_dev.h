@@ -181,14 +181,18 @@ __device__ __forceinline__ bool deposit_ok(const PState &s, int N, int mode) {
 struct FxW {
     unsigned long long a4, a3, a2, a1;
 };
+// With S = 2^20 and a1 = wx wy (one 32 x 32 -> 64-bit product), the other
+// corners are exactly S wx - a1, S wy - a1 and S^2 - S wx - S wy + a1.
 __device__ __forceinline__ FxW fx_weights(float ox, float oy) {
-    long long wx = (long long)rintf(ox * 1048576.0f);
-    long long wy = (long long)rintf(oy * 1048576.0f);
+    const int wx = (int)rintf(ox * 1048576.0f);
+    const int wy = (int)rintf(oy * 1048576.0f);
+    const long long a1 = (long long)wx * (long long)wy;
+    const long long sx = (long long)wx << FX_BITS, sy = (long long)wy << FX_BITS;
     FxW w;
-    w.a4 = (unsigned long long)((FX_ONE - wx) * (FX_ONE - wy));
-    w.a3 = (unsigned long long)(wx * (FX_ONE - wy));
-    w.a2 = (unsigned long long)((FX_ONE - wx) * wy);
-    w.a1 = (unsigned long long)(wx * wy);
+    w.a4 = (unsigned long long)(FX_ONE * FX_ONE - sx - sy + a1);
+    w.a3 = (unsigned long long)(sx - a1);
+    w.a2 = (unsigned long long)(sy - a1);
+    w.a1 = (unsigned long long)a1;
     return w;
 }
 
