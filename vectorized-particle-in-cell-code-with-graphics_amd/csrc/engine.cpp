@@ -238,14 +238,16 @@ void setup_species(pic_handle *h, Species &s, bool electron) {
     double coef = 0.5 * s.dt * s.q_over_m;  // particle.cpp:1103
     double omega_dt = s.dt * s.q_over_m * P.magnetic_field;
     // sort interval: explicit, or chosen from the thermal displacement per
-    // step d = sigma dt / dx (cells): about 6 / (d + 0.5) passes, 1..8
-    // (measured best: 4 at d = 0.85, 2 at d = 3.4; slow ions stay in place)
+    // step d = sigma dt / dx (cells): about 7 / d passes, 1..16 (measured
+    // best: 8 at d = 0.85, 2 at d = 3.4, 16 for slow ions; the in-place pass
+    // loses little to disorder, its deposit window falls back to a fixed
+    // square around the block's mean cell)
     if (P.sort_interval > 0) {
         s.sort_interval = P.sort_interval;
     } else {
         double d = s.sigma_load * s.dt / h->dx;
-        int k = (int)std::lround(6.0 / (d + 0.5));
-        s.sort_interval = std::max(1, std::min(8, k));
+        int k = (int)std::lround(7.0 / std::max(d, 0.05));
+        s.sort_interval = std::max(1, std::min(16, k));
     }
     pc.cE = (float)coef;
     pc.dtdx = (float)(s.dt / h->dx);
