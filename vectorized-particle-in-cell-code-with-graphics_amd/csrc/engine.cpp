@@ -116,6 +116,7 @@ int alloc_mg(pic_handle *h, MgLevels &M, int N) {
     HIPCHK(h, hipHostMalloc((void **)&M.host_flags, 2 * sizeof(int), 0));
     *M.relres = 0.0;
     M.host_flags[0] = M.host_flags[1] = 0;
+    HIPCHK(h, hipEventCreateWithFlags(&M.done, hipEventDisableTiming));
     return PIC_OK;
 }
 
@@ -137,6 +138,8 @@ void free_mg(MgLevels &M) {
     M.ualt = nullptr;
     if (M.relres) hipHostFree(M.relres);
     if (M.host_flags) hipHostFree(M.host_flags);
+    if (M.done) hipEventDestroy(M.done);
+    M.done = nullptr;
     M.partials = nullptr;
     M.flags = nullptr;
     M.relres = nullptr;
@@ -612,7 +615,10 @@ void absorb_species(pic_handle *h, int si, bool plop) {
     clear_holes(h, si);  // the compaction dropped them
 }
 
-int solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, double phiLY) {
+// phi from the solve with the boundary values; with_e: E (and its fp32 copy
+// for the gather) from the same values in the same launch
+int solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, double phiLY,
+                    bool with_e = false) {
     const pic_params &P = h->P;
     int N = h->N;
     MgLevels &M = h->M;
@@ -627,7 +633,11 @@ int solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, doub
         mg_fixed_solve(h->st, M, P.mg_ncycle, P.length);
         h->mg_cycles = P.mg_ncycle;
     }
-    launch_phi_bc(h->st, M.d.u[M.ng - 1], N, phi0X, phiR, phi0Y, phiLY, h->phi);
+    if (with_e)
+        launch_phi_efield(h->st, M.d.u[M.ng - 1], N, phi0X, phiR, phi0Y, phiLY, h->dx, h->phi,
+                          h->ex64, h->ey64, h->E32);
+    else
+        launch_phi_bc(h->st, M.d.u[M.ng - 1], N, phi0X, phiR, phi0Y, phiLY, h->phi);
     h->have_phi = true;
     return PIC_OK;
 }
@@ -638,8 +648,7 @@ int field_step(pic_handle *h, int64_t step) {
     int p = prof_begin(h, 3);
     launch_rho(h->st, h->sp[1].q, h->sp[0].q, h->N, h->sp[1].scale, h->sp[0].scale,
                P.corner8 ? 8.0 : 4.0, h->rho);
-    CHECK(solve_potential(h, P.phi_left, phiR, P.phi_bottom, P.phi_top));
-    launch_efield(h->st, h->phi, h->N, h->dx, h->ex64, h->ey64, h->E32);
+    CHECK(solve_potential(h, P.phi_left, phiR, P.phi_bottom, P.phi_top, true));
     prof_end(h, p);
     return PIC_OK;
 }

@@ -6,8 +6,10 @@
 // association, so FIXED mode reproduces Mglin bit for bit.  Levels up to
 // 65x65 (or what the LDS holds) run inside one workgroup with all arrays in
 // LDS; finer levels use one launch per operation.
+#include <chrono>
 #include <cmath>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "pic_internal.h"
@@ -123,6 +125,71 @@ __global__ void k_efield(const double *__restrict__ phi, int N, double twoDX,
     ex64[k] = ex;
     ey64[k] = ey;
     E32[k] = make_float2((float)ex, (float)ey);
+}
+
+// phi and E in one pass: phi at any node from the solver's iterate and the
+// boundary values (k_phi_bc's rule), E as k_efield computes it from phi.  The
+// values are those of the two-kernel path, bit for bit.
+struct PhiBc {
+    double x0, xR, y0, yL;
+};
+__device__ __forceinline__ double phi_at(const double *u, int N, int j, int i, const PhiBc &b) {
+    double v = u[(long)j * N + i];
+    if (i == 0) v = b.x0;
+    if (i == N - 1) v = b.xR;
+    if (j == 0) v = b.y0;
+    if (j == N - 1) v = b.yL;
+    return v;
+}
+__device__ __forceinline__ double exu(const double *u, int N, int j, int i, double twoDX,
+                                      const PhiBc &b) {
+    if (j < 1 || j > N - 2 || i < 1 || i > N - 2) return 0.0;
+    return (phi_at(u, N, j, i - 1, b) - phi_at(u, N, j, i + 1, b)) / twoDX;
+}
+__device__ __forceinline__ double eyu(const double *u, int N, int j, int i, double twoDX,
+                                      const PhiBc &b) {
+    if (j < 1 || j > N - 2 || i < 1 || i > N - 2) return 0.0;
+    return (phi_at(u, N, j - 1, i, b) - phi_at(u, N, j + 1, i, b)) / twoDX;
+}
+
+__global__ void k_phi_efield(const double *__restrict__ u, int N, PhiBc b, double twoDX,
+                             double *__restrict__ phi, double *__restrict__ ex64,
+                             double *__restrict__ ey64, float2 *__restrict__ E32) {
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long tot32 = (long)(N + 1) * N + 1, tot64 = (long)(N + 1) * (N + 1);
+    if (k >= tot64) return;
+    if (k >= (long)N * N) {
+        if (k < tot32) E32[k] = make_float2(0.0f, 0.0f);
+        ex64[k] = 0.0;
+        ey64[k] = 0.0;
+        return;
+    }
+    int j = (int)(k / N), i = (int)(k % N);
+    phi[k] = phi_at(u, N, j, i, b);
+    double ex, ey;
+    if (i == 0)
+        ex = 2.0 * exu(u, N, j, 1, twoDX, b) - exu(u, N, j, 2, twoDX, b);
+    else if (i == N - 1)
+        ex = 2.0 * exu(u, N, j, N - 2, twoDX, b) - exu(u, N, j, N - 3, twoDX, b);
+    else
+        ex = exu(u, N, j, i, twoDX, b);
+    if (j == 0)
+        ey = 2.0 * eyu(u, N, 1, i, twoDX, b) - eyu(u, N, 2, i, twoDX, b);
+    else if (j == N - 1)
+        ey = 2.0 * eyu(u, N, N - 2, i, twoDX, b) - eyu(u, N, N - 3, i, twoDX, b);
+    else
+        ey = eyu(u, N, j, i, twoDX, b);
+    ex64[k] = ex;
+    ey64[k] = ey;
+    E32[k] = make_float2((float)ex, (float)ey);
+}
+
+void launch_phi_efield(hipStream_t st, const double *u, int N, double phi0X, double phiR,
+                       double phi0Y, double phiLY, double dx, double *phi, double *ex64,
+                       double *ey64, float2 *E32) {
+    long tot = (long)(N + 1) * (N + 1);
+    hipLaunchKernelGGL(k_phi_efield, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, u, N,
+                       PhiBc{phi0X, phiR, phi0Y, phiLY}, 2.0 * dx, phi, ex64, ey64, E32);
 }
 
 void launch_efield(hipStream_t st, const double *phi, int N, double dx, double *ex64, double *ey64,
@@ -857,25 +924,28 @@ __global__ void __launch_bounds__(TOL_THREADS)
     }
 }
 
-// the converged iterate ends in u or ualt (cycles alternate); bring it to u
-__global__ void k_tol_final(double *__restrict__ u, const double *__restrict__ ualt, long tot,
-                            const int *__restrict__ flags) {
-    if ((flags[1] & 1) == 0) return;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < tot;
-         k += (long)gridDim.x * blockDim.x)
-        u[k] = ualt[k];
-}
-
 __global__ void k_tol_reset(int *flags) {
     flags[0] = 0;
     flags[1] = 0;
     flags[2] = 0;
 }
 
+// Wait for the queued cycles: the solve's host check sits between two phases
+// of the step, so the wake-up latency of a blocking wait is idle GPU time;
+// poll the batch's event for up to 2 ms, then block (also reports errors).
+static hipError_t wait_batch(hipStream_t st, hipEvent_t ev) {
+    hipError_t e = hipEventRecord(ev, st);
+    if (e != hipSuccess) return e;
+    const auto t0 = std::chrono::steady_clock::now();
+    while ((e = hipEventQuery(ev)) == hipErrorNotReady)
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(2000))
+            return hipEventSynchronize(ev);
+    return e;
+}
+
 int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max_iter,
                  double *relres_out) {
     int lf = M.ng - 1, n = lvl_size(lf);
-    long tot = (long)n * n;
     double h = length / (n - 1), h2i = 1.0 / (h * h);
     double *u = M.d.u[lf];
     const double *f = M.d.rhsF[lf];
@@ -903,13 +973,13 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     int batch = M.pred < 1 ? 1 : M.pred;
     for (;;) {
         for (int k = 0; k < batch && queued < max_iter; k++, queued++) cycle();
-        if (hipStreamSynchronize(st) != hipSuccess) return -1;
+        if (wait_batch(st, M.done) != hipSuccess) return -1;
         if (M.host_flags[0] || queued >= max_iter) break;
         batch = 1;
     }
-    hipLaunchKernelGGL(k_tol_final, dim3((unsigned)std::min<long>((tot + 255) / 256, 1024)), dim3(256),
-                       0, st, u, (const double *)M.ualt, tot, (const int *)M.flags);
     int used = M.host_flags[1];
+    // the iterate of an odd number of applied cycles is in ualt: it becomes u
+    if (used & 1) std::swap(M.d.u[lf], M.ualt);
     M.pred = used + (M.host_flags[0] ? 0 : 1);
     if (relres_out) *relres_out = *M.relres;
     return used;

@@ -85,6 +85,7 @@ struct MgLevels {
     float *dst_S = nullptr;      // S[p][q] = sin(pi (p+1)(q+1) / M)
     float *dst_D = nullptr;      // (2/M)^2 / (lambda_p + lambda_q), h^2 applied in the kernel
     int nu = 2;                  // red-black sweeps per leg of the fused V-cycle
+    hipEvent_t done = nullptr;   // end of a queued batch of cycles (host wait)
 };
 
 // where created particles go during a pass (see k_particles.hip)
@@ -231,6 +232,9 @@ void launch_fold(hipStream_t st, const double *rho, int N, double coef, double p
                  double phi0Y, double phiLY, double dxdy_inv, double *rhs);
 void launch_phi_bc(hipStream_t st, const double *u, int N, double phi0X, double phiR,
                    double phi0Y, double phiLY, double *phi);
+void launch_phi_efield(hipStream_t st, const double *u, int N, double phi0X, double phiR,
+                       double phi0Y, double phiLY, double dx, double *phi, double *ex64,
+                       double *ey64, float2 *E32);
 void launch_efield(hipStream_t st, const double *phi, int N, double dx, double *ex64, double *ey64,
                    float2 *E32);
 template <typename T>
