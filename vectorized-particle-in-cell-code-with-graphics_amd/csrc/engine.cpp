@@ -108,7 +108,11 @@ int alloc_mg(pic_handle *h, MgLevels &M, int N) {
     size_t nf = (size_t)N * N;
     HIPCHK(h, hipMemset(M.d.u[M.ng - 1], 0, nf * sizeof(double)));  // cold start of the iterate
     M.nparts = (int)std::min<size_t>(1024, (nf + 255) / 256);
-    CHECK(dalloc(h, &M.partials, 2 * (size_t)M.nparts));
+    // block partial sums: the residual pass's blocks, or one per 32 x 32 tile
+    // of the top level (the fused post-smoothing and residual kernel)
+    const size_t tiles = (size_t)((N + 31) / 32) * (size_t)((N + 31) / 32);
+    CHECK(dalloc(h, &M.partials, 2 * std::max((size_t)M.nparts, tiles)));
+    CHECK(dalloc(h, &M.r32alt, nf));
     CHECK(dalloc(h, &M.flags, 4));
     HIPCHK(h, hipMemset(M.flags, 0, 4 * sizeof(int)));
     CHECK(dalloc(h, &M.ualt, nf));
@@ -136,6 +140,8 @@ void free_mg(MgLevels &M) {
     if (M.flags) hipFree(M.flags);
     if (M.ualt) hipFree(M.ualt);
     M.ualt = nullptr;
+    if (M.r32alt) hipFree(M.r32alt);
+    M.r32alt = nullptr;
     if (M.relres) hipHostFree(M.relres);
     if (M.host_flags) hipHostFree(M.host_flags);
     if (M.done) hipEventDestroy(M.done);

@@ -854,6 +854,47 @@ __device__ __forceinline__ double block_sum_d(double v) {
     return t;
 }
 
+// End of a residual pass: this block's sums of r^2 and f^2 go to partials[b];
+// the last of the nb blocks to finish (ticket counter flags[2]) sums the
+// partials in block order, sets flags[0] when ||r|| <= tol ||f||, counts an
+// applied cycle in flags[1] when `applied`, and publishes both and the
+// relative residual to the host-mapped copies.
+__device__ __forceinline__ void tol_finish(double rr, double ff, int b, int nb,
+                                           double *__restrict__ partials, double tol,
+                                           int *__restrict__ flags, bool applied,
+                                           double *__restrict__ relres_host,
+                                           int *__restrict__ host_flags) {
+    rr = block_sum_d(rr);
+    ff = block_sum_d(ff);
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        partials[2 * b] = rr;
+        partials[2 * b + 1] = ff;
+        __threadfence();
+        last = atomicAdd(&flags[2], 1) == nb - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    rr = 0.0;
+    ff = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        rr += partials[2 * k];
+        ff += partials[2 * k + 1];
+    }
+    rr = block_sum_d(rr);
+    ff = block_sum_d(ff);
+    if (threadIdx.x == 0) {
+        const double rel = (ff > 0.0) ? sqrt(rr / ff) : sqrt(rr);
+        if (applied) flags[1] += 1;
+        if (rel <= tol) flags[0] = 1;
+        flags[2] = 0;
+        *relres_host = rel;
+        host_flags[0] = flags[0];
+        host_flags[1] = flags[1];
+    }
+}
+
 // One residual pass of the refinement loop, fused: v = u_in + e (FP64
 // accumulation of the FP32 correction; e null: v = u_in) written to u_out
 // (null: not written), r = f - A v in FP32 for the next correction solve,
@@ -893,41 +934,104 @@ __global__ void __launch_bounds__(TOL_THREADS)
         if (u_out) u_out[k] = v;
         r32[k] = out;
     }
-    rr = block_sum_d(rr);
-    ff = block_sum_d(ff);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = rr;
-        partials[2 * blockIdx.x + 1] = ff;
-        __threadfence();
-        last = atomicAdd(&flags[2], 1) == (int)gridDim.x - 1;
+    tol_finish(rr, ff, (int)blockIdx.x, (int)gridDim.x, partials, tol, flags, e != nullptr,
+               relres_host, host_flags);
+}
+
+// The top level's post leg fused with the refinement's residual pass: the
+// halo takes one more ring (2 nu + 1) so the correction e is final on the
+// tile and its first ring; then, as k_tol_update, v = u_in + e in FP64 is
+// written to u_out, r = f - A v (the same expression and association) to
+// r_out as FP32, and the tile's sums of r^2 and f^2 go to tol_finish.  The
+// iterate and the residual are those of k_mg_post + k_tol_update bit for
+// bit; only the block split of the norm's sum differs.  r_out must not be
+// the rhs this leg reads (the halo reads neighbouring tiles' nodes).
+template <int NU>
+__global__ void __launch_bounds__(256)
+    k_mg_post_update(const float *__restrict__ us, const float *__restrict__ rhs,
+                     const float *__restrict__ cv, int n, float h2,
+                     const double *__restrict__ u_in, double *__restrict__ u_out,
+                     const double *__restrict__ f, double h2i, float *__restrict__ r_out,
+                     double *__restrict__ partials, double tol, int *__restrict__ flags,
+                     double *__restrict__ relres_host, int *__restrict__ host_flags) {
+    SKIP_IF_DONE(flags);
+    constexpr int H = 2 * NU + 1, W = MGT + 2 * H;
+    __shared__ float su[W * W], sf[W * W];
+    const int x0 = blockIdx.x * MGT, y0 = blockIdx.y * MGT, nc = (n + 1) / 2;
+    for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
+        const int ly = k / W, lx = k % W;
+        const int i = y0 - H + ly, j = x0 - H + lx;
+        const bool in = i >= 0 && j >= 0 && i < n && j < n;
+        su[k] = in ? us[(long)i * n + j] + interp_node(cv, nc, i, j) : 0.0f;
+        sf[k] = in ? rhs[(long)i * n + j] : 0.0f;
     }
     __syncthreads();
-    if (!last) return;
-    __threadfence();
-    rr = 0.0;
-    ff = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-        rr += partials[2 * b];
-        ff += partials[2 * b + 1];
+    tile_smooth<NU>(su, sf, W, H, x0, y0, 1, n, h2);  // e valid on the tile + 1 ring
+    double rr = 0.0, ff = 0.0;
+    for (int k = threadIdx.x; k < MGT * MGT; k += blockDim.x) {
+        const int ly = k / MGT, lx = k % MGT, i = y0 + ly, j = x0 + lx;
+        if (i >= n || j >= n) continue;
+        const long g = (long)i * n + j;
+        const int c = (ly + H) * W + lx + H;
+        double v = u_in[g];
+        float out = 0.0f;
+        if (i > 0 && j > 0 && i < n - 1 && j < n - 1) {
+            v += (double)su[c];
+            const double un = u_in[g + n] + (double)su[c + W], us_ = u_in[g - n] + (double)su[c - W];
+            const double ue = u_in[g + 1] + (double)su[c + 1], uw = u_in[g - 1] + (double)su[c - 1];
+            const double r = -h2i * (un + us_ + ue + uw - 4.0 * v) + f[g];
+            rr += r * r;
+            out = (float)r;
+            ff += f[g] * f[g];
+        }
+        u_out[g] = v;
+        r_out[g] = out;
     }
-    rr = block_sum_d(rr);
-    ff = block_sum_d(ff);
-    if (threadIdx.x == 0) {
-        const double rel = (ff > 0.0) ? sqrt(rr / ff) : sqrt(rr);
-        if (e) flags[1] += 1;
-        if (rel <= tol) flags[0] = 1;
-        flags[2] = 0;
-        *relres_host = rel;
-        host_flags[0] = flags[0];
-        host_flags[1] = flags[1];
-    }
+    tol_finish(rr, ff, (int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y),
+               partials, tol, flags, true, relres_host, host_flags);
 }
 
 __global__ void k_tol_reset(int *flags) {
     flags[0] = 0;
     flags[1] = 0;
     flags[2] = 0;
+}
+
+// one refinement cycle above the coarsest level: the V-cycle on the
+// correction equation A e = r_in from e = 0, its top post leg fused with
+// the FP64 update u_out = u_in + e and the new residual r_out
+static void vcycle_top(hipStream_t st, MgLevels &M, int j, const float *r_in, const double *u_in,
+                       double *u_out, const double *f, double h2i, float *r_out, double tol,
+                       double length) {
+    MgArrays<float> &A = M.f;
+    const int *skip = M.flags;
+    const int nf = lvl_size(j);
+    const double h = length / (nf - 1);
+    const int tiles = (nf + MGT - 1) / MGT;
+    const float h2 = (float)(h * h), h2f = (float)(1.0 / (h * h));
+    const dim3 g(tiles, tiles), b(256);
+    const float *z = nullptr;
+    if (M.nu == 3)
+        hipLaunchKernelGGL(k_mg_pre<3>, g, b, 0, st, z, r_in, nf, h2, h2f, A.us[j], A.cres[j - 1], skip);
+    else if (M.nu == 2)
+        hipLaunchKernelGGL(k_mg_pre<2>, g, b, 0, st, z, r_in, nf, h2, h2f, A.us[j], A.cres[j - 1], skip);
+    else
+        hipLaunchKernelGGL(k_mg_pre<1>, g, b, 0, st, z, r_in, nf, h2, h2f, A.us[j], A.cres[j - 1], skip);
+    if (j - 1 == M.lc)
+        coarse_dst_launch(st, M, A.cres[j - 1], A.cv[j - 1], length, skip);
+    else
+        vcycle_fused(st, M, j - 1, A.cv[j - 1], A.cres[j - 1], length, skip);
+#define POST_UPDATE(NU)                                                                           \
+    hipLaunchKernelGGL(k_mg_post_update<NU>, g, b, 0, st, A.us[j], r_in, A.cv[j - 1], nf, h2,     \
+                       u_in, u_out, f, h2i, r_out, M.partials, tol, M.flags, M.relres,           \
+                       M.host_flags)
+    if (M.nu == 3)
+        POST_UPDATE(3);
+    else if (M.nu == 2)
+        POST_UPDATE(2);
+    else
+        POST_UPDATE(1);
+#undef POST_UPDATE
 }
 
 // Wait for the queued cycles: the solve's host check sits between two phases
@@ -951,18 +1055,32 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     const double *f = M.d.rhsF[lf];
     float *r32 = M.f.rhsF[lf], *e32 = M.f.u[lf];
     const int *skip = M.flags;
-    // cycle c reads the iterate from buf[c % 2] and writes buf[(c + 1) % 2]
+    // cycle c reads the iterate from buf[c % 2] and writes buf[(c + 1) % 2];
+    // above the coarsest level the residual alternates between rb[0], rb[1]
     double *buf[2] = {u, M.ualt};
+    float *rb[2] = {r32, M.r32alt};
     int queued = 0;
+    // the top post leg fused with the residual pass saves a launch per cycle
+    // where the levels are latency-bound; at 1025^2 the wider halo's extra
+    // reads cost more than the launch (measured), so the passes stay apart
+    const bool fuse_top = lf > M.lc && n <= 513;
     auto cycle = [&]() {
-        if (lf == M.lc)
-            coarse_dst_launch(st, M, r32, e32, length, skip);
-        else
+        if (lf > M.lc && !fuse_top) {
             vcycle_fused(st, M, lf, e32, r32, length, skip);
-        hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st,
-                           (const double *)buf[queued & 1], (const float *)e32,
-                           buf[(queued + 1) & 1], f, n, h2i, r32, M.partials, tol, M.flags,
-                           M.relres, M.host_flags);
+            hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st,
+                               (const double *)buf[queued & 1], (const float *)e32,
+                               buf[(queued + 1) & 1], f, n, h2i, r32, M.partials, tol, M.flags,
+                               M.relres, M.host_flags);
+        } else if (lf == M.lc) {
+            coarse_dst_launch(st, M, r32, e32, length, skip);
+            hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st,
+                               (const double *)buf[queued & 1], (const float *)e32,
+                               buf[(queued + 1) & 1], f, n, h2i, r32, M.partials, tol, M.flags,
+                               M.relres, M.host_flags);
+        } else {
+            vcycle_top(st, M, lf, rb[queued & 1], buf[queued & 1], buf[(queued + 1) & 1], f, h2i,
+                       rb[(queued + 1) & 1], tol, length);
+        }
     };
     hipLaunchKernelGGL(k_tol_reset, dim3(1), dim3(1), 0, st, M.flags);
     hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st, (const double *)u,

@@ -76,6 +76,7 @@ struct MgLevels {
     int *flags = nullptr;        // [2]: blocks done in the current residual pass
     double *partials = nullptr;  // per-block sums of r^2 and f^2
     double *ualt = nullptr;      // second FP64 iterate: cycles alternate u <-> ualt
+    float *r32alt = nullptr;     // second FP32 residual: cycles alternate with f.rhsF[top]
     int nparts = 0;
     double *relres = nullptr;    // pinned host: last relative residual
     int *host_flags = nullptr;   // pinned host copy of flags
