@@ -480,22 +480,23 @@ __global__ void __launch_bounds__(1024)
 // fp32).  It replaces the coarse V-cycle of the tolerance mode's correction
 // solve; the FP64 refinement around it is unchanged.
 
-// C = A B for M x M row-major matrices in LDS (M a multiple of 4, M <= 64);
-// thread t < (M/4)^2 computes a 4x4 tile: per k four broadcast reads of A
-// and one float4 of B feed 16 FMAs.  Ends with a barrier.
-__device__ void lds_gemm(const float *A, const float *B, float *C, int M) {
+// C = A B for M x M matrices in LDS, rows ld floats apart (M, ld multiples
+// of 4, M <= 64): thread t < (M/4)^2 computes a 4x4 tile, per k four
+// broadcast reads of A and one float4 of B feeding 16 FMAs.  Ends with a
+// barrier.
+__device__ void lds_gemm(const float *A, const float *B, float *C, int M, int ld) {
     const int T = M >> 2;
     const int t = threadIdx.x;
     if (t < T * T) {
         const int ti = t / T, tj = t - (t / T) * T;
-        const float *a = A + (4 * ti) * M;
+        const float *a = A + (4 * ti) * ld;
         float c[4][4] = {};
         for (int k = 0; k < M; k += 4) {  // four k per step: float4 reads of A and B
             float4 ar[4], br[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                ar[r] = *reinterpret_cast<const float4 *>(a + r * M + k);
-                br[r] = *reinterpret_cast<const float4 *>(B + (k + r) * M + 4 * tj);
+                ar[r] = *reinterpret_cast<const float4 *>(a + r * ld + k);
+                br[r] = *reinterpret_cast<const float4 *>(B + (k + r) * ld + 4 * tj);
             }
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) {
@@ -512,35 +513,85 @@ __device__ void lds_gemm(const float *A, const float *B, float *C, int M) {
         }
 #pragma unroll
         for (int r = 0; r < 4; r++)
-            *reinterpret_cast<float4 *>(C + (4 * ti + r) * M + 4 * tj) =
+            *reinterpret_cast<float4 *>(C + (4 * ti + r) * ld + 4 * tj) =
                 make_float4(c[r][0], c[r][1], c[r][2], c[r][3]);
     }
     __syncthreads();
 }
 
-// the DST solve by one workgroup; lds_raw: 3 M^2 floats
+// The same product on the matrix cores (M a multiple of 16): one 16 x 16
+// block of C per wavefront, v_mfma_f32_16x16x4f32 over k.  Lane layout of
+// the instruction: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k,
+// C[i][j] in lane j + 16 (i / 4), register i % 4.  A row stride ld = M + 4
+// keeps the 16 rows of an A fragment on 8 banks (2-way at most).  Ends with
+// a barrier.
+typedef float mf4 __attribute__((ext_vector_type(4)));
+__device__ void lds_gemm_mfma(const float *A, const float *B, float *C, int M, int ld) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int nb = M >> 4, li = lane & 15, lk = lane >> 4;
+    for (int blk = w; blk < nb * nb; blk += nw) {
+        const int bi = blk / nb, bj = blk - (blk / nb) * nb;
+        const float *a = A + (16 * bi + li) * ld + lk;
+        const float *b = B + lk * ld + 16 * bj + li;
+        mf4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < M; k += 4)
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[k], b[k * ld], c, 0, 0, 0);
+        float *o = C + (16 * bi + 4 * lk) * ld + 16 * bj + li;
+        o[0] = c[0];
+        o[ld] = c[1];
+        o[2 * ld] = c[2];
+        o[3 * ld] = c[3];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void dst_gemm(const float *A, const float *B, float *C, int M, int ld) {
+    if ((M & 15) == 0)
+        lds_gemm_mfma(A, B, C, M, ld);
+    else
+        lds_gemm(A, B, C, M, ld);
+}
+
+// LDS floats of the DST solve: three M x M matrices with row stride M + 4
+__host__ __device__ inline long dst_lds_floats(int M) { return 3L * M * (M + 4); }
+
+// the DST solve by one workgroup of 1024 threads; lds_raw: dst_lds_floats(M)
 __device__ void dst_solve(const float *__restrict__ rhs_in, float *__restrict__ u_out, int lc,
                           float h2, const float *__restrict__ Sg, const float *__restrict__ Dg,
                           double *lds_raw) {
-    const int M = 1 << (lc + 1), n = M + 1;
+    const int M = 1 << (lc + 1), n = M + 1, ld = M + 4;
     float *S = reinterpret_cast<float *>(lds_raw);
-    float *X = S + M * M, *Y = X + M * M;
+    float *X = S + M * ld, *Y = X + M * ld;
+    // the eigenvalue scale, read now so its latency overlaps the products
+    float dk[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int k = threadIdx.x + r * blockDim.x;
+        dk[r] = k < M * M ? Dg[k] * h2 : 0.0f;
+    }
     for (int k = threadIdx.x; k < M * M; k += blockDim.x) {
         const int p = k >> (lc + 1), q = k & (M - 1);
-        S[k] = Sg[k];
-        X[k] = (p < M - 1 && q < M - 1) ? rhs_in[(p + 1) * n + q + 1] : 0.0f;
+        S[p * ld + q] = Sg[k];
+        X[p * ld + q] = (p < M - 1 && q < M - 1) ? rhs_in[(p + 1) * n + q + 1] : 0.0f;
     }
     __syncthreads();
-    lds_gemm(S, X, Y, M);  // Y = S f
-    lds_gemm(Y, S, X, M);  // X = S f S
-    for (int k = threadIdx.x; k < M * M; k += blockDim.x) X[k] = X[k] * (Dg[k] * h2);
+    dst_gemm(S, X, Y, M, ld);  // Y = S f
+    dst_gemm(Y, S, X, M, ld);  // X = S f S
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int k = threadIdx.x + r * blockDim.x;
+        if (k < M * M) {
+            const int p = k >> (lc + 1), q = k & (M - 1);
+            X[p * ld + q] = X[p * ld + q] * dk[r];
+        }
+    }
     __syncthreads();
-    lds_gemm(S, X, Y, M);
-    lds_gemm(Y, S, X, M);  // X = u on the interior
+    dst_gemm(S, X, Y, M, ld);
+    dst_gemm(Y, S, X, M, ld);  // X = u on the interior
     for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
         const int i = k / n, j = k - (k / n) * n;
         const bool in = i > 0 && j > 0 && i < n - 1 && j < n - 1;
-        u_out[k] = in ? X[(i - 1) * M + (j - 1)] : 0.0f;
+        u_out[k] = in ? X[(i - 1) * ld + (j - 1)] : 0.0f;
     }
 }
 
@@ -723,8 +774,8 @@ static void coarse_dst_launch(hipStream_t st, const MgLevels &M, const float *rh
                               double length, const int *skip) {
     const int lc = M.lc, m = 1 << (lc + 1);
     double h = length / (double)m;
-    hipLaunchKernelGGL(k_mg_coarse_dst, dim3(1), dim3(1024), 3ull * m * m * sizeof(float), st, rhs,
-                       u, lc, (float)(h * h), M.dst_S, M.dst_D, skip);
+    hipLaunchKernelGGL(k_mg_coarse_dst, dim3(1), dim3(1024), (size_t)dst_lds_floats(m) * sizeof(float),
+                       st, rhs, u, lc, (float)(h * h), M.dst_S, M.dst_D, skip);
 }
 
 int mg_dst_tables(MgLevels &M) {
