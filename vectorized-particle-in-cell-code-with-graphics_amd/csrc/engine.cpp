@@ -441,6 +441,16 @@ void commit(pic_handle *h, int src, int dst_s, int deposit) {
                   d.cap, &h->cnt[dst_s], deposit, h->N, d.pc.mode, d.q);
 }
 
+// commit(h, 1, 1, dep_i) and commit(h, 0, 0, dep_e) in one launch
+void commit_both(pic_handle *h, int dep_i, int dep_e) {
+    auto args = [&](int s, int dep) {
+        Species &d = h->sp[s];
+        return CommitArgs{d.stage, &h->cnt[s], d.buf[d.cur], &h->cnt[s].n[d.cur], (uint32_t)d.cap,
+                          &h->cnt[s], dep, d.pc.mode, d.q};
+    };
+    launch_commit_pair(h->st, args(1, dep_i), args(0, dep_e), h->N);
+}
+
 void update_scale(pic_handle *h, Species &s) {
     s.scale = ldexp(s.qc / (h->dx * h->dx) * s.weight, -40);
 }
@@ -675,8 +685,7 @@ int fused_step(pic_handle *h) {
         // ionisation products: new ions join this step's ion pass (they were
         // appended before the ion push in the reference, pic.cpp:1071-1111);
         // new electrons were pushed with their parents and are deposited here
-        commit(h, 1, 1, 0);
-        commit(h, 0, 0, 1);
+        commit_both(h, 0, 1);
     }
     if (h->step % h->P.dt_dti == 0) {
         if (h->refcx) CHECK(ref_ion_cx(h));
@@ -688,8 +697,7 @@ int fused_step(pic_handle *h) {
     }
     if (h->spawning && (plop_on(h) || h->wall_on)) {
         // pairs re-injected for ions lost this step; ion-impact secondaries
-        commit(h, 1, 1, 1);
-        commit(h, 0, 0, 1);
+        commit_both(h, 1, 1);
     }
     if (ref_steady(h)) steady_boundary(h);
     // pic.cpp:1130-1141: electrons, then ions, before the charge is used
@@ -1024,8 +1032,7 @@ int pic_run(pic_handle *h) {
         wall_pass(h, 0);
     }
     if (h->spawning && (plop_on(h) || h->wall_on)) {
-        commit(h, 1, 1, 0);
-        commit(h, 0, 0, 0);
+        commit_both(h, 0, 0);
     }
     CHECK(diag_prepare_time(h, (int32_t)std::min<int64_t>(h->step + h->run_time, INT32_MAX)));
     for (int32_t t = 0; t < h->run_time; t++) CHECK(fused_step(h));
@@ -1117,8 +1124,7 @@ int pic_e_collisions(pic_handle *h) {
     launch_mcc(h->st, s.buf[s.cur], &h->cnt[0].n[s.cur], s.cap, s.pc, s.mc, (uint32_t)h->step,
                make_spawn(h, false, -1), &h->cnt[0].events[0][0]);
     if (s.mc.mode == PIC_DEV_MODE_PHYS) {  // e_collisions.cpp:405-441 appends the products
-        commit(h, 1, 1, 0);
-        commit(h, 0, 0, 0);
+        commit_both(h, 0, 0);
     }
     return PIC_OK;
 }
@@ -1170,8 +1176,7 @@ int pic_boundary(pic_handle *h, int32_t which) {
     absorb_species(h, si, plop);
     wall_pass(h, si);
     if (plop || h->wall_on) {
-        commit(h, 1, 1, 0);
-        commit(h, 0, 0, 0);
+        commit_both(h, 0, 0);
     }
     return PIC_OK;
 }

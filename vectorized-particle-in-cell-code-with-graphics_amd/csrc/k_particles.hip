@@ -1126,9 +1126,45 @@ __global__ void __launch_bounds__(256)
 // Sort (key, val) pairs [0, c) ascending by key in one workgroup: bitonic
 // network over the next power of two, padded with ~0 keys (the arrays hold
 // at least that many entries: the staging capacity is a power of two).
+#define SORT_LDS 4096  // pairs sorted in LDS (48 KB); larger batches sort in place
 __device__ void block_sort_pairs(unsigned long long *key, uint32_t *val, uint32_t c) {
     uint32_t P = 1;
     while (P < c) P <<= 1;
+    if (P <= SORT_LDS) {
+        // the same network on an LDS copy: no global round trip per stage
+        __shared__ unsigned long long sk[SORT_LDS];
+        __shared__ uint32_t sv[SORT_LDS];
+        for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+            sk[t] = t < c ? key[t] : ~0ull;
+            sv[t] = t < c ? val[t] : 0u;
+        }
+        __syncthreads();
+        for (uint32_t k = 2; k <= P; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+                    uint32_t o = t ^ j;
+                    if (o > t) {
+                        unsigned long long a = sk[t], b = sk[o];
+                        bool up = (t & k) == 0;
+                        if ((a > b) == up) {
+                            sk[t] = b;
+                            sk[o] = a;
+                            uint32_t x = sv[t];
+                            sv[t] = sv[o];
+                            sv[o] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
+            key[t] = sk[t];
+            val[t] = sv[t];
+        }
+        __syncthreads();
+        return;
+    }
     for (uint32_t t = c + threadIdx.x; t < P; t += blockDim.x) key[t] = ~0ull;
     __syncthreads();
     for (uint32_t k = 2; k <= P; k <<= 1) {
@@ -1156,10 +1192,10 @@ __device__ void block_sort_pairs(unsigned long long *key, uint32_t *val, uint32_
 // in one workgroup, then append in key order to dst with consecutive new
 // ids, optionally depositing their charge.  Sorting makes the result
 // independent of the order in which the staging atomics fired.
-__global__ void __launch_bounds__(1024)
-    k_commit(StageView sv, SpCounters *__restrict__ scnt, SpView dst, uint32_t *__restrict__ n_dst,
-             uint32_t cap_dst, SpCounters *__restrict__ dcnt, int deposit, int N, int mode,
-             unsigned long long *__restrict__ q) {
+__device__ __forceinline__ void commit_body(StageView sv, SpCounters *__restrict__ scnt, SpView dst,
+                                            uint32_t *__restrict__ n_dst, uint32_t cap_dst,
+                                            SpCounters *__restrict__ dcnt, int deposit, int N,
+                                            int mode, unsigned long long *__restrict__ q) {
     __shared__ uint32_t s_c, s_base, s_nid;
     if (threadIdx.x == 0) {
         uint32_t c = scnt->stage_count;
@@ -1221,6 +1257,20 @@ __global__ void __launch_bounds__(1024)
         if (dropped) dcnt->overflow += dropped;
         scnt->stage_count = 0;
     }
+}
+
+__global__ void __launch_bounds__(1024)
+    k_commit(StageView sv, SpCounters *__restrict__ scnt, SpView dst, uint32_t *__restrict__ n_dst,
+             uint32_t cap_dst, SpCounters *__restrict__ dcnt, int deposit, int N, int mode,
+             unsigned long long *__restrict__ q) {
+    commit_body(sv, scnt, dst, n_dst, cap_dst, dcnt, deposit, N, mode, q);
+}
+
+// two independent commits (different species: disjoint staging, stores,
+// counters and charge), workgroup b takes the b-th
+__global__ void __launch_bounds__(1024) k_commit_pair(CommitArgs a, CommitArgs b, int N) {
+    const CommitArgs &c = blockIdx.x == 0 ? a : b;
+    commit_body(c.sv, c.scnt, c.dst, c.n_dst, c.cap_dst, c.dcnt, c.deposit, N, c.mode, c.q);
 }
 
 // ------------------------------------------------------------ loader
@@ -1558,6 +1608,10 @@ void launch_mcc(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap
                 unsigned long long *events) {
     hipLaunchKernelGGL(k_mcc, dim3(blocks_for(cap, 256)), dim3(256), 0, st, sp, n, pc, mc, step,
                        spn, events);
+}
+
+void launch_commit_pair(hipStream_t st, const CommitArgs &a, const CommitArgs &b, int N) {
+    hipLaunchKernelGGL(k_commit_pair, dim3(2), dim3(1024), 0, st, a, b, N);
 }
 
 void launch_commit(hipStream_t st, const StageView &sv, SpCounters *scnt, const SpView &dst,

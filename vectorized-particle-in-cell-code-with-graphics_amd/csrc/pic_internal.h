@@ -119,6 +119,19 @@ void launch_push_inplace(hipStream_t st, const SpView &sp, const uint32_t *n, in
 void launch_mcc(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
                 const PushConst &pc, const mcc::MccConst &mc, uint32_t step, const Spawn &spn,
                 unsigned long long *events);
+// one species' commit of its staging buffer (k_commit)
+struct CommitArgs {
+    StageView sv;
+    SpCounters *scnt;
+    SpView dst;
+    uint32_t *n_dst;
+    uint32_t cap_dst;
+    SpCounters *dcnt;
+    int deposit, mode;
+    unsigned long long *q;
+};
+// both species' commits in one launch (one workgroup each)
+void launch_commit_pair(hipStream_t st, const CommitArgs &a, const CommitArgs &b, int N);
 void launch_commit(hipStream_t st, const StageView &sv, SpCounters *scnt, const SpView &dst,
                    uint32_t *n_dst, int64_t cap_dst, SpCounters *dcnt, int deposit, int N, int mode,
                    unsigned long long *q);
