@@ -113,6 +113,7 @@ int alloc_mg(pic_handle *h, MgLevels &M, int N) {
     const size_t tiles = (size_t)((N + 31) / 32) * (size_t)((N + 31) / 32);
     CHECK(dalloc(h, &M.partials, 2 * std::max((size_t)M.nparts, tiles)));
     CHECK(dalloc(h, &M.r32alt, nf));
+    CHECK(dalloc(h, &M.uprev, nf));
     CHECK(dalloc(h, &M.flags, 4));
     HIPCHK(h, hipMemset(M.flags, 0, 4 * sizeof(int)));
     CHECK(dalloc(h, &M.ualt, nf));
@@ -142,6 +143,9 @@ void free_mg(MgLevels &M) {
     M.ualt = nullptr;
     if (M.r32alt) hipFree(M.r32alt);
     M.r32alt = nullptr;
+    if (M.uprev) hipFree(M.uprev);
+    M.uprev = nullptr;
+    M.have_prev = false;
     if (M.relres) hipHostFree(M.relres);
     if (M.host_flags) hipHostFree(M.host_flags);
     if (M.done) hipEventDestroy(M.done);
@@ -632,15 +636,17 @@ void absorb_species(pic_handle *h, int si, bool plop) {
 }
 
 // phi from the solve with the boundary values; with_e: E (and its fp32 copy
-// for the gather) from the same values in the same launch
+// for the gather) from the same values in the same launch; timestep: the
+// tolerance solve starts from the linear extrapolation of the last two steps
 int solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, double phiLY,
-                    bool with_e = false) {
+                    bool with_e = false, bool timestep = false) {
     const pic_params &P = h->P;
     int N = h->N;
     MgLevels &M = h->M;
     launch_fold(h->st, h->rho, N, 1.0 / P.epsilon, phi0X, phiR, phi0Y, phiLY,
                 1.0 / (h->dx * h->dx), M.d.rhsF[M.ng - 1]);
     if (P.mg_mode == PIC_MG_TOL) {
+        if (timestep) mg_extrapolate_start(h->st, M);
         int used = mg_tol_solve(h->st, M, P.length, P.mg_tol, P.mg_max_iter, &h->mg_relres);
         if (used < 0) return seterr(h, PIC_ERR_DEVICE, "multigrid: stream error");
         h->mg_cycles = used;
@@ -664,7 +670,7 @@ int field_step(pic_handle *h, int64_t step) {
     int p = prof_begin(h, 3);
     launch_rho(h->st, h->sp[1].q, h->sp[0].q, h->N, h->sp[1].scale, h->sp[0].scale,
                P.corner8 ? 8.0 : 4.0, h->rho);
-    CHECK(solve_potential(h, P.phi_left, phiR, P.phi_bottom, P.phi_top, true));
+    CHECK(solve_potential(h, P.phi_left, phiR, P.phi_bottom, P.phi_top, true, true));
     prof_end(h, p);
     return PIC_OK;
 }

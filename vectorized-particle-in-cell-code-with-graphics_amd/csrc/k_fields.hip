@@ -1085,6 +1085,25 @@ static void vcycle_top(hipStream_t st, MgLevels &M, int j, const float *r_in, co
 #undef POST_UPDATE
 }
 
+// u <- 2 u - uprev, uprev <- u (the old u); first: uprev <- u only
+__global__ void k_extrapolate(double *__restrict__ u, double *__restrict__ uprev, long tot,
+                              int first) {
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < tot;
+         k += (long)gridDim.x * blockDim.x) {
+        const double t = u[k];
+        if (!first) u[k] = 2.0 * t - uprev[k];
+        uprev[k] = t;
+    }
+}
+
+void mg_extrapolate_start(hipStream_t st, MgLevels &M) {
+    const int lf = M.ng - 1, n = lvl_size(lf);
+    const long tot = (long)n * n;
+    hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)std::min<long>((tot + 255) / 256, 1024)),
+                       dim3(256), 0, st, M.d.u[lf], M.uprev, tot, M.have_prev ? 0 : 1);
+    M.have_prev = true;
+}
+
 // Wait for the queued cycles: the solve's host check sits between two phases
 // of the step, so the wake-up latency of a blocking wait is idle GPU time;
 // poll the batch's event for up to 2 ms, then block (also reports errors).

@@ -77,6 +77,8 @@ struct MgLevels {
     double *partials = nullptr;  // per-block sums of r^2 and f^2
     double *ualt = nullptr;      // second FP64 iterate: cycles alternate u <-> ualt
     float *r32alt = nullptr;     // second FP32 residual: cycles alternate with f.rhsF[top]
+    double *uprev = nullptr;     // the previous step's solution (extrapolated start)
+    bool have_prev = false;      // uprev holds a solution
     int nparts = 0;
     double *relres = nullptr;    // pinned host: last relative residual
     int *host_flags = nullptr;   // pinned host copy of flags
@@ -259,6 +261,9 @@ void mg_set_coarse_attr(long lds_bytes);
 void mg_fixed_solve(hipStream_t st, MgLevels &M, int ncycle, double length);
 // residual-controlled solve of A u = M.d.rhsF[ng-1] into M.d.u[ng-1] (warm
 // started from its current content); returns V-cycles used, -1 on HIP error
+// start the next timestep's solve from 2 u_n - u_(n-1) (the first call only
+// records u_n)
+void mg_extrapolate_start(hipStream_t st, MgLevels &M);
 int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max_iter,
                  double *relres_out);
 
