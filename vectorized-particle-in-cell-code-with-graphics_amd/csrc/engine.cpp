@@ -108,9 +108,10 @@ int alloc_mg(pic_handle *h, MgLevels &M, int N) {
     size_t nf = (size_t)N * N;
     HIPCHK(h, hipMemset(M.d.u[M.ng - 1], 0, nf * sizeof(double)));  // cold start of the iterate
     M.nparts = (int)std::min<size_t>(1024, (nf + 255) / 256);
-    // block partial sums: the residual pass's blocks, or one per 32 x 32 tile
-    // of the top level (the fused post-smoothing and residual kernel)
-    const size_t tiles = (size_t)((N + 31) / 32) * (size_t)((N + 31) / 32);
+    // block partial sums: the residual pass's blocks, or one per tile of the
+    // top level (the fused post-smoothing and residual kernel; tiles of 16 x
+    // 16 or larger)
+    const size_t tiles = (size_t)((N + 15) / 16) * (size_t)((N + 15) / 16);
     CHECK(dalloc(h, &M.partials, 2 * std::max((size_t)M.nparts, tiles)));
     CHECK(dalloc(h, &M.r32alt, nf));
     CHECK(dalloc(h, &M.uprev, nf));

@@ -619,18 +619,21 @@ __global__ void __launch_bounds__(1024)
 // bound by its loads and launch, not by the LDS sweeps) and cut the cycles a
 // solve needs: the measured two-grid factor is 0.13 for V(1,1), 0.03 for
 // V(2,2) (257^2, exact 65^2 coarse level).
-#define MGT 32
-template <int NU> struct MgHalo {
-    static constexpr int PRE = 2 * NU + 2, WPRE = MGT + 2 * PRE;
-    static constexpr int POST = 2 * NU, WPOST = MGT + 2 * POST;
+// The tile edge T is 16 for levels up to 257^2, where the legs are bound by
+// latency and more workgroups finish sooner despite the wider relative halo,
+// and 32 above, where the halo's extra reads matter (mg_tile).
+template <int NU, int T> struct MgHalo {
+    static constexpr int PRE = 2 * NU + 2, WPRE = T + 2 * PRE;
+    static constexpr int POST = 2 * NU, WPOST = T + 2 * POST;
 };
 
 // Gauss-Seidel sweep of one colour over the tile plus `ring` halo rings
 // (interior nodes only).  s, f: W x W LDS tiles whose (0,0) is global
 // (y0 - H, x0 - H).
+template <int T>
 __device__ __forceinline__ void tile_sweep(float *s, const float *f, int W, int H, int x0, int y0,
                                            int ring, int n, float h2, int color) {
-    const int lo = H - ring, w = MGT + 2 * ring;
+    const int lo = H - ring, w = T + 2 * ring;
     for (int k = threadIdx.x; k < w * w; k += blockDim.x) {
         const int ly = lo + k / w, lx = lo + k % w;
         const int i = y0 - H + ly, j = x0 - H + lx;
@@ -641,15 +644,15 @@ __device__ __forceinline__ void tile_sweep(float *s, const float *f, int W, int 
 }
 
 // nu red-black pairs; the valid region ends `last` rings outside the tile
-template <int NU>
+template <int NU, int T>
 __device__ __forceinline__ void tile_smooth(float *s, const float *f, int W, int H, int x0, int y0,
                                             int last, int n, float h2) {
 #pragma unroll
     for (int k = 0; k < NU; k++) {
         const int ring = last + 2 * (NU - k) - 1;
-        tile_sweep(s, f, W, H, x0, y0, ring, n, h2, 0);
+        tile_sweep<T>(s, f, W, H, x0, y0, ring, n, h2, 0);
         __syncthreads();
-        tile_sweep(s, f, W, H, x0, y0, ring - 1, n, h2, 1);
+        tile_sweep<T>(s, f, W, H, x0, y0, ring - 1, n, h2, 1);
         __syncthreads();
     }
 }
@@ -658,14 +661,14 @@ __device__ __forceinline__ float tile_resid(const float *s, const float *f, int 
     return -h2i * (s[c + W] + s[c - W] + s[c + 1] + s[c - 1] - 4.0f * s[c]) + f[c];
 }
 
-// pre leg of one 32 x 32 tile (tx, ty); su, sf: WPRE^2 LDS floats each
-template <int NU>
+// pre leg of one T x T tile (tx, ty); su, sf: WPRE^2 LDS floats each
+template <int NU, int T>
 __device__ __forceinline__ void mg_pre_tile(int tx, int ty, const float *__restrict__ u_in,
                                             const float *__restrict__ rhs, int n, float h2,
                                             float h2i, float *__restrict__ u_out,
                                             float *__restrict__ cres, float *su, float *sf) {
-    constexpr int H = MgHalo<NU>::PRE, W = MgHalo<NU>::WPRE;
-    const int x0 = tx * MGT, y0 = ty * MGT;
+    constexpr int H = MgHalo<NU, T>::PRE, W = MgHalo<NU, T>::WPRE;
+    const int x0 = tx * T, y0 = ty * T;
     for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
         const int ly = k / W, lx = k % W;
         const int i = y0 - H + ly, j = x0 - H + lx;
@@ -674,14 +677,14 @@ __device__ __forceinline__ void mg_pre_tile(int tx, int ty, const float *__restr
         sf[k] = in ? rhs[(long)i * n + j] : 0.0f;
     }
     __syncthreads();
-    tile_smooth<NU>(su, sf, W, H, x0, y0, 2, n, h2);  // valid on the tile + 2 rings
-    for (int k = threadIdx.x; k < MGT * MGT; k += blockDim.x) {
-        const int ly = k / MGT, lx = k % MGT, i = y0 + ly, j = x0 + lx;
+    tile_smooth<NU, T>(su, sf, W, H, x0, y0, 2, n, h2);  // valid on the tile + 2 rings
+    for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
+        const int ly = k / T, lx = k % T, i = y0 + ly, j = x0 + lx;
         if (i < n && j < n) u_out[(long)i * n + j] = su[(ly + H) * W + lx + H];
     }
     const int nc = (n + 1) / 2;
-    for (int k = threadIdx.x; k < (MGT / 2) * (MGT / 2); k += blockDim.x) {
-        const int ic = y0 / 2 + k / (MGT / 2), jc = x0 / 2 + k % (MGT / 2);
+    for (int k = threadIdx.x; k < (T / 2) * (T / 2); k += blockDim.x) {
+        const int ic = y0 / 2 + k / (T / 2), jc = x0 / 2 + k % (T / 2);
         if (ic >= nc || jc >= nc) continue;
         float v = 0.0f;
         if (ic > 0 && jc > 0 && ic < nc - 1 && jc < nc - 1) {
@@ -694,25 +697,25 @@ __device__ __forceinline__ void mg_pre_tile(int tx, int ty, const float *__restr
     }
 }
 
-template <int NU>
+template <int NU, int T>
 __global__ void __launch_bounds__(256)
     k_mg_pre(const float *__restrict__ u_in, const float *__restrict__ rhs, int n, float h2,
              float h2i, float *__restrict__ u_out, float *__restrict__ cres,
              const int *__restrict__ skip) {
     SKIP_IF_DONE(skip);
-    constexpr int W = MgHalo<NU>::WPRE;
+    constexpr int W = MgHalo<NU, T>::WPRE;
     __shared__ float su[W * W], sf[W * W];
-    mg_pre_tile<NU>(blockIdx.x, blockIdx.y, u_in, rhs, n, h2, h2i, u_out, cres, su, sf);
+    mg_pre_tile<NU, T>(blockIdx.x, blockIdx.y, u_in, rhs, n, h2, h2i, u_out, cres, su, sf);
 }
 
 // post leg of one tile; su, sf: WPOST^2 LDS floats each
-template <int NU>
+template <int NU, int T>
 __device__ __forceinline__ void mg_post_tile(int tx, int ty, const float *__restrict__ us,
                                              const float *__restrict__ rhs,
                                              const float *__restrict__ cv, int n, float h2,
                                              float *__restrict__ u_out, float *su, float *sf) {
-    constexpr int H = MgHalo<NU>::POST, W = MgHalo<NU>::WPOST;
-    const int x0 = tx * MGT, y0 = ty * MGT, nc = (n + 1) / 2;
+    constexpr int H = MgHalo<NU, T>::POST, W = MgHalo<NU, T>::WPOST;
+    const int x0 = tx * T, y0 = ty * T, nc = (n + 1) / 2;
     for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
         const int ly = k / W, lx = k % W;
         const int i = y0 - H + ly, j = x0 - H + lx;
@@ -721,53 +724,65 @@ __device__ __forceinline__ void mg_post_tile(int tx, int ty, const float *__rest
         sf[k] = in ? rhs[(long)i * n + j] : 0.0f;
     }
     __syncthreads();
-    tile_smooth<NU>(su, sf, W, H, x0, y0, 0, n, h2);  // valid on the tile
-    for (int k = threadIdx.x; k < MGT * MGT; k += blockDim.x) {
-        const int ly = k / MGT, lx = k % MGT, i = y0 + ly, j = x0 + lx;
+    tile_smooth<NU, T>(su, sf, W, H, x0, y0, 0, n, h2);  // valid on the tile
+    for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
+        const int ly = k / T, lx = k % T, i = y0 + ly, j = x0 + lx;
         if (i < n && j < n) u_out[(long)i * n + j] = su[(ly + H) * W + lx + H];
     }
 }
 
-template <int NU>
+template <int NU, int T>
 __global__ void __launch_bounds__(256)
     k_mg_post(const float *__restrict__ us, const float *__restrict__ rhs,
               const float *__restrict__ cv, int n, float h2, float *__restrict__ u_out,
               const int *__restrict__ skip) {
     SKIP_IF_DONE(skip);
-    constexpr int W = MgHalo<NU>::WPOST;
+    constexpr int W = MgHalo<NU, T>::WPOST;
     __shared__ float su[W * W], sf[W * W];
-    mg_post_tile<NU>(blockIdx.x, blockIdx.y, us, rhs, cv, n, h2, u_out, su, sf);
+    mg_post_tile<NU, T>(blockIdx.x, blockIdx.y, us, rhs, cv, n, h2, u_out, su, sf);
 }
 
 static void coarse_dst_launch(hipStream_t st, const MgLevels &M, const float *rhs, float *u,
                               double length, const int *skip);
 
-// fused V(1,1) at level j > lc on (Uj, Rj); Uj's initial guess is zero
+static inline int mg_tile(int nf) { return nf <= 257 ? 16 : 32; }
+// LAUNCH(nu, T) for the run-time sweep count nu (1..3) and tile edge T (16, 32)
+#define MG_DISPATCH(NUV, TV, LAUNCH)                  \
+    do {                                              \
+        if ((TV) == 16) {                             \
+            if ((NUV) == 3) LAUNCH(3, 16);            \
+            else if ((NUV) == 2) LAUNCH(2, 16);       \
+            else LAUNCH(1, 16);                       \
+        } else {                                      \
+            if ((NUV) == 3) LAUNCH(3, 32);            \
+            else if ((NUV) == 2) LAUNCH(2, 32);       \
+            else LAUNCH(1, 32);                       \
+        }                                             \
+    } while (0)
+
+// fused V(nu,nu) at level j > lc on (Uj, Rj); Uj's initial guess is zero
 static void vcycle_fused(hipStream_t st, MgLevels &M, int j, float *Uj, const float *Rj,
                          double length, const int *skip) {
     MgArrays<float> &A = M.f;
     const int nf = lvl_size(j);
     const double h = length / (nf - 1);
-    const int tiles = (nf + MGT - 1) / MGT;
+    const int T = mg_tile(nf), tiles = (nf + T - 1) / T;
     const float h2 = (float)(h * h), h2i = (float)(1.0 / (h * h));
     const dim3 g(tiles, tiles), b(256);
     const float *z = nullptr;
-    if (M.nu == 3)
-        hipLaunchKernelGGL(k_mg_pre<3>, g, b, 0, st, z, Rj, nf, h2, h2i, A.us[j], A.cres[j - 1], skip);
-    else if (M.nu == 2)
-        hipLaunchKernelGGL(k_mg_pre<2>, g, b, 0, st, z, Rj, nf, h2, h2i, A.us[j], A.cres[j - 1], skip);
-    else
-        hipLaunchKernelGGL(k_mg_pre<1>, g, b, 0, st, z, Rj, nf, h2, h2i, A.us[j], A.cres[j - 1], skip);
+#define PRE(NU, TT)                                                                          \
+    hipLaunchKernelGGL((k_mg_pre<NU, TT>), g, b, 0, st, z, Rj, nf, h2, h2i, A.us[j], A.cres[j - 1], \
+                       skip)
+    MG_DISPATCH(M.nu, T, PRE);
+#undef PRE
     if (j - 1 == M.lc)
         coarse_dst_launch(st, M, A.cres[j - 1], A.cv[j - 1], length, skip);
     else
         vcycle_fused(st, M, j - 1, A.cv[j - 1], A.cres[j - 1], length, skip);
-    if (M.nu == 3)
-        hipLaunchKernelGGL(k_mg_post<3>, g, b, 0, st, A.us[j], Rj, A.cv[j - 1], nf, h2, Uj, skip);
-    else if (M.nu == 2)
-        hipLaunchKernelGGL(k_mg_post<2>, g, b, 0, st, A.us[j], Rj, A.cv[j - 1], nf, h2, Uj, skip);
-    else
-        hipLaunchKernelGGL(k_mg_post<1>, g, b, 0, st, A.us[j], Rj, A.cv[j - 1], nf, h2, Uj, skip);
+#define POST(NU, TT) \
+    hipLaunchKernelGGL((k_mg_post<NU, TT>), g, b, 0, st, A.us[j], Rj, A.cv[j - 1], nf, h2, Uj, skip)
+    MG_DISPATCH(M.nu, T, POST);
+#undef POST
 }
 
 static void coarse_dst_launch(hipStream_t st, const MgLevels &M, const float *rhs, float *u,
@@ -997,7 +1012,7 @@ __global__ void __launch_bounds__(TOL_THREADS)
 // iterate and the residual are those of k_mg_post + k_tol_update bit for
 // bit; only the block split of the norm's sum differs.  r_out must not be
 // the rhs this leg reads (the halo reads neighbouring tiles' nodes).
-template <int NU>
+template <int NU, int T>
 __global__ void __launch_bounds__(256)
     k_mg_post_update(const float *__restrict__ us, const float *__restrict__ rhs,
                      const float *__restrict__ cv, int n, float h2,
@@ -1006,9 +1021,9 @@ __global__ void __launch_bounds__(256)
                      double *__restrict__ partials, double tol, int *__restrict__ flags,
                      double *__restrict__ relres_host, int *__restrict__ host_flags) {
     SKIP_IF_DONE(flags);
-    constexpr int H = 2 * NU + 1, W = MGT + 2 * H;
+    constexpr int H = 2 * NU + 1, W = T + 2 * H;
     __shared__ float su[W * W], sf[W * W];
-    const int x0 = blockIdx.x * MGT, y0 = blockIdx.y * MGT, nc = (n + 1) / 2;
+    const int x0 = blockIdx.x * T, y0 = blockIdx.y * T, nc = (n + 1) / 2;
     for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
         const int ly = k / W, lx = k % W;
         const int i = y0 - H + ly, j = x0 - H + lx;
@@ -1017,10 +1032,10 @@ __global__ void __launch_bounds__(256)
         sf[k] = in ? rhs[(long)i * n + j] : 0.0f;
     }
     __syncthreads();
-    tile_smooth<NU>(su, sf, W, H, x0, y0, 1, n, h2);  // e valid on the tile + 1 ring
+    tile_smooth<NU, T>(su, sf, W, H, x0, y0, 1, n, h2);  // e valid on the tile + 1 ring
     double rr = 0.0, ff = 0.0;
-    for (int k = threadIdx.x; k < MGT * MGT; k += blockDim.x) {
-        const int ly = k / MGT, lx = k % MGT, i = y0 + ly, j = x0 + lx;
+    for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
+        const int ly = k / T, lx = k % T, i = y0 + ly, j = x0 + lx;
         if (i >= n || j >= n) continue;
         const long g = (long)i * n + j;
         const int c = (ly + H) * W + lx + H;
@@ -1058,30 +1073,24 @@ static void vcycle_top(hipStream_t st, MgLevels &M, int j, const float *r_in, co
     const int *skip = M.flags;
     const int nf = lvl_size(j);
     const double h = length / (nf - 1);
-    const int tiles = (nf + MGT - 1) / MGT;
+    const int T = mg_tile(nf), tiles = (nf + T - 1) / T;
     const float h2 = (float)(h * h), h2f = (float)(1.0 / (h * h));
     const dim3 g(tiles, tiles), b(256);
     const float *z = nullptr;
-    if (M.nu == 3)
-        hipLaunchKernelGGL(k_mg_pre<3>, g, b, 0, st, z, r_in, nf, h2, h2f, A.us[j], A.cres[j - 1], skip);
-    else if (M.nu == 2)
-        hipLaunchKernelGGL(k_mg_pre<2>, g, b, 0, st, z, r_in, nf, h2, h2f, A.us[j], A.cres[j - 1], skip);
-    else
-        hipLaunchKernelGGL(k_mg_pre<1>, g, b, 0, st, z, r_in, nf, h2, h2f, A.us[j], A.cres[j - 1], skip);
+#define PRE(NU, TT)                                                                             \
+    hipLaunchKernelGGL((k_mg_pre<NU, TT>), g, b, 0, st, z, r_in, nf, h2, h2f, A.us[j],          \
+                       A.cres[j - 1], skip)
+    MG_DISPATCH(M.nu, T, PRE);
+#undef PRE
     if (j - 1 == M.lc)
         coarse_dst_launch(st, M, A.cres[j - 1], A.cv[j - 1], length, skip);
     else
         vcycle_fused(st, M, j - 1, A.cv[j - 1], A.cres[j - 1], length, skip);
-#define POST_UPDATE(NU)                                                                           \
-    hipLaunchKernelGGL(k_mg_post_update<NU>, g, b, 0, st, A.us[j], r_in, A.cv[j - 1], nf, h2,     \
-                       u_in, u_out, f, h2i, r_out, M.partials, tol, M.flags, M.relres,           \
+#define POST_UPDATE(NU, TT)                                                                       \
+    hipLaunchKernelGGL((k_mg_post_update<NU, TT>), g, b, 0, st, A.us[j], r_in, A.cv[j - 1], nf, h2, \
+                       u_in, u_out, f, h2i, r_out, M.partials, tol, M.flags, M.relres,             \
                        M.host_flags)
-    if (M.nu == 3)
-        POST_UPDATE(3);
-    else if (M.nu == 2)
-        POST_UPDATE(2);
-    else
-        POST_UPDATE(1);
+    MG_DISPATCH(M.nu, T, POST_UPDATE);
 #undef POST_UPDATE
 }
 
