@@ -619,9 +619,10 @@ __global__ void __launch_bounds__(1024)
 // bound by its loads and launch, not by the LDS sweeps) and cut the cycles a
 // solve needs: the measured two-grid factor is 0.13 for V(1,1), 0.03 for
 // V(2,2) (257^2, exact 65^2 coarse level).
-// The tile edge T is 16 for levels up to 257^2, where the legs are bound by
+// The tile edge T is 16 for levels up to 513^2, where the legs are bound by
 // latency and more workgroups finish sooner despite the wider relative halo,
-// and 32 above, where the halo's extra reads matter (mg_tile).
+// and 32 above, where the halo's extra reads matter (mg_tile; measured at
+// 257^2 and 1025^2).
 template <int NU, int T> struct MgHalo {
     static constexpr int PRE = 2 * NU + 2, WPRE = T + 2 * PRE;
     static constexpr int POST = 2 * NU, WPOST = T + 2 * POST;
@@ -745,7 +746,7 @@ __global__ void __launch_bounds__(256)
 static void coarse_dst_launch(hipStream_t st, const MgLevels &M, const float *rhs, float *u,
                               double length, const int *skip);
 
-static inline int mg_tile(int nf) { return nf <= 257 ? 16 : 32; }
+static inline int mg_tile(int nf) { return nf <= 513 ? 16 : 32; }
 // LAUNCH(nu, T) for the run-time sweep count nu (1..3) and tile edge T (16, 32)
 #define MG_DISPATCH(NUV, TV, LAUNCH)                  \
     do {                                              \
