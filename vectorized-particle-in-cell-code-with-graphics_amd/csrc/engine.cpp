@@ -639,27 +639,42 @@ void absorb_species(pic_handle *h, int si, bool plop) {
 // phi from the solve with the boundary values; with_e: E (and its fp32 copy
 // for the gather) from the same values in the same launch; timestep: the
 // tolerance solve starts from the linear extrapolation of the last two steps
+// phi from the solve with the boundary values; with_e: E (and its fp32 copy
+// for the gather) from the same values in the same launch.  timestep: the
+// fused step's field_start has already formed the rhs, the extrapolated start
+// and the flag reset, and phi/E are queued before the host waits for the
+// last cycles (they take the final iterate by the cycle count's parity).
 int solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, double phiLY,
                     bool with_e = false, bool timestep = false) {
     const pic_params &P = h->P;
     int N = h->N;
     MgLevels &M = h->M;
-    launch_fold(h->st, h->rho, N, 1.0 / P.epsilon, phi0X, phiR, phi0Y, phiLY,
-                1.0 / (h->dx * h->dx), M.d.rhsF[M.ng - 1]);
+    if (!timestep)
+        launch_fold(h->st, h->rho, N, 1.0 / P.epsilon, phi0X, phiR, phi0Y, phiLY,
+                    1.0 / (h->dx * h->dx), M.d.rhsF[M.ng - 1]);
+    bool queued_e = false;
     if (P.mg_mode == PIC_MG_TOL) {
-        if (timestep) mg_extrapolate_start(h->st, M);
-        int used = mg_tol_solve(h->st, M, P.length, P.mg_tol, P.mg_max_iter, &h->mg_relres);
+        const bool early = timestep && with_e;
+        std::function<void(const double *, const double *)> phi_e;
+        if (early)
+            phi_e = [&](const double *u0, const double *u1) {
+                launch_phi_efield(h->st, u0, N, phi0X, phiR, phi0Y, phiLY, h->dx, h->phi,
+                                  h->ex64, h->ey64, h->E32, u1, M.flags);
+            };
+        int used = mg_tol_solve(h->st, M, P.length, P.mg_tol, P.mg_max_iter, &h->mg_relres,
+                                timestep, phi_e);
         if (used < 0) return seterr(h, PIC_ERR_DEVICE, "multigrid: stream error");
         h->mg_cycles = used;
         h->mg_cycles_total += used;
+        queued_e = early;
     } else {
         mg_fixed_solve(h->st, M, P.mg_ncycle, P.length);
         h->mg_cycles = P.mg_ncycle;
     }
-    if (with_e)
+    if (with_e && !queued_e)
         launch_phi_efield(h->st, M.d.u[M.ng - 1], N, phi0X, phiR, phi0Y, phiLY, h->dx, h->phi,
                           h->ex64, h->ey64, h->E32);
-    else
+    else if (!with_e)
         launch_phi_bc(h->st, M.d.u[M.ng - 1], N, phi0X, phiR, phi0Y, phiLY, h->phi);
     h->have_phi = true;
     return PIC_OK;
@@ -667,10 +682,21 @@ int solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, doub
 
 int field_step(pic_handle *h, int64_t step) {
     const pic_params &P = h->P;
+    MgLevels &M = h->M;
+    const int top = M.ng - 1;
     double phiR = P.phi_right * cos(P.phi_period * (double)step);
     int p = prof_begin(h, 3);
-    launch_rho(h->st, h->sp[1].q, h->sp[0].q, h->N, h->sp[1].scale, h->sp[0].scale,
-               P.corner8 ? 8.0 : 4.0, h->rho);
+    // rho, rhs, the tolerance solve's extrapolated start and flag reset
+    const bool tol = P.mg_mode == PIC_MG_TOL;
+    int extrap = 0;
+    if (tol) {
+        extrap = M.have_prev ? 2 : 1;
+        M.have_prev = true;
+    }
+    launch_field_start(h->st, h->sp[1].q, h->sp[0].q, h->N, h->sp[1].scale, h->sp[0].scale,
+                       P.corner8 ? 8.0 : 4.0, h->rho, 1.0 / P.epsilon, P.phi_left, phiR,
+                       P.phi_bottom, P.phi_top, 1.0 / (h->dx * h->dx), M.d.rhsF[top], M.d.u[top],
+                       M.uprev, extrap, tol ? M.flags : nullptr);
     CHECK(solve_potential(h, P.phi_left, phiR, P.phi_bottom, P.phi_top, true, true));
     prof_end(h, p);
     return PIC_OK;

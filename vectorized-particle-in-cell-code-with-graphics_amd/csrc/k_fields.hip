@@ -56,6 +56,50 @@ void launch_rho(hipStream_t st, const unsigned long long *qi, const unsigned lon
     hipLaunchKernelGGL(k_rho, g, dim3(128), 0, st, qi, qe, N, scale_i, scale_e, corner, rho);
 }
 
+__global__ void k_field_start(const unsigned long long *__restrict__ qi,
+                              const unsigned long long *__restrict__ qe, int N, double scale_i,
+                              double scale_e, double corner, double *__restrict__ rho, double coef,
+                              double phi0X, double phiR, double phi0Y, double phiLY,
+                              double dxdy_inv, double *__restrict__ rhs, double *__restrict__ u,
+                              double *__restrict__ uprev, int extrap, int *__restrict__ flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (flags && i == 0 && j == 0) flags[0] = flags[1] = flags[2] = 0;
+    if (i >= N) return;
+    // k_rho
+    bool ex = (i == 0 || i == N - 1), ey = (j == 0 || j == N - 1);
+    double fac = (ex && ey) ? corner : ((ex || ey) ? 2.0 : 1.0);
+    long k = (long)j * N + i;
+    double ri = ((double)(long long)qi[k] * scale_i) * fac;
+    double re = ((double)(long long)qe[k] * scale_e) * fac;
+    const double r = ri + re;
+    rho[k] = r;
+    // k_fold
+    double f = -coef * r;
+    if (i == 1) f = f - phi0X * dxdy_inv;
+    if (i == N - 2) f = f - phiR * dxdy_inv;
+    if (j == 1) f = f - phi0Y * dxdy_inv;
+    if (j == N - 2) f = f - phiLY * dxdy_inv;
+    rhs[k] = f;
+    // the solve's start: the linear extrapolation of the last two solutions
+    if (extrap) {
+        const double t = u[k];
+        if (extrap == 2) u[k] = 2.0 * t - uprev[k];
+        uprev[k] = t;
+    }
+}
+
+void launch_field_start(hipStream_t st, const unsigned long long *qi, const unsigned long long *qe,
+                        int N, double scale_i, double scale_e, double corner, double *rho,
+                        double coef, double phi0X, double phiR, double phi0Y, double phiLY,
+                        double dxdy_inv, double *rhs, double *u, double *uprev, int extrap,
+                        int *flags) {
+    dim3 g((N + 127) / 128, N);
+    hipLaunchKernelGGL(k_field_start, g, dim3(128), 0, st, qi, qe, N, scale_i, scale_e, corner,
+                       rho, coef, phi0X, phiR, phi0Y, phiLY, dxdy_inv, rhs, u, uprev, extrap,
+                       flags);
+}
+
 void launch_fold(hipStream_t st, const double *rho, int N, double coef, double phi0X, double phiR,
                  double phi0Y, double phiLY, double dxdy_inv, double *rhs) {
     dim3 g((N + 127) / 128, N);
@@ -152,9 +196,11 @@ __device__ __forceinline__ double eyu(const double *u, int N, int j, int i, doub
     return (phi_at(u, N, j - 1, i, b) - phi_at(u, N, j + 1, i, b)) / twoDX;
 }
 
-__global__ void k_phi_efield(const double *__restrict__ u, int N, PhiBc b, double twoDX,
+__global__ void k_phi_efield(const double *__restrict__ u0, const double *__restrict__ u1,
+                             const int *__restrict__ flags, int N, PhiBc b, double twoDX,
                              double *__restrict__ phi, double *__restrict__ ex64,
                              double *__restrict__ ey64, float2 *__restrict__ E32) {
+    const double *__restrict__ u = (flags && (flags[1] & 1)) ? u1 : u0;
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long tot32 = (long)(N + 1) * N + 1, tot64 = (long)(N + 1) * (N + 1);
     if (k >= tot64) return;
@@ -186,10 +232,10 @@ __global__ void k_phi_efield(const double *__restrict__ u, int N, PhiBc b, doubl
 
 void launch_phi_efield(hipStream_t st, const double *u, int N, double phi0X, double phiR,
                        double phi0Y, double phiLY, double dx, double *phi, double *ex64,
-                       double *ey64, float2 *E32) {
+                       double *ey64, float2 *E32, const double *u1, const int *flags) {
     long tot = (long)(N + 1) * (N + 1);
-    hipLaunchKernelGGL(k_phi_efield, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, u, N,
-                       PhiBc{phi0X, phiR, phi0Y, phiLY}, 2.0 * dx, phi, ex64, ey64, E32);
+    hipLaunchKernelGGL(k_phi_efield, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, u, u1,
+                       flags, N, PhiBc{phi0X, phiR, phi0Y, phiLY}, 2.0 * dx, phi, ex64, ey64, E32);
 }
 
 void launch_efield(hipStream_t st, const double *phi, int N, double dx, double *ex64, double *ey64,
@@ -1025,6 +1071,24 @@ __global__ void __launch_bounds__(256)
     constexpr int H = 2 * NU + 1, W = T + 2 * H;
     __shared__ float su[W * W], sf[W * W];
     const int x0 = blockIdx.x * T, y0 = blockIdx.y * T, nc = (n + 1) / 2;
+    // one node per thread (16 x 16 tiles): its FP64 operands are read now, so
+    // their latency overlaps the sweeps
+    constexpr bool ONE = T * T == 256;
+    double pu = 0.0, pn = 0.0, ps = 0.0, pe = 0.0, pw = 0.0, pf = 0.0;
+    if (ONE) {
+        const int i = y0 + (int)threadIdx.x / T, j = x0 + (int)threadIdx.x % T;
+        if (i < n && j < n) {
+            const long g = (long)i * n + j;
+            pu = u_in[g];
+            if (i > 0 && j > 0 && i < n - 1 && j < n - 1) {
+                pn = u_in[g + n];
+                ps = u_in[g - n];
+                pe = u_in[g + 1];
+                pw = u_in[g - 1];
+                pf = f[g];
+            }
+        }
+    }
     for (int k = threadIdx.x; k < W * W; k += blockDim.x) {
         const int ly = k / W, lx = k % W;
         const int i = y0 - H + ly, j = x0 - H + lx;
@@ -1040,16 +1104,19 @@ __global__ void __launch_bounds__(256)
         if (i >= n || j >= n) continue;
         const long g = (long)i * n + j;
         const int c = (ly + H) * W + lx + H;
-        double v = u_in[g];
+        double v = ONE ? pu : u_in[g];
         float out = 0.0f;
         if (i > 0 && j > 0 && i < n - 1 && j < n - 1) {
+            const double fg = ONE ? pf : f[g];
             v += (double)su[c];
-            const double un = u_in[g + n] + (double)su[c + W], us_ = u_in[g - n] + (double)su[c - W];
-            const double ue = u_in[g + 1] + (double)su[c + 1], uw = u_in[g - 1] + (double)su[c - 1];
-            const double r = -h2i * (un + us_ + ue + uw - 4.0 * v) + f[g];
+            const double un = (ONE ? pn : u_in[g + n]) + (double)su[c + W];
+            const double us_ = (ONE ? ps : u_in[g - n]) + (double)su[c - W];
+            const double ue = (ONE ? pe : u_in[g + 1]) + (double)su[c + 1];
+            const double uw = (ONE ? pw : u_in[g - 1]) + (double)su[c - 1];
+            const double r = -h2i * (un + us_ + ue + uw - 4.0 * v) + fg;
             rr += r * r;
             out = (float)r;
-            ff += f[g] * f[g];
+            ff += fg * fg;
         }
         u_out[g] = v;
         r_out[g] = out;
@@ -1095,25 +1162,6 @@ static void vcycle_top(hipStream_t st, MgLevels &M, int j, const float *r_in, co
 #undef POST_UPDATE
 }
 
-// u <- 2 u - uprev, uprev <- u (the old u); first: uprev <- u only
-__global__ void k_extrapolate(double *__restrict__ u, double *__restrict__ uprev, long tot,
-                              int first) {
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < tot;
-         k += (long)gridDim.x * blockDim.x) {
-        const double t = u[k];
-        if (!first) u[k] = 2.0 * t - uprev[k];
-        uprev[k] = t;
-    }
-}
-
-void mg_extrapolate_start(hipStream_t st, MgLevels &M) {
-    const int lf = M.ng - 1, n = lvl_size(lf);
-    const long tot = (long)n * n;
-    hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)std::min<long>((tot + 255) / 256, 1024)),
-                       dim3(256), 0, st, M.d.u[lf], M.uprev, tot, M.have_prev ? 0 : 1);
-    M.have_prev = true;
-}
-
 // Wait for the queued cycles: the solve's host check sits between two phases
 // of the step, so the wake-up latency of a blocking wait is idle GPU time;
 // poll the batch's event for up to 2 ms, then block (also reports errors).
@@ -1128,7 +1176,8 @@ static hipError_t wait_batch(hipStream_t st, hipEvent_t ev) {
 }
 
 int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max_iter,
-                 double *relres_out) {
+                 double *relres_out, bool flags_reset,
+                 const std::function<void(const double *, const double *)> &after_batch) {
     int lf = M.ng - 1, n = lvl_size(lf);
     double h = length / (n - 1), h2i = 1.0 / (h * h);
     double *u = M.d.u[lf];
@@ -1162,7 +1211,7 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
                        rb[(queued + 1) & 1], tol, length);
         }
     };
-    hipLaunchKernelGGL(k_tol_reset, dim3(1), dim3(1), 0, st, M.flags);
+    if (!flags_reset) hipLaunchKernelGGL(k_tol_reset, dim3(1), dim3(1), 0, st, M.flags);
     hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st, (const double *)u,
                        (const float *)nullptr, (double *)nullptr, f, n, h2i, r32, M.partials, tol,
                        M.flags, M.relres, M.host_flags);
@@ -1171,6 +1220,7 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     int batch = M.pred < 1 ? 1 : M.pred;
     for (;;) {
         for (int k = 0; k < batch && queued < max_iter; k++, queued++) cycle();
+        if (after_batch) after_batch(buf[0], buf[1]);
         if (wait_batch(st, M.done) != hipSuccess) return -1;
         if (M.host_flags[0] || queued >= max_iter) break;
         batch = 1;

@@ -1,6 +1,8 @@
 // pic_internal.h -- host/device structures shared by the library's sources.
 #pragma once
 
+#include <functional>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -248,9 +250,19 @@ void launch_fold(hipStream_t st, const double *rho, int N, double coef, double p
                  double phi0Y, double phiLY, double dxdy_inv, double *rhs);
 void launch_phi_bc(hipStream_t st, const double *u, int N, double phi0X, double phiR,
                    double phi0Y, double phiLY, double *phi);
+// u1, flags (may be null): the iterate is u1 when flags[1] is odd
 void launch_phi_efield(hipStream_t st, const double *u, int N, double phi0X, double phiR,
                        double phi0Y, double phiLY, double dx, double *phi, double *ex64,
-                       double *ey64, float2 *E32);
+                       double *ey64, float2 *E32, const double *u1 = nullptr,
+                       const int *flags = nullptr);
+// the fused timestep's field set-up per node: rho (k_rho), the folded rhs
+// (k_fold), the tolerance solve's start (extrap 1: uprev <- u; 2: also
+// u <- 2 u - uprev) and, with flags, the solve's flag reset
+void launch_field_start(hipStream_t st, const unsigned long long *qi, const unsigned long long *qe,
+                        int N, double scale_i, double scale_e, double corner, double *rho,
+                        double coef, double phi0X, double phiR, double phi0Y, double phiLY,
+                        double dxdy_inv, double *rhs, double *u, double *uprev, int extrap,
+                        int *flags);
 void launch_efield(hipStream_t st, const double *phi, int N, double dx, double *ex64, double *ey64,
                    float2 *E32);
 template <typename T>
@@ -261,11 +273,12 @@ void mg_set_coarse_attr(long lds_bytes);
 void mg_fixed_solve(hipStream_t st, MgLevels &M, int ncycle, double length);
 // residual-controlled solve of A u = M.d.rhsF[ng-1] into M.d.u[ng-1] (warm
 // started from its current content); returns V-cycles used, -1 on HIP error
-// start the next timestep's solve from 2 u_n - u_(n-1) (the first call only
-// records u_n)
-void mg_extrapolate_start(hipStream_t st, MgLevels &M);
+// after_batch(u0, u1) is queued after each batch of cycles, before the host
+// waits for it; the final iterate is u1 when the applied cycle count
+// (flags[1]) is odd, else u0.  flags_reset: the caller has cleared flags.
 int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max_iter,
-                 double *relres_out);
+                 double *relres_out, bool flags_reset = false,
+                 const std::function<void(const double *, const double *)> &after_batch = nullptr);
 
 
 }  // namespace pic
