@@ -41,6 +41,8 @@ extern "C" {
 /* vectors in tests/test_oracle_philox.py.                                   */
 /* ------------------------------------------------------------------------ */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Philox2x32-10 (the same paper): counter (c0, c1), one key word */
+void orc_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]);
 /* uniform in the open interval (0,1): ((x >> 9) + 0.5) * 2^-23, exact in fp32 */
 float orc_u01(uint32_t x);
 /* 53-bit uniform in (0,1) from two words (the loader's and the neutral table's) */

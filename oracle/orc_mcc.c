@@ -30,6 +30,16 @@ static void draw(const orc_mcc_params *c, uint32_t id, uint32_t step, uint32_t s
     for (int k = 0; k < 4; k++) u[k] = (double)orc_u01(raw[k]);
 }
 
+/* physical-mode candidate test: one Philox2x32-10 word, counter (id, step),
+ * key = key0 xor a per-species constant; only candidates draw stream RS_MAIN */
+#define KEY_CAND_E 0x43414E45u
+#define KEY_CAND_I 0x43414E49u
+static int candidate(const orc_mcc_params *c, uint32_t id, uint32_t step, int electron) {
+    uint32_t ctr[2] = {id, step}, r[2];
+    orc_philox2x32_10(ctr, c->key0 ^ (electron ? KEY_CAND_E : KEY_CAND_I), r);
+    return (double)orc_u01(r[0]) < c->p_null;
+}
+
 static void neutral(const orc_mcc_params *c, uint32_t id, uint32_t step, uint32_t s0, double *nx,
                     double *ny, double *nz) {
     double u[4];
@@ -107,8 +117,8 @@ int orc_collide_electron(const orc_mcc_params *c, double *vx, double *vy, double
                          uint32_t step, double child[6]) {
     double u[4];
     uint32_t raw[4];
+    if (!candidate(c, id, step, 1)) return 0;
     draw(c, id, step, RS_MAIN, u, raw);
-    if (!(u[0] < c->p_null)) return 0;
     double nx, ny, nz;
     neutral(c, id, step, 0u, &nx, &ny, &nz);
     double gx = *vx - nx, gy = *vy - ny, gz = *vz - nz;
@@ -172,8 +182,8 @@ int orc_collide_ion(const orc_mcc_params *c, double *vx, double *vy, double *vz,
                     uint32_t step) {
     double u[4];
     uint32_t raw[4];
+    if (!candidate(c, id, step, 0)) return 0;
     draw(c, id, step, 16u + RS_MAIN, u, raw);
-    if (!(u[0] < c->p_null)) return 0;
     double nx, ny, nz;
     neutral(c, id, step, 16u, &nx, &ny, &nz);
     double gx = *vx - nx, gy = *vy - ny, gz = *vz - nz;

@@ -34,6 +34,19 @@ double orc_u01d(uint32_t hi, uint32_t lo) {
     return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
 }
 
+void orc_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], k = key;
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi, lo;
+        mulhilo32(0xD256D193u, c0, &hi, &lo);
+        c0 = hi ^ k ^ c1;
+        c1 = lo;
+        k += PHILOX_W0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+}
+
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
     uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
     uint32_t k0 = key[0], k1 = key[1];

@@ -159,6 +159,13 @@ def P(a):
     return a.ctypes.data_as(vp) if a is not None else None
 
 
+def philox2(ctr, key):
+    """Philox2x32-10 of the oracle"""
+    out = (C.c_uint32 * 2)()
+    lib().orc_philox2x32_10((C.c_uint32 * 2)(*ctr), C.c_uint32(key), out)
+    return tuple(out)
+
+
 def philox(ctr, key):
     out = (C.c_uint32 * 4)()
     lib().orc_philox4x32_10((C.c_uint32 * 4)(*ctr), (C.c_uint32 * 2)(*key), out)

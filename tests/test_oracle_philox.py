@@ -1,5 +1,6 @@
-"""Philox4x32-10 of the oracle against the published Random123 known-answer vectors
-(Salmon et al. SC'11; kat_vectors of the Random123 distribution)."""
+"""Philox4x32-10 and Philox2x32-10 of the oracle against the published Random123
+known-answer vectors (Salmon et al. SC'11; kat_vectors of the Random123
+distribution)."""
 import numpy as np
 
 import orc
@@ -14,9 +15,22 @@ KAT = [
 ]
 
 
+# philox2x32 10: counter, key, output
+KAT2 = [
+    ((0x00000000, 0x00000000), 0x00000000, (0xFF1DAE59, 0x6CD10DF2)),
+    ((0xFFFFFFFF, 0xFFFFFFFF), 0xFFFFFFFF, (0x2C3F628B, 0xAB4FD7AD)),
+    ((0x243F6A88, 0x85A308D3), 0x13198A2E, (0xDD7CE038, 0xF62A4C12)),
+]
+
+
 def test_known_answer_vectors():
     for ctr, key, want in KAT:
         assert tuple(orc.philox(ctr, key)) == want
+
+
+def test_known_answer_vectors_2x32():
+    for ctr, key, want in KAT2:
+        assert orc.philox2(ctr, key) == want
 
 
 def test_uniform_open_interval_and_moments():

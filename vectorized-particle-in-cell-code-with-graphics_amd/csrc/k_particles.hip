@@ -85,7 +85,7 @@ __device__ __forceinline__ int advance_one(PState &s, uint32_t id, const float2 
 #define CQ_CAP 256  // collision queue entries per block (11 words each)
 
 // Block-level null-collision step (physical mode) for the P particles of every
-// thread.  Every particle makes its candidate test (u < 1 - exp(-nu' dt)); the
+// thread.  Every particle makes its candidate test (mcc::candidate); the
 // candidates -- about 13% of the electrons in the argon discharge, some 130 of
 // a block's 1024 particles -- go into one LDS queue so the collision
 // kinematics run once per block on dense lanes.  qs[r] returns the particle's
@@ -112,13 +112,7 @@ __device__ __forceinline__ void block_collide_n(const bool (&active)[P], const P
     uint32_t wtot = 0;
 #pragma unroll
     for (int r = 0; r < P; r++) {
-        bool cand = false;
-        if (active[r]) {
-            uint32_t x[4];
-            philox4x32_10(id[r], step, (mc.electron ? 0u : 16u) + RS_MAIN, RNG_DOMAIN_MCC,
-                          mc.key0, mc.key1, x);
-            cand = u01(x[0]) < mc.p_null;
-        }
+        const bool cand = active[r] && mcc::candidate(mc, id[r], step);
         const uint64_t bal = __ballot(cand);
         pos[r] = cand ? wtot + (uint32_t)__popcll(bal & below) : 0xFFFFFFFFu;
         wtot += (uint32_t)__popcll(bal);

@@ -286,6 +286,17 @@ struct Child {
 #define RS_INJECT 10  // particle source: + 256 k for the k-th draw of a particle
 #define RS_REDUCE 11  // population reduction: selection key
 
+// Physical-mode null-collision candidate test: u < 1 - exp(-nu' dt) with u
+// from one Philox2x32-10 word (counter (id, step), the species' key); only a
+// candidate draws its collision from Philox4x32-10 stream RS_MAIN.
+#define RNG_KEY_CAND_E 0x43414E45u
+#define RNG_KEY_CAND_I 0x43414E49u
+HD bool candidate(const MccConst &c, uint32_t id, uint32_t step) {
+    uint32_t r[2];
+    philox2x32_10(id, step, c.key0 ^ (c.electron ? RNG_KEY_CAND_E : RNG_KEY_CAND_I), r);
+    return u01(r[0]) < c.p_null;
+}
+
 // Electron collision.  Returns the event; on EV_IONIZE in PHYS mode `ch`
 // holds the ejected electron and the new ion; in REF mode EV_IONIZE means the
 // parent is removed.  inside: the particle passes the reference's
@@ -294,8 +305,8 @@ __device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, fl
                                                 uint32_t id, uint32_t step, bool inside, Child &ch) {
     const uint32_t s0 = (c.electron ? 0u : 16u);
     uint32_t r[4];
-    philox4x32_10(id, step, s0 + RS_MAIN, RNG_DOMAIN_MCC, c.key0, c.key1, r);
     if (c.mode == PIC_DEV_MODE_REF) {
+        philox4x32_10(id, step, s0 + RS_MAIN, RNG_DOMAIN_MCC, c.key0, c.key1, r);
         if (!inside) return EV_NONE;
         float v2 = vx * vx + vy * vy + vz * vz;
         float E = (0.5f / c.q) * c.mass * v2;
@@ -335,7 +346,8 @@ __device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, fl
         return ev;
     }
     // PHYS null collision
-    if (!(u01(r[0]) < c.p_null)) return EV_NONE;
+    if (!candidate(c, id, step)) return EV_NONE;
+    philox4x32_10(id, step, s0 + RS_MAIN, RNG_DOMAIN_MCC, c.key0, c.key1, r);
     float nx, ny, nz;
     normals3(id, step, s0 + RS_NEUTRAL, c, nx, ny, nz);
     nx *= c.vtn;
@@ -414,9 +426,9 @@ __device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, fl
 __device__ __forceinline__ int collide_ion(const MccConst &c, float &vx, float &vy, float &vz,
                                            uint32_t id, uint32_t step) {
     const uint32_t s0 = 16u;
+    if (!candidate(c, id, step)) return EV_NONE;
     uint32_t r[4];
     philox4x32_10(id, step, s0 + RS_MAIN, RNG_DOMAIN_MCC, c.key0, c.key1, r);
-    if (!(u01(r[0]) < c.p_null)) return EV_NONE;
     float nx, ny, nz;
     normals3(id, step, s0 + RS_NEUTRAL, c, nx, ny, nz);
     nx *= c.vtn;

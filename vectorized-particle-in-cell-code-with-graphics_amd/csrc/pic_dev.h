@@ -253,6 +253,20 @@ __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[3] = c3;
 }
 
+// Philox2x32-10 (Salmon et al., SC'11): two words from a two-word counter and
+// a one-word key, one 32 x 32 -> 64-bit product per round
+__host__ __device__ inline void philox2x32_10(uint32_t c0, uint32_t c1, uint32_t k, uint32_t out[2]) {
+    for (int r = 0; r < 10; r++) {
+        uint64_t p = (uint64_t)0xD256D193u * c0;
+        uint32_t hi = (uint32_t)(p >> 32), lo = (uint32_t)p;
+        c0 = hi ^ k ^ c1;
+        c1 = lo;
+        k += 0x9E3779B9u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+}
+
 // uniform in (0,1), exact in fp32
 __host__ __device__ inline float u01(uint32_t x) {
     return ((float)(x >> 9) + 0.5f) * (1.0f / 8388608.0f);
