@@ -331,12 +331,13 @@ __global__ void __launch_bounds__(256)
                  PushConst pc, mcc::MccConst mc, uint32_t step, uint32_t *__restrict__ binCount,
                  uint32_t *__restrict__ lost6, int do_push) {
     pin_variant<V>(pc, mc, sp);
+    const uint32_t n = *nptr;
+    if (blockIdx.x * (blockDim.x * IPP) >= n) return;  // past the live count
     __shared__ uint32_t sl[6];
     __shared__ unsigned long long lds[BLOCK_LDS_U64];
     __shared__ BlockHash H;
     if (threadIdx.x < 6) sl[threadIdx.x] = 0;
     hash_clear(H);
-    const uint32_t n = *nptr;
     const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
     const bool need_vz = pc.bfield != 0 || mc.on;
     const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
@@ -755,6 +756,7 @@ __global__ void __launch_bounds__(256)
     pin_variant<V>(pc, mc, src);
     pin_spawn<V>(sp);
     if (V != PV_GENERIC) dst.w = nullptr;
+    if (blockIdx.x * (blockDim.x * IPP) >= *nptr) return;  // past the live count
     __shared__ unsigned long long smem[2 * BLOCK_LDS_U64];
     __shared__ uint32_t se[8], s_total;
     static_assert(sizeof(BlockHash) <= BLOCK_LDS_U64 * sizeof(unsigned long long), "hash size");
@@ -916,6 +918,10 @@ __global__ void __launch_bounds__(256)
                    uint32_t *__restrict__ lost6, uint32_t *__restrict__ holes) {
     pin_variant<V>(pc, mc, sp);
     pin_spawn<V>(spn);
+    const uint32_t n = *nptr;
+    // the grid covers the capacity: blocks past the live count leave at once
+    // (uniformly, before any barrier)
+    if (blockIdx.x * (blockDim.x * IPP) >= n) return;
     __shared__ uint32_t sl[6];
     __shared__ uint32_t shole;
     __shared__ uint32_t se[8];
@@ -923,7 +929,6 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x < 6) sl[threadIdx.x] = 0;
     if (threadIdx.x < 8) se[threadIdx.x] = 0;
     if (threadIdx.x == 0) shole = 0;
-    const uint32_t n = *nptr;
     const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
     // vz and the id are read only where something uses them: the magnetic
     // rotation, collisions, re-injection keyed by the id, and the wall
