@@ -706,8 +706,7 @@ int fused_step(pic_handle *h) {
     int N = h->N;
     int pt = prof_begin(h, 0);
     int pp = prof_begin(h, 1);
-    launch_fill<unsigned long long>(h->st, h->sp[0].q, (long)N * N, 0ull);
-    launch_fill<unsigned long long>(h->st, h->sp[1].q, (long)N * N, 0ull);
+    launch_zero2(h->st, h->sp[0].q, h->sp[1].q, (long)N * N);
     // particle source after the electron collisions, before the gather
     // (pic.cpp:1071-1082): electrons are staged already moved, ions unmoved
     inject(h, 0, 1);

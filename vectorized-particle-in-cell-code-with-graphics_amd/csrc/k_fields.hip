@@ -367,6 +367,21 @@ template void launch_fill<unsigned long long>(hipStream_t, unsigned long long *,
                                               unsigned long long);
 template void launch_fill<uint32_t>(hipStream_t, uint32_t *, long, uint32_t);
 
+// both species' fixed-point charge cleared in one launch
+__global__ void k_zero2(unsigned long long *__restrict__ a, unsigned long long *__restrict__ b,
+                        long n) {
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        a[k] = 0ull;
+        b[k] = 0ull;
+    }
+}
+
+void launch_zero2(hipStream_t st, unsigned long long *a, unsigned long long *b, long n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_zero2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, n);
+}
+
 // --------------------------------------- multigrid: one-workgroup coarse part
 
 static __host__ __device__ inline int lvl_size(int l) { return (1 << (l + 1)) + 1; }

@@ -267,6 +267,7 @@ void launch_efield(hipStream_t st, const double *phi, int N, double dx, double *
                    float2 *E32);
 template <typename T>
 void launch_fill(hipStream_t st, T *p, long n, T v);
+void launch_zero2(hipStream_t st, unsigned long long *a, unsigned long long *b, long n);
 int mg_coarse_level_for(long lds_bytes, int ng);
 int mg_dst_tables(MgLevels &M);
 void mg_set_coarse_attr(long lds_bytes);
