@@ -568,7 +568,11 @@ int read_counters(pic_handle *h) {
     return PIC_OK;
 }
 
-// counting sort + (optional) collisions and push + deposit of one species
+// counting sort + (optional) collisions and push + deposit of one species.
+// The keys are the cells at the start of the pass; with a push, the
+// particles lost in the pass leave holes in the sorted store (counted in the
+// holes counter, dropped by the next sort pass); without one the output is
+// compact.
 void push_species(pic_handle *h, int si, int do_push, int deposit) {
     Species &s = h->sp[si];
     int c = s.cur;
@@ -576,9 +580,9 @@ void push_species(pic_handle *h, int si, int do_push, int deposit) {
     mcc::MccConst mc = s.mc;
     if (!do_push) mc.on = 0;
     Spawn spn = make_spawn(h, si == 1 && do_push && plop_on(h), do_push ? si : -1);
+    clear_holes(h, si);  // the pass drops the old holes
     int p1 = prof_begin(h, 4);
-    launch_push_count(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, mc, step,
-                      h->binCount, &h->cnt[si].lost[0][0], do_push);
+    launch_key_count(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, s.pc, h->binCount, do_push);
     prof_end(h, p1);
     int p2 = prof_begin(h, 5);
     launch_scan(h->st, h->binCount, h->binStart, h->nbins_pad, h->scanPartial, &h->cnt[si].n[1 - c]);
@@ -586,10 +590,11 @@ void push_species(pic_handle *h, int si, int do_push, int deposit) {
     int p3 = prof_begin(h, 2);
     launch_push_scatter(h->st, s.buf[c], s.buf[1 - c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, mc,
                         step, h->binCount, h->binStart, s.q, do_push, deposit, spn,
-                        &h->cnt[si].events[0][0], &h->cnt[si].lost[0][0]);
+                        &h->cnt[si].events[0][0], &h->cnt[si].lost[0][0],
+                        &h->cnt[si].holes[0]);
     prof_end(h, p3);
     s.cur = 1 - c;
-    clear_holes(h, si);
+    s.holes = do_push != 0;
     s.since_sort = 0;
 }
 
@@ -957,7 +962,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     if (hipMemcpy(h->cnt, h->cnt_host, 2 * sizeof(SpCounters), hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_global_error(PIC_ERR_DEVICE, "counter upload failed"));
     int N = h->N;
-    if ((rc = dalloc(h, &h->binCount, h->nbins_pad)) != PIC_OK ||
+    if ((rc = dalloc(h, &h->binCount, (size_t)h->nbins_pad * BIN_STRIDE)) != PIC_OK ||
         (rc = dalloc(h, &h->binStart, h->nbins_pad + 1)) != PIC_OK ||
         (rc = dalloc(h, &h->scanPartial, h->nbins_pad / 4096 + 1)) != PIC_OK ||
         (rc = dalloc(h, &h->rho, (size_t)N * N)) != PIC_OK ||
@@ -966,7 +971,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         (rc = dalloc(h, &h->ey64, (size_t)(N + 1) * (N + 1))) != PIC_OK ||
         (rc = dalloc(h, &h->E32, (size_t)(N + 1) * N + 1)) != PIC_OK)
         return fail(rc);
-    hipMemsetAsync(h->binCount, 0, 4ull * h->nbins_pad, h->st);
+    hipMemsetAsync(h->binCount, 0, 4ull * h->nbins_pad * BIN_STRIDE, h->st);
     hipMemsetAsync(h->rho, 0, 8ull * N * N, h->st);
     hipMemsetAsync(h->phi, 0, 8ull * N * N, h->st);
     hipMemsetAsync(h->ex64, 0, 8ull * (N + 1) * (N + 1), h->st);

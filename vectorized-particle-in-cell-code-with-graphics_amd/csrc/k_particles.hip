@@ -1,15 +1,16 @@
 // k_particles.hip -- particle kernels.
 //
-// Fused timestep of one species (two passes over the particle store):
-//   pass 1  k_push_count:   gather + push + wall test, histogram of the new
-//                           cells (counting-sort key, aggregated per block in
-//                           LDS), per-wall loss counts;
+// Sort pass of one species (a timestep that also reorders the store):
+//   pass 1  k_key_count:    histogram of the sort keys of the particles'
+//                           cells at the start of the step (counting-sort
+//                           key, aggregated per block in LDS); holes of the
+//                           store are not counted, so they are dropped;
 //   scan    k_scan_*:       exclusive scan of the histogram -> bin starts;
-//   pass 2  k_push_scatter: the identical gather + push (bitwise the same
-//                           result), scatter of the survivor into its cell's
-//                           slot of the other buffer (one claim per key per
-//                           block; stream compaction of the lost particles
-//                           comes for free), and the fixed-point CIC deposit
+//   pass 2  k_push_scatter: collisions + gather + push + wall test, the
+//                           particle written to its key's slot of the other
+//                           buffer (one claim per key per block; a particle
+//                           lost in this step leaves a hole there), per-wall
+//                           loss counts, and the fixed-point CIC deposit
 //                           accumulated in an LDS window around the block's
 //                           cells, flushed with one 64-bit atomic per touched
 //                           node.
@@ -56,8 +57,7 @@ __device__ __forceinline__ bool ref_mcc_inside(const PState &s, int N) {
 }
 
 // collision (if enabled and not already settled by block_collide_n) then push;
-// returns the particle's status.  `ev` and `ch` report the collision.  Both
-// passes call this with identical inputs.
+// returns the particle's status.  `ev` and `ch` report the collision.
 __device__ __forceinline__ int advance_one(PState &s, uint32_t id, const float2 *__restrict__ E,
                                            const PushConst &pc, const mcc::MccConst &mc,
                                            uint32_t step, int do_push, int &ev, mcc::Child &ch,
@@ -299,84 +299,55 @@ __device__ __forceinline__ void pin_spawn(Spawn &spn) {
     }
 }
 
-// one particle of k_push_count
-__device__ __forceinline__ void count_round(PState &s, uint32_t pid, bool valid,
-                                            const float2 *__restrict__ E, const PushConst &pc,
-                                            const mcc::MccConst &mc, uint32_t step, int do_push,
-                                            bool blk, int qs, BlockHash &H, uint32_t *sl,
-                                            const unsigned long long *lds) {
-    int ev = mcc::EV_NONE;
-    mcc::Child ch{};
-    const bool done = collide_take(blk, qs, s, ev, ch, lds);
-    bool kept = false;
-    uint32_t key = 0u;
-    if (valid) {
-        const int st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, done);
-        kept = st == ST_KEPT;
-        if (kept)
-            key = bin_key(s.jx, s.jy, pc.tilesX) >> pc.keyshift;
-        else if (st != ST_HOLE)
-            atomicAdd(&sl[st - 1], 1u);
-    }
-    uint32_t rank;
-    hash_insert(H, key, kept, rank);
-}
-
-// Pass 1 of the sort pass: collide + push + wall test of IPP particles per
-// thread (strided by the block size), histogram of the new keys through the
-// block's hash table, per-wall loss counts.
-template <int V>
+// Pass 1 of the sort pass: the histogram of the sort keys of the particles'
+// current cells, aggregated per block through the LDS hash table (one global
+// atomic per distinct key per block).  In a timestep's sort pass (do_push)
+// every particle that is not a hole takes a slot, keyed by its cell before the
+// step: one that leaves during the step leaves a hole in its slot, which the
+// next sort pass drops.  Without a push (compaction) only the particles
+// inside the domain are counted, so the output is compact.  Reads 4 B per
+// particle (12 B without a push).
 __global__ void __launch_bounds__(256)
-    k_push_count(SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E,
-                 PushConst pc, mcc::MccConst mc, uint32_t step, uint32_t *__restrict__ binCount,
-                 uint32_t *__restrict__ lost6, int do_push) {
-    pin_variant<V>(pc, mc, sp);
+    k_key_count(SpView sp, const uint32_t *__restrict__ nptr, PushConst pc,
+                uint32_t *__restrict__ binCount, int do_push) {
     const uint32_t n = *nptr;
     if (blockIdx.x * (blockDim.x * IPP) >= n) return;  // past the live count
-    __shared__ uint32_t sl[6];
-    __shared__ unsigned long long lds[BLOCK_LDS_U64];
     __shared__ BlockHash H;
-    if (threadIdx.x < 6) sl[threadIdx.x] = 0;
     hash_clear(H);
     const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
-    const bool need_vz = pc.bfield != 0 || mc.on;
-    const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
-    PState s[IPP];
-    uint32_t pid[IPP];
+    uint32_t key[IPP];
+    bool in[IPP];
 #pragma unroll
-    for (int r = 0; r < IPP; r++) {  // every load first
+    for (int r = 0; r < IPP; r++) {
         const uint32_t i = base + r * blockDim.x;
-        s[r] = PState{};
-        pid[r] = 0u;
+        key[r] = 0u;
+        in[r] = false;
         if (i < n) {
-            s[r] = load_state(sp, i, need_vz);
-            if (mc.on) pid[r] = sp.id[i];
+            const uint32_t c = sp.cell[i];
+            PState s{};
+            s.jx = cell_jx(c);
+            s.jy = cell_jy(c);
+            if (do_push) {
+                in[r] = s.jx != CELL_HOLE;
+            } else if (s.jx != CELL_HOLE) {
+                s.ox = sp.ox[i];
+                s.oy = sp.oy[i];
+                in[r] = classify(s, pc.N, pc.mode) == ST_KEPT;
+            }
+            if (in[r]) key[r] = bin_key(s.jx, s.jy, pc.tilesX) >> pc.keyshift;
         }
     }
-    __syncthreads();  // table and counters cleared
-    int qs[IPP];
-    if (blk) {
-        bool act[IPP];
+    __syncthreads();  // table cleared
 #pragma unroll
-        for (int r = 0; r < IPP; r++) act[r] = base + r * blockDim.x < n && s[r].jx != CELL_HOLE;
-        block_collide_n<IPP>(act, s, pid, mc, step, qs, lds);
+    for (int r = 0; r < IPP; r++) {
+        uint32_t rank;
+        hash_insert(H, key[r], in[r], rank);
     }
-    // the rounds are spelled out so every array index is a compile-time constant
-    count_round(s[0], pid[0], base + 0 * blockDim.x < n, E, pc, mc, step, do_push, blk, qs[0], H,
-                sl, lds);
-    count_round(s[1], pid[1], base + 1 * blockDim.x < n, E, pc, mc, step, do_push, blk, qs[1], H,
-                sl, lds);
-    count_round(s[2], pid[2], base + 2 * blockDim.x < n, E, pc, mc, step, do_push, blk, qs[2], H,
-                sl, lds);
-    count_round(s[3], pid[3], base + 3 * blockDim.x < n, E, pc, mc, step, do_push, blk, qs[3], H,
-                sl, lds);
     __syncthreads();
     for (int e = threadIdx.x; e < HASH_SIZE; e += blockDim.x) {
         const uint32_t k = H.key[e];
-        if (k != HASH_EMPTY) atomicAdd(&binCount[k], H.cnt[e]);
+        if (k != HASH_EMPTY) atomicAdd(&binCount[(size_t)k * BIN_STRIDE], H.cnt[e]);
     }
-    if (threadIdx.x < 6 && sl[threadIdx.x])
-        atomicAdd(&lost6[cnt_slot() * 6 + threadIdx.x], sl[threadIdx.x]);
 }
 
 // ------------------------------------------------------------ scan
@@ -429,14 +400,11 @@ __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t *t
 
 __global__ void __launch_bounds__(SCAN_THREADS)
     k_scan_partial(const uint32_t *__restrict__ in, uint32_t *__restrict__ partial) {
-    const uint4 *p = reinterpret_cast<const uint4 *>(in + (size_t)blockIdx.x * SCAN_TILE) +
-                     threadIdx.x * (SCAN_PER_THREAD / 4);
+    const uint32_t *p =
+        in + ((size_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD) * BIN_STRIDE;
     uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_PER_THREAD / 4; k++) {
-        uint4 q = p[k];
-        s += q.x + q.y + q.z + q.w;
-    }
+    for (int k = 0; k < SCAN_PER_THREAD; k++) s += p[k * BIN_STRIDE];
     uint32_t tot;
     block_excl_scan(s, &tot);
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
@@ -465,15 +433,9 @@ __global__ void __launch_bounds__(SCAN_THREADS)
                  uint32_t *__restrict__ out) {
     size_t base = (size_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
     uint32_t v[SCAN_PER_THREAD];
-    const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+    const uint32_t *p = in + base * BIN_STRIDE;
 #pragma unroll
-    for (int k = 0; k < SCAN_PER_THREAD / 4; k++) {
-        uint4 q = p[k];
-        v[4 * k] = q.x;
-        v[4 * k + 1] = q.y;
-        v[4 * k + 2] = q.z;
-        v[4 * k + 3] = q.w;
-    }
+    for (int k = 0; k < SCAN_PER_THREAD; k++) v[k] = p[k * BIN_STRIDE];
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_PER_THREAD; k++) s += v[k];
@@ -717,13 +679,13 @@ __device__ __forceinline__ void stage_products(int ev, const mcc::Child &ch, con
 
 // one particle of k_push_scatter (before the claims)
 __device__ __forceinline__ void scatter_round(PState &s, uint32_t pid, float w, bool valid,
-                                              bool &kept, bool &dep, uint32_t &ent,
+                                              bool &claim, bool &kept, bool &dep, uint32_t &ent,
                                               uint32_t &rank, const float2 *__restrict__ E,
                                               const PushConst &pc, const mcc::MccConst &mc,
                                               uint32_t step, int do_push, bool blk, int qs,
-                                              BlockHash &H, uint32_t *se,
-                                              const unsigned long long *lds, const Spawn &sp,
-                                              uint32_t *__restrict__ lost6) {
+                                              BlockHash &H, uint32_t *se, uint32_t *sl,
+                                              uint32_t *shole, const unsigned long long *lds,
+                                              const Spawn &sp, uint32_t *__restrict__ lost6) {
     const PState pre = s;
     int ev = mcc::EV_NONE, st = -1;
     mcc::Child ch{};
@@ -731,7 +693,12 @@ __device__ __forceinline__ void scatter_round(PState &s, uint32_t pid, float w, 
     if (valid) st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, done);
     kept = st == ST_KEPT;
     dep = kept && deposit_ok(s, pc.N, pc.mode);
-    ent = hash_insert(H, kept ? bin_key(s.jx, s.jy, pc.tilesX) >> pc.keyshift : 0u, kept, rank);
+    // the slot k_key_count counted: keyed by the cell before the step
+    claim = valid && pre.jx != CELL_HOLE && (do_push || kept);
+    ent = hash_insert(H, claim ? bin_key(pre.jx, pre.jy, pc.tilesX) >> pc.keyshift : 0u, claim,
+                      rank);
+    if (valid && st != ST_KEPT && st != ST_HOLE) atomicAdd(&sl[st - 1], 1u);
+    if (claim && !kept) atomicAdd(shole, 1u);
     if (mc.on) {
         if (EV_KIND(ev) > 0) atomicAdd(&se[EV_KIND(ev)], 1u);
         if (ev & EV_OVER_BOUND) atomicAdd(&se[6], 1u);
@@ -739,12 +706,13 @@ __device__ __forceinline__ void scatter_round(PState &s, uint32_t pid, float w, 
     stage_products(ev, ch, pre, st, valid, pid, E, pc, mc, step, sp, lost6, w, s);
 }
 
-// Pass 2 of the sort pass: the identical collide + push of IPP particles per
-// thread, one claim (atomicSub on the bin count, which runs back to zero) per
+// Pass 2 of the sort pass: the collide + push of IPP particles per thread,
+// one claim (atomicSub on the bin count, which runs back to zero) per
 // distinct key of the block, the deposit through one LDS window, the products
-// staged.  The survivors are then staged in LDS in slot order (each key's
-// claimed range contiguous) and written out by consecutive threads, so every
-// store instruction covers consecutive slots.  The 32 KB of LDS serve three
+// staged.  The claimed slots (survivors, and holes for the particles lost in
+// this step) are then staged in LDS in slot order (each key's claimed range
+// contiguous) and written out by consecutive threads, so every store
+// instruction covers consecutive slots.  The 32 KB of LDS serve three
 // phases: collision queue + hash table, deposit window, write staging.
 template <int V>
 __global__ void __launch_bounds__(256)
@@ -752,19 +720,21 @@ __global__ void __launch_bounds__(256)
                    const float2 *__restrict__ E, PushConst pc, mcc::MccConst mc, uint32_t step,
                    uint32_t *__restrict__ binCount, const uint32_t *__restrict__ binStart,
                    unsigned long long *__restrict__ q, int do_push, int deposit, Spawn sp,
-                   unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6) {
+                   unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6,
+                   uint32_t *__restrict__ holes) {
     pin_variant<V>(pc, mc, src);
     pin_spawn<V>(sp);
     if (V != PV_GENERIC) dst.w = nullptr;
     if (blockIdx.x * (blockDim.x * IPP) >= *nptr) return;  // past the live count
     __shared__ unsigned long long smem[2 * BLOCK_LDS_U64];
-    __shared__ uint32_t se[8], s_total;
+    __shared__ uint32_t se[8], sl[6], s_total, shole;
     static_assert(sizeof(BlockHash) <= BLOCK_LDS_U64 * sizeof(unsigned long long), "hash size");
     static_assert(8 * 256 * IPP <= 2 * BLOCK_LDS_U64 * 2, "staging size");
     unsigned long long *lds = smem;  // collision queue, then deposit window
     BlockHash &H = *reinterpret_cast<BlockHash *>(smem + BLOCK_LDS_U64);
     if (threadIdx.x < 8) se[threadIdx.x] = 0;
-    if (threadIdx.x == 0) s_total = 0;
+    if (threadIdx.x < 6) sl[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_total = shole = 0;
     hash_clear(H);
     const uint32_t n = *nptr;
     const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
@@ -772,7 +742,7 @@ __global__ void __launch_bounds__(256)
     PState s[IPP];
     uint32_t pid[IPP], ent[IPP], rank[IPP];
     float w[IPP];
-    bool kept[IPP], dep[IPP];
+    bool claim[IPP], kept[IPP], dep[IPP];
 #pragma unroll
     for (int r = 0; r < IPP; r++) {  // every load first
         const uint32_t i = base + r * blockDim.x;
@@ -794,14 +764,18 @@ __global__ void __launch_bounds__(256)
         block_collide_n<IPP>(act, s, pid, mc, step, qs, lds);
     }
     // the rounds are spelled out so every array index is a compile-time constant
-    scatter_round(s[0], pid[0], w[0], base + 0 * blockDim.x < n, kept[0], dep[0], ent[0], rank[0],
-                  E, pc, mc, step, do_push, blk, qs[0], H, se, lds, sp, lost6);
-    scatter_round(s[1], pid[1], w[1], base + 1 * blockDim.x < n, kept[1], dep[1], ent[1], rank[1],
-                  E, pc, mc, step, do_push, blk, qs[1], H, se, lds, sp, lost6);
-    scatter_round(s[2], pid[2], w[2], base + 2 * blockDim.x < n, kept[2], dep[2], ent[2], rank[2],
-                  E, pc, mc, step, do_push, blk, qs[2], H, se, lds, sp, lost6);
-    scatter_round(s[3], pid[3], w[3], base + 3 * blockDim.x < n, kept[3], dep[3], ent[3], rank[3],
-                  E, pc, mc, step, do_push, blk, qs[3], H, se, lds, sp, lost6);
+    scatter_round(s[0], pid[0], w[0], base + 0 * blockDim.x < n, claim[0], kept[0], dep[0],
+                  ent[0], rank[0], E, pc, mc, step, do_push, blk, qs[0], H, se, sl, &shole,
+                  lds, sp, lost6);
+    scatter_round(s[1], pid[1], w[1], base + 1 * blockDim.x < n, claim[1], kept[1], dep[1],
+                  ent[1], rank[1], E, pc, mc, step, do_push, blk, qs[1], H, se, sl, &shole,
+                  lds, sp, lost6);
+    scatter_round(s[2], pid[2], w[2], base + 2 * blockDim.x < n, claim[2], kept[2], dep[2],
+                  ent[2], rank[2], E, pc, mc, step, do_push, blk, qs[2], H, se, sl, &shole,
+                  lds, sp, lost6);
+    scatter_round(s[3], pid[3], w[3], base + 3 * blockDim.x < n, claim[3], kept[3], dep[3],
+                  ent[3], rank[3], E, pc, mc, step, do_push, blk, qs[3], H, se, sl, &shole,
+                  lds, sp, lost6);
     __syncthreads();  // every round has read its collision result
     // one claim per distinct key: the bin count runs down from its total, so
     // the block's c particles of key k take the ascending slots
@@ -812,7 +786,7 @@ __global__ void __launch_bounds__(256)
         const uint32_t k = H.key[e];
         if (k != HASH_EMPTY) {
             const uint32_t c = H.cnt[e];
-            const uint32_t old = atomicSub(&binCount[k], c);
+            const uint32_t old = atomicSub(&binCount[(size_t)k * BIN_STRIDE], c);
             H.key[e] = binStart[k] + old - c;
             H.cnt[e] = atomicAdd(&s_total, c);
         }
@@ -821,8 +795,8 @@ __global__ void __launch_bounds__(256)
     uint32_t slot[IPP], lidx[IPP];
 #pragma unroll
     for (int r = 0; r < IPP; r++) {
-        slot[r] = kept[r] ? H.key[ent[r]] + rank[r] : 0xFFFFFFFFu;
-        lidx[r] = kept[r] ? H.cnt[ent[r]] + rank[r] : 0u;
+        slot[r] = claim[r] ? H.key[ent[r]] + rank[r] : 0xFFFFFFFFu;
+        lidx[r] = claim[r] ? H.cnt[ent[r]] + rank[r] : 0u;
     }
     __syncthreads();  // the table is read; the deposit window reuses the LDS
     if (deposit) block_deposit_n<IPP>(dep, s, w, q, pc.N, lds);
@@ -831,10 +805,10 @@ __global__ void __launch_bounds__(256)
     constexpr uint32_t SN = 256 * IPP;
 #pragma unroll
     for (int r = 0; r < IPP; r++) {
-        if (kept[r]) {
+        if (claim[r]) {  // a claimed slot of a particle that left holds a hole
             const uint32_t l = lidx[r];
             stg[l] = slot[r];
-            stg[SN + l] = cell_pack(s[r].jx, s[r].jy);
+            stg[SN + l] = kept[r] ? cell_pack(s[r].jx, s[r].jy) : cell_pack(CELL_HOLE, CELL_HOLE);
             stg[2 * SN + l] = __float_as_uint(s[r].ox);
             stg[3 * SN + l] = __float_as_uint(s[r].oy);
             stg[4 * SN + l] = __float_as_uint(s[r].vx);
@@ -862,6 +836,9 @@ __global__ void __launch_bounds__(256)
         if (threadIdx.x < 8 && se[threadIdx.x])
             atomicAdd(&events[cnt_slot() * 8 + threadIdx.x], (unsigned long long)se[threadIdx.x]);
     }
+    if (threadIdx.x < 6 && sl[threadIdx.x])
+        atomicAdd(&lost6[cnt_slot() * 6 + threadIdx.x], sl[threadIdx.x]);
+    if (threadIdx.x == 0 && shole) atomicAdd(&holes[cnt_slot()], shole);
 }
 
 
@@ -1574,23 +1551,21 @@ static int pass_variant(const SpView &sp, const PushConst &pc, const mcc::MccCon
     default: hipLaunchKernelGGL(kern<PV_GENERIC>, grid, dim3(256), 0, __VA_ARGS__); break;   \
     }
 
-void launch_push_count(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
-                       const float2 *E, const PushConst &pc, const mcc::MccConst &mc, uint32_t step,
-                       uint32_t *binCount, uint32_t *lost6, int do_push) {
-    const int v = pass_variant(sp, pc, mc, nullptr);
-    LAUNCH_VARIANT(v, k_push_count, dim3(blocks_for(cap, 256 * IPP)), st, sp, n, E, pc, mc, step,
-                   binCount, lost6, do_push);
+void launch_key_count(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                      const PushConst &pc, uint32_t *binCount, int do_push) {
+    hipLaunchKernelGGL(k_key_count, dim3(blocks_for(cap, 256 * IPP)), dim3(256), 0, st, sp, n, pc,
+                       binCount, do_push);
 }
 
 void launch_push_scatter(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
                          int64_t cap, const float2 *E, const PushConst &pc, const mcc::MccConst &mc,
                          uint32_t step, uint32_t *binCount, const uint32_t *binStart,
                          unsigned long long *q, int do_push, int deposit, const Spawn &spn,
-                         unsigned long long *events, uint32_t *lost6) {
+                         unsigned long long *events, uint32_t *lost6, uint32_t *holes) {
     const int v = (dst.w == nullptr) ? pass_variant(src, pc, mc, &spn) : PV_GENERIC;
     LAUNCH_VARIANT(v, k_push_scatter, dim3(blocks_for(cap, 256 * IPP)), st, src, dst, n,
                    (uint32_t)cap, E, pc, mc, step, binCount, binStart, q, do_push, deposit, spn,
-                   events, lost6);
+                   events, lost6, holes);
 }
 
 void launch_push_inplace(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,

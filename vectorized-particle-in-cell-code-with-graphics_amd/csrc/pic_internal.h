@@ -108,14 +108,17 @@ struct Spawn {
 };
 
 // ---- k_particles.hip
-void launch_push_count(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
-                       const float2 *E, const PushConst &pc, const mcc::MccConst &mc, uint32_t step,
-                       uint32_t *binCount, uint32_t *lost6, int do_push);
+// words between consecutive counters of the sort histogram (binCount): each
+// key's counter has a 128-byte line of its own, so the per-block atomics of
+// different keys do not queue on a shared line
+#define BIN_STRIDE 32
+void launch_key_count(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                      const PushConst &pc, uint32_t *binCount, int do_push);
 void launch_push_scatter(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
                          int64_t cap, const float2 *E, const PushConst &pc, const mcc::MccConst &mc,
                          uint32_t step, uint32_t *binCount, const uint32_t *binStart,
                          unsigned long long *q, int do_push, int deposit, const Spawn &spn,
-                         unsigned long long *events, uint32_t *lost6);
+                         unsigned long long *events, uint32_t *lost6, uint32_t *holes);
 void launch_push_inplace(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
                          const float2 *E, const PushConst &pc, const mcc::MccConst &mc,
                          uint32_t step, unsigned long long *q, int deposit, const Spawn &spn,
