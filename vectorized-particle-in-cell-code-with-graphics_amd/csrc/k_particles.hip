@@ -781,15 +781,31 @@ __global__ void __launch_bounds__(256)
     // the block's c particles of key k take the ascending slots
     // start(k) + old - c + rank, rank < c, and the staging positions
     // lbase + rank (a corrupt histogram, old < c, gives slots >= cap, which
-    // are skipped)
-    for (int e = threadIdx.x; e < HASH_SIZE; e += blockDim.x) {
-        const uint32_t k = H.key[e];
-        if (k != HASH_EMPTY) {
-            const uint32_t c = H.cnt[e];
-            const uint32_t old = atomicSub(&binCount[(size_t)k * BIN_STRIDE], c);
-            H.key[e] = binStart[k] + old - c;
-            H.cnt[e] = atomicAdd(&s_total, c);
+    // are skipped).  Each thread takes HASH_SIZE / 256 entries and issues all
+    // of its claims before it uses the first result, so their latencies
+    // overlap.
+    {
+        constexpr int EPT = HASH_SIZE / 256;
+        uint32_t kk[EPT], cc[EPT], old[EPT], st0[EPT];
+#pragma unroll
+        for (int j = 0; j < EPT; j++) {
+            kk[j] = H.key[threadIdx.x + j * 256];
+            cc[j] = H.cnt[threadIdx.x + j * 256];
         }
+#pragma unroll
+        for (int j = 0; j < EPT; j++) {
+            old[j] = st0[j] = 0u;
+            if (kk[j] != HASH_EMPTY) {
+                old[j] = atomicSub(&binCount[(size_t)kk[j] * BIN_STRIDE], cc[j]);
+                st0[j] = binStart[kk[j]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < EPT; j++)
+            if (kk[j] != HASH_EMPTY) {
+                H.key[threadIdx.x + j * 256] = st0[j] + old[j] - cc[j];
+                H.cnt[threadIdx.x + j * 256] = atomicAdd(&s_total, cc[j]);
+            }
     }
     __syncthreads();
     uint32_t slot[IPP], lidx[IPP];
