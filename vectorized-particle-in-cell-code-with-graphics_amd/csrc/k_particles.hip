@@ -299,24 +299,41 @@ __device__ __forceinline__ void pin_spawn(Spawn &spn) {
     }
 }
 
+// wavefront reductions of the device library (DPP row shifts and lane reads:
+// no LDS round trips); every lane gets the result.  Called with all lanes
+// active.
+extern "C" __device__ int __ockl_wfred_min_i32(int);
+extern "C" __device__ int __ockl_wfred_max_i32(int);
+extern "C" __device__ int __ockl_wfred_add_i32(int);
+__device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
+__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
+__device__ __forceinline__ int wave_sum(int v) { return __ockl_wfred_add_i32(v); }
+
 // Pass 1 of the sort pass: the histogram of the sort keys of the particles'
-// current cells, aggregated per block through the LDS hash table (one global
-// atomic per distinct key per block).  In a timestep's sort pass (do_push)
-// every particle that is not a hole takes a slot, keyed by its cell before the
-// step: one that leaves during the step leaves a hole in its slot, which the
-// next sort pass drops.  Without a push (compaction) only the particles
-// inside the domain are counted, so the output is compact.  Reads 4 B per
-// particle (12 B without a push).
+// current cells, one global atomic per distinct key per block.  In a
+// timestep's sort pass (do_push) every particle that is not a hole takes a
+// slot, keyed by its cell before the step: one that leaves during the step
+// leaves a hole in its slot, which the next sort pass drops.  Without a push
+// (compaction) only the particles inside the domain are counted, so the
+// output is compact.  Reads 4 B per particle (12 B without a push).  The
+// keys of a block of a sorted store lie in a narrow range (a few rows of
+// tiles): then the block counts them in a direct LDS histogram over that
+// range (one LDS atomic per particle); a wider range (an unsorted store)
+// takes the LDS hash table.
+#define KC_WIN (2 * HASH_SIZE)  // direct-histogram range: the hash table's LDS
 __global__ void __launch_bounds__(256)
     k_key_count(SpView sp, const uint32_t *__restrict__ nptr, PushConst pc,
                 uint32_t *__restrict__ binCount, int do_push) {
     const uint32_t n = *nptr;
     if (blockIdx.x * (blockDim.x * IPP) >= n) return;  // past the live count
-    __shared__ BlockHash H;
-    hash_clear(H);
+    __shared__ uint32_t hist[KC_WIN];  // the direct histogram or the hash table
+    __shared__ int wmin[4], wmax[4];
+    static_assert(sizeof(BlockHash) == sizeof(hist), "histogram size");
+    BlockHash &H = *reinterpret_cast<BlockHash *>(hist);
     const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
     uint32_t key[IPP];
     bool in[IPP];
+    int kmin = 0x7FFFFFFF, kmax = -1;
 #pragma unroll
     for (int r = 0; r < IPP; r++) {
         const uint32_t i = base + r * blockDim.x;
@@ -334,9 +351,38 @@ __global__ void __launch_bounds__(256)
                 s.oy = sp.oy[i];
                 in[r] = classify(s, pc.N, pc.mode) == ST_KEPT;
             }
-            if (in[r]) key[r] = bin_key(s.jx, s.jy, pc.tilesX) >> pc.keyshift;
+            if (in[r]) {
+                key[r] = bin_key(s.jx, s.jy, pc.tilesX) >> pc.keyshift;
+                kmin = min(kmin, (int)key[r]);
+                kmax = max(kmax, (int)key[r]);
+            }
         }
     }
+    kmin = wave_min(kmin);
+    kmax = wave_max(kmax);
+    if ((threadIdx.x & 63) == 0) {
+        wmin[threadIdx.x >> 6] = kmin;
+        wmax[threadIdx.x >> 6] = kmax;
+    }
+    __syncthreads();
+    kmin = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
+    kmax = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+    if (kmax < kmin) return;  // no particle counted (uniform)
+    if (kmax - kmin < KC_WIN) {
+        const int span = kmax - kmin + 1;
+        for (int e = threadIdx.x; e < span; e += blockDim.x) hist[e] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < IPP; r++)
+            if (in[r]) atomicAdd(&hist[key[r] - (uint32_t)kmin], 1u);
+        __syncthreads();
+        for (int e = threadIdx.x; e < span; e += blockDim.x) {
+            const uint32_t c = hist[e];
+            if (c) atomicAdd(&binCount[(size_t)(kmin + e) * BIN_STRIDE], c);
+        }
+        return;
+    }
+    hash_clear(H);
     __syncthreads();  // table cleared
 #pragma unroll
     for (int r = 0; r < IPP; r++) {
@@ -471,15 +517,6 @@ uint32_t scan_pad(uint32_t nbins) { return (nbins + SCAN_TILE - 1) / SCAN_TILE *
 // ------------------------------------------------------------ pass 2
 #define DEP_WCAP BLOCK_LDS_U64  // LDS window capacity (nodes), 16 KB of uint64
 
-// wavefront reductions of the device library (DPP row shifts and lane reads:
-// no LDS round trips); every lane gets the result.  Called with all lanes
-// active.
-extern "C" __device__ int __ockl_wfred_min_i32(int);
-extern "C" __device__ int __ockl_wfred_max_i32(int);
-extern "C" __device__ int __ockl_wfred_add_i32(int);
-__device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
-__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
-__device__ __forceinline__ int wave_sum(int v) { return __ockl_wfred_add_i32(v); }
 
 // Deposit the block's particles (dep true) through an LDS window spanning the
 // bounding box of their cells (block reduction), flushed with one 64-bit
