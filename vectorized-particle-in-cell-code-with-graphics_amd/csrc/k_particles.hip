@@ -196,6 +196,10 @@ __device__ __forceinline__ bool collide_take(bool blk, int qs, PState &s, int &e
 #define HASH_BITS 11
 #define HASH_SIZE (1 << HASH_BITS)  // >= 2 x 256 x IPP
 #define HASH_EMPTY 0xFFFFFFFFu
+// direct key table of the scatter pass: key ranges up to SCAT_WIN entries,
+// SCAT_EPT entries per thread of the 256
+#define SCAT_WIN 1024
+#define SCAT_EPT (SCAT_WIN / 256)
 
 struct BlockHash {
     uint32_t key[HASH_SIZE];   // pass 2, after the claims: slot of the entry's rank-0 particle
@@ -716,11 +720,10 @@ __device__ __forceinline__ void stage_products(int ev, const mcc::Child &ch, con
 
 // one particle of k_push_scatter (before the claims)
 __device__ __forceinline__ void scatter_round(PState &s, uint32_t pid, float w, bool valid,
-                                              bool &claim, bool &kept, bool &dep, uint32_t &ent,
-                                              uint32_t &rank, const float2 *__restrict__ E,
-                                              const PushConst &pc, const mcc::MccConst &mc,
-                                              uint32_t step, int do_push, bool blk, int qs,
-                                              BlockHash &H, uint32_t *se, uint32_t *sl,
+                                              bool claim, bool &kept, bool &dep,
+                                              const float2 *__restrict__ E, const PushConst &pc,
+                                              const mcc::MccConst &mc, uint32_t step, int do_push,
+                                              bool blk, int qs, uint32_t *se, uint32_t *sl,
                                               uint32_t *shole, const unsigned long long *lds,
                                               const Spawn &sp, uint32_t *__restrict__ lost6) {
     const PState pre = s;
@@ -730,10 +733,6 @@ __device__ __forceinline__ void scatter_round(PState &s, uint32_t pid, float w, 
     if (valid) st = advance_one(s, pid, E, pc, mc, step, do_push, ev, ch, done);
     kept = st == ST_KEPT;
     dep = kept && deposit_ok(s, pc.N, pc.mode);
-    // the slot k_key_count counted: keyed by the cell before the step
-    claim = valid && pre.jx != CELL_HOLE && (do_push || kept);
-    ent = hash_insert(H, claim ? bin_key(pre.jx, pre.jy, pc.tilesX) >> pc.keyshift : 0u, claim,
-                      rank);
     if (valid && st != ST_KEPT && st != ST_HOLE) atomicAdd(&sl[st - 1], 1u);
     if (claim && !kept) atomicAdd(shole, 1u);
     if (mc.on) {
@@ -746,7 +745,10 @@ __device__ __forceinline__ void scatter_round(PState &s, uint32_t pid, float w, 
 // Pass 2 of the sort pass: the collide + push of IPP particles per thread,
 // one claim (atomicSub on the bin count, which runs back to zero) per
 // distinct key of the block, the deposit through one LDS window, the products
-// staged.  The claimed slots (survivors, and holes for the particles lost in
+// staged.  The keys are the cells before the step, so the ranks and the
+// claims come first and the claims' latency overlaps the collisions and the
+// push.  A block of a sorted store ranks its particles in a direct LDS
+// table over its narrow key range; a wider range takes the hash table.  The claimed slots (survivors, and holes for the particles lost in
 // this step) are then staged in LDS in slot order (each key's claimed range
 // contiguous) and written out by consecutive threads, so every store
 // instruction covers consecutive slots.  The 32 KB of LDS serve three
@@ -772,7 +774,7 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x < 8) se[threadIdx.x] = 0;
     if (threadIdx.x < 6) sl[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_total = shole = 0;
-    hash_clear(H);
+    __shared__ int wmin[4], wmax[4];
     const uint32_t n = *nptr;
     const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
     const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
@@ -780,6 +782,7 @@ __global__ void __launch_bounds__(256)
     uint32_t pid[IPP], ent[IPP], rank[IPP];
     float w[IPP];
     bool claim[IPP], kept[IPP], dep[IPP];
+    int kmin = 0x7FFFFFFF, kmax = -1;
 #pragma unroll
     for (int r = 0; r < IPP; r++) {  // every load first
         const uint32_t i = base + r * blockDim.x;
@@ -791,8 +794,63 @@ __global__ void __launch_bounds__(256)
             pid[r] = src.id[i];
             w[r] = load_w(src, i);
         }
+        // the slot k_key_count counted, keyed by the cell before the step
+        // (without a push nothing moves: the particles inside keep slots)
+        claim[r] = i < n && s[r].jx != CELL_HOLE &&
+                   (do_push || classify(s[r], pc.N, pc.mode) == ST_KEPT);
+        ent[r] = claim[r] ? bin_key(s[r].jx, s[r].jy, pc.tilesX) >> pc.keyshift : 0u;
+        if (claim[r]) {
+            kmin = min(kmin, (int)ent[r]);
+            kmax = max(kmax, (int)ent[r]);
+        }
     }
-    __syncthreads();  // table and counters cleared
+    kmin = wave_min(kmin);
+    kmax = wave_max(kmax);
+    if ((threadIdx.x & 63) == 0) {
+        wmin[threadIdx.x >> 6] = kmin;
+        wmax[threadIdx.x >> 6] = kmax;
+    }
+    __syncthreads();  // counters cleared, key range known
+    kmin = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
+    kmax = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+    // direct table: entry = key - kmin, H.cnt the per-key count, then the
+    // staging base; H.key the slot base
+    const int span = kmax - kmin + 1;  // <= 0: no claims
+    const bool direct = span <= SCAT_WIN;
+    if (direct) {
+        for (int e = threadIdx.x; e < span; e += blockDim.x) H.cnt[e] = 0u;
+    } else {
+        hash_clear(H);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < IPP; r++) {
+        if (direct) {
+            ent[r] -= (uint32_t)kmin;
+            rank[r] = claim[r] ? atomicAdd(&H.cnt[ent[r]], 1u) : 0u;
+        } else {
+            ent[r] = hash_insert(H, ent[r], claim[r], rank[r]);
+        }
+    }
+    __syncthreads();
+    // one claim per distinct key: the bin count runs down from its total, so
+    // the block's c particles of key k take the slots start(k) + old - c +
+    // rank, rank < c (a corrupt histogram, old < c, gives slots >= cap, which
+    // are skipped).  The claims of a direct table are issued here and their
+    // results used after the push.
+    uint32_t old[SCAT_EPT], st0[SCAT_EPT];
+#pragma unroll
+    for (int j = 0; j < SCAT_EPT; j++) {
+        const int e = threadIdx.x + j * 256;
+        old[j] = st0[j] = 0u;
+        if (direct && e < span) {
+            const uint32_t c = H.cnt[e];
+            if (c) {
+                old[j] = atomicSub(&binCount[(size_t)(kmin + e) * BIN_STRIDE], c);
+                st0[j] = binStart[kmin + e];
+            }
+        }
+    }
     int qs[IPP];
     if (blk) {
         bool act[IPP];
@@ -801,27 +859,30 @@ __global__ void __launch_bounds__(256)
         block_collide_n<IPP>(act, s, pid, mc, step, qs, lds);
     }
     // the rounds are spelled out so every array index is a compile-time constant
-    scatter_round(s[0], pid[0], w[0], base + 0 * blockDim.x < n, claim[0], kept[0], dep[0],
-                  ent[0], rank[0], E, pc, mc, step, do_push, blk, qs[0], H, se, sl, &shole,
-                  lds, sp, lost6);
-    scatter_round(s[1], pid[1], w[1], base + 1 * blockDim.x < n, claim[1], kept[1], dep[1],
-                  ent[1], rank[1], E, pc, mc, step, do_push, blk, qs[1], H, se, sl, &shole,
-                  lds, sp, lost6);
-    scatter_round(s[2], pid[2], w[2], base + 2 * blockDim.x < n, claim[2], kept[2], dep[2],
-                  ent[2], rank[2], E, pc, mc, step, do_push, blk, qs[2], H, se, sl, &shole,
-                  lds, sp, lost6);
-    scatter_round(s[3], pid[3], w[3], base + 3 * blockDim.x < n, claim[3], kept[3], dep[3],
-                  ent[3], rank[3], E, pc, mc, step, do_push, blk, qs[3], H, se, sl, &shole,
-                  lds, sp, lost6);
+    scatter_round(s[0], pid[0], w[0], base + 0 * blockDim.x < n, claim[0], kept[0],
+                  dep[0], E, pc, mc, step, do_push, blk, qs[0], se, sl, &shole, lds, sp, lost6);
+    scatter_round(s[1], pid[1], w[1], base + 1 * blockDim.x < n, claim[1], kept[1],
+                  dep[1], E, pc, mc, step, do_push, blk, qs[1], se, sl, &shole, lds, sp, lost6);
+    scatter_round(s[2], pid[2], w[2], base + 2 * blockDim.x < n, claim[2], kept[2],
+                  dep[2], E, pc, mc, step, do_push, blk, qs[2], se, sl, &shole, lds, sp, lost6);
+    scatter_round(s[3], pid[3], w[3], base + 3 * blockDim.x < n, claim[3], kept[3],
+                  dep[3], E, pc, mc, step, do_push, blk, qs[3], se, sl, &shole, lds, sp, lost6);
     __syncthreads();  // every round has read its collision result
-    // one claim per distinct key: the bin count runs down from its total, so
-    // the block's c particles of key k take the ascending slots
-    // start(k) + old - c + rank, rank < c, and the staging positions
-    // lbase + rank (a corrupt histogram, old < c, gives slots >= cap, which
-    // are skipped).  Each thread takes HASH_SIZE / 256 entries and issues all
-    // of its claims before it uses the first result, so their latencies
-    // overlap.
-    {
+    if (direct) {
+#pragma unroll
+        for (int j = 0; j < SCAT_EPT; j++) {
+            const int e = threadIdx.x + j * 256;
+            if (e < span) {
+                const uint32_t c = H.cnt[e];
+                if (c) {
+                    H.key[e] = st0[j] + old[j] - c;
+                    H.cnt[e] = atomicAdd(&s_total, c);
+                }
+            }
+        }
+    } else {
+        // hash table: each thread takes HASH_SIZE / 256 entries and issues
+        // all of its claims before it uses the first result
         constexpr int EPT = HASH_SIZE / 256;
         uint32_t kk[EPT], cc[EPT], old[EPT], st0[EPT];
 #pragma unroll

@@ -222,11 +222,13 @@ __device__ __forceinline__ void dep_global(unsigned long long *q, int N, int jx,
 
 // counting-sort key on the (N+1)^2 halo grid (jx, jy in [-1, N-1]): one bin
 // per cell, ordered tile-major with 8x8 tiles so a contiguous key range is a
-// compact patch.  Per-cell bins spread the histogram and slot-claim atomics
-// over many addresses (whole-tile bins serialise them: measured 2x slower at
-// 200 particles per cell).
+// compact patch (the sort key drops the low bits: one bin per tile).  A cell
+// off the tile grid (a particle the steady-state walls leave outside) takes
+// the nearest edge cell's key, so every key stays inside the histogram.
 __device__ __forceinline__ uint32_t bin_key(int jx, int jy, int tilesX) {
-    uint32_t hx = (uint32_t)(jx + 1), hy = (uint32_t)(jy + 1);
+    const int top = 8 * tilesX - 1;
+    const uint32_t hx = (uint32_t)min(max(jx + 1, 0), top);
+    const uint32_t hy = (uint32_t)min(max(jy + 1, 0), top);
     return ((((hy >> 3) * (uint32_t)tilesX + (hx >> 3)) << 6) | ((hy & 7u) << 3) | (hx & 7u));
 }
 
