@@ -223,15 +223,21 @@ def test_sort_interval_does_not_change_results(mode):
     assert out[1]["le"][:4].sum() > 0
 
 
-def test_reference_steady_walls_match_the_oracle():
+@pytest.mark.parametrize("sort_interval", [0, 1])
+def test_reference_steady_walls_match_the_oracle(sort_interval):
     """REF mode with the steady-state walls: nobody leaves; each step the
     outside electrons and ions are paired per side in id order and
     re-injected.  Cells, offsets, ids and charge are bit-exact against the
     oracle after each step run from the same state; the re-injected
     velocities (fp32 cos/log of the device and of the host libm) agree to
-    a few ulp."""
-    p = workloads.discharge(32, 40, mode=pic2d.MODE_REF, mg_mode=pic2d.MG_FIXED, te=8.0)
+    a few ulp.  With a sort pass every step and hot electrons, the unpaired
+    particles that stay outside, some more than a cell beyond a wall, are
+    sorted too (their sort keys are clamped to the tile grid)."""
+    te = 8.0 if sort_interval == 0 else 80.0
+    p = workloads.discharge(32, 40, mode=pic2d.MODE_REF, mg_mode=pic2d.MG_FIXED, te=te)
     p.wall = pic2d.WALL_STEADY
+    p.sort_interval = sort_interval
+    print(f"steady walls, sort interval {sort_interval}", flush=True)
     p.sigma_load_i = 3.0e5   # hot ions: ions leave too, so pairs form on every side
     with pic2d.Pic(p) as sim:
         c32, _ = orc.configs_from_params(p)
@@ -262,6 +268,13 @@ def test_reference_steady_walls_match_the_oracle():
             pairs += int(moved.sum())
         assert pairs > 0
         assert sim.loss_counts(0).sum() == 0 and sim.loss_counts(1).sum() == 0
+        far = 0
+        for st in (sim.get_state(0), sim.get_state(1)):
+            jx, jy = pic2d.cell_unpack(st["cell"])
+            far += int(((jx < -1) | (jx > p.nodes) | (jy < -1) | (jy > p.nodes)).sum())
+        print(f"particles more than a cell outside: {far}", flush=True)
+        if sort_interval == 1:
+            assert far > 0
 
 
 def test_batch_diagnostics_match_the_oracle():
