@@ -118,10 +118,11 @@ int alloc_mg(pic_handle *h, MgLevels &M, int N) {
     CHECK(dalloc(h, &M.flags, 4));
     HIPCHK(h, hipMemset(M.flags, 0, 4 * sizeof(int)));
     CHECK(dalloc(h, &M.ualt, nf));
-    HIPCHK(h, hipHostMalloc((void **)&M.relres, sizeof(double), 0));
-    HIPCHK(h, hipHostMalloc((void **)&M.host_flags, 2 * sizeof(int), 0));
+    // coherent: the device's writes reach the host while kernels still run
+    HIPCHK(h, hipHostMalloc((void **)&M.relres, sizeof(double), hipHostMallocCoherent));
+    HIPCHK(h, hipHostMalloc((void **)&M.host_flags, 4 * sizeof(int), hipHostMallocCoherent));
     *M.relres = 0.0;
-    M.host_flags[0] = M.host_flags[1] = 0;
+    M.host_flags[0] = M.host_flags[1] = M.host_flags[2] = M.host_flags[3] = 0;
     HIPCHK(h, hipEventCreateWithFlags(&M.done, hipEventDisableTiming));
     return PIC_OK;
 }

@@ -64,7 +64,7 @@ __global__ void k_field_start(const unsigned long long *__restrict__ qi,
                               double *__restrict__ uprev, int extrap, int *__restrict__ flags) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y;
-    if (flags && i == 0 && j == 0) flags[0] = flags[1] = flags[2] = 0;
+    if (flags && i == 0 && j == 0) flags[0] = flags[1] = flags[2] = flags[3] = 0;
     if (i >= N) return;
     // k_rho
     bool ex = (i == 0 || i == N - 1), ey = (j == 0 || j == N - 1);
@@ -986,7 +986,9 @@ __device__ __forceinline__ double block_sum_d(double v) {
 // the last of the nb blocks to finish (ticket counter flags[2]) sums the
 // partials in block order, sets flags[0] when ||r|| <= tol ||f||, counts an
 // applied cycle in flags[1] when `applied`, and publishes both and the
-// relative residual to the host-mapped copies.
+// relative residual to the host-mapped copies, then (after a system-scope
+// fence) the count of residual passes of this solve (flags[3]) in
+// host_flags[2], on which the host waits.
 __device__ __forceinline__ void tol_finish(double rr, double ff, int b, int nb,
                                            double *__restrict__ partials, double tol,
                                            int *__restrict__ flags, bool applied,
@@ -1020,6 +1022,9 @@ __device__ __forceinline__ void tol_finish(double rr, double ff, int b, int nb,
         *relres_host = rel;
         host_flags[0] = flags[0];
         host_flags[1] = flags[1];
+        __threadfence_system();
+        flags[3] += 1;
+        host_flags[2] = flags[3];
     }
 }
 
@@ -1144,6 +1149,7 @@ __global__ void k_tol_reset(int *flags) {
     flags[0] = 0;
     flags[1] = 0;
     flags[2] = 0;
+    flags[3] = 0;
 }
 
 // one refinement cycle above the coarsest level: the V-cycle on the
@@ -1180,14 +1186,24 @@ static void vcycle_top(hipStream_t st, MgLevels &M, int j, const float *r_in, co
 // Wait for the queued cycles: the solve's host check sits between two phases
 // of the step, so the wake-up latency of a blocking wait is idle GPU time;
 // poll the batch's event for up to 2 ms, then block (also reports errors).
-static hipError_t wait_batch(hipStream_t st, hipEvent_t ev) {
-    hipError_t e = hipEventRecord(ev, st);
+// Waits for the queued cycles' verdict: the device has published (host_flags
+// in coherent host memory, the pass count last) either convergence or all
+// `passes` residual passes of the solve.  The host reads the count as soon
+// as the last pass's final block writes it, so the next launches queue up
+// behind the work still running (phi and E) instead of after an idle GPU.
+// After 2 ms of spinning (or if the device never publishes, e.g. on a
+// fault) it blocks on the stream's event, which also reports errors.
+static hipError_t wait_published(hipStream_t st, MgLevels &M, int passes) {
+    hipError_t e = hipEventRecord(M.done, st);
     if (e != hipSuccess) return e;
     const auto t0 = std::chrono::steady_clock::now();
-    while ((e = hipEventQuery(ev)) == hipErrorNotReady)
-        if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(2000))
-            return hipEventSynchronize(ev);
-    return e;
+    for (;;) {
+        const int p = __atomic_load_n(&M.host_flags[2], __ATOMIC_ACQUIRE);
+        if (p >= passes || (p > 0 && __atomic_load_n(&M.host_flags[0], __ATOMIC_ACQUIRE)))
+            return hipSuccess;
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(2000)) break;
+    }
+    return hipEventSynchronize(M.done);
 }
 
 int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max_iter,
@@ -1226,6 +1242,10 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
                        rb[(queued + 1) & 1], tol, length);
         }
     };
+    // nothing on the device writes the host copies now (the last solve's
+    // remaining cycles were skipped): clear them for this solve's verdict
+    __atomic_store_n(&M.host_flags[2], 0, __ATOMIC_RELEASE);
+    M.host_flags[0] = M.host_flags[1] = 0;
     if (!flags_reset) hipLaunchKernelGGL(k_tol_reset, dim3(1), dim3(1), 0, st, M.flags);
     hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st, (const double *)u,
                        (const float *)nullptr, (double *)nullptr, f, n, h2i, r32, M.partials, tol,
@@ -1236,7 +1256,8 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     for (;;) {
         for (int k = 0; k < batch && queued < max_iter; k++, queued++) cycle();
         if (after_batch) after_batch(buf[0], buf[1]);
-        if (wait_batch(st, M.done) != hipSuccess) return -1;
+        // the initial residual pass plus one per queued cycle
+        if (wait_published(st, M, 1 + queued) != hipSuccess) return -1;
         if (M.host_flags[0] || queued >= max_iter) break;
         batch = 1;
     }
