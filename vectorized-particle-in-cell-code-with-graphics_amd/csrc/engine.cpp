@@ -223,6 +223,8 @@ void species_free(Species &s) {
     for (void *p : wq)
         if (p) hipFree(p);
     if (s.rstate) hipFree(s.rstate);
+    if (s.mq) hipFree(s.mq);
+    if (s.mq_cnt) hipFree(s.mq_cnt);
     if (s.q) hipFree(s.q);
     if (s.exp_) hipFree(s.exp_);
     if (s.eyp) hipFree(s.eyp);
@@ -381,6 +383,9 @@ Spawn make_spawn(pic_handle *h, bool plop, int si) {
     sp.si_sigma = (float)h->P.vti;
     sp.plop_ref = 0;
     sp.wall = (h->wall_on && si >= 0) ? 1 : 0;
+    sp.lost_ids = 0;
+    sp.mq = sp.mq_cnt = nullptr;
+    sp.mq_seg = 0;
     if (sp.wall) {
         sp.wq = h->sp[si].wq;
         sp.cw = &h->cnt[si];
@@ -414,6 +419,26 @@ void wall_pass(pic_handle *h, int si) {
 
 bool plop_on(pic_handle *h) {
     return h->P.wall == PIC_WALL_STEADY && h->P.mode == PIC_MODE_PHYSICAL;
+}
+
+// Spawn of a fused particle pass of species si.  With the physical-mode
+// steady walls the ion pass queues its ions lost through a wall in the wall
+// queue (unused by these walls otherwise) and plop_drain stages their
+// re-injected pairs after the pass.
+Spawn pass_spawn(pic_handle *h, int si, int do_push) {
+    Spawn sp = make_spawn(h, false, do_push ? si : -1);
+    if (do_push && si == 1 && plop_on(h)) {
+        sp.lost_ids = 1;
+        sp.wq = h->sp[1].wq;
+        sp.cw = &h->cnt[1];
+    }
+    return sp;
+}
+
+void plop_drain(pic_handle *h, int si, int do_push) {
+    if (!(do_push && si == 1 && plop_on(h))) return;
+    launch_plop_drain(h->st, h->sp[1].wq, &h->cnt[1], make_spawn(h, true, si), h->sp[1].mc,
+                      (uint32_t)h->step, h->N);
 }
 
 // the reference's steady-state walls (pairing of outside electrons and ions)
@@ -569,6 +594,16 @@ int read_counters(pic_handle *h) {
     return PIC_OK;
 }
 
+// Physical-mode collisions run in a pass of their own ahead of the particle
+// pass when few particles are candidates (the argon ions: a fraction of a
+// percent): that pass then has the registers of a collisionless one, for 8
+// B per particle read ahead.  Where many are (the electrons: ~13 %), the
+// candidates' velocities would cost a read and write of nearly every line,
+// and the particle pass collides them in its block queue instead.
+bool collide_ahead(const Species &s) {
+    return s.mc.on && s.mc.mode == PIC_DEV_MODE_PHYS && s.mc.p_null < 0.02f;
+}
+
 // counting sort + (optional) collisions and push + deposit of one species.
 // The keys are the cells at the start of the pass; with a push, the
 // particles lost in the pass leave holes in the sorted store (counted in the
@@ -580,7 +615,12 @@ void push_species(pic_handle *h, int si, int do_push, int deposit) {
     uint32_t step = (uint32_t)h->step;
     mcc::MccConst mc = s.mc;
     if (!do_push) mc.on = 0;
-    Spawn spn = make_spawn(h, si == 1 && do_push && plop_on(h), do_push ? si : -1);
+    Spawn spn = pass_spawn(h, si, do_push);
+    if (do_push && collide_ahead(s) && s.mq) {
+        launch_collide(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, mc, step, spn,
+                       &h->cnt[si].events[0][0], &h->cnt[si].lost[0][0], s.mq, s.mq_cnt);
+        mc.on = 0;
+    }
     clear_holes(h, si);  // the pass drops the old holes
     int p1 = prof_begin(h, 4);
     launch_key_count(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, s.pc, h->binCount, do_push);
@@ -594,6 +634,7 @@ void push_species(pic_handle *h, int si, int do_push, int deposit) {
                         &h->cnt[si].events[0][0], &h->cnt[si].lost[0][0],
                         &h->cnt[si].holes[0]);
     prof_end(h, p3);
+    plop_drain(h, si, do_push);
     s.cur = 1 - c;
     s.holes = do_push != 0;
     s.since_sort = 0;
@@ -604,12 +645,26 @@ void push_species(pic_handle *h, int si, int do_push, int deposit) {
 void push_species_inplace(pic_handle *h, int si) {
     Species &s = h->sp[si];
     int c = s.cur;
-    Spawn spn = make_spawn(h, si == 1 && plop_on(h), si);
+    Spawn spn = pass_spawn(h, si, 1);
+    // few candidates: the pass makes the candidate test only and leaves each
+    // candidate to k_collide_q (collide, push, store, deposit) after it
+    const bool defer = collide_ahead(s) && s.mq;
+    if (defer) {
+        launch_fill<uint32_t>(h->st, s.mq_cnt, MQ_SEGS * MQ_STRIDE, 0u);
+        spn.mq = s.mq;
+        spn.mq_cnt = s.mq_cnt;
+        spn.mq_seg = collide_seg_cap(s.cap);
+    }
     int p = prof_begin(h, 6);
     launch_push_inplace(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, s.mc,
                         (uint32_t)h->step, s.q, 1, spn, &h->cnt[si].events[0][0],
                         &h->cnt[si].lost[0][0], &h->cnt[si].holes[0]);
     prof_end(h, p);
+    if (defer)
+        launch_collide_deferred(h->st, s.buf[c], h->E32, s.pc, s.mc, (uint32_t)h->step, spn,
+                                &h->cnt[si].events[0][0], &h->cnt[si].lost[0][0], s.q,
+                                &h->cnt[si].holes[0]);
+    plop_drain(h, si, 1);
     s.holes = true;
     s.since_sort++;
 }
@@ -883,6 +938,11 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         update_scale(h, h->sp[s]);
         Species &sp = h->sp[s];
         if (h->wall_on && (rc = alloc_wall_queue(h, sp, n0[s])) != PIC_OK) return fail(rc);
+        // the physical-mode steady walls queue the lost ions there for
+        // re-injection (enable_weights adds the weights when needed)
+        if (!h->wall_on && s == 1 && plop_on(h) &&
+            (rc = alloc_wall_queue(h, sp, n0[s])) != PIC_OK)
+            return fail(rc);
         sp.reduce = P.reduce != 0;
         if (sp.reduce || n_inj > 0 || mixed[s])
             if ((rc = enable_weights(h, sp)) != PIC_OK) return fail(rc);
@@ -912,6 +972,13 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     }
     setup_mcc(h, h->sp[0], true);
     setup_mcc(h, h->sp[1], false);
+    for (int s = 0; s < 2; s++) {
+        Species &sp = h->sp[s];
+        if (!collide_ahead(sp)) continue;
+        if ((rc = dalloc(h, &sp.mq, (size_t)MQ_SEGS * collide_seg_cap(sp.cap))) != PIC_OK ||
+            (rc = dalloc(h, &sp.mq_cnt, (size_t)MQ_SEGS * MQ_STRIDE)) != PIC_OK)
+            return fail(rc);
+    }
     if (h->wall_on) {
         WallConst &c = h->wc;
         c.mode = P.mode == PIC_MODE_PHYSICAL ? PIC_DEV_MODE_PHYS : PIC_DEV_MODE_REF;
@@ -1472,9 +1539,20 @@ int pic_estimate_bytes(const pic_params *pp, int64_t *bytes) {
         if (wall_on) b += std::min<int64_t>(std::max<int64_t>(65536, n0 / 8), (int64_t)1 << 26) * 36;
         if (P.reduce) b += (int64_t)sizeof(ReduceState);
         if (s == 1 && P.have_collisions && P.mode == PIC_MODE_REF) b += cap * 40 + N * 4;
+        // the physical-mode steady walls' queue of lost ions
+        if (s == 1 && !wall_on && P.wall == PIC_WALL_STEADY && P.mode == PIC_MODE_PHYSICAL)
+            b += std::min<int64_t>(std::max<int64_t>(65536, n0 / 8), (int64_t)1 << 26) *
+                 (36 + (weighted ? 4 : 0));
+        // candidate queue of collisions run ahead (counted for both species)
+        if (P.have_collisions && P.mode == PIC_MODE_PHYSICAL)
+            b += (int64_t)MQ_SEGS * collide_seg_cap(cap) * 4 + MQ_SEGS * MQ_STRIDE * 4;
     }
     const int64_t tiles = (N + 1 + 7) / 8;
-    b += 2 * (tiles * tiles * 64 + 4096) * 4;           // sort histogram and starts
+    {  // sort histogram (one counter per 128-byte line) and starts
+        const int ks = P.sort_key >= 0 ? P.sort_key : 6;
+        const int64_t nb = scan_pad((uint32_t)((tiles * tiles * 64) >> ks));
+        b += nb * BIN_STRIDE * 4 + (nb + 1) * 4 + (nb / 4096 + 1) * 4;
+    }
     b += N * N * 16 + (N + 1) * (N + 1) * 16 + ((N + 1) * N + 1) * 8;  // rho, phi, E
     b += (int64_t)(60.0 * 4.0 / 3.0 * (double)(N * N)) + (1 << 20);   // multigrid levels
     if (P.graph_steps > 0 || P.time_data_freq > 0)

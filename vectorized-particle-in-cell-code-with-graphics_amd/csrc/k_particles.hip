@@ -273,12 +273,24 @@ __device__ __forceinline__ uint32_t hash_insert(BlockHash &H, uint32_t key, bool
 
 // Variants of the particle passes.  The generic kernel takes every option at
 // run time; the others pin options that the host has checked (physical mode,
-// no magnetic field, no per-particle weights; collisions off, or on for one
-// species), so the compiler drops the code and registers of everything else
-// (the collisionless in-place pass: 42 instead of 93 VGPRs, 8 instead of 5
-// waves per SIMD).  A pinned value always equals the one passed in, so every
-// variant computes what the generic kernel computes.
-enum PassVariant { PV_GENERIC = 0, PV_PLAIN = 1, PV_COLL_E = 2, PV_COLL_I = 3 };
+// no magnetic field, no per-particle weights, no pair sampling in the pass;
+// then: no collisions in the pass and nothing staged (PLAIN); no collisions
+// in the pass -- none, or run ahead in k_collide -- and lost particles
+// queuing their ids for re-injection (SPAWN); or the block-queued collisions
+// of one species (COLL_E, COLL_I); or, in place, the candidate test only,
+// each candidate left untouched for k_collide_q (DEFER)), so the compiler
+// drops the code and registers of everything else (the collisionless
+// in-place pass: 42 instead of 94 VGPRs, 8 instead of 5 waves per SIMD).  A
+// pinned value always equals the one passed in, so every variant computes
+// what the generic kernel computes.
+enum PassVariant {
+    PV_GENERIC = 0,
+    PV_PLAIN = 1,
+    PV_COLL_E = 2,
+    PV_COLL_I = 3,
+    PV_SPAWN = 4,
+    PV_DEFER = 5
+};
 
 template <int V>
 __device__ __forceinline__ void pin_variant(PushConst &pc, mcc::MccConst &mc, SpView &sp) {
@@ -286,21 +298,51 @@ __device__ __forceinline__ void pin_variant(PushConst &pc, mcc::MccConst &mc, Sp
     pc.mode = PIC_DEV_MODE_PHYS;
     pc.bfield = 0;
     sp.w = nullptr;
-    if (V == PV_PLAIN) {
+    if (V == PV_PLAIN || V == PV_SPAWN) {
         mc.on = 0;
+    } else if (V == PV_DEFER) {
+        mc.on = 1;
+        mc.mode = PIC_DEV_MODE_PHYS;
     } else {
         mc.on = 1;
         mc.mode = PIC_DEV_MODE_PHYS;
         mc.electron = (V == PV_COLL_E) ? 1 : 0;
     }
 }
-// the collisionless variant stages nothing: no re-injection, no wall queue
+// no variant but the generic one samples re-injected pairs in the pass (the
+// fused step queues lost ions for k_plop_drain); PLAIN stages nothing at all
 template <int V>
 __device__ __forceinline__ void pin_spawn(Spawn &spn) {
-    if (V == PV_PLAIN) {
-        spn.plop = 0;
-        spn.wall = 0;
+    if (V != PV_GENERIC) spn.plop = 0;
+    if (V == PV_PLAIN || V == PV_SPAWN || V == PV_DEFER) spn.wall = 0;
+    if (V == PV_PLAIN) spn.lost_ids = 0;
+}
+
+// A deferred pass's candidates of one round: the wavefront appends the
+// store indices of its candidates to its block's queue segment (one atomic
+// per wavefront).  Called by every lane.
+__device__ __forceinline__ void defer_queue(const Spawn &spn, bool cand, uint32_t i) {
+    const uint64_t bal = __ballot(cand);
+    if (!bal) return;  // uniform over the wavefront
+    const int lane = threadIdx.x & 63, lead = __ffsll((unsigned long long)bal) - 1;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t seg = blockIdx.x % MQ_SEGS;
+    uint32_t b = 0u;
+    if (lane == lead) b = atomicAdd(&spn.mq_cnt[seg * MQ_STRIDE], (uint32_t)__popcll(bal));
+    b = (uint32_t)__shfl((int)b, lead, 64);
+    const uint32_t t = b + (uint32_t)__popcll(bal & below);
+    if (cand && t < spn.mq_seg) spn.mq[(size_t)seg * spn.mq_seg + t] = i;
+}
+
+// a particle lost through a wall queues its id and weight for k_plop_drain
+__device__ __forceinline__ void queue_lost(const Spawn &spn, uint32_t id, float w) {
+    const uint32_t slot = atomicAdd(&spn.cw->wall_count, 1u);
+    if (slot >= spn.wq.cap) {
+        atomicAdd(&spn.cw->overflow, 1u);
+        return;
     }
+    spn.wq.key[slot] = id;
+    if (spn.wq.w) spn.wq.w[slot] = w;
 }
 
 // wavefront reductions of the device library (DPP row shifts and lane reads:
@@ -735,6 +777,7 @@ __device__ __forceinline__ void scatter_round(PState &s, uint32_t pid, float w, 
     dep = kept && deposit_ok(s, pc.N, pc.mode);
     if (valid && st != ST_KEPT && st != ST_HOLE) atomicAdd(&sl[st - 1], 1u);
     if (claim && !kept) atomicAdd(shole, 1u);
+    if (sp.lost_ids && valid && st >= ST_LEFT && st <= ST_TOP) queue_lost(sp, pid, w);
     if (mc.on) {
         if (EV_KIND(ev) > 0) atomicAdd(&se[EV_KIND(ev)], 1u);
         if (ev & EV_OVER_BOUND) atomicAdd(&se[6], 1u);
@@ -965,13 +1008,20 @@ __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint
                                               const Spawn &spn, uint32_t *__restrict__ lost6,
                                               bool need_vz, bool blk, int qs, uint32_t *sl,
                                               uint32_t *shole, uint32_t *se,
-                                              const unsigned long long *lds, float w) {
+                                              const unsigned long long *lds, float w,
+                                              bool defer) {
     const PState pre = s;
     int ev = mcc::EV_NONE;
     mcc::Child ch{};
-    const bool done = collide_take(blk, qs, s, ev, ch, lds);
+    // deferred: a candidate stays as it is for k_collide_q
+    bool cand = false;
+    if (defer) {
+        cand = live && mcc::candidate(mc, pid, step);
+        defer_queue(spn, cand, i);
+    }
+    const bool done = defer || collide_take(blk, qs, s, ev, ch, lds);
     int st = ST_HOLE;
-    if (live) st = advance_one(s, pid, E, pc, mc, step, 1, ev, ch, done);
+    if (live && !cand) st = advance_one(s, pid, E, pc, mc, step, 1, ev, ch, done);
     const bool kept = (st == ST_KEPT);
     dep = kept && deposit_ok(s, pc.N, pc.mode);
     if (kept) {
@@ -981,10 +1031,13 @@ __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint
         sp.vx[i] = s.vx;
         sp.vy[i] = s.vy;
         if (need_vz) sp.vz[i] = s.vz;
-    } else if (live) {
+    } else if (live && !cand) {
         sp.cell[i] = cell_pack(CELL_HOLE, CELL_HOLE);
         atomicAdd(&sl[st - 1], 1u);
         atomicAdd(shole, 1u);
+        // the id is read only here: a pass that needs it for nothing else
+        // does not load it for every particle
+        if (spn.lost_ids && st >= ST_LEFT && st <= ST_TOP) queue_lost(spn, sp.id[i], w);
     }
     if (mc.on) {
         if (EV_KIND(ev) > 0) atomicAdd(&se[EV_KIND(ev)], 1u);
@@ -1002,11 +1055,11 @@ __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint
 // block's counters then serve 256 * IPP particles.  Reads 20-28 B and writes
 // 20-24 B per particle, with no histogram or slot claims.
 template <int V>
-__global__ void __launch_bounds__(256)
-    k_push_inplace(SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E,
-                   PushConst pc, mcc::MccConst mc, uint32_t step, unsigned long long *__restrict__ q,
-                   int deposit, Spawn spn, unsigned long long *__restrict__ events,
-                   uint32_t *__restrict__ lost6, uint32_t *__restrict__ holes) {
+__device__ __forceinline__ void push_inplace_body(
+    SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E, PushConst pc,
+    mcc::MccConst mc, uint32_t step, unsigned long long *__restrict__ q, int deposit, Spawn spn,
+    unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6,
+    uint32_t *__restrict__ holes) {
     pin_variant<V>(pc, mc, sp);
     pin_spawn<V>(spn);
     const uint32_t n = *nptr;
@@ -1024,9 +1077,10 @@ __global__ void __launch_bounds__(256)
     // vz and the id are read only where something uses them: the magnetic
     // rotation, collisions, re-injection keyed by the id, and the wall
     // queue (surface physics needs the full velocity and the id's draws)
-    const bool need_vz = pc.bfield != 0 || mc.on || spn.wall;
+    // (a deferred pass moves no candidate: the others keep their vz)
+    const bool need_vz = pc.bfield != 0 || (mc.on && V != PV_DEFER) || spn.wall;
     const bool need_id = mc.on || spn.plop || spn.wall;
-    const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS;
+    const bool blk = mc.on && mc.mode == PIC_DEV_MODE_PHYS && V != PV_DEFER;
     PState s[IPP];
     uint32_t pid[IPP];
     bool live[IPP];
@@ -1051,13 +1105,17 @@ __global__ void __launch_bounds__(256)
     bool dep[IPP];
     // the rounds are spelled out so every array index is a compile-time constant
     inplace_round(sp, base + 0 * blockDim.x, n, s[0], pid[0], live[0], dep[0], E, pc, mc, step,
-                  spn, lost6, need_vz, blk, qs[0], sl, &shole, se, lds, w[0]);
+                  spn, lost6, need_vz, blk, qs[0], sl, &shole, se, lds, w[0],
+                  V == PV_DEFER);
     inplace_round(sp, base + 1 * blockDim.x, n, s[1], pid[1], live[1], dep[1], E, pc, mc, step,
-                  spn, lost6, need_vz, blk, qs[1], sl, &shole, se, lds, w[1]);
+                  spn, lost6, need_vz, blk, qs[1], sl, &shole, se, lds, w[1],
+                  V == PV_DEFER);
     inplace_round(sp, base + 2 * blockDim.x, n, s[2], pid[2], live[2], dep[2], E, pc, mc, step,
-                  spn, lost6, need_vz, blk, qs[2], sl, &shole, se, lds, w[2]);
+                  spn, lost6, need_vz, blk, qs[2], sl, &shole, se, lds, w[2],
+                  V == PV_DEFER);
     inplace_round(sp, base + 3 * blockDim.x, n, s[3], pid[3], live[3], dep[3], E, pc, mc, step,
-                  spn, lost6, need_vz, blk, qs[3], sl, &shole, se, lds, w[3]);
+                  spn, lost6, need_vz, blk, qs[3], sl, &shole, se, lds, w[3],
+                  V == PV_DEFER);
     // block_deposit_n passes a barrier before it writes the window, so every
     // round has read its collision result from the shared LDS by then
     if (deposit) block_deposit_n<IPP>(dep, s, w, q, pc.N, lds);
@@ -1067,6 +1125,213 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x < 8 && se[threadIdx.x])
         atomicAdd(&events[cnt_slot() * 8 + threadIdx.x], (unsigned long long)se[threadIdx.x]);
     if (threadIdx.x == 0 && shole) atomicAdd(&holes[cnt_slot()], shole);
+}
+
+#define PUSH_INPLACE_PARAMS                                                                    \
+    SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E, PushConst pc, \
+        mcc::MccConst mc, uint32_t step, unsigned long long *__restrict__ q, int deposit,     \
+        Spawn spn, unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6,     \
+        uint32_t *__restrict__ holes
+template <int V>
+__global__ void __launch_bounds__(256) k_push_inplace(PUSH_INPLACE_PARAMS) {
+    push_inplace_body<V>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes);
+}
+// The collisionless variants fit 8 waves per SIMD in 42 VGPRs; left to
+// itself the scheduler gives SPAWN (PLAIN plus a rare queue write) 84 and 5
+// waves, so both ask for 8 (no spills).
+template <>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+    k_push_inplace<PV_PLAIN>(PUSH_INPLACE_PARAMS) {
+    push_inplace_body<PV_PLAIN>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes);
+}
+template <>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+    k_push_inplace<PV_SPAWN>(PUSH_INPLACE_PARAMS) {
+    push_inplace_body<PV_SPAWN>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes);
+}
+template <>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+    k_push_inplace<PV_DEFER>(PUSH_INPLACE_PARAMS) {
+    push_inplace_body<PV_DEFER>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes);
+}
+
+// Physical-mode null-collision MCC of one species as passes of their own,
+// ahead of the particle pass of the same step (collisions precede the push,
+// e_collisions.cpp:480-597 and ion_collisions.cpp:255-308 before
+// move_particles), for a species with few candidates (the argon ions: about
+// 0.5 %).  Keeping the collision code out of the particle pass gives that
+// pass the registers (and occupancy) of the collisionless one.  The draws
+// are the per-particle streams of collide_electron / collide_ion, so every
+// result equals the per-lane one.
+//
+// Pass 1, k_cand: every particle's candidate test from its id (one
+// Philox2x32 word; reads the cell and the id, 8 B).  A block appends its
+// candidates' store indices to one of MQ_SEGS queue segments (by block
+// index) with one atomic on that segment's counter, each counter on a
+// 128-byte line of its own: no counter takes more than 1/MQ_SEGS of the
+// blocks (atomics on one address queue behind each other: with 64 counters
+// this pass took 50 us for the C2 ions).  A segment holds every particle of
+// its blocks, so it cannot overflow.
+#define CIPP 8
+__global__ void __launch_bounds__(256)
+    k_cand(SpView sp, const uint32_t *__restrict__ nptr, mcc::MccConst mc, uint32_t step,
+           uint32_t *__restrict__ mq, uint32_t seg_cap, uint32_t *__restrict__ mq_cnt) {
+    const uint32_t n = *nptr;
+    if (blockIdx.x * (256 * CIPP) >= n) return;  // past the live count
+    __shared__ uint32_t wcnt[4], sbase;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t base = blockIdx.x * (256 * CIPP) + threadIdx.x;
+    uint32_t pos[CIPP], cl[CIPP], idv[CIPP];
+#pragma unroll
+    for (int r = 0; r < CIPP; r++) {  // every load first, none waiting on another
+        const uint32_t i = base + r * 256;
+        cl[r] = i < n ? sp.cell[i] : cell_pack(CELL_HOLE, CELL_HOLE);
+        idv[r] = i < n ? sp.id[i] : 0u;
+    }
+    uint32_t wtot = 0;
+#pragma unroll
+    for (int r = 0; r < CIPP; r++) {
+        const bool cand = cell_jx(cl[r]) != CELL_HOLE && mcc::candidate(mc, idv[r], step);
+        const uint64_t bal = __ballot(cand);
+        pos[r] = cand ? wtot + (uint32_t)__popcll(bal & below) : 0xFFFFFFFFu;
+        wtot += (uint32_t)__popcll(bal);
+    }
+    if (lane == 0) wcnt[w] = wtot;
+    __syncthreads();
+    uint32_t off = 0, total = 0;
+    for (int k = 0; k < 4; k++) {
+        if (k < w) off += wcnt[k];
+        total += wcnt[k];
+    }
+    const uint32_t seg = blockIdx.x % MQ_SEGS;
+    if (threadIdx.x == 0) sbase = total ? atomicAdd(&mq_cnt[seg * MQ_STRIDE], total) : 0u;
+    __syncthreads();
+    uint32_t *q = mq + (size_t)seg * seg_cap;
+#pragma unroll
+    for (int r = 0; r < CIPP; r++) {
+        const uint32_t t = sbase + off + pos[r];
+        if (pos[r] != 0xFFFFFFFFu && t < seg_cap) q[t] = base + r * 256;
+    }
+}
+
+// Pass 2, k_collide_q: the queued candidates collide on dense lanes (a
+// stride loop over each segment), their velocities read and written back by
+// store index (12 B each way).  An ionisation stages the ejected electron
+// (born at the parent's position and pushed this step) and the new ion, as
+// the particle passes do.  `defer`: the queue comes from a deferred in-place
+// pass, which left its candidates as they were: each is then also pushed,
+// stored in place or left as a hole (loss counted, lost id queued for
+// re-injection), and deposited with global atomics -- what the pass does
+// for the others.
+__global__ void __launch_bounds__(256)
+    k_collide_q(SpView sp, const float2 *__restrict__ E, PushConst pc, mcc::MccConst mc,
+                uint32_t step, Spawn spn, const uint32_t *__restrict__ mq, uint32_t seg_cap,
+                const uint32_t *__restrict__ mq_cnt, unsigned long long *__restrict__ events,
+                uint32_t *__restrict__ lost6, int defer, unsigned long long *__restrict__ q,
+                uint32_t *__restrict__ holes) {
+    __shared__ uint32_t se[8];
+    if (threadIdx.x < 8) se[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t seg = blockIdx.x % MQ_SEGS, chunk = blockIdx.x / MQ_SEGS;
+    const uint32_t nch = gridDim.x / MQ_SEGS;
+    const uint32_t cnt = min(mq_cnt[seg * MQ_STRIDE], seg_cap);
+    const uint32_t *queue = mq + (size_t)seg * seg_cap;
+    for (uint32_t t = chunk * 256 + threadIdx.x; t < cnt; t += nch * 256) {
+        const uint32_t i = queue[t];
+        const uint32_t pid = sp.id[i];
+        float vx = sp.vx[i], vy = sp.vy[i], vz = sp.vz[i];
+        mcc::Child ch{};
+        const int ev = mc.electron ? mcc::collide_electron(mc, vx, vy, vz, pid, step, true, ch)
+                                   : mcc::collide_ion(mc, vx, vy, vz, pid, step);
+        if (EV_KIND(ev) > 0) atomicAdd(&se[EV_KIND(ev)], 1u);
+        if (ev & EV_OVER_BOUND) atomicAdd(&se[6], 1u);
+        const PState pre = load_state(sp, i, false);
+        const float pw = load_w(sp, i);
+        if (defer) {
+            PState s = pre;
+            s.vx = vx;
+            s.vy = vy;
+            s.vz = vz;
+            push_one(s, E, pc);
+            const int st = classify(s, pc.N, pc.mode);
+            if (st == ST_KEPT) {
+                sp.cell[i] = cell_pack(s.jx, s.jy);
+                sp.ox[i] = s.ox;
+                sp.oy[i] = s.oy;
+                if (deposit_ok(s, pc.N, pc.mode))
+                    dep_global(q, pc.N, s.jx, s.jy, fx_weights_w(s.ox, s.oy, pw));
+            } else {
+                sp.cell[i] = cell_pack(CELL_HOLE, CELL_HOLE);
+                atomicAdd(&lost6[cnt_slot() * 6 + st - 1], 1u);
+                atomicAdd(&holes[cnt_slot()], 1u);
+                if (spn.lost_ids && st >= ST_LEFT && st <= ST_TOP) queue_lost(spn, pid, pw);
+            }
+            vx = s.vx;
+            vy = s.vy;
+            vz = s.vz;
+        }
+        sp.vx[i] = vx;
+        sp.vy[i] = vy;
+        sp.vz[i] = vz;
+        if (EV_KIND(ev) == mcc::EV_IONIZE) {
+            // ejected electron: born at the parent's position, pushed this
+            // step as the reference pushes particles appended before
+            // move_particles
+            PState ce = pre;
+            ce.vx = ch.vx;
+            ce.vy = ch.vy;
+            ce.vz = ch.vz;
+            push_one(ce, E, pc);
+            const int cst = classify(ce, pc.N, pc.mode);
+            if (cst != ST_KEPT) {
+                // left the domain during its first push: it still takes its
+                // id, but is not stored
+                atomicAdd(&lost6[cnt_slot() * 6 + cst - 1], 1u);
+                ce.jx = ce.jy = CELL_DEAD;
+            }
+            stage_put(spn.se, &spn.ce->stage_count, &spn.ce->overflow, ce, (unsigned long long)pid,
+                      pw);
+            // new ion at the same place; the ion pass of this step moves it;
+            // products take the parent's weight (e_collisions.cpp:413,441)
+            PState ci = pre;
+            ci.vx = ch.ivx;
+            ci.vy = ch.ivy;
+            ci.vz = ch.ivz;
+            stage_put(spn.si, &spn.ci->stage_count, &spn.ci->overflow, ci, (unsigned long long)pid,
+                      pw);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && se[threadIdx.x])
+        atomicAdd(&events[cnt_slot() * 8 + threadIdx.x], (unsigned long long)se[threadIdx.x]);
+}
+
+// Re-injection for the ions lost through a wall in the particle pass just
+// run (physical mode, steady walls): the pass queued each such ion (key = id,
+// with its weight) in the wall queue, which these walls do not otherwise
+// use, and here every queued ion stages the electron-ion pair drawn from its
+// own stream (mcc::plop_pair), keyed (1 << 32 | id), with its weight -- what
+// the pass itself staged before.  The pair sampler (three Philox blocks, six
+// logs and cosines) then stays out of the pass and its registers.  One
+// workgroup: a step loses few ions.  The queue is left empty.
+__global__ void __launch_bounds__(1024)
+    k_plop_drain(StageView wq, SpCounters *__restrict__ cw, Spawn sp, mcc::MccConst mc,
+                 uint32_t step, int N) {
+    uint32_t n = cw->wall_count;
+    if (n > wq.cap) n = wq.cap;
+    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+        const uint32_t pid = (uint32_t)wq.key[e];
+        const float w = wq.w ? wq.w[e] : 1.0f;
+        PState pe, pi;
+        mcc::plop_pair(mc, pid, step, N, sp.se_sigma, sp.si_sigma, sp.plop_ref, pe, pi);
+        const unsigned long long k = (1ull << 32) | pid;
+        // the re-injected pair replaces the lost ion: it carries its weight
+        stage_put(sp.se, &sp.ce->stage_count, &sp.ce->overflow, pe, k, w);
+        stage_put(sp.si, &sp.ci->stage_count, &sp.ci->overflow, pi, k, w);
+    }
+    __syncthreads();  // every thread has read the count
+    if (threadIdx.x == 0) cw->wall_count = 0;
 }
 
 // ------------------------------------------------------- per-phase path
@@ -1651,9 +1916,14 @@ void launch_inject(hipStream_t st, const StageView &sv, SpCounters *cnt, const f
 // the variant whose pinned options match the pass's (PV_GENERIC otherwise)
 static int pass_variant(const SpView &sp, const PushConst &pc, const mcc::MccConst &mc,
                         const Spawn *spn) {
-    if (pc.mode != PIC_DEV_MODE_PHYS || pc.bfield != 0 || sp.w != nullptr) return PV_GENERIC;
-    if (!mc.on) return (!spn || (!spn->plop && !spn->wall)) ? PV_PLAIN : PV_GENERIC;
+    if (pc.mode != PIC_DEV_MODE_PHYS || pc.bfield != 0 || sp.w != nullptr || (spn && spn->plop))
+        return PV_GENERIC;
+    if (!mc.on) {
+        if (spn && spn->wall) return PV_GENERIC;  // the wall queue takes the whole state
+        return (spn && spn->lost_ids) ? PV_SPAWN : PV_PLAIN;
+    }
     if (mc.mode != PIC_DEV_MODE_PHYS) return PV_GENERIC;
+    if (spn && spn->mq) return spn->wall ? PV_GENERIC : PV_DEFER;
     return mc.electron ? PV_COLL_E : PV_COLL_I;
 }
 
@@ -1662,6 +1932,8 @@ static int pass_variant(const SpView &sp, const PushConst &pc, const mcc::MccCon
     case PV_PLAIN: hipLaunchKernelGGL(kern<PV_PLAIN>, grid, dim3(256), 0, __VA_ARGS__); break; \
     case PV_COLL_E: hipLaunchKernelGGL(kern<PV_COLL_E>, grid, dim3(256), 0, __VA_ARGS__); break; \
     case PV_COLL_I: hipLaunchKernelGGL(kern<PV_COLL_I>, grid, dim3(256), 0, __VA_ARGS__); break; \
+    case PV_SPAWN: hipLaunchKernelGGL(kern<PV_SPAWN>, grid, dim3(256), 0, __VA_ARGS__); break; \
+    case PV_DEFER: hipLaunchKernelGGL(kern<PV_DEFER>, grid, dim3(256), 0, __VA_ARGS__); break; \
     default: hipLaunchKernelGGL(kern<PV_GENERIC>, grid, dim3(256), 0, __VA_ARGS__); break;   \
     }
 
@@ -1689,6 +1961,37 @@ void launch_push_inplace(hipStream_t st, const SpView &sp, const uint32_t *n, in
     const int v = pass_variant(sp, pc, mc, &spn);
     LAUNCH_VARIANT(v, k_push_inplace, dim3(blocks_for(cap, 256 * IPP)), st, sp, n, E, pc, mc, step,
                    q, deposit, spn, events, lost6, holes);
+}
+
+uint32_t collide_seg_cap(int64_t cap) {
+    const int64_t blocks = blocks_for(cap, 256 * CIPP);
+    return (uint32_t)((blocks + MQ_SEGS - 1) / MQ_SEGS * (256 * CIPP));
+}
+
+void launch_collide(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                    const float2 *E, const PushConst &pc, const mcc::MccConst &mc, uint32_t step,
+                    const Spawn &spn, unsigned long long *events, uint32_t *lost6, uint32_t *mq,
+                    uint32_t *mq_cnt) {
+    const uint32_t seg_cap = collide_seg_cap(cap);
+    launch_fill<uint32_t>(st, mq_cnt, MQ_SEGS * MQ_STRIDE, 0u);
+    hipLaunchKernelGGL(k_cand, dim3(blocks_for(cap, 256 * CIPP)), dim3(256), 0, st, sp, n, mc, step,
+                       mq, seg_cap, mq_cnt);
+    hipLaunchKernelGGL(k_collide_q, dim3(MQ_SEGS), dim3(256), 0, st, sp, E, pc, mc, step, spn,
+                       mq, seg_cap, mq_cnt, events, lost6, 0, (unsigned long long *)nullptr,
+                       (uint32_t *)nullptr);
+}
+
+void launch_collide_deferred(hipStream_t st, const SpView &sp, const float2 *E,
+                             const PushConst &pc, const mcc::MccConst &mc, uint32_t step,
+                             const Spawn &spn, unsigned long long *events, uint32_t *lost6,
+                             unsigned long long *q, uint32_t *holes) {
+    hipLaunchKernelGGL(k_collide_q, dim3(MQ_SEGS), dim3(256), 0, st, sp, E, pc, mc, step, spn,
+                       spn.mq, spn.mq_seg, spn.mq_cnt, events, lost6, 1, q, holes);
+}
+
+void launch_plop_drain(hipStream_t st, const StageView &wq, SpCounters *cw, const Spawn &sp,
+                       const mcc::MccConst &mc, uint32_t step, int N) {
+    hipLaunchKernelGGL(k_plop_drain, dim3(1), dim3(1024), 0, st, wq, cw, sp, mc, step, N);
 }
 
 void launch_mcc(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,

@@ -38,6 +38,8 @@ struct Species {
     pic::ReduceState *rstate = nullptr;
     int64_t injected = 0;            // particles created by the source
     pic::StageView wq = {};          // particles that reached a wall (surface physics)
+    // candidate queue and its segment counters (collisions run ahead)
+    uint32_t *mq = nullptr, *mq_cnt = nullptr;
 };
 
 struct ProfPair {

@@ -105,6 +105,13 @@ struct Spawn {
     int wall;
     StageView wq;
     SpCounters *cw;
+    // re-injection after the pass: a particle lost through a wall queues only
+    // its id and weight in wq (k_plop_drain samples the pair)
+    int lost_ids;
+    // deferred collisions (in-place pass): a candidate is queued here by
+    // store index, untouched, for k_collide_q to collide, push and deposit
+    uint32_t *mq, *mq_cnt;
+    uint32_t mq_seg;
 };
 
 // ---- k_particles.hip
@@ -123,6 +130,21 @@ void launch_push_inplace(hipStream_t st, const SpView &sp, const uint32_t *n, in
                          const float2 *E, const PushConst &pc, const mcc::MccConst &mc,
                          uint32_t step, unsigned long long *q, int deposit, const Spawn &spn,
                          unsigned long long *events, uint32_t *lost6, uint32_t *holes);
+// candidate queue of the collisions run ahead: MQ_SEGS segments of
+// collide_seg_cap(cap) indices, one counter per segment every MQ_STRIDE words
+#define MQ_SEGS 1024
+#define MQ_STRIDE 32
+uint32_t collide_seg_cap(int64_t cap);
+void launch_collide(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
+                    const float2 *E, const PushConst &pc, const mcc::MccConst &mc, uint32_t step,
+                    const Spawn &spn, unsigned long long *events, uint32_t *lost6, uint32_t *mq,
+                    uint32_t *mq_cnt);
+void launch_collide_deferred(hipStream_t st, const SpView &sp, const float2 *E,
+                             const PushConst &pc, const mcc::MccConst &mc, uint32_t step,
+                             const Spawn &spn, unsigned long long *events, uint32_t *lost6,
+                             unsigned long long *q, uint32_t *holes);
+void launch_plop_drain(hipStream_t st, const StageView &wq, SpCounters *cw, const Spawn &sp,
+                       const mcc::MccConst &mc, uint32_t step, int N);
 void launch_mcc(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
                 const PushConst &pc, const mcc::MccConst &mc, uint32_t step, const Spawn &spn,
                 unsigned long long *events);
