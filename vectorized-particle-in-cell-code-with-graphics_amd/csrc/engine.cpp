@@ -649,8 +649,7 @@ void push_species_inplace(pic_handle *h, int si) {
     // few candidates: the pass makes the candidate test only and leaves each
     // candidate to k_collide_q (collide, push, store, deposit) after it
     const bool defer = collide_ahead(s) && s.mq;
-    if (defer) {
-        launch_fill<uint32_t>(h->st, s.mq_cnt, MQ_SEGS * MQ_STRIDE, 0u);
+    if (defer) {  // the queue counters are zero (k_collide_q leaves them so)
         spn.mq = s.mq;
         spn.mq_cnt = s.mq_cnt;
         spn.mq_seg = collide_seg_cap(s.cap);
@@ -978,6 +977,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         if ((rc = dalloc(h, &sp.mq, (size_t)MQ_SEGS * collide_seg_cap(sp.cap))) != PIC_OK ||
             (rc = dalloc(h, &sp.mq_cnt, (size_t)MQ_SEGS * MQ_STRIDE)) != PIC_OK)
             return fail(rc);
+        launch_fill<uint32_t>(h->st, sp.mq_cnt, MQ_SEGS * MQ_STRIDE, 0u);
     }
     if (h->wall_on) {
         WallConst &c = h->wc;

@@ -1227,7 +1227,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_collide_q(SpView sp, const float2 *__restrict__ E, PushConst pc, mcc::MccConst mc,
                 uint32_t step, Spawn spn, const uint32_t *__restrict__ mq, uint32_t seg_cap,
-                const uint32_t *__restrict__ mq_cnt, unsigned long long *__restrict__ events,
+                uint32_t *__restrict__ mq_cnt, unsigned long long *__restrict__ events,
                 uint32_t *__restrict__ lost6, int defer, unsigned long long *__restrict__ q,
                 uint32_t *__restrict__ holes) {
     __shared__ uint32_t se[8];
@@ -1305,6 +1305,9 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     if (threadIdx.x < 8 && se[threadIdx.x])
         atomicAdd(&events[cnt_slot() * 8 + threadIdx.x], (unsigned long long)se[threadIdx.x]);
+    // one block per segment (the launches below): the segment's counter is
+    // read; clear it for the next queue
+    if (threadIdx.x == 0 && nch == 1) mq_cnt[seg * MQ_STRIDE] = 0u;
 }
 
 // Re-injection for the ions lost through a wall in the particle pass just
@@ -1973,7 +1976,7 @@ void launch_collide(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t
                     const Spawn &spn, unsigned long long *events, uint32_t *lost6, uint32_t *mq,
                     uint32_t *mq_cnt) {
     const uint32_t seg_cap = collide_seg_cap(cap);
-    launch_fill<uint32_t>(st, mq_cnt, MQ_SEGS * MQ_STRIDE, 0u);
+    // the counters are zero: set so at allocation, cleared by k_collide_q
     hipLaunchKernelGGL(k_cand, dim3(blocks_for(cap, 256 * CIPP)), dim3(256), 0, st, sp, n, mc, step,
                        mq, seg_cap, mq_cnt);
     hipLaunchKernelGGL(k_collide_q, dim3(MQ_SEGS), dim3(256), 0, st, sp, E, pc, mc, step, spn,
