@@ -1183,9 +1183,6 @@ static void vcycle_top(hipStream_t st, MgLevels &M, int j, const float *r_in, co
 #undef POST_UPDATE
 }
 
-// Wait for the queued cycles: the solve's host check sits between two phases
-// of the step, so the wake-up latency of a blocking wait is idle GPU time;
-// poll the batch's event for up to 2 ms, then block (also reports errors).
 // Waits for the queued cycles' verdict: the device has published (host_flags
 // in coherent host memory, the pass count last) either convergence or all
 // `passes` residual passes of the solve.  The host reads the count as soon
