@@ -211,6 +211,21 @@ int pic_inject_particles(pic_handle *h, int32_t s);
  * pic_params.reduce; the fused timestep runs it every step. */
 int pic_contain_particle_growth(pic_handle *h, int32_t s);
 
+/* Cross sections the collision kernels use, evaluated on the host in FP64
+ * (e_collisions::sigma_Ar_elastic / sigma_Ar_ionize / sigma_H_elastic /
+ * sigma_H_ionize, e_collisions.h:192-195; ion_collisions::sigma_Ar_cx /
+ * sigma_H_cx, ion_collisions.cpp:170-199).  channel: PIC_XS_*; mode:
+ * PIC_MODE_REF (the fits as the reference writes them) or PIC_MODE_PHYSICAL
+ * (breakpoints closed, clamped >= 0; excitation and Ar+ isotropic elastic
+ * exist only here).  energy in eV (H+ charge exchange: eV as well);
+ * *sigma in 1e-20 m^2.  Needs no GPU. */
+#define PIC_XS_ELASTIC 0    /* e-neutral elastic                              */
+#define PIC_XS_EXCITATION 1 /* e-Ar lumped electronic excitation (physical)   */
+#define PIC_XS_IONIZATION 2 /* e-neutral ionisation                           */
+#define PIC_XS_CX 3         /* ion-neutral charge exchange                    */
+#define PIC_XS_ION_ISO 4    /* Ar+-Ar isotropic elastic (physical)            */
+int pic_cross_section(int32_t gas, int32_t channel, int32_t mode, double energy, double *sigma);
+
 /* Step counter used by the per-phase entry points (collision random streams
  * and the ion sub-cycling test); pic_fields_grid_electric_field, the last
  * phase of a step, advances it by one.  No reference counterpart. */

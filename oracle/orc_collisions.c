@@ -4,12 +4,46 @@
  *
  * FP64 restatement of the reference's fits in units of 1e-20 m^2
  * (e_collisions.cpp:205-308, ion_collisions.cpp:170-199), plus the two
- * channels the physical mode adds (Ar excitation shape, Ar+ isotropic
- * elastic of Phelps 1994), and the null-collision bound.
+ * channels the physical mode adds (Ar excitation from the tabulated levels
+ * shipped with the reference, Ar+ isotropic elastic of Phelps 1994), and the
+ * null-collision bound.
  */
 #include <math.h>
 
 #include "oracle.h"
+
+/* e-Ar electronic excitation levels (physical mode): the six tables of
+ * References/PhelpsAr_e_cross_section.pdf, section 1, tables (2)-(7) (data of
+ * Chutjian & Cartwright 1981), energy eV / cross section 1e-20 m^2.  Each is
+ * linear between rows, 0 below its first row, held at its last row above. */
+static const double exc0_E[23] = {11.549, 11.9, 12.0, 12.5, 12.8, 13.0, 13.1, 13.4, 14.0, 15.0, 16.0, 17.0, 20.0, 25.0, 30.0, 35.0, 40.0, 50.0, 60.0, 70.0, 80.0, 100.0, 120.0};
+static const double exc0_S[23] = {0.0, 0.0029, 0.0028, 0.0066, 0.009, 0.009, 0.0115, 0.0094, 0.0125, 0.022, 0.0335, 0.04, 0.033, 0.024, 0.017, 0.0115, 0.0073, 0.0036, 0.0022, 0.0016, 0.001, 0.0006, 0.0005};
+static const double exc1_E[23] = {11.624, 12.0, 13.0, 14.0, 15.0, 16.0, 17.0, 18.0, 20.0, 22.0, 24.0, 26.0, 28.0, 30.0, 35.0, 40.0, 45.0, 50.0, 60.0, 70.0, 80.0, 90.0, 100.0};
+static const double exc1_S[23] = {0.0, 0.004, 0.012, 0.02, 0.028, 0.035, 0.04, 0.043, 0.047, 0.051, 0.052, 0.053, 0.053, 0.052, 0.05, 0.048, 0.047, 0.045, 0.041, 0.038, 0.035, 0.032, 0.031};
+static const double exc2_E[23] = {11.723, 11.9, 12.0, 12.5, 12.8, 13.0, 13.1, 13.4, 14.0, 15.0, 16.0, 17.0, 20.0, 25.0, 30.0, 35.0, 40.0, 50.0, 60.0, 70.0, 80.0, 100.0, 120.0};
+static const double exc2_S[23] = {0.0, 0.0006, 0.0006, 0.0013, 0.0018, 0.0018, 0.0023, 0.0019, 0.0025, 0.0045, 0.0061, 0.0075, 0.0066, 0.006, 0.0045, 0.0029, 0.0019, 0.0012, 0.0009, 0.0007, 0.0006, 0.0005, 0.0004};
+static const double exc3_E[22] = {11.828, 13.0, 14.0, 15.0, 16.0, 17.0, 18.0, 20.0, 22.0, 24.0, 26.0, 28.0, 30.0, 35.0, 40.0, 45.0, 50.0, 60.0, 70.0, 80.0, 90.0, 100.0};
+static const double exc3_S[22] = {0.0, 0.029, 0.049, 0.065, 0.076, 0.086, 0.094, 0.103, 0.114, 0.123, 0.132, 0.14, 0.147, 0.16, 0.17, 0.173, 0.175, 0.175, 0.17, 0.164, 0.156, 0.15};
+static const double exc4_E[21] = {12.9, 13.0, 13.1, 13.4, 14.0, 15.0, 16.0, 17.0, 18.0, 20.0, 22.0, 25.0, 30.0, 35.0, 40.0, 50.0, 60.0, 70.0, 80.0, 100.0, 120.0};
+static const double exc4_S[21] = {0.0, 0.003, 0.009, 0.0315, 0.07, 0.15, 0.24, 0.32, 0.4, 0.5, 0.53, 0.53, 0.52, 0.5, 0.48, 0.43, 0.4, 0.37, 0.33, 0.28, 0.25};
+static const double exc5_E[15] = {14.09, 15.0, 16.0, 17.0, 20.0, 25.0, 30.0, 35.0, 40.0, 50.0, 60.0, 70.0, 80.0, 100.0, 120.0};
+static const double exc5_S[15] = {0.0, 0.028, 0.054, 0.075, 0.105, 0.145, 0.16, 0.19, 0.21, 0.235, 0.25, 0.26, 0.265, 0.27, 0.26};
+
+static double table_level(const double *e, const double *s, int n, double E) {
+    if (E < e[0]) return 0.0;
+    if (E >= e[n - 1]) return s[n - 1];
+    for (int k = 0; k + 1 < n; k++)
+        if (E < e[k + 1]) return s[k] + (s[k + 1] - s[k]) * (E - e[k]) / (e[k + 1] - e[k]);
+    return s[n - 1];
+}
+
+/* lumped excitation: the sum of the six levels, 0 below the channel threshold */
+static double ar_excitation(double E, double thr) {
+    if (E < thr) return 0.0;
+    return table_level(exc0_E, exc0_S, 23, E) + table_level(exc1_E, exc1_S, 23, E) +
+           table_level(exc2_E, exc2_S, 23, E) + table_level(exc3_E, exc3_S, 22, E) +
+           table_level(exc4_E, exc4_S, 21, E) + table_level(exc5_E, exc5_S, 15, E);
+}
 
 double orc_sigma(int kind, double E, double thr) {
     switch (kind) {
@@ -44,8 +78,8 @@ double orc_sigma(int kind, double E, double thr) {
                                    : 130.0 * pow(E, -0.74);
             return s > 0.0 ? s : 0.0;
         }
-        case ORC_SIG_AR_EXC: /* physical mode only: 3.262 ln(x)/x, x = E/thr */
-            return (E > thr) ? 3.262 * log(E / thr) / (E / thr) : 0.0;
+        case ORC_SIG_AR_EXC: /* physical mode only: sum of the tabulated levels */
+            return ar_excitation(E, thr);
         case ORC_SIG_H_EL: /* e_collisions.cpp:267-283 */
             if (E < 1.0e-2) return 3.834 * pow(E, 2.0) - 20.82 * E + 40.98;
             if (E < 20.0) return -6.15 * log(E) + 20.91;

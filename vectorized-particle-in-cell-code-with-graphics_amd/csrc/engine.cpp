@@ -1245,6 +1245,51 @@ int pic_ion_collisions(pic_handle *h) {
     return PIC_OK;
 }
 
+int pic_cross_section(int32_t gas, int32_t channel, int32_t mode, double E, double *sigma) {
+    if (!sigma || (gas != PIC_GAS_ARGON && gas != PIC_GAS_HYDROGEN) ||
+        (mode != PIC_MODE_REF && mode != PIC_MODE_PHYSICAL) || !(E >= 0.0))
+        return set_global_error(PIC_ERR_ARG, "bad argument");
+    const bool ref = mode == PIC_MODE_REF;
+    double s = 0.0;
+    if (gas == PIC_GAS_ARGON) {
+        switch (channel) {
+            case PIC_XS_ELASTIC: s = ref ? mcc::ar_el_ref<double>(E) : mcc::ar_el_phys<double>(E); break;
+            case PIC_XS_EXCITATION:
+                if (ref) return set_global_error(PIC_ERR_UNSUPPORTED, "no excitation channel in reference mode");
+                s = mcc::ar_exc_phys<double>(E, 11.55);
+                break;
+            case PIC_XS_IONIZATION:
+                s = ref ? mcc::ar_iz_ref<double>(E) : mcc::ar_iz_phys<double>(E, 15.76);
+                break;
+            case PIC_XS_CX: s = mcc::ar_cx<double>(E); break;
+            case PIC_XS_ION_ISO:
+                if (ref) return set_global_error(PIC_ERR_UNSUPPORTED, "no ion elastic channel in reference mode");
+                s = mcc::ar_iso_phys<double>(E);
+                break;
+            default: return set_global_error(PIC_ERR_ARG, "unknown channel");
+        }
+    } else {
+        switch (channel) {
+            case PIC_XS_ELASTIC: {
+                const double v = mcc::h_el<double>(E);
+                s = ref ? v : (v > 0.0 ? v : 0.0);
+                break;
+            }
+            case PIC_XS_IONIZATION:  // the Janev bracket; physical mode converts 1e-17 -> 1e-20 m^2
+                s = (ref ? 1.0 : 1000.0) * mcc::h_iz_janev<double>(E);
+                break;
+            case PIC_XS_CX:  // keV/amu of H+ (the reference divides by AMU in kg: sigma 0)
+                s = mcc::h_cx_janev<double>(ref ? E / 1000.0 / kAmu : E / 1000.0 / 1.00728);
+                break;
+            case PIC_XS_EXCITATION:
+            case PIC_XS_ION_ISO: return set_global_error(PIC_ERR_UNSUPPORTED, "no such hydrogen channel");
+            default: return set_global_error(PIC_ERR_ARG, "unknown channel");
+        }
+    }
+    *sigma = s;
+    return PIC_OK;
+}
+
 int pic_set_step(pic_handle *h, int64_t step) {
     if (!h || step < 0) return set_global_error(PIC_ERR_ARG, "bad argument");
     h->step = step;

@@ -110,14 +110,54 @@ HD T ar_iz_phys(T E, T thr) {
     return s > (T)0 ? s : (T)0;
 }
 
-// e-Ar excitation, lumped 11.55 eV level.  The reference has no excitation
-// channel; this Bethe-type shape (peak 1.2e-20 m^2 at e * 11.55 eV) is this
-// library's choice, used only in PHYS mode.
+// e-Ar electronic excitation, one lumped channel (PHYS only; the reference
+// has no excitation channel).  Cross section: the sum of the six tabulated
+// excitation levels (thresholds 11.549-14.09 eV; data of Chutjian &
+// Cartwright, Phys. Rev. A 23 (1981) 2178) of the recommended Ar set that
+// ships with the reference, References/PhelpsAr_e_cross_section.pdf,
+// section 1, tables (2)-(7).  Each table is linear between its rows, zero
+// below its threshold and held at its last row above its last energy (the
+// usual convention of Boltzmann solvers for tabulated data); a sum of such
+// functions is linear between the union of their energies, tabulated below
+// (35 rows, sums exact to 1e-10).  The channel fires above thr (11.55 eV,
+// the lumped level and energy loss of Vahedi & Surendra, Comput. Phys.
+// Commun. 87 (1995) 179).  Units 1e-20 m^2 (= angstrom^2).
+#define AR_EXC_ROWS 35
+#define AR_EXC_TABLE_E                                                                          \
+    {11.549, 11.624, 11.723, 11.828, 11.9, 12.0, 12.5, 12.8, 12.9, 13.0, 13.1, 13.4, 14.0, 14.09, \
+     15.0,   16.0,   17.0,   18.0,   20.0, 22.0, 24.0, 25.0, 26.0, 28.0, 30.0, 35.0, 40.0, 45.0,  \
+     50.0,   60.0,   70.0,   80.0,   90.0, 100.0, 120.0}
+#define AR_EXC_TABLE_S                                                                      \
+    {0.0,          0.0006196581, 0.0024907983, 0.0048312732, 0.0082177402, 0.0116559727,   \
+     0.0325279863, 0.0452511945, 0.0485255973, 0.0548,       0.0666,       0.095,          \
+     0.154,        0.164395,     0.2975,       0.4446,       0.5685,       0.6668666667,   \
+     0.7946,       0.85176,      0.87392,      0.885,        0.8893,       0.8959,         \
+     0.9005,       0.9144,       0.9172,       0.9045,       0.8898,       0.8691,         \
+     0.8403,       0.7956,       0.76185,      0.7321,       0.6919}
+static __constant__ const float ar_exc_E_dev[AR_EXC_ROWS] = AR_EXC_TABLE_E;
+static __constant__ const float ar_exc_S_dev[AR_EXC_ROWS] = AR_EXC_TABLE_S;
+static const double ar_exc_E_host[AR_EXC_ROWS] = AR_EXC_TABLE_E;
+static const double ar_exc_S_host[AR_EXC_ROWS] = AR_EXC_TABLE_S;
+
 template <typename T, typename M = LibMath>
 HD T ar_exc_phys(T E, T thr) {
-    if (E <= thr) return (T)0.0;
-    T x = M::quot(E, thr);
-    return M::quot((T)3.262 * M::logn(x), x);
+    if (!(E >= thr) || !(E >= (T)11.549)) return (T)0.0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float *te = ar_exc_E_dev, *ts = ar_exc_S_dev;
+#else
+    const double *te = ar_exc_E_host, *ts = ar_exc_S_host;
+#endif
+    if (E >= (T)te[AR_EXC_ROWS - 1]) return (T)ts[AR_EXC_ROWS - 1];
+    int lo = 0, hi = AR_EXC_ROWS - 1;  // te[lo] <= E < te[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((T)te[mid] <= E)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const T e0 = (T)te[lo], e1 = (T)te[hi], s0 = (T)ts[lo], s1 = (T)ts[hi];
+    return s0 + M::quot((s1 - s0) * (E - e0), e1 - e0);
 }
 
 // e-H elastic (e_collisions.cpp:267-283)

@@ -246,6 +246,7 @@ def load_library(path=LIB_PATH):
         "pic_estimate_bytes": [P(Params), P(i64)],
         "pic_batch_advance": [P(vp), i32, i32],
         "pic_serve": [P(ServeRequest), i32, i64, i32, P(ServeResult), P(ServeStats)],
+        "pic_cross_section": [i32, i32, i32, dbl, P(dbl)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -286,6 +287,22 @@ def mglin(u, ncycle=2, length=1.0):
     n = a.shape[0]
     _check(load_library().pic_mglin(_ptr(a), n, ncycle, length))
     return a
+
+
+XS_ELASTIC, XS_EXCITATION, XS_IONIZATION, XS_CX, XS_ION_ISO = 0, 1, 2, 3, 4
+
+
+def cross_section(gas, channel, energy, mode=MODE_PHYSICAL):
+    """Cross section [1e-20 m^2] the collision kernels use at `energy` eV
+    (e_collisions::sigma_Ar_elastic and its siblings, e_collisions.h:192-195);
+    host-side FP64, no GPU needed.  Accepts a scalar or an array."""
+    lib = load_library()
+    out = C.c_double()
+    vals = []
+    for e in np.atleast_1d(np.asarray(energy, dtype=np.float64)):
+        _check(lib.pic_cross_section(gas, channel, mode, float(e), C.byref(out)))
+        vals.append(out.value)
+    return vals[0] if np.ndim(energy) == 0 else np.array(vals)
 
 
 def poisson_tol(f, u0=None, length=1.0, tol=1e-10, max_iter=60):
