@@ -164,12 +164,17 @@ int pic_destroy(pic_handle *h);
 int pic_set_runtime(pic_handle *h, int32_t run_time);
 
 /* pic::pic_run (pic.cpp:1040-1279): pre-loop wall pass, then run_time steps of
- * the fused timestep (collisions, gather+push, walls, deposit, rho, phi, E). */
+ * the fused timestep (collisions, gather+push, walls, deposit, rho, phi, E).
+ * Returns when the run is complete.  PIC_ERR_CAPACITY: particles were dropped
+ * because the particle store or a staging buffer was full (the run went on
+ * without them; pic_get_overflow counts them). */
 int pic_run(pic_handle *h);
 
-/* The timestep loop of pic_run without its pre-loop wall pass: advances
- * nsteps steps (no reference counterpart; used by drivers that step a run
- * repeatedly, e.g. a benchmark or a serving loop). */
+/* The timestep loop of pic_run without its pre-loop wall pass: queues nsteps
+ * steps on the handle's stream (no reference counterpart; used by drivers
+ * that step a run repeatedly, e.g. a benchmark or a serving loop).  Returns
+ * PIC_ERR_CAPACITY when particles dropped by an earlier call are seen (once
+ * per increase of the count); pic_sync reports those of the last call. */
 int pic_advance(pic_handle *h, int32_t nsteps);
 
 /* pic::dump_data (pic.cpp:1285-1295) -> particle::dump_data
@@ -268,7 +273,9 @@ int pic_get_collision_counts(pic_handle *h, int32_t s, int64_t *out8);
 /* particles dropped because a store or staging buffer was full (0 normally) */
 int pic_get_overflow(pic_handle *h, int64_t *dropped);
 int pic_get_nodes(pic_handle *h, int32_t *nodes);
-const char *pic_last_error(pic_handle *h);              /* NULL handle: last global error */
+/* message of the handle's last error; NULL handle: the last error of a call
+ * that had no handle (or failed before it had one), kept per calling thread */
+const char *pic_last_error(pic_handle *h);
 
 /* Native-layout state access (SoA: packed int16 cell (jx | jy<<16), fp32
  * offsets in [0,1), fp32 velocities, uint32 id) for tests and tools. */

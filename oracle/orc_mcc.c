@@ -348,15 +348,16 @@ int orc_ref_run_mcc(const orc_ref_config *c, const orc_mcc_params *me, const orc
                 continue;
             }
             if (plop) {
-                double u0[4], u1[4], u2[4];
-                uint32_t r0[4], r1[4], r2[4];
+                double u0[4], u1[4], u2[4], u3[4];
+                uint32_t r0[4], r1[4], r2[4], r3[4];
                 draw(mi, iid[k], step, 16u + RS_PLOP, u0, r0);
                 draw(mi, iid[k], step, 16u + RS_PLOP2, u1, r1);
                 draw(mi, iid[k], step, 16u + RS_PLOP2 + 1u, u2, r2);
+                draw(mi, iid[k], step, 16u + RS_PLOP2 + 2u, u3, r3);
                 double cells = (double)(N - 3);
                 double tx = 1.0 + u0[0] * cells, ty = 1.0 + u0[1] * cells;
                 uint32_t wa[6] = {r1[0], r1[1], r1[2], r1[3], r2[0], r2[1]};
-                uint32_t wb[6] = {r2[2], r2[3], r0[2], r0[3], r1[1] ^ r2[2], r1[3] ^ r2[0]};
+                uint32_t wb[6] = {r2[2], r2[3], r0[2], r0[3], r3[0], r3[1]};
                 double nrm[6];
                 for (int m = 0; m < 6; m++)
                     nrm[m] = sqrt(-2.0 * log((double)orc_u01(wa[m]))) *

@@ -272,13 +272,15 @@ static long collect_cands(long n, const uint32_t *cell, const uint32_t *id, int 
 static void plop32(uint32_t ion_id, uint32_t step, int N, uint32_t key0, uint32_t key1, int *jx,
                    int *jy, float *ox, float *oy, float nrm[6]) {
     const uint32_t key[2] = {key0, key1};
-    uint32_t r[4], a[4], b[4];
+    uint32_t r[4], a[4], b[4], d[4];
     uint32_t c0[4] = {ion_id, step, 16u + 3u, 0x4D434331u};
     uint32_t c1[4] = {ion_id, step, 16u + 4u, 0x4D434331u};
     uint32_t c2[4] = {ion_id, step, 16u + 5u, 0x4D434331u};
+    uint32_t c3[4] = {ion_id, step, 16u + 6u, 0x4D434331u};
     orc_philox4x32_10(c0, key, r);
     orc_philox4x32_10(c1, key, a);
     orc_philox4x32_10(c2, key, b);
+    orc_philox4x32_10(c3, key, d);
     float span = (float)(N - 3);
     float tx = 1.0f + orc_u01(r[0]) * span, ty = 1.0f + orc_u01(r[1]) * span;
     int ix = (int)floorf(tx), iy = (int)floorf(ty);
@@ -290,7 +292,7 @@ static void plop32(uint32_t ion_id, uint32_t step, int N, uint32_t key0, uint32_
     *ox = fx;
     *oy = fy;
     uint32_t w[6] = {a[0], a[1], a[2], a[3], b[0], b[1]};
-    uint32_t w2[6] = {b[2], b[3], r[2], r[3], a[1] ^ b[2], a[3] ^ b[0]};
+    uint32_t w2[6] = {b[2], b[3], r[2], r[3], d[0], d[1]};
     for (int k = 0; k < 6; k++) {
         float R1 = orc_u01(w[k]), R2 = orc_u01(w2[k]);
         nrm[k] = cosf(6.2831853f * R2) * sqrtf(fabsf(logf(R1)));

@@ -22,10 +22,11 @@ using namespace pic;
 using namespace pic_host;
 
 namespace pic_host {
-static std::mutex g_err_mu;
-static std::string g_err;
+// the last error of a call without a handle, per calling thread: the
+// pointer pic_last_error(NULL) returns stays valid until that thread's next
+// failing call, whatever other threads do meanwhile
+static thread_local std::string g_err;
 int set_global_error(int code, const std::string &msg) {
-    std::lock_guard<std::mutex> lk(g_err_mu);
     g_err = msg;
     return code;
 }
@@ -594,6 +595,36 @@ int read_counters(pic_handle *h) {
     return PIC_OK;
 }
 
+// Particles dropped for lack of capacity (store or staging buffer full) make
+// the next call that sees them fail with PIC_ERR_CAPACITY, once per increase
+// of the count.  The counters travel to pinned memory behind the queued work
+// (no wait); `wait` blocks until the last copy has landed.
+void queue_overflow_copy(pic_handle *h) {
+    for (int s = 0; s < 2; s++)
+        hipMemcpyAsync(&h->ovf_host[s], &h->cnt[s].overflow, sizeof(uint32_t),
+                       hipMemcpyDeviceToHost, h->st);
+    if (hipEventRecord(h->ovf_ev, h->st) == hipSuccess) h->ovf_pending = true;
+}
+
+int check_overflow(pic_handle *h, bool wait) {
+    if (!h->ovf_pending) return PIC_OK;
+    if (wait) {
+        HIPCHK(h, hipEventSynchronize(h->ovf_ev));
+    } else if (hipEventQuery(h->ovf_ev) != hipSuccess) {
+        return PIC_OK;  // not landed yet: a later call reports it
+    }
+    h->ovf_pending = false;
+    const uint64_t dropped = (uint64_t)h->ovf_host[0] + h->ovf_host[1];
+    if (dropped > h->ovf_reported) {
+        h->ovf_reported = dropped;
+        return seterr(h, PIC_ERR_CAPACITY,
+                      std::to_string(dropped) + " particles dropped: particle store or staging "
+                      "buffer full (electrons " + std::to_string(h->ovf_host[0]) + ", ions " +
+                      std::to_string(h->ovf_host[1]) + ")");
+    }
+    return PIC_OK;
+}
+
 // Physical-mode collisions run in a pass of their own ahead of the particle
 // pass when few particles are candidates (the argon ions: a fraction of a
 // percent): that pass then has the registers of a collisionless one, for 8
@@ -1020,9 +1051,13 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     }
     if ((rc = dalloc(h, &h->cnt, 2)) != PIC_OK) return fail(rc);
     if ((rc = diag_alloc(h)) != PIC_OK) return fail(rc);
-    if (hipHostMalloc((void **)&h->cnt_host, 2 * sizeof(SpCounters), 0) != hipSuccess)
+    if (hipHostMalloc((void **)&h->cnt_host, 2 * sizeof(SpCounters), 0) != hipSuccess ||
+        hipHostMalloc((void **)&h->ovf_host, 2 * sizeof(uint32_t), 0) != hipSuccess)
         return fail(set_global_error(PIC_ERR_ALLOC, "hipHostMalloc failed"));
+    if (hipEventCreateWithFlags(&h->ovf_ev, hipEventDisableTiming) != hipSuccess)
+        return fail(set_global_error(PIC_ERR_DEVICE, "hipEventCreate failed"));
     memset(h->cnt_host, 0, 2 * sizeof(SpCounters));
+    h->ovf_host[0] = h->ovf_host[1] = 0;
     for (int s = 0; s < 2; s++) {
         h->cnt_host[s].n[0] = (uint32_t)n0[s];
         h->cnt_host[s].next_id = (uint32_t)n0[s];
@@ -1101,6 +1136,8 @@ int pic_destroy(pic_handle *h) {
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (h->cnt_host) hipHostFree(h->cnt_host);
+    if (h->ovf_host) hipHostFree(h->ovf_host);
+    if (h->ovf_ev) hipEventDestroy(h->ovf_ev);
     diag_free(h);
     void *cx[] = {h->cx.key[0], h->cx.key[1], h->cx.val[0], h->cx.val[1], h->cx.temp, h->cx.pcol};
     for (void *p : cx)
@@ -1143,6 +1180,9 @@ int pic_run(pic_handle *h) {
     for (int32_t t = 0; t < h->run_time; t++) CHECK(fused_step(h));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return seterr(h, PIC_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
+    // pic::pic_run returns when the run is complete
+    queue_overflow_copy(h);
+    CHECK(check_overflow(h, true));
     // pic.cpp:1262-1274: timeData.txt at the end of the run
     if (diag_time_on(h) && !h->out_dir.empty()) CHECK(diag_write_time(h, h->out_dir));
     return PIC_OK;
@@ -1150,17 +1190,19 @@ int pic_run(pic_handle *h) {
 
 int pic_advance(pic_handle *h, int32_t nsteps) {
     if (!h || nsteps < 0) return set_global_error(PIC_ERR_ARG, "bad argument");
+    CHECK(check_overflow(h, false));  // drops of an earlier call that have landed
     CHECK(diag_prepare_time(h, (int32_t)std::min<int64_t>(h->step + nsteps, INT32_MAX)));
     for (int32_t t = 0; t < nsteps; t++) CHECK(fused_step(h));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return seterr(h, PIC_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
+    queue_overflow_copy(h);
     return PIC_OK;
 }
 
 int pic_sync(pic_handle *h) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
     HIPCHK(h, hipStreamSynchronize(h->st));
-    return PIC_OK;
+    return check_overflow(h, true);
 }
 
 // ------------------------------------------------------ per-phase entry

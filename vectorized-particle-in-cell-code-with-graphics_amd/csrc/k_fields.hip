@@ -987,8 +987,8 @@ __device__ __forceinline__ double block_sum_d(double v) {
 // partials in block order, sets flags[0] when ||r|| <= tol ||f||, counts an
 // applied cycle in flags[1] when `applied`, and publishes both and the
 // relative residual to the host-mapped copies, then (after a system-scope
-// fence) the count of residual passes of this solve (flags[3]) in
-// host_flags[2], on which the host waits.
+// fence) the count of residual passes of this solve (flags[3]) with the
+// convergence bit MG_PUB_CONV in host_flags[2], on which the host waits.
 __device__ __forceinline__ void tol_finish(double rr, double ff, int b, int nb,
                                            double *__restrict__ partials, double tol,
                                            int *__restrict__ flags, bool applied,
@@ -1024,7 +1024,10 @@ __device__ __forceinline__ void tol_finish(double rr, double ff, int b, int nb,
         host_flags[1] = flags[1];
         __threadfence_system();
         flags[3] += 1;
-        host_flags[2] = flags[3];
+        // one word carries the pass count and the verdict, so the host that
+        // acquires it also sees the applied-cycle count and the residual
+        // written before the fence
+        host_flags[2] = flags[3] | (flags[0] ? MG_PUB_CONV : 0);
     }
 }
 
@@ -1195,9 +1198,9 @@ static hipError_t wait_published(hipStream_t st, MgLevels &M, int passes) {
     if (e != hipSuccess) return e;
     const auto t0 = std::chrono::steady_clock::now();
     for (;;) {
-        const int p = __atomic_load_n(&M.host_flags[2], __ATOMIC_ACQUIRE);
-        if (p >= passes || (p > 0 && __atomic_load_n(&M.host_flags[0], __ATOMIC_ACQUIRE)))
-            return hipSuccess;
+        const int w = __atomic_load_n(&M.host_flags[2], __ATOMIC_ACQUIRE);
+        const int p = w & MG_PUB_COUNT;
+        if (p >= passes || (p > 0 && (w & MG_PUB_CONV))) return hipSuccess;
         if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(2000)) break;
     }
     return hipEventSynchronize(M.done);
@@ -1250,18 +1253,22 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     // queue the predicted number of cycles without waiting (converged cycles
     // skip their work on the device), then confirm on the host
     int batch = M.pred < 1 ? 1 : M.pred;
+    bool conv = false;
     for (;;) {
         for (int k = 0; k < batch && queued < max_iter; k++, queued++) cycle();
         if (after_batch) after_batch(buf[0], buf[1]);
         // the initial residual pass plus one per queued cycle
         if (wait_published(st, M, 1 + queued) != hipSuccess) return -1;
-        if (M.host_flags[0] || queued >= max_iter) break;
+        // the published word (acquired) orders the reads of host_flags[1]
+        // and the residual after it
+        conv = (__atomic_load_n(&M.host_flags[2], __ATOMIC_ACQUIRE) & MG_PUB_CONV) != 0;
+        if (conv || queued >= max_iter) break;
         batch = 1;
     }
     int used = M.host_flags[1];
     // the iterate of an odd number of applied cycles is in ualt: it becomes u
     if (used & 1) std::swap(M.d.u[lf], M.ualt);
-    M.pred = used + (M.host_flags[0] ? 0 : 1);
+    M.pred = used + (conv ? 0 : 1);
     if (relres_out) *relres_out = *M.relres;
     return used;
 }

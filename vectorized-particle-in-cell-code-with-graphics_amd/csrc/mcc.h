@@ -321,7 +321,7 @@ struct Child {
 #define RS_IONIZE 2
 #define RS_PLOP 3
 #define RS_PLOP2 4
-// 5: second draw of a re-injected pair
+// 5, 6: second and third draws of a re-injected pair
 #define RS_REFCX 9   // reference-mode charge exchange: neutral table index
 #define RS_INJECT 10  // particle source: + 256 k for the k-th draw of a particle
 #define RS_REDUCE 11  // population reduction: selection key
@@ -508,10 +508,13 @@ __device__ __forceinline__ int collide_ion(const MccConst &c, float &vx, float &
 // v = vt cos(2 pi R2) sqrt|ln R1| (boundary.cpp:833-845), sigma = vt/sqrt 2.
 __device__ __forceinline__ void plop_pair(const MccConst &c, uint32_t id, uint32_t step, int N,
                                           float se, float si, int ref, PState &e, PState &i) {
-    uint32_t r[4], a[4], b[4];
+    // 14 words from four counters: 2 for the position, 12 for the six
+    // Box-Muller pairs, none used twice
+    uint32_t r[4], a[4], b[4], d[4];
     philox4x32_10(id, step, 16u + RS_PLOP, RNG_DOMAIN_MCC, c.key0, c.key1, r);
     philox4x32_10(id, step, 16u + RS_PLOP2, RNG_DOMAIN_MCC, c.key0, c.key1, a);
     philox4x32_10(id, step, 16u + RS_PLOP2 + 1u, RNG_DOMAIN_MCC, c.key0, c.key1, b);
+    philox4x32_10(id, step, 16u + RS_PLOP2 + 2u, RNG_DOMAIN_MCC, c.key0, c.key1, d);
     float span = (float)(N - 3);
     float tx = 1.0f + u01(r[0]) * span, ty = 1.0f + u01(r[1]) * span;
     int jx = (int)floorf(tx), jy = (int)floorf(ty);
@@ -524,7 +527,7 @@ __device__ __forceinline__ void plop_pair(const MccConst &c, uint32_t id, uint32
     e.oy = i.oy = oy;
     float n[6];
     uint32_t w[6] = {a[0], a[1], a[2], a[3], b[0], b[1]};
-    uint32_t w2[6] = {b[2], b[3], r[2], r[3], a[1] ^ b[2], a[3] ^ b[0]};
+    uint32_t w2[6] = {b[2], b[3], r[2], r[3], d[0], d[1]};
     for (int k = 0; k < 6; k++) {
         float R1 = u01(w[k]), R2 = u01(w2[k]);
         n[k] = ref ? cosf(6.2831853f * R2) * sqrtf(fabsf(logf(R1)))

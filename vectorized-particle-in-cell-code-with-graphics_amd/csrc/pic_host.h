@@ -78,6 +78,13 @@ struct pic_handle {
     pic_host::Species sp[2];
     pic::SpCounters *cnt = nullptr;       // device, [2]
     pic::SpCounters *cnt_host = nullptr;  // pinned mirror
+    // particles dropped for lack of capacity: both species' overflow counters
+    // copied (asynchronously) after every pic_advance, checked at the next
+    // call or at pic_sync; an increase is reported as PIC_ERR_CAPACITY
+    uint32_t *ovf_host = nullptr;  // pinned, [2]
+    hipEvent_t ovf_ev = nullptr;
+    bool ovf_pending = false;
+    uint64_t ovf_reported = 0;
     uint32_t *binCount = nullptr, *binStart = nullptr, *scanPartial = nullptr;
     double *rho = nullptr, *phi = nullptr, *ex64 = nullptr, *ey64 = nullptr;
     float2 *E32 = nullptr;

@@ -70,6 +70,11 @@ struct MgArrays {
     T *us[16] = {};    // pre-smoothed iterate of the fused V-cycle (fp32 levels > lc)
 };
 
+// host_flags[2] of the tolerance solve: residual passes published so far,
+// and the convergence bit
+#define MG_PUB_COUNT 0x3FFFFFFF
+#define MG_PUB_CONV 0x40000000
+
 struct MgLevels {
     int ng = 0, lc = 0;
     MgArrays<double> d;  // FP64 full multigrid (fixed mode) and the FP64 iterate
