@@ -284,7 +284,7 @@ int pic_get_state(pic_handle *h, int32_t s, uint32_t *cell, float *ox, float *oy
 int pic_set_state(pic_handle *h, int32_t s, const uint32_t *cell, const float *ox,
                   const float *oy, const float *vx, const float *vy, const float *vz,
                   const uint32_t *id, int64_t n);
-int pic_get_qfixed(pic_handle *h, int32_t s, int64_t *out); /* nodes*nodes, 2^-40 units   */
+int pic_get_qfixed(pic_handle *h, int32_t s, int64_t *out); /* nodes*nodes, 2^-48 units   */
 /* per-particle weights (particleVec.h: weight) in pic_get_state order; the
  * charge of a particle is weight * qc (pic.cl:46-49).  Setting them switches
  * the species to per-particle weights. */

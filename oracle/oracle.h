@@ -144,7 +144,7 @@ void orc32_deposit(long n, const uint32_t *cell, const float *ox, const float *o
 
 /* Fixed-point charge -> rho -> Poisson rhs (particle.cpp:830-845 post-scale,
  * fields.cpp:100-143 sum order, fields.cpp:184-204 rhs and Dirichlet folding).
- * scale_s = qc_s/(dx*dy) * weight_s * 2^-40.  corner8: corners x8 (reference)
+ * scale_s = qc_s/(dx*dy) * weight_s * 2^-48.  corner8: corners x8 (reference)
  * else x4.  rho and rhs are n*n. */
 void orc_fixed_to_rhs(const int64_t *q_ion, const int64_t *q_e, int n, double scale_i,
                       double scale_e, int corner8, double eps, double phi0X, double phiR,
@@ -189,7 +189,16 @@ typedef struct {
     double cos_a, sin_a, cos_we, sin_we, cos_wi, sin_wi, bfield;
     double phi_left, phi_right, phi_bottom, phi_top, phi_period;
     long start_step;
+    /* 0: the reference's FMG with ncycle V-cycles (fields.cpp:174); > 0: the
+     * same folded right-hand side solved to ||f - Au|| <= mg_tol ||f||
+     * (FMG start, then FP64 V(1,1) cycles): a converged FP64 solve */
+    double mg_tol;
 } orc_ref_config;
+
+/* fields::grid_potential (fields.cpp:169-220) with the solve selected by
+ * c->mg_tol (see orc_ref_config) */
+int orc_ref_potential_cfg(const orc_ref_config *c, const double *rho, double timestep,
+                          double *phi);
 
 /* FP64 reference step loop (also the CPU baseline; OpenMP-parallel gather,
  * push and deposit when more than one thread is set).  Ex/Ey: (N+1)^2. */

@@ -136,7 +136,7 @@ long orc_reduce(const orc_reduce_params *c, uint32_t step, long n, const float *
  * orc32_deposit times the particle weight, rounded to nearest (even) once */
 void orc32_deposit_w(long n, const uint32_t *cell, const float *ox, const float *oy,
                      const float *w, int N, int mode, int64_t *qfix) {
-    const int64_t ONE = (int64_t)1 << 20;
+    const int64_t ONE = (int64_t)1 << 24;
     for (long g = 0; g < n; g++) {
         int jx = orc_cell_jx(cell[g]), jy = orc_cell_jy(cell[g]);
         float fx = ox[g], fy = oy[g];
@@ -147,8 +147,8 @@ void orc32_deposit_w(long n, const uint32_t *cell, const float *ox, const float 
             ok = jx >= 0 && jx <= N - 2 && !(jx == 0 && fx == 0.0f) && jy >= 0 && jy <= N - 2 &&
                  !(jy == 0 && fy == 0.0f);
         if (!ok) continue;
-        int64_t wx = (int64_t)rintf(fx * 1048576.0f);
-        int64_t wy = (int64_t)rintf(fy * 1048576.0f);
+        int64_t wx = (int64_t)rintf(fx * 16777216.0f);
+        int64_t wy = (int64_t)rintf(fy * 16777216.0f);
         int64_t a[4] = {(ONE - wx) * (ONE - wy), wx * (ONE - wy), (ONE - wx) * wy, wx * wy};
         if (w[g] != 1.0f)
             for (int k = 0; k < 4; k++) a[k] = (int64_t)llrint((double)a[k] * (double)w[g]);

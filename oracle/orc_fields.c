@@ -247,6 +247,38 @@ int orc_ref_grid_potential(const double *rho, int n, double dx, double dy, doubl
     return 0;
 }
 
+int orc_ref_potential_cfg(const orc_ref_config *c, const double *rho, double timestep,
+                          double *phi) {
+    if (!(c->mg_tol > 0.0))
+        return orc_ref_grid_potential(rho, c->N, c->dx, c->dy, c->eps, c->phi_left, c->phi_right,
+                                      c->phi_bottom, c->phi_top, timestep, c->phi_period,
+                                      c->ncycle, c->L, phi);
+    /* the folded right-hand side and boundary values of fields.cpp:169-220,
+     * with the converged solve in place of Mglin's fixed cycles */
+    const int n = c->N;
+    const double coef = 1.0 / c->eps, divdxdy = 1.0 / (c->dx * c->dy);
+    const double phi_right = c->phi_right * cos(c->phi_period * timestep);
+    for (int k = 0; k < n * n; k++) phi[k] = -coef * rho[k];
+    for (int j = 0; j < n; j++) {
+        phi[j * n + 1] = phi[j * n + 1] - c->phi_left * divdxdy;
+        phi[j * n + n - 2] = phi[j * n + n - 2] - phi_right * divdxdy;
+    }
+    for (int i = 0; i < n; i++) {
+        phi[1 * n + i] = phi[1 * n + i] - c->phi_bottom * divdxdy;
+        phi[(n - 2) * n + i] = phi[(n - 2) * n + i] - c->phi_top * divdxdy;
+    }
+    if (orc_mg_tol(phi, n, c->L, c->mg_tol, 1000, NULL) < 0) return -1;
+    for (int j = 0; j < n; j++) {
+        phi[j * n + 0] = c->phi_left;
+        phi[j * n + n - 1] = phi_right;
+    }
+    for (int i = 0; i < n; i++) {
+        phi[i] = c->phi_bottom;
+        phi[(n - 1) * n + i] = c->phi_top;
+    }
+    return 0;
+}
+
 void orc_ref_grid_efield(const double *phi, int n, double dx, double *Ex, double *Ey) {
     double twoDX = 2.0 * dx;
     for (int j = 1; j < n - 1; j++)

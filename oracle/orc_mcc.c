@@ -386,8 +386,7 @@ int orc_ref_run_mcc(const orc_ref_config *c, const orc_mcc_params *me, const orc
         orc_i_deposit_par(e, *ne, c->qc_dxdy_e, c, qe);
         orc_ref_charge_postscale(qe, N, N);
         orc_ref_grid_rho(qi, qe, N, rho);
-        if (orc_ref_grid_potential(rho, N, dx, dx, c->eps, c->phi_left, c->phi_right, c->phi_bottom,
-                                   c->phi_top, (double)step, c->phi_period, c->ncycle, L, phi) != 0) {
+        if (orc_ref_potential_cfg(c, rho, (double)step, phi) != 0) {
             free(scratch);
             return -1;
         }

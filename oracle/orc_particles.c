@@ -285,7 +285,7 @@ void orc32_push(long n, uint32_t *cell, float *ox, float *oy, float *vx, float *
 
 void orc32_deposit(long n, const uint32_t *cell, const float *ox, const float *oy, int N,
                    int mode, int64_t *qfix) {
-    const int64_t ONE = (int64_t)1 << 20;
+    const int64_t ONE = (int64_t)1 << 24;
     for (long g = 0; g < n; g++) {
         int jx = orc_cell_jx(cell[g]), jy = orc_cell_jy(cell[g]);
         float fx = ox[g], fy = oy[g];
@@ -296,8 +296,8 @@ void orc32_deposit(long n, const uint32_t *cell, const float *ox, const float *o
             ok = jx >= 0 && jx <= N - 2 && !(jx == 0 && fx == 0.0f) && jy >= 0 && jy <= N - 2 &&
                  !(jy == 0 && fy == 0.0f);
         if (!ok) continue;
-        int64_t wx = (int64_t)rintf(fx * 1048576.0f);
-        int64_t wy = (int64_t)rintf(fy * 1048576.0f);
+        int64_t wx = (int64_t)rintf(fx * 16777216.0f);
+        int64_t wy = (int64_t)rintf(fy * 16777216.0f);
         long b = (long)jy * N + jx;
         qfix[b] += (ONE - wx) * (ONE - wy);
         qfix[b + 1] += wx * (ONE - wy);

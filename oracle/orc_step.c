@@ -99,9 +99,7 @@ int orc_ref_run(const orc_ref_config *c, int nsteps, long *ne, orc_particle *e, 
         orc_i_deposit_par(e, *ne, c->qc_dxdy_e, c, qe);
         orc_ref_charge_postscale(qe, N, N);
         orc_ref_grid_rho(qi, qe, N, rho);
-        if (orc_ref_grid_potential(rho, N, c->dx, c->dy, c->eps, c->phi_left, c->phi_right,
-                                   c->phi_bottom, c->phi_top, (double)step, c->phi_period,
-                                   c->ncycle, c->L, phi) != 0) {
+        if (orc_ref_potential_cfg(c, rho, (double)step, phi) != 0) {
             free(scratch);
             return -1;
         }

@@ -11,6 +11,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_MGLIN_SO = os.path.join(ROOT, "oracle", "_ref", "libref_mglin.so")
+# fixed-point charge: a unit-weight particle deposits 2^FX_TOTAL_BITS in all
+FX_TOTAL_BITS = 48
 
 vp = C.c_void_p
 MODE_REF, MODE_PHYS, MODE_REF_STEADY = 0, 1, 2
@@ -45,7 +47,7 @@ class CfgRef(C.Structure):
                                   "qc_dxdy_e", "qc_dxdy_i", "cos_a", "sin_a", "cos_we", "sin_we",
                                   "cos_wi", "sin_wi", "bfield", "phi_left", "phi_right",
                                   "phi_bottom", "phi_top", "phi_period")
-    ] + [("start_step", C.c_long)]
+    ] + [("start_step", C.c_long), ("mg_tol", C.c_double)]
 
 
 class MccParams(C.Structure):
@@ -326,8 +328,8 @@ def configs_from_params(p, mode=None):
                 mg_max_cycles=p.mg_max_iter, dt_dti=p.dt_dti, preloop=0, L=p.length, dx=dx,
                 eps=p.epsilon, mg_tol=p.mg_tol, phi_left=p.phi_left, phi_right=p.phi_right,
                 phi_bottom=p.phi_bottom, phi_top=p.phi_top, phi_period=p.phi_period,
-                scale_e=math.ldexp(p.qce / (dx * dx) * 1.0, -40),
-                scale_i=math.ldexp(p.qci / (dx * dx) * 1.0, -40), start_step=0,
+                scale_e=math.ldexp(p.qce / (dx * dx) * 1.0, -FX_TOTAL_BITS),
+                scale_i=math.ldexp(p.qci / (dx * dx) * 1.0, -FX_TOTAL_BITS), start_step=0,
                 pe=pp(qme, p.dt), pi=pp(qmi, p.dti), key0=getattr(p, "seed", 0) & 0xFFFFFFFF,
                 key1=(getattr(p, "seed", 0) >> 32) & 0xFFFFFFFF, steady_se=getattr(p, "vte", 0.0),
                 steady_si=getattr(p, "vti", 0.0))

@@ -83,17 +83,22 @@ def test_fused_step_tracks_fp64_reference():
                 s2.run(steps)
                 g = by_id(s2.get_state(0))
                 phi = s2.get_phi()
-                gex, _ = s2.get_E()
+                gex, gey = s2.get_E()
             a = orc.state_to_aos(g, dx)
             idx = {int(k): j for j, k in enumerate(g["id"])}
             pairs = [(idx[int(k)], j) for j, k in enumerate(rr["eid"]) if int(k) in idx]
-            assert len(pairs) >= 0.99 * len(rr["eid"])
+            assert len(pairs) >= 0.9999 * len(rr["eid"])
             ga, gb = a[[x for x, _ in pairs]], rr["e"][[y for _, y in pairs]]
+            errs = {}
             for f in ("x", "y", "vx", "vy"):
-                rel = np.linalg.norm(ga[f] - gb[f]) / np.linalg.norm(gb[f])
-                assert rel < 1e-6, (steps, f, rel)
-            rel_phi = np.linalg.norm(phi - rr["phi"]) / np.linalg.norm(rr["phi"])
-            assert rel_phi < 2e-5, (steps, rel_phi)
+                errs[f] = np.linalg.norm(ga[f] - gb[f]) / np.linalg.norm(gb[f])
+            errs["phi"] = np.linalg.norm(phi - rr["phi"]) / np.linalg.norm(rr["phi"])
+            errs["Ex"] = np.linalg.norm(gex - rr["Ex"]) / np.linalg.norm(rr["Ex"])
+            errs["Ey"] = np.linalg.norm(gey - rr["Ey"]) / np.linalg.norm(rr["Ey"])
+            print(f"{steps} steps vs FP64:", " ".join(f"{k} {v:.2e}" for k, v in errs.items()),
+                  flush=True)
+            for k, v in errs.items():
+                assert v < 1e-6, (steps, k, v)
 
 
 def test_phased_entry_points_reproduce_fused_step():

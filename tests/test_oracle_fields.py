@@ -69,12 +69,13 @@ def test_potential_with_dirichlet_walls_against_dense_solve():
 
 def test_fixed_to_rhs_postscale_and_sum_order():
     N = 9
-    qi = np.full(N * N, 1 << 40, dtype=np.int64)
-    qe = np.full(N * N, 3 << 40, dtype=np.int64)
+    qi = np.full(N * N, 1 << orc.FX_TOTAL_BITS, dtype=np.int64)
+    qe = np.full(N * N, 3 << orc.FX_TOTAL_BITS, dtype=np.int64)
     rho = np.zeros(N * N)
     rhs = np.zeros(N * N)
     eps, s_i, s_e = 2.0, 1.0, -1.0
-    orc.lib().orc_fixed_to_rhs(orc.P(qi), orc.P(qe), N, math.ldexp(s_i, -40), math.ldexp(s_e, -40),
+    fx = orc.FX_TOTAL_BITS
+    orc.lib().orc_fixed_to_rhs(orc.P(qi), orc.P(qe), N, math.ldexp(s_i, -fx), math.ldexp(s_e, -fx),
                                1, eps, 0.0, 0.0, 0.0, 0.0, 1.0, orc.P(rho), orc.P(rhs))
     rho = rho.reshape(N, N)
     # interior 1 - 3 = -2; edges x2; corners x8 (particle.cpp:830-845)

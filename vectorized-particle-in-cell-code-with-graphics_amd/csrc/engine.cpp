@@ -484,7 +484,7 @@ void commit_both(pic_handle *h, int dep_i, int dep_e) {
 }
 
 void update_scale(pic_handle *h, Species &s) {
-    s.scale = ldexp(s.qc / (h->dx * h->dx) * s.weight, -40);
+    s.scale = ldexp(s.qc / (h->dx * h->dx) * s.weight, -2 * FX_BITS);
 }
 
 // switch a species to per-particle weights: every particle takes the

@@ -38,8 +38,12 @@
 #define POS_GUARD 8192.0f
 #define CELL_FAR 16000
 
-// fixed-point deposit: weights quantised to 2^-20 of a cell, 2^40 per particle
-#define FX_BITS 20
+// fixed-point deposit: weights quantised to 2^-24 of a cell, 2^48 per
+// particle (an fp32 offset in [0.5, 1) has exactly 24 fraction bits, so the
+// weights keep every bit of the offset there); an int64 node accumulator
+// holds 2^15 unit-weight particles of one species without overflow
+#define FX_BITS 24
+#define FX_SCALE_F 16777216.0f
 #define FX_ONE (1ll << FX_BITS)
 
 // per-species constants of one push (particle.cpp:1103-1112 in fp32)
@@ -177,15 +181,15 @@ __device__ __forceinline__ bool deposit_ok(const PState &s, int N, int mode) {
 }
 
 // fixed-point CIC corner weights (Birdsall & Langdon bilinear weighting, as in
-// pic.cl:40-43), summing to exactly 2^40
+// pic.cl:40-43), summing to exactly 2^48
 struct FxW {
     unsigned long long a4, a3, a2, a1;
 };
-// With S = 2^20 and a1 = wx wy (one 32 x 32 -> 64-bit product), the other
+// With S = 2^24 and a1 = wx wy (one 32 x 32 -> 64-bit product), the other
 // corners are exactly S wx - a1, S wy - a1 and S^2 - S wx - S wy + a1.
 __device__ __forceinline__ FxW fx_weights(float ox, float oy) {
-    const int wx = (int)rintf(ox * 1048576.0f);
-    const int wy = (int)rintf(oy * 1048576.0f);
+    const int wx = (int)rintf(ox * FX_SCALE_F);
+    const int wy = (int)rintf(oy * FX_SCALE_F);
     const long long a1 = (long long)wx * (long long)wy;
     const long long sx = (long long)wx << FX_BITS, sy = (long long)wy << FX_BITS;
     FxW w;

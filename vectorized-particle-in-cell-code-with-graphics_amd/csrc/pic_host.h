@@ -21,7 +21,7 @@ struct Species {
     float *exp_ = nullptr, *eyp = nullptr;  // per-phase gather output
     bool gathered = false;
     double q_over_m = 0, qc = 0, mass = 0, dt = 0, weight = 1.0, sigma_load = 0;
-    double scale = 0;  // qc/(dx*dy) * weight * 2^-40
+    double scale = 0;  // qc/(dx*dy) * weight * 2^-48
     PushConst pc = {};
     mcc::MccConst mc = {};
     pic::StageView stage = {};  // particles created during a step
