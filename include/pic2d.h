@@ -216,6 +216,24 @@ int pic_inject_particles(pic_handle *h, int32_t s);
  * pic_params.reduce; the fused timestep runs it every step. */
 int pic_contain_particle_growth(pic_handle *h, int32_t s);
 
+/* Accuracy of the last field solve against an FP64 multigrid (no reference
+ * counterpart: the error report of the split-precision PIC_MG_TOL solver,
+ * which replaces Mglin's fixed-cycle FP64 FMG).  The last solve's folded
+ * right-hand side (fields.cpp:169-220) is solved again by the FP64 multigrid
+ * alone -- the full multigrid of pic_mglin (mglin.cpp:21-55), then FP64
+ * V(1,1) cycles (mglin.cpp:176-196) -- until ||f - Au||_2 <= tol ||f||_2, and
+ * compared node by node: out5 = relative L2 error of phi, of Ex, of Ey (the
+ * nodes*nodes values), the reference solve's relative residual, its V-cycles
+ * after the full multigrid.  Synchronises the handle's stream. */
+int pic_field_error(pic_handle *h, double tol, double *out5);
+/* pic_field_error(h, tol) after every `every`-th step of pic_run and
+ * pic_advance (0: off), one row of 6 doubles per report: step, the solver's
+ * relative residual of that step, error of phi, Ex, Ey, the reference
+ * solve's relative residual.  pic_get_accuracy_report copies at most
+ * cap_rows rows and returns the count in *rows. */
+int pic_set_accuracy_report(pic_handle *h, int32_t every, double tol);
+int pic_get_accuracy_report(pic_handle *h, double *out, int32_t cap_rows, int32_t *rows);
+
 /* Cross sections the collision kernels use, evaluated on the host in FP64
  * (e_collisions::sigma_Ar_elastic / sigma_Ar_ionize / sigma_H_elastic /
  * sigma_H_ionize, e_collisions.h:192-195; ion_collisions::sigma_Ar_cx /

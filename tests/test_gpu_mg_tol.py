@@ -124,3 +124,40 @@ def test_tolerance_mode_steps_against_converged_fp64(tol):
     assert worst[0] < 1e-6 and worst[1] < 1e-6
     if tol <= 1e-10:
         assert worst[0] < 1e-9
+
+
+def test_field_error_report_against_oracle():
+    """pic_field_error: the library's FP64 reference solve reproduces the
+    error the oracle measures.  With the reference's fixed-cycle FMG
+    (ncycle 2, not converged) phi is off by a visible amount, which both
+    must agree on; in tolerance mode the periodic report keeps one row per
+    `every` steps with errors below the solver's target."""
+    p = workloads.discharge(64, 20, mode=pic2d.MODE_PHYSICAL, mg_mode=pic2d.MG_FIXED, te=4.0)
+    N = p.nodes
+    dx = p.length / (N - 1)
+    with pic2d.Pic(p) as sim:
+        sim.run(3)
+        fe = sim.field_error(1e-13)
+        rho, phi = sim.get_rho(), sim.get_phi()
+        gex, gey = sim.get_E()
+    u, _, _ = orc.mg_tol(-rho / p.epsilon, p.length, 1e-13, max_cycles=300)
+    ex, ey = orc.ref_efield(u, dx)
+    want = (np.linalg.norm(phi - u) / np.linalg.norm(u),
+            np.linalg.norm(gex - ex) / np.linalg.norm(ex),
+            np.linalg.norm(gey - ey) / np.linalg.norm(ey))
+    print("field_error fixed mode:", fe, "oracle:", want, flush=True)
+    assert fe["ref_relres"] <= 1e-13
+    assert want[0] > 1e-7  # fixed cycles leave a measurable error
+    for got, w in zip((fe["phi"], fe["ex"], fe["ey"]), want):
+        assert abs(got - w) <= 1e-4 * w
+    p.mg_mode = pic2d.MG_TOL
+    p.mg_tol = 1e-10
+    with pic2d.Pic(p) as sim:
+        sim.set_accuracy_report(3)
+        sim.run(9)
+        rows = sim.accuracy_report()
+    print("accuracy report rows:", rows.tolist(), flush=True)
+    assert rows.shape == (3, 6)
+    assert np.array_equal(rows[:, 0], [2, 5, 8])
+    assert np.all(rows[:, 1] <= 1e-10) and np.all(rows[:, 5] <= 1e-12)
+    assert np.all(rows[:, 2:5] < 1e-8)

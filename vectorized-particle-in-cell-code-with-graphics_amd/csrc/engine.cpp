@@ -740,6 +740,10 @@ int solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, doub
     const pic_params &P = h->P;
     int N = h->N;
     MgLevels &M = h->M;
+    h->last_bc[0] = phi0X;
+    h->last_bc[1] = phiR;
+    h->last_bc[2] = phi0Y;
+    h->last_bc[3] = phiLY;
     if (!timestep)
         launch_fold(h->st, h->rho, N, 1.0 / P.epsilon, phi0X, phiR, phi0Y, phiLY,
                     1.0 / (h->dx * h->dx), M.d.rhsF[M.ng - 1]);
@@ -793,6 +797,62 @@ int field_step(pic_handle *h, int64_t step) {
     return PIC_OK;
 }
 
+#define REF_BLOCKS 512  // blocks of the report's reduction kernels
+
+// FP64 levels of the accuracy report (rhsF, u and the V-cycle arrays), the
+// reference potential and field, and the block sums
+int alloc_ref(pic_handle *h) {
+    if (h->ref_phi) return PIC_OK;
+    MgLevels &R = h->ref;
+    R.ng = h->M.ng;
+    R.lc = h->M.lc;
+    for (int l = R.lc; l < R.ng; l++) {
+        size_t s = (size_t)((1 << (l + 1)) + 1) * ((1 << (l + 1)) + 1);
+        CHECK(dalloc(h, &R.d.rhsF[l], s));
+        CHECK(dalloc(h, &R.d.u[l], s));
+        if (l < R.ng - 1) {
+            CHECK(dalloc(h, &R.d.cres[l], s));
+            CHECK(dalloc(h, &R.d.cv[l], s));
+        }
+    }
+    const size_t N = (size_t)h->N, E = (N + 1) * (N + 1);
+    CHECK(dalloc(h, &h->ref_ex, E));
+    CHECK(dalloc(h, &h->ref_ey, E));
+    CHECK(dalloc(h, &h->ref_E32, E));
+    CHECK(dalloc(h, &h->ref_partials, 6 * (size_t)REF_BLOCKS));
+    CHECK(dalloc(h, &h->ref_phi, N * N));
+    h->ref_host.assign(6 * REF_BLOCKS, 0.0);
+    return PIC_OK;
+}
+
+// relative L2 errors of phi, Ex, Ey against the FP64 reference solve of the
+// last solve's right-hand side (e[0..2]), its relative residual (e[3]) and
+// V-cycles after the full multigrid (e[4]); e[5] = 0
+int field_error(pic_handle *h, double tol, double *e) {
+    if (!h->have_phi) return seterr(h, PIC_ERR_ARG, "no potential solved yet");
+    CHECK(alloc_ref(h));
+    MgLevels &R = h->ref;
+    const int N = h->N, lf = R.ng - 1;
+    HIPCHK(h, hipMemcpyAsync(R.d.rhsF[lf], h->M.d.rhsF[lf], sizeof(double) * N * N,
+                             hipMemcpyDeviceToDevice, h->st));
+    double rr = 0.0;
+    const int cyc = mg_fp64_reference(h->st, R, h->P.length, tol, 200, h->ref_partials,
+                                      h->ref_host.data(), REF_BLOCKS, &rr);
+    if (cyc < 0) return seterr(h, PIC_ERR_DEVICE, "reference solve: stream error");
+    const double *b = h->last_bc;
+    launch_phi_bc(h->st, R.d.u[lf], N, b[0], b[1], b[2], b[3], h->ref_phi);
+    launch_efield(h->st, h->ref_phi, N, h->dx, h->ref_ex, h->ref_ey, h->ref_E32);
+    double s[6];
+    if (field_diff_sums(h->st, h->phi, h->ref_phi, h->ex64, h->ref_ex, h->ey64, h->ref_ey,
+                        (long)N * N, h->ref_partials, h->ref_host.data(), REF_BLOCKS, s) != 0)
+        return seterr(h, PIC_ERR_DEVICE, "reference comparison: stream error");
+    for (int k = 0; k < 3; k++) e[k] = s[2 * k + 1] > 0 ? sqrt(s[2 * k] / s[2 * k + 1]) : sqrt(s[2 * k]);
+    e[3] = rr;
+    e[4] = cyc;
+    e[5] = 0.0;
+    return PIC_OK;
+}
+
 int fused_step(pic_handle *h) {
     int N = h->N;
     int pt = prof_begin(h, 0);
@@ -831,6 +891,12 @@ int fused_step(pic_handle *h) {
     diag_step(h, h->step);
     prof_end(h, pt);
     h->have_rho = true;
+    if (h->acc_every > 0 && (h->step + 1) % h->acc_every == 0) {
+        double e[6];
+        CHECK(field_error(h, h->acc_tol, e));
+        const double row[6] = {(double)h->step, h->mg_relres, e[0], e[1], e[2], e[3]};
+        h->acc_rows.insert(h->acc_rows.end(), row, row + 6);
+    }
     h->step++;
     return PIC_OK;
 }
@@ -1143,6 +1209,10 @@ int pic_destroy(pic_handle *h) {
     for (void *p : cx)
         if (p) hipFree(p);
     free_mg(h->M);
+    free_mg(h->ref);
+    for (void *p : {(void *)h->ref_phi, (void *)h->ref_ex, (void *)h->ref_ey, (void *)h->ref_E32,
+                    (void *)h->ref_partials})
+        if (p) hipFree(p);
     for (ProfPair &p : h->prof_live) {
         hipEventDestroy(p.a);
         hipEventDestroy(p.b);
@@ -1284,6 +1354,31 @@ int pic_ion_collisions(pic_handle *h) {
     if (!s.mc.on) return PIC_OK;
     launch_mcc(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], s.cap, s.pc, s.mc, (uint32_t)h->step,
                make_spawn(h, false, -1), &h->cnt[1].events[0][0]);
+    return PIC_OK;
+}
+
+int pic_field_error(pic_handle *h, double tol, double *out5) {
+    if (!h || !out5 || !(tol > 0.0)) return set_global_error(PIC_ERR_ARG, "bad argument");
+    double e[6];
+    CHECK(field_error(h, tol, e));
+    for (int k = 0; k < 5; k++) out5[k] = e[k];
+    return PIC_OK;
+}
+
+int pic_set_accuracy_report(pic_handle *h, int32_t every, double tol) {
+    if (!h || every < 0 || !(tol > 0.0)) return set_global_error(PIC_ERR_ARG, "bad argument");
+    h->acc_every = every;
+    h->acc_tol = tol;
+    return PIC_OK;
+}
+
+int pic_get_accuracy_report(pic_handle *h, double *out, int32_t cap_rows, int32_t *rows) {
+    if (!h || !rows || cap_rows < 0 || (cap_rows > 0 && !out))
+        return set_global_error(PIC_ERR_ARG, "bad argument");
+    const int32_t n = (int32_t)(h->acc_rows.size() / 6);
+    *rows = n;
+    const int32_t m = std::min(n, cap_rows);
+    for (int64_t k = 0; k < (int64_t)m * 6; k++) out[k] = h->acc_rows[k];
     return PIC_OK;
 }
 

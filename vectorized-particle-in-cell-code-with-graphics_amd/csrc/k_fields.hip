@@ -1273,4 +1273,98 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     return used;
 }
 
+// ------------------------------------------- FP64 reference solve (report)
+// The accuracy report of the split-precision solver: the same folded
+// right-hand side solved by the FP64 multigrid alone (the full multigrid of
+// the fixed mode, then FP64 V(1,1) cycles, mglin.cpp:176-196) until
+// ||f - Au||_2 <= tol ||f||_2 on the interior.
+
+// per-block sums of the FP64 residual r = f - A u and of f, squared
+__global__ void __launch_bounds__(TOL_THREADS)
+    k_resid_sums(const double *__restrict__ u, const double *__restrict__ f, int n, double h2i,
+                 double *__restrict__ partials) {
+    double rr = 0.0, ff = 0.0;
+    const long tot = (long)n * n;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < tot;
+         k += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(k / n), j = (int)(k % n);
+        if (i > 0 && j > 0 && i < n - 1 && j < n - 1) {
+            const double r = -h2i * (u[k + n] + u[k - n] + u[k + 1] + u[k - 1] - 4.0 * u[k]) + f[k];
+            rr += r * r;
+            ff += f[k] * f[k];
+        }
+    }
+    rr = block_sum_d(rr);
+    ff = block_sum_d(ff);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = rr;
+        partials[2 * blockIdx.x + 1] = ff;
+    }
+}
+
+// per-block sums of (a - b)^2 and b^2 over n entries, for three pairs
+__global__ void __launch_bounds__(TOL_THREADS)
+    k_diff_sums3(const double *__restrict__ a0, const double *__restrict__ b0,
+                 const double *__restrict__ a1, const double *__restrict__ b1,
+                 const double *__restrict__ a2, const double *__restrict__ b2, long n,
+                 double *__restrict__ partials) {
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long)gridDim.x * blockDim.x) {
+        const double d0 = a0[k] - b0[k], d1 = a1[k] - b1[k], d2 = a2[k] - b2[k];
+        s[0] += d0 * d0;
+        s[1] += b0[k] * b0[k];
+        s[2] += d1 * d1;
+        s[3] += b1[k] * b1[k];
+        s[4] += d2 * d2;
+        s[5] += b2[k] * b2[k];
+    }
+    for (int m = 0; m < 6; m++) {
+        const double t = block_sum_d(s[m]);
+        if (threadIdx.x == 0) partials[6 * blockIdx.x + m] = t;
+    }
+}
+
+// sums of the blocks' partials in block order (deterministic)
+static int host_block_sums(hipStream_t st, const double *dev, double *host, int nb, int w,
+                           double *out) {
+    if (hipMemcpyAsync(host, dev, sizeof(double) * nb * w, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return -1;
+    for (int m = 0; m < w; m++) out[m] = 0.0;
+    for (int b = 0; b < nb; b++)
+        for (int m = 0; m < w; m++) out[m] += host[w * b + m];
+    return 0;
+}
+
+int mg_fp64_reference(hipStream_t st, MgLevels &R, double length, double tol, int max_cycles,
+                      double *partials, double *host, int nb, double *relres) {
+    const int lf = R.ng - 1, n = lvl_size(lf);
+    const double h = length / (n - 1), h2i = 1.0 / (h * h);
+    double *u = R.d.u[lf];
+    const double *f = R.d.rhsF[lf];
+    mg_fixed_solve(st, R, 2, length);
+    int c = 0;
+    for (;;) {
+        hipLaunchKernelGGL(k_resid_sums, dim3(nb), dim3(TOL_THREADS), 0, st, (const double *)u, f,
+                           n, h2i, partials);
+        double sums[2];
+        if (host_block_sums(st, partials, host, nb, 2, sums) != 0) return -1;
+        *relres = sums[1] > 0.0 ? sqrt(sums[0] / sums[1]) : sqrt(sums[0]);
+        if (*relres <= tol || c >= max_cycles) break;
+        vcycle_launch<double>(st, R.d, R.lc, lf, u, f, length, nullptr);
+        c++;
+    }
+    return c;
+}
+
+int field_diff_sums(hipStream_t st, const double *a0, const double *b0, const double *a1,
+                    const double *b1, const double *a2, const double *b2, long n,
+                    double *partials, double *host, int nb, double *out6) {
+    hipLaunchKernelGGL(k_diff_sums3, dim3(nb), dim3(TOL_THREADS), 0, st, a0, b0, a1, b1, a2, b2, n,
+                       partials);
+    return host_block_sums(st, partials, host, nb, 6, out6);
+}
+
 }  // namespace pic

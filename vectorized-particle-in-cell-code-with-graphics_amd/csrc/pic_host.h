@@ -101,6 +101,16 @@ struct pic_handle {
     int64_t diag_graph_steps = 0;          // steps summed into the graph window
     std::string out_dir;                   // where pic_run writes its files ("" = none)
     double last_phiR = 0.0;
+    double last_bc[4] = {0, 0, 0, 0};  // phi0X, phi_right(t), phi0Y, phiLY of the last solve
+    // accuracy report: FP64 reference solve of the last right-hand side
+    pic::MgLevels ref;                 // FP64 levels only (allocated on first use)
+    double *ref_phi = nullptr, *ref_ex = nullptr, *ref_ey = nullptr;
+    float2 *ref_E32 = nullptr;
+    double *ref_partials = nullptr;    // device block sums
+    std::vector<double> ref_host;      // their host copy
+    int32_t acc_every = 0;             // report every acc_every steps (0: off)
+    double acc_tol = 1e-12;
+    std::vector<double> acc_rows;      // step, solver relres, err phi, Ex, Ey, ref relres
     int mg_cycles = 0;          // V-cycles (tolerance mode) of the last solve
     int64_t mg_cycles_total = 0;
     double mg_relres = 0.0;     // relative residual after the last solve

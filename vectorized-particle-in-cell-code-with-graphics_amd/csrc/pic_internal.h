@@ -302,6 +302,17 @@ int mg_coarse_level_for(long lds_bytes, int ng);
 int mg_dst_tables(MgLevels &M);
 void mg_set_coarse_attr(long lds_bytes);
 void mg_fixed_solve(hipStream_t st, MgLevels &M, int ncycle, double length);
+// FP64 reference solve of R.d.rhsF[top] into R.d.u[top]: full multigrid, then
+// FP64 V(1,1) cycles to ||f - Au|| <= tol ||f|| (at most max_cycles);
+// partials: device, >= 2 nb doubles; host: >= 6 nb doubles.  Returns the
+// V-cycles after the full multigrid (-1 on a stream error); synchronises.
+int mg_fp64_reference(hipStream_t st, MgLevels &R, double length, double tol, int max_cycles,
+                      double *partials, double *host, int nb, double *relres);
+// sums of (a_k - b_k)^2 and b_k^2 over n entries for three pairs (out6);
+// partials: device, >= 6 nb doubles; synchronises
+int field_diff_sums(hipStream_t st, const double *a0, const double *b0, const double *a1,
+                    const double *b1, const double *a2, const double *b2, long n,
+                    double *partials, double *host, int nb, double *out6);
 // residual-controlled solve of A u = M.d.rhsF[ng-1] into M.d.u[ng-1] (warm
 // started from its current content); returns V-cycles used, -1 on HIP error
 // after_batch(u0, u1) is queued after each batch of cycles, before the host

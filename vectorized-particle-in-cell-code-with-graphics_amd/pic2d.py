@@ -247,6 +247,9 @@ def load_library(path=LIB_PATH):
         "pic_batch_advance": [P(vp), i32, i32],
         "pic_serve": [P(ServeRequest), i32, i64, i32, P(ServeResult), P(ServeStats)],
         "pic_cross_section": [i32, i32, i32, dbl, P(dbl)],
+        "pic_field_error": [vp, dbl, vp],
+        "pic_set_accuracy_report": [vp, i32, dbl],
+        "pic_get_accuracy_report": [vp, vp, i32, P(i32)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -557,6 +560,23 @@ class Pic:
         a = np.zeros(8, dtype=np.int64)
         self._c(self.lib.pic_get_collision_counts(self.h, s, _ptr(a)))
         return a
+
+    def field_error(self, tol=1e-12):
+        """relative L2 errors of phi, Ex, Ey against an FP64 multigrid solve of
+        the last right-hand side converged to tol; returns a dict"""
+        a = np.zeros(5)
+        self._c(self.lib.pic_field_error(self.h, float(tol), _ptr(a)))
+        return dict(phi=a[0], ex=a[1], ey=a[2], ref_relres=a[3], ref_cycles=int(a[4]))
+
+    def set_accuracy_report(self, every, tol=1e-12):
+        self._c(self.lib.pic_set_accuracy_report(self.h, int(every), float(tol)))
+
+    def accuracy_report(self, cap=1 << 14):
+        """rows (step, solver relres, err phi, err Ex, err Ey, reference relres)"""
+        out = np.zeros((cap, 6))
+        n = C.c_int32()
+        self._c(self.lib.pic_get_accuracy_report(self.h, _ptr(out), cap, C.byref(n)))
+        return out[: min(n.value, cap)]
 
     def overflow(self):
         v = C.c_int64()
