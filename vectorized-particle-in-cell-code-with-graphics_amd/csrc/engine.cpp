@@ -725,6 +725,9 @@ void absorb_species(pic_handle *h, int si, bool plop) {
                   make_spawn(h, plop, si));
     s.cur = 1 - c;
     clear_holes(h, si);  // the compaction dropped them
+    // the compaction keeps 256-particle runs in order but not the runs: the
+    // next pass sorts, so the in-place passes after it see compact blocks
+    s.since_sort = 1 << 20;
 }
 
 // phi from the solve with the boundary values; with_e: E (and its fp32 copy
@@ -805,7 +808,11 @@ int alloc_ref(pic_handle *h) {
     if (h->ref_phi) return PIC_OK;
     MgLevels &R = h->ref;
     R.ng = h->M.ng;
-    R.lc = h->M.lc;
+    // the refinement V-cycles of the reference start above the level solved
+    // in one workgroup (a grid that fits it whole, 65^2 with 160 KB of LDS,
+    // would have no coarse-grid arrays for the V-cycle to restrict into);
+    // a lower coarsest level fits the LDS as well
+    R.lc = std::min(h->M.lc, R.ng - 2);
     for (int l = R.lc; l < R.ng; l++) {
         size_t s = (size_t)((1 << (l + 1)) + 1) * ((1 << (l + 1)) + 1);
         CHECK(dalloc(h, &R.d.rhsF[l], s));

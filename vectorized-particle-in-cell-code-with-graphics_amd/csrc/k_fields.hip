@@ -1341,6 +1341,8 @@ static int host_block_sums(hipStream_t st, const double *dev, double *host, int 
 int mg_fp64_reference(hipStream_t st, MgLevels &R, double length, double tol, int max_cycles,
                       double *partials, double *host, int nb, double *relres) {
     const int lf = R.ng - 1, n = lvl_size(lf);
+    // the V-cycle at lf restricts into cres[lf-1]: levels lc..lf-1 must exist
+    if (R.lc >= lf || !R.d.cres[lf - 1] || !R.d.cv[lf - 1]) return -1;
     const double h = length / (n - 1), h2i = 1.0 / (h * h);
     double *u = R.d.u[lf];
     const double *f = R.d.rhsF[lf];
