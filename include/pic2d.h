@@ -258,6 +258,18 @@ int pic_set_step(pic_handle *h, int64_t step);
  * holds the rhs on entry and the solution on exit. */
 int pic_mglin(double *u, int32_t n, int32_t ncycle, double length);
 
+/* pic_mglin without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  u is a
+ * DEVICE pointer to n*n doubles (the rhs on entry, the solution on exit, as
+ * pic_mglin); the work is queued on `stream` (a hipStream_t of the current
+ * device; NULL = its default stream) and not waited for.  The multigrid
+ * levels live in a library-owned static device array of 19.7 MB per GPU
+ * (grids up to n = 1025; larger n returns PIC_ERR_CAPACITY), so no call
+ * allocates or frees.  Calls that overlap on one GPU share that array and
+ * race: order them on one stream.  Results are pic_mglin's, bit for bit. */
+int pic_mglin_noalloc(double *u, int32_t n, int32_t ncycle, double length, void *stream);
+
 /* Residual-controlled Poisson solve (no reference counterpart; the
  * PIC_MG_TOL solver of the timestep): A u = f with the 5-point Laplacian of
  * mglin.cpp:133-144, zero walls; u (n*n) is the initial guess on entry and

@@ -1,6 +1,7 @@
 """Host-only checks of the C ABI (no GPU needed): the library loads, exports
 every function include/pic2d.h declares, and the parameters.h loader evaluates
 the reference's macros as the reference's compiler would."""
+import ctypes
 import math
 import os
 
@@ -185,3 +186,16 @@ def test_estimate_and_serve_arguments():
     with pytest.raises(pic2d.PicError) as e:       # can never be admitted: refused up front
         pic2d.serve([(big, 1, 0.0)], eb // 2)
     assert e.value.code == -6
+
+
+def test_mglin_noalloc_rejects_bad_sizes_before_any_device_call():
+    """the no-allocation FP64 multigrid checks its arguments on the host: a
+    grid beyond its static scratch array (n > 1025) is PIC_ERR_CAPACITY, a
+    grid that is not 2^k + 1 PIC_ERR_GRID (no GPU needed for either)"""
+    lib = pic2d.load_library()
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: the checks come first
+    assert lib.pic_mglin_noalloc(fake, 2049, 2, 1.0, None) == -6
+    assert b"1025" in lib.pic_last_error(None)
+    assert lib.pic_mglin_noalloc(fake, 1000, 2, 1.0, None) == -2
+    assert lib.pic_mglin_noalloc(None, 65, 2, 1.0, None) == -1
+    assert lib.pic_mglin_noalloc(fake, 65, 0, 1.0, None) == -1

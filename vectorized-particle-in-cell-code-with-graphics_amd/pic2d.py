@@ -248,6 +248,7 @@ def load_library(path=LIB_PATH):
         "pic_serve": [P(ServeRequest), i32, i64, i32, P(ServeResult), P(ServeStats)],
         "pic_cross_section": [i32, i32, i32, dbl, P(dbl)],
         "pic_field_error": [vp, dbl, vp],
+        "pic_mglin_noalloc": [vp, i32, i32, dbl, vp],
         "pic_set_accuracy_report": [vp, i32, dbl],
         "pic_get_accuracy_report": [vp, vp, i32, P(i32)],
     }
