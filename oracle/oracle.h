@@ -265,6 +265,7 @@ double orc_sigma(int kind, double E, double thr);
 /* streams.  Pinned by tests/test_oracle_collisions.py (rates against         */
 /* sigma*g*n, energy and momentum bookkeeping of every channel).             */
 /* ------------------------------------------------------------------------ */
+#define ORC_MCC_BINS 64
 typedef struct {
     int gas;          /* 0 argon, 1 hydrogen                                  */
     int electron;     /* 1 electrons, 0 ions                                  */
@@ -276,11 +277,22 @@ typedef struct {
     double e_exc, e_iz, opal_B, kev_amu_div;
     double nu_max, p_null; /* set by orc_mcc_setup                            */
     uint32_t key0, key1;
+    /* electrons (set by orc_mcc_setup): null-collision bound per energy bin
+     * of the particle, two bins per octave of E = m v^2 / 2q from 2^-12 eV
+     * (bin 0 takes everything below, bin 63 everything above); bin_nu[b] =
+     * 1.02 max of the total frequency over the relative speeds the bin's
+     * particles reach against neutrals up to 10 vtn fast (257 speeds),
+     * bin_p[b] = 1 - exp(-bin_nu[b] dt).  binned = 0: the global bound. */
+    int binned, pad_;
+    double bin_p[ORC_MCC_BINS], bin_nu[ORC_MCC_BINS];
 } orc_mcc_params;
 
 /* nu' = 1.02 * max of the total collision frequency over 1e-6 .. 3e4 eV
- * (electrons) or 1e-6 .. 1e3 eV (ions); p_null = 1 - exp(-nu' dt) */
+ * (electrons) or 1e-6 .. 1e3 eV (ions); p_null = 1 - exp(-nu' dt); the
+ * electrons' per-bin bounds */
 void orc_mcc_setup(orc_mcc_params *c);
+/* the bin of a particle of speed^2 v2 (electrons; see orc_mcc_params) */
+int orc_energy_bin(const orc_mcc_params *c, double v2);
 /* returns 0 none, 1 elastic, 2 excitation, 3 ionisation (child = ejected
  * electron v[0..2], new ion v[3..5]) */
 int orc_collide_electron(const orc_mcc_params *c, double *vx, double *vy, double *vz, uint32_t id,

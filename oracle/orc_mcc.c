@@ -34,10 +34,30 @@ static void draw(const orc_mcc_params *c, uint32_t id, uint32_t step, uint32_t s
  * key = key0 xor a per-species constant; only candidates draw stream RS_MAIN */
 #define KEY_CAND_E 0x43414E45u
 #define KEY_CAND_I 0x43414E49u
-static int candidate(const orc_mcc_params *c, uint32_t id, uint32_t step, int electron) {
+int orc_energy_bin(const orc_mcc_params *c, double v2) {
+    const double E = 0.5 * c->mass * v2 / c->q;
+    double t = (log2(E) + 12.0) * 2.0;
+    if (!(t > 0.0)) t = 0.0;
+    if (t > ORC_MCC_BINS - 1) t = ORC_MCC_BINS - 1;
+    return (int)t;
+}
+
+/* the bound of a particle of speed^2 v2: the bin's (electrons) or global */
+static void bound(const orc_mcc_params *c, double v2, double *p, double *nu) {
+    if (c->binned) {
+        const int b = orc_energy_bin(c, v2);
+        *p = c->bin_p[b];
+        *nu = c->bin_nu[b];
+    } else {
+        *p = c->p_null;
+        *nu = c->nu_max;
+    }
+}
+
+static int candidate(const orc_mcc_params *c, uint32_t id, uint32_t step, int electron, double p) {
     uint32_t ctr[2] = {id, step}, r[2];
     orc_philox2x32_10(ctr, c->key0 ^ (electron ? KEY_CAND_E : KEY_CAND_I), r);
-    return (double)orc_u01(r[0]) < c->p_null;
+    return (double)orc_u01(r[0]) < p;
 }
 
 static void neutral(const orc_mcc_params *c, uint32_t id, uint32_t step, uint32_t s0, double *nx,
@@ -90,6 +110,24 @@ void orc_mcc_setup(orc_mcc_params *c) {
     }
     c->nu_max = best * 1.02;
     c->p_null = 1.0 - exp(-c->nu_max * c->dt);
+    c->binned = c->electron;
+    if (!c->binned) return;
+    for (int b = 0; b < ORC_MCC_BINS; b++) {
+        const double elo = b == 0 ? 0.0 : exp2(b / 2.0 - 12.0);
+        const double ehi = b == ORC_MCC_BINS - 1 ? 1e7 : exp2((b + 1) / 2.0 - 12.0);
+        const double vlo = sqrt(2.0 * c->q * elo / c->mass), vhi = sqrt(2.0 * c->q * ehi / c->mass);
+        const double glo = fmax(0.0, vlo - 10.0 * c->vtn), ghi = vhi + 10.0 * c->vtn;
+        double m = 0.0;
+        for (int k = 0; k <= 256; k++) {
+            const double g = glo + (ghi - glo) * k / 256.0;
+            const double E = 0.5 * c->mass * g * g / c->q;
+            double a, e, d;
+            e_nu(c, E, g, &a, &e, &d);
+            if (a + e + d > m) m = a + e + d;
+        }
+        c->bin_nu[b] = m * 1.02;
+        c->bin_p[b] = 1.0 - exp(-c->bin_nu[b] * c->dt);
+    }
 }
 
 static double cos_chi(double E, double R) {
@@ -117,7 +155,9 @@ int orc_collide_electron(const orc_mcc_params *c, double *vx, double *vy, double
                          uint32_t step, double child[6]) {
     double u[4];
     uint32_t raw[4];
-    if (!candidate(c, id, step, 1)) return 0;
+    double pb, nub;
+    bound(c, *vx * *vx + *vy * *vy + *vz * *vz, &pb, &nub);
+    if (!candidate(c, id, step, 1, pb)) return 0;
     draw(c, id, step, RS_MAIN, u, raw);
     double nx, ny, nz;
     neutral(c, id, step, 0u, &nx, &ny, &nz);
@@ -126,7 +166,7 @@ int orc_collide_electron(const orc_mcc_params *c, double *vx, double *vy, double
     double E = (0.5 / c->q) * c->mass * g2;
     double nel, nex, niz;
     e_nu(c, E, g, &nel, &nex, &niz);
-    double x = u[1] * c->nu_max;
+    double x = u[1] * nub;
     double cp = cos(6.2831853 * u[3]), sp = sin(6.2831853 * u[3]);
     double hx, hy, hz;
     if (x < nel) {
@@ -182,7 +222,7 @@ int orc_collide_ion(const orc_mcc_params *c, double *vx, double *vy, double *vz,
                     uint32_t step) {
     double u[4];
     uint32_t raw[4];
-    if (!candidate(c, id, step, 0)) return 0;
+    if (!candidate(c, id, step, 0, c->p_null)) return 0;
     draw(c, id, step, 16u + RS_MAIN, u, raw);
     double nx, ny, nz;
     neutral(c, id, step, 16u, &nx, &ny, &nz);

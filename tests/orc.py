@@ -54,7 +54,8 @@ class MccParams(C.Structure):
     _fields_ = [("gas", C.c_int), ("electron", C.c_int)] + [
         (k, C.c_double) for k in ("ng", "dt", "mass", "M", "q", "vtn", "e_exc", "e_iz", "opal_B",
                                   "kev_amu_div", "nu_max", "p_null")
-    ] + [("key0", C.c_uint32), ("key1", C.c_uint32)]
+    ] + [("key0", C.c_uint32), ("key1", C.c_uint32), ("binned", C.c_int), ("pad_", C.c_int),
+         ("bin_p", C.c_double * 64), ("bin_nu", C.c_double * 64)]
 
 
 class RefCxParams(C.Structure):
@@ -120,6 +121,8 @@ def lib():
         _lib.orc_sigma.argtypes = [C.c_int, C.c_double, C.c_double]
         _lib.orc_sigma.restype = C.c_double
         _lib.orc_mcc_setup.argtypes = [C.POINTER(MccParams)]
+        _lib.orc_energy_bin.argtypes = [C.POINTER(MccParams), C.c_double]
+        _lib.orc_energy_bin.restype = C.c_int
         _lib.orc_mcc_apply.argtypes = [C.POINTER(MccParams), C.c_long, vp, vp, vp, vp, C.c_uint32,
                                        vp, vp]
         _lib.orc_ref_run_mcc.argtypes = [C.POINTER(CfgRef), C.POINTER(MccParams),
@@ -418,6 +421,15 @@ def mcc_params(p, electron):
                   key0=p.seed & 0xFFFFFFFF, key1=(p.seed >> 32) & 0xFFFFFFFF)
     lib().orc_mcc_setup(C.byref(c))
     return c
+
+
+def e_bound(c, E):
+    """(p, nu) of the null-collision bound an electron of energy E [eV] gets:
+    its energy bin's (binned parameters) or the global one"""
+    if not c.binned:
+        return c.p_null, c.nu_max
+    b = lib().orc_energy_bin(C.byref(c), 2.0 * Q_E * E / c.mass)
+    return c.bin_p[b], c.bin_nu[b]
 
 
 def mcc_apply(c, vx, vy, vz, ids, step):

@@ -5,7 +5,8 @@ kinematics exact: excitation leaves exactly E - 11.55 eV, ionisation splits
 exactly E - 15.76 eV between the two electrons, charge exchange leaves the ion
 at rest, isotropic ion scattering keeps |v' - v/2| = |v|/2; event counts are
 binomial with probabilities from the oracle's cross sections and null-collision
-bound (checked at 4 sigma).  The reference rule set: elastic with probability
+bound -- the electron's energy bin's, the ions' global one (checked at 4
+sigma).  The reference rule set: elastic with probability
 P_el/2, every electron above 15.755 eV removed.  Collisional runs are
 deterministic and the per-phase entry points reproduce the fused step."""
 import math
@@ -71,8 +72,8 @@ def test_electron_beam_rates_and_exact_kinematics():
         ev = sim.collision_counts(0)
         st = sim.get_state(0)
         ions = sim.get_state(1)
-    nu = orc.nu_max(ME, p.neutral_density, True)
-    pn = 1 - math.exp(-nu * p.dt)
+    # the bound of the beam's energy bin (electrons: one bound per energy bin)
+    pn, nu = orc.e_bound(orc.mcc_params(p, True), E0)
     nel, nex, niz = (x[0] for x in orc.e_nu_parts(np.array([E0]), np.array([v0]),
                                                     p.neutral_density))
     assert binom_ok(ev[1], n, pn * nel / nu), (ev[1], n * pn * nel / nu)

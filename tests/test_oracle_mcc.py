@@ -7,9 +7,13 @@ the centre of mass); excitation removes exactly 11.55 eV; ionisation shares
 exactly E - 15.76 eV between the two electrons and leaves the ion at rest;
 charge exchange leaves the ion at rest; isotropic ion scattering keeps
 |v' - v/2| = |v|/2 with <cos> of the scattering angle 0.  Event counts are
-binomial with p = (1 - exp(-nu' dt)) nu_k / nu' (null-collision method), and
-the mean scattering cosine matches the closed form of the Vahedi-Surendra
-angular law, <cos chi> = (2+E)/E - 2/ln(1+E).
+binomial with p = (1 - exp(-nu' dt)) nu_k / nu' (null-collision method, nu'
+the bound of the particle's energy bin for electrons, the global bound for
+ions), and the mean scattering cosine matches the closed form of the
+Vahedi-Surendra angular law, <cos chi> = (2+E)/E - 2/ln(1+E).  Every
+electron energy bin's bound covers the true total frequency of its
+particles, and the bins cut the candidates of a 2 eV Maxwellian by a large
+factor against the global bound.
 """
 import math
 
@@ -61,8 +65,9 @@ def test_electron_channels_rates_and_exact_kinematics():
         ox, oy, oz, ev, ch = orc.mcc_apply(c, vx, vy, vz, ids, 7)
         g = math.sqrt(2 * Q * E / ME)
         nel, nex, niz = (x[0] for x in orc.e_nu_parts(np.array([E]), g, NG))
+        pb, nub = orc.e_bound(c, E)
         for kind, nu in ((1, nel), (2, nex), (3, niz)):
-            p = c.p_null * nu / c.nu_max
+            p = pb * nu / nub
             assert binomial_ok(int((ev == kind).sum()), n, p), (E, kind)
         assert (ev[(ev != 0)] <= 3).all()
         cf = MI / (ME + MI)
@@ -178,3 +183,54 @@ def test_collisional_step_loop_conserves_pairs_with_reinjection():
     assert r["iid"].max() < r["next_id"][1] and r["eid"].max() < r["next_id"][0]
     for a in (r["e"], r["i"]):
         assert np.all((a["x"] >= 0) & (a["x"] < L) & (a["y"] >= 0) & (a["y"] < L))
+
+
+def test_electron_energy_bins_bound_the_true_frequency():
+    """every bin's bound is at least the total collision frequency of any
+    relative speed its particles can have (cold and 300 K neutrals), and it
+    is not loose by orders of magnitude where electrons live"""
+    for vtn in (0.0, 250.0):
+        c = params(True, 5e-11, vtn=vtn)
+        assert c.binned
+        E = np.geomspace(1e-5, 2e4, 20000)
+        for e in E:
+            b = orc.lib().orc_energy_bin(orc.C.byref(c), 2 * Q * e / ME)
+            lo = 0.0 if b == 0 else 2.0 ** (b / 2 - 12)
+            assert b == 63 or lo <= e * (1 + 1e-12)
+            v = math.sqrt(2 * Q * e / ME)
+            for g in (max(v - 10 * vtn, 0.0), v, v + 10 * vtn):
+                Eg = 0.5 * ME * g * g / Q
+                nu = sum(x[0] for x in orc.e_nu_parts(np.array([Eg]), g, NG))
+                assert nu <= c.bin_nu[b] * (1 + 1e-12), (vtn, e, g, nu, c.bin_nu[b])
+        # where the 2 eV electrons are, the bound is well below the global one
+        assert c.bin_nu[orc.lib().orc_energy_bin(orc.C.byref(c), 2 * Q * 2.0 / ME)] < 0.2 * c.nu_max
+
+
+def test_binned_rates_follow_the_bins_bound_at_large_dt():
+    """at nu' dt ~ 1 the collision probability p_b nu / nu_b depends on the
+    bound used: the events follow the particle's bin, not the global bound"""
+    rng = np.random.default_rng(4)
+    c = params(True, 1e-9)
+    n = 200000
+    for E in (1.0, 5.0):
+        vx, vy, vz = beam(E, ME, n, rng)
+        _, _, _, ev, _ = orc.mcc_apply(c, vx, vy, vz, np.arange(n, dtype=np.uint32), 9)
+        g = math.sqrt(2 * Q * E / ME)
+        nel = orc.e_nu_parts(np.array([E]), g, NG)[0][0]
+        pb, nub = orc.e_bound(c, E)
+        binned, glob = pb * nel / nub, c.p_null * nel / c.nu_max
+        assert abs(binned - glob) > 0.2 * binned  # the two laws differ here
+        assert binomial_ok(int((ev == 1).sum()), n, binned), (E, binned, glob)
+
+
+def test_maxwellian_candidates_drop_with_the_bins():
+    """the fraction of 2 eV-Maxwellian electrons that are candidates, from the
+    bins, against the global bound's p_null"""
+    rng = np.random.default_rng(5)
+    c = params(True, 5.6e-11)
+    sig = math.sqrt(2.0 * Q / ME)
+    v = rng.standard_normal((100000, 3)) * sig
+    E = 0.5 * ME * (v ** 2).sum(1) / Q
+    frac = np.mean([orc.e_bound(c, e)[0] for e in E[:20000]])
+    print(f"candidate fraction: binned {frac:.4f}, global {c.p_null:.4f}")
+    assert frac < 0.3 * c.p_null

@@ -226,6 +226,7 @@ void species_free(Species &s) {
     if (s.rstate) hipFree(s.rstate);
     if (s.mq) hipFree(s.mq);
     if (s.mq_cnt) hipFree(s.mq_cnt);
+    if (s.mc_bins) hipFree(s.mc_bins);
     if (s.q) hipFree(s.q);
     if (s.exp_) hipFree(s.exp_);
     if (s.eyp) hipFree(s.eyp);
@@ -297,7 +298,30 @@ double nu_max_scan(const mcc::MccConst &c) {
     return best * 1.02;
 }
 
-void setup_mcc(pic_handle *h, Species &s, bool electron) {
+// per-bin null-collision bounds of the physical-mode electrons (the bin
+// rule is energy_bin's, mcc.h): the largest total collision frequency over
+// the relative speeds a bin's particles reach against neutrals up to 10
+// sigma fast, scanned at 257 speeds, times 1.02; nu[b] and p[b] = 1 -
+// exp(-nu[b] dt)
+void electron_bins(const mcc::MccConst &c, double *nu, double *p) {
+    for (int b = 0; b < MCC_BINS; b++) {
+        const double vlo = sqrt(2.0 * c.q * mcc::mcc_bin_lo(b) / c.mass);
+        const double vhi = sqrt(2.0 * c.q * mcc::mcc_bin_hi(b) / c.mass);
+        const double glo = std::max(0.0, vlo - 10.0 * c.vtn), ghi = vhi + 10.0 * c.vtn;
+        double best = 0.0;
+        for (int k = 0; k <= 256; k++) {
+            const double g = glo + (ghi - glo) * k / 256.0;
+            const double E = 0.5 * c.mass * g * g / c.q;
+            double a, e, d;
+            mcc::e_nu_parts<double>(c, E, g, a, e, d);
+            best = std::max(best, a + e + d);
+        }
+        nu[b] = best * 1.02;
+        p[b] = 1.0 - exp(-nu[b] * c.dt);
+    }
+}
+
+int setup_mcc(pic_handle *h, Species &s, bool electron) {
     const pic_params &P = h->P;
     mcc::MccConst &c = s.mc;
     c = mcc::MccConst();
@@ -328,11 +352,22 @@ void setup_mcc(pic_handle *h, Species &s, bool electron) {
     c.on = P.have_collisions != 0;
     // reference-mode ions collide through the per-column accumulator (k_refcx.hip)
     if (!electron && c.mode == PIC_DEV_MODE_REF) c.on = 0;
+    c.bins = nullptr;
     if (c.on && c.mode == PIC_DEV_MODE_PHYS) {
         double nu = nu_max_scan(c);
         c.nu_max = (float)nu;
         c.p_null = (float)(1.0 - exp(-nu * s.dt));
+        if (electron) {
+            double nub[MCC_BINS], pb[MCC_BINS];
+            electron_bins(c, nub, pb);
+            float2 t[MCC_BINS];
+            for (int b = 0; b < MCC_BINS; b++) t[b] = make_float2((float)pb[b], (float)nub[b]);
+            CHECK(dalloc(h, &s.mc_bins, MCC_BINS));
+            HIPCHK(h, hipMemcpy(s.mc_bins, t, sizeof(t), hipMemcpyHostToDevice));
+            c.bins = s.mc_bins;
+        }
     }
+    return PIC_OK;
 }
 
 // collisions.cpp:66-89: the reference's neutral velocity table, drawn from
@@ -632,7 +667,9 @@ int check_overflow(pic_handle *h, bool wait) {
 // candidates' velocities would cost a read and write of nearly every line,
 // and the particle pass collides them in its block queue instead.
 bool collide_ahead(const Species &s) {
-    return s.mc.on && s.mc.mode == PIC_DEV_MODE_PHYS && s.mc.p_null < 0.02f;
+    // (a species with energy bins collides in the pass: the deferred test
+    // does not read vz)
+    return s.mc.on && s.mc.mode == PIC_DEV_MODE_PHYS && s.mc.p_null < 0.02f && !s.mc.bins;
 }
 
 // counting sort + (optional) collisions and push + deposit of one species.
@@ -1073,8 +1110,8 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         if ((rc = dalloc(h, &h->ntab, tab.size())) != PIC_OK) return fail(rc);
         hipMemcpy(h->ntab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice);
     }
-    setup_mcc(h, h->sp[0], true);
-    setup_mcc(h, h->sp[1], false);
+    if ((rc = setup_mcc(h, h->sp[0], true)) != PIC_OK || (rc = setup_mcc(h, h->sp[1], false)) != PIC_OK)
+        return fail(rc);
     for (int s = 0; s < 2; s++) {
         Species &sp = h->sp[s];
         if (!collide_ahead(sp)) continue;

@@ -1046,10 +1046,10 @@ __global__ void __launch_bounds__(TOL_THREADS)
                  int *__restrict__ host_flags) {
     SKIP_IF_DONE(flags);
     double rr = 0.0, ff = 0.0;
-    const long tot = (long)n * n;
-    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < tot;
-         k += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(k / n), j = (int)(k % n);
+    // whole rows per block (no index division); threads stride along the row
+    for (int i = (int)blockIdx.x; i < n; i += (int)gridDim.x)
+    for (int j = (int)threadIdx.x; j < n; j += (int)blockDim.x) {
+        const long k = (long)i * n + j;
         float out = 0.0f;
         double v = u_in[k];
         if (i > 0 && j > 0 && i < n - 1 && j < n - 1) {

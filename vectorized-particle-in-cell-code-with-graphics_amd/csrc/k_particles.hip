@@ -112,7 +112,9 @@ __device__ __forceinline__ void block_collide_n(const bool (&active)[P], const P
     uint32_t wtot = 0;
 #pragma unroll
     for (int r = 0; r < P; r++) {
-        const bool cand = active[r] && mcc::candidate(mc, id[r], step);
+        const bool cand =
+            active[r] && mcc::candidate(mc, id[r], step,
+                                        s[r].vx * s[r].vx + s[r].vy * s[r].vy + s[r].vz * s[r].vz);
         const uint64_t bal = __ballot(cand);
         pos[r] = cand ? wtot + (uint32_t)__popcll(bal & below) : 0xFFFFFFFFu;
         wtot += (uint32_t)__popcll(bal);
@@ -1016,7 +1018,8 @@ __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint
     // deferred: a candidate stays as it is for k_collide_q
     bool cand = false;
     if (defer) {
-        cand = live && mcc::candidate(mc, pid, step);
+        // (a deferring species has no energy bins: its bound is global)
+        cand = live && mcc::candidate(mc, pid, step, 0.0f);
         defer_queue(spn, cand, i);
     }
     const bool done = defer || collide_take(blk, qs, s, ev, ch, lds);
@@ -1192,7 +1195,7 @@ __global__ void __launch_bounds__(256)
     uint32_t wtot = 0;
 #pragma unroll
     for (int r = 0; r < CIPP; r++) {
-        const bool cand = cell_jx(cl[r]) != CELL_HOLE && mcc::candidate(mc, idv[r], step);
+        const bool cand = cell_jx(cl[r]) != CELL_HOLE && mcc::candidate(mc, idv[r], step, 0.0f);
         const uint64_t bal = __ballot(cand);
         pos[r] = cand ? wtot + (uint32_t)__popcll(bal & below) : 0xFFFFFFFFu;
         wtot += (uint32_t)__popcll(bal);

@@ -230,7 +230,42 @@ struct MccConst {
     const float *ntab;  // REF neutral velocity table (3 x 10000, SoA)
     int ntab_n;
     uint32_t key0, key1;
+    // PHYS electrons: null-collision bound per energy bin of the particle,
+    // {1 - exp(-nu_b dt), nu_b} (MCC_BINS entries, device), or null: the
+    // global bound (nu_max, p_null) for every particle
+    const float2 *bins;
 };
+
+// Energy bins of the null-collision bound: two per octave of the particle's
+// kinetic energy E = m v^2 / 2q, bin b covering [2^(b/2 - 12), 2^((b+1)/2 - 12))
+// eV; bin 0 takes everything below, the last everything above.  The bound of
+// a bin is the largest n_g sigma_tot(E_rel) g over every relative speed g its
+// particles can have against the neutral gas (the particle's speed range
+// widened by 10 neutral sigmas), times 1.02: the method stays exact (a
+// particle's collision probability is p_b nu(E_rel) / nu_b = the exact one
+// to O(nu dt)^2, as with one global bound) while a slow electron, whose true
+// rate is tens of times below the global bound set near 15 eV, is rarely a
+// candidate.  Particles over their bin's bound are counted (EV_OVER_BOUND).
+#define MCC_BINS 64
+#define MCC_BIN_PER_OCT 2
+#define MCC_BIN_E0_LOG2 (-12)
+// lower and upper energy [eV] of bin b (host; the first bin starts at 0, the
+// last ends at 1e7 eV)
+inline double mcc_bin_lo(int b) { return b == 0 ? 0.0 : exp2((double)b / MCC_BIN_PER_OCT + MCC_BIN_E0_LOG2); }
+inline double mcc_bin_hi(int b) {
+    return b == MCC_BINS - 1 ? 1e7 : exp2((double)(b + 1) / MCC_BIN_PER_OCT + MCC_BIN_E0_LOG2);
+}
+// bin of a particle of speed^2 v2 (device: hardware log2, NaN and 0 -> bin 0)
+__device__ __forceinline__ int energy_bin(const MccConst &c, float v2) {
+    const float E = (0.5f / c.q) * c.mass * v2;
+    float t = (__builtin_amdgcn_logf(E) - (float)MCC_BIN_E0_LOG2) * (float)MCC_BIN_PER_OCT;
+    t = fminf(fmaxf(t, 0.0f), (float)(MCC_BINS - 1));
+    return (int)t;
+}
+// the bound that applies to a particle: {p, nu}
+__device__ __forceinline__ float2 null_bound(const MccConst &c, float v2) {
+    return c.bins ? c.bins[energy_bin(c, v2)] : make_float2(c.p_null, c.nu_max);
+}
 
 // event codes (counters)
 enum Event { EV_NONE = 0, EV_ELASTIC = 1, EV_EXCITE = 2, EV_IONIZE = 3, EV_CX = 4, EV_ISO = 5 };
@@ -331,10 +366,11 @@ struct Child {
 // candidate draws its collision from Philox4x32-10 stream RS_MAIN.
 #define RNG_KEY_CAND_E 0x43414E45u
 #define RNG_KEY_CAND_I 0x43414E49u
-HD bool candidate(const MccConst &c, uint32_t id, uint32_t step) {
+// v2: the particle's speed^2 (selects its energy bin when c.bins is set)
+__device__ __forceinline__ bool candidate(const MccConst &c, uint32_t id, uint32_t step, float v2) {
     uint32_t r[2];
     philox2x32_10(id, step, c.key0 ^ (c.electron ? RNG_KEY_CAND_E : RNG_KEY_CAND_I), r);
-    return u01(r[0]) < c.p_null;
+    return u01(r[0]) < (c.bins ? c.bins[energy_bin(c, v2)].x : c.p_null);
 }
 
 // Electron collision.  Returns the event; on EV_IONIZE in PHYS mode `ch`
@@ -386,7 +422,12 @@ __device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, fl
         return ev;
     }
     // PHYS null collision
-    if (!candidate(c, id, step)) return EV_NONE;
+    const float2 bound = null_bound(c, vx * vx + vy * vy + vz * vz);
+    {
+        uint32_t rc[2];
+        philox2x32_10(id, step, c.key0 ^ (c.electron ? RNG_KEY_CAND_E : RNG_KEY_CAND_I), rc);
+        if (!(u01(rc[0]) < bound.x)) return EV_NONE;
+    }
     philox4x32_10(id, step, s0 + RS_MAIN, RNG_DOMAIN_MCC, c.key0, c.key1, r);
     float nx, ny, nz;
     normals3(id, step, s0 + RS_NEUTRAL, c, nx, ny, nz);
@@ -399,8 +440,8 @@ __device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, fl
     float E = (0.5f / c.q) * c.mass * g2;
     float nel, nex, niz;
     e_nu_parts<float, HwMath>(c, E, g, nel, nex, niz);
-    const int over = (nel + nex + niz > c.nu_max) ? EV_OVER_BOUND : 0;
-    float x = u01(r[1]) * c.nu_max;
+    const int over = (nel + nex + niz > bound.y) ? EV_OVER_BOUND : 0;
+    float x = u01(r[1]) * bound.y;
     float cp = cos_rev(u01(r[3])), sp = sin_rev(u01(r[3]));
     const float rg = __builtin_amdgcn_rcpf(g);
     if (x < nel) {
@@ -466,7 +507,7 @@ __device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, fl
 __device__ __forceinline__ int collide_ion(const MccConst &c, float &vx, float &vy, float &vz,
                                            uint32_t id, uint32_t step) {
     const uint32_t s0 = 16u;
-    if (!candidate(c, id, step)) return EV_NONE;
+    if (!candidate(c, id, step, vx * vx + vy * vy + vz * vz)) return EV_NONE;
     uint32_t r[4];
     philox4x32_10(id, step, s0 + RS_MAIN, RNG_DOMAIN_MCC, c.key0, c.key1, r);
     float nx, ny, nz;

@@ -40,6 +40,7 @@ struct Species {
     pic::StageView wq = {};          // particles that reached a wall (surface physics)
     // candidate queue and its segment counters (collisions run ahead)
     uint32_t *mq = nullptr, *mq_cnt = nullptr;
+    float2 *mc_bins = nullptr;  // energy-bin null-collision bounds (mc.bins)
 };
 
 struct ProfPair {
