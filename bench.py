@@ -265,6 +265,8 @@ def main():
     ap.add_argument("--mg-sweeps", type=int, default=-1,
                     help="tolerance mode: red-black sweeps per V-cycle leg, 1-3 (-1: the "
                          "library default)")
+    ap.add_argument("--null-bound", choices=["binned", "global"], default="binned",
+                    help="electron null-collision bound: per energy bin (default) or global")
     ap.add_argument("--runs", type=int, default=64, help="ensemble: number of runs")
     ap.add_argument("--requests", type=int, default=24, help="serving: requests per load")
     ap.add_argument("--step-scale", type=float, default=0.2,
@@ -297,6 +299,7 @@ def main():
         p.sort_key = args.sort_key
     if args.mg_sweeps > 0:
         p.mg_sweeps = args.mg_sweeps
+    p.null_bound = 1 if args.null_bound == "global" else 0
     log("workload", args.workload, "nodes", p.nodes, "particles/species", p.init_particle_no)
     sim = pic2d.Pic(p)
     e0 = i0 = None

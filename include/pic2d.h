@@ -136,7 +136,15 @@ typedef struct pic_params {
     int32_t mg_sweeps;           /* PIC_MG_TOL: red-black sweeps before and after
                                     the coarse correction on each level, 1..3
                                     (V(1,1)..V(3,3)); -1 = chosen by the library   */
+    int32_t null_bound;          /* physical-mode electron collisions: the null-
+                                    collision bound per energy bin of the electron
+                                    (PIC_NULL_BOUND_BINNED, the default) or one
+                                    bound for all (PIC_NULL_BOUND_GLOBAL, Vahedi &
+                                    Surendra 1995); both are exact                */
 } pic_params;
+
+#define PIC_NULL_BOUND_BINNED 0
+#define PIC_NULL_BOUND_GLOBAL 1
 
 /* particleVec.h:26-35: the 48-byte AoS record of the reference */
 typedef struct pic_particle {

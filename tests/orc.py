@@ -420,6 +420,8 @@ def mcc_params(p, electron):
                   opal_B=10.0 if argon else 8.3, kev_amu_div=p.ion_mass / AMU,
                   key0=p.seed & 0xFFFFFFFF, key1=(p.seed >> 32) & 0xFFFFFFFF)
     lib().orc_mcc_setup(C.byref(c))
+    if getattr(p, "null_bound", 0) == 1:  # PIC_NULL_BOUND_GLOBAL
+        c.binned = 0
     return c
 
 

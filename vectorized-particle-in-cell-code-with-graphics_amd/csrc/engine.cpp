@@ -357,7 +357,7 @@ int setup_mcc(pic_handle *h, Species &s, bool electron) {
         double nu = nu_max_scan(c);
         c.nu_max = (float)nu;
         c.p_null = (float)(1.0 - exp(-nu * s.dt));
-        if (electron) {
+        if (electron && P.null_bound != PIC_NULL_BOUND_GLOBAL) {
             double nub[MCC_BINS], pb[MCC_BINS];
             electron_bins(c, nub, pb);
             float2 t[MCC_BINS];
@@ -1008,6 +1008,8 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     }
     if (P.no_ion_species != 1)
         return set_global_error(PIC_ERR_UNSUPPORTED, "NO_ION_SPECIES must be 1");
+    if (P.null_bound != PIC_NULL_BOUND_BINNED && P.null_bound != PIC_NULL_BOUND_GLOBAL)
+        return set_global_error(PIC_ERR_ARG, "null_bound must be PIC_NULL_BOUND_BINNED or _GLOBAL");
     if (P.mg_mode != PIC_MG_FIXED && P.mg_mode != PIC_MG_TOL)
         return set_global_error(PIC_ERR_ARG, "unknown multigrid mode");
     if (P.mg_mode == PIC_MG_TOL && (!(P.mg_tol > 0.0) || P.mg_max_iter < 1))

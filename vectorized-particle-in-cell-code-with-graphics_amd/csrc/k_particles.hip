@@ -100,7 +100,7 @@ template <int P>
 __device__ __forceinline__ void block_collide_n(const bool (&active)[P], const PState (&s)[P],
                                                 const uint32_t (&id)[P], const mcc::MccConst &mc,
                                                 uint32_t step, int (&qs)[P],
-                                                unsigned long long *lds) {
+                                                unsigned long long *lds, const float2 *tab) {
     static_assert(11 * CQ_CAP * 4 <= BLOCK_LDS_U64 * 8, "collision queue size");
     __shared__ uint32_t wcnt[4];
     float *q = reinterpret_cast<float *>(lds);  // SoA: v (3), id, event, child (6)
@@ -114,7 +114,8 @@ __device__ __forceinline__ void block_collide_n(const bool (&active)[P], const P
     for (int r = 0; r < P; r++) {
         const bool cand =
             active[r] && mcc::candidate(mc, id[r], step,
-                                        s[r].vx * s[r].vx + s[r].vy * s[r].vy + s[r].vz * s[r].vz);
+                                        s[r].vx * s[r].vx + s[r].vy * s[r].vy + s[r].vz * s[r].vz,
+                                        tab);
         const uint64_t bal = __ballot(cand);
         pos[r] = cand ? wtot + (uint32_t)__popcll(bal & below) : 0xFFFFFFFFu;
         wtot += (uint32_t)__popcll(bal);
@@ -147,7 +148,7 @@ __device__ __forceinline__ void block_collide_n(const bool (&active)[P], const P
     for (uint32_t t = threadIdx.x; t < nq; t += blockDim.x) {
         float vx = q[t], vy = q[CQ_CAP + t], vz = q[2 * CQ_CAP + t];
         mcc::Child c{};
-        const int e = mc.electron ? mcc::collide_electron(mc, vx, vy, vz, qid[t], step, true, c)
+        const int e = mc.electron ? mcc::collide_electron(mc, vx, vy, vz, qid[t], step, true, c, tab)
                                   : mcc::collide_ion(mc, vx, vy, vz, qid[t], step);
         q[t] = vx;
         q[CQ_CAP + t] = vy;
@@ -855,7 +856,10 @@ __global__ void __launch_bounds__(256)
         wmin[threadIdx.x >> 6] = kmin;
         wmax[threadIdx.x >> 6] = kmax;
     }
-    __syncthreads();  // counters cleared, key range known
+    // the electrons' null-collision bounds in LDS (loaded beside the state)
+    __shared__ float2 sbins[MCC_BINS];
+    if (mc.bins && threadIdx.x < MCC_BINS) sbins[threadIdx.x] = mc.bins[threadIdx.x];
+    __syncthreads();  // counters cleared, key range known, bounds staged
     kmin = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
     kmax = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
     // direct table: entry = key - kmin, H.cnt the per-key count, then the
@@ -901,7 +905,7 @@ __global__ void __launch_bounds__(256)
         bool act[IPP];
 #pragma unroll
         for (int r = 0; r < IPP; r++) act[r] = base + r * blockDim.x < n && s[r].jx != CELL_HOLE;
-        block_collide_n<IPP>(act, s, pid, mc, step, qs, lds);
+        block_collide_n<IPP>(act, s, pid, mc, step, qs, lds, mc.bins ? sbins : nullptr);
     }
     // the rounds are spelled out so every array index is a compile-time constant
     scatter_round(s[0], pid[0], w[0], base + 0 * blockDim.x < n, claim[0], kept[0],
@@ -1102,9 +1106,12 @@ __device__ __forceinline__ void push_inplace_body(
             if (live[r]) w[r] = load_w(sp, i);
         }
     }
-    __syncthreads();  // counters initialised
+    // the electrons' null-collision bounds in LDS (loaded beside the state)
+    __shared__ float2 sbins[MCC_BINS];
+    if (blk && mc.bins && threadIdx.x < MCC_BINS) sbins[threadIdx.x] = mc.bins[threadIdx.x];
+    __syncthreads();  // counters initialised, bounds staged
     int qs[IPP];
-    if (blk) block_collide_n<IPP>(live, s, pid, mc, step, qs, lds);
+    if (blk) block_collide_n<IPP>(live, s, pid, mc, step, qs, lds, mc.bins ? sbins : nullptr);
     bool dep[IPP];
     // the rounds are spelled out so every array index is a compile-time constant
     inplace_round(sp, base + 0 * blockDim.x, n, s[0], pid[0], live[0], dep[0], E, pc, mc, step,

@@ -262,9 +262,10 @@ __device__ __forceinline__ int energy_bin(const MccConst &c, float v2) {
     t = fminf(fmaxf(t, 0.0f), (float)(MCC_BINS - 1));
     return (int)t;
 }
-// the bound that applies to a particle: {p, nu}
-__device__ __forceinline__ float2 null_bound(const MccConst &c, float v2) {
-    return c.bins ? c.bins[energy_bin(c, v2)] : make_float2(c.p_null, c.nu_max);
+// the bound that applies to a particle: {p, nu}; tab: the bins (c.bins or a
+// block's LDS copy of them)
+__device__ __forceinline__ float2 null_bound(const MccConst &c, float v2, const float2 *tab) {
+    return tab ? tab[energy_bin(c, v2)] : make_float2(c.p_null, c.nu_max);
 }
 
 // event codes (counters)
@@ -366,11 +367,16 @@ struct Child {
 // candidate draws its collision from Philox4x32-10 stream RS_MAIN.
 #define RNG_KEY_CAND_E 0x43414E45u
 #define RNG_KEY_CAND_I 0x43414E49u
-// v2: the particle's speed^2 (selects its energy bin when c.bins is set)
-__device__ __forceinline__ bool candidate(const MccConst &c, uint32_t id, uint32_t step, float v2) {
+// v2: the particle's speed^2 (selects its energy bin when c.bins is set);
+// tab: c.bins or a block's LDS copy of them
+__device__ __forceinline__ bool candidate(const MccConst &c, uint32_t id, uint32_t step, float v2,
+                                          const float2 *tab) {
     uint32_t r[2];
     philox2x32_10(id, step, c.key0 ^ (c.electron ? RNG_KEY_CAND_E : RNG_KEY_CAND_I), r);
-    return u01(r[0]) < (c.bins ? c.bins[energy_bin(c, v2)].x : c.p_null);
+    return u01(r[0]) < (tab ? tab[energy_bin(c, v2)].x : c.p_null);
+}
+__device__ __forceinline__ bool candidate(const MccConst &c, uint32_t id, uint32_t step, float v2) {
+    return candidate(c, id, step, v2, c.bins);
 }
 
 // Electron collision.  Returns the event; on EV_IONIZE in PHYS mode `ch`
@@ -378,7 +384,8 @@ __device__ __forceinline__ bool candidate(const MccConst &c, uint32_t id, uint32
 // parent is removed.  inside: the particle passes the reference's
 // "inside the domain" test (e_collisions.cpp:507-511), used in REF mode.
 __device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, float &vy, float &vz,
-                                                uint32_t id, uint32_t step, bool inside, Child &ch) {
+                                                uint32_t id, uint32_t step, bool inside, Child &ch,
+                                                const float2 *tab) {
     const uint32_t s0 = (c.electron ? 0u : 16u);
     uint32_t r[4];
     if (c.mode == PIC_DEV_MODE_REF) {
@@ -422,7 +429,7 @@ __device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, fl
         return ev;
     }
     // PHYS null collision
-    const float2 bound = null_bound(c, vx * vx + vy * vy + vz * vz);
+    const float2 bound = null_bound(c, vx * vx + vy * vy + vz * vz, tab);
     {
         uint32_t rc[2];
         philox2x32_10(id, step, c.key0 ^ (c.electron ? RNG_KEY_CAND_E : RNG_KEY_CAND_I), rc);
@@ -499,6 +506,10 @@ __device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, fl
         return EV_IONIZE | over;
     }
     return EV_NONE | over;
+}
+__device__ __forceinline__ int collide_electron(const MccConst &c, float &vx, float &vy, float &vz,
+                                                uint32_t id, uint32_t step, bool inside, Child &ch) {
+    return collide_electron(c, vx, vy, vz, id, step, inside, ch, c.bins);
 }
 
 // Ion collision (PHYS: isotropic elastic + charge exchange against a

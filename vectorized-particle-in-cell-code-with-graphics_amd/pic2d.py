@@ -127,6 +127,7 @@ class Params(C.Structure):
         ("work_function", C.c_double),
         ("sort_key", C.c_int32),
         ("mg_sweeps", C.c_int32),
+        ("null_bound", C.c_int32),
     ]
 
     @property
