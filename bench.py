@@ -267,6 +267,9 @@ def main():
                          "library default)")
     ap.add_argument("--null-bound", choices=["binned", "global"], default="binned",
                     help="electron null-collision bound: per energy bin (default) or global")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay the solve's V-cycle batches from captured hipGraphs instead "
+                         "of launching them kernel by kernel")
     ap.add_argument("--runs", type=int, default=64, help="ensemble: number of runs")
     ap.add_argument("--requests", type=int, default=24, help="serving: requests per load")
     ap.add_argument("--step-scale", type=float, default=0.2,
@@ -300,6 +303,7 @@ def main():
     if args.mg_sweeps > 0:
         p.mg_sweeps = args.mg_sweeps
     p.null_bound = 1 if args.null_bound == "global" else 0
+    p.graphs = 1 if args.graphs else 0
     log("workload", args.workload, "nodes", p.nodes, "particles/species", p.init_particle_no)
     sim = pic2d.Pic(p)
     e0 = i0 = None

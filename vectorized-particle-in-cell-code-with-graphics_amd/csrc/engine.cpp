@@ -153,6 +153,8 @@ void free_mg(MgLevels &M) {
     if (M.host_flags) hipHostFree(M.host_flags);
     if (M.done) hipEventDestroy(M.done);
     M.done = nullptr;
+    for (MgLevels::CycleGraph &g : M.graphs) hipGraphExecDestroy(g.exec);
+    M.graphs.clear();
     M.partials = nullptr;
     M.flags = nullptr;
     M.relres = nullptr;
@@ -1194,6 +1196,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     hipMemsetAsync(h->E32, 0, 8ull * ((N + 1) * N + 1), h->st);  // pic.cpp:1086: first gather sees E = 0
     if ((rc = alloc_mg(h, h->M, N)) != PIC_OK) return fail(rc);
     h->M.nu = P.mg_sweeps > 0 ? P.mg_sweeps : 2;
+    h->M.use_graphs = P.graphs != 0;
 
     for (int s = 0; s < 2; s++) {
         Species &sp = h->sp[s];

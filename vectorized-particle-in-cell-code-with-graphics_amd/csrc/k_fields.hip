@@ -1250,12 +1250,42 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st, (const double *)u,
                        (const float *)nullptr, (double *)nullptr, f, n, h2i, r32, M.partials, tol,
                        M.flags, M.relres, M.host_flags);
+    // k cycles from `queued` on: launched one kernel at a time, or replayed
+    // from the hipGraph of that batch (k, parity of queued, iterate buffer),
+    // captured on the batch's first use; a capture records the launches of
+    // this thread on st without running them
+    auto queue_cycles = [&](int k) -> bool {
+        if (!M.use_graphs) {
+            for (int c = 0; c < k; c++, queued++) cycle();
+            return true;
+        }
+        const int par = queued & 1;
+        hipGraphExec_t ex = nullptr;
+        for (const MgLevels::CycleGraph &g : M.graphs)
+            if (g.k == k && g.parity == par && g.u0 == buf[0]) ex = g.exec;
+        if (!ex) {
+            hipGraph_t gr = nullptr;
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return false;
+            const int q0 = queued;
+            for (int c = 0; c < k; c++, queued++) cycle();
+            queued = q0;
+            if (hipStreamEndCapture(st, &gr) != hipSuccess) return false;
+            const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+            hipGraphDestroy(gr);
+            if (e != hipSuccess) return false;
+            M.graphs.push_back({k, par, buf[0], ex});
+        }
+        if (hipGraphLaunch(ex, st) != hipSuccess) return false;
+        queued += k;
+        return true;
+    };
     // queue the predicted number of cycles without waiting (converged cycles
     // skip their work on the device), then confirm on the host
     int batch = M.pred < 1 ? 1 : M.pred;
     bool conv = false;
     for (;;) {
-        for (int k = 0; k < batch && queued < max_iter; k++, queued++) cycle();
+        if (!queue_cycles(std::min(batch, max_iter - queued))) return -1;
         if (after_batch) after_batch(buf[0], buf[1]);
         // the initial residual pass plus one per queued cycle
         if (wait_published(st, M, 1 + queued) != hipSuccess) return -1;

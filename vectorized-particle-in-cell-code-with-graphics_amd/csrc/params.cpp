@@ -458,6 +458,7 @@ extern "C" int pic_params_default(pic_params *P) {
     P->sort_key = -1;
     P->mg_sweeps = -1;
     P->null_bound = PIC_NULL_BOUND_BINNED;
+    P->graphs = 0;
     return PIC_OK;
 }
 
@@ -623,6 +624,7 @@ extern "C" int pic_params_load(const char *path, pic_params *P) {
     ok = ok && inum("PIC_MG_MAX_ITER", P->mg_max_iter);
     ok = ok && inum("PIC_MG_SWEEPS", P->mg_sweeps);
     ok = ok && inum("PIC_NULL_BOUND", P->null_bound);
+    ok = ok && inum("PIC_GRAPHS", P->graphs);
     ok = ok && inum("PIC_SORT_INTERVAL", P->sort_interval);
     // diagnostic defaults follow the loaded grid and period; explicit macros win
     double period = 0.0;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <functional>
+#include <vector>
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -96,6 +97,15 @@ struct MgLevels {
     float *dst_D = nullptr;      // (2/M)^2 / (lambda_p + lambda_q), h^2 applied in the kernel
     int nu = 2;                  // red-black sweeps per leg of the fused V-cycle
     hipEvent_t done = nullptr;   // end of a queued batch of cycles (host wait)
+    // tolerance mode: a batch of k cycles from a given parity and iterate
+    // buffer is replayed from a captured hipGraph (captured on first use)
+    bool use_graphs = false;
+    struct CycleGraph {
+        int k, parity;
+        const double *u0;
+        hipGraphExec_t exec;
+    };
+    std::vector<CycleGraph> graphs;
 };
 
 // where created particles go during a pass (see k_particles.hip)

@@ -128,6 +128,7 @@ class Params(C.Structure):
         ("sort_key", C.c_int32),
         ("mg_sweeps", C.c_int32),
         ("null_bound", C.c_int32),
+        ("graphs", C.c_int32),
     ]
 
     @property
