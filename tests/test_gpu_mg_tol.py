@@ -161,3 +161,24 @@ def test_field_error_report_against_oracle():
     assert np.array_equal(rows[:, 0], [2, 5, 8])
     assert np.all(rows[:, 1] <= 1e-10) and np.all(rows[:, 5] <= 1e-12)
     assert np.all(rows[:, 2:5] < 1e-8)
+
+
+def test_graph_replayed_cycles_match_launched_cycles():
+    """the tolerance solve's V-cycle batches replayed from captured hipGraphs
+    give the states, charge, phi and E of the kernel-by-kernel launches, bit
+    for bit, over steps that reuse the cached graphs"""
+    out = []
+    for graphs in (0, 1):
+        p = workloads.discharge(128, 20, te=4.0)
+        p.mg_tol = 1e-10
+        p.graphs = graphs
+        with pic2d.Pic(p) as sim:
+            sim.run(12)
+            out.append((sim.get_state(0), sim.get_qfixed(0), sim.get_phi(), sim.get_E32(),
+                        sim.mg_stats()))
+    a, b = out
+    o0, o1 = np.argsort(a[0]["id"]), np.argsort(b[0]["id"])
+    for k in ("id", "cell", "ox", "oy", "vx", "vy", "vz"):
+        assert np.array_equal(a[0][k][o0], b[0][k][o1]), k
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+    assert a[4][1] == b[4][1]  # the same number of V-cycles

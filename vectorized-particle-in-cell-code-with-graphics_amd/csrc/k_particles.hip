@@ -1032,12 +1032,17 @@ __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint
     const bool kept = (st == ST_KEPT);
     dep = kept && deposit_ok(s, pc.N, pc.mode);
     if (kept) {
-        sp.cell[i] = cell_pack(s.jx, s.jy);
+        // the slot holds this particle's previous state: a word whose bits
+        // did not change (the cell of a particle that stayed in its cell, vz
+        // without a collision or rotation) is not written again, and a
+        // wavefront with no changed lane issues no store
+        const uint32_t c1 = cell_pack(s.jx, s.jy);
+        if (c1 != cell_pack(pre.jx, pre.jy)) sp.cell[i] = c1;
         sp.ox[i] = s.ox;
         sp.oy[i] = s.oy;
         sp.vx[i] = s.vx;
         sp.vy[i] = s.vy;
-        if (need_vz) sp.vz[i] = s.vz;
+        if (need_vz && __float_as_uint(s.vz) != __float_as_uint(pre.vz)) sp.vz[i] = s.vz;
     } else if (live && !cand) {
         sp.cell[i] = cell_pack(CELL_HOLE, CELL_HOLE);
         atomicAdd(&sl[st - 1], 1u);
