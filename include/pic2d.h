@@ -30,6 +30,8 @@ extern "C" {
 #define PIC_ERR_CAPACITY (-6)    /* particle store full                      */
 #define PIC_ERR_UNSUPPORTED (-7) /* option outside this build's scope        */
 #define PIC_ERR_ALLOC (-8)       /* device or host allocation failed         */
+#define PIC_ERR_NUMERIC (-9)     /* the potential or field became non-finite */
+#define PIC_ERR_TIMEOUT (-10)    /* pic_serve_wait: the request is not done  */
 
 /* --------------------------------------------------------------- options */
 #define PIC_MODE_REF 0      /* the reference's wall tests, frozen particles,
@@ -240,6 +242,9 @@ int pic_contain_particle_growth(pic_handle *h, int32_t s);
  * nodes*nodes values), the reference solve's relative residual, its V-cycles
  * after the full multigrid.  Synchronises the handle's stream. */
 int pic_field_error(pic_handle *h, double tol, double *out5);
+/* *finite = 1 when every value of phi, Ex and Ey is finite, else 0
+ * (synchronises the handle's stream; no reference counterpart) */
+int pic_field_finite(pic_handle *h, int32_t *finite);
 /* pic_field_error(h, tol) after every `every`-th step of pic_run and
  * pic_advance (0: off), one row of 6 doubles per report: step, the solver's
  * relative residual of that step, error of phi, Ex, Ey, the reference
@@ -389,6 +394,47 @@ typedef struct pic_serve_stats {
  * (every result carries its own status). */
 int pic_serve(const pic_serve_request *reqs, int32_t n, int64_t hbm_budget, int32_t max_active,
               pic_serve_result *res, pic_serve_stats *stats);
+
+/* A serving loop as an object (no reference counterpart).  Requests are
+ * submitted at any time and queue in FIFO order; a scheduler thread admits
+ * the head of the queue whenever the estimated memory of the active runs plus
+ * its own fits hbm_budget and fewer than max_active runs are active, so
+ * requests that arrive while others run join them as memory and slots free
+ * up (continuous batching); each admitted run steps on its own stream and
+ * thread, and its results are those of the same run alone, bit for bit.
+ * Between steps a run obeys pic_serve_cancel; a run whose potential or field
+ * turns non-finite stops with PIC_ERR_NUMERIC, the others go on.  Thread
+ * safe; pic_server_destroy cancels what is queued or running and waits. */
+typedef struct pic_server pic_server;
+#define PIC_REQ_QUEUED 0
+#define PIC_REQ_RUNNING 1
+#define PIC_REQ_DONE 2
+#define PIC_REQ_CANCELLED 3
+#define PIC_REQ_FAILED 4
+typedef struct pic_serve_status {
+    int32_t state;              /* PIC_REQ_*                                        */
+    int32_t error;              /* PIC_OK, or the PIC_ERR_* that failed it          */
+    int32_t steps_done, steps;  /* steps completed / requested                      */
+    double submit_s, admit_s, finish_s; /* seconds since pic_server_create          */
+    double step_p50_ms, step_p99_ms;    /* per-step latency, issue to completion    */
+    int64_t particle_steps;     /* sum over its steps of the live particles         */
+    double phi_error, e_error;  /* DONE: relative L2 error of the last step's phi
+                                   and E against the FP64 multigrid converged to
+                                   1e-12 (pic_field_error); -1 otherwise            */
+    uint64_t fingerprint;       /* DONE: FNV-1a of the final fixed-point charge of
+                                   both species and of phi (bit patterns)           */
+    int64_t device_bytes;       /* device memory the run allocated                  */
+} pic_serve_status;
+int pic_server_create(int64_t hbm_budget, int32_t max_active, pic_server **out);
+int pic_server_destroy(pic_server *srv);
+/* queues a run of `steps` fused steps (after pic_run's pre-loop wall pass) */
+int pic_serve_submit(pic_server *srv, const pic_params *p, int32_t steps, int64_t *id);
+int pic_serve_poll(pic_server *srv, int64_t id, pic_serve_status *out);
+/* a queued request never starts; a running one stops before its next step */
+int pic_serve_cancel(pic_server *srv, int64_t id);
+/* blocks until the request is DONE, CANCELLED or FAILED (PIC_OK) or timeout_s
+ * passes (PIC_ERR_TIMEOUT; timeout_s < 0 waits without limit); out may be NULL */
+int pic_serve_wait(pic_server *srv, int64_t id, double timeout_s, pic_serve_status *out);
 int pic_get_E32(pic_handle *h, float *out);  /* fp32 (Ex,Ey) pairs, (nodes+1)*nodes+1      */
 int pic_set_E32(pic_handle *h, const float *in);
 

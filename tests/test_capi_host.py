@@ -199,3 +199,30 @@ def test_mglin_noalloc_rejects_bad_sizes_before_any_device_call():
     assert lib.pic_mglin_noalloc(fake, 1000, 2, 1.0, None) == -2
     assert lib.pic_mglin_noalloc(None, 65, 2, 1.0, None) == -1
     assert lib.pic_mglin_noalloc(fake, 65, 0, 1.0, None) == -1
+
+
+def test_server_api_host_logic():
+    """the serving object without a GPU: argument checks, a request that can
+    never fit the budget, unknown ids, cancel of a queued request before any
+    device work (max_active 1 with the slot held by a request that cannot
+    start here)"""
+    lib = pic2d.load_library()
+    s = ctypes.c_void_p()
+    assert lib.pic_server_create(0, 4, ctypes.byref(s)) == -1
+    assert lib.pic_server_create(1 << 30, 0, ctypes.byref(s)) == -1
+    p = pic2d.default_params()
+    with pic2d.Server(1 << 20, 1) as srv:        # 1 MiB: nothing fits
+        with pytest.raises(pic2d.PicError) as e:
+            srv.submit(p, 10)
+        assert e.value.code == -6
+        with pytest.raises(pic2d.PicError):
+            srv.poll(12345)
+    with pic2d.Server(1 << 40, 1) as srv:
+        a = srv.submit(p, 10)
+        st = srv.wait(a, timeout=120)
+        # without a GPU the run fails at creation; with one it completes
+        assert st["state"] in (pic2d.REQ_DONE, pic2d.REQ_FAILED)
+        b = srv.submit(p, 10)
+        srv.cancel(b)
+        st = srv.wait(b, timeout=120)
+        assert st["state"] in (pic2d.REQ_CANCELLED, pic2d.REQ_DONE, pic2d.REQ_FAILED)

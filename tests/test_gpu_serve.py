@@ -74,3 +74,78 @@ def test_serving_loop():
     admits = [r["admit_s"] for r in res]
     finishes = [r["finish_s"] for r in res]
     assert all(admits[k + 1] >= finishes[k] - 1e-3 for k in range(len(runs) - 1))
+
+
+# ---------------------------------------------------------------- the server
+BUDGET = 200 * 10 ** 9
+
+
+def serve_alone(p, steps):
+    with pic2d.Server(BUDGET, 1) as srv:
+        return srv.wait(srv.submit(p, steps), timeout=240)
+
+
+def test_server_results_equal_runs_alone_and_report_phi_error():
+    runs = small_runs()[:3]
+    steps = 15
+    alone = [serve_alone(p, steps) for p in runs]
+    for st in alone:
+        assert st["state"] == pic2d.REQ_DONE and st["error"] == 0
+        assert 0 <= st["phi_error"] < 1e-9 and 0 <= st["e_error"] < 1e-8, st
+    with pic2d.Server(BUDGET, 8) as srv:
+        ids = [srv.submit(p, steps) for p in runs]
+        mixed = [srv.wait(i, timeout=240) for i in ids]
+    print("fingerprints alone", [hex(s["fingerprint"]) for s in alone],
+          "mixed", [hex(s["fingerprint"]) for s in mixed], flush=True)
+    for a, m in zip(alone, mixed):
+        assert m["state"] == pic2d.REQ_DONE and m["steps_done"] == steps
+        assert m["fingerprint"] == a["fingerprint"] and m["particle_steps"] == a["particle_steps"]
+        assert m["phi_error"] == a["phi_error"]
+
+
+def test_server_quarantines_a_non_finite_run():
+    """a run whose charge scale is NaN gets a NaN potential on its first step:
+    it stops with PIC_ERR_NUMERIC and frees its slot; the runs beside it finish
+    with the results they have alone"""
+    runs = small_runs()[:2]
+    steps = 12
+    alone = [serve_alone(p, steps) for p in runs]
+    bad = small_runs()[2]
+    bad.qce = float("nan")
+    bad.qci = float("nan")
+    with pic2d.Server(BUDGET, 8) as srv:
+        ids = [srv.submit(runs[0], steps), srv.submit(bad, steps), srv.submit(runs[1], steps)]
+        st = [srv.wait(i, timeout=240) for i in ids]
+    assert st[1]["state"] == pic2d.REQ_FAILED and st[1]["error"] == pic2d.PIC_ERR_NUMERIC
+    assert st[1]["steps_done"] == 1
+    for a, m in zip(alone, (st[0], st[2])):
+        assert m["state"] == pic2d.REQ_DONE and m["fingerprint"] == a["fingerprint"]
+
+
+def test_server_cancel_mid_run_and_fifo_admission():
+    runs = small_runs()
+    long_run, short = runs[3], runs[0]
+    want = serve_alone(short, 10)
+    with pic2d.Server(BUDGET, 2) as srv:
+        lid = srv.submit(long_run, 100000)
+        sid = srv.submit(short, 10)
+        st = srv.poll(lid)
+        while st["steps_done"] < 20 and st["state"] in (pic2d.REQ_QUEUED, pic2d.REQ_RUNNING):
+            st = srv.poll(lid)
+        srv.cancel(lid)
+        ls = srv.wait(lid, timeout=120)
+        ss = srv.wait(sid, timeout=240)
+        assert ls["state"] == pic2d.REQ_CANCELLED and 20 <= ls["steps_done"] < 100000
+        assert ss["state"] == pic2d.REQ_DONE and ss["fingerprint"] == want["fingerprint"]
+    # one slot: the second request is admitted after the first finishes
+    with pic2d.Server(BUDGET, 1) as srv:
+        a, b = srv.submit(short, 5), srv.submit(runs[1], 5)
+        sa, sb = srv.wait(a, timeout=240), srv.wait(b, timeout=240)
+        assert sb["admit_s"] >= sa["finish_s"] and sa["admit_s"] <= sb["admit_s"]
+        # a queued request cancelled before its turn never runs
+        c, d = srv.submit(long_run, 100000), srv.submit(short, 5)
+        srv.cancel(d)
+        sd = srv.wait(d, timeout=60)
+        assert sd["state"] == pic2d.REQ_CANCELLED and sd["steps_done"] == 0
+        srv.cancel(c)
+        assert srv.wait(c, timeout=120)["state"] == pic2d.REQ_CANCELLED

@@ -1259,6 +1259,7 @@ int pic_destroy(pic_handle *h) {
         if (p) hipFree(p);
     free_mg(h->M);
     free_mg(h->ref);
+    if (h->nonfinite) hipFree(h->nonfinite);
     for (void *p : {(void *)h->ref_phi, (void *)h->ref_ex, (void *)h->ref_ey, (void *)h->ref_E32,
                     (void *)h->ref_partials})
         if (p) hipFree(p);
@@ -1411,6 +1412,21 @@ int pic_field_error(pic_handle *h, double tol, double *out5) {
     double e[6];
     CHECK(field_error(h, tol, e));
     for (int k = 0; k < 5; k++) out5[k] = e[k];
+    return PIC_OK;
+}
+
+int pic_field_finite(pic_handle *h, int32_t *finite) {
+    if (!h || !finite) return set_global_error(PIC_ERR_ARG, "bad argument");
+    if (!h->nonfinite) {
+        CHECK(dalloc(h, &h->nonfinite, 1));
+    }
+    launch_fill<uint32_t>(h->st, h->nonfinite, 1, 0u);
+    const long n = (long)h->N * h->N, ne = (long)(h->N + 1) * (h->N + 1);
+    launch_count_nonfinite(h->st, h->phi, n, h->ex64, h->ey64, ne, h->nonfinite);
+    unsigned bad = 0;
+    HIPCHK(h, hipMemcpyAsync(&bad, h->nonfinite, sizeof(unsigned), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    *finite = bad == 0;
     return PIC_OK;
 }
 

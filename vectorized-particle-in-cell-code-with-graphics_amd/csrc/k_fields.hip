@@ -1391,6 +1391,22 @@ int mg_fp64_reference(hipStream_t st, MgLevels &R, double length, double tol, in
     return c;
 }
 
+// non-finite values among n entries of a, b, c (counted into *count)
+__global__ void k_count_nonfinite(const double *__restrict__ a, long na, const double *__restrict__ b,
+                                  const double *__restrict__ c, long nbc, unsigned *__restrict__ count) {
+    unsigned bad = 0;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < na; k += (long)gridDim.x * blockDim.x)
+        bad += !isfinite(a[k]);
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < nbc; k += (long)gridDim.x * blockDim.x)
+        bad += !isfinite(b[k]) + !isfinite(c[k]);
+    if (bad) atomicAdd(count, bad);
+}
+
+void launch_count_nonfinite(hipStream_t st, const double *a, long na, const double *b,
+                            const double *c, long nbc, unsigned *count) {
+    hipLaunchKernelGGL(k_count_nonfinite, dim3(256), dim3(256), 0, st, a, na, b, c, nbc, count);
+}
+
 int field_diff_sums(hipStream_t st, const double *a0, const double *b0, const double *a1,
                     const double *b1, const double *a2, const double *b2, long n,
                     double *partials, double *host, int nb, double *out6) {

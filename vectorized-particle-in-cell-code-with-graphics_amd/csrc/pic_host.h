@@ -112,6 +112,7 @@ struct pic_handle {
     int32_t acc_every = 0;             // report every acc_every steps (0: off)
     double acc_tol = 1e-12;
     std::vector<double> acc_rows;      // step, solver relres, err phi, Ex, Ey, ref relres
+    uint32_t *nonfinite = nullptr;     // device counter of pic_field_finite
     int mg_cycles = 0;          // V-cycles (tolerance mode) of the last solve
     int64_t mg_cycles_total = 0;
     double mg_relres = 0.0;     // relative residual after the last solve

@@ -318,6 +318,9 @@ void mg_fixed_solve(hipStream_t st, MgLevels &M, int ncycle, double length);
 // V-cycles after the full multigrid (-1 on a stream error); synchronises.
 int mg_fp64_reference(hipStream_t st, MgLevels &R, double length, double tol, int max_cycles,
                       double *partials, double *host, int nb, double *relres);
+// adds the non-finite values among a[0..na) and b, c[0..nbc) to *count
+void launch_count_nonfinite(hipStream_t st, const double *a, long na, const double *b,
+                            const double *c, long nbc, unsigned *count);
 // sums of (a_k - b_k)^2 and b_k^2 over n entries for three pairs (out6);
 // partials: device, >= 6 nb doubles; synchronises
 int field_diff_sums(hipStream_t st, const double *a0, const double *b0, const double *a1,

@@ -34,6 +34,8 @@ ERRORS = {
     -6: "PIC_ERR_CAPACITY",
     -7: "PIC_ERR_UNSUPPORTED",
     -8: "PIC_ERR_ALLOC",
+    -9: "PIC_ERR_NUMERIC",
+    -10: "PIC_ERR_TIMEOUT",
 }
 MODE_REF, MODE_PHYSICAL = 0, 1
 MG_FIXED, MG_TOL = 0, 1
@@ -166,6 +168,22 @@ PARTICLE_DTYPE = np.dtype(
 _lib = None
 
 
+class ServeStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("error", C.c_int32), ("steps_done", C.c_int32),
+                ("steps", C.c_int32), ("submit_s", C.c_double), ("admit_s", C.c_double),
+                ("finish_s", C.c_double), ("step_p50_ms", C.c_double),
+                ("step_p99_ms", C.c_double), ("particle_steps", C.c_int64),
+                ("phi_error", C.c_double), ("e_error", C.c_double), ("fingerprint", C.c_uint64),
+                ("device_bytes", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = 0, 1, 2, 3, 4
+PIC_ERR_CAPACITY, PIC_ERR_NUMERIC, PIC_ERR_TIMEOUT = -6, -9, -10
+
+
 def declared_functions(header=HEADER_PATH):
     """Names of every function declared in include/pic2d.h."""
     text = open(header).read()
@@ -251,6 +269,13 @@ def load_library(path=LIB_PATH):
         "pic_cross_section": [i32, i32, i32, dbl, P(dbl)],
         "pic_field_error": [vp, dbl, vp],
         "pic_mglin_noalloc": [vp, i32, i32, dbl, vp],
+        "pic_field_finite": [vp, P(i32)],
+        "pic_server_create": [i64, i32, P(vp)],
+        "pic_server_destroy": [vp],
+        "pic_serve_submit": [vp, P(Params), i32, P(i64)],
+        "pic_serve_poll": [vp, i64, P(ServeStatus)],
+        "pic_serve_cancel": [vp, i64],
+        "pic_serve_wait": [vp, i64, dbl, P(ServeStatus)],
         "pic_set_accuracy_report": [vp, i32, dbl],
         "pic_get_accuracy_report": [vp, vp, i32, P(i32)],
     }
@@ -351,6 +376,54 @@ def serve(requests, hbm_budget, max_active=64):
     stats = {k: getattr(st, k) for k, _ in ServeStats._fields_}
     _check(rc)
     return out, stats
+
+
+class Server:
+    """the serving loop (pic_server_create ...): submit runs at any time, poll,
+    cancel, wait; continuous batching under an HBM budget"""
+
+    def __init__(self, hbm_budget, max_active=64):
+        self.lib = load_library()
+        self.s = C.c_void_p()
+        _check(self.lib.pic_server_create(int(hbm_budget), int(max_active), C.byref(self.s)))
+
+    def submit(self, params: Params, steps) -> int:
+        rid = C.c_int64()
+        _check(self.lib.pic_serve_submit(self.s, C.byref(params), int(steps), C.byref(rid)))
+        return rid.value
+
+    def poll(self, rid) -> dict:
+        st = ServeStatus()
+        _check(self.lib.pic_serve_poll(self.s, int(rid), C.byref(st)))
+        return st.as_dict()
+
+    def cancel(self, rid):
+        _check(self.lib.pic_serve_cancel(self.s, int(rid)))
+
+    def wait(self, rid, timeout=None) -> dict:
+        """the request's status once DONE, CANCELLED or FAILED; raises
+        PicError(PIC_ERR_TIMEOUT) if timeout (s) passes first"""
+        st = ServeStatus()
+        _check(self.lib.pic_serve_wait(self.s, int(rid), -1.0 if timeout is None else float(timeout),
+                                       C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.s:
+            self.lib.pic_server_destroy(self.s)
+            self.s = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Pic:
@@ -580,6 +653,11 @@ class Pic:
         n = C.c_int32()
         self._c(self.lib.pic_get_accuracy_report(self.h, _ptr(out), cap, C.byref(n)))
         return out[: min(n.value, cap)]
+
+    def field_finite(self):
+        f = C.c_int32()
+        self._c(self.lib.pic_field_finite(self.h, C.byref(f)))
+        return bool(f.value)
 
     def overflow(self):
         v = C.c_int64()
