@@ -211,10 +211,13 @@ def run_serving(args):
         log(f"serving: load {load}, {len(reqs)} requests at {rate:.2f}/s")
         res, st = pic2d.serve(reqs, budget, args.max_active)
         wait = [r["admit_s"] - q[2] for r, q in zip(res, reqs)]
+        phi_err = max(r["phi_error"] for r in res)
+        e_err = max(r["e_error"] for r in res)
         loads.append({"offered_load": load, "requests": len(reqs), "rate_per_s": rate,
                       "throughput": st["throughput"], "step_p50_ms": st["step_p50_ms"],
                       "step_p99_ms": st["step_p99_ms"], "queue_wait_mean_s": float(np.mean(wait)),
-                      "max_concurrent": st["max_concurrent"], "peak_bytes": st["peak_bytes"]})
+                      "max_concurrent": st["max_concurrent"], "peak_bytes": st["peak_bytes"],
+                      "max_request_phi_rel_l2": phi_err, "max_request_e_rel_l2": e_err})
     full = [l for l in loads if l["offered_load"] == 1.0][0]
     return {
         "metric": "particle-steps/s",

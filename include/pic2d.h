@@ -376,6 +376,8 @@ typedef struct pic_serve_result {
     int64_t particle_steps;     /* sum over its steps of the live particles   */
     double step_p50_ms, step_p99_ms;  /* per-step latency (issue to completion) */
     int32_t status;             /* PIC_OK or the run's error                   */
+    double phi_error, e_error;  /* last step against the FP64 multigrid
+                                   (pic_serve_status), -1 if the run failed    */
 } pic_serve_result;
 
 typedef struct pic_serve_stats {

@@ -407,6 +407,8 @@ int pic_serve(const pic_serve_request *reqs, int32_t n, int64_t hbm_budget, int3
         res[k].step_p50_ms = st.step_p50_ms;
         res[k].step_p99_ms = st.step_p99_ms;
         res[k].status = st.error;
+        res[k].phi_error = st.phi_error;
+        res[k].e_error = st.e_error;
         ps += st.particle_steps;
         if (rc_first == PIC_OK && st.error != PIC_OK) rc_first = st.error;
         std::lock_guard<std::mutex> g(srv->mu);

@@ -150,7 +150,8 @@ class ServeResult(C.Structure):
     """Mirror of ``pic_serve_result``."""
     _fields_ = [("admit_s", C.c_double), ("finish_s", C.c_double),
                 ("particle_steps", C.c_int64), ("step_p50_ms", C.c_double),
-                ("step_p99_ms", C.c_double), ("status", C.c_int32)]
+                ("step_p99_ms", C.c_double), ("status", C.c_int32), ("phi_error", C.c_double),
+                ("e_error", C.c_double)]
 
 
 class ServeStats(C.Structure):
