@@ -114,7 +114,7 @@ long orc_reduce(const orc_reduce_params *c, uint32_t step, long n, const float *
         long m = 0;
         for (long i = 0; i < n; i++)
             if (grp[i] == g) keys[m++] = rkey(id[i], step, c);
-        qsort(keys, (size_t)m, sizeof(uint64_t), cmp_u64);
+        if (m > 1) qsort(keys, (size_t)m, sizeof(uint64_t), cmp_u64);
         uint64_t thr = keys[r - 1];
         double gw = N > r ? (double)N / (double)(N - r) : 1.0;
         for (long i = 0; i < n; i++) {

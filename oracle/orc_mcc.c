@@ -264,7 +264,7 @@ static int kv_cmp(const void *a, const void *b) {
 /* append staged particles in creator-key order with consecutive ids */
 static long append_sorted(orc_particle *dst, uint32_t *did, long n, long cap, const orc_particle *st,
                           orc_kv *kv, long ns, uint32_t *next_id) {
-    qsort(kv, (size_t)ns, sizeof(orc_kv), kv_cmp);
+    if (ns > 1) qsort(kv, (size_t)ns, sizeof(orc_kv), kv_cmp);
     for (long r = 0; r < ns; r++) {
         if (n < cap) {
             dst[n] = st[kv[r].src];
@@ -497,7 +497,7 @@ long orc32_ref_cx(long n, const uint32_t *cell, const float *ox, const float *oy
         v[m].P = orc32_ref_cx_prob(c, vx[k], vy[k], vz[k]);
         m++;
     }
-    qsort(v, (size_t)m, sizeof(orc_cx_ent), cx_cmp);
+    if (m > 1) qsort(v, (size_t)m, sizeof(orc_cx_ent), cx_cmp);
     long fired = 0;
     const uint32_t key[2] = {c->key0, c->key1};
     for (long t = 0; t < m; t++) {

@@ -263,7 +263,7 @@ static long collect_cands(long n, const uint32_t *cell, const uint32_t *id, int 
         if (jy < 0) out[m++] = (orc_cand){2, id[k], k};
         else if (jy >= N) out[m++] = (orc_cand){3, id[k], k};
     }
-    qsort(out, (size_t)m, sizeof(orc_cand), cand_cmp);
+    if (m > 1) qsort(out, (size_t)m, sizeof(orc_cand), cand_cmp);
     return m;
 }
 
