@@ -282,6 +282,73 @@ def test_reference_steady_walls_match_the_oracle(sort_interval):
             assert far > 0
 
 
+@pytest.mark.parametrize("skew", [False, True])
+def test_reference_steady_walls_many_candidates(skew):
+    """The steady walls with tens of thousands of particles outside (45 % of
+    the electrons, 30 % of the ions, on every side and in the corners, up to
+    two cells beyond a wall): the pairing is bit-exact against the oracle.
+    With `skew` every outside particle has an id below 2^16 and every inside
+    one an id above 2^30, so all candidates of a side fall into one id bucket
+    and the bucket is ordered by counting ranks instead of in LDS."""
+    p = workloads.discharge(64, 20, mode=pic2d.MODE_REF, mg_mode=pic2d.MG_FIXED)
+    p.wall = pic2d.WALL_STEADY
+    N = p.nodes
+    print(f"steady walls, many candidates, skew {skew}", flush=True)
+    rng = np.random.default_rng(11)
+    with pic2d.Pic(p) as sim:
+        sim.run(1)
+        states = []
+        for s, frac in ((0, 0.45), (1, 0.3)):
+            d = sim.get_state(s)
+            n = len(d["id"])
+            out = rng.random(n) < frac
+            side = rng.integers(0, 6, n)  # left, right, bottom, top, left-bottom, right-top
+            jx, jy = pic2d.cell_unpack(d["cell"])
+            jx, jy = jx.astype(np.int64), jy.astype(np.int64)
+            lo = -1 - rng.integers(0, 2, n)
+            hi = N + rng.integers(0, 2, n)
+            jx = np.where(out & np.isin(side, (0, 4)), lo, jx)
+            jx = np.where(out & np.isin(side, (1, 5)), hi, jx)
+            jy = np.where(out & np.isin(side, (2, 4)), lo, jy)
+            jy = np.where(out & np.isin(side, (3, 5)), hi, jy)
+            d["cell"] = pic2d.cell_pack(jx, jy)
+            if skew:
+                ids = np.empty(n, dtype=np.uint32)
+                ids[out] = rng.permutation(int(out.sum())).astype(np.uint32)
+                ids[~out] = (1 << 30) + rng.permutation(int((~out).sum())).astype(np.uint32)
+                d["id"] = ids
+            sim.set_state(s, d)
+            states.append(d)
+            print(f"species {s}: {int(out.sum())} outside of {n}", flush=True)
+        E0 = sim.get_E32()
+        c32, _ = orc.configs_from_params(p)
+        c32.preloop = 0
+        c32.start_step = 1
+        r = orc.run32(c32, 1, states[0], states[1], E0)
+        sim.advance(1)
+        e1, i1 = by_id(sim.get_state(0)), by_id(sim.get_state(1))
+        re, ri = by_id(r["e"]), by_id(r["i"])
+        for a, b, what in ((e1, re, "e"), (i1, ri, "i")):
+            assert np.array_equal(a["id"], b["id"]), what
+            for k in ("cell", "ox", "oy"):
+                bad = np.flatnonzero(a[k] != b[k])
+                assert bad.size == 0, (what, k, bad.size)
+            for k in ("vx", "vy", "vz"):
+                np.testing.assert_allclose(a[k], b[k], rtol=4e-7, atol=0, err_msg=f"{what}.{k}")
+        assert np.array_equal(sim.get_qfixed(0), r["qe"])
+        assert np.array_equal(sim.get_qfixed(1), r["qi"])
+        # most outside ions found a partner: well over 16384 pairs
+        def outside(c):
+            jx, jy = pic2d.cell_unpack(c)
+            return (jx < 0) | (jx >= N) | (jy < 0) | (jy >= N)
+        before = outside(by_id(states[1])["cell"])
+        after = outside(i1["cell"])
+        moved = int((before & ~after).sum())
+        print(f"ions outside before {int(before.sum())}, re-injected {moved}", flush=True)
+        assert moved > 16384
+        assert sim.overflow() == 0
+
+
 def test_batch_diagnostics_match_the_oracle():
     """Graph-window sums and timeData rows collected on the device against the
     oracle fed with the same per-step states: field and density averages,

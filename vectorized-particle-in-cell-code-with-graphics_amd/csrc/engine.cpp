@@ -236,6 +236,10 @@ void species_free(Species &s) {
                   s.stage.vy,   s.stage.vz, s.stage.key, s.stage.idx};
     for (void *p : st)
         if (p) hipFree(p);
+    const pic::SteadyBufs &c = s.steady;
+    void *sb[] = {c.hist, c.start, c.cursor, c.id_in, c.idx_in, c.id_out, c.idx_out};
+    for (void *p : sb)
+        if (p) hipFree(p);
     s = Species();
 }
 
@@ -497,10 +501,34 @@ int ref_ion_cx(pic_handle *h) {
 // re-injected pairs are deposited into this step's charge
 void steady_boundary(pic_handle *h) {
     Species &e = h->sp[0], &i = h->sp[1];
-    launch_steady(h->st, e.buf[e.cur], &h->cnt[0].n[e.cur], e.cap, i.buf[i.cur],
-                  &h->cnt[1].n[i.cur], i.cap, e.stage, i.stage, &h->cnt[0], &h->cnt[1], i.mc,
-                  (uint32_t)h->step, h->N, (float)h->P.vte, (float)h->P.vti, e.q, i.q,
-                  e.pc.mode);
+    SteadyArgs a;
+    for (int s = 0; s < 2; s++) {
+        Species &sp = h->sp[s];
+        a.sp[s] = sp.buf[sp.cur];
+        a.n[s] = &h->cnt[s].n[sp.cur];
+        a.cnt[s] = &h->cnt[s];
+        a.b[s] = sp.steady;
+        a.cap[s] = sp.cap;
+    }
+    launch_steady(h->st, a, i.mc, (uint32_t)h->step, h->N, (float)h->P.vte, (float)h->P.vti, e.q,
+                  i.q, e.pc.mode);
+}
+
+// candidate buffers of the steady walls: every particle of the store may be
+// outside on an axis; nb id ranges per side keep the buckets near 64 entries
+int alloc_steady(pic_handle *h, Species &s, uint32_t nb) {
+    SteadyBufs &b = s.steady;
+    b.nb = nb;
+    CHECK(dalloc(h, &b.hist, 2 * (size_t)nb));
+    CHECK(dalloc(h, &b.start, 2 * (size_t)nb + 1));
+    CHECK(dalloc(h, &b.cursor, 2 * (size_t)nb));
+    if (hipMemsetAsync(b.hist, 0, sizeof(uint32_t) * 2 * nb, h->st) != hipSuccess)
+        return seterr(h, PIC_ERR_DEVICE, "hipMemsetAsync failed");
+    CHECK(dalloc(h, &b.id_in, (size_t)s.cap));
+    CHECK(dalloc(h, &b.idx_in, (size_t)s.cap));
+    CHECK(dalloc(h, &b.id_out, (size_t)s.cap));
+    CHECK(dalloc(h, &b.idx_out, (size_t)s.cap));
+    return PIC_OK;
 }
 
 // append species `src`'s staged particles to species `dst_s`'s current buffer
@@ -1082,6 +1110,12 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         update_scale(h, h->sp[s]);
         Species &sp = h->sp[s];
         if (h->wall_on && (rc = alloc_wall_queue(h, sp, n0[s])) != PIC_OK) return fail(rc);
+        if (ref_steady(h)) {
+            const int64_t cmax = std::max(n0[0], n0[1]) + 4096 * n_inj;
+            uint32_t nb = 16;
+            while (nb < 65536 && (int64_t)nb * 64 < cmax) nb <<= 1;
+            if ((rc = alloc_steady(h, sp, nb)) != PIC_OK) return fail(rc);
+        }
         // the physical-mode steady walls queue the lost ions there for
         // re-injection (enable_weights adds the weights when needed)
         if (!h->wall_on && s == 1 && plop_on(h) &&

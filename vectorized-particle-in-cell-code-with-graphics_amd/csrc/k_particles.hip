@@ -1646,77 +1646,138 @@ __global__ void __launch_bounds__(1024) k_commit_pair(CommitArgs a, CommitArgs b
     commit_body(c.sv, c.scnt, c.dst, c.n_dst, c.cap_dst, c.dcnt, c.deposit, N, c.mode, c.q);
 }
 
-// ------------------------------------------------------------ loader
-// Uniform positions in (0,L)^2 and a 3-V Maxwellian, from Philox4x32-10 keyed
-// by the seed with counter (id, species, call, 0).
 // ---------------------------------------------- reference steady-state walls
-// Side flags of a particle for boundary.cpp:906-912 (x) and the y analogue:
-// 1 left (x < 0), 2 right ((int)(x/dx) >= N), 4 bottom (y < 0), 8 top.
-__device__ __forceinline__ int steady_sides(int jx, int jy, int N) {
-    int f = 0;
-    if (jx < 0) f |= 1;
-    else if (jx >= N) f |= 2;
-    if (jy < 0) f |= 4;
-    else if (jy >= N) f |= 8;
-    return f;
+// steady_e_boundary (boundary.cpp:890-1097) pairs, per side, the m-th
+// electron outside with the m-th ion outside and re-injects each pair; the
+// x sides first (left x < 0, right (int)(x/dx) >= N, boundary.cpp:906-912),
+// then the y sides among the particles the x pass left outside.  Order
+// within a side = particle id (the reference's vector order depends on its
+// removal history, which this store does not keep).
+//
+// Each axis is one pass over both species with any number of candidates:
+// a histogram of (side, id range) buckets, an exclusive scan of the buckets,
+// a scatter into bucket order, a sort of each bucket by id (one workgroup per
+// bucket) and the pairing, one thread per pair.  A particle the x pass
+// re-injects lands inside, so collecting the y candidates from the store
+// after the x pass leaves out exactly the particles the x pass paired.
+
+// side of a particle on an axis: 0 low (x < 0 / y < 0), 1 high (cell >= N),
+// -1 inside or an empty slot
+__device__ __forceinline__ int steady_side(uint32_t c, int N, int axis) {
+    const int jx = cell_jx(c);
+    if (jx == CELL_HOLE || jx == CELL_DEAD) return -1;
+    const int j = axis == 0 ? jx : cell_jy(c);
+    return j < 0 ? 0 : (j >= N ? 1 : -1);
 }
 
-// Candidates of the steady walls: one entry (side << 32 | id, index) per
-// side a particle is outside of (a corner particle has an x and a y entry).
-__global__ void __launch_bounds__(256)
-    k_steady_collect(SpView sp, const uint32_t *__restrict__ nptr, int N, StageView sv,
-                     uint32_t *__restrict__ count, uint32_t *__restrict__ overflow) {
-    const uint32_t n = *nptr;
+// bucket of (side, id): side-major, then nb equal ranges of [0, idmax)
+__device__ __forceinline__ uint32_t steady_bucket(int side, uint32_t id, uint32_t idmax,
+                                                  uint32_t nb) {
+    uint32_t r = (uint32_t)(((unsigned long long)id * nb) / idmax);
+    return (uint32_t)side * nb + (r < nb ? r : nb - 1);
+}
+
+__device__ __forceinline__ uint32_t steady_idmax(const SpCounters *c) {
+    const uint32_t m = c->next_id;
+    return m ? m : 1u;
+}
+
+__global__ void __launch_bounds__(256) k_steady_hist(SteadyArgs a, int N, int axis) {
+    const int s = blockIdx.y;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    int f = 0;
-    uint32_t c = 0;
-    if (i < n) {
-        c = sp.cell[i];
-        const int jx = cell_jx(c);
-        if (jx != CELL_HOLE && jx != CELL_DEAD) f = steady_sides(jx, cell_jy(c), N);
+    if (i >= *a.n[s]) return;
+    const int side = steady_side(a.sp[s].cell[i], N, axis);
+    if (side < 0) return;
+    const SteadyBufs &b = a.b[s];
+    atomicAdd(&b.hist[steady_bucket(side, a.sp[s].id[i], steady_idmax(a.cnt[s]), b.nb)], 1u);
+}
+
+// exclusive scan of a species' 2 nb bucket counts into start[0..2 nb] (and
+// the scatter cursors); the counts are cleared for the next pass
+__global__ void __launch_bounds__(1024) k_steady_scan(SteadyArgs a) {
+    const SteadyBufs &b = a.b[blockIdx.x];
+    const uint32_t nt = 2 * b.nb;
+    __shared__ uint32_t carry_s;
+    uint32_t carry = 0;
+    for (uint32_t r0 = 0; r0 < nt; r0 += blockDim.x) {
+        const uint32_t r = r0 + threadIdx.x;
+        const uint32_t v = r < nt ? b.hist[r] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan_1024(v, &tot);
+        if (r < nt) {
+            b.start[r] = carry + ex;
+            b.cursor[r] = carry + ex;
+            b.hist[r] = 0u;
+        }
+        if (threadIdx.x == 0) carry_s = carry + tot;
+        __syncthreads();
+        carry = carry_s;
+        __syncthreads();
     }
-    const int lane = threadIdx.x & 63;
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int axis = 0; axis < 2; axis++) {
-        const int side = axis == 0 ? ((f & 1) ? 0 : ((f & 2) ? 1 : -1))
-                                   : ((f & 4) ? 2 : ((f & 8) ? 3 : -1));
-        const bool has = side >= 0;
-        const uint64_t bal = __ballot(has);
-        if (!bal) continue;
-        const int leader = __ffsll((unsigned long long)bal) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(bal));
-        base = __shfl(base, leader, 64);
-        if (has) {
-            uint32_t slot = base + (uint32_t)__popcll(bal & below);
-            if (slot < sv.cap) {
-                sv.key[slot] = ((unsigned long long)side << 32) | sp.id[i];
-                sv.idx[slot] = i;
-            } else {
-                atomicAdd(overflow, 1u);
+    if (threadIdx.x == 0) b.start[nt] = carry;
+}
+
+__global__ void __launch_bounds__(256) k_steady_scatter(SteadyArgs a, int N, int axis) {
+    const int s = blockIdx.y;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *a.n[s]) return;
+    const int side = steady_side(a.sp[s].cell[i], N, axis);
+    if (side < 0) return;
+    const SteadyBufs &b = a.b[s];
+    const uint32_t id = a.sp[s].id[i];
+    const uint32_t pos = atomicAdd(&b.cursor[steady_bucket(side, id, steady_idmax(a.cnt[s]), b.nb)], 1u);
+    b.id_in[pos] = id;
+    b.idx_in[pos] = i;
+}
+
+// one workgroup per bucket: its entries in id order (ids are unique within a
+// species); up to STEADY_LDS entries by a bitonic network in LDS, larger
+// buckets by counting each entry's rank
+#define STEADY_LDS 2048
+__global__ void __launch_bounds__(256) k_steady_sort(SteadyArgs a) {
+    const SteadyBufs &b = a.b[blockIdx.y];
+    const uint32_t lo = b.start[blockIdx.x], k = b.start[blockIdx.x + 1] - lo;
+    if (k == 0) return;
+    if (k > STEADY_LDS) {
+        for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) {
+            const uint32_t v = b.id_in[lo + j];
+            uint32_t r = 0;
+            for (uint32_t t = 0; t < k; t++) r += b.id_in[lo + t] < v ? 1u : 0u;
+            b.id_out[lo + r] = v;
+            b.idx_out[lo + r] = b.idx_in[lo + j];
+        }
+        return;
+    }
+    __shared__ uint32_t sk[STEADY_LDS], sv[STEADY_LDS];
+    uint32_t P = 1;
+    while (P < k) P <<= 1;
+    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+        sk[t] = t < k ? b.id_in[lo + t] : ~0u;
+        sv[t] = t < k ? b.idx_in[lo + t] : 0u;
+    }
+    __syncthreads();
+    for (uint32_t m = 2; m <= P; m <<= 1) {
+        for (uint32_t j = m >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+                const uint32_t o = t ^ j;
+                if (o > t) {
+                    const uint32_t x = sk[t], y = sk[o];
+                    if ((x > y) == ((t & m) == 0)) {
+                        sk[t] = y;
+                        sk[o] = x;
+                        const uint32_t w = sv[t];
+                        sv[t] = sv[o];
+                        sv[o] = w;
+                    }
+                }
             }
+            __syncthreads();
         }
     }
-}
-
-// first position in sorted key[0, c) whose key >= v
-__device__ __forceinline__ uint32_t lower_bound_u64(const unsigned long long *key, uint32_t c,
-                                                    unsigned long long v) {
-    uint32_t lo = 0, hi = c;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (key[mid] < v) lo = mid + 1;
-        else hi = mid;
+    for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) {
+        b.id_out[lo + t] = sk[t];
+        b.idx_out[lo + t] = sv[t];
     }
-    return lo;
-}
-
-// is id among the first m entries of the sorted range [b, b + m) of side s
-__device__ __forceinline__ bool in_prefix(const unsigned long long *key, uint32_t b, uint32_t m,
-                                          int s, uint32_t id) {
-    unsigned long long v = ((unsigned long long)s << 32) | id;
-    uint32_t p = b + lower_bound_u64(key + b, m, v);
-    return p < b + m && key[p] == v;
 }
 
 // move the pair (e[ie], ion[ii]) to one point with fresh velocities and
@@ -1746,91 +1807,29 @@ __device__ __forceinline__ void plop_store(SpView e, SpView ion, uint32_t ie, ui
         dep_global(qi, N, pi.jx, pi.jy, fx_weights_w(pi.ox, pi.oy, load_w(ion, ii)));
 }
 
-// Reference steady-state electron wall (steady_e_boundary, boundary.cpp:
-// 890-1097): per side, the m-th electron outside pairs with the m-th ion
-// outside (the reference's scan with a moving ion cursor is an order-
-// preserving match; ion_proportion = 1); each pair is re-injected.  X sides
-// first, then the y sides among the particles the x pass left outside.
-// Order = particle id (the reference's vector order depends on its removal
-// history, which this store does not keep).  One workgroup.
-__global__ void __launch_bounds__(1024)
-    k_steady_pair(SpView e, SpView ion, StageView ce, StageView ci, SpCounters *__restrict__ cnte,
-                  SpCounters *__restrict__ cnti, mcc::MccConst mc, uint32_t step, int N, float se,
-                  float si, unsigned long long *__restrict__ qe, unsigned long long *__restrict__ qi,
+// the pairs of both sides of an axis: thread k < m0 pairs rank k of the low
+// side, m0 <= k < m0 + m1 rank k - m0 of the high side
+__global__ void __launch_bounds__(256)
+    k_steady_pair(SteadyArgs a, mcc::MccConst mc, uint32_t step, int N, float se, float si,
+                  unsigned long long *__restrict__ qe, unsigned long long *__restrict__ qi,
                   int mode) {
-    __shared__ uint32_t rb_e[5], rb_i[5], s_ne, s_ni, chunk_e, chunk_i;
-    if (threadIdx.x == 0) {
-        uint32_t a = cnte->cand_count, b = cnti->cand_count;
-        s_ne = a < ce.cap ? a : ce.cap;
-        s_ni = b < ci.cap ? b : ci.cap;
-    }
-    __syncthreads();
-    const uint32_t ne = s_ne, ni = s_ni;
-    block_sort_pairs(ce.key, ce.idx, ne);
-    block_sort_pairs(ci.key, ci.idx, ni);
-    if (threadIdx.x < 5) {
-        unsigned long long v = (unsigned long long)threadIdx.x << 32;
-        rb_e[threadIdx.x] = lower_bound_u64(ce.key, ne, v);
-        rb_i[threadIdx.x] = lower_bound_u64(ci.key, ni, v);
-    }
-    __syncthreads();
-    // x sides
-    uint32_t mx[2];
-    for (int sd = 0; sd < 2; sd++) {
-        uint32_t a = rb_e[sd + 1] - rb_e[sd], b = rb_i[sd + 1] - rb_i[sd];
-        mx[sd] = a < b ? a : b;
-        for (uint32_t m = threadIdx.x; m < mx[sd]; m += blockDim.x)
-            plop_store(e, ion, ce.idx[rb_e[sd] + m], ci.idx[rb_i[sd] + m], mc, step, N, se, si, qe,
-                       qi, mode);
-    }
-    __syncthreads();
-    // y sides: drop the entries of particles re-injected by the x pass (stable
-    // compaction in place), then pair
-    for (int sd = 2; sd < 4; sd++) {
-        uint32_t ce_n = 0, ci_n = 0;
-        for (int sp = 0; sp < 2; sp++) {
-            unsigned long long *key = sp == 0 ? ce.key : ci.key;
-            uint32_t *val = sp == 0 ? ce.idx : ci.idx;
-            const uint32_t *rb = sp == 0 ? rb_e : rb_i;
-            uint32_t kept_before = 0;
-            for (uint32_t r0 = rb[sd]; r0 < rb[sd + 1]; r0 += blockDim.x) {
-                uint32_t r = r0 + threadIdx.x;
-                bool live = false;
-                unsigned long long kv = 0;
-                uint32_t vv = 0;
-                if (r < rb[sd + 1]) {
-                    kv = key[r];
-                    vv = val[r];
-                    uint32_t id = (uint32_t)kv;
-                    live = !in_prefix(key, rb[0], mx[0], 0, id) &&
-                           !in_prefix(key, rb[1], mx[1], 1, id);
-                }
-                uint32_t tot;
-                uint32_t ex = block_excl_scan_1024(live ? 1u : 0u, &tot);
-                if (threadIdx.x == 0) (sp == 0 ? chunk_e : chunk_i) = tot;
-                __syncthreads();  // every read of this chunk precedes the writes
-                if (live) {
-                    key[rb[sd] + kept_before + ex] = kv;
-                    val[rb[sd] + kept_before + ex] = vv;
-                }
-                kept_before += (sp == 0 ? chunk_e : chunk_i);
-                __syncthreads();
-            }
-            if (sp == 0) ce_n = kept_before;
-            else ci_n = kept_before;
-        }
-        uint32_t m_n = ce_n < ci_n ? ce_n : ci_n;
-        for (uint32_t m = threadIdx.x; m < m_n; m += blockDim.x)
-            plop_store(e, ion, ce.idx[rb_e[sd] + m], ci.idx[rb_i[sd] + m], mc, step, N, se, si, qe,
-                       qi, mode);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        cnte->cand_count = 0;
-        cnti->cand_count = 0;
-    }
+    const SteadyBufs &be = a.b[0], &bi = a.b[1];
+    const uint32_t nb = be.nb;
+    const uint32_t e0 = be.start[nb] - be.start[0], e1 = be.start[2 * nb] - be.start[nb];
+    const uint32_t i0 = bi.start[nb] - bi.start[0], i1 = bi.start[2 * nb] - bi.start[nb];
+    const uint32_t m0 = e0 < i0 ? e0 : i0, m1 = e1 < i1 ? e1 : i1;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m0 + m1) return;
+    const int side = k < m0 ? 0 : 1;
+    const uint32_t r = side == 0 ? k : k - m0;
+    const uint32_t ie = be.idx_out[be.start[side * nb] + r];
+    const uint32_t ii = bi.idx_out[bi.start[side * nb] + r];
+    plop_store(a.sp[0], a.sp[1], ie, ii, mc, step, N, se, si, qe, qi, mode);
 }
 
+// ------------------------------------------------------------ loader
+// Uniform positions in (0,L)^2 and a 3-V Maxwellian, from Philox4x32-10 keyed
+// by the seed with counter (id, species, call, 0).
 __global__ void __launch_bounds__(256)
     k_load_random(SpView sp, uint32_t n, uint32_t key0, uint32_t key1, uint32_t species, int N,
                   double sigma) {
@@ -2053,17 +2052,21 @@ void launch_absorb(hipStream_t st, const SpView &src, const SpView &dst, const u
                        N, mode, lost6, mc, step, spn);
 }
 
-void launch_steady(hipStream_t st, const SpView &e, const uint32_t *ne, int64_t cap_e,
-                   const SpView &ion, const uint32_t *ni, int64_t cap_i, const StageView &ce,
-                   const StageView &ci, SpCounters *cnte, SpCounters *cnti, const mcc::MccConst &mc,
-                   uint32_t step, int N, float se, float si, unsigned long long *qe,
-                   unsigned long long *qi, int mode) {
-    hipLaunchKernelGGL(k_steady_collect, dim3(blocks_for(cap_e, 256)), dim3(256), 0, st, e, ne, N,
-                       ce, &cnte->cand_count, &cnte->overflow);
-    hipLaunchKernelGGL(k_steady_collect, dim3(blocks_for(cap_i, 256)), dim3(256), 0, st, ion, ni,
-                       N, ci, &cnti->cand_count, &cnti->overflow);
-    hipLaunchKernelGGL(k_steady_pair, dim3(1), dim3(1024), 0, st, e, ion, ce, ci, cnte, cnti, mc,
-                       step, N, se, si, qe, qi, mode);
+void launch_steady(hipStream_t st, const SteadyArgs &a, const mcc::MccConst &mc, uint32_t step,
+                   int N, float se, float si, unsigned long long *qe, unsigned long long *qi,
+                   int mode) {
+    const int64_t cap = std::max(a.cap[0], a.cap[1]);
+    const uint32_t pairs = (uint32_t)std::min(a.cap[0], a.cap[1]);
+    for (int axis = 0; axis < 2; axis++) {
+        hipLaunchKernelGGL(k_steady_hist, dim3(blocks_for(cap, 256), 2), dim3(256), 0, st, a, N,
+                           axis);
+        hipLaunchKernelGGL(k_steady_scan, dim3(2), dim3(1024), 0, st, a);
+        hipLaunchKernelGGL(k_steady_scatter, dim3(blocks_for(cap, 256), 2), dim3(256), 0, st, a, N,
+                           axis);
+        hipLaunchKernelGGL(k_steady_sort, dim3(2 * a.b[0].nb, 2), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(k_steady_pair, dim3(blocks_for(pairs, 256)), dim3(256), 0, st, a, mc,
+                           step, N, se, si, qe, qi, mode);
+    }
 }
 
 void launch_load_random(hipStream_t st, const SpView &sp, uint32_t n, uint64_t seed,

@@ -41,6 +41,7 @@ struct Species {
     // candidate queue and its segment counters (collisions run ahead)
     uint32_t *mq = nullptr, *mq_cnt = nullptr;
     float2 *mc_bins = nullptr;  // energy-bin null-collision bounds (mc.bins)
+    pic::SteadyBufs steady = {};  // the reference's steady walls (REF mode)
 };
 
 struct ProfPair {

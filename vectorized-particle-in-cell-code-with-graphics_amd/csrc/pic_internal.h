@@ -33,7 +33,7 @@ struct SpCounters {
     uint32_t next_id;        // id of the next particle created
     uint32_t overflow;       // particles dropped for lack of capacity
     uint32_t stage_count;    // entries in the staging buffer
-    uint32_t cand_count;     // steady-wall candidates in the staging buffer
+    uint32_t pad_;
     // left, right, bottom, top, removed by collision, -
     uint32_t lost[CNT_SLOTS][6];
     uint32_t holes[CNT_SLOTS];  // empty slots left by in-place steps (current buffer)
@@ -51,6 +51,23 @@ __device__ __forceinline__ uint32_t cnt_slot() { return blockIdx.x & (CNT_SLOTS 
 // wait in a staging buffer; a commit pass sorts them by their creator's key
 // (so the layout and the new ids are deterministic), appends them to a
 // particle buffer and optionally deposits their charge.
+// Candidates of the reference's steady walls for one species and one axis:
+// bucket counts, bucket starts (2 nb + 1) and scatter cursors over nb id
+// ranges per side; the candidates (id, index into the store) in bucket
+// order, then sorted by id within each bucket.  cap = the species' store.
+struct SteadyBufs {
+    uint32_t *hist, *start, *cursor;
+    uint32_t *id_in, *idx_in, *id_out, *idx_out;
+    uint32_t nb;
+};
+struct SteadyArgs {
+    SpView sp[2];
+    const uint32_t *n[2];
+    const SpCounters *cnt[2];
+    SteadyBufs b[2];
+    int64_t cap[2];
+};
+
 struct StageView {
     uint32_t *cell;
     float *ox, *oy, *vx, *vy, *vz;
@@ -191,11 +208,10 @@ void launch_deposit(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t
 void launch_absorb(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
                    uint32_t *nout, int64_t cap, int N, int mode, uint32_t *lost6,
                    const mcc::MccConst &mc, uint32_t step, const Spawn &spn);
-void launch_steady(hipStream_t st, const SpView &e, const uint32_t *ne, int64_t cap_e,
-                   const SpView &ion, const uint32_t *ni, int64_t cap_i, const StageView &ce,
-                   const StageView &ci, SpCounters *cnte, SpCounters *cnti, const mcc::MccConst &mc,
-                   uint32_t step, int N, float se, float si, unsigned long long *qe,
-                   unsigned long long *qi, int mode);
+// reference steady-state walls: both axes' pairing passes (k_particles.hip)
+void launch_steady(hipStream_t st, const SteadyArgs &a, const mcc::MccConst &mc, uint32_t step,
+                   int N, float se, float si, unsigned long long *qe, unsigned long long *qi,
+                   int mode);
 void launch_load_random(hipStream_t st, const SpView &sp, uint32_t n, uint64_t seed,
                         uint32_t species, int N, double sigma);
 
