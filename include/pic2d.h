@@ -198,6 +198,22 @@ int pic_advance(pic_handle *h, int32_t nsteps);
  * precision = significant digits (6 = the reference's ostream default). */
 int pic_dump_data(pic_handle *h, const char *dir, int32_t precision);
 
+/* Exact-resume checkpoint (no reference counterpart: the reference restarts
+ * from the dataDump text above, which rounds every value).  The file holds
+ * the parameters, the step, both species' stores as they lie, their
+ * counters and charge, rho, phi, E, the tolerance solve's iterate and
+ * previous solution, the diagnostics' sums and rows, and a checksum; a run
+ * continued from it takes exactly the steps the writer would have taken.
+ * pic_checkpoint_params reads the parameters so that the caller can create
+ * the handle (pic_create, restart 0) that pic_checkpoint_read fills; that
+ * handle must have the writer's parameters apart from run_time,
+ * write_graphs and graphs (PIC_ERR_ARG otherwise).  PIC_ERR_FORMAT: not a
+ * checkpoint, truncated or corrupt; after a failed read the handle's state
+ * is undefined (destroy it). */
+int pic_checkpoint_write(pic_handle *h, const char *path);
+int pic_checkpoint_params(const char *path, pic_params *out);
+int pic_checkpoint_read(pic_handle *h, const char *path);
+
 /* ------------------------------------------------ per-phase entry points */
 /* particle::get_particle_electric_field (particle.cpp:870-1085, pic.cl:56-92) */
 int pic_species_gather(pic_handle *h, int32_t s);

@@ -226,3 +226,18 @@ def test_server_api_host_logic():
         srv.cancel(b)
         st = srv.wait(b, timeout=120)
         assert st["state"] in (pic2d.REQ_CANCELLED, pic2d.REQ_DONE, pic2d.REQ_FAILED)
+
+
+def test_checkpoint_params_rejects_other_files(tmp_path):
+    """pic_checkpoint_params reads only the header (no device call): a text
+    file, an empty file and a truncated header are PIC_ERR_FORMAT, a missing
+    file PIC_ERR_FILE"""
+    bad = tmp_path / "x.ckpt"
+    for data in (b"", b"0.1 0.2 0.3 0.4 0.5 1.0\n" * 10, b"PIC2DCK1" + b"\0" * 16):
+        bad.write_bytes(data)
+        with pytest.raises(pic2d.PicError) as e:
+            pic2d.checkpoint_params(bad)
+        assert e.value.code == pic2d.PIC_ERR_FORMAT
+    with pytest.raises(pic2d.PicError) as e:
+        pic2d.checkpoint_params(tmp_path / "missing.ckpt")
+    assert e.value.code == pic2d.PIC_ERR_FILE

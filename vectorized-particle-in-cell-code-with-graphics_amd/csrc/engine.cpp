@@ -2124,3 +2124,289 @@ int pic_profile_read(pic_handle *h, double *out, int32_t n) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ checkpoint
+// Exact-resume file (no counterpart in the reference, whose restart reads the
+// lossy dataDump text, particle.cpp:359-401): the run's parameters, its step
+// and host-side schedule state, then every array the next step reads -- both
+// species' stores as they lie (empty slots included, so the layout and the
+// sort schedule continue unchanged), their counters and charge, rho, phi, E,
+// the tolerance solve's iterate and previous solution, the diagnostics'
+// window sums and timeData rows and the accuracy-report rows.  Little-endian;
+// an FNV-1a 64 checksum of all preceding bytes ends the file.
+namespace {
+
+const char CK_MAGIC[8] = {'P', 'I', 'C', '2', 'D', 'C', 'K', '1'};
+constexpr uint32_t CK_VERSION = 1;
+
+struct CkSpecies {
+    int64_t n;          // particles in the store (empty slots included)
+    int32_t cur;        // which of the two buffers holds them
+    int32_t since_sort, holes, weighted;
+    double weight, scale;
+    int64_t injected;
+};
+
+struct CkHeader {
+    char magic[8];
+    uint32_t version, params_bytes, species_bytes, nblobs;
+    int64_t step, mg_cycles_total, diag_graph_steps;
+    uint64_t ovf_reported;
+    double mg_relres, last_phiR, last_bc[4], acc_tol;
+    int32_t mg_cycles, have_rho, have_phi, have_prev, pred, diag_rows, diag_rows_cap, acc_every;
+    int64_t acc_rows;   // doubles in the accuracy-report rows
+    CkSpecies sp[2];
+};
+
+struct CkBlob {
+    uint32_t tag;
+    void *ptr;     // device pointer, or host when host is set
+    size_t bytes;
+    bool host;
+};
+
+// the arrays of a checkpoint, in file order: n[s] particles per species,
+// rows_cap timeData rows, acc_rows doubles of accuracy-report rows at acc
+std::vector<CkBlob> ck_blobs(pic_handle *h, const int64_t n[2], int rows_cap, double *acc,
+                             int64_t acc_rows) {
+    std::vector<CkBlob> b;
+    auto add = [&](void *p, size_t bytes, bool host = false) {
+        b.push_back({(uint32_t)b.size() + 1, p, bytes, host});
+    };
+    const size_t N = (size_t)h->N, nn = N * N, E = (N + 1) * (N + 1);
+    for (int s = 0; s < 2; s++) {
+        Species &sp = h->sp[s];
+        const SpView &v = sp.buf[sp.cur];
+        const size_t m = 4 * (size_t)n[s];
+        for (void *p : {(void *)v.cell, (void *)v.ox, (void *)v.oy, (void *)v.vx, (void *)v.vy,
+                        (void *)v.vz, (void *)v.id})
+            add(p, m);
+        if (sp.weighted) add(v.w, m);
+        add(sp.q, nn * sizeof(unsigned long long));
+    }
+    add(h->cnt, 2 * sizeof(SpCounters));
+    add(h->rho, nn * sizeof(double));
+    add(h->phi, nn * sizeof(double));
+    add(h->ex64, E * sizeof(double));
+    add(h->ey64, E * sizeof(double));
+    add(h->E32, ((N + 1) * N + 1) * sizeof(float2));
+    add(h->M.d.u[h->M.ng - 1], nn * sizeof(double));
+    if (h->M.uprev) add(h->M.uprev, nn * sizeof(double));
+    const DiagDev &D = h->diag;
+    if (D.rho) {
+        for (double *p : {D.rho, D.phi, D.ex, D.ey}) add(p, nn * sizeof(double));
+        for (int s = 0; s < 2; s++) {
+            add(D.q[s], nn * sizeof(double));
+            add(D.vsx[s], N * sizeof(long long));
+            add(D.vsy[s], N * sizeof(long long));
+            add(D.cntx[s], N * sizeof(uint32_t));
+            add(D.cnty[s], N * sizeof(uint32_t));
+            add(D.vpx[s], N * sizeof(double));
+            add(D.vpy[s], N * sizeof(double));
+            add(D.dist[s], 4 * (size_t)h->P.no_bins * sizeof(unsigned long long));
+        }
+    }
+    if (D.tdata) add(D.tdata, (size_t)rows_cap * DIAG_TCOLS * sizeof(double));
+    if (acc_rows > 0) add(acc, (size_t)acc_rows * sizeof(double), true);
+    return b;
+}
+
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    void add(const void *p, size_t n) {
+        const unsigned char *c = static_cast<const unsigned char *>(p);
+        for (size_t k = 0; k < n; k++) h = (h ^ c[k]) * 1099511628211ull;
+    }
+};
+
+constexpr size_t CK_CHUNK = (size_t)64 << 20;
+
+// parameters that may differ between the writing and the reading handle:
+// how long pic_run runs, whether it writes graph files, graph replay
+pic_params ck_comparable(const pic_params &p) {
+    pic_params q = p;
+    q.run_time = 0;
+    q.write_graphs = 0;
+    q.graphs = 0;
+    return q;
+}
+
+}  // namespace
+
+int pic_checkpoint_write(pic_handle *h, const char *path) {
+    if (!h || !path) return set_global_error(PIC_ERR_ARG, "null argument");
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    CHECK(check_overflow(h, true));
+    CHECK(read_counters(h));
+    CkHeader H;
+    memset(&H, 0, sizeof(H));
+    memcpy(H.magic, CK_MAGIC, 8);
+    H.version = CK_VERSION;
+    H.params_bytes = sizeof(pic_params);
+    H.species_bytes = sizeof(CkSpecies);
+    H.step = h->step;
+    H.mg_cycles_total = h->mg_cycles_total;
+    H.diag_graph_steps = h->diag_graph_steps;
+    H.ovf_reported = h->ovf_reported;
+    H.mg_relres = h->mg_relres;
+    H.last_phiR = h->last_phiR;
+    for (int k = 0; k < 4; k++) H.last_bc[k] = h->last_bc[k];
+    H.acc_tol = h->acc_tol;
+    H.mg_cycles = h->mg_cycles;
+    H.have_rho = h->have_rho;
+    H.have_phi = h->have_phi;
+    H.have_prev = h->M.have_prev;
+    H.pred = h->M.pred;
+    H.diag_rows = h->diag_rows;
+    H.diag_rows_cap = h->diag_rows_cap;
+    H.acc_every = h->acc_every;
+    H.acc_rows = (int64_t)h->acc_rows.size();
+    int64_t n[2];
+    for (int s = 0; s < 2; s++) {
+        const Species &sp = h->sp[s];
+        n[s] = h->cnt_host[s].n[sp.cur];
+        H.sp[s] = CkSpecies{n[s], sp.cur, sp.since_sort, sp.holes ? 1 : 0, sp.weighted ? 1 : 0,
+                            sp.weight, sp.scale, sp.injected};
+    }
+    const std::vector<CkBlob> blobs =
+        ck_blobs(h, n, h->diag_rows_cap, h->acc_rows.data(), H.acc_rows);
+    H.nblobs = (uint32_t)blobs.size();
+    FILE *f = fopen(path, "wb");
+    if (!f) return seterr(h, PIC_ERR_FILE, std::string("cannot write ") + path);
+    Fnv fnv;
+    bool ok = true;
+    auto put = [&](const void *p, size_t bytes) {
+        if (ok && fwrite(p, 1, bytes, f) != bytes) ok = false;
+        fnv.add(p, bytes);
+    };
+    put(&H, sizeof(H));
+    put(&h->P, sizeof(pic_params));
+    std::vector<char> buf;
+    for (const CkBlob &b : blobs) {
+        const uint64_t hdr[2] = {b.tag, b.bytes};
+        put(hdr, sizeof(hdr));
+        for (size_t off = 0; off < b.bytes && ok; off += CK_CHUNK) {
+            const size_t k = std::min(CK_CHUNK, b.bytes - off);
+            if (b.host) {
+                put(static_cast<const char *>(b.ptr) + off, k);
+                continue;
+            }
+            buf.resize(k);
+            if (hipMemcpy(buf.data(), static_cast<const char *>(b.ptr) + off, k,
+                          hipMemcpyDeviceToHost) != hipSuccess) {
+                fclose(f);
+                return seterr(h, PIC_ERR_DEVICE, "checkpoint: device copy failed");
+            }
+            put(buf.data(), k);
+        }
+    }
+    const uint64_t sum = fnv.h;
+    if (ok && fwrite(&sum, 1, sizeof(sum), f) != sizeof(sum)) ok = false;
+    if (fclose(f) != 0) ok = false;
+    if (!ok) return seterr(h, PIC_ERR_FILE, std::string("write failed: ") + path);
+    return PIC_OK;
+}
+
+int pic_checkpoint_params(const char *path, pic_params *out) {
+    if (!path || !out) return set_global_error(PIC_ERR_ARG, "null argument");
+    FILE *f = fopen(path, "rb");
+    if (!f) return set_global_error(PIC_ERR_FILE, std::string("cannot open ") + path);
+    CkHeader H;
+    const bool ok = fread(&H, 1, sizeof(H), f) == sizeof(H) && memcmp(H.magic, CK_MAGIC, 8) == 0 &&
+                    H.version == CK_VERSION && H.params_bytes == sizeof(pic_params) &&
+                    fread(out, 1, sizeof(pic_params), f) == sizeof(pic_params);
+    fclose(f);
+    if (!ok) return set_global_error(PIC_ERR_FORMAT, std::string("not a checkpoint: ") + path);
+    return PIC_OK;
+}
+
+int pic_checkpoint_read(pic_handle *h, const char *path) {
+    if (!h || !path) return set_global_error(PIC_ERR_ARG, "null argument");
+    FILE *f = fopen(path, "rb");
+    if (!f) return seterr(h, PIC_ERR_FILE, std::string("cannot open ") + path);
+    struct Closer {
+        FILE *f;
+        ~Closer() { fclose(f); }
+    } closer{f};
+    Fnv fnv;
+    auto get = [&](void *p, size_t bytes) {
+        if (fread(p, 1, bytes, f) != bytes) return false;
+        fnv.add(p, bytes);
+        return true;
+    };
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    CkHeader H;
+    pic_params P;
+    if (!get(&H, sizeof(H)) || memcmp(H.magic, CK_MAGIC, 8) != 0 || H.version != CK_VERSION ||
+        H.params_bytes != sizeof(pic_params) || H.species_bytes != sizeof(CkSpecies) ||
+        !get(&P, sizeof(P)))
+        return seterr(h, PIC_ERR_FORMAT, std::string("not a checkpoint: ") + path);
+    const pic_params a = ck_comparable(P), b = ck_comparable(h->P);
+    if (memcmp(&a, &b, sizeof(pic_params)) != 0)
+        return seterr(h, PIC_ERR_ARG, "checkpoint: the handle was created with other parameters");
+    int64_t n[2];
+    for (int s = 0; s < 2; s++) {
+        const CkSpecies &c = H.sp[s];
+        if (c.n < 0 || c.n > h->sp[s].cap || c.cur < 0 || c.cur > 1)
+            return seterr(h, PIC_ERR_FORMAT, "checkpoint: particle count beyond the store");
+        n[s] = c.n;
+    }
+    if (H.acc_rows < 0 || H.acc_rows % 6 != 0 || H.acc_rows > ((int64_t)1 << 30) ||
+        H.diag_rows < 0 || H.diag_rows > H.diag_rows_cap)
+        return seterr(h, PIC_ERR_FORMAT, "checkpoint: bad header");
+    // the reading handle takes the writer's schedule, weights and row space
+    for (int s = 0; s < 2; s++) {
+        Species &sp = h->sp[s];
+        if (H.sp[s].weighted) CHECK(enable_weights(h, sp));
+        sp.cur = H.sp[s].cur;
+    }
+    if (diag_time_on(h) && H.diag_rows_cap > h->diag_rows_cap)
+        CHECK(diag_prepare_time(h, (H.diag_rows_cap - 1) * h->P.time_data_freq));
+    std::vector<double> acc((size_t)H.acc_rows);
+    std::vector<CkBlob> blobs = ck_blobs(h, n, H.diag_rows_cap, acc.data(), H.acc_rows);
+    if (H.nblobs != blobs.size())
+        return seterr(h, PIC_ERR_FORMAT, "checkpoint: array list differs from the run's");
+    std::vector<char> buf;
+    for (const CkBlob &bl : blobs) {
+        uint64_t hdr[2];
+        if (!get(hdr, sizeof(hdr)) || hdr[0] != bl.tag || hdr[1] != bl.bytes)
+            return seterr(h, PIC_ERR_FORMAT,
+                          "checkpoint: array " + std::to_string(bl.tag) + " differs from the run's");
+        for (size_t off = 0; off < bl.bytes; off += CK_CHUNK) {
+            const size_t k = std::min(CK_CHUNK, bl.bytes - off);
+            char *dst = static_cast<char *>(bl.ptr) + off;
+            if (!get(bl.host ? dst : (buf.resize(k), buf.data()), k))
+                return seterr(h, PIC_ERR_FORMAT, "checkpoint: truncated");
+            if (!bl.host) HIPCHK(h, hipMemcpy(dst, buf.data(), k, hipMemcpyHostToDevice));
+        }
+    }
+    uint64_t sum = 0;
+    if (fread(&sum, 1, sizeof(sum), f) != sizeof(sum) || sum != fnv.h)
+        return seterr(h, PIC_ERR_FORMAT, "checkpoint: checksum mismatch");
+    h->step = H.step;
+    h->mg_cycles_total = H.mg_cycles_total;
+    h->diag_graph_steps = H.diag_graph_steps;
+    h->ovf_reported = H.ovf_reported;
+    h->mg_relres = H.mg_relres;
+    h->last_phiR = H.last_phiR;
+    for (int k = 0; k < 4; k++) h->last_bc[k] = H.last_bc[k];
+    h->acc_tol = H.acc_tol;
+    h->mg_cycles = H.mg_cycles;
+    h->have_rho = H.have_rho != 0;
+    h->have_phi = H.have_phi != 0;
+    h->M.have_prev = H.have_prev != 0;
+    h->M.pred = H.pred;
+    h->diag_rows = H.diag_rows;
+    h->acc_every = H.acc_every;
+    h->acc_rows.swap(acc);
+    for (int s = 0; s < 2; s++) {
+        Species &sp = h->sp[s];
+        sp.since_sort = H.sp[s].since_sort;
+        sp.holes = H.sp[s].holes != 0;
+        sp.weight = H.sp[s].weight;
+        sp.scale = H.sp[s].scale;
+        sp.injected = H.sp[s].injected;
+    }
+    CHECK(read_counters(h));
+    return PIC_OK;
+}

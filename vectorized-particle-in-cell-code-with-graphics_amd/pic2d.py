@@ -182,6 +182,7 @@ class ServeStatus(C.Structure):
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = 0, 1, 2, 3, 4
+PIC_ERR_ARG, PIC_ERR_FILE, PIC_ERR_FORMAT = -1, -3, -4
 PIC_ERR_CAPACITY, PIC_ERR_NUMERIC, PIC_ERR_TIMEOUT = -6, -9, -10
 
 
@@ -211,6 +212,9 @@ def load_library(path=LIB_PATH):
         "pic_set_runtime": [vp, i32],
         "pic_run": [vp],
         "pic_dump_data": [vp, C.c_char_p, i32],
+        "pic_checkpoint_write": [vp, C.c_char_p],
+        "pic_checkpoint_params": [C.c_char_p, vp],
+        "pic_checkpoint_read": [vp, C.c_char_p],
         "pic_species_gather": [vp, i32],
         "pic_species_move": [vp, i32],
         "pic_species_collect_charge": [vp, i32],
@@ -322,6 +326,13 @@ def mglin(u, ncycle=2, length=1.0):
 
 
 XS_ELASTIC, XS_EXCITATION, XS_IONIZATION, XS_CX, XS_ION_ISO = 0, 1, 2, 3, 4
+
+
+def checkpoint_params(path) -> Params:
+    """the parameters a checkpoint was written with"""
+    p = Params()
+    _check(load_library().pic_checkpoint_params(str(path).encode(), C.byref(p)))
+    return p
 
 
 def cross_section(gas, channel, energy, mode=MODE_PHYSICAL):
@@ -478,6 +489,23 @@ class Pic:
 
     def dump_data(self, directory=".", precision=17):
         self._c(self.lib.pic_dump_data(self.h, str(directory).encode(), int(precision)))
+
+    def checkpoint(self, path):
+        """exact-resume file: Pic.resume(path) continues bit for bit"""
+        self._c(self.lib.pic_checkpoint_write(self.h, str(path).encode()))
+
+    @classmethod
+    def resume(cls, path, params=None):
+        """a run created from a checkpoint's parameters (or `params`, which
+        may differ in run_time, write_graphs and graphs) and filled from it"""
+        p = checkpoint_params(path) if params is None else params
+        sim = cls(p)
+        try:
+            sim._c(sim.lib.pic_checkpoint_read(sim.h, str(path).encode()))
+        except Exception:
+            sim.close()
+            raise
+        return sim
 
     def sync(self):
         self._c(self.lib.pic_sync(self.h))
