@@ -214,6 +214,15 @@ int pic_checkpoint_write(pic_handle *h, const char *path);
 int pic_checkpoint_params(const char *path, pic_params *out);
 int pic_checkpoint_read(pic_handle *h, const char *path);
 
+/* Launch-shape choices of a live run (no reference counterpart; used by the
+ * autotuner, autotune.py): particle passes per counting sort of electrons
+ * and ions (0: leave) and red-black sweeps per leg of the tolerance solve
+ * (0: leave, else 1..3).  Results do not depend on them.
+ * pic_get_tuning: {sort interval e, sort interval i, sort key, sweeps}. */
+int pic_set_tuning(pic_handle *h, int32_t sort_interval_e, int32_t sort_interval_i,
+                   int32_t mg_sweeps);
+int pic_get_tuning(pic_handle *h, int32_t *out4);
+
 /* ------------------------------------------------ per-phase entry points */
 /* particle::get_particle_electric_field (particle.cpp:870-1085, pic.cl:56-92) */
 int pic_species_gather(pic_handle *h, int32_t s);

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "pic_host.h"
+#include "tuned.h"
 
 using namespace pic;
 using namespace pic_host;
@@ -263,15 +264,16 @@ void setup_species(pic_handle *h, Species &s, bool electron) {
     double coef = 0.5 * s.dt * s.q_over_m;  // particle.cpp:1103
     double omega_dt = s.dt * s.q_over_m * P.magnetic_field;
     // sort interval: explicit, or chosen from the thermal displacement per
-    // step d = sigma dt / dx (cells): about 7 / d passes, 1..16 (measured
-    // best: 8 at d = 0.85, 2 at d = 3.4, 16 for slow ions; the in-place pass
-    // loses little to disorder, its deposit window falls back to a fixed
-    // square around the block's mean cell)
+    // step d = sigma dt / dx (cells): about c / d passes, 1..16, c from the
+    // tuned table (hand-set 7: 8 passes at d = 0.85, 2 at d = 3.4, 16 for
+    // slow ions; the in-place pass loses little to disorder, its deposit
+    // window falls back to a fixed square around the block's mean cell)
     if (P.sort_interval > 0) {
         s.sort_interval = P.sort_interval;
     } else {
+        const double c = tuned_row(h->N).sort_c[electron ? 0 : 1];
         double d = s.sigma_load * s.dt / h->dx;
-        int k = (int)std::lround(7.0 / std::max(d, 0.05));
+        int k = (int)std::lround(c / std::max(d, 0.05));
         s.sort_interval = std::max(1, std::min(16, k));
     }
     pc.cE = (float)coef;
@@ -1059,7 +1061,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     // 8x8-cell tile keys by default: a block's particles then share one to a
     // few keys, so the claims are few and the scattered writes contiguous
     // (measured 1.6x faster than per-cell keys at 1025^2, 100 ppc)
-    h->keyshift = P.sort_key >= 0 ? P.sort_key : 6;
+    h->keyshift = P.sort_key >= 0 ? P.sort_key : tuned_row(h->N).sort_key;
     h->nbins = ((uint32_t)h->tilesX * h->tilesX * 64u) >> h->keyshift;
     h->nbins_pad = scan_pad(h->nbins);
     int rc = PIC_OK;
@@ -1229,7 +1231,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     hipMemsetAsync(h->ey64, 0, 8ull * (N + 1) * (N + 1), h->st);
     hipMemsetAsync(h->E32, 0, 8ull * ((N + 1) * N + 1), h->st);  // pic.cpp:1086: first gather sees E = 0
     if ((rc = alloc_mg(h, h->M, N)) != PIC_OK) return fail(rc);
-    h->M.nu = P.mg_sweeps > 0 ? P.mg_sweeps : 2;
+    h->M.nu = P.mg_sweeps > 0 ? P.mg_sweeps : tuned_row(h->N).mg_sweeps;
     h->M.use_graphs = P.graphs != 0;
 
     for (int s = 0; s < 2; s++) {
@@ -1532,6 +1534,34 @@ int pic_set_step(pic_handle *h, int64_t step) {
     return PIC_OK;
 }
 
+int pic_set_tuning(pic_handle *h, int32_t sort_interval_e, int32_t sort_interval_i,
+                   int32_t mg_sweeps) {
+    if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    const int32_t si[2] = {sort_interval_e, sort_interval_i};
+    for (int s = 0; s < 2; s++)
+        if (si[s] < 0 || si[s] > 64) return seterr(h, PIC_ERR_ARG, "sort interval must be 0..64");
+    if (mg_sweeps != 0 && (mg_sweeps < 1 || mg_sweeps > 3))
+        return seterr(h, PIC_ERR_ARG, "mg_sweeps must be 0 or 1..3");
+    for (int s = 0; s < 2; s++)
+        if (si[s] > 0) h->sp[s].sort_interval = si[s];
+    if (mg_sweeps > 0 && mg_sweeps != h->M.nu) {
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        for (MgLevels::CycleGraph &g : h->M.graphs) hipGraphExecDestroy(g.exec);
+        h->M.graphs.clear();  // captured with the old sweep count
+        h->M.nu = mg_sweeps;
+    }
+    return PIC_OK;
+}
+
+int pic_get_tuning(pic_handle *h, int32_t *out4) {
+    if (!h || !out4) return set_global_error(PIC_ERR_ARG, "null argument");
+    out4[0] = h->sp[0].sort_interval;
+    out4[1] = h->sp[1].sort_interval;
+    out4[2] = h->keyshift;
+    out4[3] = h->M.nu;
+    return PIC_OK;
+}
+
 int pic_get_collision_counts(pic_handle *h, int32_t s, int64_t *out8) {
     CHECK(check_species(h, s));
     if (!out8) return seterr(h, PIC_ERR_ARG, "null output");
@@ -1819,6 +1849,8 @@ int pic_estimate_bytes(const pic_params *pp, int64_t *bytes) {
         b += sc * (36 + (weighted ? 4 : 0));
         if (wall_on) b += std::min<int64_t>(std::max<int64_t>(65536, n0 / 8), (int64_t)1 << 26) * 36;
         if (P.reduce) b += (int64_t)sizeof(ReduceState);
+        // the reference steady walls' candidate buffers
+        if (P.wall == PIC_WALL_STEADY && P.mode == PIC_MODE_REF) b += cap * 16 + (1 << 20);
         if (s == 1 && P.have_collisions && P.mode == PIC_MODE_REF) b += cap * 40 + N * 4;
         // the physical-mode steady walls' queue of lost ions
         if (s == 1 && !wall_on && P.wall == PIC_WALL_STEADY && P.mode == PIC_MODE_PHYSICAL)
@@ -1830,7 +1862,7 @@ int pic_estimate_bytes(const pic_params *pp, int64_t *bytes) {
     }
     const int64_t tiles = (N + 1 + 7) / 8;
     {  // sort histogram (one counter per 128-byte line) and starts
-        const int ks = P.sort_key >= 0 ? P.sort_key : 6;
+        const int ks = P.sort_key >= 0 ? P.sort_key : tuned_row(N).sort_key;
         const int64_t nb = scan_pad((uint32_t)((tiles * tiles * 64) >> ks));
         b += nb * BIN_STRIDE * 4 + (nb + 1) * 4 + (nb / 4096 + 1) * 4;
     }

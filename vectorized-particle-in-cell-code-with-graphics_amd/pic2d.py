@@ -215,6 +215,8 @@ def load_library(path=LIB_PATH):
         "pic_checkpoint_write": [vp, C.c_char_p],
         "pic_checkpoint_params": [C.c_char_p, vp],
         "pic_checkpoint_read": [vp, C.c_char_p],
+        "pic_set_tuning": [vp, i32, i32, i32],
+        "pic_get_tuning": [vp, vp],
         "pic_species_gather": [vp, i32],
         "pic_species_move": [vp, i32],
         "pic_species_collect_charge": [vp, i32],
@@ -493,6 +495,18 @@ class Pic:
     def checkpoint(self, path):
         """exact-resume file: Pic.resume(path) continues bit for bit"""
         self._c(self.lib.pic_checkpoint_write(self.h, str(path).encode()))
+
+    def set_tuning(self, sort_interval_e=0, sort_interval_i=0, mg_sweeps=0):
+        """passes per counting sort of each species, tolerance-solve sweeps
+        per leg (0: leave as is); results do not depend on them"""
+        self._c(self.lib.pic_set_tuning(self.h, int(sort_interval_e), int(sort_interval_i),
+                                        int(mg_sweeps)))
+
+    def tuning(self):
+        """{sort_interval_e, sort_interval_i, sort_key, mg_sweeps} in effect"""
+        a = (C.c_int32 * 4)()
+        self._c(self.lib.pic_get_tuning(self.h, a))
+        return dict(zip(("sort_interval_e", "sort_interval_i", "sort_key", "mg_sweeps"), a))
 
     @classmethod
     def resume(cls, path, params=None):
