@@ -61,8 +61,8 @@ def make_params(name):
     return table[name]()
 
 
-def describe(name, p):
-    nu = p.mg_sweeps if p.mg_sweeps > 0 else 2  # the library's default
+def describe(name, p, tuning):
+    nu = tuning["mg_sweeps"]
     return {
         "workload": name,
         "nodes": int(p.nodes),
@@ -75,6 +75,8 @@ def describe(name, p):
         "multigrid": ("fp64 FMG ncycle=%d" % p.mg_ncycle) if p.mg_mode == pic2d.MG_FIXED
         else ("fp32 red-black V(%d,%d) cycles + fp64 refinement, tol %g"
               % (nu, nu, p.mg_tol)),
+        # launch shapes in effect (explicit, or the library's tuned table)
+        "tuning": tuning,
     }
 
 
@@ -318,6 +320,7 @@ def main():
         stride = max(1, -(-p.init_particle_no // CPU_SAMPLE))
         e0 = {k: v[::stride].copy() for k, v in sim.get_state(0).items()}
         i0 = {k: v[::stride].copy() for k, v in sim.get_state(1).items()}
+    tuning = sim.tuning()
     sim.run(0)  # pre-loop wall pass of pic_run
     log("warmup", args.warmup, "steps")
     sim.advance(args.warmup)
@@ -406,7 +409,7 @@ def main():
         "kernels": kern,
         "count_kernel_ms": prof[4] / k3n,
         "scan_ms": prof[5] / k3n,
-        "sort_interval": int(p.sort_interval),
+        "sort_interval": [tuning["sort_interval_e"], tuning["sort_interval_i"]],
         "field_solve_ms_per_step": prof[3] / args.profile_steps,
         "mg_cycles_per_step": (mg1[1] - mg0[1]) / args.profile_steps
         if p.mg_mode == pic2d.MG_TOL else p.mg_ncycle,
@@ -431,7 +434,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (Philox-seeded uniform positions, Maxwellian velocities)",
-        "config": describe(args.workload, p),
+        "config": describe(args.workload, p, tuning),
         "latency_ms": {"p50": float(np.percentile(steps_ms, 50)) if steps_ms.size else None,
                        "p99": float(np.percentile(steps_ms, 99)) if steps_ms.size else None},
         "roofline": roof,
