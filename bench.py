@@ -139,6 +139,36 @@ def calibration_copy(nbytes):
     log(f"calibration copy: 3 x {nbytes >> 20} MiB read + {nbytes >> 20} MiB written")
 
 
+def copy_ceiling(nbytes=1 << 30, reps=5):
+    """Device-to-device copy bandwidth through the HIP runtime's blit kernel
+    (read + written bytes / time, HIP events): the measured ceiling beside
+    the nominal 8 TB/s peak."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(ev0))
+    hip.hipEventCreate(C.byref(ev1))
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # warm: hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipEventRecord(ev0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None)
+    hip.hipEventRecord(ev1, None)
+    hip.hipEventSynchronize(ev1)
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+    hip.hipEventDestroy(ev0)
+    hip.hipEventDestroy(ev1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+
 def live_count(sims):
     return sum(s.particle_count(0) + s.particle_count(1) for s in sims)
 
@@ -322,6 +352,12 @@ def main():
         i0 = {k: v[::stride].copy() for k, v in sim.get_state(1).items()}
     tuning = sim.tuning()
     sim.run(0)  # pre-loop wall pass of pic_run
+    # the first step from the freshly loaded (unsorted) state, timed alone
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.advance(1)
+    sim.sync()
+    cold_ms = 1e3 * (time.perf_counter() - t0)
     log("warmup", args.warmup, "steps")
     sim.advance(args.warmup)
     sim.sync()
@@ -414,13 +450,28 @@ def main():
         "mg_cycles_per_step": (mg1[1] - mg0[1]) / args.profile_steps
         if p.mg_mode == pic2d.MG_TOL else p.mg_ncycle,
         "mg_relres_last": mg1[2],
-        # SURVEY-style push+gather+deposit roofline: 48 B/particle + 16 B/node
-        # per species pass over the whole push phase (sort, collisions and
+        # push + gather + deposit roofline: 48 B/particle + 16 B/node per
+        # species pass over the whole push phase (sort, collisions and
         # commits included)
         "push_phase_frac_of_hbm_peak": round(
             (48.0 * 2 * npart_per_launch + 16.0 * 2 * p.nodes ** 2) /
             (push_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
+    # multigrid roofline: about 50 B per node per level visit of a V-cycle
+    # (smoothing, residual, restriction, prolongation; 4/3 N^2 nodes over the
+    # levels) and 40 B per finest node per FP64 refinement pass; the fixed
+    # FMG visits (4/3)^2 N^2 nodes ncycle times.  The field phase's time
+    # (rho, right-hand side, solve, E) is the denominator: latency-bound.
+    nn = float(p.nodes) ** 2
+    if p.mg_mode == pic2d.MG_TOL:
+        mg_bytes = phase["mg_cycles_per_step"] * (50.0 * nn * 4.0 / 3.0 + 40.0 * nn)
+    else:
+        mg_bytes = 50.0 * p.mg_ncycle * (4.0 / 3.0) ** 2 * nn
+    field_ms = phase["field_solve_ms_per_step"]
+    phase["mg_alg_bytes_per_step"] = mg_bytes
+    phase["mg_frac_of_hbm_peak"] = (round(mg_bytes / (field_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                    if field_ms > 0 else None)
+    phase["cold_first_step_ms"] = cold_ms
     out = {
         "metric": "particle-steps/s",
         "value": value,
@@ -443,6 +494,13 @@ def main():
         "cpu_baseline": None,
     }
     roof["phase_frac"] = phase["push_phase_frac_of_hbm_peak"]
+    try:
+        ceil = copy_ceiling()
+        roof["copy_ceiling_GBs"] = round(ceil, 1)
+        roof["frac_of_copy_ceiling"] = round(achieved / ceil, 4)
+    except Exception as e:  # the ceiling is a reference figure, not the metric
+        log("copy ceiling not measured:", e)
+        roof["copy_ceiling_GBs"] = None
     sim.close()
 
     if not args.no_cpu_baseline:
