@@ -322,11 +322,14 @@ def test_sort_interval_does_not_change_collisional_results():
     assert out[1]["ev0"][3] > 0 and out[1]["li"][:4].sum() > 0
 
 
-def test_reference_ion_charge_exchange_matches_the_oracle():
+@pytest.mark.parametrize("skew", [False, True])
+def test_reference_ion_charge_exchange_matches_the_oracle(skew):
     """Reference mode: the per-column accumulator (ion_collisions.cpp:297-303)
     in (column, id) order, persisting across calls.  The device and host
     expf may differ in the last bit, which can move a column's crossing of 1
-    by one ion; the fired sets agree up to such shifts (a handful)."""
+    by one ion; the fired sets agree up to such shifts (a handful).  With
+    `skew`, 6000 ions share one column and one ion's id is 2^31, so the
+    column's ions all fall into one id bucket, ordered by counting ranks."""
     p = collisional(ppc=40, mode=pic2d.MODE_REF)
     p.sigma_cx = pic2d.SIGMA_ENERGY
     p.vtn = math.sqrt(1.38066e-23 * 300.0 / MI)
@@ -342,6 +345,9 @@ def test_reference_ion_charge_exchange_matches_the_oracle():
               vy=(rng.standard_normal(n) * 3000).astype(np.float32),
               vz=(rng.standard_normal(n) * 3000).astype(np.float32),
               id=rng.permutation(n).astype(np.uint32))
+    if skew:
+        st["cell"][:6000] = pic2d.cell_pack(np.full(6000, 3), rng.integers(0, N - 1, 6000))
+        st["id"][st["id"] == n - 1] = np.uint32(1 << 31)
     c = orc.refcx_params(p)
     table = orc.neutral_table(p.seed, p.vtn)
     pcol = np.zeros(N, np.float32)

@@ -493,9 +493,8 @@ bool ref_steady(pic_handle *h) {
 // ion_collisions::collision in reference mode (ion_collisions.cpp:255-308)
 int ref_ion_cx(pic_handle *h) {
     Species &s = h->sp[1];
-    if (launch_refcx(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], (uint32_t)s.cap, h->N, s.mc,
-                     (uint32_t)h->step, h->cx, &h->cnt[1].events[0][0]) != 0)
-        return seterr(h, PIC_ERR_DEVICE, "reference charge exchange: radix sort failed");
+    launch_refcx(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], (uint32_t)s.cap, h->N, s.mc,
+                 (uint32_t)h->step, h->cx, &h->cnt[1], &h->cnt[1].events[0][0]);
     return PIC_OK;
 }
 
@@ -1190,11 +1189,16 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         RefCxBuffers &B = h->cx;
         const size_t cap = (size_t)h->sp[1].cap;
         for (int k = 0; k < 2; k++) {
-            if ((rc = dalloc(h, &B.key[k], cap)) != PIC_OK) return fail(rc);
+            if ((rc = dalloc(h, &B.id[k], cap)) != PIC_OK) return fail(rc);
             if ((rc = dalloc(h, &B.val[k], cap)) != PIC_OK) return fail(rc);
         }
-        B.temp_bytes = refcx_temp_bytes((uint32_t)cap);
-        if ((rc = dalloc(h, (char **)&B.temp, B.temp_bytes)) != PIC_OK) return fail(rc);
+        B.nb = refcx_ranges(h->N);
+        const size_t nt = (size_t)h->N * B.nb;
+        if ((rc = dalloc(h, &B.hist, nt)) != PIC_OK || (rc = dalloc(h, &B.start, nt + 1)) != PIC_OK ||
+            (rc = dalloc(h, &B.cursor, nt)) != PIC_OK)
+            return fail(rc);
+        if (hipMemset(B.hist, 0, sizeof(uint32_t) * nt) != hipSuccess)
+            return fail(set_global_error(PIC_ERR_DEVICE, "memset failed"));
         if ((rc = dalloc(h, &B.pcol, (size_t)h->N)) != PIC_OK) return fail(rc);
         if (hipMemset(B.pcol, 0, sizeof(float) * h->N) != hipSuccess)
             return fail(set_global_error(PIC_ERR_DEVICE, "memset failed"));
@@ -1290,7 +1294,8 @@ int pic_destroy(pic_handle *h) {
     if (h->ovf_host) hipHostFree(h->ovf_host);
     if (h->ovf_ev) hipEventDestroy(h->ovf_ev);
     diag_free(h);
-    void *cx[] = {h->cx.key[0], h->cx.key[1], h->cx.val[0], h->cx.val[1], h->cx.temp, h->cx.pcol};
+    void *cx[] = {h->cx.id[0], h->cx.id[1], h->cx.val[0], h->cx.val[1], h->cx.hist,
+                  h->cx.start, h->cx.cursor, h->cx.pcol};
     for (void *p : cx)
         if (p) hipFree(p);
     free_mg(h->M);
@@ -1851,7 +1856,8 @@ int pic_estimate_bytes(const pic_params *pp, int64_t *bytes) {
         if (P.reduce) b += (int64_t)sizeof(ReduceState);
         // the reference steady walls' candidate buffers
         if (P.wall == PIC_WALL_STEADY && P.mode == PIC_MODE_REF) b += cap * 16 + (1 << 20);
-        if (s == 1 && P.have_collisions && P.mode == PIC_MODE_REF) b += cap * 40 + N * 4;
+        if (s == 1 && P.have_collisions && P.mode == PIC_MODE_REF)
+            b += cap * 24 + 12 * N * (int64_t)refcx_ranges((int)N) + N * 4;
         // the physical-mode steady walls' queue of lost ions
         if (s == 1 && !wall_on && P.wall == PIC_WALL_STEADY && P.mode == PIC_MODE_PHYSICAL)
             b += std::min<int64_t>(std::max<int64_t>(65536, n0 / 8), (int64_t)1 << 26) *

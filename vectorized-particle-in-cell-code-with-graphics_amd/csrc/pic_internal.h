@@ -261,16 +261,18 @@ void launch_wall(hipStream_t st, int species, const StageView &wq, SpCounters *c
 
 // ---- k_refcx.hip: the reference's per-column ion charge exchange
 struct RefCxBuffers {
-    unsigned long long *key[2] = {};  // sort keys (jp << 32 | id), in / out
+    uint32_t *id[2] = {};             // eligible ions' ids: bucket order, then sorted
     unsigned long long *val[2] = {};  // probability bits << 32 | store index
-    void *temp = nullptr;             // radix-sort scratch
-    size_t temp_bytes = 0;
+    uint32_t *hist = nullptr;         // N nb bucket counts (zero between steps)
+    uint32_t *start = nullptr;        // N nb + 1 bucket starts
+    uint32_t *cursor = nullptr;       // N nb scatter cursors
+    uint32_t nb = 1;                  // id ranges per column (refcx_ranges)
     float *pcol = nullptr;            // probabilityCX[N], kept across steps
 };
-size_t refcx_temp_bytes(uint32_t cap);
-int launch_refcx(hipStream_t st, const SpView &sp, const uint32_t *n, uint32_t cap, int N,
-                 const mcc::MccConst &mc, uint32_t step, RefCxBuffers &B,
-                 unsigned long long *events);
+uint32_t refcx_ranges(int N);
+void launch_refcx(hipStream_t st, const SpView &sp, const uint32_t *n, uint32_t cap, int N,
+                  const mcc::MccConst &mc, uint32_t step, RefCxBuffers &B, const SpCounters *cnt,
+                  unsigned long long *events);
 
 // ---- k_diag.hip: batch-mode diagnostics
 #define DIAG_TCOLS 8  // step, n_e, n_i, phi(P1), phi(P2), phi(P3), q_e(P3), q_i(P3)
