@@ -982,32 +982,34 @@ __device__ __forceinline__ double block_sum_d(double v) {
     return t;
 }
 
-// End of a residual pass: this block's sums of r^2 and f^2 go to partials[b];
-// the last of the nb blocks to finish (ticket counter flags[2]) sums the
-// partials in block order, sets flags[0] when ||r|| <= tol ||f||, counts an
-// applied cycle in flags[1] when `applied`, and publishes both and the
-// relative residual to the host-mapped copies, then (after a system-scope
-// fence) the count of residual passes of this solve (flags[3]) with the
-// convergence bit MG_PUB_CONV in host_flags[2], on which the host waits.
-__device__ __forceinline__ void tol_finish(double rr, double ff, int b, int nb,
-                                           double *__restrict__ partials, double tol,
-                                           int *__restrict__ flags, bool applied,
-                                           double *__restrict__ relres_host,
-                                           int *__restrict__ host_flags) {
+// End of a residual pass: this block's sums of r^2 and f^2 go to
+// partials[b].  The pass's total and verdict come from k_tol_finish, the
+// next launch on the stream: a last-block ticket inside the pass would need
+// a device-scope fence in every block (an L2 write-back per block on a
+// multi-XCD device), which cost more than the launch.
+__device__ __forceinline__ void tol_partial(double rr, double ff, int b,
+                                            double *__restrict__ partials) {
     rr = block_sum_d(rr);
     ff = block_sum_d(ff);
-    __shared__ bool last;
     if (threadIdx.x == 0) {
         partials[2 * b] = rr;
         partials[2 * b + 1] = ff;
-        __threadfence();
-        last = atomicAdd(&flags[2], 1) == nb - 1;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    rr = 0.0;
-    ff = 0.0;
+}
+
+// Verdict of a residual pass of nb blocks (one workgroup): sums the partials
+// in block order, sets flags[0] when ||r|| <= tol ||f||, counts an applied
+// cycle in flags[1] when `applied`, and publishes both and the relative
+// residual to the host-mapped copies, then (after a system-scope fence) the
+// count of residual passes of this solve (flags[3]) with the convergence bit
+// MG_PUB_CONV in host_flags[2], on which the host waits.  Skipped, as its
+// pass, once the solve has converged.
+__global__ void __launch_bounds__(TOL_THREADS)
+    k_tol_finish(const double *__restrict__ partials, int nb, double tol,
+                 int *__restrict__ flags, int applied, double *__restrict__ relres_host,
+                 int *__restrict__ host_flags) {
+    SKIP_IF_DONE(flags);
+    double rr = 0.0, ff = 0.0;
     for (int k = threadIdx.x; k < nb; k += blockDim.x) {
         rr += partials[2 * k];
         ff += partials[2 * k + 1];
@@ -1018,7 +1020,6 @@ __device__ __forceinline__ void tol_finish(double rr, double ff, int b, int nb,
         const double rel = (ff > 0.0) ? sqrt(rr / ff) : sqrt(rr);
         if (applied) flags[1] += 1;
         if (rel <= tol) flags[0] = 1;
-        flags[2] = 0;
         *relres_host = rel;
         host_flags[0] = flags[0];
         host_flags[1] = flags[1];
@@ -1031,54 +1032,83 @@ __device__ __forceinline__ void tol_finish(double rr, double ff, int b, int nb,
     }
 }
 
+static void tol_finish_launch(hipStream_t st, MgLevels &M, int nb, double tol, bool applied) {
+    hipLaunchKernelGGL(k_tol_finish, dim3(1), dim3(TOL_THREADS), 0, st,
+                       (const double *)M.partials, nb, tol, M.flags, applied ? 1 : 0, M.relres,
+                       M.host_flags);
+}
+
 // One residual pass of the refinement loop, fused: v = u_in + e (FP64
 // accumulation of the FP32 correction; e null: v = u_in) written to u_out
 // (null: not written), r = f - A v in FP32 for the next correction solve,
-// block partial sums of r^2 and f^2; the last block to finish (ticket
-// counter flags[2]) sums the partials in block order, sets flags[0] when
-// ||r|| <= tol ||f||, and counts an applied cycle in flags[1] when e is set.
-// u_in and u_out are different arrays (neighbours read u_in + e).
+// block partial sums of r^2 and f^2 (k_tol_finish, launched next, forms
+// the verdict).  u_in and u_out are different arrays (neighbours read
+// u_in + e).
 __global__ void __launch_bounds__(TOL_THREADS)
     k_tol_update(const double *__restrict__ u_in, const float *__restrict__ e,
                  double *__restrict__ u_out, const double *__restrict__ f, int n, double h2i,
-                 float *__restrict__ r32, double *__restrict__ partials, double tol,
-                 int *__restrict__ flags, double *__restrict__ relres_host,
-                 int *__restrict__ host_flags) {
+                 float *__restrict__ r32, double *__restrict__ partials,
+                 const int *__restrict__ flags) {
     SKIP_IF_DONE(flags);
     double rr = 0.0, ff = 0.0;
-    // whole rows per block (no index division); threads stride along the row
-    for (int i = (int)blockIdx.x; i < n; i += (int)gridDim.x)
-    for (int j = (int)threadIdx.x; j < n; j += (int)blockDim.x) {
-        const long k = (long)i * n + j;
-        float out = 0.0f;
-        double v = u_in[k];
-        if (i > 0 && j > 0 && i < n - 1 && j < n - 1) {
-            if (e) {
-                v += (double)e[k];
-                const double un = u_in[k + n] + (double)e[k + n], us = u_in[k - n] + (double)e[k - n];
-                const double ue = u_in[k + 1] + (double)e[k + 1], uw = u_in[k - 1] + (double)e[k - 1];
-                const double r = -h2i * (un + us + ue + uw - 4.0 * v) + f[k];
-                rr += r * r;
-                out = (float)r;
-            } else {
-                const double r = resid_node(u_in, f, n, i, j, h2i);
-                rr += r * r;
-                out = (float)r;
+    // whole rows per block (no index division); threads stride along the
+    // row, U nodes per thread at a time: the loads of all U are issued
+    // before the first is used, so their latencies overlap (a thread still
+    // visits its nodes in increasing j: the same sums)
+    constexpr int U = 4;
+    for (int i = (int)blockIdx.x; i < n; i += (int)gridDim.x) {
+        const bool irow = i > 0 && i < n - 1;
+        for (int j0 = (int)threadIdx.x; j0 < n; j0 += U * (int)blockDim.x) {
+            double c[U], un[U], us[U], ue[U], uw[U], fv[U];
+#pragma unroll
+            for (int t = 0; t < U; t++) {
+                const int j = j0 + t * (int)blockDim.x;
+                const long k = (long)i * n + j;
+                c[t] = un[t] = us[t] = ue[t] = uw[t] = fv[t] = 0.0;
+                if (j < n) {
+                    c[t] = u_in[k];
+                    if (irow && j > 0 && j < n - 1) {
+                        un[t] = u_in[k + n];
+                        us[t] = u_in[k - n];
+                        ue[t] = u_in[k + 1];
+                        uw[t] = u_in[k - 1];
+                        if (e) {
+                            c[t] += (double)e[k];
+                            un[t] += (double)e[k + n];
+                            us[t] += (double)e[k - n];
+                            ue[t] += (double)e[k + 1];
+                            uw[t] += (double)e[k - 1];
+                        }
+                        fv[t] = f[k];
+                    }
+                }
             }
-            ff += f[k] * f[k];
+#pragma unroll
+            for (int t = 0; t < U; t++) {
+                const int j = j0 + t * (int)blockDim.x;
+                if (j >= n) break;
+                const long k = (long)i * n + j;
+                float out = 0.0f;
+                if (irow && j > 0 && j < n - 1) {
+                    // the stencil of resid_node: north, south, east, west
+                    const double r = -h2i * (un[t] + us[t] + ue[t] + uw[t] - 4.0 * c[t]) + fv[t];
+                    rr += r * r;
+                    out = (float)r;
+                    ff += fv[t] * fv[t];
+                }
+                if (u_out) u_out[k] = c[t];
+                r32[k] = out;
+            }
         }
-        if (u_out) u_out[k] = v;
-        r32[k] = out;
     }
-    tol_finish(rr, ff, (int)blockIdx.x, (int)gridDim.x, partials, tol, flags, e != nullptr,
-               relres_host, host_flags);
+    tol_partial(rr, ff, (int)blockIdx.x, partials);
 }
 
 // The top level's post leg fused with the refinement's residual pass: the
 // halo takes one more ring (2 nu + 1) so the correction e is final on the
 // tile and its first ring; then, as k_tol_update, v = u_in + e in FP64 is
 // written to u_out, r = f - A v (the same expression and association) to
-// r_out as FP32, and the tile's sums of r^2 and f^2 go to tol_finish.  The
+// r_out as FP32, and the tile's sums of r^2 and f^2 go to the partials.  The
 // iterate and the residual are those of k_mg_post + k_tol_update bit for
 // bit; only the block split of the norm's sum differs.  r_out must not be
 // the rhs this leg reads (the halo reads neighbouring tiles' nodes).
@@ -1088,8 +1118,7 @@ __global__ void __launch_bounds__(256)
                      const float *__restrict__ cv, int n, float h2,
                      const double *__restrict__ u_in, double *__restrict__ u_out,
                      const double *__restrict__ f, double h2i, float *__restrict__ r_out,
-                     double *__restrict__ partials, double tol, int *__restrict__ flags,
-                     double *__restrict__ relres_host, int *__restrict__ host_flags) {
+                     double *__restrict__ partials, const int *__restrict__ flags) {
     SKIP_IF_DONE(flags);
     constexpr int H = 2 * NU + 1, W = T + 2 * H;
     __shared__ float su[W * W], sf[W * W];
@@ -1144,8 +1173,7 @@ __global__ void __launch_bounds__(256)
         u_out[g] = v;
         r_out[g] = out;
     }
-    tol_finish(rr, ff, (int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y),
-               partials, tol, flags, true, relres_host, host_flags);
+    tol_partial(rr, ff, (int)(blockIdx.y * gridDim.x + blockIdx.x), partials);
 }
 
 __global__ void k_tol_reset(int *flags) {
@@ -1180,10 +1208,10 @@ static void vcycle_top(hipStream_t st, MgLevels &M, int j, const float *r_in, co
         vcycle_fused(st, M, j - 1, A.cv[j - 1], A.cres[j - 1], length, skip);
 #define POST_UPDATE(NU, TT)                                                                       \
     hipLaunchKernelGGL((k_mg_post_update<NU, TT>), g, b, 0, st, A.us[j], r_in, A.cv[j - 1], nf, h2, \
-                       u_in, u_out, f, h2i, r_out, M.partials, tol, M.flags, M.relres,             \
-                       M.host_flags)
+                       u_in, u_out, f, h2i, r_out, M.partials, M.flags)
     MG_DISPATCH(M.nu, T, POST_UPDATE);
 #undef POST_UPDATE
+    tol_finish_launch(st, M, tiles * tiles, tol, true);
 }
 
 // Waits for the queued cycles' verdict: the device has published (host_flags
@@ -1229,14 +1257,14 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
             vcycle_fused(st, M, lf, e32, r32, length, skip);
             hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st,
                                (const double *)buf[queued & 1], (const float *)e32,
-                               buf[(queued + 1) & 1], f, n, h2i, r32, M.partials, tol, M.flags,
-                               M.relres, M.host_flags);
+                               buf[(queued + 1) & 1], f, n, h2i, r32, M.partials, M.flags);
+            tol_finish_launch(st, M, M.nparts, tol, true);
         } else if (lf == M.lc) {
             coarse_dst_launch(st, M, r32, e32, length, skip);
             hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st,
                                (const double *)buf[queued & 1], (const float *)e32,
-                               buf[(queued + 1) & 1], f, n, h2i, r32, M.partials, tol, M.flags,
-                               M.relres, M.host_flags);
+                               buf[(queued + 1) & 1], f, n, h2i, r32, M.partials, M.flags);
+            tol_finish_launch(st, M, M.nparts, tol, true);
         } else {
             vcycle_top(st, M, lf, rb[queued & 1], buf[queued & 1], buf[(queued + 1) & 1], f, h2i,
                        rb[(queued + 1) & 1], tol, length);
@@ -1248,8 +1276,9 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     M.host_flags[0] = M.host_flags[1] = 0;
     if (!flags_reset) hipLaunchKernelGGL(k_tol_reset, dim3(1), dim3(1), 0, st, M.flags);
     hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st, (const double *)u,
-                       (const float *)nullptr, (double *)nullptr, f, n, h2i, r32, M.partials, tol,
-                       M.flags, M.relres, M.host_flags);
+                       (const float *)nullptr, (double *)nullptr, f, n, h2i, r32, M.partials,
+                       M.flags);
+    tol_finish_launch(st, M, M.nparts, tol, false);
     // k cycles from `queued` on: launched one kernel at a time, or replayed
     // from the hipGraph of that batch (k, parity of queued, iterate buffer),
     // captured on the batch's first use; a capture records the launches of
