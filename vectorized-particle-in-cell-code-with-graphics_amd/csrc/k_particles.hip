@@ -581,12 +581,21 @@ uint32_t scan_pad(uint32_t nbins) { return (nbins + SCAN_TILE - 1) / SCAN_TILE *
 // block's cells exceeds the window (a few fast particles far from the rest),
 // the window is a fixed square around the block's mean cell: the particles
 // inside it go through LDS, the others deposit with global atomics.
-#define DEP_HALF 21  // fallback window: (2 DEP_HALF + 2)^2 = 1936 nodes <= DEP_WCAP
-template <int P>
+// The window holds WCAP nodes (DEP_WCAP by default); the fallback square has
+// edge 2 half + 2 with half = (floor(sqrt(WCAP)) - 2) / 2 (21 for 2048 nodes:
+// 44 x 44 = 1936; 31 for 4096: 64 x 64).
+constexpr int dep_half(int wcap) {
+    int r = 0;
+    while ((r + 1) * (r + 1) <= wcap) r++;
+    return (r - 2) / 2;
+}
+template <int P, int WCAP = DEP_WCAP>
 __device__ __forceinline__ void block_deposit_n(const bool (&dep)[P], const PState (&s)[P],
                                                 const float (&pw)[P],
                                                 unsigned long long *__restrict__ q, int N,
                                                 unsigned long long *sq) {
+    constexpr int DEP_HALF = dep_half(WCAP);
+    static_assert((2 * DEP_HALF + 2) * (2 * DEP_HALF + 2) <= WCAP, "fallback window size");
     __shared__ int red[4][6];
     const int big = 1 << 30;
     int mnx = big, mny = big, mxx = -big, mxy = -big, sx = 0, sy = 0, cnt = 0;
@@ -641,7 +650,7 @@ __device__ __forceinline__ void block_deposit_n(const bool (&dep)[P], const PSta
     if (cnt == 0) return;  // nothing to deposit (uniform: all threads)
     // a depositing particle has jx in [0, N-2]: its nodes jx, jx+1 <= N-1
     int x0 = mnx, y0 = mny, ww = mxx - mnx + 2, wh = mxy - mny + 2;
-    if (ww * wh > DEP_WCAP) {
+    if (ww * wh > WCAP) {
         const int cx = sx / cnt, cy = sy / cnt;
         x0 = cx - DEP_HALF;
         y0 = cy - DEP_HALF;
@@ -649,7 +658,7 @@ __device__ __forceinline__ void block_deposit_n(const bool (&dep)[P], const PSta
     }
     const int area = ww * wh;
     int copies = DEP_COPIES;
-    while (copies > 1 && copies * area > DEP_WCAP) copies >>= 1;
+    while (copies > 1 && copies * area > WCAP) copies >>= 1;
     for (int k = threadIdx.x; k < copies * area; k += blockDim.x) sq[k] = 0ull;
     __syncthreads();
     const int cb = (int)(threadIdx.x & (copies - 1)) * area;
@@ -962,7 +971,10 @@ __global__ void __launch_bounds__(256)
         lidx[r] = claim[r] ? H.cnt[ent[r]] + rank[r] : 0u;
     }
     __syncthreads();  // the table is read; the deposit window reuses the LDS
-    if (deposit) block_deposit_n<IPP>(dep, s, w, q, pc.N, lds);
+    // the whole 32 KB (the table's half too): a 64 x 64 window, as the
+    // particles of a sort pass have moved two steps since their store was
+    // sorted
+    if (deposit) block_deposit_n<IPP, 2 * BLOCK_LDS_U64>(dep, s, w, q, pc.N, lds);
     __syncthreads();
     uint32_t *stg = reinterpret_cast<uint32_t *>(smem);  // 8 words x 256 IPP, SoA
     constexpr uint32_t SN = 256 * IPP;
