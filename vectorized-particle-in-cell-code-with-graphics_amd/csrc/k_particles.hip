@@ -802,30 +802,58 @@ __device__ __forceinline__ void scatter_round(PState &s, uint32_t pid, float w, 
 // distinct key of the block, the deposit through one LDS window, the products
 // staged.  The keys are the cells before the step, so the ranks and the
 // claims come first and the claims' latency overlaps the collisions and the
-// push.  A block of a sorted store ranks its particles in a direct LDS
-// table over its narrow key range; a wider range takes the hash table.  The claimed slots (survivors, and holes for the particles lost in
-// this step) are then staged in LDS in slot order (each key's claimed range
-// contiguous) and written out by consecutive threads, so every store
-// instruction covers consecutive slots.  The 32 KB of LDS serve three
-// phases: collision queue + hash table, deposit window, write staging.
+// push.  A block of a sorted store ranks its particles in a direct LDS table
+// over its narrow key range (up to SCAT_WIN keys); a block with a wider range
+// (an unsorted store, or the particles appended since the last sort) ranks
+// them in an LDS hash table and makes its claims before the collisions.  The
+// slots a direct-table block claimed (survivors, and holes for the particles
+// lost in this step) are staged in LDS in slot order (each key's claimed
+// range contiguous) and written out by consecutive threads, so every store
+// instruction covers consecutive slots; a wide block writes its particles
+// directly.
+//
+// LDS: one arena serves three phases -- the rank table (8 KB) beside the
+// collision queue (11 KB), or the hash table alone; then the deposit window
+// (as many nodes as the arena holds); then the write staging in two halves
+// of 512 particles.  Its size follows the waves per SIMD each variant's
+// registers allow: 25 KB for the collisionless variants (6 blocks per CU),
+// 32 KB for the collisional ones (4-5 blocks per CU).  The pass is bound by
+// memory latency: with 3 instead of 4 blocks per CU it took 20 % longer.
+#ifndef SCAT_WAVES_PLAIN
+#define SCAT_WAVES_PLAIN 6
+#endif
+#ifndef SCAT_ARENA_COLL
+#define SCAT_ARENA_COLL 4096
+#endif
 template <int V>
-__global__ void __launch_bounds__(256)
-    k_push_scatter(SpView src, SpView dst, const uint32_t *__restrict__ nptr, uint32_t cap,
-                   const float2 *__restrict__ E, PushConst pc, mcc::MccConst mc, uint32_t step,
-                   uint32_t *__restrict__ binCount, const uint32_t *__restrict__ binStart,
-                   unsigned long long *__restrict__ q, int do_push, int deposit, Spawn sp,
-                   unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6,
-                   uint32_t *__restrict__ holes) {
+constexpr int scat_arena_u64() {
+    return (V == PV_PLAIN || V == PV_SPAWN) ? 3200 : SCAT_ARENA_COLL;
+}
+static_assert(SCAT_WIN + (11 * CQ_CAP * 4) / 8 <= 3200, "table + queue");
+static_assert(8 * (256 * IPP / 2) * 4 <= 3200 * 8, "staging half");
+static_assert(sizeof(BlockHash) <= 3200 * 8, "hash table");
+#define PUSH_SCATTER_PARAMS                                                                   \
+    SpView src, SpView dst, const uint32_t *__restrict__ nptr, uint32_t cap,                  \
+        const float2 *__restrict__ E, PushConst pc, mcc::MccConst mc, uint32_t step,          \
+        uint32_t *__restrict__ binCount, const uint32_t *__restrict__ binStart,               \
+        unsigned long long *__restrict__ q, int do_push, int deposit, Spawn sp,               \
+        unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6,                \
+        uint32_t *__restrict__ holes
+#define PUSH_SCATTER_ARGS                                                                     \
+    src, dst, nptr, cap, E, pc, mc, step, binCount, binStart, q, do_push, deposit, sp, events, \
+        lost6, holes
+template <int V>
+__device__ __forceinline__ void push_scatter_body(PUSH_SCATTER_PARAMS) {
     pin_variant<V>(pc, mc, src);
     pin_spawn<V>(sp);
     if (V != PV_GENERIC) dst.w = nullptr;
     if (blockIdx.x * (blockDim.x * IPP) >= *nptr) return;  // past the live count
-    __shared__ unsigned long long smem[2 * BLOCK_LDS_U64];
+    constexpr int ARENA = scat_arena_u64<V>();
+    __shared__ unsigned long long arena[ARENA];
     __shared__ uint32_t se[8], sl[6], s_total, shole;
-    static_assert(sizeof(BlockHash) <= BLOCK_LDS_U64 * sizeof(unsigned long long), "hash size");
-    static_assert(8 * 256 * IPP <= 2 * BLOCK_LDS_U64 * 2, "staging size");
-    unsigned long long *lds = smem;  // collision queue, then deposit window
-    BlockHash &H = *reinterpret_cast<BlockHash *>(smem + BLOCK_LDS_U64);
+    uint32_t *tcnt = reinterpret_cast<uint32_t *>(arena);  // per key: count, then staging base
+    uint32_t *tkey = tcnt + SCAT_WIN;                      // per key: slot of its rank 0
+    unsigned long long *cq = arena + SCAT_WIN;             // collision queue behind the table
     if (threadIdx.x < 8) se[threadIdx.x] = 0;
     if (threadIdx.x < 6) sl[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_total = shole = 0;
@@ -871,38 +899,70 @@ __global__ void __launch_bounds__(256)
     __syncthreads();  // counters cleared, key range known, bounds staged
     kmin = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
     kmax = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
-    // direct table: entry = key - kmin, H.cnt the per-key count, then the
-    // staging base; H.key the slot base
     const int span = kmax - kmin + 1;  // <= 0: no claims
-    const bool direct = span <= SCAT_WIN;
-    if (direct) {
-        for (int e = threadIdx.x; e < span; e += blockDim.x) H.cnt[e] = 0u;
-    } else {
-        hash_clear(H);
-    }
+    const bool direct = span <= SCAT_WIN;  // uniform over the block
+    if (direct)
+        for (int e = threadIdx.x; e < span; e += blockDim.x) tcnt[e] = 0u;
     __syncthreads();
+    // table block: rank of each particle within its key; wide block: one
+    // claim per particle, its slot start(k) + old - 1 (old = 0 on a corrupt
+    // histogram gives a slot >= cap, which is skipped)
+    uint32_t slot[IPP];
 #pragma unroll
     for (int r = 0; r < IPP; r++) {
+        rank[r] = 0u;
+        slot[r] = 0xFFFFFFFFu;
         if (direct) {
             ent[r] -= (uint32_t)kmin;
-            rank[r] = claim[r] ? atomicAdd(&H.cnt[ent[r]], 1u) : 0u;
-        } else {
-            ent[r] = hash_insert(H, ent[r], claim[r], rank[r]);
+            if (claim[r]) rank[r] = atomicAdd(&tcnt[ent[r]], 1u);
         }
     }
+    if (!direct) {
+        // wide block: ranks in the LDS hash table, one claim per distinct
+        // key, and every slot known before the collision queue takes the
+        // LDS (the claims' latency is not hidden: such blocks are few once
+        // the store has been sorted)
+        BlockHash &H = *reinterpret_cast<BlockHash *>(arena);
+        hash_clear(H);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < IPP; r++) ent[r] = hash_insert(H, ent[r], claim[r], rank[r]);
+        __syncthreads();
+        constexpr int EPT = HASH_SIZE / 256;
+        uint32_t kk[EPT], cc[EPT], ho[EPT], hs[EPT];
+#pragma unroll
+        for (int j = 0; j < EPT; j++) {
+            kk[j] = H.key[threadIdx.x + j * 256];
+            cc[j] = H.cnt[threadIdx.x + j * 256];
+        }
+#pragma unroll
+        for (int j = 0; j < EPT; j++) {
+            ho[j] = hs[j] = 0u;
+            if (kk[j] != HASH_EMPTY) {
+                ho[j] = atomicSub(&binCount[(size_t)kk[j] * BIN_STRIDE], cc[j]);
+                hs[j] = binStart[kk[j]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < EPT; j++)
+            if (kk[j] != HASH_EMPTY) H.key[threadIdx.x + j * 256] = hs[j] + ho[j] - cc[j];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < IPP; r++)
+            if (claim[r]) slot[r] = H.key[ent[r]] + rank[r];
+    }
     __syncthreads();
-    // one claim per distinct key: the bin count runs down from its total, so
-    // the block's c particles of key k take the slots start(k) + old - c +
-    // rank, rank < c (a corrupt histogram, old < c, gives slots >= cap, which
-    // are skipped).  The claims of a direct table are issued here and their
-    // results used after the push.
+    // one claim per distinct key of a table block: the bin count runs down
+    // from its total, so the block's c particles of key k take the slots
+    // start(k) + old - c + rank, rank < c.  The claims are issued here and
+    // their results used after the push.
     uint32_t old[SCAT_EPT], st0[SCAT_EPT];
 #pragma unroll
     for (int j = 0; j < SCAT_EPT; j++) {
         const int e = threadIdx.x + j * 256;
         old[j] = st0[j] = 0u;
         if (direct && e < span) {
-            const uint32_t c = H.cnt[e];
+            const uint32_t c = tcnt[e];
             if (c) {
                 old[j] = atomicSub(&binCount[(size_t)(kmin + e) * BIN_STRIDE], c);
                 st0[j] = binStart[kmin + e];
@@ -914,97 +974,92 @@ __global__ void __launch_bounds__(256)
         bool act[IPP];
 #pragma unroll
         for (int r = 0; r < IPP; r++) act[r] = base + r * blockDim.x < n && s[r].jx != CELL_HOLE;
-        block_collide_n<IPP>(act, s, pid, mc, step, qs, lds, mc.bins ? sbins : nullptr);
+        block_collide_n<IPP>(act, s, pid, mc, step, qs, cq, mc.bins ? sbins : nullptr);
     }
     // the rounds are spelled out so every array index is a compile-time constant
     scatter_round(s[0], pid[0], w[0], base + 0 * blockDim.x < n, claim[0], kept[0],
-                  dep[0], E, pc, mc, step, do_push, blk, qs[0], se, sl, &shole, lds, sp, lost6);
+                  dep[0], E, pc, mc, step, do_push, blk, qs[0], se, sl, &shole, cq, sp, lost6);
     scatter_round(s[1], pid[1], w[1], base + 1 * blockDim.x < n, claim[1], kept[1],
-                  dep[1], E, pc, mc, step, do_push, blk, qs[1], se, sl, &shole, lds, sp, lost6);
+                  dep[1], E, pc, mc, step, do_push, blk, qs[1], se, sl, &shole, cq, sp, lost6);
     scatter_round(s[2], pid[2], w[2], base + 2 * blockDim.x < n, claim[2], kept[2],
-                  dep[2], E, pc, mc, step, do_push, blk, qs[2], se, sl, &shole, lds, sp, lost6);
+                  dep[2], E, pc, mc, step, do_push, blk, qs[2], se, sl, &shole, cq, sp, lost6);
     scatter_round(s[3], pid[3], w[3], base + 3 * blockDim.x < n, claim[3], kept[3],
-                  dep[3], E, pc, mc, step, do_push, blk, qs[3], se, sl, &shole, lds, sp, lost6);
+                  dep[3], E, pc, mc, step, do_push, blk, qs[3], se, sl, &shole, cq, sp, lost6);
     __syncthreads();  // every round has read its collision result
     if (direct) {
 #pragma unroll
         for (int j = 0; j < SCAT_EPT; j++) {
             const int e = threadIdx.x + j * 256;
             if (e < span) {
-                const uint32_t c = H.cnt[e];
+                const uint32_t c = tcnt[e];
                 if (c) {
-                    H.key[e] = st0[j] + old[j] - c;
-                    H.cnt[e] = atomicAdd(&s_total, c);
+                    tkey[e] = st0[j] + old[j] - c;
+                    tcnt[e] = atomicAdd(&s_total, c);
                 }
             }
         }
-    } else {
-        // hash table: each thread takes HASH_SIZE / 256 entries and issues
-        // all of its claims before it uses the first result
-        constexpr int EPT = HASH_SIZE / 256;
-        uint32_t kk[EPT], cc[EPT], old[EPT], st0[EPT];
-#pragma unroll
-        for (int j = 0; j < EPT; j++) {
-            kk[j] = H.key[threadIdx.x + j * 256];
-            cc[j] = H.cnt[threadIdx.x + j * 256];
-        }
-#pragma unroll
-        for (int j = 0; j < EPT; j++) {
-            old[j] = st0[j] = 0u;
-            if (kk[j] != HASH_EMPTY) {
-                old[j] = atomicSub(&binCount[(size_t)kk[j] * BIN_STRIDE], cc[j]);
-                st0[j] = binStart[kk[j]];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < EPT; j++)
-            if (kk[j] != HASH_EMPTY) {
-                H.key[threadIdx.x + j * 256] = st0[j] + old[j] - cc[j];
-                H.cnt[threadIdx.x + j * 256] = atomicAdd(&s_total, cc[j]);
-            }
     }
     __syncthreads();
-    uint32_t slot[IPP], lidx[IPP];
+    uint32_t lidx[IPP];
 #pragma unroll
     for (int r = 0; r < IPP; r++) {
-        slot[r] = claim[r] ? H.key[ent[r]] + rank[r] : 0xFFFFFFFFu;
-        lidx[r] = claim[r] ? H.cnt[ent[r]] + rank[r] : 0u;
+        lidx[r] = 0u;
+        if (direct && claim[r]) {
+            slot[r] = tkey[ent[r]] + rank[r];
+            lidx[r] = tcnt[ent[r]] + rank[r];
+        }
+        // weighted stores stay direct
+        if (dst.w && claim[r] && slot[r] < cap) dst.w[slot[r]] = w[r];
     }
     __syncthreads();  // the table is read; the deposit window reuses the LDS
-    // the whole 32 KB (the table's half too): a 64 x 64 window, as the
-    // particles of a sort pass have moved two steps since their store was
-    // sorted
-    if (deposit) block_deposit_n<IPP, 2 * BLOCK_LDS_U64>(dep, s, w, q, pc.N, lds);
+    if (deposit) block_deposit_n<IPP, ARENA>(dep, s, w, q, pc.N, arena);
     __syncthreads();
-    uint32_t *stg = reinterpret_cast<uint32_t *>(smem);  // 8 words x 256 IPP, SoA
-    constexpr uint32_t SN = 256 * IPP;
+    if (direct) {
+        // the claimed slots staged in slot order, half of the block's at a
+        // time (a claimed slot of a particle that left holds a hole)
+        uint32_t *stg = reinterpret_cast<uint32_t *>(arena);  // 8 words x SH, SoA
+        constexpr uint32_t SH = 256 * IPP / 2;
+        const uint32_t tot = s_total;
+        for (uint32_t lo = 0; lo < tot; lo += SH) {  // uniform: tot is the block's
 #pragma unroll
-    for (int r = 0; r < IPP; r++) {
-        if (claim[r]) {  // a claimed slot of a particle that left holds a hole
-            const uint32_t l = lidx[r];
-            stg[l] = slot[r];
-            stg[SN + l] = kept[r] ? cell_pack(s[r].jx, s[r].jy) : cell_pack(CELL_HOLE, CELL_HOLE);
-            stg[2 * SN + l] = __float_as_uint(s[r].ox);
-            stg[3 * SN + l] = __float_as_uint(s[r].oy);
-            stg[4 * SN + l] = __float_as_uint(s[r].vx);
-            stg[5 * SN + l] = __float_as_uint(s[r].vy);
-            stg[6 * SN + l] = __float_as_uint(s[r].vz);
-            stg[7 * SN + l] = pid[r];
-            if (dst.w && slot[r] < cap) dst.w[slot[r]] = w[r];  // weighted stores stay direct
+            for (int r = 0; r < IPP; r++) {
+                if (claim[r] && lidx[r] >= lo && lidx[r] < lo + SH) {
+                    const uint32_t l = lidx[r] - lo;
+                    stg[l] = slot[r];
+                    stg[SH + l] =
+                        kept[r] ? cell_pack(s[r].jx, s[r].jy) : cell_pack(CELL_HOLE, CELL_HOLE);
+                    stg[2 * SH + l] = __float_as_uint(s[r].ox);
+                    stg[3 * SH + l] = __float_as_uint(s[r].oy);
+                    stg[4 * SH + l] = __float_as_uint(s[r].vx);
+                    stg[5 * SH + l] = __float_as_uint(s[r].vy);
+                    stg[6 * SH + l] = __float_as_uint(s[r].vz);
+                    stg[7 * SH + l] = pid[r];
+                }
+            }
+            __syncthreads();
+            const uint32_t m = min(SH, tot - lo);
+            for (uint32_t l = threadIdx.x; l < m; l += blockDim.x) {
+                const uint32_t sl = stg[l];
+                if (sl < cap) {
+                    dst.cell[sl] = stg[SH + l];
+                    dst.ox[sl] = __uint_as_float(stg[2 * SH + l]);
+                    dst.oy[sl] = __uint_as_float(stg[3 * SH + l]);
+                    dst.vx[sl] = __uint_as_float(stg[4 * SH + l]);
+                    dst.vy[sl] = __uint_as_float(stg[5 * SH + l]);
+                    dst.vz[sl] = __uint_as_float(stg[6 * SH + l]);
+                    dst.id[sl] = stg[7 * SH + l];
+                }
+            }
+            __syncthreads();  // the half is written out before the next fills it
         }
-    }
-    __syncthreads();
-    const uint32_t tot = s_total;
-    for (uint32_t l = threadIdx.x; l < tot; l += blockDim.x) {
-        const uint32_t sl = stg[l];
-        if (sl < cap) {
-            dst.cell[sl] = stg[SN + l];
-            dst.ox[sl] = __uint_as_float(stg[2 * SN + l]);
-            dst.oy[sl] = __uint_as_float(stg[3 * SN + l]);
-            dst.vx[sl] = __uint_as_float(stg[4 * SN + l]);
-            dst.vy[sl] = __uint_as_float(stg[5 * SN + l]);
-            dst.vz[sl] = __uint_as_float(stg[6 * SN + l]);
-            dst.id[sl] = stg[7 * SN + l];
+    } else {
+#pragma unroll
+        for (int r = 0; r < IPP; r++) {
+            if (claim[r] && slot[r] < cap) {
+                PState o = s[r];
+                if (!kept[r]) o.jx = o.jy = CELL_HOLE;
+                store_state(dst, slot[r], o, pid[r]);
+            }
         }
     }
     if (mc.on) {
@@ -1015,6 +1070,34 @@ __global__ void __launch_bounds__(256)
         atomicAdd(&lost6[cnt_slot() * 6 + threadIdx.x], sl[threadIdx.x]);
     if (threadIdx.x == 0 && shole) atomicAdd(&holes[cnt_slot()], shole);
 }
+
+template <int V>
+__global__ void __launch_bounds__(256) k_push_scatter(PUSH_SCATTER_PARAMS) {
+    push_scatter_body<V>(PUSH_SCATTER_ARGS);
+}
+// The collisionless variants fit 6 waves per SIMD in 76 VGPRs without spills
+// (7 spill, and measured slower); left to itself the compiler gives them 110
+// (4 waves).  The collisional ones keep their 117 (4 waves).
+template <>
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(SCAT_WAVES_PLAIN, SCAT_WAVES_PLAIN)))
+    k_push_scatter<PV_PLAIN>(PUSH_SCATTER_PARAMS) {
+    push_scatter_body<PV_PLAIN>(PUSH_SCATTER_ARGS);
+}
+template <>
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(SCAT_WAVES_PLAIN, SCAT_WAVES_PLAIN)))
+    k_push_scatter<PV_SPAWN>(PUSH_SCATTER_PARAMS) {
+    push_scatter_body<PV_SPAWN>(PUSH_SCATTER_ARGS);
+}
+#ifdef SCAT_WAVES_COLL
+template <>
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(SCAT_WAVES_COLL, SCAT_WAVES_COLL)))
+    k_push_scatter<PV_COLL_E>(PUSH_SCATTER_PARAMS) {
+    push_scatter_body<PV_COLL_E>(PUSH_SCATTER_ARGS);
+}
+#endif
 
 
 // one particle of k_push_inplace: collide (block-wide when blk), push, store
