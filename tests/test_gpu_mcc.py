@@ -322,6 +322,37 @@ def test_sort_interval_does_not_change_collisional_results():
     assert out[1]["ev0"][3] > 0 and out[1]["li"][:4].sum() > 0
 
 
+def test_fused_sort_histogram_matches_a_counted_one():
+    """An in-place pass ahead of a sort pass counts that pass's keys itself,
+    and the commits of the same step add the particles they append; a call
+    that ends with such a histogram pending drops it, and the next sort pass
+    counts the store instead.  Sixteen steps in one call (fused histograms)
+    and in sixteen calls of one step (every histogram counted by k_key_count)
+    give the same particles, charge and counts."""
+    out = {}
+    for calls in (1, 16):
+        p = collisional(cells=32, ppc=60, plop=True, te=10.0)
+        p.sigma_load_i = math.sqrt(Q * 2000.0 / MI)  # ions reach the walls: re-injection
+        p.sort_interval = 2
+        print(f"sort histogram: 16 steps in {calls} call(s)", flush=True)
+        with pic2d.Pic(p) as sim:
+            for _ in range(calls):
+                sim.advance(16 // calls)
+            out[calls] = dict(e=sim.get_state(0), i=sim.get_state(1), phi=sim.get_phi(),
+                              ev0=sim.collision_counts(0), le=sim.loss_counts(0),
+                              li=sim.loss_counts(1), n=(sim.particle_size(0), sim.particle_size(1)))
+    for s in ("e", "i"):
+        a = {kk: v[np.argsort(out[1][s]["id"])] for kk, v in out[1][s].items()}
+        b = {kk: v[np.argsort(out[16][s]["id"])] for kk, v in out[16][s].items()}
+        for kk in ("id", "cell", "ox", "oy", "vx", "vy", "vz"):
+            assert np.array_equal(a[kk], b[kk]), (s, kk)
+    for key in ("phi", "ev0", "le", "li", "n"):
+        assert np.array_equal(out[1][key], out[16][key]), key
+    # ionisation products and re-injected pairs were appended between the
+    # in-place passes and the sort passes
+    assert out[1]["ev0"][3] > 0 and out[1]["li"][:4].sum() > 0
+
+
 @pytest.mark.parametrize("skew", [False, True])
 def test_reference_ion_charge_exchange_matches_the_oracle(skew):
     """Reference mode: the per-column accumulator (ion_collisions.cpp:297-303)

@@ -532,8 +532,47 @@ int alloc_steady(pic_handle *h, Species &s, uint32_t nb) {
     return PIC_OK;
 }
 
+// Histogram of a species' sort pass (binCount runs back to zero in each
+// sort pass, so it is zero between them).
+uint32_t *bin_count(pic_handle *h, int si) {
+    return h->binCount + (size_t)si * h->nbins_pad * BIN_STRIDE;
+}
+
+// The in-place pass ahead of a sort pass can count the sort pass's keys (the
+// cells its particles end the step in) when nothing but the fused step's
+// commits, which count what they append, changes the store in between:
+// physical mode without the particle source, the population reduction or
+// surface physics.
+bool hist_fusable(const pic_handle *h, int si) {
+    const Species &s = h->sp[si];
+    return h->P.mode == PIC_MODE_PHYSICAL && s.pc.mode == PIC_DEV_MODE_PHYS &&
+           h->P.number_to_inject <= 0 && !s.reduce && !h->sp[0].reduce && !h->sp[1].reduce &&
+           !h->wall_on && !h->refcx;
+}
+
+// A pending histogram that will not be used (any path but the fused step's
+// next sort pass of the species) is cleared, and its sort pass counts anew.
+void drop_hist(pic_handle *h, int si) {
+    Species &s = h->sp[si];
+    if (!s.hist_pending) return;
+    launch_fill<uint32_t>(h->st, bin_count(h, si), (size_t)h->nbins_pad * BIN_STRIDE, 0u);
+    s.hist_pending = false;
+}
+
+// The fused steps of one pic_run / pic_advance call may leave a sort pass's
+// histogram pending: it is cleared when the call returns, so any other entry
+// point (or the next call) sees a zero histogram.
+struct HistScope {
+    pic_handle *h;
+    ~HistScope() {
+        drop_hist(h, 0);
+        drop_hist(h, 1);
+    }
+};
+
 // append species `src`'s staged particles to species `dst_s`'s current buffer
 void commit(pic_handle *h, int src, int dst_s, int deposit) {
+    drop_hist(h, dst_s);  // (not on the fused step's path: count its keys again)
     Species &d = h->sp[dst_s];
     launch_commit(h->st, h->sp[src].stage, &h->cnt[src], d.buf[d.cur], &h->cnt[dst_s].n[d.cur],
                   d.cap, &h->cnt[dst_s], deposit, h->N, d.pc.mode, d.q);
@@ -543,8 +582,11 @@ void commit(pic_handle *h, int src, int dst_s, int deposit) {
 void commit_both(pic_handle *h, int dep_i, int dep_e) {
     auto args = [&](int s, int dep) {
         Species &d = h->sp[s];
+        // appended particles join the histogram of a pending sort pass
         return CommitArgs{d.stage, &h->cnt[s], d.buf[d.cur], &h->cnt[s].n[d.cur], (uint32_t)d.cap,
-                          &h->cnt[s], dep, d.pc.mode, d.q};
+                          &h->cnt[s], dep, d.pc.mode, d.q,
+                          d.hist_pending ? bin_count(h, s) : nullptr, d.pc.tilesX,
+                          d.pc.keyshift};
     };
     launch_commit_pair(h->st, args(1, dep_i), args(0, dep_e), h->N);
 }
@@ -694,12 +736,15 @@ int check_overflow(pic_handle *h, bool wait) {
 // Physical-mode collisions run in a pass of their own ahead of the particle
 // pass when few particles are candidates (the argon ions: a fraction of a
 // percent): that pass then has the registers of a collisionless one, for 8
-// B per particle read ahead.  Where many are (the electrons: ~13 %), the
-// candidates' velocities would cost a read and write of nearly every line,
-// and the particle pass collides them in its block queue instead.
+// B per particle read ahead.  Where many are (the electrons: ~13 % under the
+// global bound), the candidates' velocities would cost a read and write of
+// nearly every line, and the particle pass collides them in its block queue
+// instead.  With energy-binned bounds (about 2 % candidates in a 2 eV argon
+// discharge) the deferred form was measured too: the electrons' in-place
+// pass fell from 1.69 to 1.35 ms at 1025^2, but colliding, pushing and
+// depositing the 2 M queued candidates took 0.43 ms, so they collide in the
+// pass.
 bool collide_ahead(const Species &s) {
-    // (a species with energy bins collides in the pass: the deferred test
-    // does not read vz)
     return s.mc.on && s.mc.mode == PIC_DEV_MODE_PHYS && s.mc.p_null < 0.02f && !s.mc.bins;
 }
 
@@ -721,15 +766,21 @@ void push_species(pic_handle *h, int si, int do_push, int deposit) {
         mc.on = 0;
     }
     clear_holes(h, si);  // the pass drops the old holes
-    int p1 = prof_begin(h, 4);
-    launch_key_count(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, s.pc, h->binCount, do_push);
-    prof_end(h, p1);
+    uint32_t *bins = bin_count(h, si);
+    // the in-place pass before a timestep's sort pass has made its histogram
+    if (!(do_push && s.hist_pending)) {
+        drop_hist(h, si);
+        int p1 = prof_begin(h, 4);
+        launch_key_count(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, s.pc, bins, do_push);
+        prof_end(h, p1);
+    }
+    s.hist_pending = false;
     int p2 = prof_begin(h, 5);
-    launch_scan(h->st, h->binCount, h->binStart, h->nbins_pad, h->scanPartial, &h->cnt[si].n[1 - c]);
+    launch_scan(h->st, bins, h->binStart, h->nbins_pad, h->scanPartial, &h->cnt[si].n[1 - c]);
     prof_end(h, p2);
     int p3 = prof_begin(h, 2);
     launch_push_scatter(h->st, s.buf[c], s.buf[1 - c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, mc,
-                        step, h->binCount, h->binStart, s.q, do_push, deposit, spn,
+                        step, bins, h->binStart, s.q, do_push, deposit, spn,
                         &h->cnt[si].events[0][0], &h->cnt[si].lost[0][0],
                         &h->cnt[si].holes[0]);
     prof_end(h, p3);
@@ -748,6 +799,10 @@ void push_species_inplace(pic_handle *h, int si) {
     // few candidates: the pass makes the candidate test only and leaves each
     // candidate to k_collide_q (collide, push, store, deposit) after it
     const bool defer = collide_ahead(s) && s.mq;
+    // the next pass of this species sorts: this one (and the deferred
+    // collisions after it) count its keys
+    drop_hist(h, si);
+    const bool hist = s.since_sort + 2 >= s.sort_interval && hist_fusable(h, si);
     if (defer) {  // the queue counters are zero (k_collide_q leaves them so)
         spn.mq = s.mq;
         spn.mq_cnt = s.mq_cnt;
@@ -756,12 +811,14 @@ void push_species_inplace(pic_handle *h, int si) {
     int p = prof_begin(h, 6);
     launch_push_inplace(h->st, s.buf[c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, s.mc,
                         (uint32_t)h->step, s.q, 1, spn, &h->cnt[si].events[0][0],
-                        &h->cnt[si].lost[0][0], &h->cnt[si].holes[0]);
+                        &h->cnt[si].lost[0][0], &h->cnt[si].holes[0],
+                        hist ? bin_count(h, si) : nullptr);
+    s.hist_pending = hist;
     prof_end(h, p);
     if (defer)
         launch_collide_deferred(h->st, s.buf[c], h->E32, s.pc, s.mc, (uint32_t)h->step, spn,
                                 &h->cnt[si].events[0][0], &h->cnt[si].lost[0][0], s.q,
-                                &h->cnt[si].holes[0]);
+                                &h->cnt[si].holes[0], hist ? bin_count(h, si) : nullptr);
     plop_drain(h, si, 1);
     s.holes = true;
     s.since_sort++;
@@ -1219,7 +1276,9 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
     if (hipMemcpy(h->cnt, h->cnt_host, 2 * sizeof(SpCounters), hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_global_error(PIC_ERR_DEVICE, "counter upload failed"));
     int N = h->N;
-    if ((rc = dalloc(h, &h->binCount, (size_t)h->nbins_pad * BIN_STRIDE)) != PIC_OK ||
+    // one histogram per species: an in-place pass may count the keys of its
+    // species' next sort pass while the other species' sort pass runs
+    if ((rc = dalloc(h, &h->binCount, 2 * (size_t)h->nbins_pad * BIN_STRIDE)) != PIC_OK ||
         (rc = dalloc(h, &h->binStart, h->nbins_pad + 1)) != PIC_OK ||
         (rc = dalloc(h, &h->scanPartial, h->nbins_pad / 4096 + 1)) != PIC_OK ||
         (rc = dalloc(h, &h->rho, (size_t)N * N)) != PIC_OK ||
@@ -1228,7 +1287,7 @@ int pic_create(const pic_params *Pin, int restart, const char *dump_dir, pic_han
         (rc = dalloc(h, &h->ey64, (size_t)(N + 1) * (N + 1))) != PIC_OK ||
         (rc = dalloc(h, &h->E32, (size_t)(N + 1) * N + 1)) != PIC_OK)
         return fail(rc);
-    hipMemsetAsync(h->binCount, 0, 4ull * h->nbins_pad * BIN_STRIDE, h->st);
+    hipMemsetAsync(h->binCount, 0, 2 * 4ull * h->nbins_pad * BIN_STRIDE, h->st);
     hipMemsetAsync(h->rho, 0, 8ull * N * N, h->st);
     hipMemsetAsync(h->phi, 0, 8ull * N * N, h->st);
     hipMemsetAsync(h->ex64, 0, 8ull * (N + 1) * (N + 1), h->st);
@@ -1338,7 +1397,10 @@ int pic_run(pic_handle *h) {
         commit_both(h, 0, 0);
     }
     CHECK(diag_prepare_time(h, (int32_t)std::min<int64_t>(h->step + h->run_time, INT32_MAX)));
-    for (int32_t t = 0; t < h->run_time; t++) CHECK(fused_step(h));
+    {
+        HistScope hs{h};
+        for (int32_t t = 0; t < h->run_time; t++) CHECK(fused_step(h));
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return seterr(h, PIC_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
     // pic::pic_run returns when the run is complete
@@ -1353,7 +1415,10 @@ int pic_advance(pic_handle *h, int32_t nsteps) {
     if (!h || nsteps < 0) return set_global_error(PIC_ERR_ARG, "bad argument");
     CHECK(check_overflow(h, false));  // drops of an earlier call that have landed
     CHECK(diag_prepare_time(h, (int32_t)std::min<int64_t>(h->step + nsteps, INT32_MAX)));
-    for (int32_t t = 0; t < nsteps; t++) CHECK(fused_step(h));
+    {
+        HistScope hs{h};
+        for (int32_t t = 0; t < nsteps; t++) CHECK(fused_step(h));
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return seterr(h, PIC_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
     queue_overflow_copy(h);

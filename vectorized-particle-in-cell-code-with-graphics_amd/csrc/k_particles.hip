@@ -358,31 +358,82 @@ __device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v);
 __device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
 __device__ __forceinline__ int wave_sum(int v) { return __ockl_wfred_add_i32(v); }
 
-// Pass 1 of the sort pass: the histogram of the sort keys of the particles'
-// current cells, one global atomic per distinct key per block.  In a
-// timestep's sort pass (do_push) every particle that is not a hole takes a
-// slot, keyed by its cell before the step: one that leaves during the step
-// leaves a hole in its slot, which the next sort pass drops.  Without a push
-// (compaction) only the particles inside the domain are counted, so the
-// output is compact.  Reads 4 B per particle (12 B without a push).  The
-// keys of a block of a sorted store lie in a narrow range (a few rows of
-// tiles): then the block counts them in a direct LDS histogram over that
-// range (one LDS atomic per particle); a wider range (an unsorted store)
-// takes the LDS hash table.
 #define KC_WIN (2 * HASH_SIZE)  // direct-histogram range: the hash table's LDS
+
+// Adds the block's keys (those with in[r]) to binCount, one global atomic
+// per distinct key.  The keys of a block of a sorted store lie in a narrow
+// range (a few rows of tiles): then the block counts them in a direct LDS
+// histogram over that range (one LDS atomic per particle); a range of
+// KC_WIN keys or more (an unsorted store) takes the LDS hash table.  `hist` is KC_WIN words of LDS, free on entry; every
+// thread of the block calls this.
+template <int P>
+__device__ __forceinline__ void block_key_hist(const bool (&in)[P], const uint32_t (&key)[P],
+                                               uint32_t *__restrict__ binCount, uint32_t *hist) {
+    static_assert(sizeof(BlockHash) == KC_WIN * sizeof(uint32_t), "histogram size");
+    __shared__ int wmin[4], wmax[4];
+    int kmin = 0x7FFFFFFF, kmax = -1;
+#pragma unroll
+    for (int r = 0; r < P; r++)
+        if (in[r]) {
+            kmin = min(kmin, (int)key[r]);
+            kmax = max(kmax, (int)key[r]);
+        }
+    kmin = wave_min(kmin);
+    kmax = wave_max(kmax);
+    if ((threadIdx.x & 63) == 0) {
+        wmin[threadIdx.x >> 6] = kmin;
+        wmax[threadIdx.x >> 6] = kmax;
+    }
+    __syncthreads();
+    kmin = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
+    kmax = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+    if (kmax < kmin) return;  // no particle counted (uniform)
+    if (kmax - kmin < KC_WIN) {
+        const int span = kmax - kmin + 1;
+        for (int e = threadIdx.x; e < span; e += blockDim.x) hist[e] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < P; r++)
+            if (in[r]) atomicAdd(&hist[key[r] - (uint32_t)kmin], 1u);
+        __syncthreads();
+        for (int e = threadIdx.x; e < span; e += blockDim.x) {
+            const uint32_t c = hist[e];
+            if (c) atomicAdd(&binCount[(size_t)(kmin + e) * BIN_STRIDE], c);
+        }
+        return;
+    }
+    BlockHash &H = *reinterpret_cast<BlockHash *>(hist);
+    hash_clear(H);
+    __syncthreads();  // table cleared
+#pragma unroll
+    for (int r = 0; r < P; r++) {
+        uint32_t rank;
+        hash_insert(H, key[r], in[r], rank);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < HASH_SIZE; e += blockDim.x) {
+        const uint32_t k = H.key[e];
+        if (k != HASH_EMPTY) atomicAdd(&binCount[(size_t)k * BIN_STRIDE], H.cnt[e]);
+    }
+}
+
+// Pass 1 of the sort pass: the histogram of the sort keys of the particles'
+// current cells (block_key_hist).  In a timestep's sort pass (do_push) every
+// particle that is not a hole takes a slot, keyed by its cell before the
+// step: one that leaves during the step leaves a hole in its slot, which the
+// next sort pass drops.  Without a push (compaction) only the particles
+// inside the domain are counted, so the output is compact.  Reads 4 B per
+// particle (12 B without a push).  An in-place pass ahead of a sort pass
+// makes this histogram itself (k_push_inplace, `hist`).
 __global__ void __launch_bounds__(256)
     k_key_count(SpView sp, const uint32_t *__restrict__ nptr, PushConst pc,
                 uint32_t *__restrict__ binCount, int do_push) {
     const uint32_t n = *nptr;
     if (blockIdx.x * (blockDim.x * IPP) >= n) return;  // past the live count
     __shared__ uint32_t hist[KC_WIN];  // the direct histogram or the hash table
-    __shared__ int wmin[4], wmax[4];
-    static_assert(sizeof(BlockHash) == sizeof(hist), "histogram size");
-    BlockHash &H = *reinterpret_cast<BlockHash *>(hist);
     const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
     uint32_t key[IPP];
     bool in[IPP];
-    int kmin = 0x7FFFFFFF, kmax = -1;
 #pragma unroll
     for (int r = 0; r < IPP; r++) {
         const uint32_t i = base + r * blockDim.x;
@@ -400,49 +451,10 @@ __global__ void __launch_bounds__(256)
                 s.oy = sp.oy[i];
                 in[r] = classify(s, pc.N, pc.mode) == ST_KEPT;
             }
-            if (in[r]) {
-                key[r] = bin_key(s.jx, s.jy, pc.tilesX) >> pc.keyshift;
-                kmin = min(kmin, (int)key[r]);
-                kmax = max(kmax, (int)key[r]);
-            }
+            if (in[r]) key[r] = bin_key(s.jx, s.jy, pc.tilesX) >> pc.keyshift;
         }
     }
-    kmin = wave_min(kmin);
-    kmax = wave_max(kmax);
-    if ((threadIdx.x & 63) == 0) {
-        wmin[threadIdx.x >> 6] = kmin;
-        wmax[threadIdx.x >> 6] = kmax;
-    }
-    __syncthreads();
-    kmin = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
-    kmax = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
-    if (kmax < kmin) return;  // no particle counted (uniform)
-    if (kmax - kmin < KC_WIN) {
-        const int span = kmax - kmin + 1;
-        for (int e = threadIdx.x; e < span; e += blockDim.x) hist[e] = 0u;
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < IPP; r++)
-            if (in[r]) atomicAdd(&hist[key[r] - (uint32_t)kmin], 1u);
-        __syncthreads();
-        for (int e = threadIdx.x; e < span; e += blockDim.x) {
-            const uint32_t c = hist[e];
-            if (c) atomicAdd(&binCount[(size_t)(kmin + e) * BIN_STRIDE], c);
-        }
-        return;
-    }
-    hash_clear(H);
-    __syncthreads();  // table cleared
-#pragma unroll
-    for (int r = 0; r < IPP; r++) {
-        uint32_t rank;
-        hash_insert(H, key[r], in[r], rank);
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < HASH_SIZE; e += blockDim.x) {
-        const uint32_t k = H.key[e];
-        if (k != HASH_EMPTY) atomicAdd(&binCount[(size_t)k * BIN_STRIDE], H.cnt[e]);
-    }
+    block_key_hist<IPP>(in, key, binCount, hist);
 }
 
 // ------------------------------------------------------------ scan
@@ -1103,7 +1115,7 @@ __global__ void __launch_bounds__(256)
 // one particle of k_push_inplace: collide (block-wide when blk), push, store
 // in place or leave a hole, count, stage products; dep = deposit it
 __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint32_t n, PState &s,
-                                              uint32_t pid, bool live, bool &dep,
+                                              uint32_t pid, bool live, bool &dep, bool &kept_out,
                                               const float2 *__restrict__ E, const PushConst &pc,
                                               const mcc::MccConst &mc, uint32_t step,
                                               const Spawn &spn, uint32_t *__restrict__ lost6,
@@ -1125,6 +1137,7 @@ __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint
     int st = ST_HOLE;
     if (live && !cand) st = advance_one(s, pid, E, pc, mc, step, 1, ev, ch, done);
     const bool kept = (st == ST_KEPT);
+    kept_out = kept;
     dep = kept && deposit_ok(s, pc.N, pc.mode);
     if (kept) {
         // the slot holds this particle's previous state: a word whose bits
@@ -1166,7 +1179,7 @@ __device__ __forceinline__ void push_inplace_body(
     SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E, PushConst pc,
     mcc::MccConst mc, uint32_t step, unsigned long long *__restrict__ q, int deposit, Spawn spn,
     unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6,
-    uint32_t *__restrict__ holes) {
+    uint32_t *__restrict__ holes, uint32_t *__restrict__ hist) {
     pin_variant<V>(pc, mc, sp);
     pin_spawn<V>(spn);
     const uint32_t n = *nptr;
@@ -1212,24 +1225,37 @@ __device__ __forceinline__ void push_inplace_body(
     __syncthreads();  // counters initialised, bounds staged
     int qs[IPP];
     if (blk) block_collide_n<IPP>(live, s, pid, mc, step, qs, lds, mc.bins ? sbins : nullptr);
-    bool dep[IPP];
+    bool dep[IPP], kept[IPP];
     // the rounds are spelled out so every array index is a compile-time constant
-    inplace_round(sp, base + 0 * blockDim.x, n, s[0], pid[0], live[0], dep[0], E, pc, mc, step,
+    inplace_round(sp, base + 0 * blockDim.x, n, s[0], pid[0], live[0], dep[0], kept[0], E, pc,
+                  mc, step,
                   spn, lost6, need_vz, blk, qs[0], sl, &shole, se, lds, w[0],
                   V == PV_DEFER);
-    inplace_round(sp, base + 1 * blockDim.x, n, s[1], pid[1], live[1], dep[1], E, pc, mc, step,
+    inplace_round(sp, base + 1 * blockDim.x, n, s[1], pid[1], live[1], dep[1], kept[1], E, pc,
+                  mc, step,
                   spn, lost6, need_vz, blk, qs[1], sl, &shole, se, lds, w[1],
                   V == PV_DEFER);
-    inplace_round(sp, base + 2 * blockDim.x, n, s[2], pid[2], live[2], dep[2], E, pc, mc, step,
+    inplace_round(sp, base + 2 * blockDim.x, n, s[2], pid[2], live[2], dep[2], kept[2], E, pc,
+                  mc, step,
                   spn, lost6, need_vz, blk, qs[2], sl, &shole, se, lds, w[2],
                   V == PV_DEFER);
-    inplace_round(sp, base + 3 * blockDim.x, n, s[3], pid[3], live[3], dep[3], E, pc, mc, step,
+    inplace_round(sp, base + 3 * blockDim.x, n, s[3], pid[3], live[3], dep[3], kept[3], E, pc,
+                  mc, step,
                   spn, lost6, need_vz, blk, qs[3], sl, &shole, se, lds, w[3],
                   V == PV_DEFER);
     // block_deposit_n passes a barrier before it writes the window, so every
     // round has read its collision result from the shared LDS by then
     if (deposit) block_deposit_n<IPP>(dep, s, w, q, pc.N, lds);
     __syncthreads();
+    // ahead of a sort pass: the histogram of the cells the particles end the
+    // step in, the keys that pass claims by (no k_key_count then)
+    if (hist) {
+        uint32_t key[IPP];
+#pragma unroll
+        for (int r = 0; r < IPP; r++)
+            key[r] = kept[r] ? bin_key(s[r].jx, s[r].jy, pc.tilesX) >> pc.keyshift : 0u;
+        block_key_hist<IPP>(kept, key, hist, reinterpret_cast<uint32_t *>(lds));
+    }
     if (threadIdx.x < 6 && sl[threadIdx.x])
         atomicAdd(&lost6[cnt_slot() * 6 + threadIdx.x], sl[threadIdx.x]);
     if (threadIdx.x < 8 && se[threadIdx.x])
@@ -1241,10 +1267,10 @@ __device__ __forceinline__ void push_inplace_body(
     SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E, PushConst pc, \
         mcc::MccConst mc, uint32_t step, unsigned long long *__restrict__ q, int deposit,     \
         Spawn spn, unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6,     \
-        uint32_t *__restrict__ holes
+        uint32_t *__restrict__ holes, uint32_t *__restrict__ hist
 template <int V>
 __global__ void __launch_bounds__(256) k_push_inplace(PUSH_INPLACE_PARAMS) {
-    push_inplace_body<V>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes);
+    push_inplace_body<V>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes, hist);
 }
 // The collisionless variants fit 8 waves per SIMD in 42 VGPRs; left to
 // itself the scheduler gives SPAWN (PLAIN plus a rare queue write) 84 and 5
@@ -1252,17 +1278,20 @@ __global__ void __launch_bounds__(256) k_push_inplace(PUSH_INPLACE_PARAMS) {
 template <>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
     k_push_inplace<PV_PLAIN>(PUSH_INPLACE_PARAMS) {
-    push_inplace_body<PV_PLAIN>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes);
+    push_inplace_body<PV_PLAIN>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes,
+                                hist);
 }
 template <>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
     k_push_inplace<PV_SPAWN>(PUSH_INPLACE_PARAMS) {
-    push_inplace_body<PV_SPAWN>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes);
+    push_inplace_body<PV_SPAWN>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes,
+                                hist);
 }
 template <>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
     k_push_inplace<PV_DEFER>(PUSH_INPLACE_PARAMS) {
-    push_inplace_body<PV_DEFER>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes);
+    push_inplace_body<PV_DEFER>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes,
+                                hist);
 }
 
 // Physical-mode null-collision MCC of one species as passes of their own,
@@ -1339,7 +1368,7 @@ __global__ void __launch_bounds__(256)
                 uint32_t step, Spawn spn, const uint32_t *__restrict__ mq, uint32_t seg_cap,
                 uint32_t *__restrict__ mq_cnt, unsigned long long *__restrict__ events,
                 uint32_t *__restrict__ lost6, int defer, unsigned long long *__restrict__ q,
-                uint32_t *__restrict__ holes) {
+                uint32_t *__restrict__ holes, uint32_t *__restrict__ hist) {
     __shared__ uint32_t se[8];
     if (threadIdx.x < 8) se[threadIdx.x] = 0;
     __syncthreads();
@@ -1371,6 +1400,12 @@ __global__ void __launch_bounds__(256)
                 sp.oy[i] = s.oy;
                 if (deposit_ok(s, pc.N, pc.mode))
                     dep_global(q, pc.N, s.jx, s.jy, fx_weights_w(s.ox, s.oy, pw));
+                // the histogram of the next sort pass, which the deferring
+                // in-place pass counted for every other particle
+                if (hist)
+                    atomicAdd(&hist[(size_t)(bin_key(s.jx, s.jy, pc.tilesX) >> pc.keyshift) *
+                                    BIN_STRIDE],
+                              1u);
             } else {
                 sp.cell[i] = cell_pack(CELL_HOLE, CELL_HOLE);
                 atomicAdd(&lost6[cnt_slot() * 6 + st - 1], 1u);
@@ -1663,7 +1698,9 @@ __device__ void block_sort_pairs(unsigned long long *key, uint32_t *val, uint32_
 __device__ __forceinline__ void commit_body(StageView sv, SpCounters *__restrict__ scnt, SpView dst,
                                             uint32_t *__restrict__ n_dst, uint32_t cap_dst,
                                             SpCounters *__restrict__ dcnt, int deposit, int N,
-                                            int mode, unsigned long long *__restrict__ q) {
+                                            int mode, unsigned long long *__restrict__ q,
+                                            uint32_t *__restrict__ hist = nullptr,
+                                            int tilesX = 0, int keyshift = 0) {
     __shared__ uint32_t s_c, s_base, s_nid;
     if (threadIdx.x == 0) {
         uint32_t c = scnt->stage_count;
@@ -1711,6 +1748,10 @@ __device__ __forceinline__ void commit_body(StageView sv, SpCounters *__restrict
                 if (dst.w) dst.w[slot] = w;
                 if (deposit && deposit_ok(p, N, mode))
                     dep_global(q, N, p.jx, p.jy, fx_weights_w(p.ox, p.oy, w));
+                // the store's histogram for a pending sort pass counts it too
+                if (hist)
+                    atomicAdd(&hist[(size_t)(bin_key(p.jx, p.jy, tilesX) >> keyshift) * BIN_STRIDE],
+                              1u);
             }
         }
         stored_before += chunk_total;
@@ -1738,7 +1779,8 @@ __global__ void __launch_bounds__(1024)
 // counters and charge), workgroup b takes the b-th
 __global__ void __launch_bounds__(1024) k_commit_pair(CommitArgs a, CommitArgs b, int N) {
     const CommitArgs &c = blockIdx.x == 0 ? a : b;
-    commit_body(c.sv, c.scnt, c.dst, c.n_dst, c.cap_dst, c.dcnt, c.deposit, N, c.mode, c.q);
+    commit_body(c.sv, c.scnt, c.dst, c.n_dst, c.cap_dst, c.dcnt, c.deposit, N, c.mode, c.q, c.hist,
+                c.tilesX, c.keyshift);
 }
 
 // ---------------------------------------------- reference steady-state walls
@@ -2069,10 +2111,11 @@ void launch_push_scatter(hipStream_t st, const SpView &src, const SpView &dst, c
 void launch_push_inplace(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
                          const float2 *E, const PushConst &pc, const mcc::MccConst &mc,
                          uint32_t step, unsigned long long *q, int deposit, const Spawn &spn,
-                         unsigned long long *events, uint32_t *lost6, uint32_t *holes) {
+                         unsigned long long *events, uint32_t *lost6, uint32_t *holes,
+                         uint32_t *hist) {
     const int v = pass_variant(sp, pc, mc, &spn);
     LAUNCH_VARIANT(v, k_push_inplace, dim3(blocks_for(cap, 256 * IPP)), st, sp, n, E, pc, mc, step,
-                   q, deposit, spn, events, lost6, holes);
+                   q, deposit, spn, events, lost6, holes, hist);
 }
 
 uint32_t collide_seg_cap(int64_t cap) {
@@ -2090,15 +2133,15 @@ void launch_collide(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t
                        mq, seg_cap, mq_cnt);
     hipLaunchKernelGGL(k_collide_q, dim3(MQ_SEGS), dim3(256), 0, st, sp, E, pc, mc, step, spn,
                        mq, seg_cap, mq_cnt, events, lost6, 0, (unsigned long long *)nullptr,
-                       (uint32_t *)nullptr);
+                       (uint32_t *)nullptr, (uint32_t *)nullptr);
 }
 
 void launch_collide_deferred(hipStream_t st, const SpView &sp, const float2 *E,
                              const PushConst &pc, const mcc::MccConst &mc, uint32_t step,
                              const Spawn &spn, unsigned long long *events, uint32_t *lost6,
-                             unsigned long long *q, uint32_t *holes) {
+                             unsigned long long *q, uint32_t *holes, uint32_t *hist) {
     hipLaunchKernelGGL(k_collide_q, dim3(MQ_SEGS), dim3(256), 0, st, sp, E, pc, mc, step, spn,
-                       spn.mq, spn.mq_seg, spn.mq_cnt, events, lost6, 1, q, holes);
+                       spn.mq, spn.mq_seg, spn.mq_cnt, events, lost6, 1, q, holes, hist);
 }
 
 void launch_plop_drain(hipStream_t st, const StageView &wq, SpCounters *cw, const Spawn &sp,

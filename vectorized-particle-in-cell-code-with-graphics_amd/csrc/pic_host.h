@@ -28,6 +28,9 @@ struct Species {
     int sort_interval = 1;  // particle passes per counting sort
     int since_sort = 0;   // in-place steps since the last sort pass
     bool holes = false;   // the current buffer may hold empty slots
+    // the last in-place pass (and the commits after it) counted the keys of
+    // the next sort pass into this species' histogram
+    bool hist_pending = false;
     const char *type = "";  // dataDump suffix (parameters.h:316-318)
     // per-particle weights (buf[b].w, stage.w): needed once weights can
     // differ between particles (source, reduction, a dump with mixed

@@ -161,7 +161,8 @@ void launch_push_scatter(hipStream_t st, const SpView &src, const SpView &dst, c
 void launch_push_inplace(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
                          const float2 *E, const PushConst &pc, const mcc::MccConst &mc,
                          uint32_t step, unsigned long long *q, int deposit, const Spawn &spn,
-                         unsigned long long *events, uint32_t *lost6, uint32_t *holes);
+                         unsigned long long *events, uint32_t *lost6, uint32_t *holes,
+                         uint32_t *hist = nullptr);
 // candidate queue of the collisions run ahead: MQ_SEGS segments of
 // collide_seg_cap(cap) indices, one counter per segment every MQ_STRIDE words
 #define MQ_SEGS 1024
@@ -174,7 +175,7 @@ void launch_collide(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t
 void launch_collide_deferred(hipStream_t st, const SpView &sp, const float2 *E,
                              const PushConst &pc, const mcc::MccConst &mc, uint32_t step,
                              const Spawn &spn, unsigned long long *events, uint32_t *lost6,
-                             unsigned long long *q, uint32_t *holes);
+                             unsigned long long *q, uint32_t *holes, uint32_t *hist = nullptr);
 void launch_plop_drain(hipStream_t st, const StageView &wq, SpCounters *cw, const Spawn &sp,
                        const mcc::MccConst &mc, uint32_t step, int N);
 void launch_mcc(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t cap,
@@ -190,6 +191,9 @@ struct CommitArgs {
     SpCounters *dcnt;
     int deposit, mode;
     unsigned long long *q;
+    // histogram of a pending sort pass of dst (null: none) and its key shape
+    uint32_t *hist;
+    int tilesX, keyshift;
 };
 // both species' commits in one launch (one workgroup each)
 void launch_commit_pair(hipStream_t st, const CommitArgs &a, const CommitArgs &b, int N);
