@@ -778,11 +778,18 @@ void push_species(pic_handle *h, int si, int do_push, int deposit) {
     int p2 = prof_begin(h, 5);
     launch_scan(h->st, bins, h->binStart, h->nbins_pad, h->scanPartial, &h->cnt[si].n[1 - c]);
     prof_end(h, p2);
+    // a store that was never sorted (freshly loaded or uploaded, or
+    // scrambled by a compaction) deposits from its sorted slots after the
+    // pass, where a block's particles are neighbours: in the unsorted order
+    // nearly every particle would take four global atomics
+    const bool cold = s.since_sort >= (1 << 20);
     int p3 = prof_begin(h, 2);
     launch_push_scatter(h->st, s.buf[c], s.buf[1 - c], &h->cnt[si].n[c], s.cap, h->E32, s.pc, mc,
-                        step, bins, h->binStart, s.q, do_push, deposit, spn,
+                        step, bins, h->binStart, s.q, do_push, deposit && !cold, spn,
                         &h->cnt[si].events[0][0], &h->cnt[si].lost[0][0],
                         &h->cnt[si].holes[0]);
+    if (deposit && cold)
+        launch_deposit(h->st, s.buf[1 - c], &h->cnt[si].n[1 - c], s.cap, h->N, s.pc.mode, s.q);
     prof_end(h, p3);
     plop_drain(h, si, do_push);
     s.cur = 1 - c;
@@ -841,18 +848,24 @@ void compact_species(pic_handle *h, int si) {
     if (h->sp[si].holes) push_species(h, si, 0, 0);
 }
 
-void absorb_species(pic_handle *h, int si, bool plop) {
+int absorb_species(pic_handle *h, int si, bool plop) {
     Species &s = h->sp[si];
     int c = s.cur;
-    set_count(h, si, 1 - c, 0);
+    // block counts and starts of the compaction, allocated on first use for
+    // the larger store
+    if (!h->abs_count) {
+        const uint32_t nb = absorb_blocks_pad(std::max(h->sp[0].cap, h->sp[1].cap));
+        CHECK(dalloc(h, &h->abs_count, nb));
+        CHECK(dalloc(h, &h->abs_start, (size_t)nb + 1));
+        CHECK(dalloc(h, &h->abs_partial, (size_t)nb / 4096 + 1));
+    }
     launch_absorb(h->st, s.buf[c], s.buf[1 - c], &h->cnt[si].n[c], &h->cnt[si].n[1 - c], s.cap,
                   h->N, s.pc.mode, &h->cnt[si].lost[0][0], h->sp[1].mc, (uint32_t)h->step,
-                  make_spawn(h, plop, si));
+                  make_spawn(h, plop, si), h->abs_count, h->abs_start, h->abs_partial);
     s.cur = 1 - c;
     clear_holes(h, si);  // the compaction dropped them
-    // the compaction keeps 256-particle runs in order but not the runs: the
-    // next pass sorts, so the in-place passes after it see compact blocks
-    s.since_sort = 1 << 20;
+    // the survivors keep their order: a sorted store stays sorted
+    return PIC_OK;
 }
 
 // phi from the solve with the boundary values; with_e: E (and its fp32 copy
@@ -1345,8 +1358,9 @@ int pic_destroy(pic_handle *h) {
     if (!h) return PIC_ERR_ARG;
     if (h->st) hipStreamSynchronize(h->st);
     for (int s = 0; s < 2; s++) species_free(h->sp[s]);
-    void *ptrs[] = {h->cnt, h->binCount, h->binStart, h->scanPartial, h->rho,
-                    h->phi, h->ex64,     h->ey64,     h->E32,         h->ntab};
+    void *ptrs[] = {h->cnt,  h->binCount, h->binStart,  h->scanPartial, h->rho,
+                    h->phi,  h->ex64,     h->ey64,      h->E32,         h->ntab,
+                    h->abs_count, h->abs_start, h->abs_partial};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (h->cnt_host) hipHostFree(h->cnt_host);
@@ -1388,9 +1402,9 @@ int pic_run(pic_handle *h) {
     if (ref_steady(h)) {
         steady_boundary(h);
     } else {
-        absorb_species(h, 1, plop_on(h));
+        CHECK(absorb_species(h, 1, plop_on(h)));
         wall_pass(h, 1);
-        absorb_species(h, 0, false);
+        CHECK(absorb_species(h, 0, false));
         wall_pass(h, 0);
     }
     if (h->spawning && (plop_on(h) || h->wall_on)) {
@@ -1659,7 +1673,7 @@ int pic_boundary(pic_handle *h, int32_t which) {
     }
     bool plop = which == 0 && plop_on(h);
     const int si = which == 0 ? 1 : 0;
-    absorb_species(h, si, plop);
+    CHECK(absorb_species(h, si, plop));
     wall_pass(h, si);
     if (plop || h->wall_on) {
         commit_both(h, 0, 0);
@@ -1935,7 +1949,12 @@ int pic_estimate_bytes(const pic_params *pp, int64_t *bytes) {
     {  // sort histogram (one counter per 128-byte line) and starts
         const int ks = P.sort_key >= 0 ? P.sort_key : tuned_row(N).sort_key;
         const int64_t nb = scan_pad((uint32_t)((tiles * tiles * 64) >> ks));
-        b += nb * BIN_STRIDE * 4 + (nb + 1) * 4 + (nb / 4096 + 1) * 4;
+        b += 2 * nb * BIN_STRIDE * 4 + (nb + 1) * 4 + (nb / 4096 + 1) * 4;  // per species
+    }
+    {  // the absorbing walls' compaction: block counts and starts
+        const int64_t cap = n0 + (spawning ? std::max<int64_t>(n0 / 2, 65536) : 0) + 4096 * n_inj;
+        const int64_t nb = absorb_blocks_pad(std::max<int64_t>(cap, 1));
+        b += (2 * nb + 1 + nb / 4096 + 1) * 4;
     }
     b += N * N * 16 + (N + 1) * (N + 1) * 16 + ((N + 1) * N + 1) * 8;  // rho, phi, E
     b += (int64_t)(60.0 * 4.0 / 3.0 * (double)(N * N)) + (1 << 20);   // multigrid levels

@@ -505,13 +505,16 @@ __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t *t
     return pre + x - v;
 }
 
+// the counts are S words apart (BIN_STRIDE for the sort histogram, 1 for
+// the compaction's block counts)
+template <int S>
 __global__ void __launch_bounds__(SCAN_THREADS)
     k_scan_partial(const uint32_t *__restrict__ in, uint32_t *__restrict__ partial) {
     const uint32_t *p =
-        in + ((size_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD) * BIN_STRIDE;
+        in + ((size_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD) * S;
     uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_PER_THREAD; k++) s += p[k * BIN_STRIDE];
+    for (int k = 0; k < SCAN_PER_THREAD; k++) s += p[k * S];
     uint32_t tot;
     block_excl_scan(s, &tot);
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
@@ -535,14 +538,15 @@ __global__ void __launch_bounds__(SCAN_THREADS)
     }
 }
 
+template <int S>
 __global__ void __launch_bounds__(SCAN_THREADS)
     k_scan_final(const uint32_t *__restrict__ in, const uint32_t *__restrict__ partial,
                  uint32_t *__restrict__ out) {
     size_t base = (size_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
     uint32_t v[SCAN_PER_THREAD];
-    const uint32_t *p = in + base * BIN_STRIDE;
+    const uint32_t *p = in + base * S;
 #pragma unroll
-    for (int k = 0; k < SCAN_PER_THREAD; k++) v[k] = p[k * BIN_STRIDE];
+    for (int k = 0; k < SCAN_PER_THREAD; k++) v[k] = p[k * S];
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_PER_THREAD; k++) s += v[k];
@@ -564,13 +568,20 @@ __global__ void __launch_bounds__(SCAN_THREADS)
     }
 }
 
-void launch_scan(hipStream_t st, const uint32_t *count, uint32_t *start, uint32_t nbins_pad,
-                 uint32_t *partial, uint32_t *n_out) {
+template <int S>
+static void launch_scan_s(hipStream_t st, const uint32_t *count, uint32_t *start,
+                          uint32_t nbins_pad, uint32_t *partial, uint32_t *n_out) {
     int nparts = (int)(nbins_pad / SCAN_TILE);
-    hipLaunchKernelGGL(k_scan_partial, dim3(nparts), dim3(SCAN_THREADS), 0, st, count, partial);
+    hipLaunchKernelGGL(k_scan_partial<S>, dim3(nparts), dim3(SCAN_THREADS), 0, st, count, partial);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, st, partial, nparts,
                        start + nbins_pad, n_out);
-    hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(SCAN_THREADS), 0, st, count, partial, start);
+    hipLaunchKernelGGL(k_scan_final<S>, dim3(nparts), dim3(SCAN_THREADS), 0, st, count, partial,
+                       start);
+}
+
+void launch_scan(hipStream_t st, const uint32_t *count, uint32_t *start, uint32_t nbins_pad,
+                 uint32_t *partial, uint32_t *n_out) {
+    launch_scan_s<BIN_STRIDE>(st, count, start, nbins_pad, partial, n_out);
 }
 
 uint32_t scan_pad(uint32_t nbins) { return (nbins + SCAN_TILE - 1) / SCAN_TILE * SCAN_TILE; }
@@ -1568,62 +1579,108 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// wall pass with stream compaction into the other buffer (boundary.cpp:581-677,
-// 765-824 with zero yields).  Survivor order is not preserved.  With
-// spn.plop, every ion leaving through a wall stages a re-injected pair.
-__global__ void __launch_bounds__(256)
-    k_absorb(SpView src, SpView dst, const uint32_t *__restrict__ nptr, uint32_t *__restrict__ nout,
-             int N, int mode, uint32_t *__restrict__ lost6, mcc::MccConst mc, uint32_t step,
-             Spawn spn) {
-    __shared__ uint32_t sl[6];
-    __shared__ uint32_t wbase[4], wcnt[4], bbase;
-    if (threadIdx.x < 6) sl[threadIdx.x] = 0;
-    __syncthreads();
-    uint32_t n = *nptr;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool keep = false;
-    int st = -1;
+// Wall pass of the absorbing walls (pic_run's pre-loop pass, pic_boundary;
+// boundary.cpp:581-677, 765-824 with zero yields): the particles inside are
+// kept, in store order, the others counted as losses (and, with spn.plop,
+// re-injected as pairs, or queued for the surface pass).  An
+// order-preserving compaction in three launches: k_absorb_count counts each
+// block's survivors (12 B per particle read), a scan gives every block its
+// output offset, k_absorb writes the survivors there in order (the store
+// stays sorted if it was).
+#define ABS_IPP 4
+__device__ __forceinline__ int absorb_class(const SpView &src, uint32_t i, int N, int mode) {
     PState s{};
-    uint32_t pid = 0;
-    float wt = 1.0f;
-    if (i < n) {
-        s = load_state(src, i, true);
-        pid = src.id[i];
-        wt = load_w(src, i);
-        st = classify(s, N, mode);
-        keep = (st == ST_KEPT);
-        if (!keep && st != ST_HOLE) atomicAdd(&sl[st - 1], 1u);
+    const uint32_t c = src.cell[i];
+    s.jx = cell_jx(c);
+    s.jy = cell_jy(c);
+    if (s.jx == CELL_HOLE) return ST_HOLE;
+    s.ox = src.ox[i];
+    s.oy = src.oy[i];
+    return classify(s, N, mode);
+}
+
+// the grid covers every count the scan reads (blocks past the live count
+// write 0)
+__global__ void __launch_bounds__(256)
+    k_absorb_count(SpView src, const uint32_t *__restrict__ nptr, int N, int mode,
+                   uint32_t *__restrict__ bcount) {
+    const uint32_t n = *nptr;
+    const uint32_t base = blockIdx.x * (256 * ABS_IPP) + threadIdx.x;
+    int c = 0;
+#pragma unroll
+    for (int r = 0; r < ABS_IPP; r++) {
+        const uint32_t i = base + r * 256;
+        if (i < n && absorb_class(src, i, N, mode) == ST_KEPT) c++;
     }
-    // block-level compaction: one global atomic per block
-    uint64_t bal = __ballot(keep);
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t rank = __popcll(bal & ((1ull << lane) - 1ull));
-    if (lane == 0) wcnt[w] = __popcll(bal);
+    c = wave_sum(c);
+    __shared__ int wc[4];
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = c;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
-            wbase[k] = t;
-            t += wcnt[k];
+    if (threadIdx.x == 0) bcount[blockIdx.x] = (uint32_t)(wc[0] + wc[1] + wc[2] + wc[3]);
+}
+
+__global__ void __launch_bounds__(256)
+    k_absorb(SpView src, SpView dst, const uint32_t *__restrict__ nptr,
+             const uint32_t *__restrict__ bstart, int N, int mode, uint32_t *__restrict__ lost6,
+             mcc::MccConst mc, uint32_t step, Spawn spn) {
+    const uint32_t n = *nptr;
+    if (blockIdx.x * (256 * ABS_IPP) >= n) return;  // past the live count
+    __shared__ uint32_t sl[6];
+    __shared__ uint32_t cnt[ABS_IPP][4];
+    if (threadIdx.x < 6) sl[threadIdx.x] = 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t base = blockIdx.x * (256 * ABS_IPP) + threadIdx.x;
+    PState s[ABS_IPP];
+    uint32_t pid[ABS_IPP], rank[ABS_IPP];
+    float wt[ABS_IPP];
+    int st[ABS_IPP];
+#pragma unroll
+    for (int r = 0; r < ABS_IPP; r++) {
+        const uint32_t i = base + r * 256;
+        s[r] = PState{};
+        pid[r] = 0u;
+        wt[r] = 1.0f;
+        st[r] = ST_HOLE;
+        if (i < n) {
+            s[r] = load_state(src, i, true);
+            pid[r] = src.id[i];
+            wt[r] = load_w(src, i);
+            st[r] = classify(s[r], N, mode);
         }
-        bbase = t ? atomicAdd(nout, t) : 0u;
+        const uint64_t bal = __ballot(st[r] == ST_KEPT);
+        rank[r] = (uint32_t)__popcll(bal & below);
+        if (lane == 0) cnt[r][w] = (uint32_t)__popcll(bal);
     }
     __syncthreads();
-    if (keep) {
-        store_state(dst, bbase + wbase[w] + rank, s, pid);
-        if (dst.w) dst.w[bbase + wbase[w] + rank] = wt;
+    // store order: round r's particles follow round r-1's, waves in order
+#pragma unroll
+    for (int r = 0; r < ABS_IPP; r++) {
+        uint32_t pre = bstart[blockIdx.x];
+        for (int k = 0; k < ABS_IPP * 4; k++)
+            if (k < r * 4 + w) pre += cnt[k >> 2][k & 3];
+        if (st[r] == ST_KEPT) {
+            const uint32_t o = pre + rank[r];
+            store_state(dst, o, s[r], pid[r]);
+            if (dst.w) dst.w[o] = wt[r];
+        } else if (st[r] != ST_HOLE) {
+            atomicAdd(&sl[st[r] - 1], 1u);
+            if (spn.plop && st[r] >= ST_LEFT && st[r] <= ST_TOP) {
+                PState pe, pi;
+                mcc::plop_pair(mc, pid[r], step, N, spn.se_sigma, spn.si_sigma, spn.plop_ref, pe,
+                               pi);
+                const unsigned long long k = (1ull << 32) | pid[r];
+                stage_put(spn.se, &spn.ce->stage_count, &spn.ce->overflow, pe, k, wt[r]);
+                stage_put(spn.si, &spn.ci->stage_count, &spn.ci->overflow, pi, k, wt[r]);
+            }
+            if (spn.wall && st[r] >= ST_LEFT && st[r] <= ST_TOP)
+                stage_put(spn.wq, &spn.cw->wall_count, &spn.cw->overflow, s[r],
+                          (unsigned long long)pid[r], wt[r]);
+        }
     }
+    __syncthreads();
     if (threadIdx.x < 6 && sl[threadIdx.x])
         atomicAdd(&lost6[cnt_slot() * 6 + threadIdx.x], sl[threadIdx.x]);
-    if (spn.plop && st >= ST_LEFT && st <= ST_TOP) {
-        PState pe, pi;
-        mcc::plop_pair(mc, pid, step, N, spn.se_sigma, spn.si_sigma, spn.plop_ref, pe, pi);
-        unsigned long long k = (1ull << 32) | pid;
-        stage_put(spn.se, &spn.ce->stage_count, &spn.ce->overflow, pe, k, wt);
-        stage_put(spn.si, &spn.ci->stage_count, &spn.ci->overflow, pi, k, wt);
-    }
-    if (spn.wall && st >= ST_LEFT && st <= ST_TOP)
-        stage_put(spn.wq, &spn.cw->wall_count, &spn.cw->overflow, s, (unsigned long long)pid, wt);
 }
 
 // Sort (key, val) pairs [0, c) ascending by key in one workgroup: bitonic
@@ -2183,11 +2240,19 @@ void launch_deposit(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t
     hipLaunchKernelGGL(k_deposit, dim3(blocks_for(cap, 256)), dim3(256), 0, st, sp, n, N, mode, q);
 }
 
+uint32_t absorb_blocks_pad(int64_t cap) {
+    return scan_pad((uint32_t)blocks_for(cap, 256 * ABS_IPP));
+}
+
 void launch_absorb(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
                    uint32_t *nout, int64_t cap, int N, int mode, uint32_t *lost6,
-                   const mcc::MccConst &mc, uint32_t step, const Spawn &spn) {
-    hipLaunchKernelGGL(k_absorb, dim3(blocks_for(cap, 256)), dim3(256), 0, st, src, dst, n, nout,
-                       N, mode, lost6, mc, step, spn);
+                   const mcc::MccConst &mc, uint32_t step, const Spawn &spn, uint32_t *bcount,
+                   uint32_t *bstart, uint32_t *partial) {
+    const uint32_t nb = absorb_blocks_pad(cap);
+    hipLaunchKernelGGL(k_absorb_count, dim3(nb), dim3(256), 0, st, src, n, N, mode, bcount);
+    launch_scan_s<1>(st, bcount, bstart, nb, partial, nout);
+    hipLaunchKernelGGL(k_absorb, dim3(blocks_for(cap, 256 * ABS_IPP)), dim3(256), 0, st, src, dst,
+                       n, (const uint32_t *)bstart, N, mode, lost6, mc, step, spn);
 }
 
 void launch_steady(hipStream_t st, const SteadyArgs &a, const mcc::MccConst &mc, uint32_t step,

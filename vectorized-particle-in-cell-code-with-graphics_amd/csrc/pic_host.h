@@ -91,6 +91,8 @@ struct pic_handle {
     bool ovf_pending = false;
     uint64_t ovf_reported = 0;
     uint32_t *binCount = nullptr, *binStart = nullptr, *scanPartial = nullptr;
+    // the absorbing walls' compaction: block counts, starts, scan partials
+    uint32_t *abs_count = nullptr, *abs_start = nullptr, *abs_partial = nullptr;
     double *rho = nullptr, *phi = nullptr, *ex64 = nullptr, *ey64 = nullptr;
     float2 *E32 = nullptr;
     float *ntab = nullptr;       // reference neutral velocity table (collisions.cpp:66-89)

@@ -211,7 +211,10 @@ void launch_deposit(hipStream_t st, const SpView &sp, const uint32_t *n, int64_t
                     int mode, unsigned long long *q);
 void launch_absorb(hipStream_t st, const SpView &src, const SpView &dst, const uint32_t *n,
                    uint32_t *nout, int64_t cap, int N, int mode, uint32_t *lost6,
-                   const mcc::MccConst &mc, uint32_t step, const Spawn &spn);
+                   const mcc::MccConst &mc, uint32_t step, const Spawn &spn, uint32_t *bcount,
+                   uint32_t *bstart, uint32_t *partial);
+// block counts (and starts, + 1) the compaction of a store of `cap` needs
+uint32_t absorb_blocks_pad(int64_t cap);
 // reference steady-state walls: both axes' pairing passes (k_particles.hip)
 void launch_steady(hipStream_t st, const SteadyArgs &a, const mcc::MccConst &mc, uint32_t step,
                    int N, float se, float si, unsigned long long *qe, unsigned long long *qi,
