@@ -462,3 +462,34 @@ def test_snapshot_subset_and_graph_maxima(tmp_path):
         np.testing.assert_allclose(phi, sim.get_phi(), rtol=1e-6, atol=1e-12)
         e = np.loadtxt(tmp_path / f"snapshot_{step}_e.txt")
         assert e.shape == (len(sim.particle_subset(0, 5)), 6)
+
+
+def test_absorbing_compaction_keeps_store_order():
+    """The absorbing walls' compaction (pic_boundary, pic_run's pre-loop
+    pass) keeps the survivors in store order, so a sorted store stays
+    sorted, and counts every removed particle at its wall.  Particles are
+    moved outside on both x walls and the bottom wall in a pattern that
+    spans many 1024-particle blocks."""
+    p = workloads.discharge(64, 40, te=6.0)
+    N = p.nodes
+    print(f"absorbing compaction: {N}^2 nodes, {p.init_particle_no} particles", flush=True)
+    with pic2d.Pic(p) as sim:
+        d = sim.get_state(0)
+        n = len(d["id"])
+        k = np.arange(n)
+        jx, jy = pic2d.cell_unpack(d["cell"])
+        left, right = k % 7 == 0, (k % 11 == 0) & (k % 7 != 0)
+        bottom = (k % 13 == 0) & ~left & ~right
+        jx = np.where(left, -1, np.where(right, N - 1, jx))
+        jy = np.where(bottom, -1, jy)
+        d["cell"] = pic2d.cell_pack(jx, jy)
+        sim.set_state(0, d)
+        l0 = sim.loss_counts(0)
+        sim.boundary(1)  # the electron wall
+        out = sim.get_state(0)
+        dl = sim.loss_counts(0) - l0
+    keep = ~(left | right | bottom)
+    assert np.array_equal(out["id"], d["id"][keep])
+    for key in ("cell", "ox", "oy", "vx", "vy", "vz"):
+        assert np.array_equal(out[key], d[key][keep]), key
+    assert dl[0] == left.sum() and dl[1] == right.sum() and dl[2] == bottom.sum() and dl[3] == 0
