@@ -97,8 +97,9 @@ struct MgLevels {
     int ng = 0, lc = 0;
     MgArrays<double> d;  // FP64 full multigrid (fixed mode) and the FP64 iterate
     MgArrays<float> f;   // FP32 correction solves (tolerance mode)
-    // tolerance mode state (device): flags[0] done, flags[1] V-cycles applied
-    int *flags = nullptr;        // [2]: blocks done in the current residual pass
+    // tolerance mode state (device): flags[0] done, flags[1] V-cycles applied,
+    // flags[3] residual passes published ([2] unused)
+    int *flags = nullptr;
     double *partials = nullptr;  // per-block sums of r^2 and f^2
     double *ualt = nullptr;      // second FP64 iterate: cycles alternate u <-> ualt
     float *r32alt = nullptr;     // second FP32 residual: cycles alternate with f.rhsF[top]
