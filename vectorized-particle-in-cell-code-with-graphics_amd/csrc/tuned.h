@@ -24,7 +24,7 @@ static const TunedRow TUNED[] = {
     {129, {6.809, 0.8}, 5, 2},
     {257, {6.809, 0.8}, 6, 2},
     {513, {6.809, 0.8}, 6, 3},
-    {1025, {6.809, 0.8}, 6, 2}
+    {1025, {10.213, 0.8}, 6, 2}
 };
 
 inline const TunedRow &tuned_row(int nodes) {
