@@ -1185,6 +1185,17 @@ __device__ __forceinline__ void inplace_round(const SpView &sp, uint32_t i, uint
 // load and store stays coalesced): one deposit window and one flush of the
 // block's counters then serve 256 * IPP particles.  Reads 20-28 B and writes
 // 20-24 B per particle, with no histogram or slot claims.
+// LDS of an in-place pass (u64 words): the collision queue, then the
+// deposit window, then the sort histogram.  The electrons' collisional
+// variant runs at 5 waves per SIMD (its registers): 28 KB fit five blocks
+// per CU and give its deposit a 58 x 58 fallback window for particles that
+// moved up to three steps since their store was sorted; the others, at 8
+// waves, keep 16 KB.
+template <int V>
+constexpr int inplace_lds_u64() {
+    return V == PV_COLL_E ? 3584 : BLOCK_LDS_U64;
+}
+
 template <int V>
 __device__ __forceinline__ void push_inplace_body(
     SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E, PushConst pc,
@@ -1200,7 +1211,8 @@ __device__ __forceinline__ void push_inplace_body(
     __shared__ uint32_t sl[6];
     __shared__ uint32_t shole;
     __shared__ uint32_t se[8];
-    __shared__ unsigned long long lds[BLOCK_LDS_U64];
+    constexpr int LDS_U64 = inplace_lds_u64<V>();
+    __shared__ unsigned long long lds[LDS_U64];
     if (threadIdx.x < 6) sl[threadIdx.x] = 0;
     if (threadIdx.x < 8) se[threadIdx.x] = 0;
     if (threadIdx.x == 0) shole = 0;
@@ -1256,7 +1268,7 @@ __device__ __forceinline__ void push_inplace_body(
                   V == PV_DEFER);
     // block_deposit_n passes a barrier before it writes the window, so every
     // round has read its collision result from the shared LDS by then
-    if (deposit) block_deposit_n<IPP>(dep, s, w, q, pc.N, lds);
+    if (deposit) block_deposit_n<IPP, LDS_U64>(dep, s, w, q, pc.N, lds);
     __syncthreads();
     // ahead of a sort pass: the histogram of the cells the particles end the
     // step in, the keys that pass claims by (no k_key_count then)
