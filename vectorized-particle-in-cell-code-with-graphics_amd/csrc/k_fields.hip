@@ -8,6 +8,7 @@
 // LDS; finer levels use one launch per operation.
 #include <chrono>
 #include <cmath>
+#include <thread>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -1260,7 +1261,10 @@ static void vcycle_top(hipStream_t st, MgLevels &M, int j, const float *r_in, co
 // as the last pass's final block writes it, so the next launches queue up
 // behind the work still running (phi and E) instead of after an idle GPU.
 // After 2 ms of spinning (or if the device never publishes, e.g. on a
-// fault) it blocks on the stream's event, which also reports errors.
+// fault) it blocks on the stream's event, which also reports errors.  The
+// spin yields the core between reads: with many runs stepped from their own
+// threads (ensembles, serving) the waiting threads leave the cores to the
+// threads that are launching.
 static hipError_t wait_published(hipStream_t st, MgLevels &M, int passes) {
     hipError_t e = hipEventRecord(M.done, st);
     if (e != hipSuccess) return e;
@@ -1270,6 +1274,7 @@ static hipError_t wait_published(hipStream_t st, MgLevels &M, int passes) {
         const int p = w & MG_PUB_COUNT;
         if (p >= passes || (p > 0 && (w & MG_PUB_CONV))) return hipSuccess;
         if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(2000)) break;
+        std::this_thread::yield();
     }
     return hipEventSynchronize(M.done);
 }
