@@ -876,6 +876,17 @@ int absorb_species(pic_handle *h, int si, bool plop) {
 // fused step's field_start has already formed the rhs, the extrapolated start
 // and the flag reset, and phi/E are queued before the host waits for the
 // last cycles (they take the final iterate by the cycle count's parity).
+// the verdict of a one-workgroup tolerance solve queued without a wait:
+// read before anything reports the solver's figures or starts another solve
+int mg_harvest(pic_handle *h) {
+    if (!h->M.pending) return PIC_OK;
+    const int used = mg_tol_verdict(h->M, &h->mg_relres);
+    if (used < 0) return seterr(h, PIC_ERR_DEVICE, "multigrid: stream error");
+    h->mg_cycles = used;
+    h->mg_cycles_total += used;
+    return PIC_OK;
+}
+
 int solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, double phiLY,
                     bool with_e = false, bool timestep = false) {
     const pic_params &P = h->P;
@@ -897,11 +908,15 @@ int solve_potential(pic_handle *h, double phi0X, double phiR, double phi0Y, doub
                 launch_phi_efield(h->st, u0, N, phi0X, phiR, phi0Y, phiLY, h->dx, h->phi,
                                   h->ex64, h->ey64, h->E32, u1, M.flags);
             };
+        CHECK(mg_harvest(h));
         int used = mg_tol_solve(h->st, M, P.length, P.mg_tol, P.mg_max_iter, &h->mg_relres,
                                 timestep, phi_e);
-        if (used < 0) return seterr(h, PIC_ERR_DEVICE, "multigrid: stream error");
-        h->mg_cycles = used;
-        h->mg_cycles_total += used;
+        if (used < 0 && used != MG_PENDING)
+            return seterr(h, PIC_ERR_DEVICE, "multigrid: stream error");
+        if (used >= 0) {
+            h->mg_cycles = used;
+            h->mg_cycles_total += used;
+        }
         queued_e = early;
     } else {
         mg_fixed_solve(h->st, M, P.mg_ncycle, P.length);
@@ -1039,6 +1054,7 @@ int fused_step(pic_handle *h) {
     if (h->acc_every > 0 && (h->step + 1) % h->acc_every == 0) {
         double e[6];
         CHECK(field_error(h, h->acc_tol, e));
+        CHECK(mg_harvest(h));
         const double row[6] = {(double)h->step, h->mg_relres, e[0], e[1], e[2], e[3]};
         h->acc_rows.insert(h->acc_rows.end(), row, row + 6);
     }
@@ -1442,6 +1458,7 @@ int pic_advance(pic_handle *h, int32_t nsteps) {
 int pic_sync(pic_handle *h) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
     HIPCHK(h, hipStreamSynchronize(h->st));
+    CHECK(mg_harvest(h));
     return check_overflow(h, true);
 }
 
@@ -1729,6 +1746,7 @@ int pic_poisson_tol(const double *f, double *u, int32_t n, double length, double
         hipMemcpyAsync(M.d.u[M.ng - 1], u, bytes, hipMemcpyHostToDevice, st);
         double rr = 0.0;
         int used = mg_tol_solve(st, M, length, tol, max_iter, &rr);
+        if (used == MG_PENDING) used = mg_tol_verdict(M, &rr);
         hipMemcpyAsync(u, M.d.u[M.ng - 1], bytes, hipMemcpyDeviceToHost, st);
         hipError_t e = hipStreamSynchronize(st);
         if (e != hipSuccess || used < 0)
@@ -1743,6 +1761,7 @@ int pic_poisson_tol(const double *f, double *u, int32_t n, double length, double
 
 int pic_get_mg_stats(pic_handle *h, int32_t *last_cycles, int64_t *total_cycles, double *relres) {
     if (!h) return set_global_error(PIC_ERR_ARG, "null handle");
+    CHECK(mg_harvest(h));
     if (last_cycles) *last_cycles = h->mg_cycles;
     if (total_cycles) *total_cycles = h->mg_cycles_total;
     if (relres) *relres = h->mg_relres;
@@ -2360,6 +2379,7 @@ int pic_checkpoint_write(pic_handle *h, const char *path) {
     HIPCHK(h, hipStreamSynchronize(h->st));
     CHECK(check_overflow(h, true));
     CHECK(read_counters(h));
+    CHECK(mg_harvest(h));
     CkHeader H;
     memset(&H, 0, sizeof(H));
     memcpy(H.magic, CK_MAGIC, 8);

@@ -974,10 +974,20 @@ int mg_coarse_level_for(long lds_bytes, int ng) {
     return lc;
 }
 
+__global__ void __launch_bounds__(1024)
+    k_tol_solve_dst(double *__restrict__ u0, double *__restrict__ u1,
+                    const double *__restrict__ f, int n, double h2i, float *__restrict__ r32,
+                    float *__restrict__ e32, int lc, float h2, const float *__restrict__ Sg,
+                    const float *__restrict__ Dg, double tol, int max_cycles,
+                    int *__restrict__ flags, double *__restrict__ relres_host,
+                    int *__restrict__ host_flags);
+
 void mg_set_coarse_attr(long lds_bytes) {
     hipFuncSetAttribute((const void *)k_mg_coarse<double>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     hipFuncSetAttribute((const void *)k_mg_coarse_dst, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds_bytes);
+    hipFuncSetAttribute((const void *)k_tol_solve_dst, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds_bytes);
     hipFuncSetAttribute((const void *)k_mg_coarse<float>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
@@ -1265,9 +1275,7 @@ static void vcycle_top(hipStream_t st, MgLevels &M, int j, const float *r_in, co
 // spin yields the core between reads: with many runs stepped from their own
 // threads (ensembles, serving) the waiting threads leave the cores to the
 // threads that are launching.
-static hipError_t wait_published(hipStream_t st, MgLevels &M, int passes) {
-    hipError_t e = hipEventRecord(M.done, st);
-    if (e != hipSuccess) return e;
+static hipError_t wait_flags(MgLevels &M, int passes) {
     const auto t0 = std::chrono::steady_clock::now();
     for (;;) {
         const int w = __atomic_load_n(&M.host_flags[2], __ATOMIC_ACQUIRE);
@@ -1277,6 +1285,120 @@ static hipError_t wait_published(hipStream_t st, MgLevels &M, int passes) {
         std::this_thread::yield();
     }
     return hipEventSynchronize(M.done);
+}
+
+static hipError_t wait_published(hipStream_t st, MgLevels &M, int passes) {
+    hipError_t e = hipEventRecord(M.done, st);
+    if (e != hipSuccess) return e;
+    return wait_flags(M, passes);
+}
+
+int mg_tol_verdict(MgLevels &M, double *relres_out) {
+    const int passes = M.pending;
+    if (passes <= 0) return -1;
+    M.pending = 0;
+    if (wait_flags(M, passes) != hipSuccess) return -1;
+    const bool conv = (__atomic_load_n(&M.host_flags[2], __ATOMIC_ACQUIRE) & MG_PUB_CONV) != 0;
+    const int used = M.host_flags[1];
+    M.pred = used + (conv ? 0 : 1);
+    if (relres_out) *relres_out = *M.relres;
+    return used;
+}
+
+// The whole tolerance solve of a grid whose finest level is the DST level
+// (65^2 and below: C1, the small runs of ensembles and serving), in one
+// workgroup of 1024 threads: the initial residual pass, then per cycle the
+// exact DST solve of the FP32 correction and the fused FP64 update and
+// residual -- the arithmetic of k_mg_coarse_dst and k_tol_update node for
+// node, the iterates alternating between u0 and u1 as in the launched
+// cycles -- until ||f - Au|| <= tol ||f|| or max_cycles cycles.  r32 and e32
+// stay in global memory (L2-resident; one workgroup's writes are seen by its
+// threads after a barrier).  The final iterate is copied to u0 when it ends
+// in u1, so the caller's solution buffer never changes; the verdict is
+// published as k_tol_finish publishes it, once, at the end.  One launch
+// instead of three per cycle, and no host wait: these grids are bound by
+// launch latency.
+__device__ __forceinline__ double block_sum_1024(double v, double *ws) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) ws[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += ws[k];  // fixed order
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ double small_pass(const double *__restrict__ u_in,
+                                             const float *__restrict__ e,
+                                             double *__restrict__ u_out,
+                                             const double *__restrict__ f, int n, double h2i,
+                                             float *__restrict__ r32, double *ws, double &ff_out) {
+    double rr = 0.0, ff = 0.0;
+    for (int k = (int)threadIdx.x; k < n * n; k += (int)blockDim.x) {
+        const int i = k / n, j = k - (k / n) * n;
+        double v = u_in[k];
+        float out = 0.0f;
+        if (i > 0 && j > 0 && i < n - 1 && j < n - 1) {
+            double un = u_in[k + n], us = u_in[k - n], ue = u_in[k + 1], uw = u_in[k - 1];
+            if (e) {
+                v += (double)e[k];
+                un += (double)e[k + n];
+                us += (double)e[k - n];
+                ue += (double)e[k + 1];
+                uw += (double)e[k - 1];
+            }
+            const double fk = f[k];
+            // the stencil of resid_node: north, south, east, west
+            const double r = -h2i * (un + us + ue + uw - 4.0 * v) + fk;
+            rr += r * r;
+            out = (float)r;
+            ff += fk * fk;
+        }
+        if (u_out) u_out[k] = v;
+        r32[k] = out;
+    }
+    ff_out = block_sum_1024(ff, ws);
+    return block_sum_1024(rr, ws);
+}
+
+__global__ void __launch_bounds__(1024)
+    k_tol_solve_dst(double *__restrict__ u0, double *__restrict__ u1,
+                    const double *__restrict__ f, int n, double h2i, float *__restrict__ r32,
+                    float *__restrict__ e32, int lc, float h2, const float *__restrict__ Sg,
+                    const float *__restrict__ Dg, double tol, int max_cycles,
+                    int *__restrict__ flags, double *__restrict__ relres_host,
+                    int *__restrict__ host_flags) {
+    // no static LDS: the attribute grants the dynamic region the whole
+    // per-workgroup limit, so the reduction slots sit behind the DST matrices
+    extern __shared__ double lds_raw[];
+    double *ws = lds_raw + dst_lds_floats(1 << (lc + 1)) / 2;
+    double *buf[2] = {u0, u1};
+    double ff = 0.0;
+    double rr = small_pass(u0, nullptr, nullptr, f, n, h2i, r32, ws, ff);
+    double rel = (ff > 0.0) ? sqrt(rr / ff) : sqrt(rr);
+    int q = 0;
+    while (!(rel <= tol) && q < max_cycles) {  // uniform: every thread holds the same sums
+        __syncthreads();  // r32 written
+        dst_solve(r32, e32, lc, h2, Sg, Dg, lds_raw);
+        __syncthreads();  // e32 written
+        rr = small_pass(buf[q & 1], e32, buf[(q + 1) & 1], f, n, h2i, r32, ws, ff);
+        rel = (ff > 0.0) ? sqrt(rr / ff) : sqrt(rr);
+        q++;
+    }
+    if (q & 1)  // each thread copies the nodes it wrote in the last pass
+        for (int k = (int)threadIdx.x; k < n * n; k += (int)blockDim.x) u0[k] = u1[k];
+    if (threadIdx.x == 0) {
+        const int conv = rel <= tol;
+        flags[0] = conv;
+        flags[1] = q;
+        flags[3] = 1 + q;
+        *relres_host = rel;
+        host_flags[0] = conv;
+        host_flags[1] = q;
+        __threadfence_system();
+        host_flags[2] = (1 + q) | (conv ? MG_PUB_CONV : 0);
+    }
 }
 
 int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max_iter,
@@ -1317,8 +1439,23 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     };
     // nothing on the device writes the host copies now (the last solve's
     // remaining cycles were skipped): clear them for this solve's verdict
+    if (M.pending && mg_tol_verdict(M, nullptr) < 0) return -1;
     __atomic_store_n(&M.host_flags[2], 0, __ATOMIC_RELEASE);
     M.host_flags[0] = M.host_flags[1] = 0;
+    if (lf == M.lc) {
+        // the whole solve in one workgroup (k_tol_solve_dst)
+        const int m = 1 << (M.lc + 1);
+        const double hc = length / (double)m;
+        hipLaunchKernelGGL(k_tol_solve_dst, dim3(1), dim3(1024),
+                           (size_t)dst_lds_floats(m) * sizeof(float) + 16 * sizeof(double), st,
+                           u, M.ualt, f, n, h2i,
+                           r32, e32, M.lc, (float)(hc * hc), M.dst_S, M.dst_D, tol, max_iter,
+                           M.flags, M.relres, M.host_flags);
+        if (after_batch) after_batch(u, u);
+        if (hipEventRecord(M.done, st) != hipSuccess) return -1;
+        M.pending = 1 + max_iter;
+        return MG_PENDING;
+    }
     if (!flags_reset) hipLaunchKernelGGL(k_tol_reset, dim3(1), dim3(1), 0, st, M.flags);
     hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st, (const double *)u,
                        (const float *)nullptr, (double *)nullptr, f, n, h2i, r32, M.partials,

@@ -109,6 +109,7 @@ struct MgLevels {
     double *relres = nullptr;    // pinned host: last relative residual
     int *host_flags = nullptr;   // pinned host copy of flags
     int pred = 4;                // predicted V-cycles per solve
+    int pending = 0;             // passes a queued one-workgroup solve publishes (0: none)
     // exact coarsest-level solve of the tolerance mode: discrete sine
     // transform tables of level lc, M x M with M = 2^(lc+1) (padded by one)
     float *dst_S = nullptr;      // S[p][q] = sin(pi (p+1)(q+1) / M)
@@ -360,6 +361,14 @@ int field_diff_sums(hipStream_t st, const double *a0, const double *b0, const do
 int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max_iter,
                  double *relres_out, bool flags_reset = false,
                  const std::function<void(const double *, const double *)> &after_batch = nullptr);
+// A grid whose finest level is the DST level (lf == lc) is solved whole by
+// one kernel that leaves the final iterate in M.d.u[ng-1]: mg_tol_solve then
+// queues it, and after_batch, without waiting and returns MG_PENDING;
+// mg_tol_verdict waits for that solve's verdict and returns its V-cycles (-1
+// on a HIP error or with no solve pending).  mg_tol_solve reads a pending
+// verdict itself (and drops it) before it starts another solve.
+constexpr int MG_PENDING = -2;
+int mg_tol_verdict(MgLevels &M, double *relres_out);
 
 
 }  // namespace pic
