@@ -90,6 +90,7 @@ long lds_limit() {
 int alloc_mg(pic_handle *h, MgLevels &M, int N) {
     M.ng = levels_of(N);
     long lds = lds_limit();
+    M.lds = lds;
     M.lc = mg_coarse_level_for(lds, M.ng);
     mg_set_coarse_attr(lds);
     if (mg_dst_tables(M) != 0) return seterr(h, PIC_ERR_ALLOC, "multigrid: coarse tables");

@@ -975,12 +975,10 @@ int mg_coarse_level_for(long lds_bytes, int ng) {
 }
 
 __global__ void __launch_bounds__(1024)
-    k_tol_solve_dst(double *__restrict__ u0, double *__restrict__ u1,
-                    const double *__restrict__ f, int n, double h2i, float *__restrict__ r32,
-                    float *__restrict__ e32, int lc, float h2, const float *__restrict__ Sg,
-                    const float *__restrict__ Dg, double tol, int max_cycles,
-                    int *__restrict__ flags, double *__restrict__ relres_host,
-                    int *__restrict__ host_flags);
+    k_tol_solve_dst(double *__restrict__ ug, const double *__restrict__ fg, int n, double h2i,
+                    int lc, float h2, const float *__restrict__ Sg, const float *__restrict__ Dg,
+                    double tol, int max_cycles, int *__restrict__ flags,
+                    double *__restrict__ relres_host, int *__restrict__ host_flags);
 
 void mg_set_coarse_attr(long lds_bytes) {
     hipFuncSetAttribute((const void *)k_mg_coarse<double>,
@@ -1307,17 +1305,25 @@ int mg_tol_verdict(MgLevels &M, double *relres_out) {
 
 // The whole tolerance solve of a grid whose finest level is the DST level
 // (65^2 and below: C1, the small runs of ensembles and serving), in one
-// workgroup of 1024 threads: the initial residual pass, then per cycle the
-// exact DST solve of the FP32 correction and the fused FP64 update and
-// residual -- the arithmetic of k_mg_coarse_dst and k_tol_update node for
-// node, the iterates alternating between u0 and u1 as in the launched
-// cycles -- until ||f - Au|| <= tol ||f|| or max_cycles cycles.  r32 and e32
-// stay in global memory (L2-resident; one workgroup's writes are seen by its
-// threads after a barrier).  The final iterate is copied to u0 when it ends
-// in u1, so the caller's solution buffer never changes; the verdict is
-// published as k_tol_finish publishes it, once, at the end.  One launch
-// instead of three per cycle, and no host wait: these grids are bound by
-// launch latency.
+// workgroup of 1024 threads, everything resident in LDS: the FP64 iterate
+// and right-hand side, the sine table, and the FP32 residual / correction
+// matrix R with its product scratch W (the layout of dst_solve's X and Y:
+// node (i, j) of the interior at R[(i-1) ld + (j-1)]).  The initial residual,
+// then per cycle the exact DST solve of R in place, the update u += e and
+// the residual of the updated iterate into R -- the arithmetic of
+// k_mg_coarse_dst and k_tol_update node for node, each thread summing its
+// nodes in ascending order as there, so iterates, residuals and cycle
+// counts are the launched cycles' -- until ||f - Au|| <= tol ||f|| or
+// max_cycles cycles.  The final iterate is written to u; the verdict is
+// published as k_tol_finish publishes it, once, at the end.  One launch per
+// solve and no host wait: these grids are bound by latency.
+#define SMALL_NPT 5  // nodes per thread: n^2 <= 5 x 1024 for n <= 65
+
+__host__ __device__ inline long tol_small_lds_bytes(int m) {
+    const long n = m + 1;
+    return (2 * n * n + 16) * (long)sizeof(double) + dst_lds_floats(m) * (long)sizeof(float);
+}
+
 __device__ __forceinline__ double block_sum_1024(double v, double *ws) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     const int w = threadIdx.x >> 6;
@@ -1329,65 +1335,91 @@ __device__ __forceinline__ double block_sum_1024(double v, double *ws) {
     return t;
 }
 
-__device__ __forceinline__ double small_pass(const double *__restrict__ u_in,
-                                             const float *__restrict__ e,
-                                             double *__restrict__ u_out,
-                                             const double *__restrict__ f, int n, double h2i,
-                                             float *__restrict__ r32, double *ws, double &ff_out) {
-    double rr = 0.0, ff = 0.0;
-    for (int k = (int)threadIdx.x; k < n * n; k += (int)blockDim.x) {
-        const int i = k / n, j = k - (k / n) * n;
-        double v = u_in[k];
-        float out = 0.0f;
-        if (i > 0 && j > 0 && i < n - 1 && j < n - 1) {
-            double un = u_in[k + n], us = u_in[k - n], ue = u_in[k + 1], uw = u_in[k - 1];
-            if (e) {
-                v += (double)e[k];
-                un += (double)e[k + n];
-                us += (double)e[k - n];
-                ue += (double)e[k + 1];
-                uw += (double)e[k - 1];
-            }
-            const double fk = f[k];
-            // the stencil of resid_node: north, south, east, west
-            const double r = -h2i * (un + us + ue + uw - 4.0 * v) + fk;
-            rr += r * r;
-            out = (float)r;
-            ff += fk * fk;
-        }
-        if (u_out) u_out[k] = v;
-        r32[k] = out;
-    }
-    ff_out = block_sum_1024(ff, ws);
-    return block_sum_1024(rr, ws);
-}
-
 __global__ void __launch_bounds__(1024)
-    k_tol_solve_dst(double *__restrict__ u0, double *__restrict__ u1,
-                    const double *__restrict__ f, int n, double h2i, float *__restrict__ r32,
-                    float *__restrict__ e32, int lc, float h2, const float *__restrict__ Sg,
-                    const float *__restrict__ Dg, double tol, int max_cycles,
-                    int *__restrict__ flags, double *__restrict__ relres_host,
-                    int *__restrict__ host_flags) {
+    k_tol_solve_dst(double *__restrict__ ug, const double *__restrict__ fg, int n, double h2i,
+                    int lc, float h2, const float *__restrict__ Sg, const float *__restrict__ Dg,
+                    double tol, int max_cycles, int *__restrict__ flags,
+                    double *__restrict__ relres_host, int *__restrict__ host_flags) {
+    const int m = 1 << (lc + 1), ld = m + 4, nn = n * n;
     // no static LDS: the attribute grants the dynamic region the whole
-    // per-workgroup limit, so the reduction slots sit behind the DST matrices
+    // per-workgroup limit
     extern __shared__ double lds_raw[];
-    double *ws = lds_raw + dst_lds_floats(1 << (lc + 1)) / 2;
-    double *buf[2] = {u0, u1};
+    double *U = lds_raw, *F = U + nn, *ws = F + nn;
+    float *S = reinterpret_cast<float *>(ws + 16), *R = S + m * ld, *W = R + m * ld;
+    // this thread's nodes: k = threadIdx.x + r * 1024; o[r] = its R slot, or
+    // -1 off the interior (and past the grid)
+    int o[SMALL_NPT];
     double ff = 0.0;
-    double rr = small_pass(u0, nullptr, nullptr, f, n, h2i, r32, ws, ff);
+#pragma unroll
+    for (int r = 0; r < SMALL_NPT; r++) {
+        const int k = (int)threadIdx.x + r * 1024;
+        o[r] = -1;
+        if (k < nn) {
+            const int i = k / n, j = k - (k / n) * n;
+            if (i > 0 && j > 0 && i < n - 1 && j < n - 1) o[r] = (i - 1) * ld + (j - 1);
+            const double fk = fg[k];
+            U[k] = ug[k];
+            F[k] = fk;
+            if (o[r] >= 0) ff += fk * fk;
+        }
+    }
+    float dk[4];  // the eigenvalue scale of this thread's R entries
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int k = (int)threadIdx.x + r * 1024;
+        dk[r] = k < m * m ? Dg[k] * h2 : 0.0f;
+    }
+    for (int k = (int)threadIdx.x; k < m * ld; k += 1024) R[k] = 0.0f;
+    for (int k = (int)threadIdx.x; k < m * m; k += 1024) S[(k >> (lc + 1)) * ld + (k & (m - 1))] = Sg[k];
+    ff = block_sum_1024(ff, ws);  // its barriers also publish U, F, S and R
+    // the residual of U into R, its sum of squares
+    auto residual = [&]() -> double {
+        double rr = 0.0;
+#pragma unroll
+        for (int r = 0; r < SMALL_NPT; r++) {
+            const int k = (int)threadIdx.x + r * 1024;
+            if (o[r] >= 0) {
+                // the stencil of resid_node: north, south, east, west
+                const double res =
+                    -h2i * (U[k + n] + U[k - n] + U[k + 1] + U[k - 1] - 4.0 * U[k]) + F[k];
+                rr += res * res;
+                R[o[r]] = (float)res;
+            }
+        }
+        return block_sum_1024(rr, ws);
+    };
+    double rr = residual();
     double rel = (ff > 0.0) ? sqrt(rr / ff) : sqrt(rr);
     int q = 0;
     while (!(rel <= tol) && q < max_cycles) {  // uniform: every thread holds the same sums
-        __syncthreads();  // r32 written
-        dst_solve(r32, e32, lc, h2, Sg, Dg, lds_raw);
-        __syncthreads();  // e32 written
-        rr = small_pass(buf[q & 1], e32, buf[(q + 1) & 1], f, n, h2i, r32, ws, ff);
+        dst_gemm(S, R, W, m, ld);  // W = S r
+        dst_gemm(W, S, R, m, ld);  // R = S r S
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int k = (int)threadIdx.x + r * 1024;
+            if (k < m * m) {
+                const int p = k >> (lc + 1), c = k & (m - 1);
+                R[p * ld + c] = R[p * ld + c] * dk[r];
+            }
+        }
+        __syncthreads();
+        dst_gemm(S, R, W, m, ld);
+        dst_gemm(W, S, R, m, ld);  // R = e on the interior
+#pragma unroll
+        for (int r = 0; r < SMALL_NPT; r++) {
+            const int k = (int)threadIdx.x + r * 1024;
+            if (o[r] >= 0) U[k] = U[k] + (double)R[o[r]];
+        }
+        __syncthreads();
+        rr = residual();
         rel = (ff > 0.0) ? sqrt(rr / ff) : sqrt(rr);
         q++;
     }
-    if (q & 1)  // each thread copies the nodes it wrote in the last pass
-        for (int k = (int)threadIdx.x; k < n * n; k += (int)blockDim.x) u0[k] = u1[k];
+#pragma unroll
+    for (int r = 0; r < SMALL_NPT; r++) {
+        const int k = (int)threadIdx.x + r * 1024;
+        if (o[r] >= 0) ug[k] = U[k];
+    }
     if (threadIdx.x == 0) {
         const int conv = rel <= tol;
         flags[0] = conv;
@@ -1442,15 +1474,13 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     if (M.pending && mg_tol_verdict(M, nullptr) < 0) return -1;
     __atomic_store_n(&M.host_flags[2], 0, __ATOMIC_RELEASE);
     M.host_flags[0] = M.host_flags[1] = 0;
-    if (lf == M.lc) {
+    if (lf == M.lc && n * n <= SMALL_NPT * 1024 && tol_small_lds_bytes(n - 1) <= M.lds) {
         // the whole solve in one workgroup (k_tol_solve_dst)
-        const int m = 1 << (M.lc + 1);
+        const int m = n - 1;
         const double hc = length / (double)m;
-        hipLaunchKernelGGL(k_tol_solve_dst, dim3(1), dim3(1024),
-                           (size_t)dst_lds_floats(m) * sizeof(float) + 16 * sizeof(double), st,
-                           u, M.ualt, f, n, h2i,
-                           r32, e32, M.lc, (float)(hc * hc), M.dst_S, M.dst_D, tol, max_iter,
-                           M.flags, M.relres, M.host_flags);
+        hipLaunchKernelGGL(k_tol_solve_dst, dim3(1), dim3(1024), (size_t)tol_small_lds_bytes(m),
+                           st, u, f, n, h2i, M.lc, (float)(hc * hc), M.dst_S, M.dst_D, tol,
+                           max_iter, M.flags, M.relres, M.host_flags);
         if (after_batch) after_batch(u, u);
         if (hipEventRecord(M.done, st) != hipSuccess) return -1;
         M.pending = 1 + max_iter;

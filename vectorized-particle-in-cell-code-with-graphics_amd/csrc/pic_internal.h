@@ -95,6 +95,7 @@ struct MgArrays {
 
 struct MgLevels {
     int ng = 0, lc = 0;
+    long lds = 0;        // LDS bytes a workgroup may allocate on this device
     MgArrays<double> d;  // FP64 full multigrid (fixed mode) and the FP64 iterate
     MgArrays<float> f;   // FP32 correction solves (tolerance mode)
     // tolerance mode state (device): flags[0] done, flags[1] V-cycles applied,
