@@ -209,8 +209,12 @@ int species_alloc(pic_handle *h, Species &s, int64_t cap) {
         CHECK(dalloc(h, &s.buf[b].vz, cap));
         CHECK(dalloc(h, &s.buf[b].id, cap));
     }
-    CHECK(dalloc(h, &s.q, (size_t)h->N * h->N));
-    HIPCHK(h, hipMemsetAsync(s.q, 0, sizeof(unsigned long long) * h->N * h->N, h->st));
+    for (int b = 0; b < 2; b++) {
+        CHECK(dalloc(h, &s.qbuf[b], (size_t)h->N * h->N));
+        HIPCHK(h, hipMemsetAsync(s.qbuf[b], 0, sizeof(unsigned long long) * h->N * h->N, h->st));
+    }
+    s.qcur = 0;
+    s.q = s.qbuf[0];
     return PIC_OK;
 }
 
@@ -231,7 +235,8 @@ void species_free(Species &s) {
     if (s.mq) hipFree(s.mq);
     if (s.mq_cnt) hipFree(s.mq_cnt);
     if (s.mc_bins) hipFree(s.mc_bins);
-    if (s.q) hipFree(s.q);
+    for (int b = 0; b < 2; b++)
+        if (s.qbuf[b]) hipFree(s.qbuf[b]);
     if (s.exp_) hipFree(s.exp_);
     if (s.eyp) hipFree(s.eyp);
     void *st[] = {s.stage.cell, s.stage.ox, s.stage.oy, s.stage.vx,
@@ -948,7 +953,8 @@ int field_step(pic_handle *h, int64_t step) {
     launch_field_start(h->st, h->sp[1].q, h->sp[0].q, h->N, h->sp[1].scale, h->sp[0].scale,
                        P.corner8 ? 8.0 : 4.0, h->rho, 1.0 / P.epsilon, P.phi_left, phiR,
                        P.phi_bottom, P.phi_top, 1.0 / (h->dx * h->dx), M.d.rhsF[top], M.d.u[top],
-                       M.uprev, extrap, tol ? M.flags : nullptr);
+                       M.uprev, extrap, tol ? M.flags : nullptr, h->sp[1].qbuf[h->sp[1].qcur ^ 1],
+                       h->sp[0].qbuf[h->sp[0].qcur ^ 1]);
     CHECK(solve_potential(h, P.phi_left, phiR, P.phi_bottom, P.phi_top, true, true));
     prof_end(h, p);
     return PIC_OK;
@@ -1018,7 +1024,12 @@ int fused_step(pic_handle *h) {
     int N = h->N;
     int pt = prof_begin(h, 0);
     int pp = prof_begin(h, 1);
-    launch_zero2(h->st, h->sp[0].q, h->sp[1].q, (long)N * N);
+    // this step's charge goes into the buffers the last field step zeroed
+    for (int s = 0; s < 2; s++) {
+        Species &sp = h->sp[s];
+        sp.qcur ^= 1;
+        sp.q = sp.qbuf[sp.qcur];
+    }
     // particle source after the electron collisions, before the gather
     // (pic.cpp:1071-1082): electrons are staged already moved, ions unmoved
     inject(h, 0, 1);
@@ -1945,7 +1956,7 @@ int pic_estimate_bytes(const pic_params *pp, int64_t *bytes) {
     for (int s = 0; s < 2; s++) {
         int64_t cap = n0 + (spawning ? std::max<int64_t>(n0 / 2, 65536) : 0) + 4096 * n_inj;
         cap = std::max<int64_t>(cap, 1);
-        b += 2 * cap * (28 + (weighted ? 4 : 0)) + N * N * 8;
+        b += 2 * cap * (28 + (weighted ? 4 : 0)) + N * N * 16;  // two charge buffers
         int64_t want = std::max<int64_t>(16384, std::min<int64_t>(n0 / 16, (int64_t)1 << 22)) +
                        2 * n_inj + (wall_on ? n0 / 4 : 0);
         int64_t sc = 1;

@@ -62,11 +62,16 @@ __global__ void k_field_start(const unsigned long long *__restrict__ qi,
                               double scale_e, double corner, double *__restrict__ rho, double coef,
                               double phi0X, double phiR, double phi0Y, double phiLY,
                               double dxdy_inv, double *__restrict__ rhs, double *__restrict__ u,
-                              double *__restrict__ uprev, int extrap, int *__restrict__ flags) {
+                              double *__restrict__ uprev, int extrap, int *__restrict__ flags,
+                              unsigned long long *__restrict__ zi,
+                              unsigned long long *__restrict__ ze) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y;
     if (flags && i == 0 && j == 0) flags[0] = flags[1] = flags[2] = flags[3] = 0;
     if (i >= N) return;
+    // the charge buffers of the next step start from zero
+    zi[(long)j * N + i] = 0ull;
+    ze[(long)j * N + i] = 0ull;
     // k_rho
     bool ex = (i == 0 || i == N - 1), ey = (j == 0 || j == N - 1);
     double fac = (ex && ey) ? corner : ((ex || ey) ? 2.0 : 1.0);
@@ -94,11 +99,11 @@ void launch_field_start(hipStream_t st, const unsigned long long *qi, const unsi
                         int N, double scale_i, double scale_e, double corner, double *rho,
                         double coef, double phi0X, double phiR, double phi0Y, double phiLY,
                         double dxdy_inv, double *rhs, double *u, double *uprev, int extrap,
-                        int *flags) {
+                        int *flags, unsigned long long *zi, unsigned long long *ze) {
     dim3 g((N + 127) / 128, N);
     hipLaunchKernelGGL(k_field_start, g, dim3(128), 0, st, qi, qe, N, scale_i, scale_e, corner,
                        rho, coef, phi0X, phiR, phi0Y, phiLY, dxdy_inv, rhs, u, uprev, extrap,
-                       flags);
+                       flags, zi, ze);
 }
 
 void launch_fold(hipStream_t st, const double *rho, int N, double coef, double phi0X, double phiR,
@@ -367,21 +372,6 @@ template void launch_fill<float>(hipStream_t, float *, long, float);
 template void launch_fill<unsigned long long>(hipStream_t, unsigned long long *, long,
                                               unsigned long long);
 template void launch_fill<uint32_t>(hipStream_t, uint32_t *, long, uint32_t);
-
-// both species' fixed-point charge cleared in one launch
-__global__ void k_zero2(unsigned long long *__restrict__ a, unsigned long long *__restrict__ b,
-                        long n) {
-    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) {
-        a[k] = 0ull;
-        b[k] = 0ull;
-    }
-}
-
-void launch_zero2(hipStream_t st, unsigned long long *a, unsigned long long *b, long n) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_zero2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, n);
-}
 
 // --------------------------------------- multigrid: one-workgroup coarse part
 

@@ -17,7 +17,12 @@ struct Species {
     pic::SpView buf[2] = {};
     int cur = 0;
     int64_t cap = 0;
-    unsigned long long *q = nullptr;  // fixed-point charge, N*N
+    unsigned long long *q = nullptr;  // fixed-point charge, N*N: qbuf[qcur]
+    // a timestep deposits into the buffer its start switches to; the field
+    // step zeroes the other one for the next step (it is zero whenever a
+    // step starts)
+    unsigned long long *qbuf[2] = {};
+    int qcur = 0;
     float *exp_ = nullptr, *eyp = nullptr;  // per-phase gather output
     bool gathered = false;
     double q_over_m = 0, qc = 0, mass = 0, dt = 0, weight = 1.0, sigma_load = 0;

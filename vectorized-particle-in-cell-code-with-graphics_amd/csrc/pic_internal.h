@@ -325,17 +325,17 @@ void launch_phi_efield(hipStream_t st, const double *u, int N, double phi0X, dou
                        const int *flags = nullptr);
 // the fused timestep's field set-up per node: rho (k_rho), the folded rhs
 // (k_fold), the tolerance solve's start (extrap 1: uprev <- u; 2: also
-// u <- 2 u - uprev) and, with flags, the solve's flag reset
+// u <- 2 u - uprev), with flags the solve's flag reset, and zi, ze (the
+// charge buffers of the next step, N*N each) cleared
 void launch_field_start(hipStream_t st, const unsigned long long *qi, const unsigned long long *qe,
                         int N, double scale_i, double scale_e, double corner, double *rho,
                         double coef, double phi0X, double phiR, double phi0Y, double phiLY,
                         double dxdy_inv, double *rhs, double *u, double *uprev, int extrap,
-                        int *flags);
+                        int *flags, unsigned long long *zi, unsigned long long *ze);
 void launch_efield(hipStream_t st, const double *phi, int N, double dx, double *ex64, double *ey64,
                    float2 *E32);
 template <typename T>
 void launch_fill(hipStream_t st, T *p, long n, T v);
-void launch_zero2(hipStream_t st, unsigned long long *a, unsigned long long *b, long n);
 int mg_coarse_level_for(long lds_bytes, int ng);
 int mg_dst_tables(MgLevels &M);
 void mg_set_coarse_attr(long lds_bytes);
