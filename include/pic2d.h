@@ -522,7 +522,8 @@ int pic_profile_enable(pic_handle *h, int32_t on);
  * [1] push-phase ms (collisions, gather, push, sort, deposit, commits),
  * [2] k_push_scatter ms, [3] field solve ms, [4] k_push_count ms, [5] bin-scan
  * ms, [6] k_push_inplace ms, [7] k_push_scatter launches, [8] k_push_inplace
- * launches, [9..11] 0 */
+ * launches (species passes: both species' collisionless passes in one launch,
+ * k_push_inplace_pair, count as two), [9..11] 0 */
 int pic_profile_read(pic_handle *h, double *out, int32_t n);
 /* per-step durations (ms) of the profiled steps since the last read, oldest
  * first; *n receives the count written (at most cap) */

@@ -188,7 +188,12 @@ void prof_flush(pic_handle *h) {
     for (ProfPair &p : h->prof_live) {
         float ms = 0.f;
         hipEventElapsedTime(&ms, p.a, p.b);
-        h->prof_acc[p.tag] += ms;
+        if (p.tag == 9) {  // both species' in-place passes in one launch
+            h->prof_acc[6] += ms;
+            h->prof_acc[8] += 2.0;
+        } else {
+            h->prof_acc[p.tag] += ms;
+        }
         if (p.tag == 2) h->prof_acc[7] += 1.0;
         if (p.tag == 6) h->prof_acc[8] += 1.0;
         if (p.tag == 0) h->prof_steps.push_back(ms);
@@ -837,6 +842,48 @@ void push_species_inplace(pic_handle *h, int si) {
     s.since_sort++;
 }
 
+// Both species' in-place passes of a timestep in one launch, where nothing
+// runs between them (no surface physics, spawns, re-injection or
+// reference-mode charge exchange), neither sorts this step and both are
+// collisionless; false: nothing was launched (the passes run one by one).
+bool push_pair_inplace(pic_handle *h) {
+    if (h->spawning || h->wall_on || h->refcx || plop_on(h)) return false;
+    for (int si = 0; si < 2; si++) {
+        const Species &s = h->sp[si];
+        if (s.since_sort + 1 >= s.sort_interval || s.mc.on) return false;
+    }
+    InplaceArgs a[2];
+    bool hist[2];
+    for (int si = 0; si < 2; si++) {
+        Species &s = h->sp[si];
+        drop_hist(h, si);
+        hist[si] = s.since_sort + 2 >= s.sort_interval && hist_fusable(h, si);
+        const int c = s.cur;
+        a[si] = InplaceArgs{s.buf[c],
+                            &h->cnt[si].n[c],
+                            s.cap,
+                            s.pc,
+                            s.mc,
+                            s.q,
+                            pass_spawn(h, si, 1),
+                            &h->cnt[si].events[0][0],
+                            &h->cnt[si].lost[0][0],
+                            &h->cnt[si].holes[0],
+                            hist[si] ? bin_count(h, si) : nullptr};
+    }
+    if (!inplace_pair_ok(a[0], a[1])) return false;
+    int p = prof_begin(h, 9);  // two species passes
+    launch_push_inplace_pair(h->st, a[0], a[1], h->E32, (uint32_t)h->step, 1);
+    prof_end(h, p);
+    for (int si = 0; si < 2; si++) {
+        Species &s = h->sp[si];
+        s.hist_pending = hist[si];
+        s.holes = true;
+        s.since_sort++;
+    }
+    return true;
+}
+
 // one timestep's particle pass of a species: a sort pass every
 // sort_interval-th pass, in-place passes in between
 void advance_species(pic_handle *h, int si) {
@@ -874,9 +921,6 @@ int absorb_species(pic_handle *h, int si, bool plop) {
     return PIC_OK;
 }
 
-// phi from the solve with the boundary values; with_e: E (and its fp32 copy
-// for the gather) from the same values in the same launch; timestep: the
-// tolerance solve starts from the linear extrapolation of the last two steps
 // phi from the solve with the boundary values; with_e: E (and its fp32 copy
 // for the gather) from the same values in the same launch.  timestep: the
 // fused step's field_start has already formed the rhs, the extrapolated start
@@ -1034,21 +1078,25 @@ int fused_step(pic_handle *h) {
     // (pic.cpp:1071-1082): electrons are staged already moved, ions unmoved
     inject(h, 0, 1);
     inject(h, 1, 0);
-    advance_species(h, 0);
-    wall_pass(h, 0);  // boundary::e_boundary on the electrons that reached a wall
-    if (h->spawning) {
-        // ionisation products: new ions join this step's ion pass (they were
-        // appended before the ion push in the reference, pic.cpp:1071-1111);
-        // new electrons were pushed with their parents and are deposited here
-        commit_both(h, 0, 1);
-    }
-    if (h->step % h->P.dt_dti == 0) {
-        if (h->refcx) CHECK(ref_ion_cx(h));
-        advance_species(h, 1);
-        wall_pass(h, 1);  // boundary::ion_boundary (secondaries join the electrons)
-    } else {
-        Species &s = h->sp[1];
-        launch_deposit(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], s.cap, N, s.pc.mode, s.q);
+    const bool ions_move = h->step % h->P.dt_dti == 0;
+    if (!(ions_move && push_pair_inplace(h))) {
+        advance_species(h, 0);
+        wall_pass(h, 0);  // boundary::e_boundary on the electrons that reached a wall
+        if (h->spawning) {
+            // ionisation products: new ions join this step's ion pass (they
+            // were appended before the ion push in the reference,
+            // pic.cpp:1071-1111); new electrons were pushed with their
+            // parents and are deposited here
+            commit_both(h, 0, 1);
+        }
+        if (ions_move) {
+            if (h->refcx) CHECK(ref_ion_cx(h));
+            advance_species(h, 1);
+            wall_pass(h, 1);  // boundary::ion_boundary (secondaries join the electrons)
+        } else {
+            Species &s = h->sp[1];
+            launch_deposit(h->st, s.buf[s.cur], &h->cnt[1].n[s.cur], s.cap, N, s.pc.mode, s.q);
+        }
     }
     if (h->spawning && (plop_on(h) || h->wall_on)) {
         // pairs re-injected for ions lost this step; ion-impact secondaries

@@ -1201,13 +1201,13 @@ __device__ __forceinline__ void push_inplace_body(
     SpView sp, const uint32_t *__restrict__ nptr, const float2 *__restrict__ E, PushConst pc,
     mcc::MccConst mc, uint32_t step, unsigned long long *__restrict__ q, int deposit, Spawn spn,
     unsigned long long *__restrict__ events, uint32_t *__restrict__ lost6,
-    uint32_t *__restrict__ holes, uint32_t *__restrict__ hist) {
+    uint32_t *__restrict__ holes, uint32_t *__restrict__ hist, uint32_t bid) {
     pin_variant<V>(pc, mc, sp);
     pin_spawn<V>(spn);
     const uint32_t n = *nptr;
     // the grid covers the capacity: blocks past the live count leave at once
-    // (uniformly, before any barrier)
-    if (blockIdx.x * (blockDim.x * IPP) >= n) return;
+    // (uniformly, before any barrier); bid: this block's index in the species
+    if (bid * (blockDim.x * IPP) >= n) return;
     __shared__ uint32_t sl[6];
     __shared__ uint32_t shole;
     __shared__ uint32_t se[8];
@@ -1216,7 +1216,7 @@ __device__ __forceinline__ void push_inplace_body(
     if (threadIdx.x < 6) sl[threadIdx.x] = 0;
     if (threadIdx.x < 8) se[threadIdx.x] = 0;
     if (threadIdx.x == 0) shole = 0;
-    const uint32_t base = blockIdx.x * (blockDim.x * IPP) + threadIdx.x;
+    const uint32_t base = bid * (blockDim.x * IPP) + threadIdx.x;
     // vz and the id are read only where something uses them: the magnetic
     // rotation, collisions, re-injection keyed by the id, and the wall
     // queue (surface physics needs the full velocity and the id's draws)
@@ -1293,7 +1293,8 @@ __device__ __forceinline__ void push_inplace_body(
         uint32_t *__restrict__ holes, uint32_t *__restrict__ hist
 template <int V>
 __global__ void __launch_bounds__(256) k_push_inplace(PUSH_INPLACE_PARAMS) {
-    push_inplace_body<V>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes, hist);
+    push_inplace_body<V>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes, hist,
+                         blockIdx.x);
 }
 // The collisionless variants fit 8 waves per SIMD in 42 VGPRs; left to
 // itself the scheduler gives SPAWN (PLAIN plus a rare queue write) 84 and 5
@@ -1302,19 +1303,33 @@ template <>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
     k_push_inplace<PV_PLAIN>(PUSH_INPLACE_PARAMS) {
     push_inplace_body<PV_PLAIN>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes,
-                                hist);
+                                hist, blockIdx.x);
 }
 template <>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
     k_push_inplace<PV_SPAWN>(PUSH_INPLACE_PARAMS) {
     push_inplace_body<PV_SPAWN>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes,
-                                hist);
+                                hist, blockIdx.x);
 }
 template <>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
     k_push_inplace<PV_DEFER>(PUSH_INPLACE_PARAMS) {
     push_inplace_body<PV_DEFER>(sp, nptr, E, pc, mc, step, q, deposit, spn, events, lost6, holes,
-                                hist);
+                                hist, blockIdx.x);
+}
+
+// Both species' collisionless in-place passes in one launch (the first nba
+// blocks advance a, the rest b): the kernel of each species pass as it is,
+// one launch's fixed cost instead of two on the small grids
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+    k_push_inplace_pair(InplaceArgs a, InplaceArgs b, uint32_t nba,
+                        const float2 *__restrict__ E, uint32_t step, int deposit) {
+    if (blockIdx.x < nba)
+        push_inplace_body<PV_PLAIN>(a.sp, a.n, E, a.pc, a.mc, step, a.q, deposit, a.spn,
+                                    a.events, a.lost6, a.holes, a.hist, blockIdx.x);
+    else
+        push_inplace_body<PV_PLAIN>(b.sp, b.n, E, b.pc, b.mc, step, b.q, deposit, b.spn,
+                                    b.events, b.lost6, b.holes, b.hist, blockIdx.x - nba);
 }
 
 // Physical-mode null-collision MCC of one species as passes of their own,
@@ -2185,6 +2200,19 @@ void launch_push_inplace(hipStream_t st, const SpView &sp, const uint32_t *n, in
     const int v = pass_variant(sp, pc, mc, &spn);
     LAUNCH_VARIANT(v, k_push_inplace, dim3(blocks_for(cap, 256 * IPP)), st, sp, n, E, pc, mc, step,
                    q, deposit, spn, events, lost6, holes, hist);
+}
+
+bool inplace_pair_ok(const InplaceArgs &a, const InplaceArgs &b) {
+    return pass_variant(a.sp, a.pc, a.mc, &a.spn) == PV_PLAIN &&
+           pass_variant(b.sp, b.pc, b.mc, &b.spn) == PV_PLAIN;
+}
+
+void launch_push_inplace_pair(hipStream_t st, const InplaceArgs &a, const InplaceArgs &b,
+                              const float2 *E, uint32_t step, int deposit) {
+    const uint32_t nba = (uint32_t)blocks_for(a.cap, 256 * IPP);
+    const uint32_t nb = nba + (uint32_t)blocks_for(b.cap, 256 * IPP);
+    hipLaunchKernelGGL(k_push_inplace_pair, dim3(nb), dim3(256), 0, st, a, b, nba, E, step,
+                       deposit);
 }
 
 uint32_t collide_seg_cap(int64_t cap) {

@@ -166,6 +166,23 @@ void launch_push_inplace(hipStream_t st, const SpView &sp, const uint32_t *n, in
                          uint32_t step, unsigned long long *q, int deposit, const Spawn &spn,
                          unsigned long long *events, uint32_t *lost6, uint32_t *holes,
                          uint32_t *hist = nullptr);
+// one species' arguments of an in-place pass (launch_push_inplace's)
+struct InplaceArgs {
+    SpView sp;
+    const uint32_t *n;
+    int64_t cap;
+    PushConst pc;
+    mcc::MccConst mc;
+    unsigned long long *q;
+    Spawn spn;
+    unsigned long long *events;
+    uint32_t *lost6, *holes, *hist;
+};
+// both species' in-place passes in one launch, for two passes that
+// inplace_pair_ok accepts (both collisionless: the PV_PLAIN variant)
+bool inplace_pair_ok(const InplaceArgs &a, const InplaceArgs &b);
+void launch_push_inplace_pair(hipStream_t st, const InplaceArgs &a, const InplaceArgs &b,
+                              const float2 *E, uint32_t step, int deposit);
 // candidate queue of the collisions run ahead: MQ_SEGS segments of
 // collide_seg_cap(cap) indices, one counter per segment every MQ_STRIDE words
 #define MQ_SEGS 1024
