@@ -29,17 +29,23 @@ def final(p, tuning):
         return out, sim.tuning()
 
 
-def test_sort_intervals_do_not_change_results():
-    p = workloads.with_argon(workloads.discharge(64, 40, te=4.0), pressure_pa=60.0)
+@pytest.mark.parametrize("collisions", [True, False])
+def test_sort_intervals_do_not_change_results(collisions):
+    """collisionless, the species' in-place passes of a step share one launch
+    (k_push_inplace_pair), with the next sort's histogram when both sort on
+    the following step (intervals 3, 3), and run one by one when only one of
+    them is in place (5, 3)"""
+    p = workloads.discharge(64, 40, te=4.0)
+    if collisions:
+        p = workloads.with_argon(p, pressure_pa=60.0)
     p.mg_mode = pic2d.MG_FIXED
     a, ta = final(p, None)
-    b, tb = final(p, (1, 1, 0))
-    c, tc = final(p, (5, 3, 0))
-    assert tb["sort_interval_e"] == 1 and tc["sort_interval_i"] == 3
-    print("tunings", ta, tb, tc, flush=True)
-    for key in a:
-        assert np.array_equal(a[key], b[key]), key
-        assert np.array_equal(a[key], c[key]), key
+    runs = [final(p, t) for t in ((1, 1, 0), (5, 3, 0), (3, 3, 0))]
+    assert runs[0][1]["sort_interval_e"] == 1 and runs[1][1]["sort_interval_i"] == 3
+    print("tunings", ta, [t for _, t in runs], flush=True)
+    for b, _ in runs:
+        for key in a:
+            assert np.array_equal(a[key], b[key]), key
 
 
 @pytest.mark.parametrize("nu", [1, 2, 3])
