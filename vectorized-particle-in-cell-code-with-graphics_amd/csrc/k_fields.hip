@@ -696,6 +696,67 @@ __global__ void __launch_bounds__(1024)
     dst_solve(rhs_in, u_out, lc, h2, Sg, Dg, lds_raw);
 }
 
+// The verdict of the tolerance solve's last residual pass, formed at the
+// start of the next cycle's first kernel instead of by k_tol_finish (one
+// launch fewer per cycle): every block sums the pass's partials as
+// k_tol_finish does (256 threads, the same order, so the same residual) and
+// leaves if the solve has converged; block (0, 0) also records and
+// publishes the verdict as k_tol_finish does.  partials null: no verdict.
+struct TolVerdict {
+    const double *partials;
+    int nb;
+    double tol;
+    int *flags;
+    double *relres_host;
+    int *host_flags;
+};
+
+// block sums of the residual passes (TOL_THREADS threads, fixed order)
+#define TOL_THREADS 256
+
+__device__ __forceinline__ double block_sum_d(double v) {
+    __shared__ double ws[TOL_THREADS / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) ws[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < TOL_THREADS / 64; k++) t += ws[k];  // fixed order
+    __syncthreads();
+    return t;
+}
+
+// true: the solve goes on (this block does the cycle's work).  flags[0] may
+// turn 1 during the kernel (block (0, 0) sets it on convergence, which every
+// block then finds itself), so one thread reads it for the whole block.
+__device__ __forceinline__ bool tol_verdict_go(const TolVerdict &vd) {
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = *(volatile int *)vd.flags;
+    __syncthreads();
+    if (s_done) return false;  // converged at an earlier pass: skipped, as k_tol_finish is
+    double rr = 0.0, ff = 0.0;
+    for (int k = threadIdx.x; k < vd.nb; k += blockDim.x) {
+        rr += vd.partials[2 * k];
+        ff += vd.partials[2 * k + 1];
+    }
+    rr = block_sum_d(rr);
+    ff = block_sum_d(ff);
+    const double rel = (ff > 0.0) ? sqrt(rr / ff) : sqrt(rr);
+    const bool conv = rel <= vd.tol;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        int *flags = vd.flags;
+        flags[1] += 1;  // the cycle that made the pass applied its correction
+        if (conv) flags[0] = 1;
+        *vd.relres_host = rel;
+        vd.host_flags[0] = conv ? 1 : 0;
+        vd.host_flags[1] = flags[1];
+        __threadfence_system();
+        flags[3] += 1;
+        vd.host_flags[2] = flags[3] | (conv ? MG_PUB_CONV : 0);
+    }
+    return !conv;
+}
+
 // ------------------------------------------- fused fp32 V-cycle levels
 // One V(nu,nu) visit of a level above lc takes two launches:
 //   pre:  nu red-black Gauss-Seidel sweep pairs (mglin.cpp:116-127) from a
@@ -794,8 +855,12 @@ template <int NU, int T>
 __global__ void __launch_bounds__(256)
     k_mg_pre(const float *__restrict__ u_in, const float *__restrict__ rhs, int n, float h2,
              float h2i, float *__restrict__ u_out, float *__restrict__ cres,
-             const int *__restrict__ skip) {
-    SKIP_IF_DONE(skip);
+             const int *__restrict__ skip, TolVerdict vd) {
+    if (vd.partials) {
+        if (!tol_verdict_go(vd)) return;
+    } else {
+        SKIP_IF_DONE(skip);
+    }
     constexpr int W = MgHalo<NU, T>::WPRE;
     __shared__ float su[W * W], sf[W * W];
     mg_pre_tile<NU, T>(blockIdx.x, blockIdx.y, u_in, rhs, n, h2, h2i, u_out, cres, su, sf);
@@ -853,9 +918,10 @@ static inline int mg_tile(int nf) { return nf <= 513 ? 16 : 32; }
         }                                             \
     } while (0)
 
-// fused V(nu,nu) at level j > lc on (Uj, Rj); Uj's initial guess is zero
+// fused V(nu,nu) at level j > lc on (Uj, Rj); Uj's initial guess is zero;
+// vd: the verdict the first kernel forms first (TolVerdict)
 static void vcycle_fused(hipStream_t st, MgLevels &M, int j, float *Uj, const float *Rj,
-                         double length, const int *skip) {
+                         double length, const int *skip, const TolVerdict &vd = TolVerdict{}) {
     MgArrays<float> &A = M.f;
     const int nf = lvl_size(j);
     const double h = length / (nf - 1);
@@ -865,7 +931,7 @@ static void vcycle_fused(hipStream_t st, MgLevels &M, int j, float *Uj, const fl
     const float *z = nullptr;
 #define PRE(NU, TT)                                                                          \
     hipLaunchKernelGGL((k_mg_pre<NU, TT>), g, b, 0, st, z, Rj, nf, h2, h2i, A.us[j], A.cres[j - 1], \
-                       skip)
+                       skip, vd)
     MG_DISPATCH(M.nu, T, PRE);
 #undef PRE
     if (j - 1 == M.lc)
@@ -1007,19 +1073,6 @@ void mg_fixed_solve(hipStream_t st, MgLevels &M, int ncycle, double length) {
 // FP32 V(1,1) cycle from e = 0, u += e in FP64; repeated until
 // ||r||_2 <= tol ||f||_2 over interior nodes.
 
-#define TOL_THREADS 256
-
-__device__ __forceinline__ double block_sum_d(double v) {
-    __shared__ double ws[TOL_THREADS / 64];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) ws[w] = v;
-    __syncthreads();
-    double t = 0.0;
-    for (int k = 0; k < TOL_THREADS / 64; k++) t += ws[k];  // fixed order
-    __syncthreads();
-    return t;
-}
 
 // End of a residual pass: this block's sums of r^2 and f^2 go to
 // partials[b].  The pass's total and verdict come from k_tol_finish, the
@@ -1224,10 +1277,12 @@ __global__ void k_tol_reset(int *flags) {
 
 // one refinement cycle above the coarsest level: the V-cycle on the
 // correction equation A e = r_in from e = 0, its top post leg fused with
-// the FP64 update u_out = u_in + e and the new residual r_out
+// the FP64 update u_out = u_in + e and the new residual r_out.  vd: the
+// verdict of the previous residual pass, formed by the first kernel; fin:
+// this pass's verdict by k_tol_finish (else by the next cycle's first kernel)
 static void vcycle_top(hipStream_t st, MgLevels &M, int j, const float *r_in, const double *u_in,
                        double *u_out, const double *f, double h2i, float *r_out, double tol,
-                       double length) {
+                       double length, const TolVerdict &vd, bool fin) {
     MgArrays<float> &A = M.f;
     const int *skip = M.flags;
     const int nf = lvl_size(j);
@@ -1238,7 +1293,7 @@ static void vcycle_top(hipStream_t st, MgLevels &M, int j, const float *r_in, co
     const float *z = nullptr;
 #define PRE(NU, TT)                                                                             \
     hipLaunchKernelGGL((k_mg_pre<NU, TT>), g, b, 0, st, z, r_in, nf, h2, h2f, A.us[j],          \
-                       A.cres[j - 1], skip)
+                       A.cres[j - 1], skip, vd)
     MG_DISPATCH(M.nu, T, PRE);
 #undef PRE
     if (j - 1 == M.lc)
@@ -1250,7 +1305,7 @@ static void vcycle_top(hipStream_t st, MgLevels &M, int j, const float *r_in, co
                        u_in, u_out, f, h2i, r_out, M.partials, M.flags)
     MG_DISPATCH(M.nu, T, POST_UPDATE);
 #undef POST_UPDATE
-    tol_finish_launch(st, M, tiles * tiles, tol, true);
+    if (fin) tol_finish_launch(st, M, tiles * tiles, tol, true);
 }
 
 // Waits for the queued cycles' verdict: the device has published (host_flags
@@ -1441,13 +1496,23 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     // where the levels are latency-bound; at 1025^2 the wider halo's extra
     // reads cost more than the launch (measured), so the passes stay apart
     const bool fuse_top = lf > M.lc && n <= 513;
-    auto cycle = [&]() {
+    // residual-pass blocks of a cycle (the partials its verdict sums)
+    const int top_tiles = (n + mg_tile(n) - 1) / mg_tile(n);
+    const int cyc_nb = (lf > M.lc && fuse_top) ? top_tiles * top_tiles : M.nparts;
+    // cycle c of a batch: the verdict of cycle c - 1's residual pass is formed
+    // by this cycle's first kernel (vfirst false: by k_tol_finish); fin: this
+    // cycle's own pass gets k_tol_finish (the last cycle of a batch, on which
+    // the host waits)
+    auto cycle = [&](bool vfirst, bool fin) {
+        const TolVerdict vd = vfirst ? TolVerdict{M.partials, cyc_nb, tol, M.flags, M.relres,
+                                                  M.host_flags}
+                                     : TolVerdict{};
         if (lf > M.lc && !fuse_top) {
-            vcycle_fused(st, M, lf, e32, r32, length, skip);
+            vcycle_fused(st, M, lf, e32, r32, length, skip, vd);
             hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st,
                                (const double *)buf[queued & 1], (const float *)e32,
                                buf[(queued + 1) & 1], f, n, h2i, r32, M.partials, M.flags);
-            tol_finish_launch(st, M, M.nparts, tol, true);
+            if (fin) tol_finish_launch(st, M, M.nparts, tol, true);
         } else if (lf == M.lc) {
             coarse_dst_launch(st, M, r32, e32, length, skip);
             hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st,
@@ -1456,9 +1521,11 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
             tol_finish_launch(st, M, M.nparts, tol, true);
         } else {
             vcycle_top(st, M, lf, rb[queued & 1], buf[queued & 1], buf[(queued + 1) & 1], f, h2i,
-                       rb[(queued + 1) & 1], tol, length);
+                       rb[(queued + 1) & 1], tol, length, vd, fin);
         }
     };
+    // the coarsest-level branch keeps k_tol_finish after every cycle
+    const bool vfuse = lf > M.lc;
     // nothing on the device writes the host copies now (the last solve's
     // remaining cycles were skipped): clear them for this solve's verdict
     if (M.pending && mg_tol_verdict(M, nullptr) < 0) return -1;
@@ -1487,7 +1554,7 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     // this thread on st without running them
     auto queue_cycles = [&](int k) -> bool {
         if (!M.use_graphs) {
-            for (int c = 0; c < k; c++, queued++) cycle();
+            for (int c = 0; c < k; c++, queued++) cycle(vfuse && c > 0, !vfuse || c == k - 1);
             return true;
         }
         const int par = queued & 1;
@@ -1499,7 +1566,7 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
             if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
                 return false;
             const int q0 = queued;
-            for (int c = 0; c < k; c++, queued++) cycle();
+            for (int c = 0; c < k; c++, queued++) cycle(vfuse && c > 0, !vfuse || c == k - 1);
             queued = q0;
             if (hipStreamEndCapture(st, &gr) != hipSuccess) return false;
             const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
