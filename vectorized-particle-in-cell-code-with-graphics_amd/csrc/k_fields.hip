@@ -701,11 +701,13 @@ __global__ void __launch_bounds__(1024)
 // launch fewer per cycle): every block sums the pass's partials as
 // k_tol_finish does (256 threads, the same order, so the same residual) and
 // leaves if the solve has converged; block (0, 0) also records and
-// publishes the verdict as k_tol_finish does.  partials null: no verdict.
+// publishes the verdict as k_tol_finish does (applied: the pass followed a
+// cycle, not the solve's start).  partials null: no verdict.
 struct TolVerdict {
     const double *partials;
     int nb;
     double tol;
+    int applied;
     int *flags;
     double *relres_host;
     int *host_flags;
@@ -745,7 +747,7 @@ __device__ __forceinline__ bool tol_verdict_go(const TolVerdict &vd) {
     const bool conv = rel <= vd.tol;
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
         int *flags = vd.flags;
-        flags[1] += 1;  // the cycle that made the pass applied its correction
+        if (vd.applied) flags[1] += 1;
         if (conv) flags[0] = 1;
         *vd.relres_host = rel;
         vd.host_flags[0] = conv ? 1 : 0;
@@ -1499,12 +1501,14 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     // residual-pass blocks of a cycle (the partials its verdict sums)
     const int top_tiles = (n + mg_tile(n) - 1) / mg_tile(n);
     const int cyc_nb = (lf > M.lc && fuse_top) ? top_tiles * top_tiles : M.nparts;
-    // cycle c of a batch: the verdict of cycle c - 1's residual pass is formed
-    // by this cycle's first kernel (vfirst false: by k_tol_finish); fin: this
-    // cycle's own pass gets k_tol_finish (the last cycle of a batch, on which
-    // the host waits)
+    // vfirst: the verdict of the residual pass before this cycle (the
+    // solve's initial pass, or the previous cycle's) is formed by this
+    // cycle's first kernel, else k_tol_finish formed it after the pass; fin:
+    // this cycle's own pass gets k_tol_finish (the last cycle of a batch, on
+    // which the host waits)
     auto cycle = [&](bool vfirst, bool fin) {
-        const TolVerdict vd = vfirst ? TolVerdict{M.partials, cyc_nb, tol, M.flags, M.relres,
+        const TolVerdict vd = vfirst ? TolVerdict{M.partials, queued > 0 ? cyc_nb : M.nparts, tol,
+                                                  queued > 0 ? 1 : 0, M.flags, M.relres,
                                                   M.host_flags}
                                      : TolVerdict{};
         if (lf > M.lc && !fuse_top) {
@@ -1524,8 +1528,9 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
                        rb[(queued + 1) & 1], tol, length, vd, fin);
         }
     };
-    // the coarsest-level branch keeps k_tol_finish after every cycle
-    const bool vfuse = lf > M.lc;
+    // the coarsest-level branch keeps k_tol_finish after every pass, and so
+    // does a solve allowed no cycles
+    const bool vfuse = lf > M.lc && max_iter > 0;
     // nothing on the device writes the host copies now (the last solve's
     // remaining cycles were skipped): clear them for this solve's verdict
     if (M.pending && mg_tol_verdict(M, nullptr) < 0) return -1;
@@ -1547,32 +1552,37 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
     hipLaunchKernelGGL(k_tol_update, dim3(M.nparts), dim3(TOL_THREADS), 0, st, (const double *)u,
                        (const float *)nullptr, (double *)nullptr, f, n, h2i, r32, M.partials,
                        M.flags);
-    tol_finish_launch(st, M, M.nparts, tol, false);
+    // its verdict: by the first cycle's first kernel where cycles form them
+    if (!vfuse) tol_finish_launch(st, M, M.nparts, tol, false);
     // k cycles from `queued` on: launched one kernel at a time, or replayed
     // from the hipGraph of that batch (k, parity of queued, iterate buffer),
     // captured on the batch's first use; a capture records the launches of
     // this thread on st without running them
     auto queue_cycles = [&](int k) -> bool {
+        // the batch right after the initial pass forms that pass's verdict too
+        const bool first = queued == 0;
         if (!M.use_graphs) {
-            for (int c = 0; c < k; c++, queued++) cycle(vfuse && c > 0, !vfuse || c == k - 1);
+            for (int c = 0; c < k; c++, queued++)
+                cycle(vfuse && (c > 0 || first), !vfuse || c == k - 1);
             return true;
         }
         const int par = queued & 1;
         hipGraphExec_t ex = nullptr;
         for (const MgLevels::CycleGraph &g : M.graphs)
-            if (g.k == k && g.parity == par && g.u0 == buf[0]) ex = g.exec;
+            if (g.k == k && g.parity == par && g.first == first && g.u0 == buf[0]) ex = g.exec;
         if (!ex) {
             hipGraph_t gr = nullptr;
             if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
                 return false;
             const int q0 = queued;
-            for (int c = 0; c < k; c++, queued++) cycle(vfuse && c > 0, !vfuse || c == k - 1);
+            for (int c = 0; c < k; c++, queued++)
+                cycle(vfuse && (c > 0 || first), !vfuse || c == k - 1);
             queued = q0;
             if (hipStreamEndCapture(st, &gr) != hipSuccess) return false;
             const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
             hipGraphDestroy(gr);
             if (e != hipSuccess) return false;
-            M.graphs.push_back({k, par, buf[0], ex});
+            M.graphs.push_back({k, par, first, buf[0], ex});
         }
         if (hipGraphLaunch(ex, st) != hipSuccess) return false;
         queued += k;

@@ -122,6 +122,7 @@ struct MgLevels {
     bool use_graphs = false;
     struct CycleGraph {
         int k, parity;
+        bool first;  // the batch after the solve's initial residual pass
         const double *u0;
         hipGraphExec_t exec;
     };
