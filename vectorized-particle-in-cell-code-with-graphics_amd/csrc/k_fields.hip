@@ -647,6 +647,134 @@ __device__ __forceinline__ void dst_gemm(const float *A, const float *B, float *
 // LDS floats of the DST solve: three M x M matrices with row stride M + 4
 __host__ __device__ inline long dst_lds_floats(int M) { return 3L * M * (M + 4); }
 
+// ---- the DST for M = 64 with the sine matrix folded in halves
+// S[a][b] = sin(pi a b / 64) (a, b = 1..63, 0-based index a - 1) has
+// S[a][64 - b] = (-1)^(a+1) S[a][b]: for odd a, (S x)_a sums
+// S[a][b] (x_b + x_{64-b}) over b = 1..31 plus S[a][32] x_32, for even a
+// S[a][b] (x_b - x_{64-b}) over b = 1..31.  So each 64 x 64 product becomes
+// two 32 x 32 products of a folded operand (half the matrix-core work):
+//   H1[i][p] = sin(pi (2i+1)(p+1) / 64), i, p = 0..31 (odd a = 2i+1, b = p+1)
+//   H2[i][p] = sin(pi (2i+2)(p+1) / 64), i, p = 0..30, row and column 31 zero.
+// The forward transform takes natural order to "split" order (odd
+// frequencies 0..31, even 32..62, 63 zero); the inverse, the same tables
+// transposed, takes split order back: y_c = U_c + V_c and
+// y_{64-c} = U_c - V_c with U, V the odd and even frequencies' sums.
+// Tables in LDS: H1 at Hs, H2 at Hs + 32 DSTF_LDH.  The eigenvalue scale is
+// taken in split order (dst_tables writes it so).
+#define DSTF_LDH 36
+__host__ __device__ inline long dstf_table_floats() { return 2L * 32 * DSTF_LDH; }
+
+// one 16 x 16 block of C = A B with k = 32 on the matrix cores; A(i, k) is
+// A[k lda + i] when AT, else A[i lda + k]; B(k, j) is B[j ldb + k] when BT,
+// else B[k ldb + j] (operands offset to the block); the lane layout of
+// v_mfma_f32_16x16x4f32 as in lds_gemm_mfma
+template <bool AT, bool BT>
+__device__ __forceinline__ void dstf_block(const float *A, int lda, const float *B, int ldb,
+                                           float *C, int ldc) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    mf4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int k = 4 * t + lk;
+        const float a = AT ? A[k * lda + li] : A[li * lda + k];
+        const float b = BT ? B[li * ldb + k] : B[k * ldb + li];
+        if (t & 1)
+            c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c1, 0, 0, 0);
+        else
+            c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c0, 0, 0, 0);
+    }
+    const mf4 cs = c0 + c1;
+    float *o = C + (4 * lk) * ldc + li;
+    o[0] = cs[0];
+    o[ldc] = cs[1];
+    o[2 * ldc] = cs[2];
+    o[3 * ldc] = cs[3];
+}
+
+// C rows 0..31 = Ha op B rows 0..31, rows 32..63 = Hb op B rows 32..63 (left
+// products, 16 wavefronts: one 16 x 16 block each); HT: the tables transposed
+template <bool HT>
+__device__ __forceinline__ void dstf_left(const float *Hs, const float *B, float *C, int ld) {
+    const int w = threadIdx.x >> 6, half = w >> 3, bi = (w >> 2) & 1, bj = w & 3;
+    const float *H = Hs + half * 32 * DSTF_LDH;
+    const float *A = HT ? H + 16 * bi : H + 16 * bi * DSTF_LDH;
+    dstf_block<HT, false>(A, DSTF_LDH, B + 32 * half * ld + 16 * bj, ld,
+                          C + (32 * half + 16 * bi) * ld + 16 * bj, ld);
+    __syncthreads();
+}
+
+// C cols 0..31 = A cols 0..31 op Ha, cols 32..63 = A cols 32..63 op Hb
+template <bool HT>
+__device__ __forceinline__ void dstf_right(const float *Hs, const float *A, float *C, int ld) {
+    const int w = threadIdx.x >> 6, half = w >> 3, bi = (w >> 1) & 3, bj = w & 1;
+    const float *H = Hs + half * 32 * DSTF_LDH;
+    const float *B = HT ? H + 16 * bj * DSTF_LDH : H + 16 * bj;
+    dstf_block<false, HT>(A + 16 * bi * ld + 32 * half, ld, B, DSTF_LDH,
+                          C + 16 * bi * ld + 32 * half + 16 * bj, ld);
+    __syncthreads();
+}
+
+// fold (sums then differences) of the rows (ROWS) or columns of X into W
+template <bool ROWS>
+__device__ __forceinline__ void dstf_fold(const float *X, float *W, int ld) {
+#pragma unroll 1  // elementwise: few registers beat the unrolled addresses
+    for (int r = 0; r < 4; r++) {
+        const int k = (int)threadIdx.x + r * 1024, p = k >> 6, q = k & 63;
+        const int o = ROWS ? p : q;  // the folded index
+        float v = 0.0f;
+        auto at = [&](int idx) { return ROWS ? X[idx * ld + q] : X[p * ld + idx]; };
+        if (o < 31)
+            v = at(o) + at(62 - o);
+        else if (o == 31)
+            v = at(31);
+        else if (o < 63)
+            v = at(o - 32) - at(94 - o);
+        W[p * ld + q] = v;
+    }
+    __syncthreads();
+}
+
+// unfold of the rows (ROWS) or columns of W (U in 0..31, V in 32..63) into X
+template <bool ROWS>
+__device__ __forceinline__ void dstf_unfold(const float *W, float *X, int ld) {
+#pragma unroll 1  // elementwise: few registers beat the unrolled addresses
+    for (int r = 0; r < 4; r++) {
+        const int k = (int)threadIdx.x + r * 1024, p = k >> 6, q = k & 63;
+        const int o = ROWS ? p : q;  // the natural index
+        float v = 0.0f;
+        auto at = [&](int idx) { return ROWS ? W[idx * ld + q] : W[p * ld + idx]; };
+        if (o < 31)
+            v = at(o) + at(32 + o);
+        else if (o == 31)
+            v = at(31);
+        else if (o < 63)
+            v = at(62 - o) - at(94 - o);
+        X[p * ld + q] = v;
+    }
+    __syncthreads();
+}
+
+// X (64 x 64, natural order, row and column 63 zero) <- the DST solve of X;
+// W: scratch of the same shape; dk: this thread's entries k = threadIdx.x +
+// r 1024 of the split-order eigenvalue scale
+__device__ __forceinline__ void dstf_solve(float *X, float *W, const float *Hs, const float dk[4],
+                                           int ld) {
+    dstf_fold<true>(X, W, ld);
+    dstf_left<false>(Hs, W, X, ld);  // rows in split order
+    dstf_fold<false>(X, W, ld);
+    dstf_right<true>(Hs, W, X, ld);  // columns too
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int k = (int)threadIdx.x + r * 1024, p = k >> 6, q = k & 63;
+        X[p * ld + q] = X[p * ld + q] * dk[r];
+    }
+    __syncthreads();
+    dstf_left<true>(Hs, X, W, ld);
+    dstf_unfold<true>(W, X, ld);  // rows in natural order
+    dstf_right<false>(Hs, X, W, ld);
+    dstf_unfold<false>(W, X, ld);  // columns too
+}
+
 // the DST solve by one workgroup of 1024 threads; lds_raw: dst_lds_floats(M)
 __device__ void dst_solve(const float *__restrict__ rhs_in, float *__restrict__ u_out, int lc,
                           float h2, const float *__restrict__ Sg, const float *__restrict__ Dg,
@@ -663,23 +791,31 @@ __device__ void dst_solve(const float *__restrict__ rhs_in, float *__restrict__ 
     }
     for (int k = threadIdx.x; k < M * M; k += blockDim.x) {
         const int p = k >> (lc + 1), q = k & (M - 1);
-        S[p * ld + q] = Sg[k];
         X[p * ld + q] = (p < M - 1 && q < M - 1) ? rhs_in[(p + 1) * n + q + 1] : 0.0f;
     }
-    __syncthreads();
-    dst_gemm(S, X, Y, M, ld);  // Y = S f
-    dst_gemm(Y, S, X, M, ld);  // X = S f S
+    if (M == 64) {  // the folded tables (dstf_solve), in S's space
+        for (int k = threadIdx.x; k < 2048; k += blockDim.x)
+            S[(k >> 10) * 32 * DSTF_LDH + ((k >> 5) & 31) * DSTF_LDH + (k & 31)] = Sg[k];
+        __syncthreads();
+        dstf_solve(X, Y, S, dk, ld);  // X = u on the interior
+    } else {
+        for (int k = threadIdx.x; k < M * M; k += blockDim.x)
+            S[(k >> (lc + 1)) * ld + (k & (M - 1))] = Sg[k];
+        __syncthreads();
+        dst_gemm(S, X, Y, M, ld);  // Y = S f
+        dst_gemm(Y, S, X, M, ld);  // X = S f S
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int k = threadIdx.x + r * blockDim.x;
-        if (k < M * M) {
-            const int p = k >> (lc + 1), q = k & (M - 1);
-            X[p * ld + q] = X[p * ld + q] * dk[r];
+        for (int r = 0; r < 4; r++) {
+            const int k = threadIdx.x + r * blockDim.x;
+            if (k < M * M) {
+                const int p = k >> (lc + 1), q = k & (M - 1);
+                X[p * ld + q] = X[p * ld + q] * dk[r];
+            }
         }
+        __syncthreads();
+        dst_gemm(S, X, Y, M, ld);
+        dst_gemm(Y, S, X, M, ld);  // X = u on the interior
     }
-    __syncthreads();
-    dst_gemm(S, X, Y, M, ld);
-    dst_gemm(Y, S, X, M, ld);  // X = u on the interior
     for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
         const int i = k / n, j = k - (k / n) * n;
         const bool in = i > 0 && j > 0 && i < n - 1 && j < n - 1;
@@ -968,6 +1104,27 @@ int mg_dst_tables(MgLevels &M) {
             D[(size_t)p * m + q] = (float)((2.0 / m) * (2.0 / m) / (lp + lq));
         }
     }
+    if (m == 64) {
+        // the folded form (dstf_solve): H1 (odd frequencies a = 2i+1) and H2
+        // (even a = 2i+2) over b = p+1, then the scale in split order
+        auto sinab = [&](long a, long b) {
+            return (float)std::sin(pi * (double)((a * b) % (2L * m)) / (double)m);
+        };
+        std::vector<float> H(2048, 0.0f), Ds((size_t)m * m, 0.0f);
+        for (int i = 0; i < 32; i++)
+            for (int p = 0; p < 32; p++) {
+                H[i * 32 + p] = sinab(2 * i + 1, p + 1);
+                if (i < 31 && p < 31) H[1024 + i * 32 + p] = sinab(2 * i + 2, p + 1);
+            }
+        // split index -> natural frequency index (a - 1), -1: none
+        auto nat = [](int r) { return r < 32 ? 2 * r : (r < 63 ? 2 * (r - 32) + 1 : -1); };
+        for (int r = 0; r < m; r++)
+            for (int c = 0; c < m; c++)
+                if (nat(r) >= 0 && nat(c) >= 0)
+                    Ds[(size_t)r * m + c] = D[(size_t)nat(r) * m + nat(c)];
+        S.swap(H);
+        D.swap(Ds);
+    }
     if (hipMalloc((void **)&M.dst_S, S.size() * sizeof(float)) != hipSuccess ||
         hipMalloc((void **)&M.dst_D, D.size() * sizeof(float)) != hipSuccess)
         return -1;
@@ -1032,6 +1189,7 @@ int mg_coarse_level_for(long lds_bytes, int ng) {
     return lc;
 }
 
+template <bool FOLD>
 __global__ void __launch_bounds__(1024)
     k_tol_solve_dst(double *__restrict__ ug, const double *__restrict__ fg, int n, double h2i,
                     int lc, float h2, const float *__restrict__ Sg, const float *__restrict__ Dg,
@@ -1043,8 +1201,10 @@ void mg_set_coarse_attr(long lds_bytes) {
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     hipFuncSetAttribute((const void *)k_mg_coarse_dst, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds_bytes);
-    hipFuncSetAttribute((const void *)k_tol_solve_dst, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)lds_bytes);
+    hipFuncSetAttribute((const void *)k_tol_solve_dst<true>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    hipFuncSetAttribute((const void *)k_tol_solve_dst<false>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     hipFuncSetAttribute((const void *)k_mg_coarse<float>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
@@ -1382,6 +1542,7 @@ __device__ __forceinline__ double block_sum_1024(double v, double *ws) {
     return t;
 }
 
+template <bool FOLD>
 __global__ void __launch_bounds__(1024)
     k_tol_solve_dst(double *__restrict__ ug, const double *__restrict__ fg, int n, double h2i,
                     int lc, float h2, const float *__restrict__ Sg, const float *__restrict__ Dg,
@@ -1417,7 +1578,12 @@ __global__ void __launch_bounds__(1024)
         dk[r] = k < m * m ? Dg[k] * h2 : 0.0f;
     }
     for (int k = (int)threadIdx.x; k < m * ld; k += 1024) R[k] = 0.0f;
-    for (int k = (int)threadIdx.x; k < m * m; k += 1024) S[(k >> (lc + 1)) * ld + (k & (m - 1))] = Sg[k];
+    if (FOLD)  // m = 64: the folded tables (dstf_solve)
+        for (int k = (int)threadIdx.x; k < 2048; k += 1024)
+            S[(k >> 10) * 32 * DSTF_LDH + ((k >> 5) & 31) * DSTF_LDH + (k & 31)] = Sg[k];
+    else
+        for (int k = (int)threadIdx.x; k < m * m; k += 1024)
+            S[(k >> (lc + 1)) * ld + (k & (m - 1))] = Sg[k];
     ff = block_sum_1024(ff, ws);  // its barriers also publish U, F, S and R
     // the residual of U into R, its sum of squares
     auto residual = [&]() -> double {
@@ -1439,19 +1605,23 @@ __global__ void __launch_bounds__(1024)
     double rel = (ff > 0.0) ? sqrt(rr / ff) : sqrt(rr);
     int q = 0;
     while (!(rel <= tol) && q < max_cycles) {  // uniform: every thread holds the same sums
-        dst_gemm(S, R, W, m, ld);  // W = S r
-        dst_gemm(W, S, R, m, ld);  // R = S r S
+        if (FOLD) {
+            dstf_solve(R, W, S, dk, ld);  // R = e on the interior
+        } else {
+            dst_gemm(S, R, W, m, ld);  // W = S r
+            dst_gemm(W, S, R, m, ld);  // R = S r S
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int k = (int)threadIdx.x + r * 1024;
-            if (k < m * m) {
-                const int p = k >> (lc + 1), c = k & (m - 1);
-                R[p * ld + c] = R[p * ld + c] * dk[r];
+            for (int r = 0; r < 4; r++) {
+                const int k = (int)threadIdx.x + r * 1024;
+                if (k < m * m) {
+                    const int p = k >> (lc + 1), c = k & (m - 1);
+                    R[p * ld + c] = R[p * ld + c] * dk[r];
+                }
             }
+            __syncthreads();
+            dst_gemm(S, R, W, m, ld);
+            dst_gemm(W, S, R, m, ld);  // R = e on the interior
         }
-        __syncthreads();
-        dst_gemm(S, R, W, m, ld);
-        dst_gemm(W, S, R, m, ld);  // R = e on the interior
 #pragma unroll
         for (int r = 0; r < SMALL_NPT; r++) {
             const int k = (int)threadIdx.x + r * 1024;
@@ -1540,7 +1710,8 @@ int mg_tol_solve(hipStream_t st, MgLevels &M, double length, double tol, int max
         // the whole solve in one workgroup (k_tol_solve_dst)
         const int m = n - 1;
         const double hc = length / (double)m;
-        hipLaunchKernelGGL(k_tol_solve_dst, dim3(1), dim3(1024), (size_t)tol_small_lds_bytes(m),
+        hipLaunchKernelGGL((m == 64 ? k_tol_solve_dst<true> : k_tol_solve_dst<false>), dim3(1),
+                           dim3(1024), (size_t)tol_small_lds_bytes(m),
                            st, u, f, n, h2i, M.lc, (float)(hc * hc), M.dst_S, M.dst_D, tol,
                            max_iter, M.flags, M.relres, M.host_flags);
         if (after_batch) after_batch(u, u);

@@ -113,8 +113,9 @@ struct MgLevels {
     int pending = 0;             // passes a queued one-workgroup solve publishes (0: none)
     // exact coarsest-level solve of the tolerance mode: discrete sine
     // transform tables of level lc, M x M with M = 2^(lc+1) (padded by one)
-    float *dst_S = nullptr;      // S[p][q] = sin(pi (p+1)(q+1) / M)
+    float *dst_S = nullptr;      // S[p][q] = sin(pi (p+1)(q+1) / M); M = 64: the folded halves
     float *dst_D = nullptr;      // (2/M)^2 / (lambda_p + lambda_q), h^2 applied in the kernel
+                                 // (M = 64: rows and columns in the folded, split order)
     int nu = 2;                  // red-black sweeps per leg of the fused V-cycle
     hipEvent_t done = nullptr;   // end of a queued batch of cycles (host wait)
     // tolerance mode: a batch of k cycles from a given parity and iterate
