@@ -1237,10 +1237,11 @@ void mg_fixed_solve(hipStream_t st, MgLevels &M, int ncycle, double length) {
 
 
 // End of a residual pass: this block's sums of r^2 and f^2 go to
-// partials[b].  The pass's total and verdict come from k_tol_finish, the
-// next launch on the stream: a last-block ticket inside the pass would need
-// a device-scope fence in every block (an L2 write-back per block on a
-// multi-XCD device), which cost more than the launch.
+// partials[b].  The pass's total and verdict come from a later launch
+// (k_tol_finish, or the next cycle's first kernel: TolVerdict): a
+// last-block ticket inside the pass would need a device-scope fence in
+// every block (an L2 write-back per block on a multi-XCD device), which
+// cost more than the launch.
 __device__ __forceinline__ void tol_partial(double rr, double ff, int b,
                                             double *__restrict__ partials) {
     rr = block_sum_d(rr);
@@ -1295,9 +1296,9 @@ static void tol_finish_launch(hipStream_t st, MgLevels &M, int nb, double tol, b
 // One residual pass of the refinement loop, fused: v = u_in + e (FP64
 // accumulation of the FP32 correction; e null: v = u_in) written to u_out
 // (null: not written), r = f - A v in FP32 for the next correction solve,
-// block partial sums of r^2 and f^2 (k_tol_finish, launched next, forms
-// the verdict).  u_in and u_out are different arrays (neighbours read
-// u_in + e).
+// block partial sums of r^2 and f^2 (k_tol_finish or the next cycle's
+// first kernel forms the verdict).  u_in and u_out are different arrays
+// (neighbours read u_in + e).
 __global__ void __launch_bounds__(TOL_THREADS)
     k_tol_update(const double *__restrict__ u_in, const float *__restrict__ e,
                  double *__restrict__ u_out, const double *__restrict__ f, int n, double h2i,
