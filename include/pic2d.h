@@ -324,7 +324,10 @@ int pic_poisson_tol(const double *f, double *u, int32_t n, double length, double
 
 /* --------------------------------------------------------------- getters */
 /* multigrid work of the last solve: V-cycles (tolerance mode) or ncycle,
- * cumulative V-cycles, and the relative residual reached (tolerance mode) */
+ * cumulative V-cycles, and the relative residual reached (tolerance mode).
+ * Grids up to 65^2 are solved without a host wait during the step: this
+ * call then waits for the last solve's verdict if the device has not
+ * finished it. */
 int pic_get_mg_stats(pic_handle *h, int32_t *last_cycles, int64_t *total_cycles, double *relres);
 int pic_particle_size(pic_handle *h, int32_t s, int64_t *n);        /* particle::particle_size */
 int pic_get_particles(pic_handle *h, int32_t s, pic_particle *out, int64_t cap); /* 48-B AoS copy */
