@@ -597,48 +597,9 @@ __device__ void lds_gemm_mfma(const float *A, const float *B, float *C, int M, i
     __syncthreads();
 }
 
-// The product for M = 64 (the coarsest level of every grid from 65^2 up):
-// each wavefront's 16 fragments of A and B are read from LDS before its
-// first MFMA, and the 16 products over k go to four independent
-// accumulators (k = 4 (4 t + r), r = 0..3), summed at the end, so the
-// matrix core is not held up by one accumulator's dependency chain.
-__device__ void lds_gemm_mfma64(const float *A, const float *B, float *C, int ld) {
-    constexpr int M = 64, nb = M / 16;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int li = lane & 15, lk = lane >> 4;
-    for (int blk = w; blk < nb * nb; blk += nw) {
-        const int bi = blk / nb, bj = blk - (blk / nb) * nb;
-        const float *a = A + (16 * bi + li) * ld + lk;
-        const float *b = B + lk * ld + 16 * bj + li;
-        float af[M / 4], bf[M / 4];
-#pragma unroll
-        for (int t = 0; t < M / 4; t++) {
-            af[t] = a[4 * t];
-            bf[t] = b[4 * t * ld];
-        }
-        mf4 c[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) c[r] = (mf4){0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int t = 0; t < M / 16; t++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                c[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[4 * t + r], bf[4 * t + r], c[r], 0,
-                                                             0, 0);
-        const mf4 cs = (c[0] + c[1]) + (c[2] + c[3]);
-        float *o = C + (16 * bi + 4 * lk) * ld + 16 * bj + li;
-        o[0] = cs[0];
-        o[ld] = cs[1];
-        o[2 * ld] = cs[2];
-        o[3 * ld] = cs[3];
-    }
-    __syncthreads();
-}
-
+// the products of the unfolded DST (M < 64; M = 64 takes dstf_solve)
 __device__ __forceinline__ void dst_gemm(const float *A, const float *B, float *C, int M, int ld) {
-    if (M == 64)
-        lds_gemm_mfma64(A, B, C, ld);
-    else if ((M & 15) == 0)
+    if ((M & 15) == 0)
         lds_gemm_mfma(A, B, C, M, ld);
     else
         lds_gemm(A, B, C, M, ld);
